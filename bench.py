@@ -1,0 +1,196 @@
+"""Benchmark: Mvoxels/s of the plain GPU path (exact EDT + measure + thinning)
+on the 512^3 blurred-noise volume, one MI355X.
+
+A step = vc_edt + vc_measure + vc_thin_levels over the whole volume, inputs
+resident in HBM.  Prints one JSON line (rank 0).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
+                    [--no-cpu-baseline] [--profile-steps]
+"""
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+EDT_BYTES_PER_VOXEL = 21  # (1+2) + (2+4) + (4+4+4), see DESIGN.md
+
+
+class HipEvents:
+    """hipEvent timing on the library's stream (torch's events only see
+    torch's current stream)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        out = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(out), a, b) == 0
+        return out.value
+
+
+def cpu_baseline(occ_host, levels, target_s=15.0):
+    """The oracle (single-threaded C restatement) on a bounded cubic crop of the
+    same volume: EDT + measure + levels + thinning."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    n = occ_host.shape[0]
+    side = min(n, 128)
+    c0 = (n - side) // 2
+    crop = np.ascontiguousarray(occ_host[c0:c0 + side, c0:c0 + side, c0:c0 + side])
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        d, f = O.edt_ft(crop)
+        lam = O.lambda2(crop, f)
+        lvl = O.levels(crop, lam, levels)
+        O.thin(crop, lvl, len(levels))
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > target_s or reps >= 20:
+            break
+    mvox = crop.size * reps / el / 1e6
+    return {"value": round(mvox, 3), "unit": "Mvoxels/s", "cores": 1, "kind": "port",
+            "sample": "oracle (single-thread C) EDT+measure+thinning on the central %d^3 crop "
+                      "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", action="store_true",
+                    help="no CPU baseline, no hashing (for rocprofv3 runs)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (--gpus %d requested); "
+              "multi-GPU sharding is design-only, see DESIGN.md" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import voxel_ma_amd
+    from voxel_ma_amd import synth
+
+    dev = "cuda"
+    n = a.size
+    print("[bench] generating blurred-noise volume %d^3" % n, file=sys.stderr, flush=True)
+    occ = synth.blur_c3(n, device=dev)
+    torch.cuda.synchronize()
+    fill = float(occ.float().mean().item())
+    any_x = occ.any(dim=2).any(dim=1).nonzero()
+    any_y = occ.any(dim=2).any(dim=0).nonzero()
+    any_z = occ.any(dim=1).any(dim=0).nonzero()
+    size = max(int(t[-1] - t[0]) for t in (any_x, any_y, any_z))
+    levels = synth.levels_from_tt(size)
+    occ_host = occ.cpu().numpy() if not a.profile_steps or True else None
+    vol_hash = hashlib.sha256(occ_host.tobytes()).hexdigest()[:16]
+
+    ctx = voxel_ma_amd.Context(0)
+    vol = ctx.from_device(occ.data_ptr(), occ.shape)
+    ctx.synchronize()
+    stream = ctx.stream()
+    ev = HipEvents()
+    N = n * n * n
+
+    def step(events=None):
+        if events:
+            ev.record(events[0], stream)
+        vol.edt()
+        if events:
+            ev.record(events[1], stream)
+        vol.measure()
+        if events:
+            ev.record(events[2], stream)
+        st = vol.thin_levels(levels, stats=True)
+        if events:
+            ev.record(events[3], stream)
+        return st
+
+    print("[bench] warmup %d" % a.warmup, file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+
+    evs = [[ev.create() for _ in range(4)] for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    stats = None
+    for k in range(a.steps):
+        stats = step(evs[k])
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print("[bench] timed %d steps in %.3f s" % (a.steps, wall), file=sys.stderr, flush=True)
+
+    edt_ms = sum(ev.ms(e[0], e[1]) for e in evs) / a.steps
+    meas_ms = sum(ev.ms(e[1], e[2]) for e in evs) / a.steps
+    thin_ms = sum(ev.ms(e[2], e[3]) for e in evs) / a.steps
+    ms_step = wall * 1e3 / a.steps
+    value = N / (ms_step * 1e-3) / 1e6
+
+    edt_gbs = EDT_BYTES_PER_VOXEL * N / (edt_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Mvoxels/s EDT+thinning at 512^3",
+        "value": round(value, 2),
+        "unit": "Mvoxels/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "blurred-noise %d^3 (sigma=12, 70th-percentile threshold, "
+                               "1-voxel shell), exact EDT+FT, lambda measure, subfield "
+                               "thinning at %d levels" % (n, len(levels)),
+                   "volume_sha256_16": vol_hash, "fill": round(fill, 4),
+                   "levels": [round(l, 4) for l in levels], "inside_size": size},
+        "stages_ms": {"edt": round(edt_ms, 3), "measure": round(meas_ms, 3),
+                      "thin": round(thin_ms, 3)},
+        "thin_stats": {"removed": [int(x) for x in stats.removed[:len(levels)]],
+                       "passes_with_work": int(stats.passes), "tests": int(stats.tests),
+                       "launches": int(stats.launches)},
+        "roofline": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(edt_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    if not (a.no_cpu_baseline or a.profile_steps):
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(occ_host, levels)
+    vol.release()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

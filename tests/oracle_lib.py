@@ -1,0 +1,173 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker, never by the product library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+_lib = None
+
+
+class OrEuler(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("V", "E", "F", "T", "C", "euler", "V_bnd", "E_bnd", "V_int", "E_int", "F_int")]
+
+
+class OrThinStats(ctypes.Structure):
+    _fields_ = [("removed", ctypes.c_int64 * 64), ("sweeps", ctypes.c_int64 * 64),
+                ("tests", ctypes.c_int64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run __graft_entry__.build() (or make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i64 = ctypes.c_int, ctypes.c_int64
+        L.or_edt_ft.argtypes = [P, i32, i32, i32, P, P]
+        L.or_edt_brute.argtypes = [P, i32, i32, i32, P, P]
+        L.or_envelope_1d.argtypes = [P, i32, P, P]
+        L.or_lambda2.argtypes = [P, P, i32, i32, i32, P]
+        L.or_level_threshold.argtypes = [ctypes.c_float]
+        L.or_level_threshold.restype = i64
+        L.or_levels.argtypes = [P, P, i64, P, i32, P]
+        L.or_is_simple.argtypes = [ctypes.c_uint32]
+        L.or_is_simple_topo.argtypes = [ctypes.c_uint32]
+        L.or_simple_lut.argtypes = [P]
+        L.or_thin.argtypes = [P, P, i32, i32, i32, i32, ctypes.POINTER(OrThinStats)]
+        L.or_euler_voxels.argtypes = [P, i32, i32, i32]
+        L.or_euler_voxels.restype = i64
+        L.or_components26.argtypes = [P, i32, i32, i32]
+        L.or_components26.restype = i64
+        L.or_bg_components6.argtypes = [P, i32, i32, i32]
+        L.or_bg_components6.restype = i64
+        L.or_cube_components.argtypes = [P]
+        L.or_boundary.argtypes = [P, i32, i32, i32, P, i64, ctypes.POINTER(i64), P, i64,
+                                  ctypes.POINTER(i64), ctypes.POINTER(OrEuler)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def edt_ft(occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    d2 = np.empty(occ.shape, np.int32)
+    ft = np.empty(occ.shape, np.int32)
+    rc = lib().or_edt_ft(_p(occ), *occ.shape, _p(d2), _p(ft))
+    assert rc == 0
+    return d2, ft
+
+
+def edt_brute(occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    d2 = np.empty(occ.shape, np.int32)
+    ft = np.empty(occ.shape, np.int32)
+    lib().or_edt_brute(_p(occ), *occ.shape, _p(d2), _p(ft))
+    return d2, ft
+
+
+def envelope_1d(f):
+    f = np.ascontiguousarray(f, dtype=np.int64)
+    D = np.empty(len(f), np.int64)
+    a = np.empty(len(f), np.int32)
+    lib().or_envelope_1d(_p(f), len(f), _p(D), _p(a))
+    return D, a
+
+
+def lambda2(occ, ft):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    ft = np.ascontiguousarray(ft, dtype=np.int32)
+    out = np.empty(occ.shape, np.int32)
+    lib().or_lambda2(_p(occ), _p(ft), *occ.shape, _p(out))
+    return out
+
+
+def level_thresholds(levels):
+    return np.array([lib().or_level_threshold(float(l)) for l in levels], dtype=np.int64)
+
+
+def levels(occ, lam2, lv):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    lam2 = np.ascontiguousarray(lam2, dtype=np.int32)
+    T = level_thresholds(lv)
+    out = np.empty(occ.shape, np.uint8)
+    lib().or_levels(_p(occ), _p(lam2), occ.size, _p(T), len(T), _p(out))
+    return out
+
+
+def is_simple(nb):
+    return bool(lib().or_is_simple(int(nb)))
+
+
+def is_simple_topo(nb):
+    return bool(lib().or_is_simple_topo(int(nb)))
+
+
+def simple_lut():
+    out = np.empty(1 << 21, np.uint32)
+    lib().or_simple_lut(_p(out))
+    return out
+
+
+def thin(occ, lvl, nlev):
+    label = np.ascontiguousarray(occ, dtype=np.uint8).copy()
+    lvl = np.ascontiguousarray(lvl, dtype=np.uint8)
+    st = OrThinStats()
+    rc = lib().or_thin(_p(label), _p(lvl), *label.shape, int(nlev), ctypes.byref(st))
+    assert rc == 0
+    return label, st
+
+
+def euler_voxels(occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    return lib().or_euler_voxels(_p(occ), *occ.shape)
+
+
+def components26(occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    return lib().or_components26(_p(occ), *occ.shape)
+
+
+def bg_components6(occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    return lib().or_bg_components6(_p(occ), *occ.shape)
+
+
+def betti(occ):
+    """(b0, b1, b2, chi) of the inside voxels, (26,6) adjacency."""
+    chi = euler_voxels(occ)
+    b0 = components26(occ)
+    b2 = bg_components6(occ) - 1
+    return b0, b0 + b2 - chi, b2, chi
+
+
+def cube_components():
+    t = np.empty(256, np.int32)
+    lib().or_cube_components(_p(t))
+    return t
+
+
+def boundary(occ, with_euler=True):
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    ns, nq = ctypes.c_int64(), ctypes.c_int64()
+    lib().or_boundary(_p(occ), *occ.shape, None, 0, ctypes.byref(ns), None, 0,
+                      ctypes.byref(nq), None)
+    sites = np.empty((ns.value, 3), np.float32)
+    quads = np.empty((nq.value, 4), np.int32)
+    eu = OrEuler()
+    rc = lib().or_boundary(_p(occ), *occ.shape, _p(sites), ns.value, ctypes.byref(ns), _p(quads),
+                           nq.value, ctypes.byref(nq), ctypes.byref(eu) if with_euler else None)
+    assert rc == 0
+    e = {n: getattr(eu, n) for n, _ in OrEuler._fields_} if with_euler else None
+    return sites, quads, e

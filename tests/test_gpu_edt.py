@@ -1,0 +1,77 @@
+"""GPU parity: EDT, feature transform and measure through the C ABI against
+the CPU oracle (bit-exact) and the scipy golden vectors."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import say
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+
+def _run(vc, occ):
+    vol = vc.upload(occ)
+    vol.edt().measure()
+    d, f, lam = vol.get(1), vol.get(2), vol.get(3)
+    vol.release()
+    return d, f, lam
+
+
+def _check(vc, oracle, occ, name):
+    say("edt", name, occ.shape)
+    d, f, lam = _run(vc, occ)
+    do, fo = oracle.edt_ft(occ)
+    assert np.array_equal(d, do), name
+    assert np.array_equal(f, fo), name
+    lo = oracle.lambda2(occ, fo)
+    assert np.array_equal(lam.astype(np.int64), lo.astype(np.int64)), name
+
+
+def test_simple_table_matches_oracle(vc, simple_lut_cpu):
+    say("simple table")
+    t = vc.simple_table()
+    assert np.array_equal(t, simple_lut_cpu)
+
+
+def test_edt_random_shapes(vc, oracle):
+    rng = np.random.default_rng(5)
+    for i in range(40):
+        shape = tuple(int(s) for s in rng.integers(1, 40, size=3))
+        occ = (rng.random(shape) < rng.choice([0.3, 0.7, 0.95, 0.995])).astype(np.uint8)
+        _check(vc, oracle, occ, "rand%d" % i)
+
+
+def test_edt_edge_cases(vc, oracle):
+    cases = {
+        "all_background": np.zeros((7, 8, 9), np.uint8),
+        "all_inside": np.ones((7, 8, 9), np.uint8),
+        "one_voxel": np.ones((1, 1, 1), np.uint8),
+        "line_x": (np.arange(97) % 13 != 0).astype(np.uint8).reshape(97, 1, 1),
+        "line_z": (np.arange(300) % 41 != 0).astype(np.uint8).reshape(1, 1, 300),
+        "single_bg": np.ones((33, 17, 65), np.uint8),
+        "wide_z": np.ones((3, 4, 2000), np.uint8),
+    }
+    cases["single_bg"][30, 2, 64] = 0
+    cases["wide_z"][1, 2, 1234] = 0
+    for name, occ in cases.items():
+        _check(vc, oracle, occ, name)
+
+
+def test_edt_scipy_golden(vc):
+    z = np.load(os.path.join(HERE, "golden", "edt_scipy.npz"))
+    for name in sorted({k.split("__")[0] for k in z.files}):
+        say("edt golden", name)
+        d, f, _ = _run(vc, z[name + "__occ"])
+        assert np.array_equal(d, z[name + "__d2"]), name
+
+
+def test_edt_config1_torus(vc, oracle):
+    from voxel_ma_amd import synth
+    _check(vc, oracle, synth.torus_c1(), "c1")
+
+
+def test_edt_config2_spheres(vc, oracle):
+    from voxel_ma_amd import synth
+    _check(vc, oracle, synth.spheres_c2(), "c2")

@@ -1,0 +1,233 @@
+"""voxel_ma_amd -- MI355X-native Voxel Cores volumetric hot path.
+
+Python view of the C ABI in include/vc_api.h (libvoxelcore.so, gfx950).  The
+product is the HIP library; this module only binds it with ctypes.  There is
+no CPU fallback: if the library is missing or no GPU is present every call
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libvoxelcore.so")
+
+VC_OK = 0
+VC_ERR_INVALID = -1
+VC_ERR_HIP = -2
+VC_ERR_CAPACITY = -3
+VC_ERR_EMPTY_VOL = -4
+VC_ERR_NOMEM = -5
+VC_ERR_STATE = -6
+VC_ERR_GUARD = -7
+VC_MAX_LEVELS = 64
+
+BUF_OCC, BUF_EDT, BUF_FT, BUF_LAM2, BUF_LABEL, BUF_LEVEL = range(6)
+_BUF_DTYPE = {BUF_OCC: np.uint8, BUF_EDT: np.int32, BUF_FT: np.int32,
+              BUF_LAM2: np.uint32, BUF_LABEL: np.uint8, BUF_LEVEL: np.uint8}
+
+# every symbol declared in include/vc_api.h
+EXPORTS = [
+    "vc_ctx_create", "vc_ctx_destroy", "vc_last_error", "vc_ctx_stream",
+    "vc_ctx_synchronize", "vc_simple_table_host", "vc_volume_upload",
+    "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
+    "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
+    "vc_get", "vc_get_device",
+]
+
+
+class VcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("vc error %d: %s" % (code, msg))
+        self.code = code
+
+
+class ThinStats(ctypes.Structure):
+    _fields_ = [("removed", ctypes.c_int64 * VC_MAX_LEVELS), ("passes", ctypes.c_int64),
+                ("tests", ctypes.c_int64), ("launches", ctypes.c_int64)]
+
+
+class Euler(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("V", "E", "F", "T", "C", "euler", "V_bnd", "E_bnd", "V_int", "E_int", "F_int")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def library():
+    """Load libvoxelcore.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libvoxelcore.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "vc_ctx_create": [I, PP],
+        "vc_ctx_destroy": [P],
+        "vc_last_error": [P, ctypes.c_char_p, ctypes.c_size_t],
+        "vc_ctx_stream": [P, PP],
+        "vc_ctx_synchronize": [P],
+        "vc_simple_table_host": [P, P],
+        "vc_volume_upload": [P, P, I, I, I, PP],
+        "vc_volume_from_device": [P, P, I, I, I, PP],
+        "vc_volume_release": [P, P],
+        "vc_volume_dims": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
+        "vc_edt": [P, P, I],
+        "vc_measure": [P, P],
+        "vc_set_measure_f32": [P, P, P],
+        "vc_thin_levels": [P, P, P, I, ctypes.POINTER(ThinStats)],
+        "vc_boundary_extract": [P, P, P, I64, ctypes.POINTER(I64), P, I64,
+                                ctypes.POINTER(I64), ctypes.POINTER(Euler)],
+        "vc_get": [P, P, I, P, I64],
+        "vc_get_device": [P, P, I, P, I64],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One HIP device + stream + simple-point table (vc_ctx)."""
+
+    def __init__(self, device=0):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        rc = self._lib.vc_ctx_create(int(device), ctypes.byref(h))
+        if rc != VC_OK:
+            raise VcError(rc, "vc_ctx_create(device=%d) failed" % device)
+        self.handle = h
+
+    def last_error(self):
+        buf = ctypes.create_string_buffer(1024)
+        self._lib.vc_last_error(self.handle, buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    def check(self, rc):
+        if rc != VC_OK:
+            raise VcError(rc, self.last_error())
+        return rc
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self.check(self._lib.vc_ctx_stream(self.handle, ctypes.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        self.check(self._lib.vc_ctx_synchronize(self.handle))
+
+    def simple_table(self):
+        out = np.empty(1 << 21, np.uint32)
+        self.check(self._lib.vc_simple_table_host(self.handle, _p(out)))
+        return out
+
+    def upload(self, occ):
+        occ = np.ascontiguousarray(occ, dtype=np.uint8)
+        if occ.ndim != 3:
+            raise ValueError("occupancy must be 3-D [x][y][z]")
+        return Volume(self, occ.shape, host=occ)
+
+    def from_device(self, dev_ptr, shape):
+        return Volume(self, shape, dev_ptr=dev_ptr)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.vc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Volume:
+    """A volume resident in HBM plus its pipeline results (vc_vol)."""
+
+    def __init__(self, ctx, shape, host=None, dev_ptr=None):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        h = ctypes.c_void_p()
+        L = ctx._lib
+        if host is not None:
+            rc = L.vc_volume_upload(ctx.handle, _p(host), *self.shape, ctypes.byref(h))
+        else:
+            rc = L.vc_volume_from_device(ctx.handle, ctypes.c_void_p(dev_ptr), *self.shape,
+                                         ctypes.byref(h))
+        ctx.check(rc)
+        self.handle = h
+
+    @property
+    def N(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def edt(self):
+        self.ctx.check(self.ctx._lib.vc_edt(self.ctx.handle, self.handle, 0))
+        return self
+
+    def measure(self):
+        self.ctx.check(self.ctx._lib.vc_measure(self.ctx.handle, self.handle))
+        return self
+
+    def set_measure_f32(self, dev_ptr):
+        self.ctx.check(self.ctx._lib.vc_set_measure_f32(self.ctx.handle, self.handle,
+                                                        ctypes.c_void_p(dev_ptr)))
+        return self
+
+    def thin_levels(self, levels, stats=True):
+        lv = np.ascontiguousarray(levels, dtype=np.float32)
+        st = ThinStats() if stats else None
+        self.ctx.check(self.ctx._lib.vc_thin_levels(self.ctx.handle, self.handle, _p(lv), len(lv),
+                                                    ctypes.byref(st) if stats else None))
+        return st
+
+    def boundary(self, with_euler=True):
+        L, c = self.ctx._lib, self.ctx
+        ns, nq = ctypes.c_int64(), ctypes.c_int64()
+        rc = L.vc_boundary_extract(c.handle, self.handle, None, 0, ctypes.byref(ns), None, 0,
+                                   ctypes.byref(nq), None)
+        if rc not in (VC_OK, VC_ERR_CAPACITY):
+            c.check(rc)
+        sites = np.empty((ns.value, 3), np.float32)
+        quads = np.empty((nq.value, 4), np.int32)
+        eu = Euler()
+        c.check(L.vc_boundary_extract(c.handle, self.handle, _p(sites), ns.value, ctypes.byref(ns),
+                                      _p(quads), nq.value, ctypes.byref(nq),
+                                      ctypes.byref(eu) if with_euler else None))
+        return sites, quads, (eu.as_dict() if with_euler else None)
+
+    def get(self, which):
+        out = np.empty(self.shape, _BUF_DTYPE[which])
+        self.ctx.check(self.ctx._lib.vc_get(self.ctx.handle, self.handle, which, _p(out),
+                                            out.nbytes))
+        return out
+
+    def get_device(self, which, dev_ptr, nbytes):
+        self.ctx.check(self.ctx._lib.vc_get_device(self.ctx.handle, self.handle, which,
+                                                   ctypes.c_void_p(dev_ptr), int(nbytes)))
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_volume_release(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -1,0 +1,211 @@
+"""Seeded synthetic volumes for the five workloads (inputs, not the product).
+
+All volumes are uint8 occupancy, C order [x][y][z]; voxel (i, j, k) is
+centred at integer coordinates (i, j, k), as in the reference Surfacer
+(corners at +-0.5, include/surfacing.h:97-119).
+
+  torus_c1     64^3 solid torus from a 64x32-segment triangle mesh (4,096
+               triangles), R=20, r=8, centre (32,32,32), axis z; a voxel is
+               inside iff its centre is inside the mesh (ray parity along +x;
+               rays are offset by (dy, dz) = (1.3e-6, 2.9e-6) voxels so that no
+               ray meets a mesh vertex or edge).
+  spheres_c2   256^3 union of 32 balls, centres uniform integer in [0,256)^3,
+               radii uniform integer in [8,40], |p-c|^2 <= r^2 exact; a 4-voxel
+               empty margin is enforced.
+  blur_c3      n^3 (512 by default) N(0,1) noise, separable Gaussian blur
+               sigma=12 (97 taps = truncate at 4 sigma, periodic boundary),
+               inside iff value > the exact 70th-percentile order statistic,
+               then a 1-voxel empty shell.  Built with torch (GPU when given a
+               CUDA device) -- plumbing for the input, not the measured path.
+  batch_c4     256 cubic volumes, L = round(32 * 6^u), u ~ U(0,1), each a union
+               of 4-16 capsules/ellipsoids (semi-axes/radius U[4,30], uniform
+               random rotation, centres inside the grid), 1-voxel margin.
+"""
+import math
+
+import numpy as np
+
+_DY, _DZ = 1.3e-6, 2.9e-6
+
+
+def torus_mesh(R=20.0, r=8.0, nu=64, nv=32, centre=(32.0, 32.0, 32.0)):
+    """Vertices (nu*nv, 3) and triangles (2*nu*nv, 3) of a torus around z."""
+    u = 2.0 * np.pi * np.arange(nu) / nu
+    v = 2.0 * np.pi * np.arange(nv) / nv
+    U, V = np.meshgrid(u, v, indexing="ij")
+    X = (R + r * np.cos(V)) * np.cos(U) + centre[0]
+    Y = (R + r * np.cos(V)) * np.sin(U) + centre[1]
+    Z = r * np.sin(V) + centre[2]
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    tris = []
+    for i in range(nu):
+        for j in range(nv):
+            a = i * nv + j
+            b = ((i + 1) % nu) * nv + j
+            c = ((i + 1) % nu) * nv + (j + 1) % nv
+            d = i * nv + (j + 1) % nv
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    return verts, np.array(tris, dtype=np.int64)
+
+
+def voxelize_mesh(verts, tris, shape):
+    """Inside test of voxel centres by ray parity along +x."""
+    nx, ny, nz = shape
+    crossings = [[[] for _ in range(nz)] for _ in range(ny)]
+    P = verts[tris]  # (T, 3, 3)
+    for t in range(len(tris)):
+        a, b, c = P[t]
+        ya, za = a[1], a[2]
+        e1 = b - a
+        e2 = c - a
+        det = e1[1] * e2[2] - e1[2] * e2[1]
+        if det == 0.0:
+            continue
+        ymin, ymax = min(a[1], b[1], c[1]), max(a[1], b[1], c[1])
+        zmin, zmax = min(a[2], b[2], c[2]), max(a[2], b[2], c[2])
+        for j in range(max(0, math.ceil(ymin - _DY)), min(ny - 1, math.floor(ymax - _DY)) + 1):
+            for k in range(max(0, math.ceil(zmin - _DZ)), min(nz - 1, math.floor(zmax - _DZ)) + 1):
+                py, pz = j + _DY - ya, k + _DZ - za
+                s = (py * e2[2] - pz * e2[1]) / det
+                w = (e1[1] * pz - e1[2] * py) / det
+                if s < 0.0 or w < 0.0 or s + w > 1.0:
+                    continue
+                crossings[j][k].append(a[0] + s * e1[0] + w * e2[0])
+    occ = np.zeros(shape, np.uint8)
+    xs = np.arange(nx, dtype=np.float64)
+    for j in range(ny):
+        for k in range(nz):
+            cr = crossings[j][k]
+            if not cr:
+                continue
+            cr = np.sort(np.array(cr))
+            n_right = len(cr) - np.searchsorted(cr, xs, side="right")
+            occ[:, j, k] = (n_right % 2 == 1)
+    return occ
+
+
+def torus_c1():
+    v, t = torus_mesh()
+    return voxelize_mesh(v, t, (64, 64, 64))
+
+
+def spheres_c2(n=256, count=32, rmin=8, rmax=40, seed=2, margin=4):
+    rng = np.random.default_rng(seed)
+    centres = rng.integers(0, n, size=(count, 3))
+    radii = rng.integers(rmin, rmax + 1, size=count)
+    occ = np.zeros((n, n, n), np.uint8)
+    for (cx, cy, cz), r in zip(centres, radii):
+        lo = np.maximum([cx - r, cy - r, cz - r], 0)
+        hi = np.minimum([cx + r + 1, cy + r + 1, cz + r + 1], n)
+        gx = np.arange(lo[0], hi[0])[:, None, None] - cx
+        gy = np.arange(lo[1], hi[1])[None, :, None] - cy
+        gz = np.arange(lo[2], hi[2])[None, None, :] - cz
+        ball = gx * gx + gy * gy + gz * gz <= r * r
+        occ[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] |= ball.astype(np.uint8)
+    if margin:
+        occ[:margin] = 0
+        occ[-margin:] = 0
+        occ[:, :margin] = 0
+        occ[:, -margin:] = 0
+        occ[:, :, :margin] = 0
+        occ[:, :, -margin:] = 0
+    return occ
+
+
+def gaussian_taps(sigma=12.0, truncate=4.0):
+    rad = int(truncate * sigma + 0.5)
+    x = np.arange(-rad, rad + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (x / sigma) ** 2)
+    return (w / w.sum()).astype(np.float32)
+
+
+def blur_c3(n=512, sigma=12.0, fill=0.30, seed=3, device="cpu"):
+    """Returns a torch uint8 tensor [x][y][z] on `device`."""
+    import torch
+    import torch.nn.functional as F
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    v = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float32)
+    taps = torch.from_numpy(gaussian_taps(sigma)).to(device)
+    rad = (taps.numel() - 1) // 2
+    w = taps.view(1, 1, -1)
+    for axis in range(3):
+        v = v.movedim(axis, 2).contiguous()
+        shp = v.shape
+        flat = v.view(-1, 1, shp[2])
+        flat = F.pad(flat, (rad, rad), mode="circular")
+        v = F.conv1d(flat, w).view(shp).movedim(2, axis).contiguous()
+    flat = v.view(-1)
+    k = int(math.ceil((1.0 - fill) * flat.numel()))
+    thr = torch.kthvalue(flat, k).values
+    occ = (v > thr).to(torch.uint8)
+    occ[0] = 0
+    occ[-1] = 0
+    occ[:, 0] = 0
+    occ[:, -1] = 0
+    occ[:, :, 0] = 0
+    occ[:, :, -1] = 0
+    return occ.contiguous()
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def batch_volume(L, rng):
+    occ = np.zeros((L, L, L), np.uint8)
+    g = np.indices((L, L, L)).reshape(3, -1).T.astype(np.float64)
+    for _ in range(int(rng.integers(4, 17))):
+        c = rng.uniform(0, L - 1, size=3)
+        Rm = _rotation(rng)
+        q = (g - c) @ Rm  # coordinates in the primitive frame
+        if rng.random() < 0.5:
+            ax = rng.uniform(4, 30, size=3)
+            inside = ((q / ax) ** 2).sum(axis=1) <= 1.0
+        else:
+            r = rng.uniform(4, 30)
+            h = rng.uniform(4, 30)
+            t = np.clip(q[:, 0], -h, h)
+            d2 = (q[:, 0] - t) ** 2 + q[:, 1] ** 2 + q[:, 2] ** 2
+            inside = d2 <= r * r
+        occ |= inside.reshape(L, L, L).astype(np.uint8)
+    occ[0] = occ[-1] = 0
+    occ[:, 0] = occ[:, -1] = 0
+    occ[:, :, 0] = occ[:, :, -1] = 0
+    return occ
+
+
+def batch_c4(count=256, seed=4, lmin=32, ratio=6.0):
+    rng = np.random.default_rng(seed)
+    sizes = [int(round(lmin * ratio ** rng.random())) for _ in range(count)]
+    return [batch_volume(L, rng) for L in sizes]
+
+
+DEFAULT_TT = (0.01, 0.02, 0.04)
+
+
+def inside_size(occ):
+    """Longest side of the bounding box of the inside voxel centres (float32),
+    the voxel form of VoroInfo::getInsidePartSize (reference
+    src/voroinfo.cpp:496)."""
+    occ = np.asarray(occ)
+    sizes = []
+    for axis in range(3):
+        other = tuple(a for a in range(3) if a != axis)
+        idx = np.nonzero(occ.any(axis=other))[0]
+        sizes.append(0 if len(idx) == 0 else int(idx[-1] - idx[0]))
+    return np.float32(max(sizes))
+
+
+def levels_from_tt(size, tt=DEFAULT_TT):
+    """thresh = float(insideSize * (double)t) per -tt value, ascending, as the
+    threshold loop of exportInsideVoroMesh (reference
+    src/highlevelalgo.cpp:825-836)."""
+    return sorted(float(np.float32(np.float64(np.float32(size)) * float(t))) for t in tt)
