@@ -23,11 +23,12 @@ EDT_BYTES_PER_VOXEL = 21  # (1+2) + (2+4) + (4+4+4), see DESIGN.md
 
 
 class HipEvents:
-    """hipEvent timing on the library's stream (torch's events only see
-    torch's current stream)."""
+    """hipEvent timing on the stream the library launches on (the HIP runtime
+    libvoxelcore.so links)."""
 
     def __init__(self):
-        self.h = ctypes.CDLL("libamdhip64.so")
+        import voxel_ma_amd
+        self.h = voxel_ma_amd.hip()
         self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
         self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
@@ -94,27 +95,23 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import voxel_ma_amd
     from voxel_ma_amd import synth
 
-    dev = "cuda"
     n = a.size
-    print("[bench] generating blurred-noise volume %d^3" % n, file=sys.stderr, flush=True)
-    occ = synth.blur_c3(n, device=dev)
-    torch.cuda.synchronize()
-    fill = float(occ.float().mean().item())
-    any_x = occ.any(dim=2).any(dim=1).nonzero()
-    any_y = occ.any(dim=2).any(dim=0).nonzero()
-    any_z = occ.any(dim=1).any(dim=0).nonzero()
-    size = max(int(t[-1] - t[0]) for t in (any_x, any_y, any_z))
+    print("[bench] generating blurred-noise volume %d^3 on the device" % n, file=sys.stderr,
+          flush=True)
+    ctx = voxel_ma_amd.Context(0)
+    occ_dev, _thr = ctx.synth_blurred_noise((n, n, n), sigma=12.0, fill=0.30, seed=3)
+    occ_host = occ_dev.download(np.uint8, (n, n, n))
+    fill = float(occ_host.mean())
+    size = synth.inside_size(occ_host)
     levels = synth.levels_from_tt(size)
-    occ_host = occ.cpu().numpy() if not a.profile_steps or True else None
     vol_hash = hashlib.sha256(occ_host.tobytes()).hexdigest()[:16]
 
-    ctx = voxel_ma_amd.Context(0)
-    vol = ctx.from_device(occ.data_ptr(), occ.shape)
+    vol = ctx.from_device(occ_dev.ptr, (n, n, n))
     ctx.synchronize()
+    occ_dev.free()
     stream = ctx.stream()
     ev = HipEvents()
     N = n * n * n
@@ -139,14 +136,12 @@ def main():
     ctx.synchronize()
 
     evs = [[ev.create() for _ in range(4)] for _ in range(a.steps)]
-    torch.cuda.synchronize()
     ctx.synchronize()
     t0 = time.perf_counter()
     stats = None
     for k in range(a.steps):
         stats = step(evs[k])
     ctx.synchronize()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     print("[bench] timed %d steps in %.3f s" % (a.steps, wall), file=sys.stderr, flush=True)
 
@@ -174,7 +169,7 @@ def main():
                                "1-voxel shell), exact EDT+FT, lambda measure, subfield "
                                "thinning at %d levels" % (n, len(levels)),
                    "volume_sha256_16": vol_hash, "fill": round(fill, 4),
-                   "levels": [round(l, 4) for l in levels], "inside_size": size},
+                   "levels": [round(l, 4) for l in levels], "inside_size": float(size)},
         "stages_ms": {"edt": round(edt_ms, 3), "measure": round(meas_ms, 3),
                       "thin": round(thin_ms, 3)},
         "thin_stats": {"removed": [int(x) for x in stats.removed[:len(levels)]],

@@ -69,7 +69,7 @@ def test_thin_config2_spheres(vc, oracle):
 
 def test_thin_float_measure_path(vc, oracle):
     """vc_set_measure_f32: the thinning uses the caller's float measure."""
-    import torch
+    import voxel_ma_amd
     from voxel_ma_amd import synth
     occ = synth.torus_c1()
     rng = np.random.default_rng(3)
@@ -77,12 +77,12 @@ def test_thin_float_measure_path(vc, oracle):
     levels = [1.0, 2.5, 4.0]
     say("thin float measure")
     vol = vc.upload(occ)
-    dm = torch.from_numpy(meas).cuda()
-    torch.cuda.synchronize()
-    vol.set_measure_f32(dm.data_ptr())
+    dm = voxel_ma_amd.DeviceBuffer.from_array(meas)
+    vol.set_measure_f32(dm.ptr)
     vol.thin_levels(levels)
     lab, lvl = vol.get(4), vol.get(5)
     vol.release()
+    dm.free()
     lvlo = np.full(occ.shape, 255, np.uint8)
     for k in reversed(range(len(levels))):
         lvlo[(occ > 0) & (meas < np.float32(levels[k]))] = k

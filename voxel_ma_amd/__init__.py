@@ -33,7 +33,7 @@ EXPORTS = [
     "vc_ctx_synchronize", "vc_simple_table_host", "vc_volume_upload",
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
-    "vc_get", "vc_get_device",
+    "vc_get", "vc_get_device", "vc_synth_blurred_noise",
 ]
 
 
@@ -88,6 +88,8 @@ def library():
                                 ctypes.POINTER(I64), ctypes.POINTER(Euler)],
         "vc_get": [P, P, I, P, I64],
         "vc_get_device": [P, P, I, P, I64],
+        "vc_synth_blurred_noise": [P, P, I, I, I, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_uint64, I, ctypes.POINTER(ctypes.c_float)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -99,6 +101,77 @@ def library():
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Hip:
+    """The HIP runtime libvoxelcore.so itself links (one runtime per process)."""
+
+    def __init__(self):
+        library()
+        h = ctypes.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        self.h = h
+
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = _Hip()
+    return _hip.h
+
+
+class DeviceBuffer:
+    """Raw device memory for handing device pointers across the C ABI."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        rc = hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 1))
+        if rc != 0:
+            raise VcError(VC_ERR_NOMEM, "hipMalloc(%d) failed (%d)" % (self.nbytes, rc))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        rc = hip().hipMemcpy(ctypes.c_void_p(self.ptr), _p(a), a.nbytes, self.H2D)
+        if rc != 0:
+            raise VcError(VC_ERR_HIP, "hipMemcpy H2D failed (%d)" % rc)
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes == self.nbytes
+        hip().hipDeviceSynchronize()
+        rc = hip().hipMemcpy(_p(out), ctypes.c_void_p(self.ptr), out.nbytes, self.D2H)
+        if rc != 0:
+            raise VcError(VC_ERR_HIP, "hipMemcpy D2H failed (%d)" % rc)
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -134,6 +207,17 @@ class Context:
         out = np.empty(1 << 21, np.uint32)
         self.check(self._lib.vc_simple_table_host(self.handle, _p(out)))
         return out
+
+    def synth_blurred_noise(self, shape, sigma=12.0, fill=0.30, seed=3, shell=True):
+        """Blurred-noise occupancy generated on the device (vc_synth.h);
+        returns (DeviceBuffer, threshold)."""
+        nx, ny, nz = (int(s) for s in shape)
+        buf = DeviceBuffer(nx * ny * nz)
+        thr = ctypes.c_float()
+        self.check(self._lib.vc_synth_blurred_noise(self.handle, ctypes.c_void_p(buf.ptr), nx, ny,
+                                                    nz, float(sigma), float(fill), int(seed),
+                                                    1 if shell else 0, ctypes.byref(thr)))
+        return buf, thr.value
 
     def upload(self, occ):
         occ = np.ascontiguousarray(occ, dtype=np.uint8)

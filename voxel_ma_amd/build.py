@@ -11,7 +11,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libvoxelcore.so")
 
-HIP_SOURCES = ["vc_core.hip", "vc_edt.hip", "vc_thin.hip", "vc_boundary.hip"]
+HIP_SOURCES = ["vc_core.hip", "vc_edt.hip", "vc_thin.hip", "vc_boundary.hip", "vc_synth.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

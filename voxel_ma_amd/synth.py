@@ -12,11 +12,12 @@ centred at integer coordinates (i, j, k), as in the reference Surfacer
   spheres_c2   256^3 union of 32 balls, centres uniform integer in [0,256)^3,
                radii uniform integer in [8,40], |p-c|^2 <= r^2 exact; a 4-voxel
                empty margin is enforced.
-  blur_c3      n^3 (512 by default) N(0,1) noise, separable Gaussian blur
-               sigma=12 (97 taps = truncate at 4 sigma, periodic boundary),
-               inside iff value > the exact 70th-percentile order statistic,
-               then a 1-voxel empty shell.  Built with torch (GPU when given a
-               CUDA device) -- plumbing for the input, not the measured path.
+  blurred noise  n^3 (512 in the benchmark) N(0,1) noise, separable
+               Gaussian blur sigma=12 (97 taps = truncate at 4 sigma, periodic
+               boundary), inside iff value > the exact 70th-percentile order
+               statistic, then a 1-voxel empty shell.  Generated on the device
+               by vc_synth_blurred_noise (include/vc_synth.h); blur_c3_host is
+               its numpy restatement for small sizes.
   batch_c4     256 cubic volumes, L = round(32 * 6^u), u ~ U(0,1), each a union
                of 4-16 capsules/ellipsoids (semi-axes/radius U[4,30], uniform
                random rotation, centres inside the grid), 1-voxel margin.
@@ -120,34 +121,37 @@ def gaussian_taps(sigma=12.0, truncate=4.0):
     return (w / w.sum()).astype(np.float32)
 
 
-def blur_c3(n=512, sigma=12.0, fill=0.30, seed=3, device="cpu"):
-    """Returns a torch uint8 tensor [x][y][z] on `device`."""
-    import torch
-    import torch.nn.functional as F
-
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    v = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float32)
-    taps = torch.from_numpy(gaussian_taps(sigma)).to(device)
-    rad = (taps.numel() - 1) // 2
-    w = taps.view(1, 1, -1)
-    for axis in range(3):
-        v = v.movedim(axis, 2).contiguous()
-        shp = v.shape
-        flat = v.view(-1, 1, shp[2])
-        flat = F.pad(flat, (rad, rad), mode="circular")
-        v = F.conv1d(flat, w).view(shp).movedim(2, axis).contiguous()
-    flat = v.view(-1)
-    k = int(math.ceil((1.0 - fill) * flat.numel()))
-    thr = torch.kthvalue(flat, k).values
-    occ = (v > thr).to(torch.uint8)
-    occ[0] = 0
-    occ[-1] = 0
-    occ[:, 0] = 0
-    occ[:, -1] = 0
-    occ[:, :, 0] = 0
-    occ[:, :, -1] = 0
-    return occ.contiguous()
+def blur_c3_host(n=64, sigma=12.0, fill=0.30, seed=3):
+    """Host (numpy, float64) restatement of the device generator
+    vc_synth_blurred_noise for small n: the same noise bits, blur taps and
+    exact order-statistic threshold.  Blurred values can differ from the
+    device's float32 sums in the last bits, so a few voxels at the threshold
+    may differ."""
+    N = n * n * n
+    i = np.arange(N, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed) * np.uint64(0x632BE59BD9B4E019) + i
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        r = x ^ (x >> np.uint64(31))
+    u1 = ((r >> np.uint64(40)).astype(np.float64) + 1.0) / 16777216.0
+    u2 = ((r >> np.uint64(16)) & np.uint64(0xFFFFFF)).astype(np.float64) / 16777216.0
+    v = (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(n, n, n)
+    taps = gaussian_taps(sigma).astype(np.float64)
+    rad = (len(taps) - 1) // 2
+    for axis in (2, 1, 0):
+        acc = np.zeros_like(v)
+        for t, w in enumerate(taps):
+            acc += w * np.roll(v, rad - t, axis=axis)
+        v = acc
+    k = int(math.ceil((1.0 - fill) * N))
+    thr = np.partition(v.ravel(), k - 1)[k - 1]
+    occ = (v > thr).astype(np.uint8)
+    occ[0] = occ[-1] = 0
+    occ[:, 0] = occ[:, -1] = 0
+    occ[:, :, 0] = occ[:, :, -1] = 0
+    return occ
 
 
 def _rotation(rng):
