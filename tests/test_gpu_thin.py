@@ -35,6 +35,8 @@ def _compare(vc, oracle, occ, levels, name):
     assert np.array_equal(lab, labo), name
     for k in range(len(levels)):
         assert st.removed[k] == sto.removed[k]
+    # the same number of sweeps as the dense schedule (8 passes per sweep)
+    assert st.passes == 8 * sum(sto.sweeps[k] for k in range(len(levels)))
     return lab
 
 

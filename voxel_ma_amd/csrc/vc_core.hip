@@ -262,9 +262,11 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->meas_f);
     vc_free(v->lvl);
     vc_free(v->label);
-    vc_free(v->qbits);
-    vc_free(v->qbuf);
-    vc_free(v->qcnt);
+    vc_free(v->alive);
+    vc_free(v->elig);
+    vc_free(v->sweep_removed);
+    if (v->hcnt)
+        (void)hipHostFree(v->hcnt);
     vc_free(v->tstats);
     vc_free(v->scratch);
     delete v;

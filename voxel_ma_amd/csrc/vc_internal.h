@@ -40,11 +40,11 @@ struct vc_vol
     int measure_kind = 0;     // 0 none, 1 exact lam2, 2 float
     uint8_t* lvl = nullptr;   // N
     uint8_t* label = nullptr; // N
-    // thinning work lists
-    uint32_t* qbits = nullptr; // ceil(N/32)
-    uint32_t* qbuf = nullptr;  // 16 lists x qcap
-    int64_t qcap = 0;
-    uint32_t* qcnt = nullptr;  // 16 counters + scratch
+    // thinning state: bit planes packed along z (64 voxels per word)
+    uint64_t* alive = nullptr;  // alive voxels
+    uint64_t* elig = nullptr;   // eligible at the current level (lvl <= k)
+    uint64_t* sweep_removed = nullptr; // removals per sweep of a level
+    uint32_t* hcnt = nullptr;   // pinned host staging for counters
     uint64_t* tstats = nullptr; // device stats accumulators
     // scratch for the boundary stage
     void* scratch = nullptr;

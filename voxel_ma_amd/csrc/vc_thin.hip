@@ -177,163 +177,178 @@ __global__ void __launch_bounds__(kBlock)
     label[i] = in ? 1 : 0;
 }
 
-__device__ __forceinline__ uint64_t lanes_below()
+// ------------------------------------------------------------ bit planes
+// Alive and eligible voxels are bit planes packed along z: word (x, y, w)
+// holds z in [64w, 64w+64), bit z & 63; index (x*ny + y)*nwz + w.
+struct Plane
 {
-    const int lane = threadIdx.x & 63;
-    return lane ? (~0ull >> (64 - lane)) : 0ull;
+    int nx, ny, nz, nwz;
+};
+
+__device__ __forceinline__ int64_t widx(const Plane& P, int x, int y, int w)
+{
+    return ((int64_t)x * P.ny + y) * P.nwz + w;
 }
 
-// wavefront-aggregated append of `want` lanes' `val` to list `L`
-__device__ __forceinline__ void wave_append(bool want, uint32_t val, int L,
-                                            uint32_t* qcnt, uint32_t* qbuf,
-                                            int64_t qcap)
-{
-    const uint64_t m = __ballot(want);
-    if (!m)
-        return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)(threadIdx.x & 63) == leader)
-        base = atomicAdd(&qcnt[L], (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (want)
-        qbuf[(int64_t)L * qcap + base + (uint32_t)__popcll(m & lanes_below())] =
-            val;
-}
-
-// seed level k: alive border voxels with lvl == k go to list (s, parity 0)
+// one wavefront packs one word: lane l <-> z = 64w + l
 __global__ void __launch_bounds__(kBlock)
-    thin_seed(const uint8_t* __restrict__ label, const uint8_t* __restrict__ lvl,
-              uint32_t* __restrict__ qbits, uint32_t* __restrict__ qbuf,
-              uint32_t* __restrict__ qcnt, int64_t qcap, int nx, int ny, int nz,
-              vc_pack pk, int k)
+    pack_planes(const uint8_t* __restrict__ label, const uint8_t* __restrict__ lvl,
+                uint64_t* __restrict__ A, uint64_t* __restrict__ E, Plane P,
+                int k, int pack_alive)
 {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     const int x = blockIdx.z;
-    bool want = false;
-    const int64_t sy = nz, sx = (int64_t)ny * nz;
-    const int64_t i = x * sx + y * sy + z;
-    if (z < nz && label[i] == 1 && lvl[i] == k)
+    const bool in = z < P.nz;
+    const int64_t i = ((int64_t)x * P.ny + y) * P.nz + z;
+    const uint64_t e = __ballot(in && lvl[i] <= k);
+    uint64_t a = 0;
+    if (pack_alive)
+        a = __ballot(in && label[i] == 1);
+    // lane 0 sits at z = 64w: the word exists iff 64w < nz
+    if ((threadIdx.x & 63) == 0 && z < P.nz)
     {
-        bool border = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 ||
-                      z == 0 || z == nz - 1;
-        border = border || label[i - sx] != 1 || label[i + sx] != 1 ||
-                 label[i - sy] != 1 || label[i + sy] != 1 ||
-                 label[i - 1] != 1 || label[i + 1] != 1;
-        if (border)
-        {
-            atomicOr(&qbits[i >> 5], 1u << (i & 31));
-            want = true;
-        }
-    }
-    const int s = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
-    const uint32_t val = vc_pack_xyz(pk, x, y, z);
-    // the lanes of a wavefront share x and y; only the z parity differs
-#pragma unroll
-    for (int zb = 0; zb < 2; ++zb)
-    {
-        const int t = (x & 1) | ((y & 1) << 1) | (zb << 2);
-        wave_append(want && s == t, val, t * 2, qcnt, qbuf, qcap);
+        const int64_t o = widx(P, x, y, z >> 6);
+        E[o] = e;
+        if (pack_alive)
+            A[o] = a;
     }
 }
 
-// one subfield pass: list (s, par) of level k
-__global__ void __launch_bounds__(kBlock)
-    thin_pass(uint8_t* __restrict__ label, const uint8_t* __restrict__ lvl,
-              uint32_t* __restrict__ qbits, uint32_t* __restrict__ qbuf,
-              uint32_t* __restrict__ qcnt, const uint32_t* __restrict__ lut,
-              int64_t qcap, int nx, int ny, int nz, vc_pack pk, int s, int par,
-              int k, int reset_list, unsigned long long* __restrict__ stats)
+// bits z-1, z, z+1 of a row (prev/next = the neighbouring words' edge bits)
+__device__ __forceinline__ uint32_t row3(uint64_t W, uint32_t prev63,
+                                         uint32_t next0, int b)
 {
-    const int L = s * 2 + par;
-    const uint32_t n = qcnt[L];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (b == 0)
+        return prev63 | (uint32_t)((W & 3ull) << 1);
+    if (b == 63)
+        return (uint32_t)(W >> 62) | (next0 << 2);
+    return (uint32_t)((W >> (b - 1)) & 7ull);
+}
+
+// one subfield pass, dense over the rows of the subfield: thread = word
+__global__ void __launch_bounds__(kBlock)
+    thin_pass_dense(uint64_t* A, const uint64_t* __restrict__ E,
+                    uint8_t* __restrict__ label, const uint32_t* __restrict__ lut,
+                    Plane P, int s, int k, int sweep,
+                    unsigned long long* __restrict__ sweep_removed,
+                    unsigned long long* __restrict__ stats)
+{
+    // a level ends after a sweep that removed nothing; later passes are no-ops
+    if (sweep > 0 && sweep_removed[sweep - 1] == 0)
+        return;
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+    const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nxh * nyh * P.nwz;
+    uint64_t removed = 0;
+    uint32_t ntest = 0;
+    int x = 0, y = 0, w = 0;
+    if (t < total)
     {
-        if (reset_list >= 0)
-            qcnt[reset_list] = 0;
-        if (n)
-            atomicAdd(&stats[2], 1ull);
-    }
-    const int64_t sy = nz, sx = (int64_t)ny * nz;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
-    {
-        const uint32_t idx = base + threadIdx.x;
-        const bool valid = idx < n;
-        int x = 0, y = 0, z = 0;
-        int64_t i = 0;
-        bool removed = false;
-        if (valid)
+        w = (int)(t % P.nwz);
+        const int64_t r = t / P.nwz;
+        y = 2 * (int)(r % nyh) + sy;
+        x = 2 * (int)(r / nyh) + sx;
+        uint64_t R[3][3];
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+            {
+                const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
+                                (unsigned)(y + dy) < (unsigned)P.ny;
+                R[dx + 1][dy + 1] = in ? A[widx(P, x + dx, y + dy, w)] : 0ull;
+            }
+        const uint64_t C = R[1][1];
+        const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+        uint64_t cand = C & E[widx(P, x, y, w)] & zsub;
+        if (cand)
         {
-            vc_unpack_xyz(pk, qbuf[(int64_t)L * qcap + idx], x, y, z);
-            i = x * sx + y * sy + z;
-            atomicAnd(&qbits[i >> 5], ~(1u << (i & 31)));
-            uint32_t code = 0;
-            int b = 0;
+            // edge bits of the neighbouring words, only when needed
+            uint32_t pv[3][3], nx0[3][3];
+            const bool need_prev = (cand & 1ull) && w > 0;
+            const bool need_next = (cand >> 63) && w + 1 < P.nwz;
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx)
 #pragma unroll
                 for (int dy = -1; dy <= 1; ++dy)
                 {
-                    const bool rin = (unsigned)(x + dx) < (unsigned)nx &&
-                                     (unsigned)(y + dy) < (unsigned)ny;
-                    const uint8_t* row = label + (i + dx * sx + dy * sy);
-#pragma unroll
-                    for (int dz = -1; dz <= 1; ++dz)
-                    {
-                        if (dx == 0 && dy == 0 && dz == 0)
-                            continue;
-                        const bool in = rin && (unsigned)(z + dz) < (unsigned)nz;
-                        if (in && row[dz] == 1)
-                            code |= 1u << b;
-                        ++b;
-                    }
+                    const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
+                                    (unsigned)(y + dy) < (unsigned)P.ny;
+                    pv[dx + 1][dy + 1] =
+                        (in && need_prev)
+                            ? (uint32_t)(A[widx(P, x + dx, y + dy, w - 1)] >> 63)
+                            : 0u;
+                    nx0[dx + 1][dy + 1] =
+                        (in && need_next)
+                            ? (uint32_t)(A[widx(P, x + dx, y + dy, w + 1)] & 1ull)
+                            : 0u;
                 }
-            removed = (lut[code >> 5] >> (code & 31)) & 1u;
-            if (removed)
-                label[i] = (uint8_t)(2 + k);
-        }
-        {
-            const uint64_t mv = __ballot(valid);
-            const uint64_t mr = __ballot(removed);
-            if ((threadIdx.x & 63) == 0)
+            // border filter: a voxel with all six face neighbours alive is
+            // never simple
+            const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
+            const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
+            cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
+            while (cand)
             {
-                atomicAdd(&stats[0], (unsigned long long)__popcll(mv));
-                if (mr)
-                    atomicAdd(&stats[3 + k], (unsigned long long)__popcll(mr));
-            }
-        }
-        // append the neighbours of removed voxels to their next pass
+                const int b = __ffsll((unsigned long long)cand) - 1;
+                cand &= cand - 1;
+                uint32_t code = 0;
+                int pos = 0;
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx)
+                for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-                for (int dz = -1; dz <= 1; ++dz)
-                {
-                    if (dx == 0 && dy == 0 && dz == 0)
-                        continue;
-                    const int sq = s ^ ((dx & 1) | ((dy & 1) << 1) |
-                                        ((dz & 1) << 2));
-                    const int Lq = sq * 2 + (sq > s ? par : par ^ 1);
-                    bool want = false;
-                    uint32_t val = 0;
-                    if (removed && (unsigned)(x + dx) < (unsigned)nx &&
-                        (unsigned)(y + dy) < (unsigned)ny &&
-                        (unsigned)(z + dz) < (unsigned)nz)
+                    for (int dy = 0; dy < 3; ++dy)
                     {
-                        const int64_t j = i + dx * sx + dy * sy + dz;
-                        if (label[j] == 1 && lvl[j] <= k)
+                        const uint32_t r3 =
+                            row3(R[dx][dy], pv[dx][dy], nx0[dx][dy], b);
+                        if (dx == 1 && dy == 1)
                         {
-                            const uint32_t bit = 1u << (j & 31);
-                            const uint32_t old = atomicOr(&qbits[j >> 5], bit);
-                            want = !(old & bit);
-                            val = vc_pack_xyz(pk, x + dx, y + dy, z + dz);
+                            code |= ((r3 & 1u) | ((r3 >> 1) & 2u)) << pos;
+                            pos += 2;
+                        }
+                        else
+                        {
+                            code |= r3 << pos;
+                            pos += 3;
                         }
                     }
-                    wave_append(want, val, Lq, qcnt, qbuf, qcap);
+                ++ntest;
+                if ((lut[code >> 5] >> (code & 31)) & 1u)
+                    removed |= 1ull << b;
+            }
+            if (removed)
+            {
+                A[widx(P, x, y, w)] = C & ~removed;
+                uint8_t* row = label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
+                uint64_t m = removed;
+                while (m)
+                {
+                    const int b = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    row[b] = (uint8_t)(2 + k);
                 }
+            }
+        }
+    }
+    // wavefront reductions of the counters
+    uint32_t nrem = (uint32_t)__popcll(removed);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+    {
+        nrem += __shfl_xor(nrem, o);
+        ntest += __shfl_xor(ntest, o);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        if (nrem)
+        {
+            atomicAdd(&sweep_removed[sweep], (unsigned long long)nrem);
+            atomicAdd(&stats[3 + k], (unsigned long long)nrem);
+        }
+        if (ntest)
+            atomicAdd(&stats[0], (unsigned long long)ntest);
     }
 }
 
@@ -354,38 +369,35 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
     const size_t N = (size_t)v->N;
+    Plane P{v->nx, v->ny, v->nz, (v->nz + 63) / 64};
+    const size_t nwords = (size_t)v->nx * v->ny * P.nwz;
     int rc;
     if (!v->label && (rc = vc_alloc(ctx, (void**)&v->label, N)) != VC_OK)
         return rc;
     if (!v->lvl && (rc = vc_alloc(ctx, (void**)&v->lvl, N)) != VC_OK)
         return rc;
-    const size_t nwords = (N + 31) / 32;
-    if (!v->qbits &&
-        (rc = vc_alloc(ctx, (void**)&v->qbits, 4 * nwords)) != VC_OK)
+    if (!v->alive &&
+        (rc = vc_alloc(ctx, (void**)&v->alive, 8 * nwords)) != VC_OK)
         return rc;
-    if (!v->qbuf)
-    {
-        v->qcap = (int64_t)((v->nx + 1) / 2) * ((v->ny + 1) / 2) *
-                  ((v->nz + 1) / 2);
-        if ((rc = vc_alloc(ctx, (void**)&v->qbuf,
-                           sizeof(uint32_t) * 16 * (size_t)v->qcap)) != VC_OK)
-            return rc;
-    }
-    if (!v->qcnt && (rc = vc_alloc(ctx, (void**)&v->qcnt, 64 * 4)) != VC_OK)
+    if (!v->elig && (rc = vc_alloc(ctx, (void**)&v->elig, 8 * nwords)) != VC_OK)
+        return rc;
+    const int kMaxSweeps = 1 << 16;
+    if (!v->sweep_removed &&
+        (rc = vc_alloc(ctx, (void**)&v->sweep_removed,
+                       8 * (size_t)kMaxSweeps)) != VC_OK)
         return rc;
     if (!v->tstats &&
         (rc = vc_alloc(ctx, (void**)&v->tstats, 8 * (3 + VC_MAX_LEVELS))) !=
             VC_OK)
         return rc;
-    uint32_t* hcnt = nullptr;
-    VC_HIP(ctx, hipHostMalloc((void**)&hcnt, 64 * 4, hipHostMallocDefault));
+    if (!v->hcnt)
+        VC_HIP(ctx, hipHostMalloc((void**)&v->hcnt, 8 * 64,
+                                  hipHostMallocDefault));
+    unsigned long long* hsw = (unsigned long long*)v->hcnt;
+    unsigned long long* swr = (unsigned long long*)v->sweep_removed;
 
     hipStream_t S = ctx->stream;
-    hipError_t e = hipMemsetAsync(v->qbits, 0, 4 * nwords, S);
-    if (e == hipSuccess)
-        e = hipMemsetAsync(v->qcnt, 0, 64 * 4, S);
-    if (e == hipSuccess)
-        e = hipMemsetAsync(v->tstats, 0, 8 * (3 + VC_MAX_LEVELS), S);
+    hipError_t e = hipMemsetAsync(v->tstats, 0, 8 * (3 + VC_MAX_LEVELS), S);
     int64_t launches = 0;
     const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
     if (e == hipSuccess)
@@ -409,57 +421,62 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
         launches++;
         e = hipGetLastError();
     }
-    const dim3 gseed((unsigned)((v->nz + kBlock - 1) / kBlock), (unsigned)v->ny,
+    const dim3 gpack((unsigned)((v->nz + kBlock - 1) / kBlock), (unsigned)v->ny,
                      (unsigned)v->nx);
-    const unsigned gpass = (unsigned)(ctx->cu_count > 0 ? ctx->cu_count * 4 : 1024);
-    const int kBatch = 16; // two sweeps, keeps (s, parity) aligned per batch
-    const int64_t kMaxPasses = int64_t(1) << 24;
+    int64_t passes_total = 0;
     for (int k = 0; k < n && e == hipSuccess; ++k)
     {
-        if (k > 0)
-            e = hipMemsetAsync(v->qcnt, 0, 16 * 4, S);
-        if (e != hipSuccess)
-            break;
-        thin_seed<<<gseed, kBlock, 0, S>>>(v->label, v->lvl, v->qbits, v->qbuf,
-                                            v->qcnt, v->qcap, v->nx, v->ny,
-                                            v->nz, v->pk, k);
+        pack_planes<<<gpack, kBlock, 0, S>>>(v->label, v->lvl, v->alive,
+                                              v->elig, P, k, k == 0);
         launches++;
         e = hipGetLastError();
-        int64_t t = 0;
-        int last = -1;
+        if (e == hipSuccess)
+            e = hipMemsetAsync(swr, 0, 8 * (size_t)kMaxSweeps, S);
+        // launch sweeps in batches; a sweep after the converged one is a
+        // no-op (every pass exits at its first instruction)
+        int sweep = 0;
+        int batch = 4;
         while (e == hipSuccess)
         {
-            for (int b = 0; b < kBatch && e == hipSuccess; ++b, ++t)
-            {
-                const int s = (int)(t & 7), par = (int)((t >> 3) & 1);
-                thin_pass<<<gpass, kBlock, 0, S>>>(
-                    v->label, v->lvl, v->qbits, v->qbuf, v->qcnt, ctx->d_lut,
-                    v->qcap, v->nx, v->ny, v->nz, v->pk, s, par, k, last,
-                    (unsigned long long*)v->tstats);
-                launches++;
-                e = hipGetLastError();
-                last = s * 2 + par;
-            }
+            if (sweep + batch > kMaxSweeps)
+                return vc_fail(ctx, VC_ERR_GUARD,
+                               "thinning did not converge in %d sweeps", sweep);
+            for (int j = 0; j < batch && e == hipSuccess; ++j, ++sweep)
+                for (int s = 0; s < 8 && e == hipSuccess; ++s)
+                {
+                    const int sx = s & 1, sy = (s >> 1) & 1;
+                    const int64_t words = (int64_t)((v->nx - sx + 1) >> 1) *
+                                          ((v->ny - sy + 1) >> 1) * P.nwz;
+                    const unsigned g = (unsigned)((words + kBlock - 1) / kBlock);
+                    if (g == 0)
+                        continue;
+                    thin_pass_dense<<<g, kBlock, 0, S>>>(
+                        v->alive, v->elig, v->label, ctx->d_lut, P, s, k, sweep,
+                        swr, (unsigned long long*)v->tstats);
+                    launches++;
+                    e = hipGetLastError();
+                }
             if (e != hipSuccess)
                 break;
-            e = hipMemcpyAsync(hcnt, v->qcnt, 16 * 4, hipMemcpyDeviceToHost, S);
+            e = hipMemcpyAsync(hsw, swr + sweep - batch, 8 * (size_t)batch,
+                               hipMemcpyDeviceToHost, S);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(S);
             if (e != hipSuccess)
                 break;
-            bool done = true;
-            for (int L = 0; L < 16; ++L)
-                if (L != last && hcnt[L] != 0)
-                    done = false;
-            if (done)
-                break;
-            if (t > kMaxPasses)
+            int conv = -1;
+            for (int j = 0; j < batch; ++j)
+                if (hsw[j] == 0)
+                {
+                    conv = sweep - batch + j;
+                    break;
+                }
+            if (conv >= 0)
             {
-                (void)hipHostFree(hcnt);
-                return vc_fail(ctx, VC_ERR_GUARD,
-                               "thinning did not converge in %lld passes",
-                               (long long)t);
+                passes_total += 8LL * (conv + 1);
+                break;
             }
+            batch = batch < 16 ? batch * 2 : 16;
         }
     }
     if (e == hipSuccess && st)
@@ -472,13 +489,12 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
         {
             std::memset(st, 0, sizeof(*st));
             st->tests = (int64_t)hs[0];
-            st->passes = (int64_t)hs[2];
+            st->passes = passes_total;
             st->launches = launches;
             for (int k = 0; k < n; ++k)
                 st->removed[k] = (int64_t)hs[3 + k];
         }
     }
-    (void)hipHostFree(hcnt);
     if (e != hipSuccess)
         return vc_fail(ctx, VC_ERR_HIP, "thinning: %s", hipGetErrorString(e));
     v->thin_done = true;
