@@ -264,6 +264,7 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->label);
     vc_free(v->alive);
     vc_free(v->elig);
+    vc_free(v->dflag);
     vc_free(v->sweep_removed);
     if (v->hcnt)
         (void)hipHostFree(v->hcnt);

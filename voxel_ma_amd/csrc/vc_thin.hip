@@ -190,28 +190,34 @@ __device__ __forceinline__ int64_t widx(const Plane& P, int x, int y, int w)
     return ((int64_t)x * P.ny + y) * P.nwz + w;
 }
 
-// one wavefront packs one word: lane l <-> z = 64w + l
+// one wavefront packs one word: lane l <-> z = 64w + l.  Words holding an
+// alive voxel that becomes eligible at level k are flagged for both z
+// parities (every other eligible voxel was found non-simple when the previous
+// level converged and its neighbourhood has not changed since).
 __global__ void __launch_bounds__(kBlock)
     pack_planes(const uint8_t* __restrict__ label, const uint8_t* __restrict__ lvl,
-                uint64_t* __restrict__ A, uint64_t* __restrict__ E, Plane P,
-                int k, int pack_alive)
+                uint64_t* __restrict__ A, uint64_t* __restrict__ E,
+                uint8_t* __restrict__ D0, uint8_t* __restrict__ D1, Plane P,
+                int k)
 {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     const int x = blockIdx.z;
     const bool in = z < P.nz;
     const int64_t i = ((int64_t)x * P.ny + y) * P.nz + z;
-    const uint64_t e = __ballot(in && lvl[i] <= k);
-    uint64_t a = 0;
-    if (pack_alive)
-        a = __ballot(in && label[i] == 1);
+    const uint8_t l = in ? lvl[i] : (uint8_t)255;
+    const uint64_t e = __ballot(l <= k);
+    const uint64_t enew = __ballot(l == k);
+    const uint64_t a = __ballot(in && label[i] == 1);
     // lane 0 sits at z = 64w: the word exists iff 64w < nz
     if ((threadIdx.x & 63) == 0 && z < P.nz)
     {
         const int64_t o = widx(P, x, y, z >> 6);
         E[o] = e;
-        if (pack_alive)
-            A[o] = a;
+        A[o] = a;
+        const uint8_t f = (enew & a) ? 1 : 0;
+        D0[o] = f;
+        D1[o] = f;
     }
 }
 
@@ -226,12 +232,45 @@ __device__ __forceinline__ uint32_t row3(uint64_t W, uint32_t prev63,
     return (uint32_t)((W >> (b - 1)) & 7ull);
 }
 
-// one subfield pass, dense over the rows of the subfield: thread = word
+// 26-neighbourhood code of bit b (bit order dx, dy, dz with dz fastest,
+// centre skipped -- the order of the simple-point table)
+__device__ __forceinline__ uint32_t nb_code(const uint64_t (&R)[3][3],
+                                            const uint32_t (&pv)[3][3],
+                                            const uint32_t (&nx0)[3][3], int b)
+{
+    uint32_t code = 0;
+    int pos = 0;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+        {
+            const uint32_t r3 = row3(R[dx][dy], pv[dx][dy], nx0[dx][dy], b);
+            if (dx == 1 && dy == 1)
+            {
+                code |= ((r3 & 1u) | ((r3 >> 1) & 2u)) << pos;
+                pos += 2;
+            }
+            else
+            {
+                code |= r3 << pos;
+                pos += 3;
+            }
+        }
+    return code;
+}
+
+// One subfield pass, dense over the words of the subfield's rows (thread =
+// word).  A word is visited only when its z-parity flag is set: the flag of
+// z-parity p (D0 / D1) is raised whenever a voxel of the 26-neighbourhood of
+// one of its parity-p voxels is removed.  All flag traffic is plain byte
+// stores: in pass (sx, sy, sz) the flags of parity sz are cleared only on
+// rows of parity (sx, sy), and raised only on the other rows.
 __global__ void __launch_bounds__(kBlock)
     thin_pass_dense(uint64_t* A, const uint64_t* __restrict__ E,
-                    uint8_t* __restrict__ label, const uint32_t* __restrict__ lut,
-                    Plane P, int s, int k, int sweep,
-                    unsigned long long* __restrict__ sweep_removed,
+                    uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1,
+                    const uint32_t* __restrict__ lut, Plane P, int s, int k,
+                    int sweep, unsigned long long* __restrict__ sweep_removed,
                     unsigned long long* __restrict__ stats)
 {
     // a level ends after a sweep that removed nothing; later passes are no-ops
@@ -241,31 +280,29 @@ __global__ void __launch_bounds__(kBlock)
     const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)nxh * nyh * P.nwz;
+    uint8_t* Fs = sz ? D1 : D0; // this pass's parity
+    uint8_t* Fo = sz ? D0 : D1; // the other parity
     uint64_t removed = 0;
     uint32_t ntest = 0;
-    int x = 0, y = 0, w = 0;
     if (t < total)
     {
-        w = (int)(t % P.nwz);
+        const int w = (int)(t % P.nwz);
         const int64_t r = t / P.nwz;
-        y = 2 * (int)(r % nyh) + sy;
-        x = 2 * (int)(r / nyh) + sx;
-        uint64_t R[3][3];
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
-            {
-                const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
-                                (unsigned)(y + dy) < (unsigned)P.ny;
-                R[dx + 1][dy + 1] = in ? A[widx(P, x + dx, y + dy, w)] : 0ull;
-            }
-        const uint64_t C = R[1][1];
-        const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
-        uint64_t cand = C & E[widx(P, x, y, w)] & zsub;
+        const int y = 2 * (int)(r % nyh) + sy;
+        const int x = 2 * (int)(r / nyh) + sx;
+        const int64_t o = widx(P, x, y, w);
+        uint64_t C = 0, cand = 0;
+        if (Fs[o])
+        {
+            Fs[o] = 0;
+            C = A[o];
+            const uint64_t zsub =
+                sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+            cand = C & E[o] & zsub;
+        }
         if (cand)
         {
-            // edge bits of the neighbouring words, only when needed
+            uint64_t R[3][3];
             uint32_t pv[3][3], nx0[3][3];
             const bool need_prev = (cand & 1ull) && w > 0;
             const bool need_next = (cand >> 63) && w + 1 < P.nwz;
@@ -276,51 +313,51 @@ __global__ void __launch_bounds__(kBlock)
                 {
                     const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
                                     (unsigned)(y + dy) < (unsigned)P.ny;
+                    const int64_t q = widx(P, x + dx, y + dy, w);
+                    R[dx + 1][dy + 1] =
+                        (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
                     pv[dx + 1][dy + 1] =
-                        (in && need_prev)
-                            ? (uint32_t)(A[widx(P, x + dx, y + dy, w - 1)] >> 63)
-                            : 0u;
+                        (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
                     nx0[dx + 1][dy + 1] =
-                        (in && need_next)
-                            ? (uint32_t)(A[widx(P, x + dx, y + dy, w + 1)] & 1ull)
-                            : 0u;
+                        (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
                 }
             // border filter: a voxel with all six face neighbours alive is
             // never simple
             const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
             const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
             cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
+            // four independent table lookups in flight per round
             while (cand)
             {
-                const int b = __ffsll((unsigned long long)cand) - 1;
-                cand &= cand - 1;
-                uint32_t code = 0;
-                int pos = 0;
+                int bb[4];
+                uint32_t cc[4];
+                int nb = 0;
 #pragma unroll
-                for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                    for (int dy = 0; dy < 3; ++dy)
+                for (int q = 0; q < 4; ++q)
+                {
+                    bb[q] = 0;
+                    cc[q] = 0;
+                    if (cand)
                     {
-                        const uint32_t r3 =
-                            row3(R[dx][dy], pv[dx][dy], nx0[dx][dy], b);
-                        if (dx == 1 && dy == 1)
-                        {
-                            code |= ((r3 & 1u) | ((r3 >> 1) & 2u)) << pos;
-                            pos += 2;
-                        }
-                        else
-                        {
-                            code |= r3 << pos;
-                            pos += 3;
-                        }
+                        bb[q] = __ffsll((unsigned long long)cand) - 1;
+                        cand &= cand - 1;
+                        cc[q] = nb_code(R, pv, nx0, bb[q]);
+                        nb = q + 1;
                     }
-                ++ntest;
-                if ((lut[code >> 5] >> (code & 31)) & 1u)
-                    removed |= 1ull << b;
+                }
+                uint32_t lw[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    lw[q] = q < nb ? lut[cc[q] >> 5] : 0u;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if ((lw[q] >> (cc[q] & 31)) & 1u)
+                        removed |= 1ull << bb[q];
+                ntest += nb;
             }
             if (removed)
             {
-                A[widx(P, x, y, w)] = C & ~removed;
+                A[o] = C & ~removed;
                 uint8_t* row = label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
                 uint64_t m = removed;
                 while (m)
@@ -329,6 +366,42 @@ __global__ void __launch_bounds__(kBlock)
                     m &= m - 1;
                     row[b] = (uint8_t)(2 + k);
                 }
+                // raise the flags of every word holding a neighbour
+                const bool lo = (removed & 1ull) && w > 0;
+                const bool hi = (removed >> 63) && w + 1 < P.nwz;
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy)
+                    {
+                        if ((unsigned)(x + dx) >= (unsigned)P.nx ||
+                            (unsigned)(y + dy) >= (unsigned)P.ny)
+                            continue;
+                        const int64_t q = widx(P, x + dx, y + dy, w);
+                        if (dx == 0 && dy == 0)
+                        {
+                            Fo[q] = 1;
+                            if (lo)
+                                Fo[q - 1] = 1;
+                            if (hi)
+                                Fo[q + 1] = 1;
+                        }
+                        else
+                        {
+                            D0[q] = 1;
+                            D1[q] = 1;
+                            if (lo)
+                            {
+                                D0[q - 1] = 1;
+                                D1[q - 1] = 1;
+                            }
+                            if (hi)
+                            {
+                                D0[q + 1] = 1;
+                                D1[q + 1] = 1;
+                            }
+                        }
+                    }
             }
         }
     }
@@ -381,6 +454,8 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
         return rc;
     if (!v->elig && (rc = vc_alloc(ctx, (void**)&v->elig, 8 * nwords)) != VC_OK)
         return rc;
+    if (!v->dflag && (rc = vc_alloc(ctx, (void**)&v->dflag, 2 * nwords)) != VC_OK)
+        return rc;
     const int kMaxSweeps = 1 << 16;
     if (!v->sweep_removed &&
         (rc = vc_alloc(ctx, (void**)&v->sweep_removed,
@@ -427,7 +502,8 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
     for (int k = 0; k < n && e == hipSuccess; ++k)
     {
         pack_planes<<<gpack, kBlock, 0, S>>>(v->label, v->lvl, v->alive,
-                                              v->elig, P, k, k == 0);
+                                              v->elig, v->dflag, v->dflag + nwords,
+                                              P, k);
         launches++;
         e = hipGetLastError();
         if (e == hipSuccess)
@@ -451,8 +527,9 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
                     if (g == 0)
                         continue;
                     thin_pass_dense<<<g, kBlock, 0, S>>>(
-                        v->alive, v->elig, v->label, ctx->d_lut, P, s, k, sweep,
-                        swr, (unsigned long long*)v->tstats);
+                        v->alive, v->elig, v->label, v->dflag,
+                        v->dflag + nwords, ctx->d_lut, P, s, k, sweep, swr,
+                        (unsigned long long*)v->tstats);
                     launches++;
                     e = hipGetLastError();
                 }
