@@ -20,6 +20,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 EDT_BYTES_PER_VOXEL = 21  # (1+2) + (2+4) + (4+4+4), see DESIGN.md
+# dense subfield pass over bit planes: alive state (1/8 B) read with its
+# neighbour rows amortised (x 9/8) + the eligibility plane (1/8 B) / 8
+THIN_BYTES_PER_VOXEL_PASS = (1.0 / 8) * (9.0 / 8) + (1.0 / 8) / 8
 
 
 class HipEvents:
@@ -152,6 +155,10 @@ def main():
     value = N / (ms_step * 1e-3) / 1e6
 
     edt_gbs = EDT_BYTES_PER_VOXEL * N / (edt_ms * 1e-3) / 1e9
+    # thinning: average duration of one subfield-pass launch over the timed
+    # region (host waits between batches included), algorithmic bytes per pass
+    pass_ms = thin_ms / max(1, stats.passes)
+    thin_gbs = THIN_BYTES_PER_VOXEL_PASS * N / (pass_ms * 1e-3) / 1e9
     out = {
         "metric": "Mvoxels/s EDT+thinning at 512^3",
         "value": round(value, 2),
@@ -175,9 +182,14 @@ def main():
         "thin_stats": {"removed": [int(x) for x in stats.removed[:len(levels)]],
                        "passes_with_work": int(stats.passes), "tests": int(stats.tests),
                        "launches": int(stats.launches)},
-        "roofline": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(edt_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "thin_pass_dense (subfield pass)",
+                     "achieved": round(thin_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(thin_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_voxel_pass": round(THIN_BYTES_PER_VOXEL_PASS, 4),
+                     "avg_pass_us": round(pass_ms * 1e3, 2)},
+        "roofline_edt": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21},
     }
     if not (a.no_cpu_baseline or a.profile_steps):
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -1,6 +1,7 @@
 // vc_core.hip -- context, volumes, results and error handling of the C ABI.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -265,7 +266,7 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->alive);
     vc_free(v->elig);
     vc_free(v->dflag);
-    vc_free(v->sweep_removed);
+    vc_free(v->sweep_any);
     if (v->hcnt)
         (void)hipHostFree(v->hcnt);
     vc_free(v->tstats);

@@ -44,7 +44,7 @@ struct vc_vol
     uint64_t* alive = nullptr;  // alive voxels
     uint64_t* elig = nullptr;   // eligible at the current level (lvl <= k)
     uint8_t* dflag = nullptr;   // 2 x words: revisit flags per z parity
-    uint64_t* sweep_removed = nullptr; // removals per sweep of a level
+    uint32_t* sweep_any = nullptr; // per sweep of a level: removed anything
     uint32_t* hcnt = nullptr;   // pinned host staging for counters
     uint64_t* tstats = nullptr; // device stats accumulators
     // scratch for the boundary stage

@@ -8,17 +8,18 @@
 // (lvl <= k) and (26,6)-simple in the current image.  Voxels of one subfield
 // are never 26-adjacent, so the parallel removal equals any sequential order.
 //
-// Work-efficient execution: a voxel's test result can only change when a
-// voxel of its 26-neighbourhood is removed, so instead of sweeping the volume
-// every pass the kernels keep 16 work lists (8 subfields x sweep parity).  A
-// level is seeded with the border voxels that became eligible at that level;
-// every removal appends its alive, eligible, not-yet-queued neighbours to the
-// list of the next pass of their subfield.  A level ends when every list is
-// empty, which is exactly when a dense sweep would remove nothing.  The
-// neighbour appends of a pass all target the same list per neighbour offset
-// (the subfield of p + offset is s XOR parity(offset)), so they are
-// aggregated per wavefront with one atomic per list.
+// Execution: the alive and the eligible (lvl <= k) voxels are bit planes
+// packed along z, 64 voxels per word.  A pass is one launch over the words of
+// the rows of its (x, y) parity, one thread per word: candidates of the pass's
+// z parity are filtered with word operations (alive, eligible, at least one
+// background face neighbour), and only those go through the simple-point
+// table.  The thread owns its word for the pass, so removals need no atomics.
+// Per-word revisit flags (one byte per word and z parity, plain stores) skip
+// words whose neighbourhood did not change since their last visit.  A level
+// ends after a sweep that removed nothing (later passes exit at once); the
+// host checks the per-sweep "removed anything" flags once per batch of sweeps.
 #include <cstring>
+#include <vector>
 
 #include "vc_internal.h"
 
@@ -260,36 +261,58 @@ __device__ __forceinline__ uint32_t nb_code(const uint64_t (&R)[3][3],
     return code;
 }
 
+// Counters are spread over kSlots cache lines (one atomic per workgroup into
+// slot blockIdx % kSlots): same-address device-scope atomics from every
+// wavefront serialise across the XCDs.
+constexpr int kSlots = 64;
+constexpr int kSlotWords = 8 + VC_MAX_LEVELS; // [0] tests, [8 + k] removed
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        v += __shfl_xor(v, d);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        red[wv] = v;
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i)
+        tot += red[i];
+    return tot;
+}
+
 // One subfield pass, dense over the words of the subfield's rows (thread =
-// word).  A word is visited only when its z-parity flag is set: the flag of
-// z-parity p (D0 / D1) is raised whenever a voxel of the 26-neighbourhood of
-// one of its parity-p voxels is removed.  All flag traffic is plain byte
-// stores: in pass (sx, sy, sz) the flags of parity sz are cleared only on
-// rows of parity (sx, sy), and raised only on the other rows.
+// word; grid y = row x/2, grid x covers (y/2, w) with w padded to 2^lgw).
+// A word is visited only when its z-parity flag is set: the flag of z-parity
+// p (D0 / D1) is raised whenever a voxel of the 26-neighbourhood of one of
+// its parity-p voxels is removed.  All flag traffic is plain byte stores: in
+// pass (sx, sy, sz) the flags of parity sz are cleared only on rows of parity
+// (sx, sy), and raised only on other rows.
 __global__ void __launch_bounds__(kBlock)
     thin_pass_dense(uint64_t* A, const uint64_t* __restrict__ E,
                     uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1,
-                    const uint32_t* __restrict__ lut, Plane P, int s, int k,
-                    int sweep, unsigned long long* __restrict__ sweep_removed,
-                    unsigned long long* __restrict__ stats)
+                    const uint32_t* __restrict__ lut, Plane P, int lgw, int s,
+                    int k, int sweep, uint32_t* __restrict__ sweep_any,
+                    unsigned long long* __restrict__ ctr)
 {
     // a level ends after a sweep that removed nothing; later passes are no-ops
-    if (sweep > 0 && sweep_removed[sweep - 1] == 0)
+    if (sweep > 0 && sweep_any[sweep - 1] == 0)
         return;
+    __shared__ uint32_t red[kBlock / 64];
     const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
-    const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = (int64_t)nxh * nyh * P.nwz;
+    const int nyh = (P.ny - sy + 1) >> 1;
+    const uint32_t tt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = (int)(tt & ((1u << lgw) - 1u));
+    const int yh = (int)(tt >> lgw);
+    const int x = 2 * (int)blockIdx.y + sx;
+    const int y = 2 * yh + sy;
     uint8_t* Fs = sz ? D1 : D0; // this pass's parity
     uint8_t* Fo = sz ? D0 : D1; // the other parity
     uint64_t removed = 0;
     uint32_t ntest = 0;
-    if (t < total)
+    if (yh < nyh && w < P.nwz)
     {
-        const int w = (int)(t % P.nwz);
-        const int64_t r = t / P.nwz;
-        const int y = 2 * (int)(r % nyh) + sy;
-        const int x = 2 * (int)(r / nyh) + sx;
         const int64_t o = widx(P, x, y, w);
         uint64_t C = 0, cand = 0;
         if (Fs[o])
@@ -333,26 +356,26 @@ __global__ void __launch_bounds__(kBlock)
                 uint32_t cc[4];
                 int nb = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int j = 0; j < 4; ++j)
                 {
-                    bb[q] = 0;
-                    cc[q] = 0;
+                    bb[j] = 0;
+                    cc[j] = 0;
                     if (cand)
                     {
-                        bb[q] = __ffsll((unsigned long long)cand) - 1;
+                        bb[j] = __ffsll((unsigned long long)cand) - 1;
                         cand &= cand - 1;
-                        cc[q] = nb_code(R, pv, nx0, bb[q]);
-                        nb = q + 1;
+                        cc[j] = nb_code(R, pv, nx0, bb[j]);
+                        nb = j + 1;
                     }
                 }
                 uint32_t lw[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    lw[q] = q < nb ? lut[cc[q] >> 5] : 0u;
+                for (int j = 0; j < 4; ++j)
+                    lw[j] = j < nb ? lut[cc[j] >> 5] : 0u;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if ((lw[q] >> (cc[q] & 31)) & 1u)
-                        removed |= 1ull << bb[q];
+                for (int j = 0; j < 4; ++j)
+                    if ((lw[j] >> (cc[j] & 31)) & 1u)
+                        removed |= 1ull << bb[j];
                 ntest += nb;
             }
             if (removed)
@@ -405,23 +428,21 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     }
-    // wavefront reductions of the counters
-    uint32_t nrem = (uint32_t)__popcll(removed);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
+    const uint32_t nrem = block_sum((uint32_t)__popcll(removed), red);
+    __syncthreads();
+    const uint32_t ntst = block_sum(ntest, red);
+    if (threadIdx.x == 0)
     {
-        nrem += __shfl_xor(nrem, o);
-        ntest += __shfl_xor(ntest, o);
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
+        unsigned long long* c =
+            ctr + (size_t)((blockIdx.x + blockIdx.y * gridDim.x) % kSlots) *
+                      kSlotWords;
         if (nrem)
         {
-            atomicAdd(&sweep_removed[sweep], (unsigned long long)nrem);
-            atomicAdd(&stats[3 + k], (unsigned long long)nrem);
+            sweep_any[sweep] = 1u;
+            atomicAdd(&c[8 + k], (unsigned long long)nrem);
         }
-        if (ntest)
-            atomicAdd(&stats[0], (unsigned long long)ntest);
+        if (ntst)
+            atomicAdd(&c[0], (unsigned long long)ntst);
     }
 }
 
@@ -443,6 +464,9 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
     VC_HIP(ctx, hipSetDevice(ctx->device));
     const size_t N = (size_t)v->N;
     Plane P{v->nx, v->ny, v->nz, (v->nz + 63) / 64};
+    int lgw = 0;
+    while ((1 << lgw) < P.nwz)
+        lgw++;
     const size_t nwords = (size_t)v->nx * v->ny * P.nwz;
     int rc;
     if (!v->label && (rc = vc_alloc(ctx, (void**)&v->label, N)) != VC_OK)
@@ -457,22 +481,21 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
     if (!v->dflag && (rc = vc_alloc(ctx, (void**)&v->dflag, 2 * nwords)) != VC_OK)
         return rc;
     const int kMaxSweeps = 1 << 16;
-    if (!v->sweep_removed &&
-        (rc = vc_alloc(ctx, (void**)&v->sweep_removed,
-                       8 * (size_t)kMaxSweeps)) != VC_OK)
-        return rc;
-    if (!v->tstats &&
-        (rc = vc_alloc(ctx, (void**)&v->tstats, 8 * (3 + VC_MAX_LEVELS))) !=
+    if (!v->sweep_any &&
+        (rc = vc_alloc(ctx, (void**)&v->sweep_any, 4 * (size_t)kMaxSweeps)) !=
             VC_OK)
         return rc;
+    const size_t ctr_bytes = 8 * (size_t)kSlots * kSlotWords;
+    if (!v->tstats &&
+        (rc = vc_alloc(ctx, (void**)&v->tstats, ctr_bytes)) != VC_OK)
+        return rc;
     if (!v->hcnt)
-        VC_HIP(ctx, hipHostMalloc((void**)&v->hcnt, 8 * 64,
+        VC_HIP(ctx, hipHostMalloc((void**)&v->hcnt, 4 * 64,
                                   hipHostMallocDefault));
-    unsigned long long* hsw = (unsigned long long*)v->hcnt;
-    unsigned long long* swr = (unsigned long long*)v->sweep_removed;
+    uint32_t* hsw = v->hcnt;
 
     hipStream_t S = ctx->stream;
-    hipError_t e = hipMemsetAsync(v->tstats, 0, 8 * (3 + VC_MAX_LEVELS), S);
+    hipError_t e = hipMemsetAsync(v->tstats, 0, ctr_bytes, S);
     int64_t launches = 0;
     const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
     if (e == hipSuccess)
@@ -507,7 +530,7 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
         launches++;
         e = hipGetLastError();
         if (e == hipSuccess)
-            e = hipMemsetAsync(swr, 0, 8 * (size_t)kMaxSweeps, S);
+            e = hipMemsetAsync(v->sweep_any, 0, 4 * (size_t)kMaxSweeps, S);
         // launch sweeps in batches; a sweep after the converged one is a
         // no-op (every pass exits at its first instruction)
         int sweep = 0;
@@ -521,22 +544,24 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
                 for (int s = 0; s < 8 && e == hipSuccess; ++s)
                 {
                     const int sx = s & 1, sy = (s >> 1) & 1;
-                    const int64_t words = (int64_t)((v->nx - sx + 1) >> 1) *
-                                          ((v->ny - sy + 1) >> 1) * P.nwz;
-                    const unsigned g = (unsigned)((words + kBlock - 1) / kBlock);
-                    if (g == 0)
+                    const int nxh = (v->nx - sx + 1) >> 1;
+                    const int nyh = (v->ny - sy + 1) >> 1;
+                    if (nxh == 0 || nyh == 0)
                         continue;
+                    const int64_t per_row = (int64_t)nyh << lgw;
+                    const dim3 g((unsigned)((per_row + kBlock - 1) / kBlock),
+                                 (unsigned)nxh);
                     thin_pass_dense<<<g, kBlock, 0, S>>>(
                         v->alive, v->elig, v->label, v->dflag,
-                        v->dflag + nwords, ctx->d_lut, P, s, k, sweep, swr,
-                        (unsigned long long*)v->tstats);
+                        v->dflag + nwords, ctx->d_lut, P, lgw, s, k, sweep,
+                        v->sweep_any, (unsigned long long*)v->tstats);
                     launches++;
                     e = hipGetLastError();
                 }
             if (e != hipSuccess)
                 break;
-            e = hipMemcpyAsync(hsw, swr + sweep - batch, 8 * (size_t)batch,
-                               hipMemcpyDeviceToHost, S);
+            e = hipMemcpyAsync(hsw, v->sweep_any + sweep - batch,
+                               4 * (size_t)batch, hipMemcpyDeviceToHost, S);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(S);
             if (e != hipSuccess)
@@ -558,18 +583,23 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
     }
     if (e == hipSuccess && st)
     {
-        uint64_t hs[3 + VC_MAX_LEVELS];
-        e = hipMemcpyAsync(hs, v->tstats, sizeof(hs), hipMemcpyDeviceToHost, S);
+        std::vector<unsigned long long> hs((size_t)kSlots * kSlotWords);
+        e = hipMemcpyAsync(hs.data(), v->tstats, ctr_bytes,
+                           hipMemcpyDeviceToHost, S);
         if (e == hipSuccess)
             e = hipStreamSynchronize(S);
         if (e == hipSuccess)
         {
             std::memset(st, 0, sizeof(*st));
-            st->tests = (int64_t)hs[0];
+            for (int sl = 0; sl < kSlots; ++sl)
+            {
+                const unsigned long long* c = &hs[(size_t)sl * kSlotWords];
+                st->tests += (int64_t)c[0];
+                for (int k = 0; k < n; ++k)
+                    st->removed[k] += (int64_t)c[8 + k];
+            }
             st->passes = passes_total;
             st->launches = launches;
-            for (int k = 0; k < n; ++k)
-                st->removed[k] = (int64_t)hs[3 + k];
         }
     }
     if (e != hipSuccess)
