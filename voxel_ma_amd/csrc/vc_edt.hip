@@ -20,7 +20,21 @@ namespace
 {
 
 constexpr int kBlock = 256;
-constexpr size_t kLdsBudget = 64 * 1024;
+// column tiles are capped at 32 KB of LDS so that several workgroups per CU
+// overlap their load and search phases
+constexpr size_t kLdsBudget = 32 * 1024;
+constexpr uint32_t kOob = 0xFFFFFFFEu; // neighbour outside the grid
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Consecutive tiles (which share cache lines and halos) are given to
+// workgroups of the same XCD: block L -> tile (L % 8) * per + L / 8.
+__device__ __forceinline__ int64_t xcd_tile(int64_t ntiles)
+{
+    const int64_t L = blockIdx.x;
+    const int64_t per = (ntiles + 7) / 8;
+    const int64_t T = (L % 8) * per + L / 8;
+    return T < ntiles ? T : -1;
+}
 // every candidate cost fits in int32: the volume constructor rejects shapes
 // whose squared diagonal exceeds INT32_MAX
 constexpr int kInf = 0x7FFFFFFF;
@@ -96,11 +110,15 @@ __global__ void __launch_bounds__(kBlock)
 // columns along y: tile = TZ consecutive z at fixed x, all y
 __global__ void __launch_bounds__(kBlock)
     edt_pass_y(const int16_t* __restrict__ zq, uint32_t* __restrict__ yz,
-               int ny, int nz, int TZ)
+               int nx, int ny, int nz, int TZ)
 {
     extern __shared__ int16_t colz[]; // [ny][TZ]
-    const int z0 = blockIdx.x * TZ;
-    const int x = blockIdx.y;
+    const int nzt = (nz + TZ - 1) / TZ;
+    const int64_t T = xcd_tile((int64_t)nx * nzt);
+    if (T < 0)
+        return;
+    const int z0 = (int)(T % nzt) * TZ;
+    const int x = (int)(T / nzt);
     const int tz = threadIdx.x % TZ;
     const int ty0 = threadIdx.x / TZ;
     const int tys = blockDim.x / TZ;
@@ -173,8 +191,12 @@ __global__ void __launch_bounds__(kBlock)
                vc_pack pk)
 {
     extern __shared__ uint32_t colyz[]; // [nx][TZ]
-    const int z0 = blockIdx.x * TZ;
-    const int y = blockIdx.y;
+    const int nzt = (nz + TZ - 1) / TZ;
+    const int64_t T = xcd_tile((int64_t)ny * nzt);
+    if (T < 0)
+        return;
+    const int z0 = (int)(T % nzt) * TZ;
+    const int y = (int)(T / nzt);
     const int tz = threadIdx.x % TZ;
     const int tx0 = threadIdx.x / TZ;
     const int txs = blockDim.x / TZ;
@@ -248,45 +270,96 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-    measure_lam2(const uint8_t* __restrict__ occ, const uint32_t* __restrict__ ft,
-                 uint32_t* __restrict__ lam2, int nx, int ny, int nz,
-                 vc_pack pk)
+// Per-voxel measure with an x-sliding window: a 64(z) x 8(y) workgroup walks
+// a chunk of kMeasXC slices; FT(x-1), FT(x), FT(x+1) stay in registers and
+// the y/z neighbours of slice x come from a 10 x 66 LDS window, so each FT
+// value is read about once.  A voxel q is background iff FT(q) == q, so no
+// occupancy is read.
+constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
+
+__global__ void __launch_bounds__(kMeasZ * kMeasY)
+    measure_lam2(const uint32_t* __restrict__ ft, uint32_t* __restrict__ lam2,
+                 int nx, int ny, int nz, vc_pack pk)
 {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int x = blockIdx.z;
-    if (z >= nz)
+    __shared__ uint32_t S[kMeasY + 2][kMeasZ + 2];
+    const int nzt = (nz + kMeasZ - 1) / kMeasZ;
+    const int nyt = (ny + kMeasY - 1) / kMeasY;
+    const int nxc = (nx + kMeasXC - 1) / kMeasXC;
+    const int64_t T = xcd_tile((int64_t)nzt * nyt * nxc);
+    if (T < 0)
         return;
-    const int64_t sy = nz, sx = (int64_t)ny * nz;
-    const int64_t i = x * sx + y * sy + z;
-    uint32_t m = 0;
-    const uint32_t f = ft[i];
-    if (occ[i] && f != VC_NONE)
+    const int z0 = (int)(T % nzt) * kMeasZ;
+    const int y0 = (int)((T / nzt) % nyt) * kMeasY;
+    const int xa = (int)(T / ((int64_t)nzt * nyt)) * kMeasXC;
+    const int xb = min(nx, xa + kMeasXC);
+    const int tz = threadIdx.x, ty = threadIdx.y;
+    const int tid = ty * kMeasZ + tz;
+    const int z = z0 + tz, y = y0 + ty;
+    const bool valid = z < nz && y < ny;
+    const int64_t sx = (int64_t)ny * nz;
+    const int64_t row = (int64_t)y * nz + z;
+    // halo slots: rows y0-1 and y0+8 (66 each), then columns z0-1 and z0+64
+    int hy = 0, hz = 0;
+    bool hact = false;
+    if (tid < 2 * (kMeasZ + 2))
     {
-        int fx, fy, fz;
-        vc_unpack_xyz(pk, f, fx, fy, fz);
-        auto visit = [&](bool in, int64_t j) {
-            if (!in || !occ[j])
-                return;
-            uint32_t g = ft[j];
-            if (g == VC_NONE)
-                return;
-            int gx, gy, gz;
-            vc_unpack_xyz(pk, g, gx, gy, gz);
-            uint32_t d = (uint32_t)((fx - gx) * (fx - gx)) +
-                         (uint32_t)((fy - gy) * (fy - gy)) +
-                         (uint32_t)((fz - gz) * (fz - gz));
-            m = d > m ? d : m;
-        };
-        visit(x > 0, i - sx);
-        visit(x + 1 < nx, i + sx);
-        visit(y > 0, i - sy);
-        visit(y + 1 < ny, i + sy);
-        visit(z > 0, i - 1);
-        visit(z + 1 < nz, i + 1);
+        hy = tid < kMeasZ + 2 ? -1 : kMeasY;
+        hz = (tid % (kMeasZ + 2)) - 1;
+        hact = true;
     }
-    lam2[i] = m;
+    else if (tid < 2 * (kMeasZ + 2) + 2 * kMeasY)
+    {
+        const int u = tid - 2 * (kMeasZ + 2);
+        hy = u >> 1;
+        hz = (u & 1) ? kMeasZ : -1;
+        hact = true;
+    }
+    const int gy = y0 + hy, gz = z0 + hz;
+    const bool hin = hact && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+    auto load = [&](int xx) -> uint32_t {
+        return (valid && xx >= 0 && xx < nx) ? ft[xx * sx + row] : kOob;
+    };
+    uint32_t prev = load(xa - 1), cur = load(xa);
+    for (int x = xa; x < xb; ++x)
+    {
+        const uint32_t next = load(x + 1);
+        const uint32_t halo =
+            hin ? ft[x * sx + (int64_t)gy * nz + gz] : kOob;
+        S[ty + 1][tz + 1] = valid ? cur : kOob;
+        if (hact)
+            S[hy + 1][hz + 1] = halo;
+        __syncthreads();
+        if (valid)
+        {
+            uint32_t m = 0;
+            const uint32_t me = vc_pack_xyz(pk, x, y, z);
+            if (cur != VC_NONE && cur != me)
+            {
+                int fx, fy, fz;
+                vc_unpack_xyz(pk, cur, fx, fy, fz);
+                auto visit = [&](uint32_t g, uint32_t q) {
+                    if (g == kOob || g == VC_NONE || g == q)
+                        return;
+                    int gx, gy2, gz2;
+                    vc_unpack_xyz(pk, g, gx, gy2, gz2);
+                    const uint32_t d = (uint32_t)((fx - gx) * (fx - gx)) +
+                                       (uint32_t)((fy - gy2) * (fy - gy2)) +
+                                       (uint32_t)((fz - gz2) * (fz - gz2));
+                    m = d > m ? d : m;
+                };
+                visit(prev, vc_pack_xyz(pk, x - 1, y, z));
+                visit(next, vc_pack_xyz(pk, x + 1, y, z));
+                visit(S[ty][tz + 1], vc_pack_xyz(pk, x, y - 1, z));
+                visit(S[ty + 2][tz + 1], vc_pack_xyz(pk, x, y + 1, z));
+                visit(S[ty + 1][tz], vc_pack_xyz(pk, x, y, z - 1));
+                visit(S[ty + 1][tz + 2], vc_pack_xyz(pk, x, y, z + 1));
+            }
+            lam2[x * sx + row] = m;
+        }
+        __syncthreads();
+        prev = cur;
+        cur = next;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -326,7 +399,8 @@ int vc_edt_run(vc_ctx* ctx, vc_vol* v)
         return rc;
     if (!v->ft && (rc = vc_alloc(ctx, (void**)&v->ft, 4 * N)) != VC_OK)
         return rc;
-    if ((size_t)v->nx * 4 > kLdsBudget || (size_t)v->ny * 2 > kLdsBudget)
+    // one full column per workgroup at least (TZ = 1 may use up to 64 KB)
+    if ((size_t)v->nx * 4 > 65536 || (size_t)v->ny * 2 > 65536)
         return vc_fail(ctx, VC_ERR_INVALID, "axis too long for the LDS tiles");
     // No temporaries: pass z writes int16 z' into the first half of the edt
     // buffer, pass y writes packed (y', z') into the ft buffer, and pass x
@@ -340,14 +414,16 @@ int vc_edt_run(vc_ctx* ctx, vc_vol* v)
     VC_CHECK_LAUNCH(ctx);
     {
         int tz = pick_tile(v->ny, 2);
-        dim3 g((unsigned)((v->nz + tz - 1) / tz), (unsigned)v->nx);
+        const int64_t tiles = (int64_t)v->nx * ((v->nz + tz - 1) / tz);
+        const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
         edt_pass_y<<<g, kBlock, (size_t)tz * v->ny * 2, ctx->stream>>>(
-            zq, yz, v->ny, v->nz, tz);
+            zq, yz, v->nx, v->ny, v->nz, tz);
         VC_CHECK_LAUNCH(ctx);
     }
     {
         int tz = pick_tile(v->nx, 4);
-        dim3 g((unsigned)((v->nz + tz - 1) / tz), (unsigned)v->ny);
+        const int64_t tiles = (int64_t)v->ny * ((v->nz + tz - 1) / tz);
+        const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
         edt_pass_x<<<g, kBlock, (size_t)tz * v->nx * 4, ctx->stream>>>(
             yz, v->ft, v->edt, v->nx, v->ny, v->nz, tz, v->pk);
         VC_CHECK_LAUNCH(ctx);
@@ -363,10 +439,12 @@ int vc_measure_run(vc_ctx* ctx, vc_vol* v)
     if (!v->lam2 &&
         (rc = vc_alloc(ctx, (void**)&v->lam2, 4 * (size_t)v->N)) != VC_OK)
         return rc;
-    dim3 g((unsigned)((v->nz + kBlock - 1) / kBlock), (unsigned)v->ny,
-           (unsigned)v->nx);
-    measure_lam2<<<g, kBlock, 0, ctx->stream>>>(v->occ, v->ft, v->lam2, v->nx,
-                                                 v->ny, v->nz, v->pk);
+    const int64_t tiles = (int64_t)((v->nz + kMeasZ - 1) / kMeasZ) *
+                          ((v->ny + kMeasY - 1) / kMeasY) *
+                          ((v->nx + kMeasXC - 1) / kMeasXC);
+    const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
+    measure_lam2<<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+        v->ft, v->lam2, v->nx, v->ny, v->nz, v->pk);
     VC_CHECK_LAUNCH(ctx);
     v->measure_kind = 1;
     return VC_OK;
