@@ -125,8 +125,26 @@ __global__ void __launch_bounds__(kBlock)
     const int z = z0 + tz;
     const bool zin = z < nz;
     const int64_t base = (int64_t)x * ny * nz;
-    for (int y = ty0; y < ny; y += tys)
-        colz[y * TZ + tz] = zin ? zq[base + (int64_t)y * nz + z] : (int16_t)-1;
+    // 16 independent loads in flight per thread before the LDS stores
+    constexpr int U = 16;
+    for (int y = ty0; y < ny; y += tys * U)
+    {
+        int16_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int yy = y + u * tys;
+            v[u] = (zin && yy < ny) ? zq[base + (int64_t)yy * nz + z]
+                                    : (int16_t)-1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int yy = y + u * tys;
+            if (yy < ny)
+                colz[yy * TZ + tz] = v[u];
+        }
+    }
     __syncthreads();
     if (!zin)
         return;
@@ -204,8 +222,24 @@ __global__ void __launch_bounds__(kBlock)
     const bool zin = z < nz;
     const int64_t sx = (int64_t)ny * nz;
     const int64_t off = (int64_t)y * nz + z;
-    for (int x = tx0; x < nx; x += txs)
-        colyz[x * TZ + tz] = zin ? yz[(int64_t)x * sx + off] : VC_NONE;
+    constexpr int U = 16;
+    for (int x = tx0; x < nx; x += txs * U)
+    {
+        uint32_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int xx = x + u * txs;
+            v[u] = (zin && xx < nx) ? yz[(int64_t)xx * sx + off] : VC_NONE;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int xx = x + u * txs;
+            if (xx < nx)
+                colyz[xx * TZ + tz] = v[u];
+        }
+    }
     __syncthreads();
     if (!zin)
         return;

@@ -128,56 +128,6 @@ struct LevelsF32
     float L[VC_MAX_LEVELS];
 };
 
-__global__ void __launch_bounds__(kBlock)
-    thin_init_exact(const uint8_t* __restrict__ occ,
-                    const uint32_t* __restrict__ lam2, uint8_t* __restrict__ lvl,
-                    uint8_t* __restrict__ label, int64_t N, LevelsU64 lv,
-                    int nlev)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
-    const bool in = occ[i] != 0;
-    uint8_t l = 255;
-    if (in)
-    {
-        const uint64_t m = lam2[i];
-        for (int k = 0; k < nlev; ++k)
-            if (m < lv.T[k])
-            {
-                l = (uint8_t)k;
-                break;
-            }
-    }
-    lvl[i] = l;
-    label[i] = in ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kBlock)
-    thin_init_f32(const uint8_t* __restrict__ occ,
-                  const float* __restrict__ meas, uint8_t* __restrict__ lvl,
-                  uint8_t* __restrict__ label, int64_t N, LevelsF32 lv,
-                  int nlev)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
-    const bool in = occ[i] != 0;
-    uint8_t l = 255;
-    if (in)
-    {
-        const float m = meas[i];
-        for (int k = 0; k < nlev; ++k)
-            if (m < lv.L[k])
-            {
-                l = (uint8_t)k;
-                break;
-            }
-    }
-    lvl[i] = l;
-    label[i] = in ? 1 : 0;
-}
-
 // ------------------------------------------------------------ bit planes
 // Alive and eligible voxels are bit planes packed along z: word (x, y, w)
 // holds z in [64w, 64w+64), bit z & 63; index (x*ny + y)*nwz + w.
@@ -191,34 +141,104 @@ __device__ __forceinline__ int64_t widx(const Plane& P, int x, int y, int w)
     return ((int64_t)x * P.ny + y) * P.nwz + w;
 }
 
-// one wavefront packs one word: lane l <-> z = 64w + l.  Words holding an
-// alive voxel that becomes eligible at level k are flagged for both z
-// parities (every other eligible voxel was found non-simple when the previous
-// level converged and its neighbourhood has not changed since).
+// Level set-up.  A wavefront covers 512 consecutive z of one row (8 words);
+// lane l handles z = z0 + 8l .. z0 + 8l + 7 and lanes 8j .. 8j+7 assemble
+// word j.  Words holding an alive voxel that becomes eligible at level k are
+// flagged for both z parities (every other eligible voxel was found
+// non-simple when the previous level converged and its neighbourhood has not
+// changed since).
+template <int kMode> // 0: exact measure, 1: float measure, 2: later level
 __global__ void __launch_bounds__(kBlock)
-    pack_planes(const uint8_t* __restrict__ label, const uint8_t* __restrict__ lvl,
-                uint64_t* __restrict__ A, uint64_t* __restrict__ E,
-                uint8_t* __restrict__ D0, uint8_t* __restrict__ D1, Plane P,
-                int k)
+    thin_level_init(const uint8_t* __restrict__ occ,
+                    const uint32_t* __restrict__ lam2,
+                    const float* __restrict__ meas, uint8_t* __restrict__ lvl,
+                    uint8_t* __restrict__ label, uint64_t* __restrict__ A,
+                    uint64_t* __restrict__ E, uint8_t* __restrict__ D0,
+                    uint8_t* __restrict__ D1, Plane P, int nchunk,
+                    LevelsU64 lt, LevelsF32 lf, int nlev, int k)
 {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int x = blockIdx.z;
-    const bool in = z < P.nz;
-    const int64_t i = ((int64_t)x * P.ny + y) * P.nz + z;
-    const uint8_t l = in ? lvl[i] : (uint8_t)255;
-    const uint64_t e = __ballot(l <= k);
-    const uint64_t enew = __ballot(l == k);
-    const uint64_t a = __ballot(in && label[i] == 1);
-    // lane 0 sits at z = 64w: the word exists iff 64w < nz
-    if ((threadIdx.x & 63) == 0 && z < P.nz)
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t rowi = wave / nchunk;
+    if (rowi >= (int64_t)P.nx * P.ny)
+        return;
+    const int z0 = (int)(wave % nchunk) * 512 + 8 * lane;
+    const int64_t base = rowi * P.nz;
+    uint32_t am = 0, em = 0, nm = 0;
+    uint8_t lv8[8], lb8[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
     {
-        const int64_t o = widx(P, x, y, z >> 6);
+        const int z = z0 + j;
+        uint8_t l = 255, lb = 0;
+        if (z < P.nz)
+        {
+            const int64_t i = base + z;
+            if (kMode == 2)
+            {
+                l = lvl[i];
+                lb = label[i] == 1;
+            }
+            else if (occ[i])
+            {
+                lb = 1;
+                if (kMode == 0)
+                {
+                    const uint64_t m = lam2[i];
+                    for (int q = 0; q < nlev; ++q)
+                        if (m < lt.T[q])
+                        {
+                            l = (uint8_t)q;
+                            break;
+                        }
+                }
+                else
+                {
+                    const float m = meas[i];
+                    for (int q = 0; q < nlev; ++q)
+                        if (m < lf.L[q])
+                        {
+                            l = (uint8_t)q;
+                            break;
+                        }
+                }
+            }
+        }
+        lv8[j] = l;
+        lb8[j] = lb;
+        am |= (uint32_t)lb << j;
+        em |= (uint32_t)(l <= k) << j;
+        nm |= (uint32_t)(l == k) << j;
+    }
+    if (kMode != 2)
+    {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (z0 + j < P.nz)
+            {
+                lvl[base + z0 + j] = lv8[j];
+                label[base + z0 + j] = lb8[j];
+            }
+    }
+    const int sh = 8 * (lane & 7);
+    uint64_t a = (uint64_t)am << sh, e = (uint64_t)em << sh,
+             nw = (uint64_t)nm << sh;
+#pragma unroll
+    for (int d = 1; d <= 4; d <<= 1)
+    {
+        a |= __shfl_xor(a, d);
+        e |= __shfl_xor(e, d);
+        nw |= __shfl_xor(nw, d);
+    }
+    if ((lane & 7) == 0 && z0 < P.nz)
+    {
+        const int64_t o = rowi * P.nwz + (z0 >> 6);
         E[o] = e;
-        A[o] = a;
-        const uint8_t f = (enew & a) ? 1 : 0;
+        const uint8_t f = (nw & a) ? 1 : 0;
         D0[o] = f;
         D1[o] = f;
+        if (kMode != 2)
+            A[o] = a;
     }
 }
 
@@ -497,36 +517,31 @@ int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
     hipStream_t S = ctx->stream;
     hipError_t e = hipMemsetAsync(v->tstats, 0, ctr_bytes, S);
     int64_t launches = 0;
-    const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
-    if (e == hipSuccess)
+    LevelsU64 lt{};
+    LevelsF32 lf{};
+    for (int k = 0; k < n; ++k)
     {
-        if (v->measure_kind == 2)
-        {
-            LevelsF32 lv{};
-            for (int k = 0; k < n; ++k)
-                lv.L[k] = levels[k];
-            thin_init_f32<<<nb, kBlock, 0, S>>>(v->occ, v->meas_f, v->lvl,
-                                                 v->label, v->N, lv, n);
-        }
-        else
-        {
-            LevelsU64 lv{};
-            for (int k = 0; k < n; ++k)
-                lv.T[k] = (uint64_t)vc_level_threshold(levels[k]);
-            thin_init_exact<<<nb, kBlock, 0, S>>>(v->occ, v->lam2, v->lvl,
-                                                   v->label, v->N, lv, n);
-        }
-        launches++;
-        e = hipGetLastError();
+        lt.T[k] = (uint64_t)vc_level_threshold(levels[k]);
+        lf.L[k] = levels[k];
     }
-    const dim3 gpack((unsigned)((v->nz + kBlock - 1) / kBlock), (unsigned)v->ny,
-                     (unsigned)v->nx);
+    const int nchunk = (v->nz + 511) / 512;
+    const int64_t init_threads = (int64_t)v->nx * v->ny * nchunk * 64;
+    const unsigned ginit = (unsigned)((init_threads + kBlock - 1) / kBlock);
     int64_t passes_total = 0;
     for (int k = 0; k < n && e == hipSuccess; ++k)
     {
-        pack_planes<<<gpack, kBlock, 0, S>>>(v->label, v->lvl, v->alive,
-                                              v->elig, v->dflag, v->dflag + nwords,
-                                              P, k);
+        if (k > 0)
+            thin_level_init<2><<<ginit, kBlock, 0, S>>>(
+                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
+                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
+        else if (v->measure_kind == 2)
+            thin_level_init<1><<<ginit, kBlock, 0, S>>>(
+                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
+                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
+        else
+            thin_level_init<0><<<ginit, kBlock, 0, S>>>(
+                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
+                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
         launches++;
         e = hipGetLastError();
         if (e == hipSuccess)
