@@ -1,9 +1,876 @@
-// vc_boundary.hip -- boundary sites, quads and cube-complex Euler (placeholder)
+// vc_boundary.hip -- boundary sites, boundary quads and the cube-complex
+// Euler characteristic with the reference Surfacer's exact output order
+// (src/surfacing.cpp:49-197, :323-440, :597-660), gfx950.
+//
+// Order.  Surfacer visits voxels x-outer / z-inner (= our linear index), the
+// six face offsets -x,+x,-y,+y,-z,+z of each voxel, and for a face between
+// voxels of different occupancy first the four corners m_cornersWRTNbOffset[o]
+// in order, then the face; a corner or face is emitted at its first visit.
+// A corner is touched by the voxels of its 2x2x2 block, and a block voxel
+// touches it iff it differs from one of its three block neighbours, so the
+// corner's first visitor is the first (in linear order) in-grid block voxel
+// that touches it; the visit happens at that voxel's first touching offset.
+// Every voxel therefore decides from its 3x3x3 neighbourhood which corners
+// and faces it emits, and in which order.  Counts per 256-voxel tile, an
+// exclusive scan over tiles and a per-tile scan give every output its index.
+//
+// Quads refer to sites through a corner-lattice map (written when the sites
+// are emitted, read when the quads are).  The Euler terms follow
+// computeEulerChar literally, including the (int)-truncation of the corner
+// base voxel at coordinate -0.5.  C is the number of 6-components of the
+// "thin inside layer" (inside voxels within Chebyshev distance 1 of an inside
+// voxel with a non-inside face neighbour), by union-find.
+#include <cstring>
+#include <vector>
+
 #include "vc_internal.h"
 
-int vc_boundary_run(vc_ctx* ctx, vc_vol* vol, float* sites, int64_t site_cap,
+namespace
+{
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------- reference tables
+// face offsets o: -x, +x, -y, +y, -z, +z (include/surfacing.h:172-178)
+__constant__ int8_t kOff[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
+                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+// corners shared with the neighbour across offset o (surfacing.h:185-190)
+__constant__ int8_t kCwrt[6][4] = {{2, 3, 7, 6}, {0, 1, 5, 4}, {4, 6, 2, 0},
+                                   {1, 3, 7, 5}, {0, 2, 3, 1}, {5, 7, 6, 4}};
+// face index of offset o (surfacing.h:191-197)
+__constant__ int8_t kFwrt[6] = {3, 2, 4, 5, 0, 1};
+// corner slots of face fi (surfacing.h:144-149)
+__constant__ int8_t kFcrn[6][4] = {{0, 1, 3, 2}, {4, 5, 7, 6}, {0, 4, 5, 1},
+                                   {2, 6, 7, 3}, {4, 0, 2, 6}, {5, 1, 3, 7}};
+// corner c of voxel (i,j,k) is at (i,j,k) + 0.5 * kSgn[c] (surfacing.h:97-119)
+__constant__ int8_t kSgn[8][3] = {{1, -1, -1}, {1, 1, -1},  {-1, -1, -1},
+                                  {-1, 1, -1}, {1, -1, 1},  {1, 1, 1},
+                                  {-1, -1, 1}, {-1, 1, 1}};
+
+// neighbourhood bit of offset (dx, dy, dz): dz+1 + 3(dy+1) + 9(dx+1)
+__device__ __forceinline__ int nbit(int dx, int dy, int dz)
+{
+    return (dz + 1) + 3 * (dy + 1) + 9 * (dx + 1);
+}
+
+struct Vox
+{
+    uint32_t occ;  // 27 bits, out of grid = 0
+    uint32_t ing;  // 27 bits, in grid
+};
+
+__device__ __forceinline__ bool obit(uint32_t m, int dx, int dy, int dz)
+{
+    return (m >> nbit(dx, dy, dz)) & 1u;
+}
+
+// Decisions of one voxel: new corners (slot order by first visit) and new
+// faces (by offset).
+struct Emit
+{
+    int nsite;
+    int8_t corner[8]; // corner slots in emission order
+    int nface;
+    int8_t face_o[6]; // offsets in emission order
+};
+
+__device__ Emit decide(const Vox& V)
+{
+    Emit r;
+    r.nsite = 0;
+    r.nface = 0;
+    const bool me = obit(V.occ, 0, 0, 0);
+    // events of this voxel
+    uint32_t ev = 0;
+#pragma unroll
+    for (int o = 0; o < 6; ++o)
+        if (obit(V.occ, kOff[o][0], kOff[o][1], kOff[o][2]) != me)
+            ev |= 1u << o;
+    if (!ev)
+        return r;
+    // corners: key = 4*o + ii of the first touching offset, if first visitor
+    int key[8];
+    int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const int sx = kSgn[c][0], sy = kSgn[c][1], sz = kSgn[c][2];
+        const int ox = sx > 0 ? 1 : 0, oy = sy > 0 ? 3 : 2, oz = sz > 0 ? 5 : 4;
+        int o = -1;
+        if (ev & (1u << ox))
+            o = ox;
+        else if (ev & (1u << oy))
+            o = oy;
+        else if (ev & (1u << oz))
+            o = oz;
+        if (o < 0)
+            continue;
+        // an earlier (linear order = lexicographic block order) in-grid
+        // block voxel touching the corner visits it first
+        bool earlier = false;
+        const int lx = sx > 0 ? 0 : -1, ly = sy > 0 ? 0 : -1, lz = sz > 0 ? 0 : -1;
+        for (int q = 0; q < 8; ++q)
+        {
+            const int a = q >> 2, b = (q >> 1) & 1, e = q & 1;
+            const int ux = lx + a, uy = ly + b, uz = lz + e;
+            if (ux == 0 && uy == 0 && uz == 0)
+                break; // reached this voxel
+            if (!obit(V.ing, ux, uy, uz))
+                continue;
+            const bool u = obit(V.occ, ux, uy, uz);
+            const int vx = lx + (1 - a), vy = ly + (1 - b), vz = lz + (1 - e);
+            if (obit(V.occ, vx, uy, uz) != u || obit(V.occ, ux, vy, uz) != u ||
+                obit(V.occ, ux, uy, vz) != u)
+            {
+                earlier = true;
+                break;
+            }
+        }
+        if (earlier)
+            continue;
+        int ii = 0;
+        while (kCwrt[o][ii] != c)
+            ++ii;
+        key[cnt] = 4 * o + ii;
+        r.corner[cnt] = (int8_t)c;
+        ++cnt;
+    }
+    // insertion sort by key (at most 8)
+    for (int i = 1; i < cnt; ++i)
+    {
+        const int kk = key[i];
+        const int8_t cc = r.corner[i];
+        int j = i - 1;
+        while (j >= 0 && key[j] > kk)
+        {
+            key[j + 1] = key[j];
+            r.corner[j + 1] = r.corner[j];
+            --j;
+        }
+        key[j + 1] = kk;
+        r.corner[j + 1] = cc;
+    }
+    r.nsite = cnt;
+    // faces created here: +offsets, and -offsets whose neighbour is outside
+#pragma unroll
+    for (int o = 0; o < 6; ++o)
+        if ((ev >> o) & 1u)
+        {
+            const bool nb_in = obit(V.ing, kOff[o][0], kOff[o][1], kOff[o][2]);
+            if ((o & 1) || !nb_in)
+                r.face_o[r.nface++] = (int8_t)o;
+        }
+    return r;
+}
+
+struct Dims
+{
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ Vox gather(const uint8_t* __restrict__ occ,
+                                      const Dims& D, int x, int y, int z)
+{
+    Vox V{0u, 0u};
+    const int64_t sy = D.nz, sx = (int64_t)D.ny * D.nz;
+    const int64_t i = x * sx + y * sy + z;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz)
+            {
+                const bool in = (unsigned)(x + dx) < (unsigned)D.nx &&
+                                (unsigned)(y + dy) < (unsigned)D.ny &&
+                                (unsigned)(z + dz) < (unsigned)D.nz;
+                if (in)
+                {
+                    V.ing |= 1u << nbit(dx, dy, dz);
+                    if (occ[i + dx * sx + dy * sy + dz])
+                        V.occ |= 1u << nbit(dx, dy, dz);
+                }
+            }
+    return V;
+}
+
+__device__ __forceinline__ void decode(const Dims& D, int64_t i, int& x,
+                                       int& y, int& z)
+{
+    // N < 2^31: 32-bit division
+    const uint32_t u = (uint32_t)i;
+    z = (int)(u % (uint32_t)D.nz);
+    const uint32_t r = u / (uint32_t)D.nz;
+    y = (int)(r % (uint32_t)D.ny);
+    x = (int)(r / (uint32_t)D.ny);
+}
+
+// corner lattice id of corner slot c of voxel (x, y, z)
+__device__ __forceinline__ int64_t corner_id(const Dims& D, int x, int y,
+                                             int z, int c)
+{
+    const int cx = x + (kSgn[c][0] > 0), cy = y + (kSgn[c][1] > 0),
+              cz = z + (kSgn[c][2] > 0);
+    return ((int64_t)cx * (D.ny + 1) + cy) * (D.nz + 1) + cz;
+}
+
+// block-wide exclusive scan of two counters packed as (a << 16) | b
+__device__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d)
+            incl += t;
+    }
+    if (lane == 63)
+        sh[wv] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+    {
+        if (w < wv)
+            off += sh[w];
+        tot += sh[w];
+    }
+    *total = tot;
+    return off + incl - v;
+}
+
+// pass 1: per-tile totals of (sites, faces)
+__global__ void __launch_bounds__(kBlock)
+    bnd_count(const uint8_t* __restrict__ occ, Dims D, int64_t N,
+              uint32_t* __restrict__ tile_sites, uint32_t* __restrict__ tile_faces)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ns = 0, nf = 0;
+    if (i < N)
+    {
+        int x, y, z;
+        decode(D, i, x, y, z);
+        const Emit e = decide(gather(occ, D, x, y, z));
+        ns = (uint32_t)e.nsite;
+        nf = (uint32_t)e.nface;
+    }
+    uint32_t tot;
+    block_scan_excl((ns << 16) | nf, sh, &tot);
+    if (threadIdx.x == 0)
+    {
+        tile_sites[blockIdx.x] = tot >> 16;
+        tile_faces[blockIdx.x] = tot & 0xFFFFu;
+    }
+}
+
+// exclusive scan of a uint32 array in place, one block of 1024 per chunk;
+// writes the chunk totals to `sums`
+__global__ void __launch_bounds__(1024)
+    scan_chunks(uint32_t* a, int64_t n, uint32_t* sums)
+{
+    __shared__ uint32_t sh[16];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const uint32_t v = i < n ? a[i] : 0u;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d)
+            incl += t;
+    }
+    if (lane == 63)
+        sh[wv] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < 16; ++w)
+    {
+        if (w < wv)
+            off += sh[w];
+        tot += sh[w];
+    }
+    if (i < n)
+        a[i] = off + incl - v;
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024)
+    scan_add(uint32_t* a, int64_t n, const uint32_t* sums)
+{
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    if (i < n)
+        a[i] += sums[blockIdx.x];
+}
+
+// pass 3: sites (coordinates + corner map)
+__global__ void __launch_bounds__(kBlock)
+    bnd_emit_sites(const uint8_t* __restrict__ occ, Dims D, int64_t N,
+                   const uint32_t* __restrict__ tile_sites,
+                   float* __restrict__ sites, uint32_t* __restrict__ cmap)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Emit e;
+    e.nsite = 0;
+    int x = 0, y = 0, z = 0;
+    if (i < N)
+    {
+        decode(D, i, x, y, z);
+        e = decide(gather(occ, D, x, y, z));
+    }
+    uint32_t tot;
+    const uint32_t base =
+        tile_sites[blockIdx.x] + block_scan_excl((uint32_t)e.nsite, sh, &tot);
+    for (int j = 0; j < e.nsite; ++j)
+    {
+        const int c = e.corner[j];
+        const uint32_t s = base + j;
+        sites[3 * (int64_t)s + 0] = (float)x + 0.5f * kSgn[c][0];
+        sites[3 * (int64_t)s + 1] = (float)y + 0.5f * kSgn[c][1];
+        sites[3 * (int64_t)s + 2] = (float)z + 0.5f * kSgn[c][2];
+        cmap[corner_id(D, x, y, z, c)] = s;
+    }
+}
+
+// pass 4: quads through the corner map
+__global__ void __launch_bounds__(kBlock)
+    bnd_emit_quads(const uint8_t* __restrict__ occ, Dims D, int64_t N,
+                   const uint32_t* __restrict__ tile_faces,
+                   const uint32_t* __restrict__ cmap, int32_t* __restrict__ quads)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Emit e;
+    e.nface = 0;
+    int x = 0, y = 0, z = 0;
+    if (i < N)
+    {
+        decode(D, i, x, y, z);
+        e = decide(gather(occ, D, x, y, z));
+    }
+    uint32_t tot;
+    const uint32_t base =
+        tile_faces[blockIdx.x] + block_scan_excl((uint32_t)e.nface, sh, &tot);
+    for (int j = 0; j < e.nface; ++j)
+    {
+        const int fi = kFwrt[e.face_o[j]];
+        int32_t* q = quads + 4 * (int64_t)(base + j);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            q[t] = (int32_t)cmap[corner_id(D, x, y, z, kFcrn[fi][t])];
+    }
+}
+
+// ------------------------------------------------------------- Euler terms
+// accumulators: [0] V_bnd, [1] E_bnd, [2] Vint_sum, [3] E_int, [4] F_int,
+// [5] T, spread over 64 slots
+constexpr int kSlots = 64;
+constexpr int kAcc = 8;
+
+__device__ __forceinline__ void block_add(unsigned long long v,
+                                          unsigned long long* sh,
+                                          unsigned long long* dst)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        v += __shfl_xor(v, d);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        sh[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            t += sh[w];
+        if (t)
+            atomicAdd(dst, t);
+    }
+}
+
+__device__ __forceinline__ int occ_at(const uint8_t* occ, const Dims& D, int x,
+                                      int y, int z)
+{
+    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny ||
+        (unsigned)z >= (unsigned)D.nz)
+        return 0;
+    return occ[((int64_t)x * D.ny + y) * D.nz + z] != 0;
+}
+
+// per corner-lattice point (cx, cy, cz): the boundary-vertex multiplicity if
+// it is a site, and the boundary / interior edges starting at it (+x, +y, +z)
+__global__ void __launch_bounds__(kBlock)
+    euler_lattice(const uint8_t* __restrict__ occ, Dims D,
+                  const int32_t* __restrict__ tbl, unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[kBlock / 64];
+    const int cz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cy = blockIdx.y, cx = blockIdx.z;
+    unsigned long long vb = 0, eb = 0, ei = 0;
+    if (cz <= D.nz)
+    {
+        int blk = 0; // occupancy of the 8 voxels around the corner
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    blk += occ_at(occ, D, cx - 1 + a, cy - 1 + b, cz - 1 + c);
+        if (blk != 0 && blk != 8)
+        {
+            // site: mask around the (int)-truncated base voxel
+            // (include/spaceinfo.h:140-153), bit (7 - i) for offset i
+            const int bx = cx > 0 ? cx - 1 : 0, by = cy > 0 ? cy - 1 : 0,
+                      bz = cz > 0 ? cz - 1 : 0;
+            int mask = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                mask |= occ_at(occ, D, bx + ((i >> 2) & 1), by + ((i >> 1) & 1),
+                               bz + (i & 1))
+                        << (7 - i);
+            vb = (unsigned long long)tbl[mask];
+        }
+        // edges (+x, +y, +z) from this lattice point: the 4 voxels around
+        // the edge, in cyclic order; boundary faces = value changes
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+        {
+            const int lim = ax == 0 ? D.nx : (ax == 1 ? D.ny : D.nz);
+            const int cc = ax == 0 ? cx : (ax == 1 ? cy : cz);
+            if (cc >= lim)
+                continue;
+            int v4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const int u = (q == 1 || q == 2) ? 0 : -1; // cyclic square
+                const int w = (q >= 2) ? 0 : -1;
+                int p[3] = {cx, cy, cz};
+                p[(ax + 1) % 3] += u;
+                p[(ax + 2) % 3] += w;
+                v4[q] = occ_at(occ, D, p[0], p[1], p[2]);
+            }
+            const int ch = (v4[0] != v4[1]) + (v4[1] != v4[2]) +
+                           (v4[2] != v4[3]) + (v4[3] != v4[0]);
+            if (ch == 2)
+                eb += 1;
+            else if (ch == 4)
+                eb += 2;
+            if (v4[0] + v4[1] + v4[2] + v4[3] == 4)
+                ei += 1;
+        }
+    }
+    const int slot = (blockIdx.x + blockIdx.y * 3 + blockIdx.z * 7) % kSlots;
+    unsigned long long* a = acc + slot * kAcc;
+    block_add(vb, sh, &a[0]);
+    block_add(eb, sh, &a[1]);
+    block_add(ei, sh, &a[3]);
+}
+
+// per voxel: T, F_int and the reference's interior-corner sum
+__global__ void __launch_bounds__(kBlock)
+    euler_voxels(const uint8_t* __restrict__ occ, Dims D,
+                 unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[kBlock / 64];
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    unsigned long long t = 0, fi = 0, vi = 0;
+    if (z < D.nz && occ_at(occ, D, x, y, z))
+    {
+        t = 1;
+        fi = occ_at(occ, D, x - 1, y, z) + occ_at(occ, D, x + 1, y, z) +
+             occ_at(occ, D, x, y - 1, z) + occ_at(occ, D, x, y + 1, z) +
+             occ_at(occ, D, x, y, z - 1) + occ_at(occ, D, x, y, z + 1);
+        // vox_corner_offset order; base voxel = (int)(coord +- 0.5)
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            const int bx = ((c >> 2) & 1) ? x : (x > 0 ? x - 1 : 0);
+            const int by = ((c >> 1) & 1) ? y : (y > 0 ? y - 1 : 0);
+            const int bz = (c & 1) ? z : (z > 0 ? z - 1 : 0);
+            int n = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                n += occ_at(occ, D, bx + ((i >> 2) & 1), by + ((i >> 1) & 1),
+                            bz + (i & 1));
+            vi += n == 8;
+        }
+    }
+    const int slot = (blockIdx.x + blockIdx.y * 3 + blockIdx.z * 7) % kSlots;
+    unsigned long long* a = acc + slot * kAcc;
+    block_add(vi, sh, &a[2]);
+    block_add(fi, sh, &a[4]);
+    block_add(t, sh, &a[5]);
+}
+
+// --------------------------------------------- thin layer + union-find (C)
+__global__ void __launch_bounds__(kBlock)
+    layer_boundary(const uint8_t* __restrict__ occ, Dims D,
+                   uint8_t* __restrict__ bnd)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    if (z >= D.nz)
+        return;
+    uint8_t b = 0;
+    if (occ_at(occ, D, x, y, z))
+        b = !(occ_at(occ, D, x - 1, y, z) && occ_at(occ, D, x + 1, y, z) &&
+              occ_at(occ, D, x, y - 1, z) && occ_at(occ, D, x, y + 1, z) &&
+              occ_at(occ, D, x, y, z - 1) && occ_at(occ, D, x, y, z + 1));
+    bnd[((int64_t)x * D.ny + y) * D.nz + z] = b;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    layer_init(const uint8_t* __restrict__ occ, const uint8_t* __restrict__ bnd,
+               Dims D, int32_t* __restrict__ lab)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    if (z >= D.nz)
+        return;
+    const int64_t i = ((int64_t)x * D.ny + y) * D.nz + z;
+    bool in = false;
+    if (occ[i])
+        for (int dx = -1; dx <= 1 && !in; ++dx)
+            for (int dy = -1; dy <= 1 && !in; ++dy)
+                for (int dz = -1; dz <= 1 && !in; ++dz)
+                {
+                    const int a = x + dx, b = y + dy, c = z + dz;
+                    if ((unsigned)a < (unsigned)D.nx &&
+                        (unsigned)b < (unsigned)D.ny &&
+                        (unsigned)c < (unsigned)D.nz &&
+                        bnd[((int64_t)a * D.ny + b) * D.nz + c])
+                        in = true;
+                }
+    lab[i] = in ? (int32_t)i : -1;
+}
+
+__device__ __forceinline__ int32_t uf_find(int32_t* lab, int32_t i)
+{
+    int32_t p = lab[i];
+    while (p != i)
+    {
+        const int32_t g = lab[p];
+        if (g == p)
+            break;
+        // path halving (benign races: parents only ever decrease)
+        lab[i] = g;
+        i = p;
+        p = g;
+    }
+    return p;
+}
+
+__device__ void uf_union(int32_t* lab, int32_t a, int32_t b,
+                         unsigned long long* guard_hits)
+{
+    // the smaller root wins; a CAS loop makes the link atomic (lock-free: a
+    // failed CAS means another link succeeded); the bound is a safety net
+    for (int guard = 0; guard < (1 << 20); ++guard)
+    {
+        a = uf_find(lab, a);
+        b = uf_find(lab, b);
+        if (a == b)
+            return;
+        if (a > b)
+        {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&lab[b], b, a) == b)
+            return;
+    }
+    atomicAdd(guard_hits, 1ull);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    layer_union(int32_t* lab, Dims D, unsigned long long* guard_hits)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    if (z >= D.nz)
+        return;
+    const int64_t sy = D.nz, sx = (int64_t)D.ny * D.nz;
+    const int64_t i = x * sx + y * sy + z;
+    if (lab[i] < 0)
+        return;
+    if (x + 1 < D.nx && lab[i + sx] >= 0)
+        uf_union(lab, (int32_t)i, (int32_t)(i + sx), guard_hits);
+    if (y + 1 < D.ny && lab[i + sy] >= 0)
+        uf_union(lab, (int32_t)i, (int32_t)(i + sy), guard_hits);
+    if (z + 1 < D.nz && lab[i + 1] >= 0)
+        uf_union(lab, (int32_t)i, (int32_t)(i + 1), guard_hits);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    layer_count(const int32_t* __restrict__ lab, int64_t N,
+                unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long r = (i < N && lab[i] == (int32_t)i) ? 1ull : 0ull;
+    block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
+}
+
+// the reference's per-mask surface-component table, derived from first
+// principles (6-components of the inside corners + 18-components of the
+// outside corners - 1; four antipodal hole masks and 0/255 give 0)
+void cube_table(int32_t* tbl)
+{
+    auto comps = [](int set, bool adj18) {
+        int seen = 0, n = 0;
+        for (int s = 0; s < 8; ++s)
+        {
+            if (!((set >> s) & 1) || ((seen >> s) & 1))
+                continue;
+            ++n;
+            int stack[8], sp = 0;
+            stack[sp++] = s;
+            seen |= 1 << s;
+            while (sp)
+            {
+                const int u = stack[--sp];
+                for (int v = 0; v < 8; ++v)
+                {
+                    if (!((set >> v) & 1) || ((seen >> v) & 1))
+                        continue;
+                    const int pc = __builtin_popcount(u ^ v);
+                    if (pc == 1 || (adj18 && pc == 2))
+                    {
+                        seen |= 1 << v;
+                        stack[sp++] = v;
+                    }
+                }
+            }
+        }
+        return n;
+    };
+    for (int mask = 0; mask < 256; ++mask)
+    {
+        int in = 0, out = 0;
+        for (int i = 0; i < 8; ++i)
+            if ((mask >> (7 - i)) & 1)
+                in |= 1 << i;
+            else
+                out |= 1 << i;
+        tbl[mask] = (mask == 0 || mask == 255)
+                        ? 0
+                        : comps(in, false) + comps(out, true) - 1;
+    }
+    tbl[126] = tbl[189] = tbl[219] = tbl[231] = 0;
+}
+
+int scan_inplace(vc_ctx* ctx, uint32_t* a, int64_t n)
+{
+    // exclusive scan with chunks of 1024, recursing on the chunk totals
+    const int64_t chunks = (n + 1023) / 1024;
+    uint32_t* sums = nullptr;
+    int rc = vc_alloc(ctx, (void**)&sums, 4 * (size_t)(chunks > 0 ? chunks : 1));
+    if (rc != VC_OK)
+        return rc;
+    scan_chunks<<<(unsigned)chunks, 1024, 0, ctx->stream>>>(a, n, sums);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && chunks > 1)
+    {
+        rc = scan_inplace(ctx, sums, chunks);
+        if (rc == VC_OK)
+        {
+            scan_add<<<(unsigned)chunks, 1024, 0, ctx->stream>>>(a, n, sums);
+            e = hipGetLastError();
+        }
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    vc_free(sums);
+    if (rc != VC_OK)
+        return rc;
+    if (e != hipSuccess || e2 != hipSuccess)
+        return vc_fail(ctx, VC_ERR_HIP, "scan: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e2));
+    return VC_OK;
+}
+
+} // namespace
+
+int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                     int64_t* n_sites, int32_t* quads, int64_t quad_cap,
                     int64_t* n_quads, vc_euler* eu)
 {
-    return vc_fail(ctx, VC_ERR_INVALID, "boundary extraction not built yet");
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    const Dims D{v->nx, v->ny, v->nz};
+    const int64_t N = v->N;
+    const int64_t ntiles = (N + kBlock - 1) / kBlock;
+    int rc = VC_OK;
+    if (!v->bnd_done)
+    {
+        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr;
+        if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)ntiles)) != VC_OK)
+            return rc;
+        if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)ntiles)) != VC_OK)
+        {
+            vc_free(ts);
+            return rc;
+        }
+        bnd_count<<<(unsigned)ntiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
+        hipError_t e = hipGetLastError();
+        uint32_t last_s = 0, last_f = 0, cnt_s = 0, cnt_f = 0;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&last_s, ts + ntiles - 1, 4,
+                               hipMemcpyDeviceToHost, S);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&last_f, tf + ntiles - 1, 4,
+                               hipMemcpyDeviceToHost, S);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(S);
+        if (e != hipSuccess)
+            rc = vc_fail(ctx, VC_ERR_HIP, "boundary count: %s",
+                         hipGetErrorString(e));
+        if (rc == VC_OK)
+            rc = scan_inplace(ctx, ts, ntiles);
+        if (rc == VC_OK)
+            rc = scan_inplace(ctx, tf, ntiles);
+        if (rc == VC_OK)
+        {
+            e = hipMemcpyAsync(&cnt_s, ts + ntiles - 1, 4,
+                               hipMemcpyDeviceToHost, S);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&cnt_f, tf + ntiles - 1, 4,
+                                   hipMemcpyDeviceToHost, S);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(S);
+            if (e != hipSuccess)
+                rc = vc_fail(ctx, VC_ERR_HIP, "boundary scan readback: %s",
+                             hipGetErrorString(e));
+        }
+        const int64_t ns = (int64_t)cnt_s + last_s, nf = (int64_t)cnt_f + last_f;
+        const size_t ncorner = (size_t)(v->nx + 1) * (v->ny + 1) * (v->nz + 1);
+        if (rc == VC_OK)
+        {
+            vc_free(v->bnd_sites);
+            vc_free(v->bnd_quads);
+            v->bnd_sites = nullptr;
+            v->bnd_quads = nullptr;
+            rc = vc_alloc(ctx, (void**)&v->bnd_sites, 12 * (size_t)ns);
+            if (rc == VC_OK)
+                rc = vc_alloc(ctx, (void**)&v->bnd_quads, 16 * (size_t)nf);
+            if (rc == VC_OK)
+                rc = vc_alloc(ctx, (void**)&cmap, 4 * ncorner);
+        }
+        if (rc == VC_OK)
+        {
+            bnd_emit_sites<<<(unsigned)ntiles, kBlock, 0, S>>>(
+                v->occ, D, N, ts, v->bnd_sites, cmap);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+            {
+                bnd_emit_quads<<<(unsigned)ntiles, kBlock, 0, S>>>(
+                    v->occ, D, N, tf, cmap, v->bnd_quads);
+                e = hipGetLastError();
+            }
+            // Euler terms
+            unsigned long long* acc = nullptr;
+            int32_t* dtbl = nullptr;
+            uint8_t* bnd = nullptr;
+            int32_t* lab = nullptr;
+            int32_t htbl[256];
+            cube_table(htbl);
+            if (e == hipSuccess)
+            {
+                rc = vc_alloc(ctx, (void**)&acc, 8 * kSlots * kAcc);
+                if (rc == VC_OK)
+                    rc = vc_alloc(ctx, (void**)&dtbl, sizeof(htbl));
+                if (rc == VC_OK)
+                    rc = vc_alloc(ctx, (void**)&bnd, (size_t)N);
+                if (rc == VC_OK)
+                    rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)N);
+            }
+            if (rc == VC_OK && e == hipSuccess)
+                e = hipMemcpyAsync(dtbl, htbl, sizeof(htbl),
+                                   hipMemcpyHostToDevice, S);
+            if (rc == VC_OK && e == hipSuccess)
+                e = hipMemsetAsync(acc, 0, 8 * kSlots * kAcc, S);
+            if (rc == VC_OK && e == hipSuccess)
+            {
+                const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
+                              (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
+                euler_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
+                const dim3 gv((unsigned)((v->nz + kBlock - 1) / kBlock),
+                              (unsigned)v->ny, (unsigned)v->nx);
+                euler_voxels<<<gv, kBlock, 0, S>>>(v->occ, D, acc);
+                layer_boundary<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
+                layer_init<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
+                layer_union<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
+                layer_count<<<(unsigned)ntiles, kBlock, 0, S>>>(lab, N, acc);
+                e = hipGetLastError();
+            }
+            unsigned long long hacc[kSlots * kAcc];
+            if (rc == VC_OK && e == hipSuccess)
+                e = hipMemcpyAsync(hacc, acc, sizeof(hacc),
+                                   hipMemcpyDeviceToHost, S);
+            if (rc == VC_OK && e == hipSuccess)
+                e = hipStreamSynchronize(S);
+            if (rc == VC_OK && e == hipSuccess)
+            {
+                long long s[kAcc] = {0};
+                for (int sl = 0; sl < kSlots; ++sl)
+                    for (int q = 0; q < kAcc; ++q)
+                        s[q] += (long long)hacc[sl * kAcc + q];
+                if (s[7] != 0)
+                    rc = vc_fail(ctx, VC_ERR_GUARD,
+                                 "boundary: union-find retry bound reached");
+                vc_euler& E = v->bnd_euler;
+                std::memset(&E, 0, sizeof(E));
+                E.V_bnd = s[0];
+                E.E_bnd = s[1];
+                E.V_int = s[2] / 8;
+                E.E_int = s[3];
+                E.F_int = s[4] / 2;
+                E.T = s[5];
+                E.C = s[6];
+                E.V = E.V_bnd + E.V_int;
+                E.E = E.E_bnd + E.E_int;
+                E.F = nf + E.F_int;
+                E.euler = (int64_t)(int32_t)(E.V - E.E + E.F - E.T);
+            }
+            vc_free(acc);
+            vc_free(dtbl);
+            vc_free(bnd);
+            vc_free(lab);
+            if (rc == VC_OK && e != hipSuccess)
+                rc = vc_fail(ctx, VC_ERR_HIP, "boundary emit/Euler: %s",
+                             hipGetErrorString(e));
+        }
+        vc_free(ts);
+        vc_free(tf);
+        vc_free(cmap);
+        if (rc != VC_OK)
+            return rc;
+        v->bnd_nsites = ns;
+        v->bnd_nquads = nf;
+        v->bnd_done = true;
+    }
+    *n_sites = v->bnd_nsites;
+    *n_quads = v->bnd_nquads;
+    if (eu)
+        *eu = v->bnd_euler;
+    if ((sites && site_cap < v->bnd_nsites) || (quads && quad_cap < v->bnd_nquads))
+        return vc_fail(ctx, VC_ERR_CAPACITY,
+                       "boundary: need %lld sites and %lld quads",
+                       (long long)v->bnd_nsites, (long long)v->bnd_nquads);
+    if (sites && v->bnd_nsites)
+        VC_HIP(ctx, hipMemcpyAsync(sites, v->bnd_sites, 12 * (size_t)v->bnd_nsites,
+                                   hipMemcpyDeviceToHost, S));
+    if (quads && v->bnd_nquads)
+        VC_HIP(ctx, hipMemcpyAsync(quads, v->bnd_quads, 16 * (size_t)v->bnd_nquads,
+                                   hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
 }

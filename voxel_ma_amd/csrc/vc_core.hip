@@ -270,7 +270,8 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     if (v->hcnt)
         (void)hipHostFree(v->hcnt);
     vc_free(v->tstats);
-    vc_free(v->scratch);
+    vc_free(v->bnd_sites);
+    vc_free(v->bnd_quads);
     delete v;
     return VC_OK;
 }

@@ -47,9 +47,12 @@ struct vc_vol
     uint32_t* sweep_any = nullptr; // per sweep of a level: removed anything
     uint32_t* hcnt = nullptr;   // pinned host staging for counters
     uint64_t* tstats = nullptr; // device stats accumulators
-    // scratch for the boundary stage
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
+    // boundary stage results (computed once per volume)
+    bool bnd_done = false;
+    float* bnd_sites = nullptr;  // 3 floats per site
+    int32_t* bnd_quads = nullptr; // 4 site indices per quad
+    int64_t bnd_nsites = 0, bnd_nquads = 0;
+    vc_euler bnd_euler{};
     bool edt_done = false;
     bool thin_done = false;
 };
