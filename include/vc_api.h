@@ -32,6 +32,8 @@
  *                         threshold loop of exportInsideVoroMesh
  *                         (src/ccthin.cpp:201-424,
  *                          src/highlevelalgo.cpp:815-961), on voxels
+ *   vc_cc_finalize        cellcomplex::finalize (src/cellcomplex.cpp:364-491)
+ *   vc_cc_prune           CellComplexThinning::prune (src/ccthin.cpp:201-771)
  *   vc_boundary_extract   Surfacer::extractBoundaryVts /
  *                         extractBoundaryVtsAndFaces / computeEulerChar
  *                         (src/surfacing.cpp:49-440, :619-660)
@@ -141,6 +143,35 @@ typedef struct vc_euler {
 int vc_boundary_extract(vc_ctx* ctx, vc_vol* vol, float* sites,
                         int64_t site_cap, int64_t* n_sites, int32_t* quads,
                         int64_t quad_cap, int64_t* n_quads, vc_euler* euler);
+
+/* ---- cell complex + simple-pair collapse (host side) -------------------- */
+/* The reference's 2-complex thinning is a FIFO collapse whose output depends
+ * on the queue order (src/ccthin.cpp:248-270, :534-769); it runs on the host,
+ * sequentially, in exactly that order.  No context needed. */
+typedef struct vc_cc_view {
+    int32_t nv, ne, nf;
+    const int32_t* edges;      /* 2*ne vertex ids (finalized edge list)     */
+    const int64_t* face_off;   /* nf+1 offsets into face_edges              */
+    const int32_t* face_edges; /* side edge ids of each face, in face order */
+} vc_cc_view;
+/* cellcomplex::finalize (src/cellcomplex.cpp:364-491): edge ids are the given
+ * edges first (exact-pair dedup, first occurrence wins), then the new face
+ * sides, makeEdge-normalised, in face order.  face_edges_out gets each
+ * face's side edge ids (offsets face_off).  VC_ERR_CAPACITY when edge_cap is
+ * too small (*ne_out still written). */
+int vc_cc_finalize(int32_t nv, int32_t ne_in, const int32_t* edges_in,
+                   int32_t nf, const int64_t* face_off,
+                   const int32_t* face_vts, int32_t* edges_out,
+                   int32_t edge_cap, int32_t* ne_out, int32_t* face_edges_out);
+/* CellComplexThinning::prune(f_t, l_t) (src/ccthin.cpp:201-424, :514-771):
+ * queue every face-edge pair with face measure < f_t (edge order), then every
+ * edge-vertex pair with edge measure < l_t (vertex order), then pop FIFO.
+ * Outputs removed flags (0/1) and the final edge reference counts (faces per
+ * edge).  VC_ERR_STATE where the reference prints "logic err" and exits. */
+int vc_cc_prune(const vc_cc_view* cc, const float* edge_measure,
+                const float* face_measure, float f_t, float l_t,
+                uint8_t* removed_v, uint8_t* removed_e, uint8_t* removed_f,
+                int32_t* ref_e);
 
 /* ---- results ------------------------------------------------------------- */
 /* which: */

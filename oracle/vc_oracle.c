@@ -1075,3 +1075,293 @@ int or_boundary(const uint8_t* occ, int nx, int ny, int nz, float* sites,
     free(qv);
     return rc;
 }
+
+/* ======================================================================
+ * Cell complex + FIFO simple-pair collapse (reference-pinned, R10-R12)
+ * ==================================================================== */
+typedef struct
+{
+    int64_t* keys; /* (a << 32) | (uint32)b, -1 = empty */
+    int32_t* vals;
+    int64_t cap;
+} pair_map;
+
+static uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    return x ^ (x >> 33);
+}
+
+static void pm_init(pair_map* m, int64_t n)
+{
+    int64_t cap = 16;
+    while (cap < 2 * n + 16)
+        cap <<= 1;
+    m->cap = cap;
+    m->keys = (int64_t*)malloc(sizeof(int64_t) * cap);
+    m->vals = (int32_t*)malloc(sizeof(int32_t) * cap);
+    for (int64_t i = 0; i < cap; ++i)
+        m->keys[i] = -1;
+}
+
+/* returns the slot of key (existing or empty) */
+static int64_t pm_slot(const pair_map* m, int64_t key)
+{
+    int64_t h = (int64_t)(mix64((uint64_t)key) & (uint64_t)(m->cap - 1));
+    while (m->keys[h] != -1 && m->keys[h] != key)
+        h = (h + 1) & (m->cap - 1);
+    return h;
+}
+
+static int64_t pair_key(int32_t a, int32_t b)
+{
+    return ((int64_t)(uint32_t)a << 32) | (int64_t)(uint32_t)b;
+}
+
+int or_cc_finalize(int nv, int ne_in, const int32_t* edges_in, int nf,
+                   const int64_t* foff, const int32_t* fv, int32_t* edges_out,
+                   int edge_cap, int32_t* fe_out)
+{
+    pair_map m;
+    pm_init(&m, (int64_t)ne_in + (nf ? foff[nf] : 0));
+    int ne = 0;
+    /* existing edges keep their order; duplicates collapse */
+    for (int e = 0; e < ne_in; ++e)
+    {
+        int64_t k = pair_key(edges_in[2 * e], edges_in[2 * e + 1]);
+        int64_t s = pm_slot(&m, k);
+        if (m.keys[s] == -1)
+        {
+            m.keys[s] = k;
+            m.vals[s] = ne;
+            if (ne < edge_cap)
+            {
+                edges_out[2 * ne] = edges_in[2 * e];
+                edges_out[2 * ne + 1] = edges_in[2 * e + 1];
+            }
+            ne++;
+        }
+    }
+    /* face sides (makeEdge-normalised) in face order */
+    for (int f = 0; f < nf; ++f)
+    {
+        int64_t n = foff[f + 1] - foff[f];
+        for (int64_t i = 0; i < n; ++i)
+        {
+            int32_t u = fv[foff[f] + i], w = fv[foff[f] + (i + 1) % n];
+            int32_t a = u < w ? u : w, b = u < w ? w : u;
+            int64_t k = pair_key(a, b);
+            int64_t s = pm_slot(&m, k);
+            if (m.keys[s] == -1)
+            {
+                m.keys[s] = k;
+                m.vals[s] = ne;
+                if (ne < edge_cap)
+                {
+                    edges_out[2 * ne] = a;
+                    edges_out[2 * ne + 1] = b;
+                }
+                ne++;
+            }
+            fe_out[foff[f] + i] = m.vals[s];
+        }
+    }
+    free(m.keys);
+    free(m.vals);
+    (void)nv;
+    return ne <= edge_cap ? ne : -1;
+}
+
+typedef struct
+{
+    int type; /* 1 = face-edge, 0 = edge-vertex */
+    int32_t a, b;
+} or_pair;
+
+int or_cc_prune(int nv, int ne, const int32_t* edges, int nf,
+                const int64_t* foff, const int32_t* fe, const float* em,
+                const float* fm, float f_t, float l_t, uint8_t* rm_v,
+                uint8_t* rm_e, uint8_t* rm_f, int32_t* ref_e)
+{
+    /* adjacency: vertex -> edges in edge order, edge -> faces in face order
+     * (src/cellcomplex.cpp:456-474) */
+    int64_t* voff = (int64_t*)calloc((size_t)nv + 1, sizeof(int64_t));
+    int64_t* eoff = (int64_t*)calloc((size_t)ne + 1, sizeof(int64_t));
+    for (int e = 0; e < ne; ++e)
+    {
+        voff[edges[2 * e] + 1]++;
+        voff[edges[2 * e + 1] + 1]++;
+    }
+    for (int64_t i = 0; i < (nf ? foff[nf] : 0); ++i)
+        eoff[fe[i] + 1]++;
+    for (int v = 0; v < nv; ++v)
+        voff[v + 1] += voff[v];
+    for (int e = 0; e < ne; ++e)
+        eoff[e + 1] += eoff[e];
+    int32_t* ve = (int32_t*)malloc(sizeof(int32_t) * (size_t)(voff[nv] + 1));
+    int32_t* ef = (int32_t*)malloc(sizeof(int32_t) * (size_t)(eoff[ne] + 1));
+    int64_t* vfill = (int64_t*)calloc((size_t)nv + 1, sizeof(int64_t));
+    int64_t* efill = (int64_t*)calloc((size_t)ne + 1, sizeof(int64_t));
+    for (int e = 0; e < ne; ++e)
+    {
+        int a = edges[2 * e], b = edges[2 * e + 1];
+        ve[voff[a] + vfill[a]++] = e;
+        ve[voff[b] + vfill[b]++] = e;
+    }
+    for (int f = 0; f < nf; ++f)
+        for (int64_t i = foff[f]; i < foff[f + 1]; ++i)
+        {
+            int e = fe[i];
+            ef[eoff[e] + efill[e]++] = f;
+        }
+    int32_t* rv = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nv + 1));
+    for (int v = 0; v < nv; ++v)
+        rv[v] = (int32_t)(voff[v + 1] - voff[v]);
+    for (int e = 0; e < ne; ++e)
+        ref_e[e] = (int32_t)(eoff[e + 1] - eoff[e]);
+    memset(rm_v, 0, (size_t)nv);
+    memset(rm_e, 0, (size_t)ne);
+    memset(rm_f, 0, (size_t)nf);
+
+    int64_t qcap = 1024, qh = 0, qt = 0;
+    or_pair* q = (or_pair*)malloc(sizeof(or_pair) * qcap);
+#define QPUSH(T, A, B)                                                         \
+    do                                                                         \
+    {                                                                          \
+        if (qt == qcap)                                                        \
+        {                                                                      \
+            qcap *= 2;                                                         \
+            q = (or_pair*)realloc(q, sizeof(or_pair) * qcap);                  \
+        }                                                                      \
+        q[qt].type = (T);                                                      \
+        q[qt].a = (A);                                                         \
+        q[qt].b = (B);                                                         \
+        qt++;                                                                  \
+    } while (0)
+
+    for (int e = 0; e < ne; ++e)
+        if (ref_e[e] == 1)
+        {
+            int f = ef[eoff[e]];
+            if (fm[f] < f_t)
+                QPUSH(1, f, e);
+        }
+    for (int v = 0; v < nv; ++v)
+        if (rv[v] == 1)
+        {
+            int e = ve[voff[v]];
+            if (em[e] < l_t)
+                QPUSH(0, e, v);
+        }
+    int rc = 0;
+    while (qh < qt)
+    {
+        or_pair p = q[qh++];
+        if (p.type == 1 ? rm_f[p.a] : rm_e[p.a])
+            continue;
+        if (!((p.type == 1 && ref_e[p.b] == 1) ||
+              (p.type == 0 && rv[p.b] == 1)))
+        {
+            rc = -1; /* the reference prints "logic err" and exits */
+            break;
+        }
+        if (p.type == 0)
+        {
+            rm_e[p.a] = 1;
+            rm_v[p.b] = 1;
+            int a = edges[2 * p.a], b = edges[2 * p.a + 1];
+            rv[a]--;
+            rv[b]--;
+            int other = a != p.b ? a : b;
+            if (rv[other] == 1)
+                for (int64_t i = voff[other]; i < voff[other + 1]; ++i)
+                {
+                    int e2 = ve[i];
+                    if (!rm_e[e2])
+                    {
+                        if (em[e2] < l_t)
+                            QPUSH(0, e2, other);
+                        break;
+                    }
+                }
+        }
+        else
+        {
+            rm_f[p.a] = 1;
+            rm_e[p.b] = 1;
+            ref_e[p.b]--;
+            for (int64_t i = foff[p.a]; i < foff[p.a + 1]; ++i)
+            {
+                int e2 = fe[i];
+                if (rm_e[e2])
+                    continue;
+                ref_e[e2]--;
+                if (ref_e[e2] == 1)
+                    for (int64_t j = eoff[e2]; j < eoff[e2 + 1]; ++j)
+                    {
+                        int f2 = ef[j];
+                        if (!rm_f[f2])
+                        {
+                            if (fm[f2] < f_t)
+                                QPUSH(1, f2, e2);
+                            break;
+                        }
+                    }
+            }
+            for (int k = 0; k < 2; ++k)
+            {
+                int vi = edges[2 * p.b + k];
+                rv[vi]--;
+                if (rv[vi] == 1)
+                    for (int64_t i = voff[vi]; i < voff[vi + 1]; ++i)
+                    {
+                        int e2 = ve[i];
+                        if (!rm_e[e2])
+                        {
+                            if (em[e2] < l_t)
+                                QPUSH(0, e2, vi);
+                            break;
+                        }
+                    }
+            }
+        }
+    }
+#undef QPUSH
+    free(q);
+    free(voff);
+    free(eoff);
+    free(ve);
+    free(ef);
+    free(vfill);
+    free(efill);
+    free(rv);
+    return rc;
+}
+
+int or_simple_triangulate(const int32_t* poly, int n, int32_t* tris)
+{
+    int i = n - 1, j = 0, t = 0, inc_j = 1;
+    while (i - 1 > j)
+    {
+        if (inc_j)
+        {
+            tris[3 * t] = poly[i];
+            tris[3 * t + 1] = poly[j];
+            tris[3 * t + 2] = poly[j + 1];
+            j++;
+        }
+        else
+        {
+            tris[3 * t] = poly[i];
+            tris[3 * t + 1] = poly[j];
+            tris[3 * t + 2] = poly[i - 1];
+            i--;
+        }
+        t++;
+        inc_j = !inc_j;
+    }
+    return t;
+}

@@ -93,6 +93,27 @@ int or_boundary(const uint8_t* occ, int nx, int ny, int nz, float* sites,
                 int64_t site_cap, int64_t* n_sites, int32_t* quads,
                 int64_t quad_cap, int64_t* n_quads, or_euler* euler);
 
+/* ---------------- cell complex + simple-pair collapse (reference-pinned) */
+/* cellcomplex::finalize (src/cellcomplex.cpp:364-491): edge ids = the given
+ * edges first (exact-pair dedup, first occurrence wins), then the new face
+ * sides (makeEdge-normalised) in face order.  fe_out receives each face's
+ * side edge ids (same offsets as fv).  Returns the final edge count, or -1
+ * when edge_cap is too small. */
+int or_cc_finalize(int nv, int ne_in, const int32_t* edges_in, int nf,
+                   const int64_t* foff, const int32_t* fv, int32_t* edges_out,
+                   int edge_cap, int32_t* fe_out);
+/* CellComplexThinning::prune (src/ccthin.cpp:201-424, :514-771): FIFO
+ * collapse of face-edge pairs (face measure < f_t) and edge-vertex pairs
+ * (edge measure < l_t).  Outputs removed flags and final edge ref counts.
+ * Returns 0, or -1 on the reference's "logic err" exit. */
+int or_cc_prune(int nv, int ne, const int32_t* edges, int nf,
+                const int64_t* foff, const int32_t* fe, const float* em,
+                const float* fm, float f_t, float l_t, uint8_t* rm_v,
+                uint8_t* rm_e, uint8_t* rm_f, int32_t* ref_e);
+/* util::simpleTriangulate (src/geomalgo.cpp:13-35) of one polygon; returns
+ * the number of triangles written (n - 2). */
+int or_simple_triangulate(const int32_t* poly, int n, int32_t* tris);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,10 @@ def lib():
         L.or_cube_components.argtypes = [P]
         L.or_boundary.argtypes = [P, i32, i32, i32, P, i64, ctypes.POINTER(i64), P, i64,
                                   ctypes.POINTER(i64), ctypes.POINTER(OrEuler)]
+        L.or_cc_finalize.argtypes = [i32, i32, P, i32, P, P, P, i32, P]
+        L.or_cc_prune.argtypes = [i32, i32, P, i32, P, P, P, P, ctypes.c_float, ctypes.c_float,
+                                  P, P, P, P]
+        L.or_simple_triangulate.argtypes = [P, i32, P]
         _lib = L
     return _lib
 
@@ -171,3 +175,41 @@ def boundary(occ, with_euler=True):
     assert rc == 0
     e = {n: getattr(eu, n) for n, _ in OrEuler._fields_} if with_euler else None
     return sites, quads, e
+
+
+def cc_finalize(nv, edges, faces):
+    """edges: list of (a, b); faces: list of vertex lists.  Returns (edge
+    array (ne, 2), face offsets, face edge ids)."""
+    e_in = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
+    foff = np.zeros(len(faces) + 1, np.int64)
+    for i, f in enumerate(faces):
+        foff[i + 1] = foff[i] + len(f)
+    fv = np.ascontiguousarray(np.concatenate([np.array(f, np.int32) for f in faces])
+                              if faces else np.zeros(0, np.int32))
+    cap = len(e_in) + len(fv) + 1
+    eout = np.zeros((cap, 2), np.int32)
+    fe = np.zeros(max(len(fv), 1), np.int32)
+    ne = lib().or_cc_finalize(nv, len(e_in), _p(e_in), len(faces), _p(foff), _p(fv), _p(eout),
+                              cap, _p(fe))
+    assert ne >= 0
+    return eout[:ne].copy(), foff, fe[:len(fv)].copy()
+
+
+def cc_prune(nv, edges, foff, fe, em, fm, f_t, l_t):
+    edges = np.ascontiguousarray(edges, np.int32)
+    ne, nf = len(edges), len(foff) - 1
+    em = np.ascontiguousarray(em, np.float32)
+    fm = np.ascontiguousarray(fm, np.float32)
+    fe = np.ascontiguousarray(fe, np.int32)
+    rv, re, rf = np.zeros(max(nv, 1), np.uint8), np.zeros(max(ne, 1), np.uint8), np.zeros(max(nf, 1), np.uint8)
+    ref = np.zeros(max(ne, 1), np.int32)
+    rc = lib().or_cc_prune(nv, ne, _p(edges), nf, _p(foff), _p(fe), _p(em), _p(fm), f_t, l_t,
+                           _p(rv), _p(re), _p(rf), _p(ref))
+    return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne]
+
+
+def simple_triangulate(poly):
+    p = np.ascontiguousarray(poly, np.int32)
+    out = np.zeros((max(len(p) - 2, 1), 3), np.int32)
+    n = lib().or_simple_triangulate(_p(p), len(p), _p(out))
+    return out[:n]

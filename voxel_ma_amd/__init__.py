@@ -33,7 +33,7 @@ EXPORTS = [
     "vc_ctx_synchronize", "vc_simple_table_host", "vc_volume_upload",
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
-    "vc_get", "vc_get_device", "vc_synth_blurred_noise",
+    "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_cc_finalize", "vc_cc_prune",
 ]
 
 
@@ -54,6 +54,12 @@ class Euler(ctypes.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class CcView(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int32), ("ne", ctypes.c_int32), ("nf", ctypes.c_int32),
+                ("edges", ctypes.c_void_p), ("face_off", ctypes.c_void_p),
+                ("face_edges", ctypes.c_void_p)]
 
 
 _lib = None
@@ -90,6 +96,8 @@ def library():
         "vc_get_device": [P, P, I, P, I64],
         "vc_synth_blurred_noise": [P, P, I, I, I, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_uint64, I, ctypes.POINTER(ctypes.c_float)],
+        "vc_cc_finalize": [I, I, P, I, P, P, P, I, ctypes.POINTER(I), P],
+        "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -315,3 +323,43 @@ class Volume:
             self.release()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- cell complex
+def cc_finalize(nv, edges, faces):
+    """cellcomplex::finalize through vc_cc_finalize (host side).  edges: list
+    of (a, b); faces: list of vertex lists.  Returns (edges (ne, 2), face
+    offsets, face side edge ids)."""
+    e_in = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
+    foff = np.zeros(len(faces) + 1, np.int64)
+    for i, f in enumerate(faces):
+        foff[i + 1] = foff[i] + len(f)
+    fv = np.ascontiguousarray(np.concatenate([np.array(f, np.int32) for f in faces])
+                              if faces else np.zeros(0, np.int32))
+    cap = len(e_in) + len(fv) + 1
+    eout = np.zeros((cap, 2), np.int32)
+    fe = np.zeros(max(len(fv), 1), np.int32)
+    ne = ctypes.c_int()
+    rc = library().vc_cc_finalize(nv, len(e_in), _p(e_in), len(faces), _p(foff), _p(fv),
+                                  _p(eout), cap, ctypes.byref(ne), _p(fe))
+    if rc != VC_OK:
+        raise VcError(rc, "vc_cc_finalize")
+    return eout[:ne.value].copy(), foff, fe[:len(fv)].copy()
+
+
+def cc_prune(nv, edges, foff, fe, em, fm, f_t, l_t):
+    """CellComplexThinning::prune through vc_cc_prune.  Returns (rc,
+    removed_v, removed_e, removed_f, edge ref counts)."""
+    edges = np.ascontiguousarray(edges, np.int32)
+    foff = np.ascontiguousarray(foff, np.int64)
+    fe = np.ascontiguousarray(fe, np.int32)
+    ne, nf = len(edges), len(foff) - 1
+    em = np.ascontiguousarray(em, np.float32)
+    fm = np.ascontiguousarray(fm, np.float32)
+    rv, re, rf = (np.zeros(max(nv, 1), np.uint8), np.zeros(max(ne, 1), np.uint8),
+                  np.zeros(max(nf, 1), np.uint8))
+    ref = np.zeros(max(ne, 1), np.int32)
+    view = CcView(nv, ne, nf, edges.ctypes.data, foff.ctypes.data, fe.ctypes.data)
+    rc = library().vc_cc_prune(ctypes.byref(view), _p(em), _p(fm), float(f_t), float(l_t),
+                               _p(rv), _p(re), _p(rf), _p(ref))
+    return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne]

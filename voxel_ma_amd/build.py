@@ -11,7 +11,13 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libvoxelcore.so")
 
-HIP_SOURCES = ["vc_core.hip", "vc_edt.hip", "vc_thin.hip", "vc_boundary.hip", "vc_synth.hip"]
+HIP_SOURCES = ["vc_core.hip", "vc_edt.hip", "vc_thin.hip", "vc_boundary.hip", "vc_synth.hip",
+               "vc_ccthin.cpp"]
+HOST = os.path.join(PKG, "host")
+# the drop-in C++ layer (reference class names) linked into the same library
+HOST_SOURCES = ["vc_device.cpp", "geomalgo.cpp", "cellcomplex.cpp", "ccthin.cpp",
+                "surfacing.cpp", "voxelthin.cpp", "highlevelalgo.cpp"]
+CLI = os.path.join(PKG, "voroUtility")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -33,12 +39,26 @@ def _run(cmd, cwd=None):
 
 def build_library(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    srcs += [os.path.join(HOST, s) for s in HOST_SOURCES]
     tmp = LIB + ".tmp"
     out = _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp])
     os.replace(tmp, LIB)
     if verbose and out.strip():
         print(out)
+    build_cli()
     return LIB
+
+
+def build_cli():
+    """voroUtility, linked against libvoxelcore.so (rpath = its directory)."""
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I", "/opt/rocm/include", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HOST, "voroUtility.cpp"), "-o", tmp,
+          "-L", PKG, "-lvoxelcore", "-Wl,-rpath,$ORIGIN",
+          "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def build_oracle():

@@ -1,0 +1,306 @@
+// voroUtility.cpp -- command-line front end keeping the reference's mode and
+// flag names (src/voroUtility.cpp:21-142) for the modes of the thinning path:
+//
+//   -md=vol2mesh <in> <out.off> [-needEuler]      boundary mesh + Euler
+//   -md=vol2skel <in> <out_prefix> [-tt=a,b,c]     EDT + measure + thinning,
+//                                                  one <prefix>_thinned<t>.ply
+//                                                  per threshold
+//   -md=topo -meshToCheck=<m.off>                  closed pockets of a mesh
+//
+// Volumes: .mrc (modes 0, 1, 2, 6) or raw uint8 with -dims=nx,ny,nz (C order
+// [x][y][z]).  Occupancy = value > 0.
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "highlevelalgo.h"
+#include "surfacing.h"
+
+namespace
+{
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+std::vector<double> split_doubles(const std::string& s)
+{
+    std::vector<double> out;
+    std::stringstream ss(s);
+    std::string tok;
+    while (std::getline(ss, tok, ','))
+        if (!tok.empty())
+            out.push_back(std::stod(tok));
+    return out;
+}
+
+bool ends_with(const std::string& s, const std::string& suf)
+{
+    return s.size() >= suf.size() &&
+           s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
+}
+
+// MRC: 1024-byte header (nx, ny, nz, mode, ..., nsymbt at byte 92), data with
+// x fastest; transposed into C order [x][y][z]
+std::shared_ptr<DenseVolume> read_mrc(const std::string& path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return nullptr;
+    int32_t h[256];
+    f.read(reinterpret_cast<char*>(h), 1024);
+    if (!f)
+        return nullptr;
+    const int nx = h[0], ny = h[1], nz = h[2], mode = h[3], next = h[23];
+    if (nx <= 0 || ny <= 0 || nz <= 0)
+        return nullptr;
+    f.seekg(1024 + next);
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> file_order(n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        float v = 0.0f;
+        if (mode == 0)
+        {
+            int8_t b;
+            f.read(reinterpret_cast<char*>(&b), 1);
+            v = b;
+        }
+        else if (mode == 1)
+        {
+            int16_t b;
+            f.read(reinterpret_cast<char*>(&b), 2);
+            v = b;
+        }
+        else if (mode == 2)
+            f.read(reinterpret_cast<char*>(&v), 4);
+        else if (mode == 6)
+        {
+            uint16_t b;
+            f.read(reinterpret_cast<char*>(&b), 2);
+            v = b;
+        }
+        else
+            return nullptr;
+        file_order[i] = v;
+    }
+    if (!f)
+        return nullptr;
+    std::vector<float> c_order(n);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+                c_order[((size_t)x * ny + y) * nz + z] =
+                    file_order[((size_t)z * ny + y) * nx + x];
+    return std::make_shared<DenseVolume>(nx, ny, nz, std::move(c_order));
+}
+
+std::shared_ptr<DenseVolume> read_raw(const std::string& path, const int d[3])
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f || d[0] <= 0 || d[1] <= 0 || d[2] <= 0)
+        return nullptr;
+    std::vector<uint8_t> b((size_t)d[0] * d[1] * d[2]);
+    f.read(reinterpret_cast<char*>(b.data()), (std::streamsize)b.size());
+    if (!f)
+        return nullptr;
+    return DenseVolume::fromOccupancy(d[0], d[1], d[2], b.data());
+}
+
+bool write_off(const std::string& path, const std::vector<point>& v,
+               const std::vector<uTriFace>& t)
+{
+    std::ofstream f(path);
+    if (!f)
+        return false;
+    f << "OFF\n" << v.size() << " " << t.size() << " 0\n";
+    for (const auto& p : v)
+        f << p[0] << " " << p[1] << " " << p[2] << "\n";
+    for (const auto& x : t)
+        f << "3 " << x[0] << " " << x[1] << " " << x[2] << "\n";
+    return (bool)f;
+}
+
+bool read_off(const std::string& path, std::vector<point>& v,
+              std::vector<std::vector<int>>& faces)
+{
+    std::ifstream f(path);
+    std::string magic;
+    size_t nv = 0, nf = 0, ne = 0;
+    if (!(f >> magic) || magic != "OFF" || !(f >> nv >> nf >> ne))
+        return false;
+    v.resize(nv);
+    for (auto& p : v)
+        f >> p[0] >> p[1] >> p[2];
+    faces.resize(nf);
+    for (auto& fc : faces)
+    {
+        int k = 0;
+        f >> k;
+        fc.resize(k);
+        for (auto& i : fc)
+            f >> i;
+    }
+    return (bool)f;
+}
+
+bool write_ply_points(const std::string& path, const std::vector<ivec3>& vox)
+{
+    std::ofstream f(path);
+    if (!f)
+        return false;
+    f << "ply\nformat ascii 1.0\nelement vertex " << vox.size()
+      << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    for (const auto& p : vox)
+        f << p[0] << " " << p[1] << " " << p[2] << "\n";
+    return (bool)f;
+}
+
+int usage()
+{
+    std::cerr << "usage: voroUtility -md=vol2mesh <in.mrc|in.raw> <out.off> "
+                 "[-needEuler] [-dims=nx,ny,nz]\n"
+                 "       voroUtility -md=vol2skel <in> <out_prefix> "
+                 "[-tt=0.01,0.02,0.04] [-dims=...]\n"
+                 "       voroUtility -md=topo -meshToCheck=<mesh.off>\n";
+    return 1;
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    std::map<std::string, std::string> flags;
+    std::vector<std::string> pos;
+    for (int i = 1; i < argc; ++i)
+    {
+        std::string a = argv[i];
+        if (a.size() > 1 && a[0] == '-')
+        {
+            a = a.substr(a[1] == '-' ? 2 : 1);
+            const size_t eq = a.find('=');
+            if (eq == std::string::npos)
+                flags[a] = "true";
+            else
+                flags[a.substr(0, eq)] = a.substr(eq + 1);
+        }
+        else
+            pos.push_back(a);
+    }
+    const std::string md = flags.count("md") ? flags["md"] : "";
+    try
+    {
+        auto load = [&](const std::string& path) -> std::shared_ptr<DenseVolume> {
+            if (ends_with(path, ".mrc"))
+                return read_mrc(path);
+            int d[3] = {0, 0, 0};
+            if (flags.count("dims"))
+            {
+                const auto v = split_doubles(flags["dims"]);
+                for (int k = 0; k < 3 && k < (int)v.size(); ++k)
+                    d[k] = (int)v[k];
+            }
+            return read_raw(path, d);
+        };
+        if (md == "vol2mesh")
+        {
+            if (pos.size() < 2)
+                return usage();
+            auto vol = load(pos[0]);
+            if (!vol)
+            {
+                std::cerr << "cannot read volume " << pos[0] << "\n";
+                return 1;
+            }
+            const auto t0 = Clock::now();
+            Surfacer s;
+            std::vector<point> vts;
+            std::vector<uTriFace> tris;
+            eulerchar ec;
+            const bool need_euler = flags.count("needEuler") && flags["needEuler"] != "false";
+            if (s.extractBoundaryVtsAndFaces(vol, vts, tris, need_euler ? &ec : nullptr) !=
+                SurfacerErrCode::SUCCESS)
+            {
+                std::cerr << "boundary extraction failed\n";
+                return 1;
+            }
+            std::cout << "time(GPU) -> extract boundary: " << ms_since(t0) << " ms\n";
+            if (need_euler)
+                ec.logToConsole("volume boundary (cube complex)");
+            return write_off(pos[1], vts, tris) ? 0 : 1;
+        }
+        if (md == "vol2skel")
+        {
+            if (pos.size() < 2)
+                return usage();
+            auto vol = load(pos[0]);
+            if (!vol)
+            {
+                std::cerr << "cannot read volume " << pos[0] << "\n";
+                return 1;
+            }
+            std::vector<double> tt = {0.04};
+            if (flags.count("tt"))
+                tt = split_doubles(flags["tt"]);
+            const auto t0 = Clock::now();
+            VoxelThinning thin;
+            std::vector<float> levels;
+            voxelvoro::thinVolume(vol, tt, thin, &levels);
+            std::cout << "time(THIN) -> EDT + measure + thinning: " << ms_since(t0)
+                      << " ms\n";
+            std::sort(tt.begin(), tt.end());
+            for (size_t k = 0; k < levels.size(); ++k)
+            {
+                std::vector<ivec3> vox;
+                thin.getRemainedVoxels((int)k, vox);
+                std::ostringstream name;
+                name << pos[1] << "_thinned" << tt[k] << ".ply";
+                std::cout << "level " << levels[k] << " (t=" << tt[k]
+                          << "): " << vox.size() << " voxels -> " << name.str() << "\n";
+                if (!write_ply_points(name.str(), vox))
+                    return 1;
+            }
+            return 0;
+        }
+        if (md == "topo")
+        {
+            if (!flags.count("meshToCheck"))
+                return usage();
+            std::vector<point> v;
+            std::vector<std::vector<int>> faces;
+            if (!read_off(flags["meshToCheck"], v, faces))
+            {
+                std::cerr << "cannot read mesh\n";
+                return 1;
+            }
+            cellcomplex cc(v, {}, {}, false);
+            for (const auto& f : faces)
+                cc.appendFace(f);
+            cc.finalize();
+            eulerchar ec;
+            cc.eulerChar(ec);
+            ec.logToConsole("input mesh");
+            std::cout << "# closed components: " << voxelvoro::nClosedComponents(cc)
+                      << "\n";
+            return 0;
+        }
+    }
+    catch (const std::exception& e)
+    {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+    return usage();
+}
