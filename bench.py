@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-EDT_BYTES_PER_VOXEL = 21  # (1+2) + (2+4) + (4+4+4), see DESIGN.md
+EDT_BYTES_PER_VOXEL = 21  # passes z, y, x: (1+2) + (2+4) + (4+4+4)
 # dense subfield pass over bit planes: alive state (1/8 B) read with its
 # neighbour rows amortised (x 9/8) + the eligibility plane (1/8 B) / 8
 THIN_BYTES_PER_VOXEL_PASS = (1.0 / 8) * (9.0 / 8) + (1.0 / 8) / 8
@@ -91,10 +91,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", action="store_true",
                     help="no CPU baseline, no hashing (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device-to-device copies of 1 GiB (HBM counter "
+                         "calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import voxel_ma_amd
+        nbytes = 1 << 30
+        src, dst = voxel_ma_amd.DeviceBuffer(nbytes), voxel_ma_amd.DeviceBuffer(nbytes)
+        h = voxel_ma_amd.hip()
+        for _ in range(3):
+            assert h.hipMemcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes, 3) == 0
+        h.hipDeviceSynchronize()
+        print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is design-only, see DESIGN.md" % a.gpus, file=sys.stderr)
+              "sharding over several GPUs is a design, not a measured path" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -12,8 +12,8 @@
  * src/surfacing.cpp:275-283), so "linear index order" == "Surfacer order".
  *
  * Each function names the reference file:line it restates, or says
- * "project-defined" where the reference has no counterpart.
- * Pinning of every function is documented in DESIGN.md (oracle section).
+ * "project-defined" where the reference has no counterpart (the EDT, the
+ * per-voxel measure and the voxel subfield thinning).
  */
 #ifndef VC_ORACLE_H
 #define VC_ORACLE_H
