@@ -54,6 +54,18 @@ class HipEvents:
         return out.value
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (512^3 workload only), else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if n != 512 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f).get("traffic_bytes_per_dispatch", {})
+    v = t.get(kernel)
+    return None if v is None else round(v)
+
+
 def cpu_baseline(occ_host, levels, target_s=15.0):
     """The oracle (single-threaded C restatement) on a bounded cubic crop of the
     same volume: EDT + measure + levels + thinning."""
@@ -197,12 +209,16 @@ def main():
                        "launches": int(stats.launches)},
         "roofline": {"bound": "hbm", "kernel": "thin_pass_dense (subfield pass)",
                      "achieved": round(thin_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(thin_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(thin_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("thin_pass_dense", n),
                      "bytes_per_voxel_pass": round(THIN_BYTES_PER_VOXEL_PASS, 4),
                      "avg_pass_us": round(pass_ms * 1e3, 2)},
         "roofline_edt": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21},
+                         "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21,
+                         "traffic": sum(filter(None, (measured_traffic(k, n) for k in
+                                                      ("edt_pass_z", "edt_pass_y",
+                                                       "edt_pass_x")))) or None},
     }
     if not (a.no_cpu_baseline or a.profile_steps):
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
