@@ -33,7 +33,12 @@ EXPORTS = [
     "vc_ctx_synchronize", "vc_simple_table_host", "vc_volume_upload",
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
-    "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_cc_finalize", "vc_cc_prune",
+    "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
+    "vc_cc_prune",
+    # include/vc_batch.h
+    "vc_batch_create", "vc_batch_destroy", "vc_batch_size", "vc_batch_edt", "vc_batch_measure",
+    "vc_batch_thin", "vc_batch_run", "vc_schedule_defaults", "vc_batch_get_schedule",
+    "vc_batch_set_schedule", "vc_batch_tune",
 ]
 
 
@@ -51,6 +56,16 @@ class ThinStats(ctypes.Structure):
 class Euler(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
                 ("V", "E", "F", "T", "C", "euler", "V_bnd", "E_bnd", "V_int", "E_int", "F_int")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Schedule(ctypes.Structure):
+    """vc_schedule: launch parameters of a batch (results never depend on it)."""
+    _fields_ = [("edt_lds_kb", ctypes.c_int), ("edt_grid_per_cu", ctypes.c_int),
+                ("thin_grid_per_cu", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -96,8 +111,20 @@ def library():
         "vc_get_device": [P, P, I, P, I64],
         "vc_synth_blurred_noise": [P, P, I, I, I, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_uint64, I, ctypes.POINTER(ctypes.c_float)],
+        "vc_synth_solids": [P, P, I, I, I, P, I],
         "vc_cc_finalize": [I, I, P, I, P, P, P, I, ctypes.POINTER(I), P],
         "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
+        "vc_batch_create": [P, P, I, PP],
+        "vc_batch_destroy": [P, P],
+        "vc_batch_size": [P, ctypes.POINTER(I), ctypes.POINTER(I64)],
+        "vc_batch_edt": [P, P, I],
+        "vc_batch_measure": [P, P],
+        "vc_batch_thin": [P, P, P, I, ctypes.POINTER(ThinStats)],
+        "vc_batch_run": [P, P, P, I, ctypes.POINTER(ThinStats)],
+        "vc_schedule_defaults": [ctypes.POINTER(Schedule)],
+        "vc_batch_get_schedule": [P, ctypes.POINTER(Schedule)],
+        "vc_batch_set_schedule": [P, P, ctypes.POINTER(Schedule)],
+        "vc_batch_tune": [P, P, P, I, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -227,6 +254,23 @@ class Context:
                                                     1 if shell else 0, ctypes.byref(thr)))
         return buf, thr.value
 
+    def synth_solids(self, shape, prims):
+        """Union of primitives (vc_synth_solids) into a new DeviceBuffer."""
+        nx, ny, nz = (int(s) for s in shape)
+        pr = np.ascontiguousarray(prims, dtype=np.float64).reshape(-1, 16)
+        buf = DeviceBuffer(nx * ny * nz)
+        self.check(self._lib.vc_synth_solids(self.handle, ctypes.c_void_p(buf.ptr), nx, ny, nz,
+                                             _p(pr), len(pr)))
+        return buf
+
+    def solids_volume(self, shape, prims):
+        """A Volume holding the union of primitives, generated on the device."""
+        buf = self.synth_solids(shape, prims)
+        vol = self.from_device(buf.ptr, shape)
+        self.synchronize()
+        buf.free()
+        return vol
+
     def upload(self, occ):
         occ = np.ascontiguousarray(occ, dtype=np.uint8)
         if occ.ndim != 3:
@@ -316,6 +360,92 @@ class Volume:
     def release(self):
         if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.vc_volume_release(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several volumes of one context processed together (vc_batch): one
+    launch per pass over all of them.  Results stay in the member volumes
+    (Volume.get)."""
+
+    def __init__(self, ctx, volumes):
+        self.ctx = ctx
+        self.volumes = list(volumes)
+        arr = (ctypes.c_void_p * len(self.volumes))(*[v.handle.value for v in self.volumes])
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.vc_batch_create(ctx.handle, arr, len(self.volumes), ctypes.byref(h)))
+        self.handle = h
+
+    def _levels(self, levels):
+        lv = np.ascontiguousarray(levels, dtype=np.float32)
+        if lv.ndim == 1:
+            lv = np.ascontiguousarray(np.broadcast_to(lv, (len(self.volumes), lv.size)))
+        if lv.shape[0] != len(self.volumes):
+            raise ValueError("levels: one row per volume")
+        return lv
+
+    def size(self):
+        n, vox = ctypes.c_int(), ctypes.c_int64()
+        self.ctx.check(self.ctx._lib.vc_batch_size(self.handle, ctypes.byref(n),
+                                                   ctypes.byref(vox)))
+        return n.value, vox.value
+
+    def edt(self):
+        self.ctx.check(self.ctx._lib.vc_batch_edt(self.ctx.handle, self.handle, 0))
+        return self
+
+    def measure(self):
+        self.ctx.check(self.ctx._lib.vc_batch_measure(self.ctx.handle, self.handle))
+        return self
+
+    def thin(self, levels, stats=True):
+        lv = self._levels(levels)
+        st = ThinStats() if stats else None
+        self.ctx.check(self.ctx._lib.vc_batch_thin(self.ctx.handle, self.handle, _p(lv),
+                                                   lv.shape[1],
+                                                   ctypes.byref(st) if stats else None))
+        return st
+
+    def run(self, levels, stats=True):
+        lv = self._levels(levels)
+        st = ThinStats() if stats else None
+        self.ctx.check(self.ctx._lib.vc_batch_run(self.ctx.handle, self.handle, _p(lv),
+                                                  lv.shape[1],
+                                                  ctypes.byref(st) if stats else None))
+        return st
+
+    def schedule(self):
+        sc = Schedule()
+        self.ctx.check(self.ctx._lib.vc_batch_get_schedule(self.handle, ctypes.byref(sc)))
+        return sc
+
+    def set_schedule(self, **kw):
+        sc = self.schedule()
+        for k, v in kw.items():
+            setattr(sc, k, int(v))
+        self.ctx.check(self.ctx._lib.vc_batch_set_schedule(self.ctx.handle, self.handle,
+                                                           ctypes.byref(sc)))
+        return sc
+
+    def tune(self, levels, reps=3):
+        """Autotune the schedule on this batch; returns (Schedule, (edt+measure
+        ms, thinning ms))."""
+        lv = self._levels(levels)
+        sc = Schedule()
+        t = (ctypes.c_float * 2)()
+        self.ctx.check(self.ctx._lib.vc_batch_tune(self.ctx.handle, self.handle, _p(lv),
+                                                   lv.shape[1], int(reps), ctypes.byref(sc), t))
+        return sc, (t[0], t[1])
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_batch_destroy(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):

@@ -54,6 +54,23 @@ int64_t vc_level_threshold(float level)
     return (int64_t)std::ceil(l2);
 }
 
+int vc_check_levels(vc_ctx* ctx, const float* levels, int n)
+{
+    if (n < 1 || n > VC_MAX_LEVELS)
+        return vc_fail(ctx, VC_ERR_INVALID, "n_levels %d not in [1, %d]", n,
+                       VC_MAX_LEVELS);
+    for (int k = 0; k < n; ++k)
+    {
+        if (!(levels[k] == levels[k]))
+            return vc_fail(ctx, VC_ERR_INVALID, "level %d is NaN", k);
+        if (k > 0 && levels[k] < levels[k - 1])
+            return vc_fail(ctx, VC_ERR_INVALID,
+                           "levels must be ascending (level %d < level %d)", k,
+                           k - 1);
+    }
+    return VC_OK;
+}
+
 static int bits_for(int n)
 {
     int b = 0;
@@ -266,10 +283,7 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->alive);
     vc_free(v->elig);
     vc_free(v->dflag);
-    vc_free(v->sweep_any);
-    if (v->hcnt)
-        (void)hipHostFree(v->hcnt);
-    vc_free(v->tstats);
+    vc_sched_free(v->self);
     vc_free(v->bnd_sites);
     vc_free(v->bnd_quads);
     delete v;
@@ -292,7 +306,8 @@ int vc_edt(vc_ctx* ctx, vc_vol* vol, int flags)
         return VC_ERR_INVALID;
     if (flags != 0)
         return vc_fail(ctx, VC_ERR_INVALID, "vc_edt: flags must be 0");
-    return vc_edt_run(ctx, vol);
+    int rc = vc_vol_sched(ctx, vol);
+    return rc != VC_OK ? rc : vc_edt_sched(ctx, vol->self);
 }
 
 int vc_measure(vc_ctx* ctx, vc_vol* vol)
@@ -301,7 +316,8 @@ int vc_measure(vc_ctx* ctx, vc_vol* vol)
         return VC_ERR_INVALID;
     if (!vol->edt_done)
         return vc_fail(ctx, VC_ERR_STATE, "vc_measure: run vc_edt first");
-    return vc_measure_run(ctx, vol);
+    int rc = vc_vol_sched(ctx, vol); // the EDT may have come from a batch
+    return rc != VC_OK ? rc : vc_measure_sched(ctx, vol->self);
 }
 
 int vc_set_measure_f32(vc_ctx* ctx, vc_vol* vol, const float* d_measure)
@@ -327,22 +343,14 @@ int vc_thin_levels(vc_ctx* ctx, vc_vol* vol, const float* levels, int n,
 {
     if (!ctx || !vol || vol->ctx != ctx || !levels)
         return VC_ERR_INVALID;
-    if (n < 1 || n > VC_MAX_LEVELS)
-        return vc_fail(ctx, VC_ERR_INVALID, "n_levels %d not in [1, %d]", n,
-                       VC_MAX_LEVELS);
-    for (int k = 0; k < n; ++k)
-    {
-        if (!(levels[k] == levels[k]))
-            return vc_fail(ctx, VC_ERR_INVALID, "level %d is NaN", k);
-        if (k > 0 && levels[k] < levels[k - 1])
-            return vc_fail(ctx, VC_ERR_INVALID,
-                           "levels must be ascending (level %d < level %d)", k,
-                           k - 1);
-    }
+    int rc = vc_check_levels(ctx, levels, n);
+    if (rc != VC_OK)
+        return rc;
     if (vol->measure_kind == 0)
         return vc_fail(ctx, VC_ERR_STATE,
                        "vc_thin_levels: run vc_measure or vc_set_measure_f32");
-    return vc_thin_run(ctx, vol, levels, n, st);
+    rc = vc_vol_sched(ctx, vol);
+    return rc != VC_OK ? rc : vc_thin_sched(ctx, vol->self, levels, n, st);
 }
 
 int vc_boundary_extract(vc_ctx* ctx, vc_vol* vol, float* sites,

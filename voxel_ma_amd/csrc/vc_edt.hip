@@ -14,293 +14,319 @@
 // candidates at distance d cost >= d^2, so the search stops once d^2 exceeds
 // the best value; ties go to the lower index, which makes the final feature
 // the lowest linear index among all nearest background voxels.
+//
+// Every kernel walks a work table (one volume or a batch of volumes, one
+// launch per pass over all of them); with a persistent grid each workgroup
+// loops over its share of the items.
 #include "vc_internal.h"
 
 namespace
 {
 
 constexpr int kBlock = 256;
-// column tiles are capped at 32 KB of LDS so that several workgroups per CU
-// overlap their load and search phases
-constexpr size_t kLdsBudget = 32 * 1024;
+constexpr int kRowsZ = 4; // pass z: one wavefront per row, 4 rows per item
 constexpr uint32_t kOob = 0xFFFFFFFEu; // neighbour outside the grid
-
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Consecutive tiles (which share cache lines and halos) are given to
-// workgroups of the same XCD: block L -> tile (L % 8) * per + L / 8.
-__device__ __forceinline__ int64_t xcd_tile(int64_t ntiles)
-{
-    const int64_t L = blockIdx.x;
-    const int64_t per = (ntiles + 7) / 8;
-    const int64_t T = (L % 8) * per + L / 8;
-    return T < ntiles ? T : -1;
-}
 // every candidate cost fits in int32: the volume constructor rejects shapes
 // whose squared diagonal exceeds INT32_MAX
 constexpr int kInf = 0x7FFFFFFF;
 
+// rows along z: wavefront = row; the row's background bitmask (one ballot
+// per 64 voxels) sits in the wavefront's LDS slice of `stride` words
 __global__ void __launch_bounds__(kBlock)
-    edt_pass_z(const uint8_t* __restrict__ occ, int16_t* __restrict__ zq,
-               int nz)
+    edt_pass_z(vc_work W, int stride)
 {
-    extern __shared__ uint64_t bgmask[];
-    const int64_t row = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
-    const uint8_t* r = occ + row * nz;
-    int16_t* out = zq + row * nz;
-    const int nw = (nz + 63) >> 6;
+    extern __shared__ uint64_t lds_z[];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nwaves = blockDim.x >> 6;
-    for (int w = wave; w < nw; w += nwaves)
+    const int wv = threadIdx.x >> 6;
+    uint64_t* bgmask = lds_z + (size_t)wv * stride;
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
     {
-        int z = (w << 6) + lane;
-        bool bg = (z < nz) && r[z] == 0;
-        uint64_t m = __ballot(bg);
-        if (lane == 0)
-            bgmask[w] = m;
-    }
-    __syncthreads();
-    for (int z = threadIdx.x; z < nz; z += blockDim.x)
-    {
-        const int w = z >> 6, b = z & 63;
-        const uint64_t m = bgmask[w];
-        int res;
-        if ((m >> b) & 1ull)
-            res = z;
-        else
+        int vi;
+        uint32_t lt;
+        vc_item(W, b, vi, lt);
+        const vc_vdesc& d = W.D[vi];
+        const int nz = d.nz;
+        const int64_t row = (int64_t)lt * kRowsZ + wv;
+        const bool active = row < (int64_t)d.nx * d.ny;
+        const uint8_t* r = d.occ + row * nz;
+        const int nw = (nz + 63) >> 6;
+        if (active)
+            for (int w = 0; w < nw; ++w)
+            {
+                const int z = (w << 6) + lane;
+                const bool bg = (z < nz) && r[z] == 0;
+                const uint64_t m = __ballot(bg);
+                if (lane == 0)
+                    bgmask[w] = m;
+            }
+        __syncthreads();
+        if (active)
         {
-            int L = -1, R = -1;
-            uint64_t lm = m & ((1ull << b) - 1ull); // bits below b
-            if (lm)
-                L = (w << 6) + 63 - __clzll(lm);
-            else
-                for (int ww = w - 1; ww >= 0; --ww)
+            int16_t* out = reinterpret_cast<int16_t*>(d.edt) + row * nz;
+            for (int z = lane; z < nz; z += 64)
+            {
+                const int w = z >> 6, bb = z & 63;
+                const uint64_t m = bgmask[w];
+                int res;
+                if ((m >> bb) & 1ull)
+                    res = z;
+                else
                 {
-                    uint64_t q = bgmask[ww];
-                    if (q)
-                    {
-                        L = (ww << 6) + 63 - __clzll(q);
-                        break;
-                    }
+                    int L = -1, R = -1;
+                    const uint64_t lm = m & ((1ull << bb) - 1ull); // below bb
+                    if (lm)
+                        L = (w << 6) + 63 - __clzll(lm);
+                    else
+                        for (int ww = w - 1; ww >= 0; --ww)
+                        {
+                            const uint64_t q = bgmask[ww];
+                            if (q)
+                            {
+                                L = (ww << 6) + 63 - __clzll(q);
+                                break;
+                            }
+                        }
+                    const uint64_t rm = m & ~((2ull << bb) - 1ull); // above bb
+                    if (rm)
+                        R = (w << 6) + __ffsll((unsigned long long)rm) - 1;
+                    else
+                        for (int ww = w + 1; ww < nw; ++ww)
+                        {
+                            const uint64_t q = bgmask[ww];
+                            if (q)
+                            {
+                                R = (ww << 6) + __ffsll((unsigned long long)q) - 1;
+                                break;
+                            }
+                        }
+                    if (L < 0)
+                        res = R;
+                    else if (R < 0)
+                        res = L;
+                    else
+                        res = (z - L <= R - z) ? L : R;
                 }
-            uint64_t rm = m & ~((2ull << b) - 1ull); // bits above b
-            if (rm)
-                R = (w << 6) + __ffsll((unsigned long long)rm) - 1;
-            else
-                for (int ww = w + 1; ww < nw; ++ww)
-                {
-                    uint64_t q = bgmask[ww];
-                    if (q)
-                    {
-                        R = (ww << 6) + __ffsll((unsigned long long)q) - 1;
-                        break;
-                    }
-                }
-            if (L < 0)
-                res = R;
-            else if (R < 0)
-                res = L;
-            else
-                res = (z - L <= R - z) ? L : R;
+                out[z] = (int16_t)res;
+            }
         }
-        out[z] = (int16_t)res;
+        __syncthreads(); // the slice is rewritten by the next item
     }
 }
 
-// columns along y: tile = TZ consecutive z at fixed x, all y
-__global__ void __launch_bounds__(kBlock)
-    edt_pass_y(const int16_t* __restrict__ zq, uint32_t* __restrict__ yz,
-               int nx, int ny, int nz, int TZ)
+// columns along y: item = TZ consecutive z at fixed x, all y.  z' comes from
+// the first half of the edt buffer, (y', z') goes to the ft buffer.
+__global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 {
     extern __shared__ int16_t colz[]; // [ny][TZ]
-    const int nzt = (nz + TZ - 1) / TZ;
-    const int64_t T = xcd_tile((int64_t)nx * nzt);
-    if (T < 0)
-        return;
-    const int z0 = (int)(T % nzt) * TZ;
-    const int x = (int)(T / nzt);
-    const int tz = threadIdx.x % TZ;
-    const int ty0 = threadIdx.x / TZ;
-    const int tys = blockDim.x / TZ;
-    const int z = z0 + tz;
-    const bool zin = z < nz;
-    const int64_t base = (int64_t)x * ny * nz;
-    // 16 independent loads in flight per thread before the LDS stores
-    constexpr int U = 16;
-    for (int y = ty0; y < ny; y += tys * U)
+    const uint32_t per = (W.total + 7) / 8;
+    for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
     {
-        int16_t v[U];
+        const uint32_t b = vc_xcd_item(L, per);
+        if (b >= W.total)
+            continue;
+        int vi;
+        uint32_t T;
+        vc_item(W, b, vi, T);
+        const vc_vdesc& d = W.D[vi];
+        const int ny = d.ny, nz = d.nz, TZ = d.tzy;
+        const int nzt = (nz + TZ - 1) / TZ;
+        const int z0 = (int)(T % nzt) * TZ;
+        const int x = (int)(T / nzt);
+        const int tz = threadIdx.x % TZ;
+        const int ty0 = threadIdx.x / TZ;
+        const int tys = blockDim.x / TZ;
+        const int z = z0 + tz;
+        const bool zin = z < nz;
+        const int64_t base = (int64_t)x * ny * nz;
+        const int16_t* zq = reinterpret_cast<const int16_t*>(d.edt);
+        // 16 independent loads in flight per thread before the LDS stores
+        constexpr int U = 16;
+        for (int y = ty0; y < ny; y += tys * U)
+        {
+            int16_t v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const int yy = y + u * tys;
-            v[u] = (zin && yy < ny) ? zq[base + (int64_t)yy * nz + z]
-                                    : (int16_t)-1;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const int yy = y + u * tys;
-            if (yy < ny)
-                colz[yy * TZ + tz] = v[u];
-        }
-    }
-    __syncthreads();
-    if (!zin)
-        return;
-    for (int y = ty0; y < ny; y += tys)
-    {
-        int best = kInf;
-        int by = -1;
-        {
-            int q = colz[y * TZ + tz];
-            if (q >= 0)
+            for (int u = 0; u < U; ++u)
             {
-                best = (z - q) * (z - q);
-                by = y;
+                const int yy = y + u * tys;
+                v[u] = (zin && yy < ny) ? zq[base + (int64_t)yy * nz + z]
+                                        : (int16_t)-1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const int yy = y + u * tys;
+                if (yy < ny)
+                    colz[yy * TZ + tz] = v[u];
             }
         }
-        for (int d = 1;; ++d)
-        {
-            const int d2 = d * d;
-            if (d2 > best)
-                break;
-            const bool lo = y - d >= 0, hi = y + d < ny;
-            if (!lo && !hi)
-                break;
-            if (lo)
+        __syncthreads();
+        if (zin)
+            for (int y = ty0; y < ny; y += tys)
             {
-                int q = colz[(y - d) * TZ + tz];
-                if (q >= 0)
+                int best = kInf;
+                int by = -1;
                 {
-                    int v = (z - q) * (z - q) + d2;
-                    if (v <= best)
+                    const int q = colz[y * TZ + tz];
+                    if (q >= 0)
                     {
-                        best = v;
-                        by = y - d;
+                        best = (z - q) * (z - q);
+                        by = y;
                     }
                 }
-            }
-            if (hi)
-            {
-                int q = colz[(y + d) * TZ + tz];
-                if (q >= 0)
+                for (int dd = 1;; ++dd)
                 {
-                    int v = (z - q) * (z - q) + d2;
-                    if (v < best)
+                    const int d2 = dd * dd;
+                    if (d2 > best)
+                        break;
+                    const bool lo = y - dd >= 0, hi = y + dd < ny;
+                    if (!lo && !hi)
+                        break;
+                    if (lo)
                     {
-                        best = v;
-                        by = y + d;
+                        const int q = colz[(y - dd) * TZ + tz];
+                        if (q >= 0)
+                        {
+                            const int v = (z - q) * (z - q) + d2;
+                            if (v <= best)
+                            {
+                                best = v;
+                                by = y - dd;
+                            }
+                        }
+                    }
+                    if (hi)
+                    {
+                        const int q = colz[(y + dd) * TZ + tz];
+                        if (q >= 0)
+                        {
+                            const int v = (z - q) * (z - q) + d2;
+                            if (v < best)
+                            {
+                                best = v;
+                                by = y + dd;
+                            }
+                        }
                     }
                 }
+                uint32_t outv = VC_NONE;
+                if (by >= 0)
+                    outv = ((uint32_t)by << 16) |
+                           (uint32_t)(uint16_t)colz[by * TZ + tz];
+                d.ft[base + (int64_t)y * nz + z] = outv;
             }
-        }
-        uint32_t outv = VC_NONE;
-        if (by >= 0)
-            outv = ((uint32_t)by << 16) | (uint32_t)(uint16_t)colz[by * TZ + tz];
-        yz[base + (int64_t)y * nz + z] = outv;
+        __syncthreads(); // colz is rewritten by the next item
     }
 }
 
-// columns along x: tile = TZ consecutive z at fixed y, all x
-__global__ void __launch_bounds__(kBlock)
-    edt_pass_x(const uint32_t* yz, uint32_t* ft,
-               int32_t* __restrict__ d2out, int nx, int ny, int nz, int TZ,
-               vc_pack pk)
+// columns along x: item = TZ consecutive z at fixed y, all x.  Rewrites ft in
+// place (the workgroup stages exactly the elements it later overwrites) and
+// overwrites edt once every z' of the volume has been consumed by pass y.
+__global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
 {
     extern __shared__ uint32_t colyz[]; // [nx][TZ]
-    const int nzt = (nz + TZ - 1) / TZ;
-    const int64_t T = xcd_tile((int64_t)ny * nzt);
-    if (T < 0)
-        return;
-    const int z0 = (int)(T % nzt) * TZ;
-    const int y = (int)(T / nzt);
-    const int tz = threadIdx.x % TZ;
-    const int tx0 = threadIdx.x / TZ;
-    const int txs = blockDim.x / TZ;
-    const int z = z0 + tz;
-    const bool zin = z < nz;
-    const int64_t sx = (int64_t)ny * nz;
-    const int64_t off = (int64_t)y * nz + z;
-    constexpr int U = 16;
-    for (int x = tx0; x < nx; x += txs * U)
+    const uint32_t per = (W.total + 7) / 8;
+    for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
     {
-        uint32_t v[U];
+        const uint32_t b = vc_xcd_item(L, per);
+        if (b >= W.total)
+            continue;
+        int vi;
+        uint32_t T;
+        vc_item(W, b, vi, T);
+        const vc_vdesc& d = W.D[vi];
+        const int nx = d.nx, ny = d.ny, nz = d.nz, TZ = d.tzx;
+        const int nzt = (nz + TZ - 1) / TZ;
+        const int z0 = (int)(T % nzt) * TZ;
+        const int y = (int)(T / nzt);
+        const int tz = threadIdx.x % TZ;
+        const int tx0 = threadIdx.x / TZ;
+        const int txs = blockDim.x / TZ;
+        const int z = z0 + tz;
+        const bool zin = z < nz;
+        const int64_t sx = (int64_t)ny * nz;
+        const int64_t off = (int64_t)y * nz + z;
+        uint32_t* ft = d.ft;
+        constexpr int U = 16;
+        for (int x = tx0; x < nx; x += txs * U)
+        {
+            uint32_t v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const int xx = x + u * txs;
-            v[u] = (zin && xx < nx) ? yz[(int64_t)xx * sx + off] : VC_NONE;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const int xx = x + u * txs;
-            if (xx < nx)
-                colyz[xx * TZ + tz] = v[u];
-        }
-    }
-    __syncthreads();
-    if (!zin)
-        return;
-    for (int x = tx0; x < nx; x += txs)
-    {
-        int best = kInf;
-        int bxv = -1;
-        auto cost = [&](int xx) -> int {
-            uint32_t v = colyz[xx * TZ + tz];
-            if (v == VC_NONE)
-                return kInf;
-            int yy = (int)(v >> 16), zz = (int)(v & 0xFFFFu);
-            return (y - yy) * (y - yy) + (z - zz) * (z - zz);
-        };
-        {
-            int c = cost(x);
-            if (c < kInf)
+            for (int u = 0; u < U; ++u)
             {
-                best = c;
-                bxv = x;
+                const int xx = x + u * txs;
+                v[u] = (zin && xx < nx) ? ft[(int64_t)xx * sx + off] : VC_NONE;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const int xx = x + u * txs;
+                if (xx < nx)
+                    colyz[xx * TZ + tz] = v[u];
             }
         }
-        for (int d = 1;; ++d)
-        {
-            const int dd = d * d;
-            if (dd > best)
-                break;
-            const bool lo = x - d >= 0, hi = x + d < nx;
-            if (!lo && !hi)
-                break;
-            if (lo)
+        __syncthreads();
+        if (zin)
+            for (int x = tx0; x < nx; x += txs)
             {
-                int c = cost(x - d);
-                if (c < kInf && c + dd <= best)
+                int best = kInf;
+                int bxv = -1;
+                auto cost = [&](int xx) -> int {
+                    const uint32_t v = colyz[xx * TZ + tz];
+                    if (v == VC_NONE)
+                        return kInf;
+                    const int yy = (int)(v >> 16), zz = (int)(v & 0xFFFFu);
+                    return (y - yy) * (y - yy) + (z - zz) * (z - zz);
+                };
                 {
-                    best = c + dd;
-                    bxv = x - d;
+                    const int c = cost(x);
+                    if (c < kInf)
+                    {
+                        best = c;
+                        bxv = x;
+                    }
+                }
+                for (int dd = 1;; ++dd)
+                {
+                    const int d2 = dd * dd;
+                    if (d2 > best)
+                        break;
+                    const bool lo = x - dd >= 0, hi = x + dd < nx;
+                    if (!lo && !hi)
+                        break;
+                    if (lo)
+                    {
+                        const int c = cost(x - dd);
+                        if (c < kInf && c + d2 <= best)
+                        {
+                            best = c + d2;
+                            bxv = x - dd;
+                        }
+                    }
+                    if (hi)
+                    {
+                        const int c = cost(x + dd);
+                        if (c < kInf && c + d2 < best)
+                        {
+                            best = c + d2;
+                            bxv = x + dd;
+                        }
+                    }
+                }
+                const int64_t o = (int64_t)x * sx + off;
+                if (bxv < 0)
+                {
+                    ft[o] = VC_NONE;
+                    d.edt[o] = -1;
+                }
+                else
+                {
+                    const uint32_t v = colyz[bxv * TZ + tz];
+                    ft[o] = vc_pack_xyz(d.pk, bxv, (int)(v >> 16),
+                                        (int)(v & 0xFFFFu));
+                    d.edt[o] = (int32_t)best;
                 }
             }
-            if (hi)
-            {
-                int c = cost(x + d);
-                if (c < kInf && c + dd < best)
-                {
-                    best = c + dd;
-                    bxv = x + d;
-                }
-            }
-        }
-        const int64_t o = (int64_t)x * sx + off;
-        if (bxv < 0)
-        {
-            ft[o] = VC_NONE;
-            d2out[o] = -1;
-        }
-        else
-        {
-            uint32_t v = colyz[bxv * TZ + tz];
-            ft[o] = vc_pack_xyz(pk, bxv, (int)(v >> 16), (int)(v & 0xFFFFu));
-            d2out[o] = (int32_t)best;
-        }
+        __syncthreads(); // colyz is rewritten by the next item
     }
 }
 
@@ -311,27 +337,11 @@ __global__ void __launch_bounds__(kBlock)
 // occupancy is read.
 constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
 
-__global__ void __launch_bounds__(kMeasZ * kMeasY)
-    measure_lam2(const uint32_t* __restrict__ ft, uint32_t* __restrict__ lam2,
-                 int nx, int ny, int nz, vc_pack pk)
+__global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
 {
     __shared__ uint32_t S[kMeasY + 2][kMeasZ + 2];
-    const int nzt = (nz + kMeasZ - 1) / kMeasZ;
-    const int nyt = (ny + kMeasY - 1) / kMeasY;
-    const int nxc = (nx + kMeasXC - 1) / kMeasXC;
-    const int64_t T = xcd_tile((int64_t)nzt * nyt * nxc);
-    if (T < 0)
-        return;
-    const int z0 = (int)(T % nzt) * kMeasZ;
-    const int y0 = (int)((T / nzt) % nyt) * kMeasY;
-    const int xa = (int)(T / ((int64_t)nzt * nyt)) * kMeasXC;
-    const int xb = min(nx, xa + kMeasXC);
     const int tz = threadIdx.x, ty = threadIdx.y;
     const int tid = ty * kMeasZ + tz;
-    const int z = z0 + tz, y = y0 + ty;
-    const bool valid = z < nz && y < ny;
-    const int64_t sx = (int64_t)ny * nz;
-    const int64_t row = (int64_t)y * nz + z;
     // halo slots: rows y0-1 and y0+8 (66 each), then columns z0-1 and z0+64
     int hy = 0, hz = 0;
     bool hact = false;
@@ -348,51 +358,76 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY)
         hz = (u & 1) ? kMeasZ : -1;
         hact = true;
     }
-    const int gy = y0 + hy, gz = z0 + hz;
-    const bool hin = hact && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
-    auto load = [&](int xx) -> uint32_t {
-        return (valid && xx >= 0 && xx < nx) ? ft[xx * sx + row] : kOob;
-    };
-    uint32_t prev = load(xa - 1), cur = load(xa);
-    for (int x = xa; x < xb; ++x)
+    const uint32_t per = (W.total + 7) / 8;
+    for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
     {
-        const uint32_t next = load(x + 1);
-        const uint32_t halo =
-            hin ? ft[x * sx + (int64_t)gy * nz + gz] : kOob;
-        S[ty + 1][tz + 1] = valid ? cur : kOob;
-        if (hact)
-            S[hy + 1][hz + 1] = halo;
-        __syncthreads();
-        if (valid)
+        const uint32_t b = vc_xcd_item(L, per);
+        if (b >= W.total)
+            continue;
+        int vi;
+        uint32_t T;
+        vc_item(W, b, vi, T);
+        const vc_vdesc& d = W.D[vi];
+        const int nx = d.nx, ny = d.ny, nz = d.nz;
+        const vc_pack pk = d.pk;
+        const uint32_t* ft = d.ft;
+        const int nzt = (nz + kMeasZ - 1) / kMeasZ;
+        const int nyt = (ny + kMeasY - 1) / kMeasY;
+        const int z0 = (int)(T % nzt) * kMeasZ;
+        const int y0 = (int)((T / nzt) % nyt) * kMeasY;
+        const int xa = (int)(T / ((uint32_t)nzt * nyt)) * kMeasXC;
+        const int xb = min(nx, xa + kMeasXC);
+        const int z = z0 + tz, y = y0 + ty;
+        const bool valid = z < nz && y < ny;
+        const int64_t sx = (int64_t)ny * nz;
+        const int64_t row = (int64_t)y * nz + z;
+        const int gy = y0 + hy, gz = z0 + hz;
+        const bool hin = hact && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+        auto load = [&](int xx) -> uint32_t {
+            return (valid && xx >= 0 && xx < nx) ? ft[xx * sx + row] : kOob;
+        };
+        uint32_t prev = load(xa - 1), cur = load(xa);
+        for (int x = xa; x < xb; ++x)
         {
-            uint32_t m = 0;
-            const uint32_t me = vc_pack_xyz(pk, x, y, z);
-            if (cur != VC_NONE && cur != me)
+            const uint32_t next = load(x + 1);
+            const uint32_t halo =
+                hin ? ft[x * sx + (int64_t)gy * nz + gz] : kOob;
+            S[ty + 1][tz + 1] = valid ? cur : kOob;
+            if (hact)
+                S[hy + 1][hz + 1] = halo;
+            __syncthreads();
+            if (valid)
             {
-                int fx, fy, fz;
-                vc_unpack_xyz(pk, cur, fx, fy, fz);
-                auto visit = [&](uint32_t g, uint32_t q) {
-                    if (g == kOob || g == VC_NONE || g == q)
-                        return;
-                    int gx, gy2, gz2;
-                    vc_unpack_xyz(pk, g, gx, gy2, gz2);
-                    const uint32_t d = (uint32_t)((fx - gx) * (fx - gx)) +
-                                       (uint32_t)((fy - gy2) * (fy - gy2)) +
-                                       (uint32_t)((fz - gz2) * (fz - gz2));
-                    m = d > m ? d : m;
-                };
-                visit(prev, vc_pack_xyz(pk, x - 1, y, z));
-                visit(next, vc_pack_xyz(pk, x + 1, y, z));
-                visit(S[ty][tz + 1], vc_pack_xyz(pk, x, y - 1, z));
-                visit(S[ty + 2][tz + 1], vc_pack_xyz(pk, x, y + 1, z));
-                visit(S[ty + 1][tz], vc_pack_xyz(pk, x, y, z - 1));
-                visit(S[ty + 1][tz + 2], vc_pack_xyz(pk, x, y, z + 1));
+                uint32_t m = 0;
+                const uint32_t me = vc_pack_xyz(pk, x, y, z);
+                if (cur != VC_NONE && cur != me)
+                {
+                    int fx, fy, fz;
+                    vc_unpack_xyz(pk, cur, fx, fy, fz);
+                    auto visit = [&](uint32_t g, uint32_t q) {
+                        if (g == kOob || g == VC_NONE || g == q)
+                            return;
+                        int gx, gy2, gz2;
+                        vc_unpack_xyz(pk, g, gx, gy2, gz2);
+                        const uint32_t dd =
+                            (uint32_t)((fx - gx) * (fx - gx)) +
+                            (uint32_t)((fy - gy2) * (fy - gy2)) +
+                            (uint32_t)((fz - gz2) * (fz - gz2));
+                        m = dd > m ? dd : m;
+                    };
+                    visit(prev, vc_pack_xyz(pk, x - 1, y, z));
+                    visit(next, vc_pack_xyz(pk, x + 1, y, z));
+                    visit(S[ty][tz + 1], vc_pack_xyz(pk, x, y - 1, z));
+                    visit(S[ty + 2][tz + 1], vc_pack_xyz(pk, x, y + 1, z));
+                    visit(S[ty + 1][tz], vc_pack_xyz(pk, x, y, z - 1));
+                    visit(S[ty + 1][tz + 2], vc_pack_xyz(pk, x, y, z + 1));
+                }
+                d.lam2[x * sx + row] = m;
             }
-            lam2[x * sx + row] = m;
+            __syncthreads();
+            prev = cur;
+            cur = next;
         }
-        __syncthreads();
-        prev = cur;
-        cur = next;
     }
 }
 
@@ -414,74 +449,54 @@ __global__ void __launch_bounds__(kBlock)
     out[i] = (int32_t)(((int64_t)x * ny + y) * nz + z);
 }
 
-int pick_tile(int n, size_t elem)
-{
-    int tz = 64;
-    while (tz > 1 && (size_t)tz * (size_t)n * elem > kLdsBudget)
-        tz >>= 1;
-    return tz;
-}
-
 } // namespace
 
-int vc_edt_run(vc_ctx* ctx, vc_vol* v)
+int vc_edt_sched(vc_ctx* ctx, vc_sched* s)
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
-    const size_t N = (size_t)v->N;
-    int rc;
-    if (!v->edt && (rc = vc_alloc(ctx, (void**)&v->edt, 4 * N)) != VC_OK)
-        return rc;
-    if (!v->ft && (rc = vc_alloc(ctx, (void**)&v->ft, 4 * N)) != VC_OK)
-        return rc;
-    // one full column per workgroup at least (TZ = 1 may use up to 64 KB)
-    if ((size_t)v->nx * 4 > 65536 || (size_t)v->ny * 2 > 65536)
-        return vc_fail(ctx, VC_ERR_INVALID, "axis too long for the LDS tiles");
-    // No temporaries: pass z writes int16 z' into the first half of the edt
-    // buffer, pass y writes packed (y', z') into the ft buffer, and pass x
-    // rewrites ft in place (each workgroup stages exactly the elements it
-    // later overwrites) and overwrites edt once every z' has been consumed.
-    int16_t* zq = reinterpret_cast<int16_t*>(v->edt);
-    uint32_t* yz = v->ft;
-    dim3 gz((unsigned)v->ny, (unsigned)v->nx);
-    size_t lz = sizeof(uint64_t) * (size_t)((v->nz + 63) / 64);
-    edt_pass_z<<<gz, kBlock, lz, ctx->stream>>>(v->occ, zq, v->nz);
-    VC_CHECK_LAUNCH(ctx);
+    // No temporaries: pass z writes int16 z' into the first half of each
+    // edt buffer, pass y writes packed (y', z') into the ft buffer, and pass
+    // x rewrites ft in place and then edt.
+    const int pc = s->sc.edt_grid_per_cu;
+    unsigned g = vc_sched_grid(ctx, s, VK_EZ, pc);
+    if (g)
     {
-        int tz = pick_tile(v->ny, 2);
-        const int64_t tiles = (int64_t)v->nx * ((v->nz + tz - 1) / tz);
-        const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
-        edt_pass_y<<<g, kBlock, (size_t)tz * v->ny * 2, ctx->stream>>>(
-            zq, yz, v->nx, v->ny, v->nz, tz);
+        edt_pass_z<<<g, kBlock, s->lds[VK_EZ], ctx->stream>>>(
+            vc_sched_work(s, VK_EZ), s->ez_stride);
         VC_CHECK_LAUNCH(ctx);
     }
+    g = vc_sched_grid(ctx, s, VK_EY, pc);
+    if (g)
     {
-        int tz = pick_tile(v->nx, 4);
-        const int64_t tiles = (int64_t)v->ny * ((v->nz + tz - 1) / tz);
-        const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
-        edt_pass_x<<<g, kBlock, (size_t)tz * v->nx * 4, ctx->stream>>>(
-            yz, v->ft, v->edt, v->nx, v->ny, v->nz, tz, v->pk);
+        edt_pass_y<<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
+            vc_sched_work(s, VK_EY));
         VC_CHECK_LAUNCH(ctx);
     }
-    v->edt_done = true;
+    g = vc_sched_grid(ctx, s, VK_EX, pc);
+    if (g)
+    {
+        edt_pass_x<<<g, kBlock, s->lds[VK_EX], ctx->stream>>>(
+            vc_sched_work(s, VK_EX));
+        VC_CHECK_LAUNCH(ctx);
+    }
+    for (vc_vol* v : s->vols)
+        v->edt_done = true;
     return VC_OK;
 }
 
-int vc_measure_run(vc_ctx* ctx, vc_vol* v)
+int vc_measure_sched(vc_ctx* ctx, vc_sched* s)
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
-    int rc;
-    if (!v->lam2 &&
-        (rc = vc_alloc(ctx, (void**)&v->lam2, 4 * (size_t)v->N)) != VC_OK)
-        return rc;
-    const int64_t tiles = (int64_t)((v->nz + kMeasZ - 1) / kMeasZ) *
-                          ((v->ny + kMeasY - 1) / kMeasY) *
-                          ((v->nx + kMeasXC - 1) / kMeasXC);
-    const unsigned g = (unsigned)(8 * ((tiles + 7) / 8));
-    measure_lam2<<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
-        v->ft, v->lam2, v->nx, v->ny, v->nz, v->pk);
-    VC_CHECK_LAUNCH(ctx);
-    v->measure_kind = 1;
-    return VC_OK;
+    const unsigned g = vc_sched_grid(ctx, s, VK_MS, s->sc.edt_grid_per_cu);
+    if (g)
+    {
+        measure_lam2<<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+            vc_sched_work(s, VK_MS));
+        VC_CHECK_LAUNCH(ctx);
+    }
+    for (vc_vol* v : s->vols)
+        v->measure_kind = 1;
+    return vc_sched_sync_measure(ctx, s);
 }
 
 int vc_ft_to_linear(vc_ctx* ctx, vc_vol* v, int32_t* dst)

@@ -5,9 +5,12 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/vc_api.h"
+#include "../../include/vc_batch.h"
 
 // ---------------------------------------------------------------- context
 struct vc_ctx
@@ -17,6 +20,8 @@ struct vc_ctx
     hipStream_t stream = nullptr;
     uint32_t* d_lut = nullptr; // simple-point table, 2^21 words
     std::string err;
+    // autotuned schedules by batch shape (volumes, voxels, largest axis)
+    std::map<std::vector<int64_t>, vc_schedule> tuned;
 };
 
 // packed voxel coordinate: x << (by+bz) | y << bz | z, fits in 31 bits
@@ -25,6 +30,8 @@ struct vc_pack
     int bx, by, bz;
     uint32_t my, mz;
 };
+
+struct vc_sched;
 
 struct vc_vol
 {
@@ -41,20 +48,18 @@ struct vc_vol
     uint8_t* lvl = nullptr;   // N
     uint8_t* label = nullptr; // N
     // thinning state: bit planes packed along z (64 voxels per word)
-    uint64_t* alive = nullptr;  // alive voxels
-    uint64_t* elig = nullptr;   // eligible at the current level (lvl <= k)
-    uint8_t* dflag = nullptr;   // 2 x words: revisit flags per z parity
-    uint32_t* sweep_any = nullptr; // per sweep of a level: removed anything
-    uint32_t* hcnt = nullptr;   // pinned host staging for counters
-    uint64_t* tstats = nullptr; // device stats accumulators
+    uint64_t* alive = nullptr; // alive voxels
+    uint64_t* elig = nullptr;  // eligible at the current level (lvl <= k)
+    uint8_t* dflag = nullptr;  // 2 x words: revisit flags per z parity
     // boundary stage results (computed once per volume)
     bool bnd_done = false;
-    float* bnd_sites = nullptr;  // 3 floats per site
+    float* bnd_sites = nullptr;   // 3 floats per site
     int32_t* bnd_quads = nullptr; // 4 site indices per quad
     int64_t bnd_nsites = 0, bnd_nquads = 0;
     vc_euler bnd_euler{};
     bool edt_done = false;
     bool thin_done = false;
+    vc_sched* self = nullptr; // schedule of this volume alone
 };
 
 static constexpr uint32_t VC_NONE = 0xFFFFFFFFu;
@@ -76,12 +81,110 @@ int vc_fail(vc_ctx* ctx, int code, const char* fmt, ...);
 int vc_alloc(vc_ctx* ctx, void** p, size_t bytes);
 void vc_free(void* p);
 
+// ---------------------------------------------------------------- schedules
+// Every volume kernel walks a work table: item b of a launch is one tile of
+// one volume.  A batch of volumes is one launch per pass over all of its
+// tiles; a single volume is a batch of one (no table, item = local tile).
+
+// per-volume view read by the kernels (device copy in the schedule)
+struct vc_vdesc
+{
+    int nx, ny, nz, nwz;
+    vc_pack pk;
+    int tzy, tzx; // EDT column tiles: z columns per workgroup (passes y, x)
+    int mkind;    // 1 exact lam2, 2 caller float measure
+    int pad_;
+    const uint8_t* occ;
+    int32_t* edt;
+    uint32_t* ft;
+    uint32_t* lam2;
+    const float* meas;
+    uint8_t* lvl;
+    uint8_t* label;
+    uint64_t* alive;
+    uint64_t* elig;
+    uint8_t* d0; // revisit flags, z parity 0
+    uint8_t* d1; // revisit flags, z parity 1
+};
+
+// kernels with a work table
+enum vc_kind
+{
+    VK_EZ = 0, // EDT pass z: 4 rows per item
+    VK_EY,     // EDT pass y: one (x, z-tile) column tile per item
+    VK_EX,     // EDT pass x: one (y, z-tile) column tile per item
+    VK_MS,     // measure: one 64 x 8 x 32 (z, y, x) tile per item
+    VK_LI,     // level set-up: 4 wavefronts (512 z each) per item
+    VK_TP,     // subfield pass, (x, y) parity 0..3: 256 words per item
+    VK_COUNT = VK_TP + 4
+};
+
+// work table of one kind as the kernels see it
+struct vc_work
+{
+    const vc_vdesc* D;
+    const uint64_t* items; // volume << 32 | local item; null for one volume
+    uint32_t total;
+};
+
+// sweeps per segment when vc_schedule.sweeps_per_sync is 0 (adaptive)
+constexpr int VC_SEG_MIN = 4, VC_SEG_MAX = 16;
+// ring of per-sweep "removed anything" flags
+constexpr int VC_RING = 64;
+// stats counters: kSlots cache lines of [0] tests, [8 + k] removed at level k
+constexpr int VC_CTR_SLOTS = 64;
+constexpr int VC_CTR_WORDS = 8 + VC_MAX_LEVELS;
+
+struct vc_sched
+{
+    int n = 0;
+    std::vector<vc_vol*> vols;
+    std::vector<vc_vdesc> h_desc;
+    vc_vdesc* d_desc = nullptr;
+    vc_vdesc* h_stage = nullptr; // pinned staging of h_desc
+    uint64_t* d_items[VK_COUNT] = {};
+    uint32_t total[VK_COUNT] = {};
+    size_t lds[VK_COUNT] = {};
+    int ez_stride = 0; // LDS words per wavefront in pass z
+    int64_t voxels = 0, words = 0;
+    vc_schedule sc{};
+    // per-volume level thresholds, n x VC_MAX_LEVELS each
+    uint64_t* d_T = nullptr;
+    float* d_L = nullptr;
+    void* h_lv = nullptr; // pinned staging of d_T then d_L
+    std::vector<float> lv_cache;
+    int lv_n = 0;
+    // thinning control: [0] first sweep of the running segment, [1] level
+    uint32_t* d_ctl = nullptr;
+    uint32_t* d_gany = nullptr;  // VC_RING: sweep removed anything (batch)
+    uint32_t* d_vany = nullptr;  // 4 per volume: ring of the same per volume
+    unsigned long long* d_ctr = nullptr;
+    uint32_t* h_ring = nullptr;  // pinned copy of d_gany
+    unsigned long long* h_ctr = nullptr;
+    std::map<int, hipGraphExec_t> graphs; // captured segments by length
+};
+
+int vc_sched_build(vc_ctx* ctx, vc_vol* const* vols, int n,
+                   const vc_schedule& sc, vc_sched** out);
+void vc_sched_free(vc_sched* s);
+// rebuild the tables that depend on the schedule's launch parameters
+int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc);
+// refresh the measure kind / pointer of each volume in the device copy
+int vc_sched_sync_measure(vc_ctx* ctx, vc_sched* s);
+vc_work vc_sched_work(const vc_sched* s, int kind);
+unsigned vc_sched_grid(const vc_ctx* ctx, const vc_sched* s, int kind,
+                       int per_cu);
+vc_schedule vc_schedule_default();
+int vc_vol_ensure(vc_ctx* ctx, vc_vol* v); // allocate the path's buffers
+int vc_vol_sched(vc_ctx* ctx, vc_vol* v);  // build v->self if missing
+
 // ---------------------------------------------------------------- stages
 int vc_lut_build(vc_ctx* ctx);
-int vc_edt_run(vc_ctx* ctx, vc_vol* vol);
-int vc_measure_run(vc_ctx* ctx, vc_vol* vol);
-int vc_thin_run(vc_ctx* ctx, vc_vol* vol, const float* levels, int n,
-                vc_thin_stats* st);
+int vc_edt_sched(vc_ctx* ctx, vc_sched* s);
+int vc_measure_sched(vc_ctx* ctx, vc_sched* s);
+// levels: n x n_levels (row i = ascending levels of volume i)
+int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels,
+                  int n_levels, vc_thin_stats* st);
 int vc_boundary_run(vc_ctx* ctx, vc_vol* vol, float* sites, int64_t site_cap,
                     int64_t* n_sites, int32_t* quads, int64_t quad_cap,
                     int64_t* n_quads, vc_euler* eu);
@@ -89,6 +192,8 @@ int vc_ft_to_linear(vc_ctx* ctx, vc_vol* vol, int32_t* d_dst);
 
 // host helpers
 int64_t vc_level_threshold(float level);
+// n in [1, VC_MAX_LEVELS], no NaN, ascending
+int vc_check_levels(vc_ctx* ctx, const float* levels, int n);
 
 __host__ __device__ inline uint32_t vc_pack_xyz(const vc_pack& p, int x, int y,
                                                 int z)
@@ -101,4 +206,32 @@ __host__ __device__ inline void vc_unpack_xyz(const vc_pack& p, uint32_t v,
     z = (int)(v & p.mz);
     y = (int)((v >> p.bz) & p.my);
     x = (int)(v >> (p.by + p.bz));
+}
+
+// ---------------------------------------------------------------- work items
+// item b of a work table -> (volume, local tile)
+__device__ __forceinline__ void vc_item(const vc_work& W, uint32_t b, int& v,
+                                        uint32_t& lt)
+{
+    if (W.items)
+    {
+        const uint64_t e = W.items[b];
+        v = (int)(e >> 32);
+        lt = (uint32_t)e;
+    }
+    else
+    {
+        v = 0;
+        lt = b;
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  XCD-mapped launches give consecutive items (which share cache lines
+// and halos) to workgroups of one XCD: slot L -> item (L % 8) * per + L / 8,
+// per = ceil(total / 8); with a grid that is a multiple of 8, every slot a
+// workgroup visits stays on its XCD.
+__device__ __forceinline__ uint32_t vc_xcd_item(uint32_t L, uint32_t per)
+{
+    return (L & 7u) * per + (L >> 3);
 }

@@ -123,7 +123,83 @@ __global__ void __launch_bounds__(kBlock)
     occ[i] = in ? 1 : 0;
 }
 
+// Union of solid primitives tested at voxel centres.  Each record is
+// VC_SOLID_WORDS doubles (see vc_synth.h); every operation is a separately
+// rounded IEEE double operation in the stated order (no contraction), so a
+// host restatement in the same order gives the same bytes.
+__global__ void __launch_bounds__(kBlock)
+    solids_kernel(const double* __restrict__ prims, int n_prims, uint8_t* occ,
+                  int nx, int ny, int nz)
+{
+#pragma clang fp contract(off)
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int x = blockIdx.z;
+    if (z >= nz)
+        return;
+    uint8_t in = 0;
+    const bool shell = x == 0 || y == 0 || z == 0 || x == nx - 1 ||
+                       y == ny - 1 || z == nz - 1;
+    for (int p = 0; p < n_prims && !in && !shell; ++p)
+    {
+        const double* r = prims + (size_t)p * VC_SOLID_WORDS;
+        const double dx = (double)x - r[1], dy = (double)y - r[2],
+                     dz = (double)z - r[3];
+        double q[3];
+        for (int j = 0; j < 3; ++j)
+            q[j] = (dx * r[4 + j] + dy * r[7 + j]) + dz * r[10 + j];
+        if (r[0] == 0.0)
+        {
+            const double t0 = q[0] / r[13], t1 = q[1] / r[14], t2 = q[2] / r[15];
+            in = ((t0 * t0 + t1 * t1) + t2 * t2) <= 1.0;
+        }
+        else
+        {
+            const double h = r[14];
+            const double t = q[0] < -h ? -h : (q[0] > h ? h : q[0]);
+            const double e = q[0] - t;
+            in = ((e * e + q[1] * q[1]) + q[2] * q[2]) <= r[13] * r[13];
+        }
+    }
+    occ[((int64_t)x * ny + y) * nz + z] = in;
+}
+
 } // namespace
+
+extern "C" int vc_synth_solids(vc_ctx* ctx, uint8_t* d_occ, int nx, int ny,
+                               int nz, const double* prims, int n_prims)
+{
+    if (!ctx || !d_occ || nx <= 0 || ny <= 0 || nz <= 0 || n_prims < 0 ||
+        (n_prims > 0 && !prims))
+        return VC_ERR_INVALID;
+    if (ny > 65535 || nx > 65535)
+        return vc_fail(ctx, VC_ERR_INVALID, "vc_synth_solids: axis too long");
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    double* dp = nullptr;
+    const size_t bytes = sizeof(double) * VC_SOLID_WORDS * (size_t)n_prims;
+    int rc = vc_alloc(ctx, (void**)&dp, bytes);
+    if (rc != VC_OK)
+        return rc;
+    hipError_t e = hipSuccess;
+    if (n_prims > 0)
+        e = hipMemcpyAsync(dp, prims, bytes, hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+    {
+        const dim3 g((unsigned)((nz + kBlock - 1) / kBlock), (unsigned)ny,
+                     (unsigned)nx);
+        solids_kernel<<<g, kBlock, 0, ctx->stream>>>(dp, n_prims, d_occ, nx,
+                                                     ny, nz);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    vc_free(dp);
+    if (e != hipSuccess)
+        return vc_fail(ctx, VC_ERR_HIP, "vc_synth_solids: %s",
+                       hipGetErrorString(e));
+    return VC_OK;
+}
 
 extern "C" int vc_synth_blurred_noise(vc_ctx* ctx, uint8_t* d_occ, int nx,
                                       int ny, int nz, float sigma,

@@ -15,9 +15,11 @@
 // background face neighbour), and only those go through the simple-point
 // table.  The thread owns its word for the pass, so removals need no atomics.
 // Per-word revisit flags (one byte per word and z parity, plain stores) skip
-// words whose neighbourhood did not change since their last visit.  A level
-// ends after a sweep that removed nothing (later passes exit at once); the
-// host checks the per-sweep "removed anything" flags once per batch of sweeps.
+// words whose neighbourhood did not change since their last visit.  A volume's
+// level ends after a sweep that removed nothing in it (its items skip later
+// passes); a batch's level ends when that holds for every volume.  Sweeps are
+// launched in segments (optionally one captured graph per segment length);
+// the host checks the per-sweep "removed anything" flags once per segment.
 #include <cstring>
 #include <vector>
 
@@ -118,16 +120,6 @@ __global__ void __launch_bounds__(kBlock) lut_build(uint32_t* lut)
     lut[w] = word;
 }
 
-// ------------------------------------------------------------ levels
-struct LevelsU64
-{
-    uint64_t T[VC_MAX_LEVELS];
-};
-struct LevelsF32
-{
-    float L[VC_MAX_LEVELS];
-};
-
 // ------------------------------------------------------------ bit planes
 // Alive and eligible voxels are bit planes packed along z: word (x, y, w)
 // holds z in [64w, 64w+64), bit z & 63; index (x*ny + y)*nwz + w.
@@ -141,104 +133,131 @@ __device__ __forceinline__ int64_t widx(const Plane& P, int x, int y, int w)
     return ((int64_t)x * P.ny + y) * P.nwz + w;
 }
 
-// Level set-up.  A wavefront covers 512 consecutive z of one row (8 words);
-// lane l handles z = z0 + 8l .. z0 + 8l + 7 and lanes 8j .. 8j+7 assemble
-// word j.  Words holding an alive voxel that becomes eligible at level k are
-// flagged for both z parities (every other eligible voxel was found
-// non-simple when the previous level converged and its neighbourhood has not
-// changed since).
-template <int kMode> // 0: exact measure, 1: float measure, 2: later level
+// Level set-up.  An item is 4 wavefronts; a wavefront covers 512 consecutive
+// z of one row (8 words): lane l handles z = z0 + 8l .. z0 + 8l + 7 and lanes
+// 8j .. 8j+7 assemble word j.  Words holding an alive voxel that becomes
+// eligible at level k are flagged for both z parities (every other eligible
+// voxel was found non-simple when the previous level converged and its
+// neighbourhood has not changed since).  Level 0 derives each voxel's first
+// eligible level from its volume's thresholds (exact: lam2 < T[q]; float
+// measure: m < L[q]); later levels read it back.  The launch also resets the
+// level's sweep control (sweep counter, level, "removed anything" rings).
+constexpr int kLiWaves = 4;
+
 __global__ void __launch_bounds__(kBlock)
-    thin_level_init(const uint8_t* __restrict__ occ,
-                    const uint32_t* __restrict__ lam2,
-                    const float* __restrict__ meas, uint8_t* __restrict__ lvl,
-                    uint8_t* __restrict__ label, uint64_t* __restrict__ A,
-                    uint64_t* __restrict__ E, uint8_t* __restrict__ D0,
-                    uint8_t* __restrict__ D1, Plane P, int nchunk,
-                    LevelsU64 lt, LevelsF32 lf, int nlev, int k)
+    thin_level_init(vc_work W, const uint64_t* __restrict__ Tq,
+                    const float* __restrict__ Lq, int nlev, int k, int nvol,
+                    uint32_t* __restrict__ ctl, uint32_t* __restrict__ gany,
+                    uint32_t* __restrict__ vany)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t rowi = wave / nchunk;
-    if (rowi >= (int64_t)P.nx * P.ny)
-        return;
-    const int z0 = (int)(wave % nchunk) * 512 + 8 * lane;
-    const int64_t base = rowi * P.nz;
-    uint32_t am = 0, em = 0, nm = 0;
-    uint8_t lv8[8], lb8[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
+    const int wv = threadIdx.x >> 6;
     {
-        const int z = z0 + j;
-        uint8_t l = 255, lb = 0;
-        if (z < P.nz)
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t gs = gridDim.x * blockDim.x;
+        if (g == 0)
         {
-            const int64_t i = base + z;
-            if (kMode == 2)
-            {
-                l = lvl[i];
-                lb = label[i] == 1;
-            }
-            else if (occ[i])
-            {
-                lb = 1;
-                if (kMode == 0)
-                {
-                    const uint64_t m = lam2[i];
-                    for (int q = 0; q < nlev; ++q)
-                        if (m < lt.T[q])
-                        {
-                            l = (uint8_t)q;
-                            break;
-                        }
-                }
-                else
-                {
-                    const float m = meas[i];
-                    for (int q = 0; q < nlev; ++q)
-                        if (m < lf.L[q])
-                        {
-                            l = (uint8_t)q;
-                            break;
-                        }
-                }
-            }
+            ctl[0] = 0;
+            ctl[1] = (uint32_t)k;
         }
-        lv8[j] = l;
-        lb8[j] = lb;
-        am |= (uint32_t)lb << j;
-        em |= (uint32_t)(l <= k) << j;
-        nm |= (uint32_t)(l == k) << j;
+        for (uint32_t i = g; i < (uint32_t)VC_RING; i += gs)
+            gany[i] = 0;
+        for (uint32_t i = g; i < 4u * (uint32_t)nvol; i += gs)
+            vany[i] = 0;
     }
-    if (kMode != 2)
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
     {
+        int vi;
+        uint32_t lt;
+        vc_item(W, b, vi, lt);
+        const vc_vdesc& d = W.D[vi];
+        const int nz = d.nz;
+        const int nchunk = (nz + 511) >> 9;
+        const int64_t wave = (int64_t)lt * kLiWaves + wv;
+        const int64_t rowi = wave / nchunk;
+        if (rowi >= (int64_t)d.nx * d.ny) // whole wavefront
+            continue;
+        const int z0 = (int)(wave % nchunk) * 512 + 8 * lane;
+        const int64_t base = rowi * nz;
+        const int mode = k > 0 ? 2 : (d.mkind == 2 ? 1 : 0);
+        const uint64_t* T = Tq + (size_t)vi * VC_MAX_LEVELS;
+        const float* Lf = Lq + (size_t)vi * VC_MAX_LEVELS;
+        uint32_t am = 0, em = 0, nm = 0;
+        uint8_t lv8[8], lb8[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            if (z0 + j < P.nz)
+        {
+            const int z = z0 + j;
+            uint8_t l = 255, lb = 0;
+            if (z < nz)
             {
-                lvl[base + z0 + j] = lv8[j];
-                label[base + z0 + j] = lb8[j];
+                const int64_t i = base + z;
+                if (mode == 2)
+                {
+                    l = d.lvl[i];
+                    lb = d.label[i] == 1;
+                }
+                else if (d.occ[i])
+                {
+                    lb = 1;
+                    if (mode == 0)
+                    {
+                        const uint64_t m = d.lam2[i];
+                        for (int q = 0; q < nlev; ++q)
+                            if (m < T[q])
+                            {
+                                l = (uint8_t)q;
+                                break;
+                            }
+                    }
+                    else
+                    {
+                        const float m = d.meas[i];
+                        for (int q = 0; q < nlev; ++q)
+                            if (m < Lf[q])
+                            {
+                                l = (uint8_t)q;
+                                break;
+                            }
+                    }
+                }
             }
-    }
-    const int sh = 8 * (lane & 7);
-    uint64_t a = (uint64_t)am << sh, e = (uint64_t)em << sh,
-             nw = (uint64_t)nm << sh;
+            lv8[j] = l;
+            lb8[j] = lb;
+            am |= (uint32_t)lb << j;
+            em |= (uint32_t)(l <= k) << j;
+            nm |= (uint32_t)(l == k) << j;
+        }
+        if (mode != 2)
+        {
 #pragma unroll
-    for (int d = 1; d <= 4; d <<= 1)
-    {
-        a |= __shfl_xor(a, d);
-        e |= __shfl_xor(e, d);
-        nw |= __shfl_xor(nw, d);
-    }
-    if ((lane & 7) == 0 && z0 < P.nz)
-    {
-        const int64_t o = rowi * P.nwz + (z0 >> 6);
-        E[o] = e;
-        const uint8_t f = (nw & a) ? 1 : 0;
-        D0[o] = f;
-        D1[o] = f;
-        if (kMode != 2)
-            A[o] = a;
+            for (int j = 0; j < 8; ++j)
+                if (z0 + j < nz)
+                {
+                    d.lvl[base + z0 + j] = lv8[j];
+                    d.label[base + z0 + j] = lb8[j];
+                }
+        }
+        const int sh = 8 * (lane & 7);
+        uint64_t a = (uint64_t)am << sh, e = (uint64_t)em << sh,
+                 nw = (uint64_t)nm << sh;
+#pragma unroll
+        for (int dd = 1; dd <= 4; dd <<= 1)
+        {
+            a |= __shfl_xor(a, dd);
+            e |= __shfl_xor(e, dd);
+            nw |= __shfl_xor(nw, dd);
+        }
+        if ((lane & 7) == 0 && z0 < nz)
+        {
+            const int64_t o = rowi * d.nwz + (z0 >> 6);
+            d.elig[o] = e;
+            const uint8_t f = (nw & a) ? 1 : 0;
+            d.d0[o] = f;
+            d.d1[o] = f;
+            if (mode != 2)
+                d.alive[o] = a;
+        }
     }
 }
 
@@ -281,12 +300,6 @@ __device__ __forceinline__ uint32_t nb_code(const uint64_t (&R)[3][3],
     return code;
 }
 
-// Counters are spread over kSlots cache lines (one atomic per workgroup into
-// slot blockIdx % kSlots): same-address device-scope atomics from every
-// wavefront serialise across the XCDs.
-constexpr int kSlots = 64;
-constexpr int kSlotWords = 8 + VC_MAX_LEVELS; // [0] tests, [8 + k] removed
-
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
 {
 #pragma unroll
@@ -302,168 +315,299 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return tot;
 }
 
-// One subfield pass, dense over the words of the subfield's rows (thread =
-// word; grid y = row x/2, grid x covers (y/2, w) with w padded to 2^lgw).
+// One subfield pass over the words of the subfield's (x, y) rows: an item is
+// 256 words of one volume, thread = word, words ordered (x/2, y/2, w).
 // A word is visited only when its z-parity flag is set: the flag of z-parity
-// p (D0 / D1) is raised whenever a voxel of the 26-neighbourhood of one of
+// p (d0 / d1) is raised whenever a voxel of the 26-neighbourhood of one of
 // its parity-p voxels is removed.  All flag traffic is plain byte stores: in
 // pass (sx, sy, sz) the flags of parity sz are cleared only on rows of parity
 // (sx, sy), and raised only on other rows.
+//
+// Sweep control lives on the device so that a segment of sweeps can be
+// replayed as a captured graph: sweep = ctl[0] + j, level = ctl[1].  A volume
+// whose previous sweep removed nothing has converged at this level (its next
+// sweep would visit no word) and is skipped.  The "removed anything" flags
+// are rings: vany[4v + (sweep & 3)] per volume, gany[sweep % VC_RING] for the
+// batch; the first pass of a sweep clears the entries of sweep + 1 (vany) and
+// sweep + VC_RING/2 (gany), which no pass of this sweep reads or writes.
 __global__ void __launch_bounds__(kBlock)
-    thin_pass_dense(uint64_t* A, const uint64_t* __restrict__ E,
-                    uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1,
-                    const uint32_t* __restrict__ lut, Plane P, int lgw, int s,
-                    int k, int sweep, uint32_t* __restrict__ sweep_any,
-                    unsigned long long* __restrict__ ctr)
+    thin_pass(vc_work W, const uint32_t* __restrict__ lut, int s, int j,
+              int nvol, const uint32_t* __restrict__ ctl, uint32_t* gany,
+              uint32_t* vany, unsigned long long* __restrict__ ctr)
 {
-    // a level ends after a sweep that removed nothing; later passes are no-ops
-    if (sweep > 0 && sweep_any[sweep - 1] == 0)
-        return;
+    const uint32_t sweep = ctl[0] + (uint32_t)j;
+    const int k = (int)ctl[1];
     __shared__ uint32_t red[kBlock / 64];
-    const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
-    const int nyh = (P.ny - sy + 1) >> 1;
-    const uint32_t tt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int w = (int)(tt & ((1u << lgw) - 1u));
-    const int yh = (int)(tt >> lgw);
-    const int x = 2 * (int)blockIdx.y + sx;
-    const int y = 2 * yh + sy;
-    uint8_t* Fs = sz ? D1 : D0; // this pass's parity
-    uint8_t* Fo = sz ? D0 : D1; // the other parity
-    uint64_t removed = 0;
-    uint32_t ntest = 0;
-    if (yh < nyh && w < P.nwz)
+    if (s == 0)
     {
-        const int64_t o = widx(P, x, y, w);
-        uint64_t C = 0, cand = 0;
-        if (Fs[o])
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t gs = gridDim.x * blockDim.x;
+        for (uint32_t i = g; i < (uint32_t)nvol; i += gs)
+            vany[4 * i + ((sweep + 1) & 3)] = 0;
+        if (g == 0)
+            gany[(sweep + VC_RING / 2) & (VC_RING - 1)] = 0;
+    }
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+    const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    {
+        int vi;
+        uint32_t lt;
+        vc_item(W, b, vi, lt);
+        if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
+            continue; // converged (block-uniform)
+        const vc_vdesc& d = W.D[vi];
+        const Plane P{d.nx, d.ny, d.nz, d.nwz};
+        uint64_t* A = d.alive;
+        const uint64_t* E = d.elig;
+        uint8_t* D0 = d.d0;
+        uint8_t* D1 = d.d1;
+        uint8_t* Fs = sz ? D1 : D0; // this pass's parity
+        uint8_t* Fo = sz ? D0 : D1; // the other parity
+        const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
+        const uint32_t t = lt * kBlock + threadIdx.x;
+        const uint32_t r = t / (uint32_t)P.nwz;
+        const int w = (int)(t - r * (uint32_t)P.nwz);
+        const uint32_t xh = r / (uint32_t)nyh;
+        const int yh = (int)(r - xh * (uint32_t)nyh);
+        const int x = 2 * (int)xh + sx;
+        const int y = 2 * yh + sy;
+        uint64_t removed = 0;
+        uint32_t ntest = 0;
+        if ((int)xh < nxh)
         {
-            Fs[o] = 0;
-            C = A[o];
-            const uint64_t zsub =
-                sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
-            cand = C & E[o] & zsub;
-        }
-        if (cand)
-        {
-            uint64_t R[3][3];
-            uint32_t pv[3][3], nx0[3][3];
-            const bool need_prev = (cand & 1ull) && w > 0;
-            const bool need_next = (cand >> 63) && w + 1 < P.nwz;
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-                for (int dy = -1; dy <= 1; ++dy)
-                {
-                    const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
-                                    (unsigned)(y + dy) < (unsigned)P.ny;
-                    const int64_t q = widx(P, x + dx, y + dy, w);
-                    R[dx + 1][dy + 1] =
-                        (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
-                    pv[dx + 1][dy + 1] =
-                        (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
-                    nx0[dx + 1][dy + 1] =
-                        (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
-                }
-            // border filter: a voxel with all six face neighbours alive is
-            // never simple
-            const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
-            const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
-            cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
-            // four independent table lookups in flight per round
-            while (cand)
+            const int64_t o = widx(P, x, y, w);
+            uint64_t C = 0, cand = 0;
+            if (Fs[o])
             {
-                int bb[4];
-                uint32_t cc[4];
-                int nb = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    bb[j] = 0;
-                    cc[j] = 0;
-                    if (cand)
-                    {
-                        bb[j] = __ffsll((unsigned long long)cand) - 1;
-                        cand &= cand - 1;
-                        cc[j] = nb_code(R, pv, nx0, bb[j]);
-                        nb = j + 1;
-                    }
-                }
-                uint32_t lw[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    lw[j] = j < nb ? lut[cc[j] >> 5] : 0u;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if ((lw[j] >> (cc[j] & 31)) & 1u)
-                        removed |= 1ull << bb[j];
-                ntest += nb;
+                Fs[o] = 0;
+                C = A[o];
+                cand = C & E[o] & zsub;
             }
-            if (removed)
+            if (cand)
             {
-                A[o] = C & ~removed;
-                uint8_t* row = label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
-                uint64_t m = removed;
-                while (m)
-                {
-                    const int b = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    row[b] = (uint8_t)(2 + k);
-                }
-                // raise the flags of every word holding a neighbour
-                const bool lo = (removed & 1ull) && w > 0;
-                const bool hi = (removed >> 63) && w + 1 < P.nwz;
+                uint64_t R[3][3];
+                uint32_t pv[3][3], nx0[3][3];
+                const bool need_prev = (cand & 1ull) && w > 0;
+                const bool need_next = (cand >> 63) && w + 1 < P.nwz;
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx)
 #pragma unroll
                     for (int dy = -1; dy <= 1; ++dy)
                     {
-                        if ((unsigned)(x + dx) >= (unsigned)P.nx ||
-                            (unsigned)(y + dy) >= (unsigned)P.ny)
-                            continue;
+                        const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
+                                        (unsigned)(y + dy) < (unsigned)P.ny;
                         const int64_t q = widx(P, x + dx, y + dy, w);
-                        if (dx == 0 && dy == 0)
+                        R[dx + 1][dy + 1] =
+                            (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
+                        pv[dx + 1][dy + 1] =
+                            (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
+                        nx0[dx + 1][dy + 1] =
+                            (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
+                    }
+                // border filter: a voxel with all six face neighbours alive
+                // is never simple
+                const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
+                const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
+                cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
+                // four independent table lookups in flight per round
+                while (cand)
+                {
+                    int bb[4];
+                    uint32_t cc[4];
+                    int nb = 0;
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                    {
+                        bb[jj] = 0;
+                        cc[jj] = 0;
+                        if (cand)
                         {
-                            Fo[q] = 1;
-                            if (lo)
-                                Fo[q - 1] = 1;
-                            if (hi)
-                                Fo[q + 1] = 1;
-                        }
-                        else
-                        {
-                            D0[q] = 1;
-                            D1[q] = 1;
-                            if (lo)
-                            {
-                                D0[q - 1] = 1;
-                                D1[q - 1] = 1;
-                            }
-                            if (hi)
-                            {
-                                D0[q + 1] = 1;
-                                D1[q + 1] = 1;
-                            }
+                            bb[jj] = __ffsll((unsigned long long)cand) - 1;
+                            cand &= cand - 1;
+                            cc[jj] = nb_code(R, pv, nx0, bb[jj]);
+                            nb = jj + 1;
                         }
                     }
+                    uint32_t lw[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        lw[jj] = jj < nb ? lut[cc[jj] >> 5] : 0u;
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        if ((lw[jj] >> (cc[jj] & 31)) & 1u)
+                            removed |= 1ull << bb[jj];
+                    ntest += nb;
+                }
+                if (removed)
+                {
+                    A[o] = C & ~removed;
+                    uint8_t* row =
+                        d.label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
+                    uint64_t m = removed;
+                    while (m)
+                    {
+                        const int bt = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        row[bt] = (uint8_t)(2 + k);
+                    }
+                    // raise the flags of every word holding a neighbour
+                    const bool lo = (removed & 1ull) && w > 0;
+                    const bool hi = (removed >> 63) && w + 1 < P.nwz;
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+                        for (int dy = -1; dy <= 1; ++dy)
+                        {
+                            if ((unsigned)(x + dx) >= (unsigned)P.nx ||
+                                (unsigned)(y + dy) >= (unsigned)P.ny)
+                                continue;
+                            const int64_t q = widx(P, x + dx, y + dy, w);
+                            if (dx == 0 && dy == 0)
+                            {
+                                Fo[q] = 1;
+                                if (lo)
+                                    Fo[q - 1] = 1;
+                                if (hi)
+                                    Fo[q + 1] = 1;
+                            }
+                            else
+                            {
+                                D0[q] = 1;
+                                D1[q] = 1;
+                                if (lo)
+                                {
+                                    D0[q - 1] = 1;
+                                    D1[q - 1] = 1;
+                                }
+                                if (hi)
+                                {
+                                    D0[q + 1] = 1;
+                                    D1[q + 1] = 1;
+                                }
+                            }
+                        }
+                }
             }
         }
-    }
-    const uint32_t nrem = block_sum((uint32_t)__popcll(removed), red);
-    __syncthreads();
-    const uint32_t ntst = block_sum(ntest, red);
-    if (threadIdx.x == 0)
-    {
-        unsigned long long* c =
-            ctr + (size_t)((blockIdx.x + blockIdx.y * gridDim.x) % kSlots) *
-                      kSlotWords;
-        if (nrem)
+        const uint32_t nrem = block_sum((uint32_t)__popcll(removed), red);
+        __syncthreads();
+        const uint32_t ntst = block_sum(ntest, red);
+        if (threadIdx.x == 0)
         {
-            sweep_any[sweep] = 1u;
-            atomicAdd(&c[8 + k], (unsigned long long)nrem);
+            // counters spread over VC_CTR_SLOTS cache lines: same-address
+            // device-scope atomics from every workgroup serialise across XCDs
+            unsigned long long* c = ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
+            if (nrem)
+            {
+                gany[sweep & (VC_RING - 1)] = 1u;
+                vany[4 * vi + (sweep & 3)] = 1u;
+                atomicAdd(&c[8 + k], (unsigned long long)nrem);
+            }
+            if (ntst)
+                atomicAdd(&c[0], (unsigned long long)ntst);
         }
-        if (ntst)
-            atomicAdd(&c[0], (unsigned long long)ntst);
+        __syncthreads(); // red is rewritten by the next item
     }
+}
+
+// the next segment's first sweep
+__global__ void seg_advance(uint32_t* ctl, int K)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctl[0] += (uint32_t)K;
+}
+
+int pass_count(const vc_sched* s)
+{
+    int c = 0;
+    for (int sub = 0; sub < 8; ++sub)
+        c += s->total[VK_TP + (sub & 3)] != 0;
+    return c;
+}
+
+// K sweeps (8 subfield passes each) and the segment advance
+int launch_segment(vc_ctx* ctx, vc_sched* s, int K)
+{
+    hipStream_t S = ctx->stream;
+    const int pc = s->sc.thin_grid_per_cu;
+    for (int j = 0; j < K; ++j)
+        for (int sub = 0; sub < 8; ++sub)
+        {
+            const int kind = VK_TP + (sub & 3);
+            const unsigned g = vc_sched_grid(ctx, s, kind, pc);
+            if (!g)
+                continue;
+            thin_pass<<<g, kBlock, 0, S>>>(vc_sched_work(s, kind), ctx->d_lut,
+                                            sub, j, s->n, s->d_ctl, s->d_gany,
+                                            s->d_vany, s->d_ctr);
+            VC_CHECK_LAUNCH(ctx);
+        }
+    seg_advance<<<1, 64, 0, S>>>(s->d_ctl, K);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+// a segment as a captured graph (captured once per length, then replayed)
+int run_segment(vc_ctx* ctx, vc_sched* s, int K)
+{
+    if (!s->sc.use_graph)
+        return launch_segment(ctx, s, K);
+    hipStream_t S = ctx->stream;
+    auto it = s->graphs.find(K);
+    if (it == s->graphs.end())
+    {
+        VC_HIP(ctx, hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_segment(ctx, s, K);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(S, &g);
+        if (rc != VC_OK)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return vc_fail(ctx, VC_ERR_HIP, "segment capture: %s",
+                           hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess)
+            return vc_fail(ctx, VC_ERR_HIP, "segment instantiate: %s",
+                           hipGetErrorString(e2));
+        it = s->graphs.emplace(K, ex).first;
+    }
+    VC_HIP(ctx, hipGraphLaunch(it->second, S));
+    return VC_OK;
+}
+
+int upload_levels(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev)
+{
+    const size_t nl = (size_t)s->n * nlev;
+    if (s->lv_n == nlev && s->lv_cache.size() == nl &&
+        std::memcmp(s->lv_cache.data(), levels, 4 * nl) == 0)
+        return VC_OK;
+    // the staging buffer may still feed an earlier copy
+    VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t cells = (size_t)s->n * VC_MAX_LEVELS;
+    uint64_t* hT = static_cast<uint64_t*>(s->h_lv);
+    float* hL = reinterpret_cast<float*>(hT + cells);
+    std::memset(s->h_lv, 0, 12 * cells);
+    for (int v = 0; v < s->n; ++v)
+        for (int q = 0; q < nlev; ++q)
+        {
+            const float l = levels[(size_t)v * nlev + q];
+            hT[(size_t)v * VC_MAX_LEVELS + q] = (uint64_t)vc_level_threshold(l);
+            hL[(size_t)v * VC_MAX_LEVELS + q] = l;
+        }
+    VC_HIP(ctx, hipMemcpyAsync(s->d_T, hT, 8 * cells, hipMemcpyHostToDevice,
+                               ctx->stream));
+    VC_HIP(ctx, hipMemcpyAsync(s->d_L, hL, 4 * cells, hipMemcpyHostToDevice,
+                               ctx->stream));
+    s->lv_cache.assign(levels, levels + nl);
+    s->lv_n = nlev;
+    return VC_OK;
 }
 
 } // namespace
@@ -478,147 +622,83 @@ int vc_lut_build(vc_ctx* ctx)
     return VC_OK;
 }
 
-int vc_thin_run(vc_ctx* ctx, vc_vol* v, const float* levels, int n,
-                vc_thin_stats* st)
+int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
+                  vc_thin_stats* st)
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
-    const size_t N = (size_t)v->N;
-    Plane P{v->nx, v->ny, v->nz, (v->nz + 63) / 64};
-    int lgw = 0;
-    while ((1 << lgw) < P.nwz)
-        lgw++;
-    const size_t nwords = (size_t)v->nx * v->ny * P.nwz;
-    int rc;
-    if (!v->label && (rc = vc_alloc(ctx, (void**)&v->label, N)) != VC_OK)
-        return rc;
-    if (!v->lvl && (rc = vc_alloc(ctx, (void**)&v->lvl, N)) != VC_OK)
-        return rc;
-    if (!v->alive &&
-        (rc = vc_alloc(ctx, (void**)&v->alive, 8 * nwords)) != VC_OK)
-        return rc;
-    if (!v->elig && (rc = vc_alloc(ctx, (void**)&v->elig, 8 * nwords)) != VC_OK)
-        return rc;
-    if (!v->dflag && (rc = vc_alloc(ctx, (void**)&v->dflag, 2 * nwords)) != VC_OK)
-        return rc;
-    const int kMaxSweeps = 1 << 16;
-    if (!v->sweep_any &&
-        (rc = vc_alloc(ctx, (void**)&v->sweep_any, 4 * (size_t)kMaxSweeps)) !=
-            VC_OK)
-        return rc;
-    const size_t ctr_bytes = 8 * (size_t)kSlots * kSlotWords;
-    if (!v->tstats &&
-        (rc = vc_alloc(ctx, (void**)&v->tstats, ctr_bytes)) != VC_OK)
-        return rc;
-    if (!v->hcnt)
-        VC_HIP(ctx, hipHostMalloc((void**)&v->hcnt, 4 * 64,
-                                  hipHostMallocDefault));
-    uint32_t* hsw = v->hcnt;
-
     hipStream_t S = ctx->stream;
-    hipError_t e = hipMemsetAsync(v->tstats, 0, ctr_bytes, S);
-    int64_t launches = 0;
-    LevelsU64 lt{};
-    LevelsF32 lf{};
-    for (int k = 0; k < n; ++k)
+    int rc = vc_sched_sync_measure(ctx, s);
+    if (rc == VC_OK)
+        rc = upload_levels(ctx, s, levels, nlev);
+    if (rc != VC_OK)
+        return rc;
+    const size_t ctr_bytes = 8 * (size_t)VC_CTR_SLOTS * VC_CTR_WORDS;
+    VC_HIP(ctx, hipMemsetAsync(s->d_ctr, 0, ctr_bytes, S));
+    const vc_schedule& sc = s->sc;
+    const int per_seg = pass_count(s) + 1;
+    int64_t launches = 0, passes_total = 0;
+    // every sweep before convergence removes a voxel
+    const int64_t guard = s->voxels + 2;
+    for (int k = 0; k < nlev; ++k)
     {
-        lt.T[k] = (uint64_t)vc_level_threshold(levels[k]);
-        lf.L[k] = levels[k];
-    }
-    const int nchunk = (v->nz + 511) / 512;
-    const int64_t init_threads = (int64_t)v->nx * v->ny * nchunk * 64;
-    const unsigned ginit = (unsigned)((init_threads + kBlock - 1) / kBlock);
-    int64_t passes_total = 0;
-    for (int k = 0; k < n && e == hipSuccess; ++k)
-    {
-        if (k > 0)
-            thin_level_init<2><<<ginit, kBlock, 0, S>>>(
-                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
-                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
-        else if (v->measure_kind == 2)
-            thin_level_init<1><<<ginit, kBlock, 0, S>>>(
-                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
-                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
-        else
-            thin_level_init<0><<<ginit, kBlock, 0, S>>>(
-                v->occ, v->lam2, v->meas_f, v->lvl, v->label, v->alive, v->elig,
-                v->dflag, v->dflag + nwords, P, nchunk, lt, lf, n, k);
-        launches++;
-        e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemsetAsync(v->sweep_any, 0, 4 * (size_t)kMaxSweeps, S);
-        // launch sweeps in batches; a sweep after the converged one is a
-        // no-op (every pass exits at its first instruction)
-        int sweep = 0;
-        int batch = 4;
-        while (e == hipSuccess)
+        const unsigned g = vc_sched_grid(ctx, s, VK_LI, sc.thin_grid_per_cu);
+        if (g)
         {
-            if (sweep + batch > kMaxSweeps)
+            thin_level_init<<<g, kBlock, 0, S>>>(
+                vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
+                s->d_ctl, s->d_gany, s->d_vany);
+            VC_CHECK_LAUNCH(ctx);
+            launches++;
+        }
+        int64_t base = 0;
+        int K = sc.sweeps_per_sync ? sc.sweeps_per_sync : VC_SEG_MIN;
+        for (;;)
+        {
+            if (base > guard)
                 return vc_fail(ctx, VC_ERR_GUARD,
-                               "thinning did not converge in %d sweeps", sweep);
-            for (int j = 0; j < batch && e == hipSuccess; ++j, ++sweep)
-                for (int s = 0; s < 8 && e == hipSuccess; ++s)
+                               "thinning did not converge in %lld sweeps",
+                               (long long)base);
+            rc = run_segment(ctx, s, K);
+            if (rc != VC_OK)
+                return rc;
+            launches += (int64_t)K * (per_seg - 1) + 1;
+            VC_HIP(ctx, hipMemcpyAsync(s->h_ring, s->d_gany, 4 * VC_RING,
+                                       hipMemcpyDeviceToHost, S));
+            VC_HIP(ctx, hipStreamSynchronize(S));
+            int64_t conv = -1;
+            for (int jj = 0; jj < K; ++jj)
+                if (s->h_ring[(base + jj) & (VC_RING - 1)] == 0)
                 {
-                    const int sx = s & 1, sy = (s >> 1) & 1;
-                    const int nxh = (v->nx - sx + 1) >> 1;
-                    const int nyh = (v->ny - sy + 1) >> 1;
-                    if (nxh == 0 || nyh == 0)
-                        continue;
-                    const int64_t per_row = (int64_t)nyh << lgw;
-                    const dim3 g((unsigned)((per_row + kBlock - 1) / kBlock),
-                                 (unsigned)nxh);
-                    thin_pass_dense<<<g, kBlock, 0, S>>>(
-                        v->alive, v->elig, v->label, v->dflag,
-                        v->dflag + nwords, ctx->d_lut, P, lgw, s, k, sweep,
-                        v->sweep_any, (unsigned long long*)v->tstats);
-                    launches++;
-                    e = hipGetLastError();
-                }
-            if (e != hipSuccess)
-                break;
-            e = hipMemcpyAsync(hsw, v->sweep_any + sweep - batch,
-                               4 * (size_t)batch, hipMemcpyDeviceToHost, S);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(S);
-            if (e != hipSuccess)
-                break;
-            int conv = -1;
-            for (int j = 0; j < batch; ++j)
-                if (hsw[j] == 0)
-                {
-                    conv = sweep - batch + j;
+                    conv = base + jj;
                     break;
                 }
+            base += K;
             if (conv >= 0)
             {
-                passes_total += 8LL * (conv + 1);
+                passes_total += 8 * (conv + 1);
                 break;
             }
-            batch = batch < 16 ? batch * 2 : 16;
+            if (!sc.sweeps_per_sync)
+                K = K * 2 < VC_SEG_MAX ? K * 2 : VC_SEG_MAX;
         }
     }
-    if (e == hipSuccess && st)
+    if (st)
     {
-        std::vector<unsigned long long> hs((size_t)kSlots * kSlotWords);
-        e = hipMemcpyAsync(hs.data(), v->tstats, ctr_bytes,
-                           hipMemcpyDeviceToHost, S);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(S);
-        if (e == hipSuccess)
+        VC_HIP(ctx, hipMemcpyAsync(s->h_ctr, s->d_ctr, ctr_bytes,
+                                   hipMemcpyDeviceToHost, S));
+        VC_HIP(ctx, hipStreamSynchronize(S));
+        std::memset(st, 0, sizeof(*st));
+        for (int sl = 0; sl < VC_CTR_SLOTS; ++sl)
         {
-            std::memset(st, 0, sizeof(*st));
-            for (int sl = 0; sl < kSlots; ++sl)
-            {
-                const unsigned long long* c = &hs[(size_t)sl * kSlotWords];
-                st->tests += (int64_t)c[0];
-                for (int k = 0; k < n; ++k)
-                    st->removed[k] += (int64_t)c[8 + k];
-            }
-            st->passes = passes_total;
-            st->launches = launches;
+            const unsigned long long* c = s->h_ctr + (size_t)sl * VC_CTR_WORDS;
+            st->tests += (int64_t)c[0];
+            for (int q = 0; q < nlev; ++q)
+                st->removed[q] += (int64_t)c[8 + q];
         }
+        st->passes = passes_total;
+        st->launches = launches;
     }
-    if (e != hipSuccess)
-        return vc_fail(ctx, VC_ERR_HIP, "thinning: %s", hipGetErrorString(e));
-    v->thin_done = true;
+    for (vc_vol* v : s->vols)
+        v->thin_done = true;
     return VC_OK;
 }

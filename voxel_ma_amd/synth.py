@@ -163,33 +163,64 @@ def _rotation(rng):
                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
 
 
-def batch_volume(L, rng):
-    occ = np.zeros((L, L, L), np.uint8)
-    g = np.indices((L, L, L)).reshape(3, -1).T.astype(np.float64)
-    for _ in range(int(rng.integers(4, 17))):
+SOLID_WORDS = 16  # VC_SOLID_WORDS (include/vc_synth.h)
+
+
+def solid_specs(L, rng):
+    """Primitive records (k, 16) of one batched-solids volume of edge L: 4-16
+    primitives, each an ellipsoid (semi-axes U[4,30]) or a capsule (radius and
+    half-length U[4,30]) with probability 1/2, uniform random rotation, centre
+    uniform in [0, L-1]^3.  Layout as vc_synth_solids."""
+    k = int(rng.integers(4, 17))
+    out = np.zeros((k, SOLID_WORDS), np.float64)
+    for i in range(k):
         c = rng.uniform(0, L - 1, size=3)
         Rm = _rotation(rng)
-        q = (g - c) @ Rm  # coordinates in the primitive frame
+        out[i, 1:4] = c
+        out[i, 4:13] = Rm.ravel()
         if rng.random() < 0.5:
-            ax = rng.uniform(4, 30, size=3)
-            inside = ((q / ax) ** 2).sum(axis=1) <= 1.0
+            out[i, 0] = 0.0
+            out[i, 13:16] = rng.uniform(4, 30, size=3)
         else:
-            r = rng.uniform(4, 30)
-            h = rng.uniform(4, 30)
-            t = np.clip(q[:, 0], -h, h)
-            d2 = (q[:, 0] - t) ** 2 + q[:, 1] ** 2 + q[:, 2] ** 2
-            inside = d2 <= r * r
-        occ |= inside.reshape(L, L, L).astype(np.uint8)
+            out[i, 0] = 1.0
+            out[i, 13] = rng.uniform(4, 30)
+            out[i, 14] = rng.uniform(4, 30)
+    return out
+
+
+def solids_host(L, prims):
+    """numpy restatement of vc_synth_solids for a cube of edge L: the same
+    IEEE double operations in the same order (elementwise, no matmul)."""
+    g = np.indices((L, L, L)).astype(np.float64)
+    occ = np.zeros((L, L, L), bool)
+    for r in prims:
+        dx, dy, dz = g[0] - r[1], g[1] - r[2], g[2] - r[3]
+        q = [(dx * r[4 + j] + dy * r[7 + j]) + dz * r[10 + j] for j in range(3)]
+        if r[0] == 0.0:
+            t0, t1, t2 = q[0] / r[13], q[1] / r[14], q[2] / r[15]
+            occ |= ((t0 * t0 + t1 * t1) + t2 * t2) <= 1.0
+        else:
+            h = r[14]
+            e = q[0] - np.clip(q[0], -h, h)
+            occ |= ((e * e + q[1] * q[1]) + q[2] * q[2]) <= r[13] * r[13]
+    occ = occ.astype(np.uint8)
     occ[0] = occ[-1] = 0
     occ[:, 0] = occ[:, -1] = 0
     occ[:, :, 0] = occ[:, :, -1] = 0
     return occ
 
 
-def batch_c4(count=256, seed=4, lmin=32, ratio=6.0):
+def batch_c4_specs(count=256, seed=4, lmin=32, ratio=6.0):
+    """(L, primitive records) of each volume of the batched-solids workload:
+    L = round(lmin * ratio^u), u ~ U(0,1)."""
     rng = np.random.default_rng(seed)
     sizes = [int(round(lmin * ratio ** rng.random())) for _ in range(count)]
-    return [batch_volume(L, rng) for L in sizes]
+    return [(L, solid_specs(L, rng)) for L in sizes]
+
+
+def batch_c4(count=256, seed=4, lmin=32, ratio=6.0):
+    """Host-generated volumes of batch_c4_specs (small counts in tests)."""
+    return [solids_host(L, p) for L, p in batch_c4_specs(count, seed, lmin, ratio)]
 
 
 DEFAULT_TT = (0.01, 0.02, 0.04)
