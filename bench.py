@@ -5,7 +5,12 @@ A step = vc_edt + vc_measure + vc_thin_levels over the whole volume, inputs
 resident in HBM.  Prints one JSON line (rank 0).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
-                    [--no-cpu-baseline] [--profile-steps]
+                    [--no-cpu-baseline] [--no-batch] [--batch-only]
+                    [--profile-steps]
+
+The JSON line also carries "batch": the batched-solids workload (256 cubic
+volumes of edge 32-192, one launch per pass over all of them, autotuned
+schedule), timed the same way.
 """
 import argparse
 import ctypes
@@ -94,6 +99,76 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
                       "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el)}
 
 
+def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
+    """The batched-solids workload: `count` cubic volumes, edge
+    round(32 * 6^u), unions of 4-16 capsules/ellipsoids, generated on the
+    device; per-volume levels from each volume's inside size.  Autotunes the
+    schedule, then times whole-batch EDT + measure + thinning steps."""
+    import numpy as np
+    import voxel_ma_amd
+    from voxel_ma_amd import synth
+
+    specs = synth.batch_c4_specs(count=count, seed=seed)
+    vols, levels = [], []
+    h = hashlib.sha256()
+    for L, prims in specs:
+        v = ctx.solids_volume((L, L, L), prims)
+        occ = v.get(voxel_ma_amd.BUF_OCC)
+        h.update(occ.tobytes())
+        levels.append(synth.levels_from_tt(synth.inside_size(occ)))
+        vols.append(v)
+    levels = np.array(levels, np.float32)
+    b = voxel_ma_amd.Batch(ctx, vols)
+    n, voxels = b.size()
+    print("[bench] batch of %d volumes, %d voxels: autotune" % (n, voxels), file=sys.stderr,
+          flush=True)
+    b.edt().measure()
+    t0 = time.perf_counter()
+    sc, tt = b.tune(levels, reps=2)
+    tune_s = time.perf_counter() - t0
+    stream = ctx.stream()
+    for _ in range(warmup):
+        b.run(levels, stats=False)
+    ctx.synchronize()
+    evs = [[ev.create() for _ in range(4)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    st = None
+    for k in range(steps):
+        ev.record(evs[k][0], stream)
+        b.edt()
+        ev.record(evs[k][1], stream)
+        b.measure()
+        ev.record(evs[k][2], stream)
+        st = b.thin(levels, stats=(k == steps - 1))
+        ev.record(evs[k][3], stream)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    edt_ms = sum(ev.ms(e[0], e[1]) for e in evs) / steps
+    meas_ms = sum(ev.ms(e[1], e[2]) for e in evs) / steps
+    thin_ms = sum(ev.ms(e[2], e[3]) for e in evs) / steps
+    ms_step = wall * 1e3 / steps
+    out = {
+        "workload": "%d cubic volumes, edge round(32*6^u) in [%d, %d], unions of 4-16 "
+                    "capsules/ellipsoids (seed %d), per-volume -tt levels 0.01/0.02/0.04"
+                    % (n, min(L for L, _ in specs), max(L for L, _ in specs), seed),
+        "volumes": n, "voxels": int(voxels), "volumes_sha256_16": h.hexdigest()[:16],
+        "value": round(voxels / (ms_step * 1e-3) / 1e6, 2), "unit": "Mvoxels/s",
+        "ms_per_step": round(ms_step, 3),
+        "stages_ms": {"edt": round(edt_ms, 3), "measure": round(meas_ms, 3),
+                      "thin": round(thin_ms, 3)},
+        "edt_achieved_gbs": round(EDT_BYTES_PER_VOXEL * voxels / (edt_ms * 1e-3) / 1e9, 1),
+        "thin_stats": {"removed": [int(x) for x in st.removed[:levels.shape[1]]],
+                       "passes_with_work": int(st.passes), "tests": int(st.tests),
+                       "launches": int(st.launches)},
+        "schedule": sc.as_dict(), "tune_s": round(tune_s, 2),
+        "tuned_ms": {"edt_measure": round(tt[0], 3), "thin": round(tt[1], 3)},
+    }
+    b.release()
+    for v in vols:
+        v.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +176,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-solids leg (256 volumes)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched-solids leg (for profiling)")
     ap.add_argument("--profile-steps", action="store_true",
                     help="no CPU baseline, no hashing (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -127,9 +206,13 @@ def main():
     from voxel_ma_amd import synth
 
     n = a.size
+    ctx = voxel_ma_amd.Context(0)
+    if a.batch_only:
+        print(json.dumps({"batch": batch_leg(ctx, HipEvents(), a.steps, a.warmup)}), flush=True)
+        ctx.close()
+        return
     print("[bench] generating blurred-noise volume %d^3 on the device" % n, file=sys.stderr,
           flush=True)
-    ctx = voxel_ma_amd.Context(0)
     occ_dev, _thr = ctx.synth_blurred_noise((n, n, n), sigma=12.0, fill=0.30, seed=3)
     occ_host = occ_dev.download(np.uint8, (n, n, n))
     fill = float(occ_host.mean())
@@ -220,10 +303,12 @@ def main():
                                                       ("edt_pass_z", "edt_pass_y",
                                                        "edt_pass_x")))) or None},
     }
+    vol.release()
+    if not (a.no_batch or a.profile_steps):
+        out["batch"] = batch_leg(ctx, ev, a.steps, a.warmup)
     if not (a.no_cpu_baseline or a.profile_steps):
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(occ_host, levels)
-    vol.release()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -32,6 +32,7 @@ constexpr int kInf = 0x7FFFFFFF;
 
 // rows along z: wavefront = row; the row's background bitmask (one ballot
 // per 64 voxels) sits in the wavefront's LDS slice of `stride` words
+template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     edt_pass_z(vc_work W, int stride)
 {
@@ -43,8 +44,8 @@ __global__ void __launch_bounds__(kBlock)
     {
         int vi;
         uint32_t lt;
-        vc_item(W, b, vi, lt);
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const int nz = d.nz;
         const int64_t row = (int64_t)lt * kRowsZ + wv;
         const bool active = row < (int64_t)d.nx * d.ny;
@@ -115,6 +116,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // columns along y: item = TZ consecutive z at fixed x, all y.  z' comes from
 // the first half of the edt buffer, (y', z') goes to the ft buffer.
+template <bool kB>
 __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 {
     extern __shared__ int16_t colz[]; // [ny][TZ]
@@ -126,8 +128,8 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
             continue;
         int vi;
         uint32_t T;
-        vc_item(W, b, vi, T);
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, T);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const int ny = d.ny, nz = d.nz, TZ = d.tzy;
         const int nzt = (nz + TZ - 1) / TZ;
         const int z0 = (int)(T % nzt) * TZ;
@@ -221,6 +223,7 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 // columns along x: item = TZ consecutive z at fixed y, all x.  Rewrites ft in
 // place (the workgroup stages exactly the elements it later overwrites) and
 // overwrites edt once every z' of the volume has been consumed by pass y.
+template <bool kB>
 __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
 {
     extern __shared__ uint32_t colyz[]; // [nx][TZ]
@@ -232,8 +235,8 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
             continue;
         int vi;
         uint32_t T;
-        vc_item(W, b, vi, T);
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, T);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const int nx = d.nx, ny = d.ny, nz = d.nz, TZ = d.tzx;
         const int nzt = (nz + TZ - 1) / TZ;
         const int z0 = (int)(T % nzt) * TZ;
@@ -337,6 +340,7 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
 // occupancy is read.
 constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
 
+template <bool kB>
 __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
 {
     __shared__ uint32_t S[kMeasY + 2][kMeasZ + 2];
@@ -366,8 +370,8 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
             continue;
         int vi;
         uint32_t T;
-        vc_item(W, b, vi, T);
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, T);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const int nx = d.nx, ny = d.ny, nz = d.nz;
         const vc_pack pk = d.pk;
         const uint32_t* ft = d.ft;
@@ -461,22 +465,34 @@ int vc_edt_sched(vc_ctx* ctx, vc_sched* s)
     unsigned g = vc_sched_grid(ctx, s, VK_EZ, pc);
     if (g)
     {
-        edt_pass_z<<<g, kBlock, s->lds[VK_EZ], ctx->stream>>>(
-            vc_sched_work(s, VK_EZ), s->ez_stride);
+        if (s->n > 1)
+            edt_pass_z<true><<<g, kBlock, s->lds[VK_EZ], ctx->stream>>>(
+                vc_sched_work(s, VK_EZ), s->ez_stride);
+        else
+            edt_pass_z<false><<<g, kBlock, s->lds[VK_EZ], ctx->stream>>>(
+                vc_sched_work(s, VK_EZ), s->ez_stride);
         VC_CHECK_LAUNCH(ctx);
     }
     g = vc_sched_grid(ctx, s, VK_EY, pc);
     if (g)
     {
-        edt_pass_y<<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
-            vc_sched_work(s, VK_EY));
+        if (s->n > 1)
+            edt_pass_y<true><<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
+                vc_sched_work(s, VK_EY));
+        else
+            edt_pass_y<false><<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
+                vc_sched_work(s, VK_EY));
         VC_CHECK_LAUNCH(ctx);
     }
     g = vc_sched_grid(ctx, s, VK_EX, pc);
     if (g)
     {
-        edt_pass_x<<<g, kBlock, s->lds[VK_EX], ctx->stream>>>(
-            vc_sched_work(s, VK_EX));
+        if (s->n > 1)
+            edt_pass_x<true><<<g, kBlock, s->lds[VK_EX], ctx->stream>>>(
+                vc_sched_work(s, VK_EX));
+        else
+            edt_pass_x<false><<<g, kBlock, s->lds[VK_EX], ctx->stream>>>(
+                vc_sched_work(s, VK_EX));
         VC_CHECK_LAUNCH(ctx);
     }
     for (vc_vol* v : s->vols)
@@ -490,8 +506,12 @@ int vc_measure_sched(vc_ctx* ctx, vc_sched* s)
     const unsigned g = vc_sched_grid(ctx, s, VK_MS, s->sc.edt_grid_per_cu);
     if (g)
     {
-        measure_lam2<<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
-            vc_sched_work(s, VK_MS));
+        if (s->n > 1)
+            measure_lam2<true><<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+                vc_sched_work(s, VK_MS));
+        else
+            measure_lam2<false><<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+                vc_sched_work(s, VK_MS));
         VC_CHECK_LAUNCH(ctx);
     }
     for (vc_vol* v : s->vols)

@@ -125,6 +125,8 @@ struct vc_work
     const vc_vdesc* D;
     const uint64_t* items; // volume << 32 | local item; null for one volume
     uint32_t total;
+    vc_vdesc one; // the volume itself when items is null (kernel argument:
+                  // no dependent load before the first data access)
 };
 
 // sweeps per segment when vc_schedule.sweeps_per_sync is 0 (adaptive)
@@ -209,13 +211,38 @@ __host__ __device__ inline void vc_unpack_xyz(const vc_pack& p, uint32_t v,
 }
 
 // ---------------------------------------------------------------- work items
+// Kernels are instantiated twice: kB = false for one volume (the view is the
+// kernel-argument copy W.one, no table) and kB = true for a batch (table and
+// views read through the constant address space, so that the uniform loads
+// are scalar and the fields land in scalar registers).
+typedef const __attribute__((address_space(4))) uint32_t* vc_cu32;
+typedef const __attribute__((address_space(4))) uint64_t* vc_citems;
+static_assert(sizeof(vc_vdesc) % 4 == 0, "vc_vdesc is copied as words");
+
+template <bool kB>
+__device__ __forceinline__ vc_vdesc vc_desc(const vc_work& W, int v)
+{
+    if (kB)
+    {
+        vc_vdesc out;
+        const vc_cu32 src = (vc_cu32)(const void*)(W.D + v);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(vc_vdesc) / 4); ++i)
+            dst[i] = src[i];
+        return out;
+    }
+    return W.one;
+}
+
 // item b of a work table -> (volume, local tile)
+template <bool kB>
 __device__ __forceinline__ void vc_item(const vc_work& W, uint32_t b, int& v,
                                         uint32_t& lt)
 {
-    if (W.items)
+    if (kB)
     {
-        const uint64_t e = W.items[b];
+        const uint64_t e = ((vc_citems)W.items)[b];
         v = (int)(e >> 32);
         lt = (uint32_t)e;
     }

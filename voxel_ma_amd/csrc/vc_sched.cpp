@@ -57,10 +57,14 @@ bool xcd_kind(int kind)
     return kind == VK_EY || kind == VK_EX || kind == VK_MS;
 }
 
+void drop_graphs(vc_sched* s);
+
+// captured launches hold the views by value: every change re-captures
 int upload_desc(vc_ctx* ctx, vc_sched* s)
 {
     // the staging buffer may still feed an earlier copy
     VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(s);
     std::memcpy(s->h_stage, s->h_desc.data(), sizeof(vc_vdesc) * s->n);
     VC_HIP(ctx, hipMemcpyAsync(s->d_desc, s->h_stage,
                                sizeof(vc_vdesc) * s->n,
@@ -305,6 +309,10 @@ vc_work vc_sched_work(const vc_sched* s, int kind)
     w.D = s->d_desc;
     w.items = s->d_items[kind];
     w.total = s->total[kind];
+    if (s->n == 1)
+        w.one = s->h_desc[0];
+    else
+        std::memset(&w.one, 0, sizeof(w.one));
     return w;
 }
 

@@ -144,6 +144,7 @@ __device__ __forceinline__ int64_t widx(const Plane& P, int x, int y, int w)
 // level's sweep control (sweep counter, level, "removed anything" rings).
 constexpr int kLiWaves = 4;
 
+template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     thin_level_init(vc_work W, const uint64_t* __restrict__ Tq,
                     const float* __restrict__ Lq, int nlev, int k, int nvol,
@@ -169,8 +170,8 @@ __global__ void __launch_bounds__(kBlock)
     {
         int vi;
         uint32_t lt;
-        vc_item(W, b, vi, lt);
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const int nz = d.nz;
         const int nchunk = (nz + 511) >> 9;
         const int64_t wave = (int64_t)lt * kLiWaves + wv;
@@ -180,53 +181,73 @@ __global__ void __launch_bounds__(kBlock)
         const int z0 = (int)(wave % nchunk) * 512 + 8 * lane;
         const int64_t base = rowi * nz;
         const int mode = k > 0 ? 2 : (d.mkind == 2 ? 1 : 0);
-        const uint64_t* T = Tq + (size_t)vi * VC_MAX_LEVELS;
-        const float* Lf = Lq + (size_t)vi * VC_MAX_LEVELS;
         uint32_t am = 0, em = 0, nm = 0;
         uint8_t lv8[8], lb8[8];
+        if (mode == 2)
+        {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+                const int64_t i = base + z0 + j;
+                const bool in = z0 + j < nz;
+                lv8[j] = in ? d.lvl[i] : (uint8_t)255;
+                lb8[j] = in && d.label[i] == 1;
+            }
+        }
+        else
+        {
+            // measure of the inside voxels, then the first level whose
+            // threshold exceeds it (thresholds ascending; q descending so the
+            // smallest such q is kept); thresholds are uniform scalar loads
+            uint64_t mi[8];
+            float mf[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+                const int64_t i = base + z0 + j;
+                const bool in = z0 + j < nz && d.occ[i] != 0;
+                lb8[j] = in;
+                lv8[j] = 255;
+                mi[j] = 0;
+                mf[j] = 0.0f;
+                if (in)
+                {
+                    if (mode == 0)
+                        mi[j] = d.lam2[i];
+                    else
+                        mf[j] = d.meas[i];
+                }
+            }
+            typedef const __attribute__((address_space(4))) uint64_t* cu64;
+            typedef const __attribute__((address_space(4))) float* cf32;
+            const cu64 T = (cu64)(Tq + (size_t)vi * VC_MAX_LEVELS);
+            const cf32 Lf = (cf32)(Lq + (size_t)vi * VC_MAX_LEVELS);
+            for (int q = nlev - 1; q >= 0; --q)
+            {
+                if (mode == 0)
+                {
+                    const uint64_t t = T[q];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (lb8[j] && mi[j] < t)
+                            lv8[j] = (uint8_t)q;
+                }
+                else
+                {
+                    const float t = Lf[q];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (lb8[j] && mf[j] < t)
+                            lv8[j] = (uint8_t)q;
+                }
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j)
         {
-            const int z = z0 + j;
-            uint8_t l = 255, lb = 0;
-            if (z < nz)
-            {
-                const int64_t i = base + z;
-                if (mode == 2)
-                {
-                    l = d.lvl[i];
-                    lb = d.label[i] == 1;
-                }
-                else if (d.occ[i])
-                {
-                    lb = 1;
-                    if (mode == 0)
-                    {
-                        const uint64_t m = d.lam2[i];
-                        for (int q = 0; q < nlev; ++q)
-                            if (m < T[q])
-                            {
-                                l = (uint8_t)q;
-                                break;
-                            }
-                    }
-                    else
-                    {
-                        const float m = d.meas[i];
-                        for (int q = 0; q < nlev; ++q)
-                            if (m < Lf[q])
-                            {
-                                l = (uint8_t)q;
-                                break;
-                            }
-                    }
-                }
-            }
-            lv8[j] = l;
-            lb8[j] = lb;
-            am |= (uint32_t)lb << j;
-            em |= (uint32_t)(l <= k) << j;
-            nm |= (uint32_t)(l == k) << j;
+            am |= (uint32_t)lb8[j] << j;
+            em |= (uint32_t)(lv8[j] <= k) << j;
+            nm |= (uint32_t)(lv8[j] == k) << j;
         }
         if (mode != 2)
         {
@@ -330,13 +351,16 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
 // are rings: vany[4v + (sweep & 3)] per volume, gany[sweep % VC_RING] for the
 // batch; the first pass of a sweep clears the entries of sweep + 1 (vany) and
 // sweep + VC_RING/2 (gany), which no pass of this sweep reads or writes.
+template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     thin_pass(vc_work W, const uint32_t* __restrict__ lut, int s, int j,
-              int nvol, const uint32_t* __restrict__ ctl, uint32_t* gany,
-              uint32_t* vany, unsigned long long* __restrict__ ctr)
+              int k_arg, int nvol, const uint32_t* __restrict__ ctl,
+              uint32_t* gany, uint32_t* vany,
+              unsigned long long* __restrict__ ctr)
 {
-    const uint32_t sweep = ctl[0] + (uint32_t)j;
-    const int k = (int)ctl[1];
+    // captured: sweep and level from the device; launched: from the arguments
+    const uint32_t sweep = ctl ? ctl[0] + (uint32_t)j : (uint32_t)j;
+    const int k = ctl ? (int)ctl[1] : k_arg;
     __shared__ uint32_t red[kBlock / 64];
     if (s == 0)
     {
@@ -353,10 +377,8 @@ __global__ void __launch_bounds__(kBlock)
     {
         int vi;
         uint32_t lt;
-        vc_item(W, b, vi, lt);
-        if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
-            continue; // converged (block-uniform)
-        const vc_vdesc& d = W.D[vi];
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
         const Plane P{d.nx, d.ny, d.nz, d.nwz};
         uint64_t* A = d.alive;
         const uint64_t* E = d.elig;
@@ -374,11 +396,17 @@ __global__ void __launch_bounds__(kBlock)
         const int y = 2 * yh + sy;
         uint64_t removed = 0;
         uint32_t ntest = 0;
-        if ((int)xh < nxh)
+        const bool valid = (int)xh < nxh;
+        const int64_t o = widx(P, x, y, w);
+        // the flag load is issued before the convergence check so that the
+        // two latencies overlap
+        const uint8_t fl = valid ? Fs[o] : (uint8_t)0;
+        if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
+            continue; // converged at this level (block-uniform)
+        if (valid)
         {
-            const int64_t o = widx(P, x, y, w);
             uint64_t C = 0, cand = 0;
-            if (Fs[o])
+            if (fl)
             {
                 Fs[o] = 0;
                 C = A[o];
@@ -526,8 +554,12 @@ int pass_count(const vc_sched* s)
     return c;
 }
 
-// K sweeps (8 subfield passes each) and the segment advance
-int launch_segment(vc_ctx* ctx, vc_sched* s, int K)
+// K sweeps (8 subfield passes each).  Captured (graph): sweep j of the
+// segment reads the segment's first sweep and the level from ctl, and the
+// segment ends by advancing ctl[0]; launched directly: the absolute sweep
+// (first + j) and the level are arguments.
+int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
+                   int k)
 {
     hipStream_t S = ctx->stream;
     const int pc = s->sc.thin_grid_per_cu;
@@ -538,27 +570,37 @@ int launch_segment(vc_ctx* ctx, vc_sched* s, int K)
             const unsigned g = vc_sched_grid(ctx, s, kind, pc);
             if (!g)
                 continue;
-            thin_pass<<<g, kBlock, 0, S>>>(vc_sched_work(s, kind), ctx->d_lut,
-                                            sub, j, s->n, s->d_ctl, s->d_gany,
-                                            s->d_vany, s->d_ctr);
+            if (s->n > 1)
+                thin_pass<true><<<g, kBlock, 0, S>>>(
+                    vc_sched_work(s, kind), ctx->d_lut, sub,
+                    graph ? j : (int)(first + j), k, s->n,
+                    graph ? s->d_ctl : nullptr, s->d_gany, s->d_vany, s->d_ctr);
+            else
+                thin_pass<false><<<g, kBlock, 0, S>>>(
+                    vc_sched_work(s, kind), ctx->d_lut, sub,
+                    graph ? j : (int)(first + j), k, s->n,
+                    graph ? s->d_ctl : nullptr, s->d_gany, s->d_vany, s->d_ctr);
             VC_CHECK_LAUNCH(ctx);
         }
-    seg_advance<<<1, 64, 0, S>>>(s->d_ctl, K);
-    VC_CHECK_LAUNCH(ctx);
+    if (graph)
+    {
+        seg_advance<<<1, 64, 0, S>>>(s->d_ctl, K);
+        VC_CHECK_LAUNCH(ctx);
+    }
     return VC_OK;
 }
 
 // a segment as a captured graph (captured once per length, then replayed)
-int run_segment(vc_ctx* ctx, vc_sched* s, int K)
+int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k)
 {
     if (!s->sc.use_graph)
-        return launch_segment(ctx, s, K);
+        return launch_segment(ctx, s, K, false, first, k);
     hipStream_t S = ctx->stream;
     auto it = s->graphs.find(K);
     if (it == s->graphs.end())
     {
         VC_HIP(ctx, hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
-        const int rc = launch_segment(ctx, s, K);
+        const int rc = launch_segment(ctx, s, K, true, 0, 0);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(S, &g);
         if (rc != VC_OK)
@@ -637,16 +679,24 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
     const vc_schedule& sc = s->sc;
     const int per_seg = pass_count(s) + 1;
     int64_t launches = 0, passes_total = 0;
-    // every sweep before convergence removes a voxel
-    const int64_t guard = s->voxels + 2;
+    // every sweep before convergence removes a voxel; sweep indices are int
+    // kernel arguments
+    const int64_t guard =
+        s->voxels + 2 < (int64_t)0x7FFFFFFF - 64 ? s->voxels + 2
+                                                 : (int64_t)0x7FFFFFFF - 64;
     for (int k = 0; k < nlev; ++k)
     {
         const unsigned g = vc_sched_grid(ctx, s, VK_LI, sc.thin_grid_per_cu);
         if (g)
         {
-            thin_level_init<<<g, kBlock, 0, S>>>(
-                vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
-                s->d_ctl, s->d_gany, s->d_vany);
+            if (s->n > 1)
+                thin_level_init<true><<<g, kBlock, 0, S>>>(
+                    vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
+                    s->d_ctl, s->d_gany, s->d_vany);
+            else
+                thin_level_init<false><<<g, kBlock, 0, S>>>(
+                    vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
+                    s->d_ctl, s->d_gany, s->d_vany);
             VC_CHECK_LAUNCH(ctx);
             launches++;
         }
@@ -658,10 +708,10 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
                 return vc_fail(ctx, VC_ERR_GUARD,
                                "thinning did not converge in %lld sweeps",
                                (long long)base);
-            rc = run_segment(ctx, s, K);
+            rc = run_segment(ctx, s, K, base, k);
             if (rc != VC_OK)
                 return rc;
-            launches += (int64_t)K * (per_seg - 1) + 1;
+            launches += (int64_t)K * (per_seg - 1) + (sc.use_graph ? 1 : 0);
             VC_HIP(ctx, hipMemcpyAsync(s->h_ring, s->d_gany, 4 * VC_RING,
                                        hipMemcpyDeviceToHost, S));
             VC_HIP(ctx, hipStreamSynchronize(S));
