@@ -1,0 +1,171 @@
+"""GPU checks of the measure surrogate (include/vc_surrogate.h) against the
+float64 restatement oracle/unet_ref.py: initial weights bit-exact, every
+layer's activations and every layer's weight gradient within BF16
+tolerance, training lowers the loss, tiled inference equals whole-volume
+inference bit for bit, and the gated measure drives the exact thinning
+(labels equal the oracle's thinning under the same measure)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import voxel_ma_amd as V
+from conftest import ROOT, say
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import unet_ref as U  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _occ(n, seed):
+    from voxel_ma_amd import synth
+    rng = np.random.default_rng(seed)
+    while True:
+        occ = synth.solids_host(n, synth.training_specs(n, rng))
+        if occ.sum() > 50:
+            return occ
+
+
+def _theta(seed):
+    """Initial weights plus small bias noise (keeps ReLUs alive on tiny
+    volumes)."""
+    th = U.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    o = 0
+    for _, ci, co in U.LAYERS:
+        o += 27 * ci * co
+        th[o:o + co] = rng.uniform(0.0, 0.05, size=co)
+        o += co
+    return th.astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def small(vc):
+    occ = _occ(24, 12)
+    vol = vc.upload(occ)
+    vol.edt().measure()
+    edt, lam2 = vol.get(V.BUF_EDT), vol.get(V.BUF_LAM2)
+    yield occ, edt, lam2, vol
+    vol.release()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_initial_weights_match_oracle(vc):
+    net = V.Surrogate(vc, seed=3)
+    assert np.array_equal(net.params(), U.init_params(3))
+    net.release()
+
+
+def test_forward_layers_match_fp64(vc, small):
+    occ, edt, lam2, vol = small
+    th = _theta(4)
+    net = V.Surrogate(vc, seed=4)
+    net.set_params(th)
+    loss, _ = net.gradients(vol)
+    out_ref, acts, _ = U.forward(th.astype(np.float64), U.inputs(occ, edt), keep=True)
+    for l, (name, _, co) in enumerate(U.LAYERS):
+        ref = acts[name][1]
+        got = net.activation(l, ref.shape)
+        say("forward", name, _rel(got, ref))
+        assert _rel(got, ref) < 0.03, name
+    got = net.activation(14, out_ref.shape)
+    assert _rel(got, out_ref) < 0.03
+    L_ref, _ = U.loss_and_dout(out_ref, occ, lam2)
+    assert abs(loss - L_ref) <= 0.05 * abs(L_ref)
+    net.release()
+
+
+def test_gradients_match_fp64(vc, small):
+    """Per-layer weight and bias gradients of the BF16 MFMA backward against
+    the float64 backward.  The gradient tensors are BF16 (8-bit mantissa),
+    so the error grows with the number of layers the gradient crosses: the
+    bound is a relative L2 error below 0.2 and a cosine above 0.98 for every
+    block (a wrong index, sign or transposition gives cosines far below)."""
+    occ, edt, lam2, vol = small
+    th = _theta(5)
+    net = V.Surrogate(vc, seed=5)
+    net.set_params(th)
+    _, g = net.gradients(vol)
+    _, g_ref = U.gradients(th.astype(np.float64), occ, edt, lam2)
+    o = 0
+    bad = []
+    for name, ci, co in U.LAYERS + [("head", 16, 2)]:
+        nw = (1 if name == "head" else 27) * ci * co
+        for part, sl in (("W", slice(o, o + nw)), ("b", slice(o + nw, o + nw + co))):
+            a, b = g[sl].astype(np.float64), g_ref[sl]
+            cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+            say("grad", name, part, "rel", round(_rel(a, b), 4), "cos", round(cos, 5))
+            if not (_rel(a, b) < 0.2 and cos > 0.98):
+                bad.append((name, part, _rel(a, b), cos))
+        o += nw + co
+    assert not bad, bad
+    net.release()
+
+
+def test_training_lowers_loss(vc):
+    from voxel_ma_amd import surrogate
+    net = V.Surrogate(vc, seed=6)
+    rng = np.random.default_rng(6)
+    vols = [surrogate.make_volume(vc, 64, rng) for _ in range(2)]
+    losses = [net.train_step(vols, 1e-3) for _ in range(40)]
+    say("train", losses[0], losses[-1])
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.7 * losses[0]
+    for v in vols:
+        v.release()
+    net.release()
+
+
+def test_tiled_inference_equals_whole(vc):
+    occ = np.zeros((80, 72, 96), np.uint8)
+    occ[:, :, :] = _occ(96, 13)[:80, :72, :]
+    occ[0] = occ[-1] = 0
+    vol = vc.upload(occ)
+    vol.edt()
+    net = V.Surrogate(vc, seed=7)
+    net.set_params(_theta(7))
+    lam_w, rel_w = net.infer(vol, 0)
+    say("tiled inference, core 32 halo", V.SURROGATE_HALO)
+    lam_t, rel_t = net.infer(vol, 32)
+    assert np.array_equal(lam_w, lam_t) and np.array_equal(rel_w, rel_t)
+    vol.release()
+    net.release()
+
+
+@pytest.mark.parametrize("tau", [0.0, 0.05, 1e30])
+def test_gated_measure_drives_exact_thinning(vc, oracle, tau):
+    occ = _occ(48, 14)
+    levels = [1.5, 3.0, 5.0]
+    vol = vc.upload(occ)
+    vol.edt()
+    net = V.Surrogate(vc, seed=8)
+    net.set_params(_theta(8))
+    lam, rel = net.infer(vol)
+    st = net.measure(vol, max_rel_err=tau)
+    inside = occ > 0
+    assert st.inside == int(inside.sum())
+    assert st.accepted == int((rel[inside] <= tau).sum())
+    assert st.accepted + st.fallback == st.inside
+    thin = vol.thin_levels(levels)
+    lab = vol.get(V.BUF_LABEL)
+    # the same measure on the host: surrogate where accepted, exact otherwise
+    lam2 = vol.get(V.BUF_LAM2) if st.fallback else np.zeros(occ.shape, np.uint32)
+    meas = np.where(rel <= np.float32(tau), np.maximum(lam, np.float32(0)),
+                    np.sqrt(lam2.astype(np.float32)))
+    lvl = np.full(occ.shape, 255, np.uint8)
+    for k in reversed(range(len(levels))):
+        lvl[inside & (meas < np.float32(levels[k]))] = k
+    lab_o, _ = oracle.thin(occ, lvl, len(levels))
+    assert np.array_equal(lab, lab_o)
+    # topology preserved by construction at every level
+    b = oracle.betti(occ)
+    for k in range(len(levels)):
+        assert oracle.betti(((lab == 1) | (lab > 2 + k)).astype(np.uint8)) == b
+    say("gate", tau, st.as_dict(), thin.removed[:3])
+    vol.release()
+    net.release()

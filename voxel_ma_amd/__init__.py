@@ -39,7 +39,14 @@ EXPORTS = [
     "vc_batch_create", "vc_batch_destroy", "vc_batch_size", "vc_batch_edt", "vc_batch_measure",
     "vc_batch_thin", "vc_batch_run", "vc_schedule_defaults", "vc_batch_get_schedule",
     "vc_batch_set_schedule", "vc_batch_tune",
+    # include/vc_surrogate.h
+    "vc_surrogate_create", "vc_surrogate_destroy", "vc_surrogate_get_params",
+    "vc_surrogate_set_params", "vc_surrogate_save", "vc_surrogate_load",
+    "vc_surrogate_gradients", "vc_surrogate_train_step", "vc_surrogate_infer",
+    "vc_surrogate_measure", "vc_surrogate_activation",
 ]
+SURROGATE_PARAMS = 678786
+SURROGATE_HALO = 56
 
 
 class VcError(RuntimeError):
@@ -66,6 +73,15 @@ class Schedule(ctypes.Structure):
     _fields_ = [("edt_lds_kb", ctypes.c_int), ("edt_grid_per_cu", ctypes.c_int),
                 ("thin_grid_per_cu", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
                 ("use_graph", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SurrogateStats(ctypes.Structure):
+    _fields_ = [("inside", ctypes.c_int64), ("accepted", ctypes.c_int64),
+                ("fallback", ctypes.c_int64), ("ms_infer", ctypes.c_float),
+                ("ms_exact", ctypes.c_float), ("ms_mix", ctypes.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -125,6 +141,17 @@ def library():
         "vc_batch_get_schedule": [P, ctypes.POINTER(Schedule)],
         "vc_batch_set_schedule": [P, P, ctypes.POINTER(Schedule)],
         "vc_batch_tune": [P, P, P, I, I, ctypes.POINTER(Schedule), ctypes.POINTER(ctypes.c_float)],
+        "vc_surrogate_create": [P, ctypes.c_uint64, PP],
+        "vc_surrogate_destroy": [P, P],
+        "vc_surrogate_get_params": [P, P, P, I64],
+        "vc_surrogate_set_params": [P, P, P, I64],
+        "vc_surrogate_save": [P, P, ctypes.c_char_p],
+        "vc_surrogate_load": [P, P, ctypes.c_char_p],
+        "vc_surrogate_gradients": [P, P, P, P, ctypes.POINTER(ctypes.c_float)],
+        "vc_surrogate_train_step": [P, P, P, I, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
+        "vc_surrogate_infer": [P, P, P, I, P, P],
+        "vc_surrogate_measure": [P, P, P, ctypes.c_float, I, ctypes.POINTER(SurrogateStats)],
+        "vc_surrogate_activation": [P, P, I, P, I64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -446,6 +473,85 @@ class Batch:
     def release(self):
         if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.vc_batch_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """The learned measure surrogate (vc_surrogate): 3-D U-Net on BF16 MFMA,
+    trained by the library's own forward / backward kernels."""
+
+    def __init__(self, ctx, seed=1, path=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.vc_surrogate_create(ctx.handle, int(seed), ctypes.byref(h)))
+        self.handle = h
+        if path:
+            self.load(path)
+
+    def params(self):
+        out = np.empty(SURROGATE_PARAMS, np.float32)
+        self.ctx.check(self.ctx._lib.vc_surrogate_get_params(self.ctx.handle, self.handle,
+                                                             _p(out), out.size))
+        return out
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        self.ctx.check(self.ctx._lib.vc_surrogate_set_params(self.ctx.handle, self.handle,
+                                                             _p(th), th.size))
+
+    def save(self, path):
+        self.ctx.check(self.ctx._lib.vc_surrogate_save(self.ctx.handle, self.handle,
+                                                       os.fsencode(path)))
+
+    def load(self, path):
+        self.ctx.check(self.ctx._lib.vc_surrogate_load(self.ctx.handle, self.handle,
+                                                       os.fsencode(path)))
+
+    def gradients(self, vol):
+        g = np.empty(SURROGATE_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx._lib.vc_surrogate_gradients(self.ctx.handle, self.handle,
+                                                            vol.handle, _p(g),
+                                                            ctypes.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, vols, lr):
+        arr = (ctypes.c_void_p * len(vols))(*[v.handle.value for v in vols])
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx._lib.vc_surrogate_train_step(self.ctx.handle, self.handle, arr,
+                                                             len(vols), float(lr),
+                                                             ctypes.byref(loss)))
+        return loss.value
+
+    def infer(self, vol, tile_core=0):
+        lam = np.empty(vol.shape, np.float32)
+        rel = np.empty(vol.shape, np.float32)
+        self.ctx.check(self.ctx._lib.vc_surrogate_infer(self.ctx.handle, self.handle, vol.handle,
+                                                        int(tile_core), _p(lam), _p(rel)))
+        return lam, rel
+
+    def measure(self, vol, max_rel_err=0.01, tile_core=0):
+        st = SurrogateStats()
+        self.ctx.check(self.ctx._lib.vc_surrogate_measure(self.ctx.handle, self.handle,
+                                                          vol.handle, float(max_rel_err),
+                                                          int(tile_core), ctypes.byref(st)))
+        return st
+
+    def activation(self, which, shape):
+        out = np.empty(shape, np.float32)
+        self.ctx.check(self.ctx._lib.vc_surrogate_activation(self.ctx.handle, self.handle,
+                                                             int(which), _p(out), out.size))
+        return out
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_surrogate_destroy(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):

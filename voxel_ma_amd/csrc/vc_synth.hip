@@ -153,6 +153,11 @@ __global__ void __launch_bounds__(kBlock)
             const double t0 = q[0] / r[13], t1 = q[1] / r[14], t2 = q[2] / r[15];
             in = ((t0 * t0 + t1 * t1) + t2 * t2) <= 1.0;
         }
+        else if (r[0] == 2.0)
+        {
+            const double rho = sqrt(q[0] * q[0] + q[1] * q[1]) - r[13];
+            in = (rho * rho + q[2] * q[2]) <= r[14] * r[14];
+        }
         else
         {
             const double h = r[14];

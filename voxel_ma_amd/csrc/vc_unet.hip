@@ -1,0 +1,1730 @@
+// vc_unet.hip -- the surrogate network of include/vc_surrogate.h, gfx950.
+//
+// Tensors are channels last, [X][Y][Z][C] with C a multiple of 8, BF16.
+// A 3^3 convolution is an implicit GEMM: M = voxels, N = output channels,
+// K = (tap, 8-channel chunk) pairs.  One v_mfma_f32_16x16x32_bf16 consumes
+// four chunks (lane group g = lane / 16 holds chunk kk + g), so any channel
+// count that is a multiple of 8 tiles K exactly (padded to 4 chunks with
+// zero weights).  The input-gradient pass is the same kernel with the taps
+// flipped and the channel roles swapped; the weight gradient stages 64-voxel
+// blocks of the layer input and output gradient in LDS and reduces over
+// voxels on MFMA into per-chunk partials, summed by a second kernel (fixed
+// order: the gradients are deterministic).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vc_internal.h"
+#include "../../include/vc_surrogate.h"
+
+namespace
+{
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr float kScale = 32.0f;
+constexpr float kEpsT = 1.0f / 32.0f;
+constexpr float kErrW = 0.1f;
+constexpr float kErrFloor = 1e-4f;
+constexpr int kHalo = VC_SURROGATE_HALO;
+
+struct LayerSpec
+{
+    int cin, cout, level;
+};
+// L0a L0b L1a L1b L2a L2b L3a L3b L4a L4b L5a L5b L6a L6b
+constexpr LayerSpec kLayers[14] = {
+    {2, 16, 0},   {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 48, 2},
+    {48, 48, 2},  {48, 64, 3}, {64, 64, 3}, {112, 48, 2}, {48, 48, 2},
+    {80, 32, 1},  {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
+constexpr int kHeadIn = 16, kHeadOut = 2;
+
+constexpr int cinp_of(int cin) { return (cin + 7) / 8 * 8; }
+constexpr int round4(int x) { return (x + 3) / 4 * 4; }
+
+int64_t param_offset(int layer) // offset of W of a layer (14 = head)
+{
+    int64_t o = 0;
+    for (int l = 0; l < layer; ++l)
+        o += 27LL * kLayers[l].cin * kLayers[l].cout + kLayers[l].cout;
+    return o;
+}
+int64_t total_params()
+{
+    return param_offset(14) + kHeadIn * kHeadOut + kHeadOut;
+}
+
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// ------------------------------------------------------------ convolution
+// out[v][co] = act(sum_{tap, ci} in[v + off(tap)][ci] W[tap][ci][co] + b[co])
+// W: [kc][Cout] of 8-channel chunks (chunk k = tap * cin8 + c8).
+// Region voxel (x, y, z) is padded-volume voxel (o + x, ...); outputs
+// outside the padded volume [0, P) are written as 0, so that a tile region
+// reproduces the zero padding of the whole-volume pass at every layer.
+struct Box
+{
+    int ox, oy, oz, px, py, pz;
+};
+
+template <int NT>
+__global__ void __launch_bounds__(kBlock)
+    conv_fwd(const bf16* __restrict__ in, int cinp, bf16* __restrict__ out,
+             const bf16x8* __restrict__ W, const float* __restrict__ bias,
+             int X, int Y, int Z, int64_t V, int kc, int relu, Box box)
+{
+    constexpr int MT = 2; // 16-voxel tiles per wavefront
+    constexpr int Cout = NT * 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int cin8 = cinp >> 3;
+    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wv) * (16 * MT);
+    int vx[MT], vy[MT], vz[MT];
+    bool vin[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int64_t v = vbase + m * 16 + i;
+        vin[m] = v < V;
+        const int64_t t = vin[m] ? v : 0;
+        vz[m] = (int)(t % Z);
+        const int64_t r = t / Z;
+        vy[m] = (int)(r % Y);
+        vx[m] = (int)(r / Y);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16x8 zero8 = {};
+    int tap = g / cin8, c8 = g % cin8; // chunk kk + g
+    for (int kk = 0; kk < kc; kk += 4)
+    {
+        const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+        {
+            const int xx = vx[m] + dx, yy = vy[m] + dy, zz = vz[m] + dz;
+            const bool ok = tap < 27 && vin[m] && (unsigned)xx < (unsigned)X &&
+                            (unsigned)yy < (unsigned)Y &&
+                            (unsigned)zz < (unsigned)Z;
+            a[m] = ok ? *reinterpret_cast<const bf16x8*>(
+                            in + ((((int64_t)xx * Y + yy) * Z + zz) * cinp +
+                                  c8 * 8))
+                      : zero8;
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+        {
+            const bf16x8 b = W[(int64_t)(kk + g) * Cout + n * 16 + i];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a[m], b, acc[m][n], 0, 0, 0);
+        }
+        c8 += 4;
+        while (c8 >= cin8)
+        {
+            c8 -= cin8;
+            ++tap;
+        }
+    }
+    // lane holds D[4g + r][i]: voxel vbase + m*16 + 4g + r, channel n*16 + i
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const int64_t v = vbase + m * 16 + 4 * g + r;
+            if (v >= V)
+                continue;
+            const int z = (int)(v % Z);
+            const int64_t q = v / Z;
+            const int y = (int)(q % Y), x = (int)(q / Y);
+            const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                               (unsigned)(box.oy + y) < (unsigned)box.py &&
+                               (unsigned)(box.oz + z) < (unsigned)box.pz;
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+            {
+                const int co = n * 16 + i;
+                float val = acc[m][n][r] + (bias ? bias[co] : 0.0f);
+                if (relu)
+                    val = fmaxf(val, 0.0f);
+                out[v * Cout + co] = inbox ? (bf16)val : (bf16)0.0f;
+            }
+        }
+}
+
+// Weight gradient partials.  Block (chunk, tg): taps (dx, dy) = (tg/3 - 1,
+// tg%3 - 1), dz = -1, 0, 1; voxels [chunk * vbpc * 64, ...) in 64-voxel
+// blocks.  part[chunk][tap][ci][co] = sum_v x[v + off(tap)][ci] dzt[v][co].
+// MFMA rows = output channels (A = dzt^T), columns = (dz, 16 input
+// channels) (B = shifted x), K = 32 voxels.
+template <int RT, int CT>
+__global__ void __launch_bounds__(kBlock)
+    conv_wgrad(const bf16* __restrict__ x, int cinp,
+               const bf16* __restrict__ dzt, float* __restrict__ part, int X,
+               int Y, int Z, int64_t V, int vbpc)
+{
+    constexpr int Cout = RT * 16, CP = CT * 16;
+    constexpr int SD = Cout + 2, SX = CP + 2; // LDS row strides (elements)
+    constexpr int NCOL = 3 * CT, CPW = (NCOL + 3) / 4;
+    __shared__ bf16 sdz[64 * SD];
+    __shared__ bf16 sx[66 * SX];
+    __shared__ int szc[64];
+    __shared__ int sok[64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int tg = blockIdx.y;
+    const int dx = tg / 3 - 1, dy = tg % 3 - 1;
+    const int64_t off = ((int64_t)dx * Y + dy) * Z;
+    f32x4 acc[RT][CPW];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = 0; c < CPW; ++c)
+            acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int64_t nvb = (V + 63) / 64;
+    const int64_t vb0 = (int64_t)blockIdx.x * vbpc;
+    for (int64_t vb = vb0; vb < vb0 + vbpc && vb < nvb; ++vb)
+    {
+        const int64_t v0 = vb * 64;
+        // output gradient rows v0 .. v0+63, two channels per step
+        for (int idx = tid; idx < 64 * (Cout / 2); idx += kBlock)
+        {
+            const int row = idx / (Cout / 2), cp = idx % (Cout / 2);
+            uint32_t w = 0;
+            if (v0 + row < V)
+                w = *reinterpret_cast<const uint32_t*>(dzt + (v0 + row) * Cout +
+                                                       2 * cp);
+            *reinterpret_cast<uint32_t*>(sdz + row * SD + 2 * cp) = w;
+        }
+        // input rows s = v0 - 1 + r + off, r in [0, 66)
+        for (int idx = tid; idx < 66 * (CP / 2); idx += kBlock)
+        {
+            const int r = idx / (CP / 2), cp = idx % (CP / 2);
+            const int64_t s = v0 - 1 + r + off;
+            uint32_t w = 0;
+            if (s >= 0 && s < V && 2 * cp < cinp)
+                w = *reinterpret_cast<const uint32_t*>(x + s * cinp + 2 * cp);
+            *reinterpret_cast<uint32_t*>(sx + r * SX + 2 * cp) = w;
+        }
+        if (tid < 64)
+        {
+            const int64_t v = v0 + tid;
+            int ok = 0, zc = 0;
+            if (v < V)
+            {
+                zc = (int)(v % Z);
+                const int64_t t = v / Z;
+                const int yc = (int)(t % Y), xc = (int)(t / Y);
+                ok = (unsigned)(xc + dx) < (unsigned)X &&
+                     (unsigned)(yc + dy) < (unsigned)Y;
+            }
+            szc[tid] = zc;
+            sok[tid] = ok;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+        {
+            const int k0 = kb * 32 + 8 * g;
+            bf16x8 A[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    A[rt][e] = sdz[(k0 + e) * SD + rt * 16 + i];
+            // validity of (voxel k0 + e, dz) as three 8-bit masks
+            uint32_t okm[3] = {0u, 0u, 0u};
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+            {
+                const int zc = szc[k0 + e];
+                const bool ok = sok[k0 + e] != 0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    okm[d] |= (uint32_t)(ok && (unsigned)(zc + d - 1) <
+                                                   (unsigned)Z)
+                              << e;
+            }
+#pragma unroll
+            for (int cw = 0; cw < CPW; ++cw)
+            {
+                const int col = wv + 4 * cw;
+                if (col < NCOL)
+                {
+                    const int d = col / CT, ct = col % CT;
+                    bf16x8 B;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                    {
+                        const bf16 val =
+                            sx[(k0 + e + d) * SX + ct * 16 + i];
+                        B[e] = ((okm[d] >> e) & 1u) ? val : (bf16)0.0f;
+                    }
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt)
+                        acc[rt][cw] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            A[rt], B, acc[rt][cw], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // lane holds D[4g + r][i]: output channel rt*16 + 4g + r, input channel
+    // ct*16 + i
+#pragma unroll
+    for (int cw = 0; cw < CPW; ++cw)
+    {
+        const int col = wv + 4 * cw;
+        if (col >= NCOL)
+            continue;
+        const int d = col / CT, ct = col % CT;
+        const int tap = tg * 3 + d;
+        const int ci = ct * 16 + i;
+        if (ci >= cinp)
+            continue;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                const int co = rt * 16 + 4 * g + r;
+                part[(((int64_t)blockIdx.x * 27 + tap) * cinp + ci) * Cout + co] =
+                    acc[rt][cw][r];
+            }
+    }
+}
+
+// grad[(tap * cin + ci) * cout + co] += sum over chunks of the partials
+__global__ void __launch_bounds__(kBlock)
+    wgrad_reduce(const float* __restrict__ part, int nchunks, int cin,
+                 int cinp, int cout, float* __restrict__ grad)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = 27LL * cin * cout;
+    if (t >= n)
+        return;
+    const int co = (int)(t % cout);
+    const int64_t r = t / cout;
+    const int ci = (int)(r % cin), tap = (int)(r / cin);
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; ++c)
+        s += part[(((int64_t)c * 27 + tap) * cinp + ci) * cout + co];
+    grad[t] += s;
+}
+
+// per-block column sums of a [V][C] BF16 tensor: part[block][c]
+__global__ void __launch_bounds__(kBlock)
+    colsum_part(const bf16* __restrict__ a, int C, int64_t V, int64_t vpb,
+                float* __restrict__ part)
+{
+    __shared__ float red[kBlock];
+    const int c = threadIdx.x % C;
+    const int lanes = kBlock / C; // threads per channel
+    const int j = threadIdx.x / C;
+    float s = 0.0f;
+    if (j < lanes)
+    {
+        const int64_t v0 = (int64_t)blockIdx.x * vpb;
+        for (int64_t v = v0 + j; v < v0 + vpb && v < V; v += lanes)
+            s += (float)a[v * C + c];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C)
+    {
+        float t = 0.0f;
+        for (int q = 0; q < lanes; ++q)
+            t += red[q * C + threadIdx.x];
+        part[(int64_t)blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+// dst[c] += sum_b part[b][c]
+__global__ void colsum_reduce(const float* __restrict__ part, int nb, int C,
+                              float* __restrict__ dst)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C)
+        return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b)
+        s += part[(int64_t)b * C + c];
+    dst[c] += s;
+}
+
+// ------------------------------------------------------------ elementwise
+__global__ void __launch_bounds__(kBlock)
+    relu_mask(bf16* __restrict__ g, const bf16* __restrict__ y, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && !((float)y[t] > 0.0f))
+        g[t] = (bf16)0.0f;
+}
+
+// child voxel (x, y, z) of level dims (X, Y, Z) -> parent at (X/2, Y/2, Z/2)
+__device__ __forceinline__ int64_t parent_of(int64_t v, int X, int Y, int Z)
+{
+    const int z = (int)(v % Z);
+    const int64_t r = v / Z;
+    const int y = (int)(r % Y), x = (int)(r / Y);
+    return (((int64_t)(x >> 1) * (Y >> 1)) + (y >> 1)) * (Z >> 1) + (z >> 1);
+}
+
+// p[parent][c] = mean of the 8 children (dims of the child level X, Y, Z)
+__global__ void __launch_bounds__(kBlock)
+    pool_fwd(const bf16* __restrict__ a, bf16* __restrict__ p, int C, int X,
+             int Y, int Z)
+{
+    const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)Xp * Yp * Zp * C;
+    if (t >= n)
+        return;
+    const int c = (int)(t % C);
+    const int64_t v = t / C;
+    const int z = (int)(v % Zp);
+    const int64_t r = v / Zp;
+    const int y = (int)(r % Yp), x = (int)(r / Yp);
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        const int64_t cv =
+            (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
+            (2 * z + (q & 1));
+        s += (float)a[cv * C + c];
+    }
+    p[t] = (bf16)(s * 0.125f);
+}
+
+// g_child += g_parent / 8
+__global__ void __launch_bounds__(kBlock)
+    pool_bwd_acc(bf16* __restrict__ ga, const bf16* __restrict__ gp, int C,
+                 int X, int Y, int Z)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)X * Y * Z * C;
+    if (t >= n)
+        return;
+    const int c = (int)(t % C);
+    const int64_t v = t / C;
+    const int64_t pv = parent_of(v, X, Y, Z);
+    ga[t] = (bf16)((float)ga[t] + 0.125f * (float)gp[pv * C + c]);
+}
+
+// cat[v] = [up[parent(v)] (C1 channels), skip[v] (C2)] at child dims
+__global__ void __launch_bounds__(kBlock)
+    concat_fwd(const bf16* __restrict__ up, int C1, const bf16* __restrict__ sk,
+               int C2, bf16* __restrict__ cat, int X, int Y, int Z)
+{
+    const int C = C1 + C2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)X * Y * Z * C;
+    if (t >= n)
+        return;
+    const int c = (int)(t % C);
+    const int64_t v = t / C;
+    cat[t] = c < C1 ? up[parent_of(v, X, Y, Z) * C1 + c]
+                    : sk[v * C2 + (c - C1)];
+}
+
+// g_up[parent][c] = sum over the 8 children of g_cat[child][c] (c < C1)
+__global__ void __launch_bounds__(kBlock)
+    split_up(const bf16* __restrict__ gcat, int C, int C1,
+             bf16* __restrict__ gup, int X, int Y, int Z)
+{
+    const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)Xp * Yp * Zp * C1;
+    if (t >= n)
+        return;
+    const int c = (int)(t % C1);
+    const int64_t v = t / C1;
+    const int z = (int)(v % Zp);
+    const int64_t r = v / Zp;
+    const int y = (int)(r % Yp), x = (int)(r / Yp);
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        const int64_t cv =
+            (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
+            (2 * z + (q & 1));
+        s += (float)gcat[cv * C + c];
+    }
+    gup[t] = (bf16)s;
+}
+
+// g_skip[v][c] = g_cat[v][C1 + c]
+__global__ void __launch_bounds__(kBlock)
+    split_skip(const bf16* __restrict__ gcat, int C, int C1,
+               bf16* __restrict__ gsk, int64_t V)
+{
+    const int C2 = C - C1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= V * C2)
+        return;
+    const int c = (int)(t % C2);
+    const int64_t v = t / C2;
+    gsk[t] = gcat[v * C + C1 + c];
+}
+
+// network input over a region of dims (X, Y, Z) whose voxel (0,0,0) is
+// volume voxel (ox, oy, oz): [occ, sqrt(max(edt, 0)) / 32, 0 x 6]
+__global__ void __launch_bounds__(kBlock)
+    prep_input(const uint8_t* __restrict__ occ, const int32_t* __restrict__ edt,
+               int nx, int ny, int nz, int ox, int oy, int oz,
+               bf16* __restrict__ x0, int X, int Y, int Z)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t V = (int64_t)X * Y * Z;
+    if (v >= V)
+        return;
+    const int z = (int)(v % Z);
+    const int64_t r = v / Z;
+    const int y = (int)(r % Y), x = (int)(r / Y);
+    const int gx = ox + x, gy = oy + y, gz = oz + z;
+    float o = 0.0f, d = 0.0f;
+    if ((unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny &&
+        (unsigned)gz < (unsigned)nz)
+    {
+        const int64_t i = ((int64_t)gx * ny + gy) * nz + gz;
+        o = occ[i] ? 1.0f : 0.0f;
+        const int32_t e = edt[i];
+        d = e > 0 ? sqrtf((float)e) / kScale : 0.0f;
+    }
+    bf16x8 w = {};
+    w[0] = (bf16)o;
+    w[1] = (bf16)d;
+    reinterpret_cast<bf16x8*>(x0)[v] = w;
+}
+
+// head: out[v][o] = sum_c d0[v][c] Wh[c][o] + bh[o] (FP32)
+__global__ void __launch_bounds__(kBlock)
+    head_fwd(const bf16* __restrict__ d0, const float* __restrict__ Wh,
+             float* __restrict__ out, int64_t V)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V)
+        return;
+    float o0 = Wh[kHeadIn * kHeadOut], o1 = Wh[kHeadIn * kHeadOut + 1];
+#pragma unroll
+    for (int c = 0; c < kHeadIn; ++c)
+    {
+        const float h = (float)d0[v * kHeadIn + c];
+        o0 += h * Wh[c * 2];
+        o1 += h * Wh[c * 2 + 1];
+    }
+    out[2 * v] = o0;
+    out[2 * v + 1] = o1;
+}
+
+// loss terms and dL/d(out) on inside voxels of the region (volume voxel =
+// region voxel for training: no offset); part[block] = block loss sum
+__global__ void __launch_bounds__(kBlock)
+    loss_bwd(const float* __restrict__ out, const uint8_t* __restrict__ occ,
+             const uint32_t* __restrict__ lam2, int nx, int ny, int nz,
+             int X, int Y, int Z, const unsigned long long* __restrict__ cnt,
+             float* __restrict__ dout, float* __restrict__ part)
+{
+    __shared__ float red[kBlock];
+    const unsigned long long nin = *cnt;
+    const float inv_n = 1.0f / (float)(nin ? nin : 1ull);
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t V = (int64_t)X * Y * Z;
+    float l = 0.0f;
+    if (v < V)
+    {
+        const int z = (int)(v % Z);
+        const int64_t r = v / Z;
+        const int y = (int)(r % Y), x = (int)(r / Y);
+        float g0 = 0.0f, g1 = 0.0f;
+        if (x < nx && y < ny && z < nz)
+        {
+            const int64_t i = ((int64_t)x * ny + y) * nz + z;
+            if (occ[i])
+            {
+                const float t = sqrtf((float)lam2[i]) / kScale;
+                const float den = t + kEpsT;
+                const float rr = (out[2 * v] - t) / den;
+                const float e = logf(fabsf(rr) + kErrFloor);
+                const float q = out[2 * v + 1] - e;
+                l = rr * rr + kErrW * q * q;
+                g0 = 2.0f * rr / den * inv_n;
+                g1 = 2.0f * kErrW * q * inv_n;
+            }
+        }
+        dout[2 * v] = g0;
+        dout[2 * v + 1] = g1;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0];
+}
+
+// head backward: gd0[v][c] = sum_o dout[v][o] Wh[c][o]; per-block partials
+// of dWh[c][o] (32 values) and dbh[o] (2) in part[block][34]
+__global__ void __launch_bounds__(kBlock)
+    head_bwd(const bf16* __restrict__ d0, const float* __restrict__ dout,
+             const float* __restrict__ Wh, bf16* __restrict__ gd0, int64_t V,
+             float* __restrict__ part)
+{
+    constexpr int NP = kHeadIn * kHeadOut + kHeadOut;
+    __shared__ float red[NP][kBlock / 64];
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float g0 = 0.0f, g1 = 0.0f;
+    float h[kHeadIn];
+    if (v < V)
+    {
+        g0 = dout[2 * v];
+        g1 = dout[2 * v + 1];
+    }
+#pragma unroll
+    for (int c = 0; c < kHeadIn; ++c)
+    {
+        h[c] = v < V ? (float)d0[v * kHeadIn + c] : 0.0f;
+        if (v < V)
+            gd0[v * kHeadIn + c] = (bf16)(g0 * Wh[c * 2] + g1 * Wh[c * 2 + 1]);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+    {
+        float val = p < kHeadIn * kHeadOut
+                        ? h[p >> 1] * ((p & 1) ? g1 : g0)
+                        : ((p - kHeadIn * kHeadOut) ? g1 : g0);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+            val += __shfl_xor(val, d);
+        if (lane == 0)
+            red[p][wv] = val;
+    }
+    __syncthreads();
+    if (threadIdx.x < NP)
+    {
+        float s = 0.0f;
+        for (int w = 0; w < kBlock / 64; ++w)
+            s += red[threadIdx.x][w];
+        part[(int64_t)blockIdx.x * NP + threadIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    adam_step(float* __restrict__ w, const float* __restrict__ g,
+              float* __restrict__ m, float* __restrict__ s, int64_t n,
+              float lr, float gscale, float bc1, float bc2)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const float gg = g[t] * gscale;
+    const float mm = 0.9f * m[t] + 0.1f * gg;
+    const float ss = 0.999f * s[t] + 0.001f * gg * gg;
+    m[t] = mm;
+    s[t] = ss;
+    w[t] -= lr * (mm / bc1) / (sqrtf(ss / bc2) + 1e-8f);
+}
+
+// FP32 master W[27][cin][cout] -> forward chunks Wf[kc][cout][8] and the
+// flipped, transposed input-gradient chunks Wd[kcd][cinp][8]
+__global__ void __launch_bounds__(kBlock)
+    pack_fwd(const float* __restrict__ W, int cin, int cinp, int cout, int kc,
+             bf16* __restrict__ Wf)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)kc * cout * 8)
+        return;
+    const int e = (int)(t & 7);
+    const int co = (int)((t >> 3) % cout);
+    const int k = (int)((t >> 3) / cout);
+    const int cin8 = cinp >> 3;
+    float val = 0.0f;
+    if (k < 27 * cin8)
+    {
+        const int tap = k / cin8, ci = (k % cin8) * 8 + e;
+        if (ci < cin)
+            val = W[((int64_t)tap * cin + ci) * cout + co];
+    }
+    Wf[t] = (bf16)val;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    pack_dgrad(const float* __restrict__ W, int cin, int cinp, int cout,
+               int kcd, bf16* __restrict__ Wd)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)kcd * cinp * 8)
+        return;
+    const int e = (int)(t & 7);
+    const int ci = (int)((t >> 3) % cinp);
+    const int k = (int)((t >> 3) / cinp);
+    const int cout8 = cout >> 3;
+    float val = 0.0f;
+    if (k < 27 * cout8 && ci < cin)
+    {
+        const int tap = 26 - k / cout8, co = (k % cout8) * 8 + e;
+        val = W[((int64_t)tap * cin + ci) * cout + co];
+    }
+    Wd[t] = (bf16)val;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    fill_f32(float* __restrict__ p, int64_t n, float v)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        p[t] = v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bf16_to_f32(const bf16* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        b[t] = (float)a[t];
+}
+
+// core of a region's head output -> volume outputs (lambda, relative error)
+__global__ void __launch_bounds__(kBlock)
+    copy_core(const float* __restrict__ out, int X, int Y, int Z, int ox,
+              int oy, int oz, int cx0, int cy0, int cz0, int cx1, int cy1,
+              int cz1, float* __restrict__ lam, float* __restrict__ rel,
+              int nx, int ny, int nz)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t V = (int64_t)X * Y * Z;
+    if (v >= V)
+        return;
+    const int z = (int)(v % Z);
+    const int64_t r = v / Z;
+    const int y = (int)(r % Y), x = (int)(r / Y);
+    const int gx = ox + x, gy = oy + y, gz = oz + z;
+    if (gx < cx0 || gx >= cx1 || gy < cy0 || gy >= cy1 || gz < cz0 ||
+        gz >= cz1)
+        return;
+    if (gx >= nx || gy >= ny || gz >= nz)
+        return;
+    const int64_t i = ((int64_t)gx * ny + gy) * nz + gz;
+    lam[i] = out[2 * v] * kScale;
+    rel[i] = expf(out[2 * v + 1]);
+}
+
+// gated measure: inside voxels keep the surrogate lambda when the predicted
+// relative error is <= tau, else the exact sqrt(lam2); ctr[0] accepted,
+// ctr[1] fallback, ctr[2] inside
+__global__ void __launch_bounds__(kBlock)
+    gate_count(const uint8_t* __restrict__ occ, const float* __restrict__ rel,
+               float tau, int64_t N, unsigned long long* __restrict__ ctr)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0, fb = 0;
+    if (i < N && occ[i])
+    {
+        if (rel[i] <= tau)
+            acc = 1;
+        else
+            fb = 1;
+    }
+    uint64_t a = __ballot(acc), f = __ballot(fb);
+    if ((threadIdx.x & 63) == 0 && (a | f))
+    {
+        atomicAdd(&ctr[0], (unsigned long long)__popcll(a));
+        atomicAdd(&ctr[1], (unsigned long long)__popcll(f));
+        atomicAdd(&ctr[2], (unsigned long long)__popcll(a | f));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    gate_mix(const uint8_t* __restrict__ occ, const float* __restrict__ lam,
+             const float* __restrict__ rel, const uint32_t* __restrict__ lam2,
+             float tau, int64_t N, float* __restrict__ meas)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    float m = 0.0f;
+    if (occ[i])
+        m = rel[i] <= tau ? fmaxf(lam[i], 0.0f) : sqrtf((float)lam2[i]);
+    meas[i] = m;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+} // namespace
+
+// ------------------------------------------------------------ host side
+struct vc_surrogate
+{
+    vc_ctx* ctx = nullptr;
+    int64_t np = 0;
+    float* theta = nullptr; // FP32 master
+    float* grad = nullptr;
+    float* m1 = nullptr;
+    float* m2 = nullptr;
+    int64_t step = 0;
+    bf16* Wf[14] = {};
+    bf16* Wd[14] = {};
+    int kc[14] = {}, kcd[14] = {};
+    bool packed = false;
+    // workspace for one region
+    int X = 0, Y = 0, Z = 0;
+    std::vector<void*> ws;
+    bf16* act[14] = {}; // ReLU outputs of L0a..L6b
+    bf16 *x0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+    bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
+    float* out = nullptr;
+    // backward workspace (allocated on first use)
+    bool bw = false;
+    bf16* gact[14] = {};
+    bf16 *gcat2 = nullptr, *gcat1 = nullptr, *gcat0 = nullptr;
+    bf16 *gp1 = nullptr, *gp2 = nullptr, *gp3 = nullptr;
+    float* dout = nullptr;
+    float* part = nullptr;
+    size_t part_floats = 0;
+    // volume-sized outputs of inference
+    float* vlam = nullptr;
+    float* vrel = nullptr;
+    float* vmeas = nullptr;
+    int64_t vN = 0;
+    int org[3] = {0, 0, 0}; // region origin in the padded volume
+    int pad[3] = {0, 0, 0}; // padded volume dims (multiples of 8)
+    unsigned long long* ctr = nullptr; // 64 counters
+    float* lossbuf = nullptr;          // 64 per-volume losses
+};
+
+namespace
+{
+
+void free_ws(vc_surrogate* s)
+{
+    for (void* p : s->ws)
+        vc_free(p);
+    s->ws.clear();
+    for (int l = 0; l < 14; ++l)
+    {
+        s->act[l] = nullptr;
+        s->gact[l] = nullptr;
+    }
+    s->x0 = s->p1 = s->p2 = s->p3 = s->cat2 = s->cat1 = s->cat0 = nullptr;
+    s->gcat2 = s->gcat1 = s->gcat0 = s->gp1 = s->gp2 = s->gp3 = nullptr;
+    s->out = s->dout = s->part = nullptr;
+    s->part_floats = 0;
+    s->bw = false;
+    s->X = s->Y = s->Z = 0;
+}
+
+int ws_alloc(vc_ctx* ctx, vc_surrogate* s, void** p, size_t bytes)
+{
+    int rc = vc_alloc(ctx, p, bytes);
+    if (rc == VC_OK)
+        s->ws.push_back(*p);
+    return rc;
+}
+
+int64_t level_voxels(const vc_surrogate* s, int level)
+{
+    return (int64_t)(s->X >> level) * (s->Y >> level) * (s->Z >> level);
+}
+
+// region origin (multiple of 8, may be negative) and padded-volume extent
+// at level lv
+Box level_box(const vc_surrogate* s, int lv)
+{
+    Box b;
+    b.ox = s->org[0] / (1 << lv);
+    b.oy = s->org[1] / (1 << lv);
+    b.oz = s->org[2] / (1 << lv);
+    b.px = s->pad[0] >> lv;
+    b.py = s->pad[1] >> lv;
+    b.pz = s->pad[2] >> lv;
+    return b;
+}
+
+int ensure_ws(vc_ctx* ctx, vc_surrogate* s, int X, int Y, int Z, bool bw)
+{
+    if (s->X != X || s->Y != Y || s->Z != Z)
+    {
+        VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        free_ws(s);
+        s->X = X;
+        s->Y = Y;
+        s->Z = Z;
+        int rc;
+        auto A = [&](bf16** p, int level, int C) {
+            return ws_alloc(ctx, s, (void**)p,
+                            2 * (size_t)level_voxels(s, level) * C);
+        };
+        if ((rc = A(&s->x0, 0, 8)) != VC_OK)
+            return rc;
+        for (int l = 0; l < 14; ++l)
+            if ((rc = A(&s->act[l], kLayers[l].level, kLayers[l].cout)) !=
+                VC_OK)
+                return rc;
+        if ((rc = A(&s->p1, 1, 16)) != VC_OK || (rc = A(&s->p2, 2, 32)) != VC_OK ||
+            (rc = A(&s->p3, 3, 48)) != VC_OK || (rc = A(&s->cat2, 2, 112)) != VC_OK ||
+            (rc = A(&s->cat1, 1, 80)) != VC_OK || (rc = A(&s->cat0, 0, 48)) != VC_OK)
+            return rc;
+        if ((rc = ws_alloc(ctx, s, (void**)&s->out,
+                           8 * (size_t)level_voxels(s, 0))) != VC_OK)
+            return rc;
+    }
+    if (bw && !s->bw)
+    {
+        int rc;
+        auto A = [&](bf16** p, int level, int C) {
+            return ws_alloc(ctx, s, (void**)p,
+                            2 * (size_t)level_voxels(s, level) * C);
+        };
+        for (int l = 1; l < 14; ++l)
+            if ((rc = A(&s->gact[l], kLayers[l].level, kLayers[l].cout)) !=
+                VC_OK)
+                return rc;
+        if ((rc = A(&s->gact[0], 0, 16)) != VC_OK ||
+            (rc = A(&s->gp1, 1, 16)) != VC_OK || (rc = A(&s->gp2, 2, 32)) != VC_OK ||
+            (rc = A(&s->gp3, 3, 48)) != VC_OK || (rc = A(&s->gcat2, 2, 112)) != VC_OK ||
+            (rc = A(&s->gcat1, 1, 80)) != VC_OK || (rc = A(&s->gcat0, 0, 48)) != VC_OK)
+            return rc;
+        if ((rc = ws_alloc(ctx, s, (void**)&s->dout,
+                           8 * (size_t)level_voxels(s, 0))) != VC_OK)
+            return rc;
+        // partials: weight gradients (chunks x 27 x cinp x cout, at most 256
+        // chunks) and column / loss sums
+        size_t pf = 0;
+        for (int l = 0; l < 14; ++l)
+        {
+            const size_t f = 256ull * 27 * cinp_of(kLayers[l].cin) *
+                             kLayers[l].cout;
+            pf = f > pf ? f : pf;
+        }
+        const size_t cs = (size_t)(level_voxels(s, 0) / kBlock + 1) * 64;
+        pf = cs > pf ? cs : pf;
+        if ((rc = ws_alloc(ctx, s, (void**)&s->part, 4 * pf)) != VC_OK)
+            return rc;
+        s->part_floats = pf;
+        s->bw = true;
+    }
+    return VC_OK;
+}
+
+int pack_weights(vc_ctx* ctx, vc_surrogate* s)
+{
+    hipStream_t S = ctx->stream;
+    for (int l = 0; l < 14; ++l)
+    {
+        const LayerSpec& L = kLayers[l];
+        const int cinp = cinp_of(L.cin);
+        const float* W = s->theta + param_offset(l);
+        const int64_t nf = (int64_t)s->kc[l] * L.cout * 8;
+        pack_fwd<<<blocks_for(nf), kBlock, 0, S>>>(W, L.cin, cinp, L.cout,
+                                                    s->kc[l], s->Wf[l]);
+        VC_CHECK_LAUNCH(ctx);
+        if (s->Wd[l])
+        {
+            const int64_t nd = (int64_t)s->kcd[l] * cinp * 8;
+            pack_dgrad<<<blocks_for(nd), kBlock, 0, S>>>(
+                W, L.cin, cinp, L.cout, s->kcd[l], s->Wd[l]);
+            VC_CHECK_LAUNCH(ctx);
+        }
+    }
+    s->packed = true;
+    return VC_OK;
+}
+
+template <int NT>
+void launch_conv_t(hipStream_t S, const bf16* in, int cinp, bf16* out,
+                   const bf16* W, const float* bias, int X, int Y, int Z,
+                   int kc, int relu, const Box& box)
+{
+    const int64_t V = (int64_t)X * Y * Z;
+    const unsigned g = (unsigned)((V + 127) / 128);
+    conv_fwd<NT><<<g, kBlock, 0, S>>>(in, cinp, out,
+                                      reinterpret_cast<const bf16x8*>(W), bias,
+                                      X, Y, Z, V, kc, relu, box);
+}
+
+// box: the region's origin and the padded-volume extent at this level
+int launch_conv(vc_ctx* ctx, const bf16* in, int cinp, bf16* out, int cout,
+                const bf16* W, const float* bias, int X, int Y, int Z, int kc,
+                int relu, const Box& b)
+{
+    hipStream_t S = ctx->stream;
+    switch (cout / 16)
+    {
+    case 1: launch_conv_t<1>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    case 2: launch_conv_t<2>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    case 3: launch_conv_t<3>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    case 4: launch_conv_t<4>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    case 5: launch_conv_t<5>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    case 7: launch_conv_t<7>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+    default:
+        return vc_fail(ctx, VC_ERR_INVALID, "conv: %d output channels", cout);
+    }
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+template <int RT, int CT>
+void launch_wgrad_t(hipStream_t S, const bf16* x, int cinp, const bf16* dz,
+                    float* part, int X, int Y, int Z, int nchunks, int vbpc)
+{
+    const int64_t V = (int64_t)X * Y * Z;
+    conv_wgrad<RT, CT><<<dim3((unsigned)nchunks, 9), kBlock, 0, S>>>(
+        x, cinp, dz, part, X, Y, Z, V, vbpc);
+}
+
+// layer l: gradient of W (and b) from its input x and pre-activation
+// gradient dz, accumulated into s->grad
+int layer_wgrad(vc_ctx* ctx, vc_surrogate* s, int l, const bf16* x,
+                const bf16* dz)
+{
+    const LayerSpec& L = kLayers[l];
+    const int X = s->X >> L.level, Y = s->Y >> L.level, Z = s->Z >> L.level;
+    const int64_t V = (int64_t)X * Y * Z;
+    const int cinp = cinp_of(L.cin);
+    const int64_t nvb = (V + 63) / 64;
+    const int nchunks = (int)(nvb < 256 ? nvb : 256);
+    const int vbpc = (int)((nvb + nchunks - 1) / nchunks);
+    hipStream_t S = ctx->stream;
+    const int RT = L.cout / 16, CT = (cinp + 15) / 16;
+#define VC_WG(r, c)                                                            \
+    if (RT == r && CT == c)                                                    \
+        launch_wgrad_t<r, c>(S, x, cinp, dz, s->part, X, Y, Z, nchunks, vbpc); \
+    else
+    VC_WG(1, 1) VC_WG(2, 1) VC_WG(2, 2) VC_WG(3, 2) VC_WG(3, 3) VC_WG(4, 3)
+    VC_WG(4, 4) VC_WG(3, 7) VC_WG(2, 5) VC_WG(1, 3)
+    return vc_fail(ctx, VC_ERR_INVALID, "wgrad: no kernel for %d x %d", RT, CT);
+#undef VC_WG
+    VC_CHECK_LAUNCH(ctx);
+    float* gW = s->grad + param_offset(l);
+    const int64_t nw = 27LL * L.cin * L.cout;
+    wgrad_reduce<<<blocks_for(nw), kBlock, 0, S>>>(s->part, nchunks, L.cin,
+                                                    cinp, L.cout, gW);
+    VC_CHECK_LAUNCH(ctx);
+    // bias: column sums of dz
+    const int64_t vpb = 4096;
+    const int nb = (int)((V + vpb - 1) / vpb);
+    colsum_part<<<nb, kBlock, 0, S>>>(dz, L.cout, V, vpb, s->part);
+    VC_CHECK_LAUNCH(ctx);
+    colsum_reduce<<<1, 64, 0, S>>>(s->part, nb, L.cout, gW + nw);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+// forward over the current workspace region (input x0 filled)
+int forward(vc_ctx* ctx, vc_surrogate* s)
+{
+    hipStream_t S = ctx->stream;
+    const bf16* in = s->x0;
+    int rc;
+    auto conv = [&](int l, const bf16* x) {
+        const LayerSpec& L = kLayers[l];
+        return launch_conv(ctx, x, cinp_of(L.cin), s->act[l], L.cout, s->Wf[l],
+                           s->theta + param_offset(l) + 27LL * L.cin * L.cout,
+                           s->X >> L.level, s->Y >> L.level, s->Z >> L.level,
+                           s->kc[l], 1, level_box(s, L.level));
+    };
+    auto dims = [&](int lv, int& X, int& Y, int& Z) {
+        X = s->X >> lv;
+        Y = s->Y >> lv;
+        Z = s->Z >> lv;
+    };
+    int X, Y, Z;
+    if ((rc = conv(0, in)) != VC_OK || (rc = conv(1, s->act[0])) != VC_OK)
+        return rc;
+    // encoder
+    bf16* pools[3] = {s->p1, s->p2, s->p3};
+    const int pc[3] = {16, 32, 48};
+    for (int lv = 0; lv < 3; ++lv)
+    {
+        dims(lv, X, Y, Z);
+        const int64_t n = level_voxels(s, lv + 1) * pc[lv];
+        pool_fwd<<<blocks_for(n), kBlock, 0, S>>>(s->act[2 * lv + 1], pools[lv],
+                                                   pc[lv], X, Y, Z);
+        VC_CHECK_LAUNCH(ctx);
+        if ((rc = conv(2 * lv + 2, pools[lv])) != VC_OK ||
+            (rc = conv(2 * lv + 3, s->act[2 * lv + 2])) != VC_OK)
+            return rc;
+    }
+    // decoder: level 2, 1, 0
+    bf16* cats[3] = {s->cat2, s->cat1, s->cat0};
+    const int c1s[3] = {64, 48, 32}, c2s[3] = {48, 32, 16};
+    const bf16* ups[3] = {s->act[7], s->act[9], s->act[11]};
+    const bf16* skips[3] = {s->act[5], s->act[3], s->act[1]};
+    for (int d = 0; d < 3; ++d)
+    {
+        const int lv = 2 - d;
+        dims(lv, X, Y, Z);
+        const int64_t n = level_voxels(s, lv) * (c1s[d] + c2s[d]);
+        concat_fwd<<<blocks_for(n), kBlock, 0, S>>>(ups[d], c1s[d], skips[d],
+                                                     c2s[d], cats[d], X, Y, Z);
+        VC_CHECK_LAUNCH(ctx);
+        if ((rc = conv(8 + 2 * d, cats[d])) != VC_OK ||
+            (rc = conv(9 + 2 * d, s->act[8 + 2 * d])) != VC_OK)
+            return rc;
+    }
+    const int64_t V0 = level_voxels(s, 0);
+    head_fwd<<<blocks_for(V0), kBlock, 0, S>>>(s->act[13],
+                                                s->theta + param_offset(14),
+                                                s->out, V0);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+// layer l backward: gy (gradient of its ReLU output, in place) -> masked,
+// weight gradient, and (unless gx is null) the input gradient into gx
+int layer_back(vc_ctx* ctx, vc_surrogate* s, int l, bf16* gy, const bf16* x,
+               bf16* gx)
+{
+    const LayerSpec& L = kLayers[l];
+    const int X = s->X >> L.level, Y = s->Y >> L.level, Z = s->Z >> L.level;
+    const int64_t V = (int64_t)X * Y * Z;
+    hipStream_t S = ctx->stream;
+    relu_mask<<<blocks_for(V * L.cout), kBlock, 0, S>>>(gy, s->act[l],
+                                                         V * L.cout);
+    VC_CHECK_LAUNCH(ctx);
+    int rc = layer_wgrad(ctx, s, l, x, gy);
+    if (rc != VC_OK || !gx)
+        return rc;
+    return launch_conv(ctx, gy, L.cout, gx, cinp_of(L.cin), s->Wd[l], nullptr,
+                       X, Y, Z, s->kcd[l], 0, level_box(s, L.level));
+}
+
+__global__ void __launch_bounds__(kBlock)
+    count_occ(const uint8_t* __restrict__ occ, int64_t N,
+              unsigned long long* __restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t b = __ballot(i < N && occ[i] != 0);
+    if ((threadIdx.x & 63) == 0 && b)
+        atomicAdd(cnt, (unsigned long long)__popcll(b));
+}
+
+// dst = sum of part[0..n) (fixed order) / max(count, 1)
+__global__ void __launch_bounds__(kBlock)
+    loss_reduce(const float* __restrict__ part, int n,
+                const unsigned long long* __restrict__ cnt,
+                float* __restrict__ dst)
+{
+    __shared__ double red[kBlock];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = kBlock / 2; k > 0; k >>= 1)
+    {
+        if (threadIdx.x < k)
+            red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        const unsigned long long c = *cnt;
+        *dst = (float)(red[0] / (double)(c ? c : 1ull));
+    }
+}
+
+// backward of the loss on a region equal to the (padded) volume; adds the
+// gradient into s->grad and the loss into *loss_dev
+int backward(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
+             unsigned long long* cnt)
+{
+    hipStream_t S = ctx->stream;
+    const int64_t V0 = level_voxels(s, 0);
+    const unsigned nb = blocks_for(V0);
+    VC_HIP(ctx, hipMemsetAsync(cnt, 0, 8, S));
+    count_occ<<<blocks_for(v->N), kBlock, 0, S>>>(v->occ, v->N, cnt);
+    VC_CHECK_LAUNCH(ctx);
+    loss_bwd<<<nb, kBlock, 0, S>>>(s->out, v->occ, v->lam2, v->nx, v->ny,
+                                    v->nz, s->X, s->Y, s->Z, cnt, s->dout,
+                                    s->part);
+    VC_CHECK_LAUNCH(ctx);
+    loss_reduce<<<1, kBlock, 0, S>>>(s->part, (int)nb, cnt, loss_dev);
+    VC_CHECK_LAUNCH(ctx);
+    constexpr int NP = kHeadIn * kHeadOut + kHeadOut;
+    head_bwd<<<nb, kBlock, 0, S>>>(s->act[13], s->dout,
+                                    s->theta + param_offset(14), s->gact[13],
+                                    V0, s->part);
+    VC_CHECK_LAUNCH(ctx);
+    colsum_reduce<<<1, 64, 0, S>>>(s->part, (int)nb, NP,
+                                    s->grad + param_offset(14));
+    VC_CHECK_LAUNCH(ctx);
+    int rc;
+    auto dims = [&](int lv, int& X, int& Y, int& Z) {
+        X = s->X >> lv;
+        Y = s->Y >> lv;
+        Z = s->Z >> lv;
+    };
+    int X, Y, Z;
+    // decoder levels 0, 1, 2: (second conv, first conv, concat input, its
+    // gradient, up channels, coarse gradient target, skip gradient target)
+    struct Dec
+    {
+        int lb, la, lv, C, C1;
+        bf16 *cat, *gcat, *gup, *gskip;
+    };
+    const Dec dec[3] = {
+        {13, 12, 0, 48, 32, s->cat0, s->gcat0, s->gact[11], s->gact[1]},
+        {11, 10, 1, 80, 48, s->cat1, s->gcat1, s->gact[9], s->gact[3]},
+        {9, 8, 2, 112, 64, s->cat2, s->gcat2, s->gact[7], s->gact[5]}};
+    for (const Dec& d : dec)
+    {
+        if ((rc = layer_back(ctx, s, d.lb, s->gact[d.lb], s->act[d.la],
+                             s->gact[d.la])) != VC_OK)
+            return rc;
+        if ((rc = layer_back(ctx, s, d.la, s->gact[d.la], d.cat, d.gcat)) !=
+            VC_OK)
+            return rc;
+        dims(d.lv, X, Y, Z);
+        const int64_t nu = level_voxels(s, d.lv + 1) * d.C1;
+        split_up<<<blocks_for(nu), kBlock, 0, S>>>(d.gcat, d.C, d.C1, d.gup, X,
+                                                    Y, Z);
+        VC_CHECK_LAUNCH(ctx);
+        const int64_t Vl = level_voxels(s, d.lv);
+        split_skip<<<blocks_for(Vl * (d.C - d.C1)), kBlock, 0, S>>>(
+            d.gcat, d.C, d.C1, d.gskip, Vl);
+        VC_CHECK_LAUNCH(ctx);
+    }
+    // encoder levels 3, 2, 1: the pooled input's gradient adds into the
+    // gradient of the level above's second conv output
+    bf16* pools[3] = {s->p1, s->p2, s->p3};
+    bf16* gpools[3] = {s->gp1, s->gp2, s->gp3};
+    const int pc[3] = {16, 32, 48};
+    for (int lv = 3; lv >= 1; --lv)
+    {
+        const int lb = 2 * lv + 1, la = 2 * lv;
+        if ((rc = layer_back(ctx, s, lb, s->gact[lb], s->act[la],
+                             s->gact[la])) != VC_OK)
+            return rc;
+        if ((rc = layer_back(ctx, s, la, s->gact[la], pools[lv - 1],
+                             gpools[lv - 1])) != VC_OK)
+            return rc;
+        dims(lv - 1, X, Y, Z);
+        const int64_t n = level_voxels(s, lv - 1) * pc[lv - 1];
+        pool_bwd_acc<<<blocks_for(n), kBlock, 0, S>>>(
+            s->gact[2 * lv - 1], gpools[lv - 1], pc[lv - 1], X, Y, Z);
+        VC_CHECK_LAUNCH(ctx);
+    }
+    if ((rc = layer_back(ctx, s, 1, s->gact[1], s->act[0], s->gact[0])) !=
+        VC_OK)
+        return rc;
+    return layer_back(ctx, s, 0, s->gact[0], s->x0, nullptr);
+}
+
+int round8(int n) { return (n + 7) / 8 * 8; }
+
+// forward + backward of one volume (gradient added into s->grad)
+int train_volume(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
+                 unsigned long long* cnt)
+{
+    int rc = ensure_ws(ctx, s, round8(v->nx), round8(v->ny), round8(v->nz),
+                       true);
+    if (rc != VC_OK)
+        return rc;
+    s->org[0] = s->org[1] = s->org[2] = 0;
+    s->pad[0] = s->X;
+    s->pad[1] = s->Y;
+    s->pad[2] = s->Z;
+    hipStream_t S = ctx->stream;
+    prep_input<<<blocks_for(level_voxels(s, 0)), kBlock, 0, S>>>(
+        v->occ, v->edt, v->nx, v->ny, v->nz, 0, 0, 0, s->x0, s->X, s->Y, s->Z);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = forward(ctx, s)) != VC_OK)
+        return rc;
+    return backward(ctx, s, v, loss_dev, cnt);
+}
+
+int check_labelled(vc_ctx* ctx, const vc_vol* v)
+{
+    if (!v->edt_done || !v->lam2 || v->measure_kind == 0)
+        return vc_fail(ctx, VC_ERR_STATE,
+                       "surrogate training needs vc_edt and vc_measure");
+    return VC_OK;
+}
+
+int ensure_vol_out(vc_ctx* ctx, vc_surrogate* s, int64_t N)
+{
+    if (s->vN == N)
+        return VC_OK;
+    VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    vc_free(s->vlam);
+    vc_free(s->vrel);
+    vc_free(s->vmeas);
+    s->vlam = s->vrel = s->vmeas = nullptr;
+    s->vN = 0;
+    int rc = vc_alloc(ctx, (void**)&s->vlam, 4 * (size_t)N);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->vrel, 4 * (size_t)N);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->vmeas, 4 * (size_t)N);
+    if (rc == VC_OK)
+        s->vN = N;
+    return rc;
+}
+
+// network outputs of a whole volume into s->vlam / s->vrel
+int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
+{
+    int rc = ensure_vol_out(ctx, s, v->N);
+    if (rc != VC_OK)
+        return rc;
+    hipStream_t S = ctx->stream;
+    const int dims[3] = {v->nx, v->ny, v->nz};
+    int amax = dims[0] > dims[1] ? dims[0] : dims[1];
+    amax = amax > dims[2] ? amax : dims[2];
+    const bool whole = tile_core <= 0 || tile_core >= amax;
+    const int T = whole ? 0 : round8(tile_core);
+    int R[3];
+    for (int a = 0; a < 3; ++a)
+        R[a] = whole ? round8(dims[a]) : T + 2 * kHalo;
+    if ((rc = ensure_ws(ctx, s, R[0], R[1], R[2], false)) != VC_OK)
+        return rc;
+    const int step[3] = {whole ? dims[0] : T, whole ? dims[1] : T,
+                         whole ? dims[2] : T};
+    const int halo = whole ? 0 : kHalo;
+    const int64_t VR = level_voxels(s, 0);
+    for (int tx = 0; tx < dims[0]; tx += step[0])
+        for (int ty = 0; ty < dims[1]; ty += step[1])
+            for (int tz = 0; tz < dims[2]; tz += step[2])
+            {
+                const int ox = tx - halo, oy = ty - halo, oz = tz - halo;
+                s->org[0] = ox;
+                s->org[1] = oy;
+                s->org[2] = oz;
+                for (int a = 0; a < 3; ++a)
+                    s->pad[a] = round8(dims[a]);
+                prep_input<<<blocks_for(VR), kBlock, 0, S>>>(
+                    v->occ, v->edt, v->nx, v->ny, v->nz, ox, oy, oz, s->x0,
+                    s->X, s->Y, s->Z);
+                VC_CHECK_LAUNCH(ctx);
+                if ((rc = forward(ctx, s)) != VC_OK)
+                    return rc;
+                copy_core<<<blocks_for(VR), kBlock, 0, S>>>(
+                    s->out, s->X, s->Y, s->Z, ox, oy, oz, tx, ty, tz,
+                    tx + step[0], ty + step[1], tz + step[2], s->vlam, s->vrel,
+                    v->nx, v->ny, v->nz);
+                VC_CHECK_LAUNCH(ctx);
+            }
+    return VC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int vc_surrogate_create(vc_ctx* ctx, uint64_t seed, vc_surrogate** out)
+{
+    if (!ctx || !out)
+        return VC_ERR_INVALID;
+    *out = nullptr;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    vc_surrogate* s = new (std::nothrow) vc_surrogate();
+    if (!s)
+        return VC_ERR_NOMEM;
+    s->ctx = ctx;
+    s->np = total_params();
+    int rc = VC_OK;
+    if (s->np != VC_SURROGATE_PARAMS)
+        rc = vc_fail(ctx, VC_ERR_INVALID, "parameter count %lld",
+                     (long long)s->np);
+    const size_t pb = 4 * (size_t)s->np;
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->theta, pb);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->grad, pb);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->m1, pb);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->m2, pb);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->ctr, 64 * 8);
+    if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->lossbuf, 4 * 64);
+    for (int l = 0; l < 14 && rc == VC_OK; ++l)
+    {
+        const LayerSpec& L = kLayers[l];
+        const int cinp = cinp_of(L.cin);
+        s->kc[l] = round4(27 * cinp / 8);
+        rc = vc_alloc(ctx, (void**)&s->Wf[l], 2 * (size_t)s->kc[l] * L.cout * 8);
+        if (rc == VC_OK && l > 0)
+        {
+            s->kcd[l] = round4(27 * L.cout / 8);
+            rc = vc_alloc(ctx, (void**)&s->Wd[l],
+                          2 * (size_t)s->kcd[l] * cinp * 8);
+        }
+    }
+    if (rc == VC_OK)
+    {
+        // He-uniform initial weights (oracle/unet_ref.py init_params)
+        std::vector<float> th((size_t)s->np, 0.0f);
+        int64_t o = 0;
+        for (int l = 0; l <= 14; ++l)
+        {
+            const bool head = l == 14;
+            const int ci = head ? kHeadIn : kLayers[l].cin;
+            const int co = head ? kHeadOut : kLayers[l].cout;
+            const int taps = head ? 1 : 27;
+            const int64_t n = (int64_t)taps * ci * co;
+            float a = sqrtf(6.0f / (float)(taps * ci));
+            if (head)
+                a = a * 0.1f;
+            for (int64_t j = o; j < o + n; ++j)
+            {
+                const uint64_t r =
+                    splitmix64(((seed & 0xFFFFFFFFull) << 32) + (uint64_t)j);
+                const float u = (float)(r >> 40) * (1.0f / 16777216.0f);
+                th[(size_t)j] = (2.0f * u - 1.0f) * a;
+            }
+            o += n + co;
+        }
+        hipError_t e = hipMemcpy(s->theta, th.data(), pb, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemset(s->m1, 0, pb);
+        if (e == hipSuccess)
+            e = hipMemset(s->m2, 0, pb);
+        if (e != hipSuccess)
+            rc = vc_fail(ctx, VC_ERR_HIP, "surrogate init: %s",
+                         hipGetErrorString(e));
+    }
+    if (rc == VC_OK)
+        rc = pack_weights(ctx, s);
+    if (rc != VC_OK)
+    {
+        vc_surrogate_destroy(ctx, s);
+        return rc;
+    }
+    *out = s;
+    return VC_OK;
+}
+
+int vc_surrogate_destroy(vc_ctx* ctx, vc_surrogate* s)
+{
+    if (!ctx || !s)
+        return VC_ERR_INVALID;
+    (void)hipStreamSynchronize(ctx->stream);
+    free_ws(s);
+    vc_free(s->theta);
+    vc_free(s->grad);
+    vc_free(s->m1);
+    vc_free(s->m2);
+    vc_free(s->ctr);
+    vc_free(s->lossbuf);
+    vc_free(s->vlam);
+    vc_free(s->vrel);
+    vc_free(s->vmeas);
+    for (int l = 0; l < 14; ++l)
+    {
+        vc_free(s->Wf[l]);
+        vc_free(s->Wd[l]);
+    }
+    delete s;
+    return VC_OK;
+}
+
+int vc_surrogate_get_params(vc_ctx* ctx, vc_surrogate* s, float* host,
+                            int64_t n)
+{
+    if (!ctx || !s || !host || n != s->np)
+        return VC_ERR_INVALID;
+    VC_HIP(ctx, hipMemcpyAsync(host, s->theta, 4 * (size_t)n,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VC_OK;
+}
+
+int vc_surrogate_set_params(vc_ctx* ctx, vc_surrogate* s, const float* host,
+                            int64_t n)
+{
+    if (!ctx || !s || !host || n != s->np)
+        return VC_ERR_INVALID;
+    VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    VC_HIP(ctx, hipMemcpy(s->theta, host, 4 * (size_t)n,
+                          hipMemcpyHostToDevice));
+    VC_HIP(ctx, hipMemset(s->m1, 0, 4 * (size_t)n));
+    VC_HIP(ctx, hipMemset(s->m2, 0, 4 * (size_t)n));
+    s->step = 0;
+    return pack_weights(ctx, s);
+}
+
+int vc_surrogate_save(vc_ctx* ctx, vc_surrogate* s, const char* path)
+{
+    if (!ctx || !s || !path)
+        return VC_ERR_INVALID;
+    std::vector<float> th((size_t)s->np);
+    int rc = vc_surrogate_get_params(ctx, s, th.data(), s->np);
+    if (rc != VC_OK)
+        return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f)
+        return vc_fail(ctx, VC_ERR_INVALID, "cannot write %s", path);
+    const int64_t n = s->np;
+    bool ok = std::fwrite("VCUNET01", 1, 8, f) == 8 &&
+              std::fwrite(&n, 8, 1, f) == 1 &&
+              std::fwrite(th.data(), 4, th.size(), f) == th.size();
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? VC_OK : vc_fail(ctx, VC_ERR_INVALID, "short write to %s", path);
+}
+
+int vc_surrogate_load(vc_ctx* ctx, vc_surrogate* s, const char* path)
+{
+    if (!ctx || !s || !path)
+        return VC_ERR_INVALID;
+    FILE* f = std::fopen(path, "rb");
+    if (!f)
+        return vc_fail(ctx, VC_ERR_INVALID, "cannot read %s", path);
+    char magic[8];
+    int64_t n = 0;
+    std::vector<float> th((size_t)s->np);
+    bool ok = std::fread(magic, 1, 8, f) == 8 &&
+              std::memcmp(magic, "VCUNET01", 8) == 0 &&
+              std::fread(&n, 8, 1, f) == 1 && n == s->np &&
+              std::fread(th.data(), 4, th.size(), f) == th.size();
+    std::fclose(f);
+    if (!ok)
+        return vc_fail(ctx, VC_ERR_INVALID, "%s is not a weights file of this "
+                                            "network", path);
+    return vc_surrogate_set_params(ctx, s, th.data(), s->np);
+}
+
+int vc_surrogate_gradients(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
+                           float* grads, float* loss)
+{
+    if (!ctx || !s || !vol || vol->ctx != ctx)
+        return VC_ERR_INVALID;
+    int rc = check_labelled(ctx, vol);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    fill_f32<<<blocks_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = train_volume(ctx, s, vol, s->lossbuf, s->ctr)) != VC_OK)
+        return rc;
+    if (grads)
+        VC_HIP(ctx, hipMemcpyAsync(grads, s->grad, 4 * (size_t)s->np,
+                                   hipMemcpyDeviceToHost, S));
+    float l = 0.0f;
+    VC_HIP(ctx, hipMemcpyAsync(&l, s->lossbuf, 4, hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    if (loss)
+        *loss = l;
+    return VC_OK;
+}
+
+int vc_surrogate_train_step(vc_ctx* ctx, vc_surrogate* s, vc_vol* const* vols,
+                            int n, float lr, float* loss)
+{
+    if (!ctx || !s || !vols || n < 1 || n > 64 || !(lr > 0.0f))
+        return VC_ERR_INVALID;
+    for (int i = 0; i < n; ++i)
+    {
+        if (!vols[i] || vols[i]->ctx != ctx)
+            return VC_ERR_INVALID;
+        int rc = check_labelled(ctx, vols[i]);
+        if (rc != VC_OK)
+            return rc;
+    }
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    fill_f32<<<blocks_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
+    VC_CHECK_LAUNCH(ctx);
+    for (int i = 0; i < n; ++i)
+    {
+        int rc = train_volume(ctx, s, vols[i], s->lossbuf + i, s->ctr + i);
+        if (rc != VC_OK)
+            return rc;
+    }
+    s->step++;
+    const float bc1 = 1.0f - powf(0.9f, (float)s->step);
+    const float bc2 = 1.0f - powf(0.999f, (float)s->step);
+    adam_step<<<blocks_for(s->np), kBlock, 0, S>>>(s->theta, s->grad, s->m1,
+                                                    s->m2, s->np, lr,
+                                                    1.0f / (float)n, bc1, bc2);
+    VC_CHECK_LAUNCH(ctx);
+    int rc = pack_weights(ctx, s);
+    if (rc != VC_OK)
+        return rc;
+    if (loss)
+    {
+        float l[64];
+        VC_HIP(ctx, hipMemcpyAsync(l, s->lossbuf, 4 * (size_t)n,
+                                   hipMemcpyDeviceToHost, S));
+        VC_HIP(ctx, hipStreamSynchronize(S));
+        double t = 0.0;
+        for (int i = 0; i < n; ++i)
+            t += l[i];
+        *loss = (float)(t / n);
+    }
+    return VC_OK;
+}
+
+int vc_surrogate_infer(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
+                       int tile_core, float* lambda_out, float* relerr_out)
+{
+    if (!ctx || !s || !vol || vol->ctx != ctx)
+        return VC_ERR_INVALID;
+    if (!vol->edt_done)
+        return vc_fail(ctx, VC_ERR_STATE, "surrogate inference needs vc_edt");
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = run_infer(ctx, s, vol, tile_core);
+    if (rc != VC_OK)
+        return rc;
+    hipStream_t S = ctx->stream;
+    if (lambda_out)
+        VC_HIP(ctx, hipMemcpyAsync(lambda_out, s->vlam, 4 * (size_t)vol->N,
+                                   hipMemcpyDeviceToHost, S));
+    if (relerr_out)
+        VC_HIP(ctx, hipMemcpyAsync(relerr_out, s->vrel, 4 * (size_t)vol->N,
+                                   hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+int vc_surrogate_measure(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
+                         float max_rel_err, int tile_core,
+                         vc_surrogate_stats* stats)
+{
+    if (!ctx || !s || !vol || vol->ctx != ctx || !(max_rel_err >= 0.0f))
+        return VC_ERR_INVALID;
+    if (!vol->edt_done)
+        return vc_fail(ctx, VC_ERR_STATE, "surrogate measure needs vc_edt");
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; ++k)
+        VC_HIP(ctx, hipEventCreate(&ev[k]));
+    int rc = VC_OK;
+    hipError_t e = hipEventRecord(ev[0], S);
+    if (e == hipSuccess && (rc = run_infer(ctx, s, vol, tile_core)) == VC_OK)
+    {
+        e = hipEventRecord(ev[1], S);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(s->ctr, 0, 3 * 8, S);
+        if (e == hipSuccess)
+        {
+            gate_count<<<blocks_for(vol->N), kBlock, 0, S>>>(
+                vol->occ, s->vrel, max_rel_err, vol->N, s->ctr);
+            e = hipGetLastError();
+        }
+        unsigned long long hc[3] = {0, 0, 0};
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hc, s->ctr, 3 * 8, hipMemcpyDeviceToHost, S);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(S);
+        if (e == hipSuccess)
+            e = hipEventRecord(ev[2], S);
+        if (e == hipSuccess && hc[1] > 0 && vol->measure_kind != 1)
+        {
+            // the exact measure for the fallback voxels
+            rc = vc_measure(ctx, vol);
+        }
+        if (e == hipSuccess && rc == VC_OK)
+            e = hipEventRecord(ev[3], S);
+        if (e == hipSuccess && rc == VC_OK)
+        {
+            gate_mix<<<blocks_for(vol->N), kBlock, 0, S>>>(
+                vol->occ, s->vlam, s->vrel, hc[1] > 0 ? vol->lam2 : nullptr,
+                max_rel_err, vol->N, s->vmeas);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && rc == VC_OK)
+            rc = vc_set_measure_f32(ctx, vol, s->vmeas);
+        hipEvent_t e_end;
+        if (e == hipSuccess && rc == VC_OK && stats &&
+            (e = hipEventCreate(&e_end)) == hipSuccess)
+        {
+            e = hipEventRecord(e_end, S);
+            if (e == hipSuccess)
+                e = hipEventSynchronize(e_end);
+            float t01 = 0, t23 = 0, t3e = 0;
+            if (e == hipSuccess)
+                e = hipEventElapsedTime(&t01, ev[0], ev[1]);
+            if (e == hipSuccess)
+                e = hipEventElapsedTime(&t23, ev[2], ev[3]);
+            if (e == hipSuccess)
+                e = hipEventElapsedTime(&t3e, ev[3], e_end);
+            (void)hipEventDestroy(e_end);
+            stats->inside = (int64_t)hc[2];
+            stats->accepted = (int64_t)hc[0];
+            stats->fallback = (int64_t)hc[1];
+            stats->ms_infer = t01;
+            stats->ms_exact = t23;
+            stats->ms_mix = t3e;
+        }
+    }
+    for (int k = 0; k < 4; ++k)
+        (void)hipEventDestroy(ev[k]);
+    if (rc != VC_OK)
+        return rc;
+    if (e != hipSuccess)
+        return vc_fail(ctx, VC_ERR_HIP, "surrogate measure: %s",
+                       hipGetErrorString(e));
+    return VC_OK;
+}
+
+int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
+                            float* host, int64_t n)
+{
+    if (!ctx || !s || !host || which < 0 || which > 14 || s->X == 0)
+        return VC_ERR_INVALID;
+    const int64_t V =
+        which == 14 ? level_voxels(s, 0) : level_voxels(s, kLayers[which].level);
+    const int C = which == 14 ? 2 : kLayers[which].cout;
+    if (n != V * C)
+        return vc_fail(ctx, VC_ERR_INVALID, "activation %d has %lld values",
+                       which, (long long)(V * C));
+    hipStream_t S = ctx->stream;
+    if (which == 14)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(host, s->out, 4 * (size_t)n,
+                                   hipMemcpyDeviceToHost, S));
+    }
+    else
+    {
+        float* tmp = nullptr;
+        int rc = vc_alloc(ctx, (void**)&tmp, 4 * (size_t)n);
+        if (rc != VC_OK)
+            return rc;
+        bf16_to_f32<<<blocks_for(n), kBlock, 0, S>>>(s->act[which], tmp, n);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host, tmp, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                               S);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(S);
+        vc_free(tmp);
+        if (e != hipSuccess)
+            return vc_fail(ctx, VC_ERR_HIP, "activation: %s",
+                           hipGetErrorString(e));
+    }
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,95 @@
+"""Training and evaluation of the measure surrogate (include/vc_surrogate.h).
+
+Training volumes are generated on the device (torus / ball unions,
+synth.training_specs), labelled by the plain path (vc_edt + vc_measure: the
+target is the exact lambda = sqrt(lam2)), and the network is trained by the
+library's own forward / backward kernels with Adam.
+
+    python -m voxel_ma_amd.surrogate --steps 400 --out voxel_ma_amd/data/unet.bin
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(PKG, "data", "unet.bin")
+
+
+def make_volume(ctx, L, rng):
+    from voxel_ma_amd import synth
+    prims = synth.training_specs(L, rng)
+    vol = ctx.solids_volume((L, L, L), prims)
+    vol.edt().measure()
+    return vol
+
+
+def train(ctx, net, steps, batch=2, L=128, lr=1e-3, seed=5, fresh_every=1, log=None):
+    """`steps` Adam steps, each on `batch` new volumes of edge L (a fresh
+    volume pool every `fresh_every` steps).  Returns the loss history."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    pool = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if k % fresh_every == 0:
+            for v in pool:
+                v.release()
+            pool = [make_volume(ctx, L, rng) for _ in range(batch)]
+        lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / max(steps, 1)))  # cosine decay
+        loss = net.train_step(pool, max(lr_k, lr * 0.02))
+        hist.append(loss)
+        if log and (k % 20 == 0 or k == steps - 1):
+            log("step %d loss %.5f (%.1f s)" % (k, loss, time.perf_counter() - t0))
+    for v in pool:
+        v.release()
+    return hist
+
+
+def evaluate(ctx, net, vols, max_rel_err=0.01):
+    """Relative error of the surrogate lambda against the exact measure
+    (sqrt(lam2) in float64) on inside voxels, and the gate statistics."""
+    import voxel_ma_amd as V
+    errs, acc, fb = [], 0, 0
+    for v in vols:
+        lam, rel = net.infer(v)
+        occ = v.get(V.BUF_OCC) > 0
+        exact = np.sqrt(v.get(V.BUF_LAM2).astype(np.float64))
+        e = np.abs(lam.astype(np.float64) - exact) / np.maximum(exact, 1.0)
+        errs.append(e[occ])
+        ok = rel[occ] <= max_rel_err
+        acc += int(ok.sum())
+        fb += int((~ok).sum())
+    e = np.concatenate(errs) if errs else np.zeros(0)
+    return {"median_rel_err": float(np.median(e)) if e.size else 0.0,
+            "p90_rel_err": float(np.quantile(e, 0.9)) if e.size else 0.0,
+            "frac_within_1pct": float((e <= 0.01).mean()) if e.size else 0.0,
+            "accepted": acc, "fallback": fb}
+
+
+def main():
+    import voxel_ma_amd as V
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args()
+    ctx = V.Context(0)
+    net = V.Surrogate(ctx, seed=a.seed)
+    hist = train(ctx, net, a.steps, a.batch, a.size, a.lr, a.seed,
+                 log=lambda m: print("[train]", m, file=sys.stderr, flush=True))
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    net.save(a.out)
+    rng = np.random.default_rng(a.seed + 1000)
+    held = [make_volume(ctx, a.size, rng) for _ in range(4)]
+    ev = evaluate(ctx, net, held)
+    print({"loss_first": hist[0], "loss_last": hist[-1], "held_out": ev}, flush=True)
+
+
+if __name__ == "__main__":
+    main()

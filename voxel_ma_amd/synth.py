@@ -199,6 +199,9 @@ def solids_host(L, prims):
         if r[0] == 0.0:
             t0, t1, t2 = q[0] / r[13], q[1] / r[14], q[2] / r[15]
             occ |= ((t0 * t0 + t1 * t1) + t2 * t2) <= 1.0
+        elif r[0] == 2.0:
+            rho = np.sqrt(q[0] * q[0] + q[1] * q[1]) - r[13]
+            occ |= (rho * rho + q[2] * q[2]) <= r[14] * r[14]
         else:
             h = r[14]
             e = q[0] - np.clip(q[0], -h, h)
@@ -208,6 +211,33 @@ def solids_host(L, prims):
     occ[:, 0] = occ[:, -1] = 0
     occ[:, :, 0] = occ[:, :, -1] = 0
     return occ
+
+
+def training_specs(L, rng):
+    """One training volume of edge L in the style of the torus and spheres
+    workloads, rescaled: with probability 1/2 a union of 1-3 tori (major
+    radius U[0.12, 0.3] L, minor U[0.04, 0.11] L, random rotation), else a
+    union of 4-16 balls (radius U[0.05, 0.25] L); centres uniform in the
+    middle 60% of the grid."""
+    out = []
+    lo, hi = 0.2 * (L - 1), 0.8 * (L - 1)
+    if rng.random() < 0.5:
+        for _ in range(int(rng.integers(1, 4))):
+            r = np.zeros(SOLID_WORDS)
+            r[0] = 2.0
+            r[1:4] = rng.uniform(lo, hi, size=3)
+            r[4:13] = _rotation(rng).ravel()
+            r[13] = rng.uniform(0.12, 0.3) * L
+            r[14] = rng.uniform(0.04, 0.11) * L
+            out.append(r)
+    else:
+        for _ in range(int(rng.integers(4, 17))):
+            r = np.zeros(SOLID_WORDS)
+            r[1:4] = rng.uniform(lo, hi, size=3)
+            r[4:13] = np.eye(3).ravel()
+            r[13:16] = rng.uniform(0.05, 0.25) * L
+            out.append(r)
+    return np.array(out)
 
 
 def batch_c4_specs(count=256, seed=4, lmin=32, ratio=6.0):
