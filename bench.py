@@ -10,7 +10,10 @@ resident in HBM.  Prints one JSON line (rank 0).
 
 The JSON line also carries "batch": the batched-solids workload (256 cubic
 volumes of edge 32-192, one launch per pass over all of them, autotuned
-schedule), timed the same way.
+schedule), timed the same way, and "surrogate": the same 512^3 step with the
+learned measure (gate at 1% predicted relative error, exact fallback), its
+error against the exact measure and the Euler characteristics of both
+skeletons.
 """
 import argparse
 import ctypes
@@ -169,6 +172,80 @@ def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
     return out
 
 
+def surrogate_leg(ctx, ev, vol, levels, n, steps, plain_ms, tau=0.01):
+    """Surrogate-guided pruning on the same volume: EDT + surrogate measure
+    (inference, gate at relative error tau, exact fallback) + thinning,
+    against the plain step; quality against the exact measure."""
+    import numpy as np
+    import voxel_ma_amd
+    from voxel_ma_amd import surrogate
+
+    net = voxel_ma_amd.Surrogate(ctx, seed=1, path=surrogate.WEIGHTS)
+    stream = ctx.stream()
+    N = n ** 3
+
+    def step():
+        vol.edt()
+        st = net.measure(vol, max_rel_err=tau)
+        vol.thin_levels(levels, stats=False)
+        return st
+
+    step()
+    ctx.synchronize()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = step()
+    ev.record(e1, stream)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    # quality: surrogate outputs vs the exact measure (float64), decisions,
+    # and the skeletons of the exact and the pure-surrogate measure
+    lam, rel = net.infer(vol)
+    vol.measure()
+    lam2 = vol.get(voxel_ma_amd.BUF_LAM2)
+    occ = vol.get(voxel_ma_amd.BUF_OCC) > 0
+    exact = np.sqrt(lam2.astype(np.float64))
+    err = (np.abs(lam.astype(np.float64) - exact) / np.maximum(exact, 1.0))[occ]
+    T = np.array(levels, np.float32)
+    agree = float((np.searchsorted(T, np.maximum(lam, 0), side="right")[occ] ==
+                   np.searchsorted(T, np.sqrt(lam2.astype(np.float32)), side="right")[occ]).mean())
+    del lam, rel, exact, lam2
+    vol.thin_levels(levels, stats=False)
+    sk_e = (vol.get(voxel_ma_amd.BUF_LABEL) == 1).astype(np.uint8)
+    net.measure(vol, max_rel_err=3.0e38)
+    vol.thin_levels(levels, stats=False)
+    sk_s = (vol.get(voxel_ma_amd.BUF_LABEL) == 1).astype(np.uint8)
+    iou = float((sk_e & sk_s).sum() / max(int((sk_e | sk_s).sum()), 1))
+    eul = []
+    for sk in (sk_e, sk_s):
+        v2 = ctx.upload(sk)
+        eul.append(int(v2.boundary(with_euler=True)[2]["chi"]))
+        v2.release()
+    eul.append(int(vol.boundary(with_euler=True)[2]["chi"]))
+    infer_ms = st.ms_infer
+    tflops = 116e3 * N / (infer_ms * 1e-3) / 1e12
+    net.release()
+    return {
+        "tau_rel_err": tau, "weights": "voxel_ma_amd/data/unet.bin",
+        "ms_per_step": round(ms, 3), "plain_ms_per_step": round(plain_ms, 3),
+        "speedup_vs_plain": round(plain_ms / ms, 4),
+        "accepted": int(st.accepted), "fallback": int(st.fallback),
+        "accepted_frac": round(st.accepted / max(st.inside, 1), 6),
+        "ms_infer": round(infer_ms, 3), "ms_exact_fallback": round(st.ms_exact, 3),
+        "rel_err_vs_exact_fp64": {"median": round(float(np.median(err)), 4),
+                                  "p90": round(float(np.quantile(err, 0.9)), 4),
+                                  "frac_within_1pct": round(float((err <= 0.01).mean()), 4)},
+        "level_decision_agreement": round(agree, 4),
+        "skeleton_iou_pure_surrogate": round(iou, 4),
+        "euler_chi": {"exact_skeleton": eul[0], "surrogate_skeleton": eul[1], "object": eul[2]},
+        "roofline_infer": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0,
+                           "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4),
+                           "flop_per_voxel": 116000},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +257,8 @@ def main():
                     help="skip the batched-solids leg (256 volumes)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched-solids leg (for profiling)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate-guided pruning leg")
     ap.add_argument("--profile-steps", action="store_true",
                     help="no CPU baseline, no hashing (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -303,6 +382,10 @@ def main():
                                                       ("edt_pass_z", "edt_pass_y",
                                                        "edt_pass_x")))) or None},
     }
+    if not (a.no_surrogate or a.profile_steps):
+        print("[bench] surrogate-guided pruning", file=sys.stderr, flush=True)
+        out["surrogate"] = surrogate_leg(ctx, ev, vol, levels, n, max(1, a.steps // 2),
+                                         ms_step)
     vol.release()
     if not (a.no_batch or a.profile_steps):
         out["batch"] = batch_leg(ctx, ev, a.steps, a.warmup)

@@ -133,6 +133,10 @@ typedef struct vc_euler {
     int64_t V, E, F, T, C, euler;
     /* parts: boundary V/E with multiplicity, interior V/E/F */
     int64_t V_bnd, E_bnd, V_int, E_int, F_int;
+    /* Euler characteristic of the union of the closed voxel cubes (the
+     * topological invariant; equals euler wherever the reference's vertex
+     * multiplicity heuristic holds, i.e. away from non-manifold contacts) */
+    int64_t chi;
 } vc_euler;
 /* Boundary sites (voxel corners shared by a boundary face) as 3 floats each
  * and boundary quads as 4 int32 site indices each, both in the reference

@@ -1,7 +1,8 @@
 """GPU parity: boundary sites, boundary quads and the cube-complex Euler
 characteristic through the C ABI against the oracle's literal restatement of
 Surfacer::extractBoundaryVtsAndFaces + computeEulerChar (bit-exact, same
-order)."""
+order), plus the cubical-complex Euler characteristic against the oracle's
+vertex / edge / face / cube count."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,8 @@ def _compare(vc, oracle, occ, name):
     assert q.shape == qo.shape and np.array_equal(q, qo), name
     for k in KEYS:
         assert e[k] == eo[k], (name, k, e[k], eo[k])
+    # the cubical-complex Euler characteristic (topological invariant)
+    assert e["chi"] == oracle.euler_voxels(occ), (name, e["chi"])
     return e
 
 
@@ -61,6 +64,7 @@ def test_boundary_config2_spheres(vc, oracle):
     # per-vertex multiplicity heuristic (src/surfacing.cpp:333-358) departs
     # from the cubical-complex chi (21 vs 19 here); parity is with the former
     assert e["T"] == int(occ.sum())
+    assert e["chi"] == oracle.euler_voxels(occ) != e["euler"]
 
 
 def test_boundary_capacity_error(vc):

@@ -62,7 +62,8 @@ class ThinStats(ctypes.Structure):
 
 class Euler(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
-                ("V", "E", "F", "T", "C", "euler", "V_bnd", "E_bnd", "V_int", "E_int", "F_int")]
+                ("V", "E", "F", "T", "C", "euler", "V_bnd", "E_bnd", "V_int", "E_int", "F_int",
+                 "chi")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -367,9 +367,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // ------------------------------------------------------------- Euler terms
 // accumulators: [0] V_bnd, [1] E_bnd, [2] Vint_sum, [3] E_int, [4] F_int,
-// [5] T, spread over 64 slots
+// [5] T, [6] thin-layer components, [7] union-find guard, [8] / [9] positive
+// / negative terms of the cubical Euler characteristic, spread over 64 slots
 constexpr int kSlots = 64;
-constexpr int kAcc = 8;
+constexpr int kAcc = 10;
 
 __device__ __forceinline__ void block_add(unsigned long long v,
                                           unsigned long long* sh,
@@ -400,6 +401,40 @@ __device__ __forceinline__ int occ_at(const uint8_t* occ, const Dims& D, int x,
         (unsigned)z >= (unsigned)D.nz)
         return 0;
     return occ[((int64_t)x * D.ny + y) * D.nz + z] != 0;
+}
+
+// Euler characteristic of the union of the closed voxel cubes (cubical
+// complex, the topological invariant of the 26-connected object): per
+// lattice point p, with m the inside bits of the 8 voxels p + (a, b, c),
+// a, b, c in {-1, 0} (bit (a+1)*4 + (b+1)*2 + (c+1)): the vertex p, the
+// edges and faces leaving p towards +x, +y, +z and the voxel at p each exist
+// iff one of their incident voxels is inside.
+__global__ void __launch_bounds__(kBlock)
+    chi_lattice(const uint8_t* __restrict__ occ, Dims D,
+                unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[kBlock / 64];
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, x = blockIdx.z;
+    unsigned long long pos = 0, neg = 0;
+    if (z <= D.nz)
+    {
+        uint32_t m = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            m |= (uint32_t)occ_at(occ, D, x + (q >> 2) - 1,
+                                  y + ((q >> 1) & 1) - 1, z + (q & 1) - 1)
+                 << q;
+        pos = (m != 0) + ((m & 0xC0u) != 0) + ((m & 0xA0u) != 0) +
+              ((m & 0x88u) != 0);
+        neg = ((m & 0xF0u) != 0) + ((m & 0xCCu) != 0) + ((m & 0xAAu) != 0) +
+              ((m & 0x80u) != 0);
+    }
+    const int slot = (int)((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y *
+                                                       (uint64_t)blockIdx.z)) %
+                           kSlots);
+    block_add(pos, sh, &acc[slot * kAcc + 8]);
+    block_add(neg, sh, &acc[slot * kAcc + 9]);
 }
 
 // per corner-lattice point (cx, cy, cz): the boundary-vertex multiplicity if
@@ -802,6 +837,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
                               (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
                 euler_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
+                chi_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, acc);
                 const dim3 gv((unsigned)((v->nz + kBlock - 1) / kBlock),
                               (unsigned)v->ny, (unsigned)v->nx);
                 euler_voxels<<<gv, kBlock, 0, S>>>(v->occ, D, acc);
@@ -839,6 +875,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 E.E = E.E_bnd + E.E_int;
                 E.F = nf + E.F_int;
                 E.euler = (int64_t)(int32_t)(E.V - E.E + E.F - E.T);
+                E.chi = s[8] - s[9];
             }
             vc_free(acc);
             vc_free(dtbl);

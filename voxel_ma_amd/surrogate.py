@@ -48,25 +48,59 @@ def train(ctx, net, steps, batch=2, L=128, lr=1e-3, seed=5, fresh_every=1, log=N
     return hist
 
 
+def levels_of(occ):
+    from voxel_ma_amd import synth
+    return synth.levels_from_tt(synth.inside_size(occ))
+
+
 def evaluate(ctx, net, vols, max_rel_err=0.01):
-    """Relative error of the surrogate lambda against the exact measure
-    (sqrt(lam2) in float64) on inside voxels, and the gate statistics."""
+    """Quality of the surrogate against the exact measure on volumes whose
+    vc_edt and vc_measure have run:
+      relative error |lambda_s - lambda| / max(lambda, 1), lambda =
+      sqrt(lam2) in float64, on inside voxels (median, 90th percentile,
+      fraction within 1%);
+      accepted / fallback voxels of the gate at max_rel_err;
+      decision agreement: fraction of inside voxels whose first eligible
+      level (the -tt levels of the volume) is the same under the surrogate
+      measure as under the exact one;
+      skeleton IoU at the last level between thinning with the surrogate
+      measure everywhere and thinning with the exact measure (both exact
+      thinnings, so both keep the topology of the object)."""
     import voxel_ma_amd as V
-    errs, acc, fb = [], 0, 0
+    errs, acc, fb, agree, n_in, inter, union = [], 0, 0, 0, 0, 0, 0
     for v in vols:
-        lam, rel = net.infer(v)
         occ = v.get(V.BUF_OCC) > 0
-        exact = np.sqrt(v.get(V.BUF_LAM2).astype(np.float64))
+        lam2 = v.get(V.BUF_LAM2)
+        levels = levels_of(occ)
+        lam, rel = net.infer(v)
+        exact = np.sqrt(lam2.astype(np.float64))
         e = np.abs(lam.astype(np.float64) - exact) / np.maximum(exact, 1.0)
         errs.append(e[occ])
         ok = rel[occ] <= max_rel_err
         acc += int(ok.sum())
         fb += int((~ok).sum())
+        T = np.array([np.float32(l) for l in levels], np.float32)
+        lvl_s = np.searchsorted(T, np.maximum(lam, 0.0), side="right")
+        lvl_e = np.searchsorted(T, np.sqrt(lam2.astype(np.float32)), side="right")
+        agree += int((lvl_s[occ] == lvl_e[occ]).sum())
+        n_in += int(occ.sum())
+        # skeletons: exact measure, then surrogate measure everywhere
+        v.measure()
+        v.thin_levels(levels, stats=False)
+        sk_e = v.get(V.BUF_LABEL) == 1
+        net.measure(v, max_rel_err=3.0e38)
+        v.thin_levels(levels, stats=False)
+        sk_s = v.get(V.BUF_LABEL) == 1
+        inter += int((sk_e & sk_s).sum())
+        union += int((sk_e | sk_s).sum())
+        v.measure()  # leave the exact measure installed
     e = np.concatenate(errs) if errs else np.zeros(0)
     return {"median_rel_err": float(np.median(e)) if e.size else 0.0,
             "p90_rel_err": float(np.quantile(e, 0.9)) if e.size else 0.0,
             "frac_within_1pct": float((e <= 0.01).mean()) if e.size else 0.0,
-            "accepted": acc, "fallback": fb}
+            "accepted": acc, "fallback": fb,
+            "decision_agreement": agree / max(n_in, 1),
+            "skeleton_iou": inter / max(union, 1)}
 
 
 def main():
