@@ -1,11 +1,15 @@
 """Training and evaluation of the measure surrogate (include/vc_surrogate.h).
 
-Training volumes are generated on the device (torus / ball unions,
-synth.training_specs), labelled by the plain path (vc_edt + vc_measure: the
-target is the exact lambda = sqrt(lam2)), and the network is trained by the
-library's own forward / backward kernels with Adam.
+Training volumes are generated on the device (torus / ball unions in the
+style of the torus and spheres workloads, synth.training_specs; optionally
+blurred-noise volumes at 128^3), labelled by the plain path (vc_edt +
+vc_measure: the target is the exact lambda = sqrt(lam2)), and the network is
+trained by the library's own forward / backward kernels with Adam.  The
+committed weights (4,000 volumes of 128^3, about one minute on one MI355X)
+come from
 
-    python -m voxel_ma_amd.surrogate --steps 400 --out voxel_ma_amd/data/unet.bin
+    python -m voxel_ma_amd.surrogate --steps 2000 --batch 2 --size 128 \
+        --lr 1e-3 --seed 5 --blur-frac 0 --out voxel_ma_amd/data/unet.bin
 """
 import argparse
 import os
@@ -18,17 +22,29 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(PKG, "data", "unet.bin")
 
 
-def make_volume(ctx, L, rng):
+def make_volume(ctx, L, rng, blur_frac=0.0):
+    """One labelled training volume of edge L: with probability blur_frac a
+    blurred-noise volume (sigma 12 voxels, fill 0.30, random seed >= 1000 --
+    the benchmark volume uses seed 3), else a torus / ball union
+    (synth.training_specs); vc_edt + vc_measure run on it."""
     from voxel_ma_amd import synth
-    prims = synth.training_specs(L, rng)
-    vol = ctx.solids_volume((L, L, L), prims)
+    if rng.random() < blur_frac:
+        buf, _ = ctx.synth_blurred_noise((L, L, L), sigma=12.0, fill=0.30,
+                                         seed=int(rng.integers(1000, 1 << 40)))
+        vol = ctx.from_device(buf.ptr, (L, L, L))
+        ctx.synchronize()
+        buf.free()
+    else:
+        vol = ctx.solids_volume((L, L, L), synth.training_specs(L, rng))
     vol.edt().measure()
     return vol
 
 
-def train(ctx, net, steps, batch=2, L=128, lr=1e-3, seed=5, fresh_every=1, log=None):
+def train(ctx, net, steps, batch=2, L=128, lr=1e-3, seed=5, fresh_every=1, log=None,
+          blur_frac=0.0):
     """`steps` Adam steps, each on `batch` new volumes of edge L (a fresh
-    volume pool every `fresh_every` steps).  Returns the loss history."""
+    volume pool every `fresh_every` steps; blur_frac of them blurred noise).
+    Returns the loss history."""
     rng = np.random.default_rng(seed)
     hist = []
     pool = []
@@ -37,7 +53,7 @@ def train(ctx, net, steps, batch=2, L=128, lr=1e-3, seed=5, fresh_every=1, log=N
         if k % fresh_every == 0:
             for v in pool:
                 v.release()
-            pool = [make_volume(ctx, L, rng) for _ in range(batch)]
+            pool = [make_volume(ctx, L, rng, blur_frac) for _ in range(batch)]
         lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / max(steps, 1)))  # cosine decay
         loss = net.train_step(pool, max(lr_k, lr * 0.02))
         hist.append(loss)
@@ -106,23 +122,28 @@ def evaluate(ctx, net, vols, max_rel_err=0.01):
 def main():
     import voxel_ma_amd as V
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--blur-frac", type=float, default=0.0,
+                    help="fraction of blurred-noise training volumes")
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args()
     ctx = V.Context(0)
     net = V.Surrogate(ctx, seed=a.seed)
     hist = train(ctx, net, a.steps, a.batch, a.size, a.lr, a.seed,
-                 log=lambda m: print("[train]", m, file=sys.stderr, flush=True))
+                 log=lambda m: print("[train]", m, file=sys.stderr, flush=True),
+                 blur_frac=a.blur_frac)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     net.save(a.out)
     rng = np.random.default_rng(a.seed + 1000)
-    held = [make_volume(ctx, a.size, rng) for _ in range(4)]
-    ev = evaluate(ctx, net, held)
-    print({"loss_first": hist[0], "loss_last": hist[-1], "held_out": ev}, flush=True)
+    held_solids = [make_volume(ctx, a.size, rng, 0.0) for _ in range(3)]
+    held_blur = [make_volume(ctx, a.size, rng, 1.0) for _ in range(3)]
+    print({"loss_first": hist[0], "loss_last": hist[-1],
+           "held_out_solids": evaluate(ctx, net, held_solids),
+           "held_out_blurred_noise": evaluate(ctx, net, held_blur)}, flush=True)
 
 
 if __name__ == "__main__":
