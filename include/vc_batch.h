@@ -36,6 +36,10 @@ typedef struct vc_schedule {
                             checks: 0 = adaptive (4, 8, then 16), or 1-32 */
     int use_graph;       /* 1: each segment of sweeps is a captured
                             hipGraph replayed on the context stream       */
+    int tail_sweep;      /* sweeps with index >= tail_sweep within a level
+                            use the sparse-pass variant (speculative row
+                            loads, simple-point test by flood fill instead
+                            of the table); -1 = never                     */
 } vc_schedule;
 
 /* volumes: n member volumes of this context (1 <= n <= 65535, distinct); the

@@ -79,10 +79,16 @@ def test_batch_matches_oracle_per_volume(vc, ragged):
 
 
 @pytest.mark.parametrize("sched", [
-    dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=1, use_graph=0),
-    dict(edt_lds_kb=64, edt_grid_per_cu=1, thin_grid_per_cu=1, sweeps_per_sync=32, use_graph=1),
-    dict(edt_lds_kb=32, edt_grid_per_cu=8, thin_grid_per_cu=8, sweeps_per_sync=0, use_graph=1),
-    dict(edt_lds_kb=16, edt_grid_per_cu=2, thin_grid_per_cu=0, sweeps_per_sync=3, use_graph=1),
+    dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=1, use_graph=0,
+         tail_sweep=-1),
+    dict(edt_lds_kb=64, edt_grid_per_cu=1, thin_grid_per_cu=1, sweeps_per_sync=32, use_graph=1,
+         tail_sweep=0),
+    dict(edt_lds_kb=32, edt_grid_per_cu=8, thin_grid_per_cu=8, sweeps_per_sync=0, use_graph=1,
+         tail_sweep=2),
+    dict(edt_lds_kb=16, edt_grid_per_cu=2, thin_grid_per_cu=0, sweeps_per_sync=3, use_graph=1,
+         tail_sweep=1),
+    dict(edt_lds_kb=32, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=0, use_graph=0,
+         tail_sweep=3),
 ])
 def test_batch_every_schedule_same_result(vc, ragged, sched):
     vols, levels, ref = ragged
@@ -117,6 +123,31 @@ def test_batch_graph_replays(vc, ragged):
     other.release()
     b.release()
     for m in members:
+        m.release()
+
+
+@pytest.mark.parametrize("tail", [0, 2])
+def test_sparse_pass_variant_single_volumes(vc, oracle, tail):
+    """The sparse-pass variant (speculative row loads, flood-fill simple
+    test) on single volumes of every shape class: labels equal the oracle."""
+    rng = np.random.default_rng(31 + tail)
+    shapes = [tuple(int(s) for s in rng.integers(2, 40, size=3)) for _ in range(12)]
+    shapes += [(64, 64, 64), (3, 70, 129)]
+    for i, shape in enumerate(shapes):
+        occ = (rng.random(shape) < rng.choice([0.6, 0.85, 0.97])).astype(np.uint8)
+        if shape == (64, 64, 64):
+            from voxel_ma_amd import synth
+            occ = synth.torus_c1()
+        levels = sorted(float(x) for x in rng.uniform(0.8, 6.0, size=3))
+        say("sparse variant", tail, i, shape)
+        m = vc.upload(occ)
+        b = V.Batch(vc, [m])
+        b.set_schedule(tail_sweep=tail)
+        st = b.run(levels)
+        d, f, lvl, lab, sto = _oracle_all(oracle, occ, levels)
+        assert np.array_equal(m.get(V.BUF_LABEL), lab), (tail, i, shape)
+        assert st.passes == 8 * sum(sto.sweeps[k] for k in range(3))
+        b.release()
         m.release()
 
 

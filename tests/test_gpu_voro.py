@@ -1,0 +1,67 @@
+"""GPU parity of the Voronoi-side kernels (include/vc_voro.h): inside
+tagging of points and the lambda measures of faces, edges and vertices,
+bit-exact against oracle/voro_ref.py."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import voxel_ma_amd as V
+from conftest import ROOT, say
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import voro_ref as R  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tag_points_matches_oracle(vc):
+    rng = np.random.default_rng(1)
+    occ = (rng.random((13, 9, 17)) < 0.5).astype(np.uint8)
+    vol = vc.upload(occ)
+    pts = rng.uniform(-2, 19, size=(5000, 3)).astype(np.float32)
+    # exact half-integers (round away from zero), boundary and odd values
+    half = (rng.integers(-2, 18, size=(500, 3)) + 0.5).astype(np.float32)
+    special = np.array([[-0.5, 0, 0], [-0.49999997, 0, 0], [12.5, 8.4, 16.4],
+                        [np.nan, 1, 1], [np.inf, 1, 1], [3e9, 1, 1], [0, 0, 0]], np.float32)
+    pts = np.concatenate([pts, half, special])
+    say("tag points", len(pts))
+    got = V.voro_tag_points(vol, pts)
+    assert np.array_equal(got, R.tag_points(occ, pts))
+    vol.release()
+
+
+def _complex(rng, nv=300, nf=200, nsites=80):
+    faces = []
+    for _ in range(nf):
+        k = int(rng.integers(3, 7))
+        faces.append([int(x) for x in rng.choice(nv, size=k, replace=False)])
+    edges, seen = [], set()
+    for f in faces:
+        for j in range(len(f)):
+            a, b = f[j], f[(j + 1) % len(f)]
+            k = (min(a, b), max(a, b))
+            if k not in seen:
+                seen.add(k)
+                edges.append(k)
+    fs = [tuple(int(x) for x in rng.choice(nsites, size=2, replace=False)) for _ in range(nf)]
+    sites = (rng.random((nsites, 3)) * 40 - 5).astype(np.float32)
+    return nv, edges, faces, fs, sites
+
+
+def test_measures_match_oracle(vc):
+    rng = np.random.default_rng(2)
+    for it in range(5):
+        nv, edges, faces, fs, sites = _complex(rng)
+        fvalid = (rng.random(len(faces)) < 0.7).astype(np.uint8)
+        lv_r, le_r, lf_r, ve, ef = R.measures(nv, edges, faces, fs, sites, fvalid)
+        ve_off = np.cumsum([0] + [len(x) for x in ve])
+        ef_off = np.cumsum([0] + [len(x) for x in ef])
+        say("measures", it, nv, len(edges), len(faces))
+        lv, le, lf = V.voro_measures(vc, nv, edges, fs, sites, fvalid, ve_off,
+                                     [i for x in ve for i in x], ef_off,
+                                     [i for x in ef for i in x])
+        assert np.array_equal(lf, lf_r)
+        assert np.array_equal(le, le_r)
+        assert np.array_equal(lv, lv_r)

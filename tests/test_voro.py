@@ -1,0 +1,122 @@
+"""Voronoi-side steps on plain arrays (include/vc_voro.h, host functions):
+inside extraction and close-vertex merging against oracle/voro_ref.py, which
+is pinned by hand-derived known answers of the reference's order."""
+import os
+import sys
+
+import numpy as np
+
+import voxel_ma_amd as V
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+import voro_ref as R  # noqa: E402
+
+
+def test_extract_known_answer_square():
+    """One valid quad [0,1,2,3] with its sides stored as (0,1) (1,2) (2,3)
+    (3,0): zig-zag gives (3,0,1), (3,1,2); the side (0,3) of the first
+    triangle is not the stored pair (3,0), so it is appended (the reference
+    dedupes exact pairs), then the diagonal (1,3)."""
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0)]
+    faces = [[0, 1, 2, 3]]
+    le = [1.0, 2.0, 3.0, 4.0]
+    lf = [7.0]
+    for impl in (R.extract_inside, V.voro_extract_inside):
+        vi, eo, em, tr, ff, fm = impl(4, edges, faces, [1, 1, 1, 1], le, lf)
+        assert list(vi) == [0, 1, 2, 3]
+        assert [tuple(e) for e in eo] == [(0, 1), (1, 2), (2, 3), (3, 0), (0, 3), (1, 3)]
+        assert list(em) == [1.0, 2.0, 3.0, 4.0, 7.0, 7.0]
+        assert [tuple(t) for t in tr] == [(3, 0, 1), (3, 1, 2)]
+        assert list(ff) == [0, 0] and list(fm) == [7.0, 7.0]
+
+
+def test_extract_known_answer_invalid_vertex():
+    """Vertex 1 invalid: edges touching it and the face drop out; the rest is
+    renumbered 0->0, 2->1, 3->2."""
+    edges = [(0, 1), (1, 2), (2, 3), (0, 3)]
+    faces = [[0, 1, 2, 3]]
+    for impl in (R.extract_inside, V.voro_extract_inside):
+        vi, eo, em, tr, ff, fm = impl(4, edges, faces, [1, 0, 1, 1], [1, 2, 3, 4], [9])
+        assert list(vi) == [0, 2, 3]
+        assert [tuple(e) for e in eo] == [(1, 2), (0, 2)]
+        assert list(em) == [3.0, 4.0] and len(tr) == 0
+
+
+def _random_complex(rng, nv=40, nf=25):
+    faces = []
+    for _ in range(nf):
+        k = int(rng.integers(3, 7))
+        faces.append([int(x) for x in rng.choice(nv, size=k, replace=False)])
+    # finalize-like edge list: given edges (some unnormalised) then sides
+    edges, seen = [], set()
+    for _ in range(10):
+        a, b = (int(x) for x in rng.choice(nv, size=2, replace=False))
+        if (a, b) not in seen:
+            seen.add((a, b))
+            edges.append((a, b))
+    for f in faces:
+        for j in range(len(f)):
+            a, b = f[j], f[(j + 1) % len(f)]
+            k = (min(a, b), max(a, b))
+            if k not in seen:
+                seen.add(k)
+                edges.append(k)
+    return nv, edges, faces
+
+
+def test_extract_matches_oracle_random():
+    rng = np.random.default_rng(3)
+    for it in range(60):
+        nv, edges, faces = _random_complex(rng)
+        valid = (rng.random(nv) < 0.8).astype(np.uint8)
+        le = rng.random(len(edges)).astype(np.float32)
+        lf = rng.random(len(faces)).astype(np.float32)
+        got = V.voro_extract_inside(nv, edges, faces, valid, le, lf)
+        ref = R.extract_inside(nv, edges, faces, valid, le, lf)
+        assert list(got[0]) == list(ref[0])
+        assert [tuple(e) for e in got[1]] == [tuple(e) for e in ref[1]]
+        assert np.array_equal(got[2], ref[2])
+        assert [tuple(t) for t in got[3]] == [tuple(t) for t in ref[3]]
+        assert list(got[4]) == list(ref[4]) and np.array_equal(got[5], ref[5])
+
+
+def test_merge_known_answer_path():
+    """Path 0 - 1 - 2 at x = 0, 0.5, 1.2, eps 1: seed 0 absorbs 1 (0.5 from
+    the seed); 2 is 1.2 from the seed 0, so it seeds vertex 1.  Edge (0,1)
+    collapses, (1,2) becomes (0,1)."""
+    vts = [(0, 0, 0), (0.5, 0, 0), (1.2, 0, 0)]
+    edges = [(0, 1), (1, 2)]
+    for impl in (R.merge_close, V.voro_merge_close):
+        o2n, nvts, faces, rem, ne, keep = impl(vts, edges, [], [1, 1, 1], 1.0, False)
+        assert list(o2n) == [0, 0, 1] and list(nvts) == [0, 2]
+        assert [tuple(e) for e in ne] == [(0, 1)] and list(keep) == [0, 1]
+
+
+def test_merge_known_answer_triangle_to_edge():
+    """Triangle 0,1,2 with 0 and 1 within eps: it degenerates to the edge
+    (0,1) of the new numbering and is removed."""
+    vts = [(0, 0, 0), (0.1, 0, 0), (0, 3, 0)]
+    edges = [(0, 1), (1, 2), (0, 2)]
+    for impl in (R.merge_close, V.voro_merge_close):
+        o2n, nvts, faces, rem, ne, keep = impl(vts, edges, [[0, 1, 2]], [1, 1, 1], 0.5, False)
+        assert list(o2n) == [0, 0, 1] and faces == [] and list(rem) == [1]
+        assert [tuple(e) for e in ne] == [(0, 1)] and list(keep) == [0, 1, 1]
+
+
+def test_merge_matches_oracle_random():
+    rng = np.random.default_rng(5)
+    for it in range(60):
+        nv, edges, faces = _random_complex(rng, nv=30, nf=15)
+        vts = rng.random((nv, 3)).astype(np.float32) * 3
+        valid = (rng.random(nv) < 0.85).astype(np.uint8)
+        eps = float(rng.uniform(0.2, 1.0))
+        for only in (False, True):
+            got = V.voro_merge_close(vts, edges, faces, valid, eps, only)
+            ref = R.merge_close(vts, edges, faces, valid, eps, only)
+            assert list(got[0]) == list(ref[0])
+            assert list(got[1]) == list(ref[1])
+            assert [list(f) for f in got[2]] == ref[2]
+            assert list(got[3]) == ref[3]
+            assert [tuple(e) for e in got[4]] == ref[4]
+            assert list(got[5]) == ref[5]

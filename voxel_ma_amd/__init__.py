@@ -44,6 +44,11 @@ EXPORTS = [
     "vc_surrogate_set_params", "vc_surrogate_save", "vc_surrogate_load",
     "vc_surrogate_gradients", "vc_surrogate_train_step", "vc_surrogate_infer",
     "vc_surrogate_measure", "vc_surrogate_activation",
+    # include/vc_voro.h
+    "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside", "vc_voro_merge_close",
+    # include/vc_octree.h
+    "vc_octree_parse", "vc_octree_free", "vc_octree_value", "vc_volume_from_octree",
+    "vc_octree_leaf_centers",
 ]
 SURROGATE_PARAMS = 678786
 SURROGATE_HALO = 56
@@ -73,7 +78,7 @@ class Schedule(ctypes.Structure):
     """vc_schedule: launch parameters of a batch (results never depend on it)."""
     _fields_ = [("edt_lds_kb", ctypes.c_int), ("edt_grid_per_cu", ctypes.c_int),
                 ("thin_grid_per_cu", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("tail_sweep", ctypes.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -86,6 +91,12 @@ class SurrogateStats(ctypes.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class OctreeInfo(ctypes.Structure):
+    _fields_ = [("resolution", ctypes.c_int32), ("n_internal", ctypes.c_int64),
+                ("n_empty", ctypes.c_int64), ("n_leaf", ctypes.c_int64),
+                ("lower_left", ctypes.c_float * 3), ("bbox_len", ctypes.c_float)]
 
 
 class CcView(ctypes.Structure):
@@ -153,6 +164,17 @@ def library():
         "vc_surrogate_infer": [P, P, P, I, P, P],
         "vc_surrogate_measure": [P, P, P, ctypes.c_float, I, ctypes.POINTER(SurrogateStats)],
         "vc_surrogate_activation": [P, P, I, P, I64],
+        "vc_octree_parse": [P, ctypes.c_size_t, I, PP, ctypes.POINTER(OctreeInfo)],
+        "vc_octree_free": [P],
+        "vc_octree_value": [P, I, I, I, ctypes.POINTER(ctypes.c_double)],
+        "vc_volume_from_octree": [P, P, PP],
+        "vc_octree_leaf_centers": [P, P, I64, ctypes.POINTER(I64)],
+        "vc_voro_tag_points": [P, P, P, I64, P],
+        "vc_voro_measures": [P, I, I, I, P, I, P, P, P, P, P, P, P, P, P],
+        "vc_voro_extract_inside": [I, I, I, P, P, P, P, P, P, P, I64, ctypes.POINTER(I64), P, P,
+                                   I64, ctypes.POINTER(I64), P, P, P, I64, ctypes.POINTER(I64)],
+        "vc_voro_merge_close": [I, I, I, P, P, P, P, P, P, P, ctypes.c_float, I, P, P,
+                                ctypes.POINTER(I), P, P, P, P, I64, ctypes.POINTER(I64), P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -600,3 +622,167 @@ def cc_prune(nv, edges, foff, fe, em, fm, f_t, l_t):
     rc = library().vc_cc_prune(ctypes.byref(view), _p(em), _p(fm), float(f_t), float(l_t),
                                _p(rv), _p(re), _p(rf), _p(ref))
     return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne]
+
+
+# ---------------------------------------------------------------- Voronoi side
+def _complex_arrays(nv, edges, faces_vts):
+    e = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
+    foff = np.zeros(len(faces_vts) + 1, np.int64)
+    for i, f in enumerate(faces_vts):
+        foff[i + 1] = foff[i] + len(f)
+    fv = np.ascontiguousarray(np.concatenate([np.array(f, np.int32) for f in faces_vts])
+                              if len(faces_vts) else np.zeros(1, np.int32))
+    return e, foff, fv
+
+
+def voro_tag_points(vol, pts):
+    """vc_voro_tag_points: inside flag of the voxel nearest to each point."""
+    pts = np.ascontiguousarray(np.asarray(pts, np.float32).reshape(-1, 3))
+    out = np.zeros(max(len(pts), 1), np.uint8)
+    vol.ctx.check(library().vc_voro_tag_points(vol.ctx.handle, vol.handle, _p(pts), len(pts),
+                                               _p(out)))
+    return out[:len(pts)]
+
+
+def voro_measures(ctx, nv, edges, face_sites, sites, face_valid, ve_off, ve, ef_off, ef):
+    """vc_voro_measures: (lam_v, lam_e, lam_f) on the device."""
+    e = np.array(edges, np.int32).reshape(-1, 2)
+    fs = np.ascontiguousarray(np.array(face_sites, np.int32).reshape(-1, 2))
+    st = np.ascontiguousarray(np.asarray(sites, np.float32).reshape(-1, 3))
+    fvd = np.ascontiguousarray(np.asarray(face_valid, np.uint8))
+    arrs = [np.ascontiguousarray(np.asarray(a, t)) for a, t in
+            ((ve_off, np.int64), (ve, np.int32), (ef_off, np.int64), (ef, np.int32))]
+    arrs = [a if a.size else np.zeros(1, a.dtype) for a in arrs]
+    nf, ne = len(fs), len(e)
+    lv, le, lf = (np.zeros(max(n, 1), np.float32) for n in (nv, ne, nf))
+    ctx.check(library().vc_voro_measures(ctx.handle, nv, ne, nf, _p(st), len(st), _p(fs), _p(fvd),
+                                         *[_p(a) for a in arrs], _p(lv), _p(le), _p(lf)))
+    return lv[:nv], le[:ne], lf[:nf]
+
+
+def voro_extract_inside(nv, edges, faces_vts, v_valid, lam_e, lam_f):
+    """vc_voro_extract_inside: (vts_idx, edges, em, tris, from_fi, fm)."""
+    e, foff, fv = _complex_arrays(nv, edges, faces_vts)
+    vv = np.ascontiguousarray(np.asarray(v_valid, np.uint8))
+    le = np.ascontiguousarray(np.asarray(lam_e, np.float32))
+    lf = np.ascontiguousarray(np.asarray(lam_f, np.float32))
+    le = le if le.size else np.zeros(1, np.float32)
+    lf = lf if lf.size else np.zeros(1, np.float32)
+    L = library()
+    n1, n2, n3 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    rc = L.vc_voro_extract_inside(nv, len(e), len(faces_vts), _p(e), _p(foff), _p(fv), _p(vv),
+                                  _p(le), _p(lf), None, 0, ctypes.byref(n1), None, None, 0,
+                                  ctypes.byref(n2), None, None, None, 0, ctypes.byref(n3))
+    if rc not in (VC_OK, VC_ERR_CAPACITY):
+        raise VcError(rc, "vc_voro_extract_inside")
+    vi = np.zeros(max(n1.value, 1), np.int32)
+    eo = np.zeros((max(n2.value, 1), 2), np.int32)
+    em = np.zeros(max(n2.value, 1), np.float32)
+    tr = np.zeros((max(n3.value, 1), 3), np.int32)
+    ff = np.zeros(max(n3.value, 1), np.int32)
+    fm = np.zeros(max(n3.value, 1), np.float32)
+    rc = L.vc_voro_extract_inside(nv, len(e), len(faces_vts), _p(e), _p(foff), _p(fv), _p(vv),
+                                  _p(le), _p(lf), _p(vi), n1.value, ctypes.byref(n1), _p(eo),
+                                  _p(em), n2.value, ctypes.byref(n2), _p(tr), _p(ff), _p(fm),
+                                  n3.value, ctypes.byref(n3))
+    if rc != VC_OK:
+        raise VcError(rc, "vc_voro_extract_inside")
+    return (vi[:n1.value], eo[:n2.value], em[:n2.value], tr[:n3.value], ff[:n3.value],
+            fm[:n3.value])
+
+
+def voro_merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
+    """vc_voro_merge_close: (old_to_new, new_vts, new_faces, remove_face,
+    new_edges, keep_edge)."""
+    vts = np.ascontiguousarray(np.asarray(vts, np.float32).reshape(-1, 3))
+    nv = len(vts)
+    e, foff, fv = _complex_arrays(nv, edges, faces_vts)
+    ve_off = np.zeros(nv + 1, np.int64)
+    adj = [[] for _ in range(nv)]
+    for i, (a, b) in enumerate(np.array(edges, np.int32).reshape(-1, 2)):
+        adj[a].append(i)
+        adj[b].append(i)
+    for v in range(nv):
+        ve_off[v + 1] = ve_off[v] + len(adj[v])
+    ve = np.ascontiguousarray(np.array([i for a in adj for i in a] or [0], np.int32))
+    vv = np.ascontiguousarray(np.asarray(v_valid, np.uint8)) if v_valid is not None else \
+        np.ones(max(nv, 1), np.uint8)
+    nf = len(faces_vts)
+    o2n = np.zeros(max(nv, 1), np.int32)
+    nvts = np.zeros(max(nv, 1), np.int32)
+    nn = ctypes.c_int()
+    nfo = np.zeros(nf + 1, np.int64)
+    nfv = np.zeros(max(len(fv), 1), np.int32)
+    rem = np.zeros(max(nf, 1), np.uint8)
+    keep = np.zeros(max(len(e), 1), np.uint8)
+    cap = len(e) + nf + 1
+    ne_out = np.zeros((cap, 2), np.int32)
+    nne = ctypes.c_int64()
+    rc = library().vc_voro_merge_close(nv, len(e), nf, _p(vts), _p(e), _p(foff), _p(fv),
+                                       _p(ve_off), _p(ve), _p(vv), float(eps),
+                                       1 if only_inside else 0, _p(o2n), _p(nvts),
+                                       ctypes.byref(nn), _p(nfo), _p(nfv), _p(rem), _p(ne_out),
+                                       cap, ctypes.byref(nne), _p(keep))
+    if rc != VC_OK:
+        raise VcError(rc, "vc_voro_merge_close")
+    kept = int((rem[:nf] == 0).sum())
+    faces = [list(nfv[nfo[i]:nfo[i + 1]]) for i in range(kept)]
+    return (list(o2n[:nv]), list(nvts[:nn.value]), faces, list(rem[:nf]),
+            [tuple(x) for x in ne_out[:nne.value]], list(keep[:len(e)]))
+
+
+# ---------------------------------------------------------------- octree input
+class Octree:
+    """A parsed .sof / .sog octree (vc_octree)."""
+
+    def __init__(self, data, sog=False):
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        h = ctypes.c_void_p()
+        self.info = OctreeInfo()
+        rc = library().vc_octree_parse(buf, len(data), 1 if sog else 0, ctypes.byref(h),
+                                       ctypes.byref(self.info))
+        if rc != VC_OK:
+            raise VcError(rc, "vc_octree_parse")
+        self.handle = h
+
+    @classmethod
+    def read(cls, path):
+        with open(path, "rb") as f:
+            data = f.read()
+        return cls(data, sog=path.lower().endswith(".sog"))
+
+    def value(self, x, y, z):
+        v = ctypes.c_double()
+        rc = library().vc_octree_value(self.handle, x, y, z, ctypes.byref(v))
+        if rc != VC_OK:
+            raise VcError(rc, "vc_octree_value")
+        return v.value
+
+    def leaf_centers(self):
+        n = ctypes.c_int64()
+        library().vc_octree_leaf_centers(self.handle, None, 0, ctypes.byref(n))
+        out = np.zeros((max(n.value, 1), 3), np.float32)
+        rc = library().vc_octree_leaf_centers(self.handle, _p(out), n.value, ctypes.byref(n))
+        if rc != VC_OK:
+            raise VcError(rc, "vc_octree_leaf_centers")
+        return out[:n.value]
+
+    def volume(self, ctx):
+        """Dense (R+1)^3 occupancy built on the device, as a Volume."""
+        h = ctypes.c_void_p()
+        ctx.check(library().vc_volume_from_octree(ctx.handle, self.handle, ctypes.byref(h)))
+        n = self.info.resolution + 1
+        v = Volume.__new__(Volume)
+        v.ctx, v.shape, v.handle = ctx, (n, n, n), h
+        return v
+
+    def free(self):
+        if getattr(self, "handle", None):
+            library().vc_octree_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -89,6 +89,7 @@ vc_schedule vc_schedule_default()
     sc.thin_grid_per_cu = 0;
     sc.sweeps_per_sync = 0;
     sc.use_graph = 0;
+    sc.tail_sweep = -1;
     return sc;
 }
 
@@ -160,6 +161,8 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
                        "sweeps_per_sync must be in [0, %d]", VC_RING / 2);
     if (sc.use_graph != 0 && sc.use_graph != 1)
         return vc_fail(ctx, VC_ERR_INVALID, "use_graph must be 0 or 1");
+    if (sc.tail_sweep < -1 || sc.tail_sweep > 1000000)
+        return vc_fail(ctx, VC_ERR_INVALID, "tail_sweep must be in [-1, 1e6]");
     VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_graphs(s);
     s->sc = sc;

@@ -336,14 +336,144 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return tot;
 }
 
-// One subfield pass over the words of the subfield's (x, y) rows: an item is
-// 256 words of one volume, thread = word, words ordered (x/2, y/2, w).
-// A word is visited only when its z-parity flag is set: the flag of z-parity
-// p (d0 / d1) is raised whenever a voxel of the 26-neighbourhood of one of
-// its parity-p voxels is removed.  All flag traffic is plain byte stores: in
-// pass (sx, sy, sz) the flags of parity sz are cleared only on rows of parity
-// (sx, sy), and raised only on other rows.
+// Visit of one word in a subfield pass (its flag was set and has been
+// cleared): candidates of the pass's z parity (alive, eligible, a background
+// face neighbour) go through the simple-point table; removals update the
+// word, the labels and the revisit flags of every word holding a neighbour.
+// Returns the removed bits; *ntest counts the table lookups.
 //
+// Revisit flags: the flag of z-parity p (D0 / D1) is raised whenever a voxel
+// of the 26-neighbourhood of one of the word's parity-p voxels is removed.
+// All flag traffic is plain byte stores: in pass (sx, sy, sz) the flags of
+// parity sz are cleared only on rows of parity (sx, sy), and raised only on
+// other rows.
+__device__ __forceinline__ uint64_t visit_word(
+    const Plane& P, uint64_t* A, const uint64_t* __restrict__ E,
+    uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1, uint8_t* Fo,
+    const uint32_t* __restrict__ lut, int x, int y, int w, int64_t o,
+    uint64_t zsub, int k, uint32_t* ntest)
+{
+    const uint64_t C = A[o];
+    uint64_t cand = C & E[o] & zsub;
+    if (!cand)
+        return 0;
+    uint64_t R[3][3];
+    uint32_t pv[3][3], nx0[3][3];
+    const bool need_prev = (cand & 1ull) && w > 0;
+    const bool need_next = (cand >> 63) && w + 1 < P.nwz;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+        {
+            const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
+                            (unsigned)(y + dy) < (unsigned)P.ny;
+            const int64_t q = widx(P, x + dx, y + dy, w);
+            R[dx + 1][dy + 1] = (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
+            pv[dx + 1][dy + 1] =
+                (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
+            nx0[dx + 1][dy + 1] =
+                (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
+        }
+    // border filter: a voxel with all six face neighbours alive is never
+    // simple
+    const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
+    const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
+    cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
+    uint64_t removed = 0;
+    // four independent table lookups in flight per round
+    while (cand)
+    {
+        int bb[4];
+        uint32_t cc[4];
+        int nb = 0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+        {
+            bb[jj] = 0;
+            cc[jj] = 0;
+            if (cand)
+            {
+                bb[jj] = __ffsll((unsigned long long)cand) - 1;
+                cand &= cand - 1;
+                cc[jj] = nb_code(R, pv, nx0, bb[jj]);
+                nb = jj + 1;
+            }
+        }
+        uint32_t lw[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+            lw[jj] = jj < nb ? lut[cc[jj] >> 5] : 0u;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+            if ((lw[jj] >> (cc[jj] & 31)) & 1u)
+                removed |= 1ull << bb[jj];
+        *ntest += nb;
+    }
+    if (!removed)
+        return 0;
+    A[o] = C & ~removed;
+    uint8_t* row = label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
+    uint64_t m = removed;
+    while (m)
+    {
+        const int bt = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        row[bt] = (uint8_t)(2 + k);
+    }
+    // raise the flags of every word holding a neighbour
+    const bool lo = (removed & 1ull) && w > 0;
+    const bool hi = (removed >> 63) && w + 1 < P.nwz;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+        {
+            if ((unsigned)(x + dx) >= (unsigned)P.nx ||
+                (unsigned)(y + dy) >= (unsigned)P.ny)
+                continue;
+            const int64_t q = widx(P, x + dx, y + dy, w);
+            if (dx == 0 && dy == 0)
+            {
+                Fo[q] = 1;
+                if (lo)
+                    Fo[q - 1] = 1;
+                if (hi)
+                    Fo[q + 1] = 1;
+            }
+            else
+            {
+                D0[q] = 1;
+                D1[q] = 1;
+                if (lo)
+                {
+                    D0[q - 1] = 1;
+                    D1[q - 1] = 1;
+                }
+                if (hi)
+                {
+                    D0[q + 1] = 1;
+                    D1[q + 1] = 1;
+                }
+            }
+        }
+    return removed;
+}
+
+// class-relative word t of a pass -> (x, y, w); false past the last row
+__device__ __forceinline__ bool pass_word(const Plane& P, int sx, int sy,
+                                          uint32_t t, int& x, int& y, int& w)
+{
+    const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
+    const uint32_t r = t / (uint32_t)P.nwz;
+    w = (int)(t - r * (uint32_t)P.nwz);
+    const uint32_t xh = r / (uint32_t)nyh;
+    const int yh = (int)(r - xh * (uint32_t)nyh);
+    x = 2 * (int)xh + sx;
+    y = 2 * yh + sy;
+    return (int)xh < nxh;
+}
+
 // Sweep control lives on the device so that a segment of sweeps can be
 // replayed as a captured graph: sweep = ctl[0] + j, level = ctl[1].  A volume
 // whose previous sweep removed nothing has converged at this level (its next
@@ -351,6 +481,34 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
 // are rings: vany[4v + (sweep & 3)] per volume, gany[sweep % VC_RING] for the
 // batch; the first pass of a sweep clears the entries of sweep + 1 (vany) and
 // sweep + VC_RING/2 (gany), which no pass of this sweep reads or writes.
+struct PassCtl
+{
+    uint32_t sweep;
+    int k;
+};
+
+__device__ __forceinline__ PassCtl pass_ctl(const uint32_t* ctl, int j,
+                                            int k_arg, int s, int nvol,
+                                            uint32_t* gany, uint32_t* vany)
+{
+    // captured: sweep and level from the device; launched: the arguments
+    PassCtl c;
+    c.sweep = ctl ? ctl[0] + (uint32_t)j : (uint32_t)j;
+    c.k = ctl ? (int)ctl[1] : k_arg;
+    if (s == 0)
+    {
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t gs = gridDim.x * blockDim.x;
+        for (uint32_t i = g; i < (uint32_t)nvol; i += gs)
+            vany[4 * i + ((c.sweep + 1) & 3)] = 0;
+        if (g == 0)
+            gany[(c.sweep + VC_RING / 2) & (VC_RING - 1)] = 0;
+    }
+    return c;
+}
+
+// One subfield pass, dense: an item is 256 words of one volume (thread =
+// word, words ordered (x/2, y/2, w) within the pass's row parity).
 template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     thin_pass(vc_work W, const uint32_t* __restrict__ lut, int s, int j,
@@ -358,19 +516,10 @@ __global__ void __launch_bounds__(kBlock)
               uint32_t* gany, uint32_t* vany,
               unsigned long long* __restrict__ ctr)
 {
-    // captured: sweep and level from the device; launched: from the arguments
-    const uint32_t sweep = ctl ? ctl[0] + (uint32_t)j : (uint32_t)j;
-    const int k = ctl ? (int)ctl[1] : k_arg;
+    const PassCtl pc = pass_ctl(ctl, j, k_arg, s, nvol, gany, vany);
+    const uint32_t sweep = pc.sweep;
+    const int k = pc.k;
     __shared__ uint32_t red[kBlock / 64];
-    if (s == 0)
-    {
-        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-        const uint32_t gs = gridDim.x * blockDim.x;
-        for (uint32_t i = g; i < (uint32_t)nvol; i += gs)
-            vany[4 * i + ((sweep + 1) & 3)] = 0;
-        if (g == 0)
-            gany[(sweep + VC_RING / 2) & (VC_RING - 1)] = 0;
-    }
     const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
     const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
     for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
@@ -380,144 +529,28 @@ __global__ void __launch_bounds__(kBlock)
         vc_item<kB>(W, b, vi, lt);
         const vc_vdesc d = vc_desc<kB>(W, vi);
         const Plane P{d.nx, d.ny, d.nz, d.nwz};
-        uint64_t* A = d.alive;
-        const uint64_t* E = d.elig;
-        uint8_t* D0 = d.d0;
-        uint8_t* D1 = d.d1;
-        uint8_t* Fs = sz ? D1 : D0; // this pass's parity
-        uint8_t* Fo = sz ? D0 : D1; // the other parity
-        const int nxh = (P.nx - sx + 1) >> 1, nyh = (P.ny - sy + 1) >> 1;
-        const uint32_t t = lt * kBlock + threadIdx.x;
-        const uint32_t r = t / (uint32_t)P.nwz;
-        const int w = (int)(t - r * (uint32_t)P.nwz);
-        const uint32_t xh = r / (uint32_t)nyh;
-        const int yh = (int)(r - xh * (uint32_t)nyh);
-        const int x = 2 * (int)xh + sx;
-        const int y = 2 * yh + sy;
-        uint64_t removed = 0;
-        uint32_t ntest = 0;
-        const bool valid = (int)xh < nxh;
+        uint8_t* Fs = sz ? d.d1 : d.d0; // this pass's parity
+        uint8_t* Fo = sz ? d.d0 : d.d1; // the other parity
+        int x, y, w;
+        const bool valid =
+            pass_word(P, sx, sy, lt * kBlock + threadIdx.x, x, y, w);
         const int64_t o = widx(P, x, y, w);
         // the flag load is issued before the convergence check so that the
         // two latencies overlap
         const uint8_t fl = valid ? Fs[o] : (uint8_t)0;
         if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
             continue; // converged at this level (block-uniform)
-        if (valid)
+        uint64_t removed = 0;
+        uint32_t ntest = 0;
+        if (fl)
         {
-            uint64_t C = 0, cand = 0;
-            if (fl)
-            {
-                Fs[o] = 0;
-                C = A[o];
-                cand = C & E[o] & zsub;
-            }
-            if (cand)
-            {
-                uint64_t R[3][3];
-                uint32_t pv[3][3], nx0[3][3];
-                const bool need_prev = (cand & 1ull) && w > 0;
-                const bool need_next = (cand >> 63) && w + 1 < P.nwz;
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy)
-                    {
-                        const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
-                                        (unsigned)(y + dy) < (unsigned)P.ny;
-                        const int64_t q = widx(P, x + dx, y + dy, w);
-                        R[dx + 1][dy + 1] =
-                            (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
-                        pv[dx + 1][dy + 1] =
-                            (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
-                        nx0[dx + 1][dy + 1] =
-                            (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
-                    }
-                // border filter: a voxel with all six face neighbours alive
-                // is never simple
-                const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
-                const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
-                cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
-                // four independent table lookups in flight per round
-                while (cand)
-                {
-                    int bb[4];
-                    uint32_t cc[4];
-                    int nb = 0;
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj)
-                    {
-                        bb[jj] = 0;
-                        cc[jj] = 0;
-                        if (cand)
-                        {
-                            bb[jj] = __ffsll((unsigned long long)cand) - 1;
-                            cand &= cand - 1;
-                            cc[jj] = nb_code(R, pv, nx0, bb[jj]);
-                            nb = jj + 1;
-                        }
-                    }
-                    uint32_t lw[4];
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj)
-                        lw[jj] = jj < nb ? lut[cc[jj] >> 5] : 0u;
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj)
-                        if ((lw[jj] >> (cc[jj] & 31)) & 1u)
-                            removed |= 1ull << bb[jj];
-                    ntest += nb;
-                }
-                if (removed)
-                {
-                    A[o] = C & ~removed;
-                    uint8_t* row =
-                        d.label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
-                    uint64_t m = removed;
-                    while (m)
-                    {
-                        const int bt = __ffsll((unsigned long long)m) - 1;
-                        m &= m - 1;
-                        row[bt] = (uint8_t)(2 + k);
-                    }
-                    // raise the flags of every word holding a neighbour
-                    const bool lo = (removed & 1ull) && w > 0;
-                    const bool hi = (removed >> 63) && w + 1 < P.nwz;
-#pragma unroll
-                    for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-                        for (int dy = -1; dy <= 1; ++dy)
-                        {
-                            if ((unsigned)(x + dx) >= (unsigned)P.nx ||
-                                (unsigned)(y + dy) >= (unsigned)P.ny)
-                                continue;
-                            const int64_t q = widx(P, x + dx, y + dy, w);
-                            if (dx == 0 && dy == 0)
-                            {
-                                Fo[q] = 1;
-                                if (lo)
-                                    Fo[q - 1] = 1;
-                                if (hi)
-                                    Fo[q + 1] = 1;
-                            }
-                            else
-                            {
-                                D0[q] = 1;
-                                D1[q] = 1;
-                                if (lo)
-                                {
-                                    D0[q - 1] = 1;
-                                    D1[q - 1] = 1;
-                                }
-                                if (hi)
-                                {
-                                    D0[q + 1] = 1;
-                                    D1[q + 1] = 1;
-                                }
-                            }
-                        }
-                }
-            }
+            Fs[o] = 0;
+            removed = visit_word(P, d.alive, d.elig, d.label, d.d0, d.d1, Fo,
+                                 lut, x, y, w, o, zsub, k, &ntest);
         }
+        // blocks that tested nothing skip the reductions (block-uniform)
+        if (!__syncthreads_or((removed != 0) | (ntest != 0)))
+            continue;
         const uint32_t nrem = block_sum((uint32_t)__popcll(removed), red);
         __syncthreads();
         const uint32_t ntst = block_sum(ntest, red);
@@ -525,7 +558,8 @@ __global__ void __launch_bounds__(kBlock)
         {
             // counters spread over VC_CTR_SLOTS cache lines: same-address
             // device-scope atomics from every workgroup serialise across XCDs
-            unsigned long long* c = ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
+            unsigned long long* c =
+                ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
             if (nrem)
             {
                 gany[sweep & (VC_RING - 1)] = 1u;
@@ -536,6 +570,93 @@ __global__ void __launch_bounds__(kBlock)
                 atomicAdd(&c[0], (unsigned long long)ntst);
         }
         __syncthreads(); // red is rewritten by the next item
+    }
+}
+
+// One subfield pass, sparse (late sweeps, few flagged words): one wavefront
+// per item of 256 words; each lane reads 4 flags, the flagged words are
+// compacted in LDS and the wavefront visits only those.  A quarter of the
+// dense pass's wavefronts, each with one round of flag loads.
+template <bool kB>
+__global__ void __launch_bounds__(64)
+    thin_pass_sparse(vc_work W, const uint32_t* __restrict__ lut, int s, int j,
+                     int k_arg, int nvol, const uint32_t* __restrict__ ctl,
+                     uint32_t* gany, uint32_t* vany,
+                     unsigned long long* __restrict__ ctr)
+{
+    const PassCtl pc = pass_ctl(ctl, j, k_arg, s, nvol, gany, vany);
+    const uint32_t sweep = pc.sweep;
+    const int k = pc.k;
+    __shared__ uint32_t list[kBlock];
+    const int lane = threadIdx.x;
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+    const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    {
+        int vi;
+        uint32_t lt;
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
+        const Plane P{d.nx, d.ny, d.nz, d.nwz};
+        uint8_t* Fs = sz ? d.d1 : d.d0;
+        uint8_t* Fo = sz ? d.d0 : d.d1;
+        uint8_t fl[4];
+        int64_t oo[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            int x, y, w;
+            const bool valid =
+                pass_word(P, sx, sy, lt * kBlock + q * 64 + lane, x, y, w);
+            oo[q] = widx(P, x, y, w);
+            fl[q] = valid ? Fs[oo[q]] : (uint8_t)0;
+        }
+        if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
+            continue; // converged at this level (block-uniform)
+        uint32_t n = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const uint64_t m = __ballot(fl[q] != 0);
+            if (fl[q])
+            {
+                list[n + __popcll(m & below)] = (uint32_t)(q * 64 + lane);
+                Fs[oo[q]] = 0;
+            }
+            n += (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        uint32_t nrem = 0, ntest = 0;
+        for (uint32_t i = lane; i < n; i += 64)
+        {
+            int x, y, w;
+            pass_word(P, sx, sy, lt * kBlock + list[i], x, y, w);
+            const uint64_t rm =
+                visit_word(P, d.alive, d.elig, d.label, d.d0, d.d1, Fo, lut, x,
+                           y, w, widx(P, x, y, w), zsub, k, &ntest);
+            nrem += (uint32_t)__popcll(rm);
+        }
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1)
+        {
+            nrem += __shfl_xor(nrem, dd);
+            ntest += __shfl_xor(ntest, dd);
+        }
+        if (lane == 0)
+        {
+            unsigned long long* c =
+                ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
+            if (nrem)
+            {
+                gany[sweep & (VC_RING - 1)] = 1u;
+                vany[4 * vi + (sweep & 3)] = 1u;
+                atomicAdd(&c[8 + k], (unsigned long long)nrem);
+            }
+            if (ntest)
+                atomicAdd(&c[0], (unsigned long long)ntest);
+        }
+        __syncthreads(); // list is rewritten by the next item
     }
 }
 
@@ -558,30 +679,60 @@ int pass_count(const vc_sched* s)
 // segment reads the segment's first sweep and the level from ctl, and the
 // segment ends by advancing ctl[0]; launched directly: the absolute sweep
 // (first + j) and the level are arguments.
+template <bool kB, bool kTail>
+void launch_pass(hipStream_t S, unsigned g, const vc_work& W,
+                 const uint32_t* lut, int sub, int j, int k, int n,
+                 const uint32_t* ctl, vc_sched* s)
+{
+    if (kTail)
+        thin_pass_sparse<kB><<<g, 64, 0, S>>>(W, lut, sub, j, k, n, ctl,
+                                              s->d_gany, s->d_vany, s->d_ctr);
+    else
+        thin_pass<kB><<<g, kBlock, 0, S>>>(W, lut, sub, j, k, n, ctl,
+                                           s->d_gany, s->d_vany, s->d_ctr);
+}
+
+// sweeps with index >= tail_sweep within a level use the sparse variant
+bool tail_sweep_of(const vc_sched* s, int64_t sweep)
+{
+    return s->sc.tail_sweep >= 0 && sweep >= s->sc.tail_sweep;
+}
+
 int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
-                   int k)
+                   int k, bool graph_tail)
 {
     hipStream_t S = ctx->stream;
     const int pc = s->sc.thin_grid_per_cu;
     for (int j = 0; j < K; ++j)
+    {
+        const bool tail = graph ? graph_tail : tail_sweep_of(s, first + j);
         for (int sub = 0; sub < 8; ++sub)
         {
             const int kind = VK_TP + (sub & 3);
             const unsigned g = vc_sched_grid(ctx, s, kind, pc);
             if (!g)
                 continue;
+            const vc_work W = vc_sched_work(s, kind);
+            const int jj = graph ? j : (int)(first + j);
+            const uint32_t* ctl = graph ? s->d_ctl : nullptr;
             if (s->n > 1)
-                thin_pass<true><<<g, kBlock, 0, S>>>(
-                    vc_sched_work(s, kind), ctx->d_lut, sub,
-                    graph ? j : (int)(first + j), k, s->n,
-                    graph ? s->d_ctl : nullptr, s->d_gany, s->d_vany, s->d_ctr);
+            {
+                if (tail)
+                    launch_pass<true, true>(S, g, W, ctx->d_lut, sub, jj, k,
+                                            s->n, ctl, s);
+                else
+                    launch_pass<true, false>(S, g, W, ctx->d_lut, sub, jj, k,
+                                             s->n, ctl, s);
+            }
+            else if (tail)
+                launch_pass<false, true>(S, g, W, ctx->d_lut, sub, jj, k, s->n,
+                                         ctl, s);
             else
-                thin_pass<false><<<g, kBlock, 0, S>>>(
-                    vc_sched_work(s, kind), ctx->d_lut, sub,
-                    graph ? j : (int)(first + j), k, s->n,
-                    graph ? s->d_ctl : nullptr, s->d_gany, s->d_vany, s->d_ctr);
+                launch_pass<false, false>(S, g, W, ctx->d_lut, sub, jj, k,
+                                          s->n, ctl, s);
             VC_CHECK_LAUNCH(ctx);
         }
+    }
     if (graph)
     {
         seg_advance<<<1, 64, 0, S>>>(s->d_ctl, K);
@@ -591,16 +742,19 @@ int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
 }
 
 // a segment as a captured graph (captured once per length, then replayed)
+// (graph segments use one variant, chosen by their first sweep)
 int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k)
 {
     if (!s->sc.use_graph)
-        return launch_segment(ctx, s, K, false, first, k);
+        return launch_segment(ctx, s, K, false, first, k, false);
     hipStream_t S = ctx->stream;
-    auto it = s->graphs.find(K);
+    const bool tail = tail_sweep_of(s, first);
+    const int key = K * 2 + (tail ? 1 : 0);
+    auto it = s->graphs.find(key);
     if (it == s->graphs.end())
     {
         VC_HIP(ctx, hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
-        const int rc = launch_segment(ctx, s, K, true, 0, 0);
+        const int rc = launch_segment(ctx, s, K, true, 0, 0, tail);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(S, &g);
         if (rc != VC_OK)
@@ -618,7 +772,7 @@ int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k)
         if (e2 != hipSuccess)
             return vc_fail(ctx, VC_ERR_HIP, "segment instantiate: %s",
                            hipGetErrorString(e2));
-        it = s->graphs.emplace(K, ex).first;
+        it = s->graphs.emplace(key, ex).first;
     }
     VC_HIP(ctx, hipGraphLaunch(it->second, S));
     return VC_OK;

@@ -2,6 +2,8 @@
 
 #include <memory>
 
+#include "octree.h"
+
 #include "vc_device.h"
 
 SurfacerErrCode Surfacer::run(const std::shared_ptr<Volume3DScalar>& vol,
@@ -76,6 +78,13 @@ SurfacerErrCode Surfacer::extractBoundaryVtsAndFaces(
 SurfacerErrCode Surfacer::extractBoundaryVts(const std::shared_ptr<Volume3DScalar>& vol,
                                              std::vector<point>& vts)
 {
+    // octree input: the centres of the mixed-sign leaves, in tree order
+    // (src/surfacing.cpp:286-314)
+    if (auto oct = std::dynamic_pointer_cast<OctreeVolume>(vol))
+    {
+        oct->leafCenters(vts);
+        return SurfacerErrCode::SUCCESS;
+    }
     return run(vol, &vts, nullptr, nullptr);
 }
 

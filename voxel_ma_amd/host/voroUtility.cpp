@@ -7,8 +7,8 @@
 //                                                  per threshold
 //   -md=topo -meshToCheck=<m.off>                  closed pockets of a mesh
 //
-// Volumes: .mrc (modes 0, 1, 2, 6) or raw uint8 with -dims=nx,ny,nz (C order
-// [x][y][z]).  Occupancy = value > 0.
+// Volumes: .mrc (modes 0, 1, 2, 6), .sof / .sog octrees, or raw uint8 with
+// -dims=nx,ny,nz (C order [x][y][z]).  Occupancy = value > 0.
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "highlevelalgo.h"
+#include "octree.h"
 #include "surfacing.h"
 
 namespace
@@ -202,9 +203,11 @@ int main(int argc, char** argv)
     const std::string md = flags.count("md") ? flags["md"] : "";
     try
     {
-        auto load = [&](const std::string& path) -> std::shared_ptr<DenseVolume> {
+        auto load = [&](const std::string& path) -> std::shared_ptr<Volume3DScalar> {
             if (ends_with(path, ".mrc"))
                 return read_mrc(path);
+            if (ends_with(path, ".sof") || ends_with(path, ".sog"))
+                return std::make_shared<OctreeVolume>(path.c_str());
             int d[3] = {0, 0, 0};
             if (flags.count("dims"))
             {

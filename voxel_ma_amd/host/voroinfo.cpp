@@ -1,0 +1,394 @@
+// voroinfo.cpp -- VoroInfo over include/vc_voro.h (see voroinfo.h).
+#include "voroinfo.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "../../include/vc_voro.h"
+#include "vc_device.h"
+
+#pragma clang fp contract(off)
+
+float MeasureForMA::lambdaForFace(const point& a, const point& b)
+{
+    float q = (b[0] - a[0]) * (b[0] - a[0]);
+    q += (b[1] - a[1]) * (b[1] - a[1]);
+    q += (b[2] - a[2]) * (b[2] - a[2]);
+    return std::sqrt(q);
+}
+
+namespace voxelvoro
+{
+
+void VoroInfo::setGeometry(const std::vector<point>& vts,
+                           const std::vector<std::vector<int>>& faces,
+                           const std::vector<ivec2>& face_sites,
+                           const std::vector<point>& sites,
+                           const std::vector<bool>& finite_v)
+{
+    if (face_sites.size() != faces.size())
+        throw std::runtime_error("VoroInfo: one site pair per face");
+    m_geom = cellcomplex(vts, {}, {}, false);
+    for (const auto& f : faces)
+        m_geom.appendFace(f);
+    m_geom.finalize();
+    m_face_sites = face_sites;
+    m_site_positions = sites;
+    m_is_finite_v = finite_v.empty() ? std::vector<bool>(vts.size(), true) : finite_v;
+    m_vts_valid.assign(vts.size(), false);
+    m_tagged = false;
+    m_only_inside = -1;
+    recompute_flags();
+}
+
+bool VoroInfo::isVertexValid(int vi) const
+{
+    return onlyHasInside() || (m_vts_valid[vi] && m_is_finite_v[vi]);
+}
+bool VoroInfo::isEdgeValid(int ei) const
+{
+    return onlyHasInside() || m_edge_valid[ei];
+}
+bool VoroInfo::isFaceValid(int fi) const
+{
+    return onlyHasInside() || m_face_valid[fi];
+}
+
+void VoroInfo::recompute_flags()
+{
+    m_edge_valid.assign(m_geom.numEdges(), false);
+    for (size_t e = 0; e < m_geom.numEdges(); ++e)
+    {
+        const ivec2& ed = m_geom.getEdge(e);
+        m_edge_valid[e] = isVertexValid(ed[0]) && isVertexValid(ed[1]);
+    }
+    m_face_valid.assign(m_geom.numFaces(), false);
+    std::vector<int> fv;
+    for (size_t f = 0; f < m_geom.numFaces(); ++f)
+    {
+        m_geom.getFaceVRep((int)f, fv);
+        bool ok = true;
+        for (int v : fv)
+            ok = ok && isVertexValid(v);
+        m_face_valid[f] = ok;
+    }
+}
+
+bool VoroInfo::tagVert(const point& p,
+                       const std::shared_ptr<Volume3DScalar>& vol) const
+{
+    const int v[3] = {(int)std::round(p[0]), (int)std::round(p[1]),
+                      (int)std::round(p[2])};
+    if (v[0] < 0 || v[1] < 0 || v[2] < 0 || v[0] >= vol->getSizeX() ||
+        v[1] >= vol->getSizeY() || v[2] >= vol->getSizeZ())
+        return false;
+    return vol->getDataAt(v[0], v[1], v[2]) > 0.0;
+}
+
+bool VoroInfo::tagVtsUsingUniformVol(const std::shared_ptr<Volume3DScalar>& vol)
+{
+    vc_ctx* ctx = vcdev::context();
+    const std::vector<uint8_t> occ = vol->occupancy();
+    vc_vol* dv = nullptr;
+    vcdev::check(vc_volume_upload(ctx, occ.data(), vol->getSizeX(),
+                                  vol->getSizeY(), vol->getSizeZ(), &dv),
+                 "vc_volume_upload");
+    const size_t n = m_geom.numVts();
+    std::vector<float> pts(3 * n);
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a)
+            pts[3 * i + a] = m_geom.getVert(i)[a];
+    std::vector<uint8_t> valid(n + 1, 0);
+    const int rc = vc_voro_tag_points(ctx, dv, pts.data(), (int64_t)n, valid.data());
+    vc_volume_release(ctx, dv);
+    vcdev::check(rc, "vc_voro_tag_points");
+    for (size_t i = 0; i < n; ++i)
+        m_vts_valid[i] = valid[i] != 0;
+    m_tagged = true;
+    recompute_flags();
+    // bounding box of the valid vertices
+    m_bb_valid = false;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (!isVertexValid((int)i))
+            continue;
+        const point& p = m_geom.getVert(i);
+        if (!m_bb_valid)
+        {
+            m_bb_min = m_bb_max = p;
+            m_bb_valid = true;
+        }
+        for (int a = 0; a < 3; ++a)
+        {
+            m_bb_min[a] = std::min(m_bb_min[a], p[a]);
+            m_bb_max[a] = std::max(m_bb_max[a], p[a]);
+        }
+    }
+    return true;
+}
+
+float VoroInfo::getInsidePartSize() const
+{
+    if (!m_bb_valid)
+        return 0.0f;
+    float s = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        s = std::max(s, m_bb_max[a] - m_bb_min[a]);
+    return s;
+}
+
+void VoroInfo::getValidVts(std::vector<int>& idx) const
+{
+    idx.clear();
+    for (size_t v = 0; v < m_geom.numVts(); ++v)
+        if (isVertexValid((int)v))
+            idx.push_back((int)v);
+}
+void VoroInfo::getValidEdges(std::vector<int>& idx) const
+{
+    idx.clear();
+    for (size_t e = 0; e < m_geom.numEdges(); ++e)
+        if (isEdgeValid((int)e))
+            idx.push_back((int)e);
+}
+void VoroInfo::getValidFaces(std::vector<int>& idx) const
+{
+    idx.clear();
+    for (size_t f = 0; f < m_geom.numFaces(); ++f)
+        if (isFaceValid((int)f))
+            idx.push_back((int)f);
+}
+
+void VoroInfo::all_measures(std::vector<float>& lv, std::vector<float>& le,
+                            std::vector<float>& lf) const
+{
+    const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
+              nf = (int)m_geom.numFaces();
+    std::vector<int64_t> ve_off(nv + 1, 0), ef_off(ne + 1, 0);
+    std::vector<int32_t> ve, ef;
+    for (int v = 0; v < nv; ++v)
+    {
+        for (int k = 0; k < m_geom.cntNbEdgesofVert(v); ++k)
+            ve.push_back(m_geom.nbEdgeofVert(v, k));
+        ve_off[v + 1] = (int64_t)ve.size();
+    }
+    for (int e = 0; e < ne; ++e)
+    {
+        for (int k = 0; k < m_geom.cntNbFacesofEdge(e); ++k)
+            ef.push_back(m_geom.nbFaceofEdge(e, k));
+        ef_off[e + 1] = (int64_t)ef.size();
+    }
+    std::vector<float> sites(3 * m_site_positions.size() + 3);
+    for (size_t i = 0; i < m_site_positions.size(); ++i)
+        for (int a = 0; a < 3; ++a)
+            sites[3 * i + a] = m_site_positions[i][a];
+    std::vector<int32_t> fs(2 * (size_t)nf + 2);
+    std::vector<uint8_t> fv((size_t)nf + 1);
+    for (int f = 0; f < nf; ++f)
+    {
+        fs[2 * f] = m_face_sites[f][0];
+        fs[2 * f + 1] = m_face_sites[f][1];
+        fv[f] = isFaceValid(f) ? 1 : 0;
+    }
+    ve.push_back(0);
+    ef.push_back(0);
+    lv.assign((size_t)nv + 1, 0.0f);
+    le.assign((size_t)ne + 1, 0.0f);
+    lf.assign((size_t)nf + 1, 0.0f);
+    vcdev::check(vc_voro_measures(vcdev::context(), nv, ne, nf, sites.data(),
+                                  (int32_t)m_site_positions.size(), fs.data(),
+                                  fv.data(), ve_off.data(), ve.data(),
+                                  ef_off.data(), ef.data(), lv.data(),
+                                  le.data(), lf.data()),
+                 "vc_voro_measures");
+}
+
+void VoroInfo::computeVertexMeasure(MeasureForMA::meassuretype tp,
+                                    const std::vector<int>& idx,
+                                    std::vector<float>& out) const
+{
+    out.clear();
+    if (tp != MeasureForMA::LAMBDA)
+        return;
+    std::vector<float> lv, le, lf;
+    all_measures(lv, le, lf);
+    for (int v : idx)
+        out.push_back(lv[v]);
+}
+void VoroInfo::computeEdgesMeasure(MeasureForMA::meassuretype tp,
+                                   const std::vector<int>& idx,
+                                   std::vector<float>& out) const
+{
+    out.clear();
+    if (tp != MeasureForMA::LAMBDA)
+        return;
+    std::vector<float> lv, le, lf;
+    all_measures(lv, le, lf);
+    for (int e : idx)
+        out.push_back(le[e]);
+}
+void VoroInfo::computeFacesMeasure(MeasureForMA::meassuretype tp,
+                                   const std::vector<int>& idx,
+                                   std::vector<float>& out) const
+{
+    out.clear();
+    if (tp != MeasureForMA::LAMBDA)
+        return;
+    std::vector<float> lv, le, lf;
+    all_measures(lv, le, lf);
+    for (int f : idx)
+        out.push_back(lf[f]);
+}
+
+void VoroInfo::extractInsideWithMeasure(
+    MeasureForMA::meassuretype tp, std::vector<point>& out_vts,
+    std::vector<ivec2>& out_edges, std::vector<uTriFace>& out_tris,
+    std::vector<int>& from_fi, std::vector<float>& vts_msure,
+    std::vector<float>& edges_msure, std::vector<float>& faces_msure) const
+{
+    out_vts.clear();
+    out_edges.clear();
+    out_tris.clear();
+    from_fi.clear();
+    vts_msure.clear();
+    edges_msure.clear();
+    faces_msure.clear();
+    if (tp != MeasureForMA::LAMBDA)
+        return;
+    std::vector<float> lv, le, lf;
+    all_measures(lv, le, lf);
+    const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
+              nf = (int)m_geom.numFaces();
+    const std::vector<int32_t> edges = m_geom.edgeArray();
+    std::vector<int64_t> foff(1, 0);
+    std::vector<int32_t> fvts, fv;
+    for (int f = 0; f < nf; ++f)
+    {
+        m_geom.getFaceVRep(f, fv);
+        fvts.insert(fvts.end(), fv.begin(), fv.end());
+        foff.push_back((int64_t)fvts.size());
+    }
+    fvts.push_back(0);
+    std::vector<uint8_t> valid((size_t)nv + 1, 0);
+    for (int v = 0; v < nv; ++v)
+        valid[v] = isVertexValid(v) ? 1 : 0;
+    int64_t n1 = 0, n2 = 0, n3 = 0;
+    int rc = vc_voro_extract_inside(nv, ne, nf, edges.data(), foff.data(),
+                                    fvts.data(), valid.data(), le.data(),
+                                    lf.data(), nullptr, 0, &n1, nullptr,
+                                    nullptr, 0, &n2, nullptr, nullptr, nullptr,
+                                    0, &n3);
+    if (rc != VC_OK && rc != VC_ERR_CAPACITY)
+        vcdev::check(rc, "vc_voro_extract_inside");
+    std::vector<int32_t> vi(n1 + 1), eo(2 * n2 + 2), tr(3 * n3 + 3), ff(n3 + 1);
+    std::vector<float> em(n2 + 1), fm(n3 + 1);
+    vcdev::check(vc_voro_extract_inside(nv, ne, nf, edges.data(), foff.data(),
+                                        fvts.data(), valid.data(), le.data(),
+                                        lf.data(), vi.data(), n1, &n1,
+                                        eo.data(), em.data(), n2, &n2,
+                                        tr.data(), ff.data(), fm.data(), n3,
+                                        &n3),
+                 "vc_voro_extract_inside");
+    for (int64_t i = 0; i < n1; ++i)
+    {
+        out_vts.push_back(m_geom.getVert(vi[i]));
+        vts_msure.push_back(lv[vi[i]]);
+    }
+    for (int64_t i = 0; i < n2; ++i)
+    {
+        out_edges.push_back(ivec2(eo[2 * i], eo[2 * i + 1]));
+        edges_msure.push_back(em[i]);
+    }
+    for (int64_t i = 0; i < n3; ++i)
+    {
+        out_tris.push_back(uTriFace((unsigned)tr[3 * i], (unsigned)tr[3 * i + 1],
+                                    (unsigned)tr[3 * i + 2]));
+        from_fi.push_back(ff[i]);
+        faces_msure.push_back(fm[i]);
+    }
+}
+
+void VoroInfo::mergeCloseVts(float eps, bool only_inside)
+{
+    const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
+              nf = (int)m_geom.numFaces();
+    std::vector<float> vts(3 * (size_t)nv + 3);
+    for (int v = 0; v < nv; ++v)
+        for (int a = 0; a < 3; ++a)
+            vts[3 * v + a] = m_geom.getVert(v)[a];
+    const std::vector<int32_t> edges = m_geom.edgeArray();
+    std::vector<int64_t> foff(1, 0), ve_off(nv + 1, 0);
+    std::vector<int32_t> fvts, fv, ve;
+    for (int f = 0; f < nf; ++f)
+    {
+        m_geom.getFaceVRep(f, fv);
+        fvts.insert(fvts.end(), fv.begin(), fv.end());
+        foff.push_back((int64_t)fvts.size());
+    }
+    for (int v = 0; v < nv; ++v)
+    {
+        for (int k = 0; k < m_geom.cntNbEdgesofVert(v); ++k)
+            ve.push_back(m_geom.nbEdgeofVert(v, k));
+        ve_off[v + 1] = (int64_t)ve.size();
+    }
+    fvts.push_back(0);
+    ve.push_back(0);
+    std::vector<uint8_t> valid((size_t)nv + 1, 0);
+    for (int v = 0; v < nv; ++v)
+        valid[v] = isVertexValid(v) ? 1 : 0;
+    std::vector<int32_t> o2n((size_t)nv + 1), nvts((size_t)nv + 1),
+        nfv(fvts.size() + 1), ned(2 * ((size_t)ne + nf) + 2);
+    std::vector<int64_t> nfo((size_t)nf + 2);
+    std::vector<uint8_t> rem((size_t)nf + 1), keep((size_t)ne + 1);
+    int32_t n_new = 0;
+    int64_t n_e = 0;
+    vcdev::check(vc_voro_merge_close(nv, ne, nf, vts.data(), edges.data(),
+                                     foff.data(), fvts.data(), ve_off.data(),
+                                     ve.data(), valid.data(), eps,
+                                     only_inside ? 1 : 0, o2n.data(),
+                                     nvts.data(), &n_new, nfo.data(),
+                                     nfv.data(), rem.data(), ned.data(),
+                                     (int64_t)ne + nf, &n_e, keep.data()),
+                 "vc_voro_merge_close");
+    // the new geometry: seeds, surviving edges, kept faces, then finalize
+    std::vector<point> new_v;
+    std::vector<bool> new_valid, new_finite;
+    for (int32_t i = 0; i < n_new; ++i)
+    {
+        new_v.push_back(m_geom.getVert(nvts[i]));
+        new_valid.push_back(m_vts_valid[nvts[i]]);
+        new_finite.push_back(onlyHasInside() || m_is_finite_v[nvts[i]]);
+    }
+    std::vector<ivec2> new_e;
+    for (int64_t i = 0; i < n_e; ++i)
+        new_e.push_back(ivec2(ned[2 * i], ned[2 * i + 1]));
+    cellcomplex g(new_v, new_e, {}, false);
+    int kept = 0;
+    std::vector<ivec2> new_sites;
+    for (int f = 0; f < nf; ++f)
+    {
+        if (rem[f])
+            continue;
+        g.appendFace(std::vector<int>(nfv.begin() + nfo[kept],
+                                      nfv.begin() + nfo[kept + 1]));
+        new_sites.push_back(m_face_sites[f]);
+        ++kept;
+    }
+    g.finalize();
+    m_geom = std::move(g);
+    m_face_sites = std::move(new_sites);
+    if (!onlyHasInside())
+    {
+        m_vts_valid = new_valid;
+        m_is_finite_v = new_finite;
+    }
+    else
+    {
+        m_vts_valid.assign(m_geom.numVts(), true);
+        m_is_finite_v.assign(m_geom.numVts(), true);
+    }
+    recompute_flags();
+}
+
+} // namespace voxelvoro

@@ -1,0 +1,101 @@
+// voroinfo.h -- VoroInfo with the reference's names for the Voronoi-side
+// steps of the pipeline (include/voroinfo.h, src/voroinfo.cpp): inside
+// tagging, lambda measures, inside extraction with measures, close-vertex
+// merging.  The diagram is given as data (setGeometry: vertices, face
+// polygons, the two sites of every face, site positions, finite flags), not
+// built from TetGen output.  Tagging and measures run on the device
+// (include/vc_voro.h); extraction and merging on the host, in the
+// reference's order.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "Volume3DScalar.h"
+#include "cellcomplex.h"
+#include "commondefs.h"
+
+namespace MeasureForMA
+{
+enum meassuretype
+{
+    LAMBDA = 0
+};
+// |a - b| as trimesh::dist (include/measureforMA_imp.h:1-4)
+float lambdaForFace(const point& a, const point& b);
+} // namespace MeasureForMA
+
+namespace voxelvoro
+{
+
+class VoroInfo
+{
+public:
+    VoroInfo() = default;
+    // vts: Voronoi vertices; faces: vertex polygons; face_sites: the two
+    // sites of each face; sites: site positions; finite_v: empty = all
+    // finite.  The complex is finalized (edges derived as cellcomplex).
+    void setGeometry(const std::vector<point>& vts,
+                     const std::vector<std::vector<int>>& faces,
+                     const std::vector<ivec2>& face_sites,
+                     const std::vector<point>& sites,
+                     const std::vector<bool>& finite_v = {});
+    const cellcomplex& geom() const { return m_geom; }
+    const std::vector<ivec2>& faceSites() const { return m_face_sites; }
+    const std::vector<point>& getSites() const { return m_site_positions; }
+
+    // src/voroinfo.cpp:447-496
+    bool tagVert(const point& p, const std::shared_ptr<Volume3DScalar>& vol) const;
+    bool tagVtsUsingUniformVol(const std::shared_ptr<Volume3DScalar>& vol);
+    float getInsidePartSize() const;
+    void set_only_inside(bool yesno) { m_only_inside = yesno ? 1 : 0; }
+    bool onlyHasInside() const { return m_only_inside > 0; }
+
+    bool isVertexValid(int vi) const;
+    bool isEdgeValid(int ei) const;
+    bool isFaceValid(int fi) const;
+    void getValidVts(std::vector<int>& idx) const;
+    void getValidEdges(std::vector<int>& idx) const;
+    void getValidFaces(std::vector<int>& idx) const;
+
+    // src/voroinfo.cpp:1432-1574 (LAMBDA)
+    void computeVertexMeasure(MeasureForMA::meassuretype tp,
+                              const std::vector<int>& vts_indices,
+                              std::vector<float>& vts_msure) const;
+    void computeEdgesMeasure(MeasureForMA::meassuretype tp,
+                             const std::vector<int>& edges_indices,
+                             std::vector<float>& edges_msure) const;
+    void computeFacesMeasure(MeasureForMA::meassuretype tp,
+                             const std::vector<int>& faces_indices,
+                             std::vector<float>& faces_msure) const;
+
+    // src/voroinfo.cpp:624-729
+    void extractInsideWithMeasure(MeasureForMA::meassuretype tp,
+                                  std::vector<point>& out_vts,
+                                  std::vector<ivec2>& out_edges,
+                                  std::vector<uTriFace>& out_tris,
+                                  std::vector<int>& from_fi,
+                                  std::vector<float>& vts_msure,
+                                  std::vector<float>& edges_msure,
+                                  std::vector<float>& faces_msure) const;
+
+    // src/voroinfo.cpp:1064-1384; the surviving edges are appended in
+    // ascending order (the reference iterates an unordered_set)
+    void mergeCloseVts(float eps, bool only_inside);
+
+private:
+    void all_measures(std::vector<float>& lv, std::vector<float>& le,
+                      std::vector<float>& lf) const;
+    void recompute_flags();
+
+    cellcomplex m_geom;
+    std::vector<ivec2> m_face_sites;
+    std::vector<point> m_site_positions;
+    std::vector<bool> m_vts_valid, m_is_finite_v, m_edge_valid, m_face_valid;
+    bool m_tagged = false;
+    int m_only_inside = -1;
+    point m_bb_min, m_bb_max;
+    bool m_bb_valid = false;
+};
+
+} // namespace voxelvoro
