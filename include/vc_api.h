@@ -57,6 +57,7 @@ extern "C" {
 #define VC_ERR_NOMEM (-5)     /* device allocation failed                 */
 #define VC_ERR_STATE (-6)     /* prerequisite stage not run yet           */
 #define VC_ERR_GUARD (-7)     /* a device-side iteration guard tripped    */
+#define VC_ERR_IO (-8)        /* a file could not be opened or written    */
 
 /* ---- limits ------------------------------------------------------------ */
 /* each axis <= 32767, nx*ny*nz < 2^31, packed feature coordinates fit in

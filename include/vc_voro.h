@@ -21,6 +21,10 @@
  *   vc_voro_extract_inside  VoroInfo::extractInsideWithMeasure
  *                        (src/voroinfo.cpp:624-729)
  *   vc_voro_merge_close  VoroInfo::mergeCloseVts (src/voroinfo.cpp:1064-1384)
+ *   vc_voro_radii        the radius step of VoroInfo::computeInfoRelatedtoSites
+ *                        and of the TetGen loader (src/voroinfo.cpp:290-325,
+ *                        :404-441)
+ *   vc_voro_part_inside  VoroInfo::getPartInside (src/voroinfo.cpp:573-620)
  */
 #ifndef VC_VORO_H
 #define VC_VORO_H
@@ -108,6 +112,28 @@ int vc_voro_merge_close(int32_t nv, int32_t ne, int32_t nf, const float* vts,
                         int32_t* new_face_vts, uint8_t* remove_face,
                         int32_t* new_edges, int64_t edge_cap,
                         int64_t* n_new_edges, uint8_t* keep_edge);
+
+/* Radius of every vertex (host arrays, computed on the device, waits): faces
+ * are visited in order and every vertex v of face f is given
+ * |v - site[face_sites[2f]]| (float, ((d0^2 + d1^2) + d2^2), correctly
+ * rounded sqrt), so the last face holding v decides; 0 for a vertex in no
+ * face. */
+int vc_voro_radii(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
+                  const int64_t* face_off, const int32_t* face_vts,
+                  const float* sites, int32_t nsites, const int32_t* face_sites,
+                  float* radii);
+
+/* Inside part (host): vts_idx = valid vertex ids ascending; tris = for every
+ * face in order, the triangles of its zig-zag triangulation whose three
+ * corners are valid (the face need not be), renumbered into vts_idx order;
+ * edges = the sides of those triangles, makeEdge-normalised, deduplicated
+ * and sorted ascending (the reference collects them in a std::set).
+ * Capacities and VC_ERR_CAPACITY as in vc_voro_extract_inside. */
+int vc_voro_part_inside(int32_t nv, int32_t nf, const int64_t* face_off,
+                        const int32_t* face_vts, const uint8_t* v_valid,
+                        int32_t* vts_idx, int64_t vts_cap, int64_t* n_vts,
+                        int32_t* edges_out, int64_t edge_cap, int64_t* n_edges,
+                        uint32_t* tris, int64_t tri_cap, int64_t* n_tris);
 
 #ifdef __cplusplus
 }

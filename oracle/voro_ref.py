@@ -12,6 +12,11 @@ float32 arithmetic:
   merge_close       mergeCloseVts (src/voroinfo.cpp:1064-1384), new edges
                     in ascending order (the reference iterates an
                     unordered_set)
+  radii             the radius loop of computeInfoRelatedtoSites and the
+                    TetGen loader (src/voroinfo.cpp:290-325, :404-441): faces
+                    in order, each vertex takes its distance to the face's
+                    first site, the last assignment stays
+  part_inside       getPartInside (src/voroinfo.cpp:573-620)
 Pinned by hand-derived known answers in tests/test_voro.py.
 """
 import collections
@@ -185,3 +190,32 @@ def merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
             new_faces.append(fv)
             remove.append(0)
     return old_to_new, new_vts, new_faces, remove, sorted(eset), keep
+
+
+def radii(vts, faces_vts, face_sites, sites):
+    vts = np.asarray(vts, np.float32).reshape(-1, 3)
+    sites = np.asarray(sites, np.float32).reshape(-1, 3)
+    r = np.zeros(len(vts), np.float32)
+    for f, poly in enumerate(faces_vts):
+        s = sites[face_sites[f][0]]
+        for v in poly:
+            d = (vts[v] - s).astype(np.float32)
+            q = np.float32(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1]))
+            q = np.float32(q + np.float32(d[2] * d[2]))
+            r[v] = np.sqrt(q)
+    return r
+
+
+def part_inside(nv, faces_vts, v_valid):
+    vi = [v for v in range(nv) if v_valid[v]]
+    new = {v: i for i, v in enumerate(vi)}
+    tris, es = [], set()
+    for poly in faces_vts:
+        for t in zigzag(list(poly)):
+            if all(v_valid[v] for v in t):
+                c = [new[v] for v in t]
+                tris.append(tuple(c))
+                for k in range(3):
+                    a, b = c[k], c[(k + 1) % 3]
+                    es.add((min(a, b), max(a, b)))
+    return vi, sorted(es), tris

@@ -2,11 +2,16 @@
 against the oracle."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
 
 from conftest import say
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+import export_ref  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -35,16 +40,28 @@ def test_cli_vol2mesh_matches_oracle(tmp_path, oracle):
     occ = synth.torus_c1()
     raw = str(tmp_path / "t.raw")
     occ.tofile(raw)
-    out = str(tmp_path / "t.off")
+    base = str(tmp_path / "t")
     say("cli vol2mesh torus")
-    r = subprocess.run([CLI, "-md=vol2mesh", raw, out, "-dims=64,64,64", "-needEuler"],
+    r = subprocess.run([CLI, "-md=vol2mesh", raw, base, "-dims=64,64,64", "-needEuler",
+                        "-onlyBndryVts=false", "-write2node"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    v, t = _read_off(out)
+    v, t = _read_off(base + ".off")
     s, q, e = oracle.boundary(occ)
     tris = np.concatenate([q[:, [0, 1, 2]], q[:, [2, 3, 0]]], axis=1).reshape(-1, 3)
     assert np.array_equal(v, s) and np.array_equal(t, tris)
     assert "euler = V - E + F = %d" % e["euler"] in r.stdout
+    with open(base + ".off", "rb") as f:
+        assert f.read() == export_ref.off(s, tris)
+    with open(base + ".node", "rb") as f:
+        assert f.read() == export_ref.tetnodes(s)
+    # default -onlyBndryVts: the boundary points alone
+    say("cli vol2mesh torus points")
+    r = subprocess.run([CLI, "-md=vol2mesh", raw, base + "_pts", "-dims=64,64,64"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(base + "_pts.off", "rb") as f:
+        assert f.read() == export_ref.off(s)
 
 
 def test_cli_vol2skel_matches_oracle(tmp_path, oracle):
@@ -56,7 +73,8 @@ def test_cli_vol2skel_matches_oracle(tmp_path, oracle):
     tt = (0.01, 0.03)
     say("cli vol2skel torus")
     r = subprocess.run([CLI, "-md=vol2skel", raw, prefix, "-dims=64,64,64",
-                        "-tt=%g,%g" % tt], capture_output=True, text=True, timeout=120)
+                        "-tt=%g,%g" % tt, "-outToDotma"], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     levels = synth.levels_from_tt(synth.inside_size(occ), tt)
     d, f = oracle.edt_ft(occ)
@@ -65,3 +83,11 @@ def test_cli_vol2skel_matches_oracle(tmp_path, oracle):
         pts = _read_ply_points("%s_thinned%g.ply" % (prefix, t))
         exp = np.argwhere((lab == 1) | (lab > 2 + k)).astype(np.float32)
         assert np.array_equal(pts, exp)
+        idx = exp.astype(np.int64)
+        rad = np.sqrt(d[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.float32))
+        with open("%s_thinned%g.ply" % (prefix, t), "rb") as fh:
+            assert fh.read() == export_ref.ply(exp)
+        with open("%s_thinned%g.r" % (prefix, t), "rb") as fh:
+            assert fh.read() == export_ref.radii(exp, rad)
+        with open("%s_thinned%g.ma" % (prefix, t), "rb") as fh:
+            assert fh.read() == export_ref.dotma(exp, rad)

@@ -65,3 +65,20 @@ def test_measures_match_oracle(vc):
         assert np.array_equal(lf, lf_r)
         assert np.array_equal(le, le_r)
         assert np.array_equal(lv, lv_r)
+
+
+def test_radii_match_oracle(vc):
+    rng = np.random.default_rng(4)
+    for it in range(5):
+        nv, edges, faces, fs, sites = _complex(rng)
+        vts = (rng.random((nv, 3)) * 40 - 5).astype(np.float32)
+        say("radii", it, nv, len(faces))
+        got = V.voro_radii(vc, vts, faces, fs, sites)
+        assert np.array_equal(got, R.radii(vts, faces, fs, sites))
+    # known answer: the last face holding a vertex decides; no face -> 0
+    got = V.voro_radii(vc, [(3, 4, 0), (3, 4, 12), (0, 0, 0), (9, 9, 9)],
+                       [[0, 1, 2], [0, 2]], [(0, 1), (2, 0)],
+                       [(0, 0, 0), (1, 1, 1), (0, 0, 4)])
+    assert got[0] == np.float32(np.sqrt(np.float32(41))) and got[1] == 13.0
+    assert got[2] == 4.0 and got[3] == 0.0
+    assert len(V.voro_radii(vc, [], [], [], [(0, 0, 0)])) == 0

@@ -120,3 +120,44 @@ def test_merge_matches_oracle_random():
             assert list(got[3]) == ref[3]
             assert [tuple(e) for e in got[4]] == ref[4]
             assert list(got[5]) == ref[5]
+
+
+def test_part_inside_known_answer():
+    """Pentagon [0..4] with vertex 4 invalid: zig-zag gives (4,0,1), (4,1,3),
+    (3,1,2); only (3,1,2) has three valid corners -> (3,1,2) in the new
+    numbering 0..3, sides (1,3), (1,2), (2,3) sorted.  The invalid face still
+    contributes its valid triangle (the reference tests triangles, not
+    faces)."""
+    for impl in (R.part_inside, V.voro_part_inside):
+        vi, eo, tr = impl(5, [[0, 1, 2, 3, 4]], [1, 1, 1, 1, 0])
+        assert list(vi) == [0, 1, 2, 3]
+        assert [tuple(int(x) for x in t) for t in tr] == [(3, 1, 2)]
+        assert [tuple(int(x) for x in e) for e in eo] == [(1, 2), (1, 3), (2, 3)]
+
+
+def test_part_inside_matches_oracle_random():
+    rng = np.random.default_rng(11)
+    for it in range(60):
+        nv, edges, faces = _random_complex(rng)
+        valid = (rng.random(nv) < 0.8).astype(np.uint8)
+        got = V.voro_part_inside(nv, faces, valid)
+        ref = R.part_inside(nv, faces, valid)
+        assert list(got[0]) == ref[0]
+        assert [tuple(int(x) for x in e) for e in got[1]] == ref[1]
+        assert [tuple(int(x) for x in t) for t in got[2]] == ref[2]
+
+
+def test_radii_oracle_known_answer():
+    """Vertex 0 lies in faces 0 and 1; face 1 comes last, so its first site
+    (site 2 at (0, 0, 4)) decides: |(3, 4, 0) - (0, 0, 4)| = sqrt(41).
+    Vertex 1 only in face 0 (first site 0 at the origin): |(3, 4, 12)| = 13.
+    Vertex 3 is in no face: 0."""
+    vts = [(3, 4, 0), (3, 4, 12), (0, 0, 0), (9, 9, 9)]
+    faces = [[0, 1, 2], [2, 0, 1]]
+    fs = [(0, 1), (2, 0)]
+    sites = [(0, 0, 0), (1, 1, 1), (0, 0, 4)]
+    r = R.radii(vts, [[0, 1, 2], [0, 2]], fs, sites)
+    assert r[0] == np.float32(np.sqrt(np.float32(41))) and r[1] == 13.0
+    assert r[2] == 4.0 and r[3] == 0.0
+    r2 = R.radii(vts, faces, fs, sites)
+    assert r2[1] == np.float32(np.sqrt(np.float32(9 + 16 + 64)))

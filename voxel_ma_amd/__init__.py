@@ -21,6 +21,7 @@ VC_ERR_EMPTY_VOL = -4
 VC_ERR_NOMEM = -5
 VC_ERR_STATE = -6
 VC_ERR_GUARD = -7
+VC_ERR_IO = -8
 VC_MAX_LEVELS = 64
 
 BUF_OCC, BUF_EDT, BUF_FT, BUF_LAM2, BUF_LABEL, BUF_LEVEL = range(6)
@@ -46,9 +47,13 @@ EXPORTS = [
     "vc_surrogate_measure", "vc_surrogate_activation",
     # include/vc_voro.h
     "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside", "vc_voro_merge_close",
+    "vc_voro_radii", "vc_voro_part_inside",
     # include/vc_octree.h
     "vc_octree_parse", "vc_octree_free", "vc_octree_value", "vc_volume_from_octree",
     "vc_octree_leaf_centers",
+    # include/vc_export.h
+    "vc_write_ply", "vc_write_radii", "vc_write_dotma", "vc_write_off", "vc_write_tetnodes",
+    "vc_xform_apply",
 ]
 SURROGATE_PARAMS = 678786
 SURROGATE_HALO = 56
@@ -175,6 +180,15 @@ def library():
                                    I64, ctypes.POINTER(I64), P, P, P, I64, ctypes.POINTER(I64)],
         "vc_voro_merge_close": [I, I, I, P, P, P, P, P, P, P, ctypes.c_float, I, P, P,
                                 ctypes.POINTER(I), P, P, P, P, I64, ctypes.POINTER(I64), P],
+        "vc_voro_radii": [P, I, I, P, P, P, P, I, P, P],
+        "vc_voro_part_inside": [I, I, P, P, P, P, I64, ctypes.POINTER(I64), P, I64,
+                                ctypes.POINTER(I64), P, I64, ctypes.POINTER(I64)],
+        "vc_write_ply": [ctypes.c_char_p, P, I64, P, I64, P, I64],
+        "vc_write_radii": [ctypes.c_char_p, P, P, I64, P],
+        "vc_write_dotma": [ctypes.c_char_p, P, P, I64, P, I64, P, I64],
+        "vc_write_off": [ctypes.c_char_p, P, I64, P, I64, P],
+        "vc_write_tetnodes": [ctypes.c_char_p, P, I64],
+        "vc_xform_apply": [P, P, I64, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -729,6 +743,126 @@ def voro_merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
     faces = [list(nfv[nfo[i]:nfo[i + 1]]) for i in range(kept)]
     return (list(o2n[:nv]), list(nvts[:nn.value]), faces, list(rem[:nf]),
             [tuple(x) for x in ne_out[:nne.value]], list(keep[:len(e)]))
+
+
+def voro_radii(ctx, vts, faces_vts, face_sites, sites):
+    """vc_voro_radii: per-vertex radius (last face holding the vertex decides)."""
+    vts = np.ascontiguousarray(np.asarray(vts, np.float32).reshape(-1, 3))
+    nv = len(vts)
+    _, foff, fv = _complex_arrays(nv, [], faces_vts)
+    fs = np.ascontiguousarray(np.array(face_sites, np.int32).reshape(-1, 2))
+    st = np.ascontiguousarray(np.asarray(sites, np.float32).reshape(-1, 3))
+    fs = fs if fs.size else np.zeros((1, 2), np.int32)
+    st1 = st if st.size else np.zeros((1, 3), np.float32)
+    r = np.zeros(max(nv, 1), np.float32)
+    ctx.check(library().vc_voro_radii(ctx.handle, nv, len(faces_vts), _p(vts), _p(foff), _p(fv),
+                                      _p(st1), len(st), _p(fs), _p(r)))
+    return r[:nv]
+
+
+def voro_part_inside(nv, faces_vts, v_valid):
+    """vc_voro_part_inside: (vts_idx, edges, tris) of the valid part."""
+    _, foff, fv = _complex_arrays(nv, [], faces_vts)
+    vv = np.ascontiguousarray(np.asarray(v_valid, np.uint8))
+    vv = vv if vv.size else np.zeros(1, np.uint8)
+    L = library()
+    n1, n2, n3 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    rc = L.vc_voro_part_inside(nv, len(faces_vts), _p(foff), _p(fv), _p(vv), None, 0,
+                               ctypes.byref(n1), None, 0, ctypes.byref(n2), None, 0,
+                               ctypes.byref(n3))
+    if rc not in (VC_OK, VC_ERR_CAPACITY):
+        raise VcError(rc, "vc_voro_part_inside")
+    vi = np.zeros(max(n1.value, 1), np.int32)
+    eo = np.zeros((max(n2.value, 1), 2), np.int32)
+    tr = np.zeros((max(n3.value, 1), 3), np.uint32)
+    rc = L.vc_voro_part_inside(nv, len(faces_vts), _p(foff), _p(fv), _p(vv), _p(vi), n1.value,
+                               ctypes.byref(n1), _p(eo), n2.value, ctypes.byref(n2), _p(tr),
+                               n3.value, ctypes.byref(n3))
+    if rc != VC_OK:
+        raise VcError(rc, "vc_voro_part_inside")
+    return vi[:n1.value], eo[:n2.value], tr[:n3.value]
+
+
+# ---------------------------------------------------------------- output files
+def _f32(a, k):
+    a = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, k))
+    return a, (a if a.size else np.zeros((1, k), np.float32))
+
+
+def _idx(a, k, dt):
+    a = np.ascontiguousarray(np.asarray(a, dt).reshape(-1, k))
+    return a, (a if a.size else np.zeros((1, k), dt))
+
+
+def _mat(m):
+    if m is None:
+        return None, None
+    m = np.ascontiguousarray(np.asarray(m, np.float64).reshape(16))
+    return m, _p(m)
+
+
+def _io(rc, what):
+    if rc != VC_OK:
+        raise VcError(rc, what)
+
+
+def write_ply(path, vts, edges=(), tris=()):
+    """vc_write_ply: ASCII PLY (vertex x y z, edge vertex1 vertex2, face
+    vertex_indices) as the reference writes it."""
+    v, v1 = _f32(vts, 3)
+    e, e1 = _idx(edges, 2, np.int32)
+    t, t1 = _idx(tris, 3, np.int32)
+    _io(library().vc_write_ply(os.fsencode(path), _p(v1), len(v), _p(e1), len(e), _p(t1),
+                               len(t)), "vc_write_ply")
+
+
+def write_radii(path, vts, radii, vox_to_model=None):
+    """vc_write_radii: .r file, vertices and radii through vox_to_model
+    (16 doubles, column-major; None = identity)."""
+    v, v1 = _f32(vts, 3)
+    r, r1 = _f32(radii, 1)
+    if len(r) != len(v):
+        raise ValueError("one radius per vertex")
+    m, mp = _mat(vox_to_model)
+    _io(library().vc_write_radii(os.fsencode(path), _p(v1), _p(r1), len(v), mp),
+        "vc_write_radii")
+
+
+def write_dotma(path, vts, radii, edges=(), tris=()):
+    """vc_write_dotma: QMAT .ma file."""
+    v, v1 = _f32(vts, 3)
+    r, r1 = _f32(radii, 1)
+    if len(r) != len(v):
+        raise ValueError("one radius per vertex")
+    e, e1 = _idx(edges, 2, np.int32)
+    t, t1 = _idx(tris, 3, np.uint32)
+    _io(library().vc_write_dotma(os.fsencode(path), _p(v1), _p(r1), len(v), _p(e1), len(e),
+                                 _p(t1), len(t)), "vc_write_dotma")
+
+
+def write_off(path, vts, tris=(), vox_to_model=None):
+    """vc_write_off: OFF as trimesh2 writes it (vertices through
+    vox_to_model)."""
+    v, v1 = _f32(vts, 3)
+    t, t1 = _idx(tris, 3, np.uint32)
+    m, mp = _mat(vox_to_model)
+    _io(library().vc_write_off(os.fsencode(path), _p(v1), len(v), _p(t1), len(t), mp),
+        "vc_write_off")
+
+
+def write_tetnodes(path, vts):
+    """vc_write_tetnodes: TetGen .node file."""
+    v, v1 = _f32(vts, 3)
+    _io(library().vc_write_tetnodes(os.fsencode(path), _p(v1), len(v)), "vc_write_tetnodes")
+
+
+def xform_apply(m, pts):
+    """vc_xform_apply: trimesh-style 4x4 transform of float points."""
+    p, p1 = _f32(pts, 3)
+    out = np.zeros_like(p1)
+    mm, mp = _mat(m)
+    _io(library().vc_xform_apply(mp, _p(p1), len(p), _p(out)), "vc_xform_apply")
+    return out[:len(p)]
 
 
 # ---------------------------------------------------------------- octree input

@@ -301,4 +301,63 @@ int vc_voro_merge_close(int32_t nv, int32_t ne, int32_t nf, const float* vts,
     return VC_OK;
 }
 
+int vc_voro_part_inside(int32_t nv, int32_t nf, const int64_t* face_off,
+                        const int32_t* face_vts, const uint8_t* v_valid,
+                        int32_t* vts_idx, int64_t vts_cap, int64_t* n_vts,
+                        int32_t* edges_out, int64_t edge_cap, int64_t* n_edges,
+                        uint32_t* tris, int64_t tri_cap, int64_t* n_tris)
+{
+    int rc = check_complex(nv, 0, nf, nullptr, face_off, face_vts);
+    if (rc != VC_OK || (nv > 0 && !v_valid) || !n_vts || !n_edges || !n_tris)
+        return VC_ERR_INVALID;
+    std::vector<int32_t> vid, new_id((size_t)nv, 0);
+    for (int32_t v = 0; v < nv; ++v)
+        if (v_valid[v])
+        {
+            new_id[(size_t)v] = (int32_t)vid.size();
+            vid.push_back(v);
+        }
+    // triangles of every face whose three corners are valid (the face itself
+    // need not be), renumbered; their sides, deduplicated and sorted
+    std::vector<uint32_t> t_out;
+    std::vector<uint64_t> eset;
+    std::vector<int32_t> ft;
+    for (int32_t f = 0; f < nf; ++f)
+    {
+        zigzag(face_vts + face_off[f], (int)(face_off[f + 1] - face_off[f]), ft);
+        for (size_t t = 0; t < ft.size(); t += 3)
+        {
+            if (!(v_valid[ft[t]] && v_valid[ft[t + 1]] && v_valid[ft[t + 2]]))
+                continue;
+            int32_t c[3];
+            for (int k = 0; k < 3; ++k)
+            {
+                c[k] = new_id[(size_t)ft[t + k]];
+                t_out.push_back((uint32_t)c[k]);
+            }
+            for (int k = 0; k < 3; ++k)
+                eset.push_back(edge_key(c[k], c[(k + 1) % 3]));
+        }
+    }
+    std::sort(eset.begin(), eset.end());
+    eset.erase(std::unique(eset.begin(), eset.end()), eset.end());
+    *n_vts = (int64_t)vid.size();
+    *n_edges = (int64_t)eset.size();
+    *n_tris = (int64_t)(t_out.size() / 3);
+    if ((vts_idx && vts_cap < *n_vts) || (edges_out && edge_cap < *n_edges) ||
+        (tris && tri_cap < *n_tris))
+        return VC_ERR_CAPACITY;
+    if (vts_idx)
+        std::copy(vid.begin(), vid.end(), vts_idx);
+    if (edges_out)
+        for (size_t i = 0; i < eset.size(); ++i)
+        {
+            edges_out[2 * i] = (int32_t)(eset[i] >> 32);
+            edges_out[2 * i + 1] = (int32_t)(eset[i] & 0xFFFFFFFFull);
+        }
+    if (tris)
+        std::copy(t_out.begin(), t_out.end(), tris);
+    return VC_OK;
+}
+
 } // extern "C"

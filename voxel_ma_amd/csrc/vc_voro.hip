@@ -107,6 +107,44 @@ __global__ void __launch_bounds__(kBlock)
     lam_v[v] = m;
 }
 
+// last[v] = the highest face id whose polygon holds v (last starts at -1)
+__global__ void __launch_bounds__(kBlock)
+    face_last(const int64_t* __restrict__ face_off,
+              const int32_t* __restrict__ face_vts, int32_t nf,
+              int32_t* __restrict__ last)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    for (int64_t j = face_off[f]; j < face_off[f + 1]; ++j)
+        atomicMax(&last[face_vts[j]], f);
+}
+
+// r[v] = |v - first site of face last[v]| (trimesh::dist order), 0 if v is
+// in no face
+__global__ void __launch_bounds__(kBlock)
+    vertex_radius(const float* __restrict__ vts, const float* __restrict__ sites,
+                  const int32_t* __restrict__ face_sites,
+                  const int32_t* __restrict__ last, int32_t nv,
+                  float* __restrict__ r)
+{
+#pragma clang fp contract(off)
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv)
+        return;
+    const int32_t f = last[v];
+    if (f < 0)
+    {
+        r[v] = 0.0f;
+        return;
+    }
+    const int32_t s = face_sites[2 * f];
+    const float d0 = vts[3 * v] - sites[3 * s];
+    const float d1 = vts[3 * v + 1] - sites[3 * s + 1];
+    const float d2 = vts[3 * v + 2] - sites[3 * s + 2];
+    r[v] = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+}
+
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // device copies that are freed on every exit path
@@ -265,6 +303,75 @@ int vc_voro_measures(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
     if (lam_v && nv > 0)
         VC_HIP(ctx, hipMemcpyAsync(lam_v, dlv, 4 * (size_t)nv,
                                    hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+int vc_voro_radii(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
+                  const int64_t* face_off, const int32_t* face_vts,
+                  const float* sites, int32_t nsites, const int32_t* face_sites,
+                  float* radii)
+{
+    if (!ctx || nv < 0 || nf < 0 || nsites < 0 || (nv > 0 && (!vts || !radii)) ||
+        (nf > 0 && (!face_off || !face_vts || !face_sites || !sites)))
+        return VC_ERR_INVALID;
+    if (nf > 0 && face_off[0] != 0)
+        return vc_fail(ctx, VC_ERR_INVALID, "face offsets must start at 0");
+    for (int32_t f = 0; f < nf; ++f)
+    {
+        if (face_off[f + 1] < face_off[f])
+            return vc_fail(ctx, VC_ERR_INVALID, "face offsets not ascending");
+        if (face_sites[2 * f] < 0 || face_sites[2 * f] >= nsites ||
+            face_sites[2 * f + 1] < 0 || face_sites[2 * f + 1] >= nsites)
+            return vc_fail(ctx, VC_ERR_INVALID, "face site out of range");
+    }
+    const int64_t nfv = nf > 0 ? face_off[nf] : 0;
+    for (int64_t j = 0; j < nfv; ++j)
+        if (face_vts[j] < 0 || face_vts[j] >= nv)
+            return vc_fail(ctx, VC_ERR_INVALID, "face vertex out of range");
+    if (nv == 0)
+        return VC_OK;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    float *dv, *ds, *dr;
+    int32_t *dfv, *dfs, *dlast;
+    int64_t* dfo;
+    int rc = sc.get(ctx, &dv, 3 * (size_t)nv);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dr, (size_t)nv);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dlast, (size_t)nv);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &ds, 3 * (size_t)nsites);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dfs, 2 * (size_t)nf);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dfo, (size_t)nf + 1);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dfv, (size_t)nfv);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipMemcpyAsync(dv, vts, 12 * (size_t)nv, hipMemcpyHostToDevice, S));
+    VC_HIP(ctx, hipMemsetAsync(dlast, 0xFF, 4 * (size_t)nv, S));
+    if (nf > 0)
+    {
+        if (nsites > 0)
+            VC_HIP(ctx, hipMemcpyAsync(ds, sites, 12 * (size_t)nsites,
+                                       hipMemcpyHostToDevice, S));
+        VC_HIP(ctx, hipMemcpyAsync(dfs, face_sites, 8 * (size_t)nf,
+                                   hipMemcpyHostToDevice, S));
+        VC_HIP(ctx, hipMemcpyAsync(dfo, face_off, 8 * ((size_t)nf + 1),
+                                   hipMemcpyHostToDevice, S));
+        if (nfv > 0)
+            VC_HIP(ctx, hipMemcpyAsync(dfv, face_vts, 4 * (size_t)nfv,
+                                       hipMemcpyHostToDevice, S));
+        face_last<<<blocks_for(nf), kBlock, 0, S>>>(dfo, dfv, nf, dlast);
+        VC_CHECK_LAUNCH(ctx);
+    }
+    vertex_radius<<<blocks_for(nv), kBlock, 0, S>>>(dv, ds, dfs, dlast, nv, dr);
+    VC_CHECK_LAUNCH(ctx);
+    VC_HIP(ctx, hipMemcpyAsync(radii, dr, 4 * (size_t)nv, hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }

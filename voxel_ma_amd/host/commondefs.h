@@ -40,12 +40,58 @@ using ivec3 = VcVec<3, int>;
 using ivec4 = VcVec<4, int>;
 using uTriFace = VcVec<3, unsigned int>;
 
-// affine voxel <-> model transform; the reference's Surfacer and thinning use
-// the identity (src/surfacing.cpp and include/surfacing.h:97-119)
+// 4x4 voxel <-> model transform, column-major doubles as trimesh::xform
+// (dense volumes and .sof octrees use the identity; .sog octrees a scale and
+// a translation, src/octree.cpp:461-475)
 struct xform
 {
+    std::array<double, 16> m{{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     static xform identity() { return xform(); }
+    static xform trans(double x, double y, double z)
+    {
+        xform t;
+        t.m[12] = x;
+        t.m[13] = y;
+        t.m[14] = z;
+        return t;
+    }
+    static xform trans(const VcVec<3, float>& p) { return trans(p[0], p[1], p[2]); }
+    static xform scale(double s)
+    {
+        xform t;
+        t.m[0] = t.m[5] = t.m[10] = s;
+        return t;
+    }
+    double& operator[](int i) { return m[i]; }
+    double operator[](int i) const { return m[i]; }
+    const double* data() const { return m.data(); }
 };
+
+// matrix product, each entry summed left to right over k
+inline xform operator*(const xform& a, const xform& b)
+{
+    xform c;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+        {
+            double s = a[i] * b[4 * j];
+            for (int k = 1; k < 4; ++k)
+                s += a[i + 4 * k] * b[k + 4 * j];
+            c.m[i + 4 * j] = s;
+        }
+    return c;
+}
+
+extern "C" int vc_xform_apply(const double* m, const float* pts, int64_t n,
+                              float* out);
+
+// point transform (homogeneous, double accumulation, rounded to float)
+inline VcVec<3, float> operator*(const xform& x, const VcVec<3, float>& p)
+{
+    VcVec<3, float> q;
+    vc_xform_apply(x.data(), p.data(), 1, q.begin());
+    return q;
+}
 
 // Euler characteristic summary (reference include/commondefs.h:60-85)
 struct eulerchar

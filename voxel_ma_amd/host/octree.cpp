@@ -40,6 +40,20 @@ bool OctreeVolume::read(const std::string& file, bool sog)
     m_sizes = ivec3(m_max_res + 1, m_max_res + 1, m_max_res + 1);
     m_lower_left = point(info.lower_left[0], info.lower_left[1], info.lower_left[2]);
     m_bbox_len = info.bbox_len;
+    if (sog)
+    {
+        // voxel -> model: scale by bbox edge / R, then translate to the lower
+        // left corner (src/octree.cpp:461-475)
+        const double sc = (double)m_bbox_len / (double)m_max_res;
+        m_vox_to_mesh = xform::trans(m_lower_left) * xform::scale(sc);
+        const point neg(-m_lower_left[0], -m_lower_left[1], -m_lower_left[2]);
+        m_mesh_to_vox = xform::scale(1.0 / sc) * xform::trans(neg);
+    }
+    else
+    {
+        m_vox_to_mesh = xform::identity();
+        m_mesh_to_vox = xform::identity();
+    }
     return true;
 }
 

@@ -1,16 +1,21 @@
 // voroUtility.cpp -- command-line front end keeping the reference's mode and
 // flag names (src/voroUtility.cpp:21-142) for the modes of the thinning path:
 //
-//   -md=vol2mesh <in> <out.off> [-needEuler]      boundary mesh + Euler
-//   -md=vol2skel <in> <out_prefix> [-tt=a,b,c]     EDT + measure + thinning,
-//                                                  one <prefix>_thinned<t>.ply
-//                                                  per threshold
+//   -md=vol2mesh <in> <out_base> [-onlyBndryVts=false] [-needEuler]
+//                [-write2node]                     <out_base>.off: boundary
+//                                                  points (default) or mesh
+//   -md=vol2skel <in> <out_prefix> [-tt=a,b,c] [-outToDotma]
+//                                                  EDT + measure + thinning;
+//                                                  per threshold t:
+//                                                  <prefix>_thinned<t>.ply
+//                                                  and .r (and .ma)
 //   -md=topo -meshToCheck=<m.off>                  closed pockets of a mesh
 //
 // Volumes: .mrc (modes 0, 1, 2, 6), .sof / .sog octrees, or raw uint8 with
 // -dims=nx,ny,nz (C order [x][y][z]).  Occupancy = value > 0.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +27,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/vc_export.h"
+#include "exporters.h"
 #include "highlevelalgo.h"
 #include "octree.h"
 #include "surfacing.h"
@@ -120,20 +127,6 @@ std::shared_ptr<DenseVolume> read_raw(const std::string& path, const int d[3])
     return DenseVolume::fromOccupancy(d[0], d[1], d[2], b.data());
 }
 
-bool write_off(const std::string& path, const std::vector<point>& v,
-               const std::vector<uTriFace>& t)
-{
-    std::ofstream f(path);
-    if (!f)
-        return false;
-    f << "OFF\n" << v.size() << " " << t.size() << " 0\n";
-    for (const auto& p : v)
-        f << p[0] << " " << p[1] << " " << p[2] << "\n";
-    for (const auto& x : t)
-        f << "3 " << x[0] << " " << x[1] << " " << x[2] << "\n";
-    return (bool)f;
-}
-
 bool read_off(const std::string& path, std::vector<point>& v,
               std::vector<std::vector<int>>& faces)
 {
@@ -157,24 +150,24 @@ bool read_off(const std::string& path, std::vector<point>& v,
     return (bool)f;
 }
 
-bool write_ply_points(const std::string& path, const std::vector<ivec3>& vox)
+bool flag_on(const std::map<std::string, std::string>& flags, const std::string& name,
+             bool dflt)
 {
-    std::ofstream f(path);
-    if (!f)
+    if (flags.count("no" + name))
         return false;
-    f << "ply\nformat ascii 1.0\nelement vertex " << vox.size()
-      << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
-    for (const auto& p : vox)
-        f << p[0] << " " << p[1] << " " << p[2] << "\n";
-    return (bool)f;
+    auto it = flags.find(name);
+    if (it == flags.end())
+        return dflt;
+    return !(it->second == "false" || it->second == "0");
 }
 
 int usage()
 {
-    std::cerr << "usage: voroUtility -md=vol2mesh <in.mrc|in.raw> <out.off> "
-                 "[-needEuler] [-dims=nx,ny,nz]\n"
+    std::cerr << "usage: voroUtility -md=vol2mesh <in.mrc|in.raw|in.sof|in.sog> "
+                 "<out_base> [-onlyBndryVts=false] [-needEuler] [-write2node] "
+                 "[-dims=nx,ny,nz]\n"
                  "       voroUtility -md=vol2skel <in> <out_prefix> "
-                 "[-tt=0.01,0.02,0.04] [-dims=...]\n"
+                 "[-tt=0.01,0.02,0.04] [-outToDotma] [-dims=...]\n"
                  "       voroUtility -md=topo -meshToCheck=<mesh.off>\n";
     return 1;
 }
@@ -228,21 +221,21 @@ int main(int argc, char** argv)
                 return 1;
             }
             const auto t0 = Clock::now();
-            Surfacer s;
-            std::vector<point> vts;
-            std::vector<uTriFace> tris;
-            eulerchar ec;
-            const bool need_euler = flags.count("needEuler") && flags["needEuler"] != "false";
-            if (s.extractBoundaryVtsAndFaces(vol, vts, tris, need_euler ? &ec : nullptr) !=
-                SurfacerErrCode::SUCCESS)
+            const bool need_euler = flag_on(flags, "needEuler", false);
+            const bool node = flag_on(flags, "write2node", false);
+            const auto rc =
+                flag_on(flags, "onlyBndryVts", true)
+                    ? voxelvoro::writeVolumeAsBoundaryPts(vol, pos[1].c_str(), node)
+                    : voxelvoro::writeVolumeAsBoundaryMesh(vol, pos[1].c_str(),
+                                                           need_euler, node);
+            std::cout << "time -> extract + write boundary: " << ms_since(t0) << " ms\n";
+            if (rc != voxelvoro::ExportErrCode::SUCCESS)
             {
-                std::cerr << "boundary extraction failed\n";
+                std::cerr << "failed to write boundary mesh from " << pos[0] << " to "
+                          << pos[1] << "\n";
                 return 1;
             }
-            std::cout << "time(GPU) -> extract boundary: " << ms_since(t0) << " ms\n";
-            if (need_euler)
-                ec.logToConsole("volume boundary (cube complex)");
-            return write_off(pos[1], vts, tris) ? 0 : 1;
+            return 0;
         }
         if (md == "vol2skel")
         {
@@ -264,15 +257,46 @@ int main(int argc, char** argv)
             std::cout << "time(THIN) -> EDT + measure + thinning: " << ms_since(t0)
                       << " ms\n";
             std::sort(tt.begin(), tt.end());
+            const xform& mat = vol->getVoxToModelMat();
+            const std::vector<int32_t> d2 = thin.squaredDistances();
+            const bool dotma = flag_on(flags, "outToDotma", false);
+            const int ny = thin.sizeY(), nz = thin.sizeZ();
             for (size_t k = 0; k < levels.size(); ++k)
             {
                 std::vector<ivec3> vox;
                 thin.getRemainedVoxels((int)k, vox);
                 std::ostringstream name;
-                name << pos[1] << "_thinned" << tt[k] << ".ply";
+                name << pos[1] << "_thinned" << tt[k];
                 std::cout << "level " << levels[k] << " (t=" << tt[k]
-                          << "): " << vox.size() << " voxels -> " << name.str() << "\n";
-                if (!write_ply_points(name.str(), vox))
+                          << "): " << vox.size() << " voxels -> " << name.str() << ".ply\n";
+                // skeleton voxels in model space; radius = distance to the
+                // nearest outside voxel, sqrt of the exact squared EDT
+                std::vector<point> v, v_model;
+                std::vector<float> r, r_model;
+                const point o = mat * point(0.0f, 0.0f, 0.0f);
+                for (const auto& p : vox)
+                {
+                    v.push_back(point((float)p[0], (float)p[1], (float)p[2]));
+                    v_model.push_back(mat * v.back());
+                    const size_t i = ((size_t)p[0] * ny + p[1]) * nz + p[2];
+                    r.push_back(std::sqrt((float)d2[i]));
+                    const point rp = mat * point(r.back(), 0.0f, 0.0f);
+                    const float d0 = rp[0] - o[0], d1 = rp[1] - o[1], dz = rp[2] - o[2];
+                    r_model.push_back(std::sqrt(d0 * d0 + d1 * d1 + dz * dz));
+                }
+                if (voxelvoro::writeToPLY((name.str() + ".ply").c_str(), v_model, {}, {}, {},
+                                          {}, {}) != voxelvoro::ExportErrCode::SUCCESS)
+                    return 1;
+                std::vector<float> flat(3 * v.size() + 3);
+                for (size_t i = 0; i < v.size(); ++i)
+                    for (int a = 0; a < 3; ++a)
+                        flat[3 * i + a] = v[i][a];
+                if (vc_write_radii((name.str() + ".r").c_str(), flat.data(), r.data(),
+                                   (int64_t)v.size(), mat.data()) != VC_OK)
+                    return 1;
+                if (dotma && voxelvoro::writeToDotma(name.str() + ".ma", v_model, {}, {},
+                                                     r_model) !=
+                                 voxelvoro::ExportErrCode::SUCCESS)
                     return 1;
             }
             return 0;

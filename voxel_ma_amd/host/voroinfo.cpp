@@ -39,7 +39,86 @@ void VoroInfo::setGeometry(const std::vector<point>& vts,
     m_vts_valid.assign(vts.size(), false);
     m_tagged = false;
     m_only_inside = -1;
+    m_r_per_v.clear();
+    m_r_valid = false;
     recompute_flags();
+}
+
+namespace
+{
+// face polygons as CSR (offsets, vertex ids), one spare entry at the end
+void face_csr(const cellcomplex& g, std::vector<int64_t>& foff,
+              std::vector<int32_t>& fvts)
+{
+    foff.assign(1, 0);
+    fvts.clear();
+    std::vector<int> fv;
+    for (size_t f = 0; f < g.numFaces(); ++f)
+    {
+        g.getFaceVRep((int)f, fv);
+        fvts.insert(fvts.end(), fv.begin(), fv.end());
+        foff.push_back((int64_t)fvts.size());
+    }
+    fvts.push_back(0);
+}
+} // namespace
+
+void VoroInfo::computeRadii()
+{
+    const int nv = (int)m_geom.numVts(), nf = (int)m_geom.numFaces();
+    std::vector<float> vts(3 * (size_t)nv + 3), sites(3 * m_site_positions.size() + 3);
+    for (int v = 0; v < nv; ++v)
+        for (int a = 0; a < 3; ++a)
+            vts[3 * v + a] = m_geom.getVert(v)[a];
+    for (size_t i = 0; i < m_site_positions.size(); ++i)
+        for (int a = 0; a < 3; ++a)
+            sites[3 * i + a] = m_site_positions[i][a];
+    std::vector<int64_t> foff;
+    std::vector<int32_t> fvts, fs(2 * (size_t)nf + 2);
+    face_csr(m_geom, foff, fvts);
+    for (int f = 0; f < nf; ++f)
+    {
+        fs[2 * f] = m_face_sites[f][0];
+        fs[2 * f + 1] = m_face_sites[f][1];
+    }
+    m_r_per_v.assign((size_t)nv + 1, 0.0f);
+    vcdev::check(vc_voro_radii(vcdev::context(), nv, nf, vts.data(), foff.data(),
+                               fvts.data(), sites.data(),
+                               (int32_t)m_site_positions.size(), fs.data(),
+                               m_r_per_v.data()),
+                 "vc_voro_radii");
+    m_r_per_v.resize((size_t)nv);
+    m_r_valid = true;
+}
+
+void VoroInfo::getPartInside(std::vector<int>& vts_indices,
+                             std::vector<ivec2>& edges,
+                             std::vector<uTriFace>& tri_faces) const
+{
+    const int nv = (int)m_geom.numVts(), nf = (int)m_geom.numFaces();
+    std::vector<int64_t> foff;
+    std::vector<int32_t> fvts;
+    face_csr(m_geom, foff, fvts);
+    std::vector<uint8_t> valid((size_t)nv + 1, 0);
+    for (int v = 0; v < nv; ++v)
+        valid[v] = isVertexValid(v) ? 1 : 0;
+    int64_t n1 = 0, n2 = 0, n3 = 0;
+    int rc = vc_voro_part_inside(nv, nf, foff.data(), fvts.data(), valid.data(),
+                                 nullptr, 0, &n1, nullptr, 0, &n2, nullptr, 0, &n3);
+    if (rc != VC_OK && rc != VC_ERR_CAPACITY)
+        vcdev::check(rc, "vc_voro_part_inside");
+    std::vector<int32_t> vi(n1 + 1), eo(2 * n2 + 2);
+    std::vector<uint32_t> tr(3 * n3 + 3);
+    vcdev::check(vc_voro_part_inside(nv, nf, foff.data(), fvts.data(), valid.data(),
+                                     vi.data(), n1, &n1, eo.data(), n2, &n2,
+                                     tr.data(), n3, &n3),
+                 "vc_voro_part_inside");
+    for (int64_t i = 0; i < n1; ++i)
+        vts_indices.push_back(vi[i]);
+    for (int64_t i = 0; i < n2; ++i)
+        edges.push_back(ivec2(eo[2 * i], eo[2 * i + 1]));
+    for (int64_t i = 0; i < n3; ++i)
+        tri_faces.push_back(uTriFace(tr[3 * i], tr[3 * i + 1], tr[3 * i + 2]));
 }
 
 bool VoroInfo::isVertexValid(int vi) const
@@ -378,6 +457,13 @@ void VoroInfo::mergeCloseVts(float eps, bool only_inside)
     g.finalize();
     m_geom = std::move(g);
     m_face_sites = std::move(new_sites);
+    if (m_r_valid)
+    {
+        std::vector<float> r;
+        for (int32_t i = 0; i < n_new; ++i)
+            r.push_back(m_r_per_v[nvts[i]]);
+        m_r_per_v = std::move(r);
+    }
     if (!onlyHasInside())
     {
         m_vts_valid = new_valid;

@@ -80,8 +80,22 @@ public:
                                   std::vector<float>& faces_msure) const;
 
     // src/voroinfo.cpp:1064-1384; the surviving edges are appended in
-    // ascending order (the reference iterates an unordered_set)
+    // ascending order (the reference iterates an unordered_set); radii, when
+    // set, follow the seeds (:1176-1188)
     void mergeCloseVts(float eps, bool only_inside);
+
+    // radius per vertex: distance to the first site of the last face holding
+    // it (src/voroinfo.cpp:290-325), computed on the device
+    void computeRadii();
+    bool radiiValid() const { return m_r_valid; }
+    const std::vector<float>& getRadii() const { return m_r_per_v; }
+    ivec2 getSitesOfFace(int fi) const { return m_face_sites[fi]; }
+    const std::vector<point>& getSitesPosition() const { return m_site_positions; }
+
+    // valid vertices, the valid triangles of every face's triangulation and
+    // their sides, sorted (src/voroinfo.cpp:573-620)
+    void getPartInside(std::vector<int>& vts_indices, std::vector<ivec2>& edges,
+                       std::vector<uTriFace>& tri_faces) const;
 
 private:
     void all_measures(std::vector<float>& lv, std::vector<float>& le,
@@ -96,6 +110,8 @@ private:
     int m_only_inside = -1;
     point m_bb_min, m_bb_max;
     bool m_bb_valid = false;
+    std::vector<float> m_r_per_v;
+    bool m_r_valid = false;
 };
 
 } // namespace voxelvoro
