@@ -31,6 +31,8 @@ EDT_BYTES_PER_VOXEL = 21  # passes z, y, x: (1+2) + (2+4) + (4+4+4)
 # dense subfield pass over bit planes: alive state (1/8 B) read with its
 # neighbour rows amortised (x 9/8) + the eligibility plane (1/8 B) / 8
 THIN_BYTES_PER_VOXEL_PASS = (1.0 / 8) * (9.0 / 8) + (1.0 / 8) / 8
+# a tile sweep does the 8 subfield passes of one sweep in one launch
+THIN_BYTES_PER_VOXEL_SWEEP = 8 * THIN_BYTES_PER_VOXEL_PASS
 
 
 class HipEvents:
@@ -342,10 +344,12 @@ def main():
     value = N / (ms_step * 1e-3) / 1e6
 
     edt_gbs = EDT_BYTES_PER_VOXEL * N / (edt_ms * 1e-3) / 1e9
-    # thinning: average duration of one subfield-pass launch over the timed
-    # region (host waits between batches included), algorithmic bytes per pass
-    pass_ms = thin_ms / max(1, stats.passes)
-    thin_gbs = THIN_BYTES_PER_VOXEL_PASS * N / (pass_ms * 1e-3) / 1e9
+    # thinning: average duration of one sweep (one tile-sweep launch: 8
+    # subfield passes) over the timed region, level set-ups and host waits
+    # between segments included; algorithmic bytes per sweep
+    sweeps = max(1, stats.passes // 8)
+    sweep_ms = thin_ms / sweeps
+    thin_gbs = THIN_BYTES_PER_VOXEL_SWEEP * N / (sweep_ms * 1e-3) / 1e9
     out = {
         "metric": "Mvoxels/s EDT+thinning at 512^3",
         "value": round(value, 2),
@@ -369,12 +373,12 @@ def main():
         "thin_stats": {"removed": [int(x) for x in stats.removed[:len(levels)]],
                        "passes_with_work": int(stats.passes), "tests": int(stats.tests),
                        "launches": int(stats.launches)},
-        "roofline": {"bound": "hbm", "kernel": "thin_pass_dense (subfield pass)",
+        "roofline": {"bound": "hbm", "kernel": "thin_tile_sweep (8 subfield passes)",
                      "achieved": round(thin_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(thin_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("thin_pass_dense", n),
-                     "bytes_per_voxel_pass": round(THIN_BYTES_PER_VOXEL_PASS, 4),
-                     "avg_pass_us": round(pass_ms * 1e3, 2)},
+                     "traffic": measured_traffic("thin_tile_sweep", n),
+                     "bytes_per_voxel_sweep": round(THIN_BYTES_PER_VOXEL_SWEEP, 4),
+                     "sweeps": sweeps, "avg_sweep_us": round(sweep_ms * 1e3, 2)},
         "roofline_edt": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21,

@@ -37,9 +37,15 @@ typedef struct vc_schedule {
     int use_graph;       /* 1: each segment of sweeps is a captured
                             hipGraph replayed on the context stream       */
     int tail_sweep;      /* sweeps with index >= tail_sweep within a level
-                            use the sparse-pass variant (speculative row
-                            loads, simple-point test by flood fill instead
-                            of the table); -1 = never                     */
+                            use the sparse-pass variant (one wavefront per
+                            256 words, flagged words compacted in LDS);
+                            -1 = never                                    */
+    int tile_sweep;      /* from the first segment of sweeps that starts at
+                            an index >= tile_sweep within a level, each
+                            sweep is one launch over 32 x 32 x 64 tiles
+                            (all 8 subfield passes in LDS, 8-row halo),
+                            visiting only tiles next to last sweep's
+                            changes; -1 = never                           */
 } vc_schedule;
 
 /* volumes: n member volumes of this context (1 <= n <= 65535, distinct); the

@@ -80,15 +80,23 @@ def test_batch_matches_oracle_per_volume(vc, ragged):
 
 @pytest.mark.parametrize("sched", [
     dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=1, use_graph=0,
-         tail_sweep=-1),
+         tail_sweep=-1, tile_sweep=-1),
     dict(edt_lds_kb=64, edt_grid_per_cu=1, thin_grid_per_cu=1, sweeps_per_sync=32, use_graph=1,
-         tail_sweep=0),
+         tail_sweep=0, tile_sweep=-1),
     dict(edt_lds_kb=32, edt_grid_per_cu=8, thin_grid_per_cu=8, sweeps_per_sync=0, use_graph=1,
-         tail_sweep=2),
+         tail_sweep=2, tile_sweep=-1),
     dict(edt_lds_kb=16, edt_grid_per_cu=2, thin_grid_per_cu=0, sweeps_per_sync=3, use_graph=1,
-         tail_sweep=1),
+         tail_sweep=1, tile_sweep=-1),
     dict(edt_lds_kb=32, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=0, use_graph=0,
-         tail_sweep=3),
+         tail_sweep=3, tile_sweep=-1),
+    dict(edt_lds_kb=32, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=0, use_graph=0,
+         tail_sweep=-1, tile_sweep=0),
+    dict(edt_lds_kb=32, edt_grid_per_cu=0, thin_grid_per_cu=2, sweeps_per_sync=1, use_graph=0,
+         tail_sweep=1, tile_sweep=2),
+    dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=0, use_graph=1,
+         tail_sweep=-1, tile_sweep=4),
+    dict(edt_lds_kb=32, edt_grid_per_cu=1, thin_grid_per_cu=1, sweeps_per_sync=2, use_graph=1,
+         tail_sweep=-1, tile_sweep=0),
 ])
 def test_batch_every_schedule_same_result(vc, ragged, sched):
     vols, levels, ref = ragged
@@ -128,8 +136,9 @@ def test_batch_graph_replays(vc, ragged):
 
 @pytest.mark.parametrize("tail", [0, 2])
 def test_sparse_pass_variant_single_volumes(vc, oracle, tail):
-    """The sparse-pass variant (speculative row loads, flood-fill simple
-    test) on single volumes of every shape class: labels equal the oracle."""
+    """The sparse-pass variant (one wavefront per 256 words, flagged words
+    compacted in LDS) on single volumes of every shape class: labels equal the
+    oracle."""
     rng = np.random.default_rng(31 + tail)
     shapes = [tuple(int(s) for s in rng.integers(2, 40, size=3)) for _ in range(12)]
     shapes += [(64, 64, 64), (3, 70, 129)]
@@ -142,7 +151,7 @@ def test_sparse_pass_variant_single_volumes(vc, oracle, tail):
         say("sparse variant", tail, i, shape)
         m = vc.upload(occ)
         b = V.Batch(vc, [m])
-        b.set_schedule(tail_sweep=tail)
+        b.set_schedule(tail_sweep=tail, tile_sweep=-1)
         st = b.run(levels)
         d, f, lvl, lab, sto = _oracle_all(oracle, occ, levels)
         assert np.array_equal(m.get(V.BUF_LABEL), lab), (tail, i, shape)
@@ -251,3 +260,55 @@ def test_solids_generator_matches_host(vc):
         got = buf.download(np.uint8, (L, L, L))
         buf.free()
         assert np.array_equal(got, synth.solids_host(L, prims)), i
+
+
+@pytest.mark.parametrize("tile", [0, 1, 4])
+def test_tile_sweep_variant_single_volumes(vc, oracle, tile):
+    """Tile sweeps (32 x 32 x 64 tiles, all 8 passes in LDS with an 8-row
+    halo) on single volumes whose axes cross tile and word borders, from the
+    first sweep or after dense sweeps: labels equal the oracle, the sweep
+    count too."""
+    rng = np.random.default_rng(71 + tile)
+    shapes = [tuple(int(s) for s in rng.integers(2, 40, size=3)) for _ in range(6)]
+    shapes += [(64, 64, 64), (70, 33, 129), (33, 97, 64), (96, 40, 200)]
+    for i, shape in enumerate(shapes):
+        occ = (rng.random(shape) < rng.choice([0.6, 0.85, 0.97])).astype(np.uint8)
+        if shape == (64, 64, 64):
+            from voxel_ma_amd import synth
+            occ = synth.torus_c1()
+        levels = sorted(float(x) for x in rng.uniform(0.8, 6.0, size=3))
+        say("tile sweeps", tile, i, shape)
+        m = vc.upload(occ)
+        b = V.Batch(vc, [m])
+        b.set_schedule(tile_sweep=tile, sweeps_per_sync=1 + i % 3)
+        st = b.run(levels)
+        d, f, lvl, lab, sto = _oracle_all(oracle, occ, levels)
+        assert np.array_equal(m.get(V.BUF_LABEL), lab), (tile, i, shape)
+        assert list(st.removed[:3]) == [int(x) for x in sto.removed[:3]], (tile, i, shape)
+        assert st.passes == 8 * sum(sto.sweeps[k] for k in range(3))
+        b.release()
+        m.release()
+
+
+def test_tile_sweeps_blobs_match_dense(vc):
+    """A 160 x 150 x 140 volume of blurred noise (many tiles, long removal
+    chains): tile sweeps from sweep 0 and from sweep 3 give the labels of the
+    dense passes."""
+    occ, _ = vc.synth_blurred_noise((160, 150, 140), sigma=4.0, fill=0.35, seed=5)
+    host = occ.download(np.uint8, (160, 150, 140))
+    occ.free()
+    from voxel_ma_amd import synth
+    levels = synth.levels_from_tt(synth.inside_size(host))
+    out = []
+    for tile in (-1, 0, 3):
+        say("blobs tile", tile)
+        m = vc.upload(host)
+        b = V.Batch(vc, [m])
+        b.set_schedule(tile_sweep=tile)
+        st = b.run(levels)
+        out.append((m.get(V.BUF_LABEL), list(st.removed[:3]), st.passes))
+        b.release()
+        m.release()
+    for lab, rem, passes in out[1:]:
+        assert np.array_equal(lab, out[0][0])
+        assert rem == out[0][1] and passes == out[0][2]

@@ -83,7 +83,8 @@ class Schedule(ctypes.Structure):
     """vc_schedule: launch parameters of a batch (results never depend on it)."""
     _fields_ = [("edt_lds_kb", ctypes.c_int), ("edt_grid_per_cu", ctypes.c_int),
                 ("thin_grid_per_cu", ctypes.c_int), ("sweeps_per_sync", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("tail_sweep", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("tail_sweep", ctypes.c_int),
+                ("tile_sweep", ctypes.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

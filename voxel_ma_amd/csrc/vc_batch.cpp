@@ -293,10 +293,13 @@ int vc_batch_tune(vc_ctx* ctx, vc_batch* b, const float* levels, int n_levels,
             }
         }
         t_front = best_t;
-        // thinning candidates: (grid per CU, graph, sweeps per sync)
-        static const int kThin[][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 8},
-                                       {0, 1, 8}, {0, 1, 16}, {8, 0, 0},
-                                       {8, 1, 0}, {8, 1, 8}, {8, 1, 16}};
+        // thinning candidates: (grid per CU, graph, sweeps per sync, first
+        // tile sweep); tile sweeps hold 78 KB of LDS, two workgroups per CU
+        static const int kThin[][4] = {{0, 0, 0, 0},  {0, 1, 0, 0},
+                                       {0, 0, 16, 0}, {2, 0, 0, 0},
+                                       {0, 0, 0, 4},  {0, 0, 0, -1},
+                                       {0, 1, 8, -1}, {8, 0, 0, -1},
+                                       {8, 1, 16, -1}};
         vc_schedule incumbent = best;
         best_t = 0.0f;
         for (size_t c = 0; c < sizeof(kThin) / sizeof(kThin[0]); ++c)
@@ -305,6 +308,7 @@ int vc_batch_tune(vc_ctx* ctx, vc_batch* b, const float* levels, int n_levels,
             sc.thin_grid_per_cu = kThin[c][0];
             sc.use_graph = kThin[c][1];
             sc.sweeps_per_sync = kThin[c][2];
+            sc.tile_sweep = kThin[c][3];
             if ((rc = vc_sched_apply(ctx, s, sc)) != VC_OK)
                 return rc;
             float t = 0.0f;

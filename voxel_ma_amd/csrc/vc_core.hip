@@ -283,6 +283,8 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->alive);
     vc_free(v->elig);
     vc_free(v->dflag);
+    vc_free(v->alive2);
+    vc_free(v->tchg);
     vc_sched_free(v->self);
     vc_free(v->bnd_sites);
     vc_free(v->bnd_quads);

@@ -51,6 +51,8 @@ struct vc_vol
     uint64_t* alive = nullptr; // alive voxels
     uint64_t* elig = nullptr;  // eligible at the current level (lvl <= k)
     uint8_t* dflag = nullptr;  // 2 x words: revisit flags per z parity
+    uint64_t* alive2 = nullptr; // tile sweeps: the other alive plane
+    uint8_t* tchg = nullptr;    // tile sweeps: 2 x tiles change flags
     // boundary stage results (computed once per volume)
     bool bnd_done = false;
     float* bnd_sites = nullptr;   // 3 floats per site
@@ -105,6 +107,9 @@ struct vc_vdesc
     uint64_t* elig;
     uint8_t* d0; // revisit flags, z parity 0
     uint8_t* d1; // revisit flags, z parity 1
+    uint64_t* alive2; // second alive plane of the tile sweeps
+    uint8_t* tchg;    // 2 x tiles: tile changed in the last sweep (by parity)
+    int ntx, nty;     // tiles along x and y (32 rows each)
 };
 
 // kernels with a work table
@@ -116,7 +121,8 @@ enum vc_kind
     VK_MS,     // measure: one 64 x 8 x 32 (z, y, x) tile per item
     VK_LI,     // level set-up: 4 wavefronts (512 z each) per item
     VK_TP,     // subfield pass, (x, y) parity 0..3: 256 words per item
-    VK_COUNT = VK_TP + 4
+    VK_TT = VK_TP + 4, // tile sweep: one 32 x 32 x 64 (x, y, z) tile per item
+    VK_COUNT
 };
 
 // work table of one kind as the kernels see it

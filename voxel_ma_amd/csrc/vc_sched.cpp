@@ -17,6 +17,7 @@ constexpr int kRowsZ = 4;       // EDT pass z: rows (wavefronts) per item
 constexpr int kLiWaves = 4;     // level set-up: wavefronts per item
 constexpr int kTpWords = 256;   // subfield pass: words per item
 constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
+constexpr int kTileXY = 32;     // tile sweep: rows per tile along x and y
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -43,6 +44,8 @@ int64_t items_of(const vc_vdesc& d, int kind)
         return cdiv(d.nz, kMeasZ) * cdiv(d.ny, kMeasY) * cdiv(d.nx, kMeasXC);
     case VK_LI:
         return cdiv((int64_t)d.nx * d.ny * cdiv(d.nz, 512), kLiWaves);
+    case VK_TT:
+        return (int64_t)d.ntx * d.nty * d.nwz;
     default:
     {
         const int p = kind - VK_TP, sx = p & 1, sy = p >> 1;
@@ -90,6 +93,7 @@ vc_schedule vc_schedule_default()
     sc.sweeps_per_sync = 0;
     sc.use_graph = 0;
     sc.tail_sweep = -1;
+    sc.tile_sweep = 0;
     return sc;
 }
 
@@ -118,6 +122,13 @@ int vc_vol_ensure(vc_ctx* ctx, vc_vol* v)
     if (!v->elig && (rc = vc_alloc(ctx, (void**)&v->elig, 8 * words)) != VC_OK)
         return rc;
     if (!v->dflag && (rc = vc_alloc(ctx, (void**)&v->dflag, 2 * words)) != VC_OK)
+        return rc;
+    if (!v->alive2 &&
+        (rc = vc_alloc(ctx, (void**)&v->alive2, 8 * words)) != VC_OK)
+        return rc;
+    const size_t tiles = (size_t)cdiv(v->nx, kTileXY) * cdiv(v->ny, kTileXY) *
+                         ((v->nz + 63) / 64);
+    if (!v->tchg && (rc = vc_alloc(ctx, (void**)&v->tchg, 2 * tiles)) != VC_OK)
         return rc;
     return VC_OK;
 }
@@ -163,6 +174,8 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
         return vc_fail(ctx, VC_ERR_INVALID, "use_graph must be 0 or 1");
     if (sc.tail_sweep < -1 || sc.tail_sweep > 1000000)
         return vc_fail(ctx, VC_ERR_INVALID, "tail_sweep must be in [-1, 1e6]");
+    if (sc.tile_sweep < -1 || sc.tile_sweep > 1000000)
+        return vc_fail(ctx, VC_ERR_INVALID, "tile_sweep must be in [-1, 1e6]");
     VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_graphs(s);
     s->sc = sc;
@@ -249,6 +262,10 @@ int vc_sched_build(vc_ctx* ctx, vc_vol* const* vols, int n,
         const size_t words = (size_t)v->nx * v->ny * d.nwz;
         d.d0 = v->dflag;
         d.d1 = v->dflag + words;
+        d.alive2 = v->alive2;
+        d.tchg = v->tchg;
+        d.ntx = (int)cdiv(v->nx, kTileXY);
+        d.nty = (int)cdiv(v->ny, kTileXY);
         s->voxels += v->N;
         s->words += (int64_t)words;
     }

@@ -181,18 +181,34 @@ __global__ void __launch_bounds__(kBlock)
         const int z0 = (int)(wave % nchunk) * 512 + 8 * lane;
         const int64_t base = rowi * nz;
         const int mode = k > 0 ? 2 : (d.mkind == 2 ? 1 : 0);
+        // rows start 8-aligned when nz is a multiple of 8: one 8-byte access
+        // per lane instead of eight byte accesses (z0 + 8 <= nz then)
+        const bool vec = (nz & 7) == 0 && z0 < nz;
         uint32_t am = 0, em = 0, nm = 0;
         uint8_t lv8[8], lb8[8];
         if (mode == 2)
         {
+            // alive voxels from the maintained plane; levels read back
+            if (z0 < nz)
+                am = (uint32_t)(d.alive[rowi * d.nwz + (z0 >> 6)] >> (z0 & 63)) &
+                     0xFFu;
+            if (vec)
+            {
+                const uint64_t v =
+                    *reinterpret_cast<const uint64_t*>(d.lvl + base + z0);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    lv8[j] = (uint8_t)(v >> (8 * j));
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    lv8[j] = z0 + j < nz ? d.lvl[base + z0 + j] : (uint8_t)255;
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-            {
-                const int64_t i = base + z0 + j;
-                const bool in = z0 + j < nz;
-                lv8[j] = in ? d.lvl[i] : (uint8_t)255;
-                lb8[j] = in && d.label[i] == 1;
-            }
+                lb8[j] = (uint8_t)((am >> j) & 1u);
         }
         else
         {
@@ -201,21 +217,55 @@ __global__ void __launch_bounds__(kBlock)
             // smallest such q is kept); thresholds are uniform scalar loads
             uint64_t mi[8];
             float mf[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
+            if (vec)
             {
-                const int64_t i = base + z0 + j;
-                const bool in = z0 + j < nz && d.occ[i] != 0;
-                lb8[j] = in;
-                lv8[j] = 255;
-                mi[j] = 0;
-                mf[j] = 0.0f;
-                if (in)
+                const uint64_t ov =
+                    *reinterpret_cast<const uint64_t*>(d.occ + base + z0);
+                uint32_t m32[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (mode == 0)
                 {
-                    if (mode == 0)
-                        mi[j] = d.lam2[i];
-                    else
-                        mf[j] = d.meas[i];
+                    const uint4 u0 = *reinterpret_cast<const uint4*>(d.lam2 + base + z0);
+                    const uint4 u1 =
+                        *reinterpret_cast<const uint4*>(d.lam2 + base + z0 + 4);
+                    m32[0] = u0.x; m32[1] = u0.y; m32[2] = u0.z; m32[3] = u0.w;
+                    m32[4] = u1.x; m32[5] = u1.y; m32[6] = u1.z; m32[7] = u1.w;
+                }
+                else
+                {
+                    const float4 f0 = *reinterpret_cast<const float4*>(d.meas + base + z0);
+                    const float4 f1 =
+                        *reinterpret_cast<const float4*>(d.meas + base + z0 + 4);
+                    mf[0] = f0.x; mf[1] = f0.y; mf[2] = f0.z; mf[3] = f0.w;
+                    mf[4] = f1.x; mf[5] = f1.y; mf[6] = f1.z; mf[7] = f1.w;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                {
+                    lb8[j] = ((ov >> (8 * j)) & 0xFFu) != 0;
+                    lv8[j] = 255;
+                    mi[j] = lb8[j] ? m32[j] : 0;
+                    if (mode == 0 || !lb8[j])
+                        mf[j] = 0.0f;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                {
+                    const int64_t i = base + z0 + j;
+                    const bool in = z0 + j < nz && d.occ[i] != 0;
+                    lb8[j] = in;
+                    lv8[j] = 255;
+                    mi[j] = 0;
+                    mf[j] = 0.0f;
+                    if (in)
+                    {
+                        if (mode == 0)
+                            mi[j] = d.lam2[i];
+                        else
+                            mf[j] = d.meas[i];
+                    }
                 }
             }
             typedef const __attribute__((address_space(4))) uint64_t* cu64;
@@ -251,13 +301,28 @@ __global__ void __launch_bounds__(kBlock)
         }
         if (mode != 2)
         {
+            if (vec)
+            {
+                uint64_t lvw = 0, lbw = 0;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (z0 + j < nz)
+                for (int j = 0; j < 8; ++j)
                 {
-                    d.lvl[base + z0 + j] = lv8[j];
-                    d.label[base + z0 + j] = lb8[j];
+                    lvw |= (uint64_t)lv8[j] << (8 * j);
+                    lbw |= (uint64_t)lb8[j] << (8 * j);
                 }
+                *reinterpret_cast<uint64_t*>(d.lvl + base + z0) = lvw;
+                *reinterpret_cast<uint64_t*>(d.label + base + z0) = lbw;
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (z0 + j < nz)
+                    {
+                        d.lvl[base + z0 + j] = lv8[j];
+                        d.label[base + z0 + j] = lb8[j];
+                    }
+            }
         }
         const int sh = 8 * (lane & 7);
         uint64_t a = (uint64_t)am << sh, e = (uint64_t)em << sh,
@@ -336,29 +401,20 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red)
     return tot;
 }
 
-// Visit of one word in a subfield pass (its flag was set and has been
-// cleared): candidates of the pass's z parity (alive, eligible, a background
-// face neighbour) go through the simple-point table; removals update the
-// word, the labels and the revisit flags of every word holding a neighbour.
-// Returns the removed bits; *ntest counts the table lookups.
-//
-// Revisit flags: the flag of z-parity p (D0 / D1) is raised whenever a voxel
-// of the 26-neighbourhood of one of the word's parity-p voxels is removed.
-// All flag traffic is plain byte stores: in pass (sx, sy, sz) the flags of
-// parity sz are cleared only on rows of parity (sx, sy), and raised only on
-// other rows.
-__device__ __forceinline__ uint64_t visit_word(
-    const Plane& P, uint64_t* A, const uint64_t* __restrict__ E,
-    uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1, uint8_t* Fo,
-    const uint32_t* __restrict__ lut, int x, int y, int w, int64_t o,
-    uint64_t zsub, int k, uint32_t* ntest)
+// 3x3 rows of alive words around word (x, y, w) and the edge bits of their z
+// neighbours (prev: bit 63 of word w-1, next: bit 0 of word w+1, loaded only
+// when a candidate sits at that end).  Returns the candidates that pass the
+// border filter: a voxel with all six face neighbours alive is never simple.
+struct Nbhd
 {
-    const uint64_t C = A[o];
-    uint64_t cand = C & E[o] & zsub;
-    if (!cand)
-        return 0;
     uint64_t R[3][3];
     uint32_t pv[3][3], nx0[3][3];
+};
+
+__device__ __forceinline__ uint64_t load_nbhd(const Plane& P, const uint64_t* A,
+                                              uint64_t C, uint64_t cand, int x,
+                                              int y, int w, Nbhd& N)
+{
     const bool need_prev = (cand & 1ull) && w > 0;
     const bool need_next = (cand >> 63) && w + 1 < P.nwz;
 #pragma unroll
@@ -369,49 +425,32 @@ __device__ __forceinline__ uint64_t visit_word(
             const bool in = (unsigned)(x + dx) < (unsigned)P.nx &&
                             (unsigned)(y + dy) < (unsigned)P.ny;
             const int64_t q = widx(P, x + dx, y + dy, w);
-            R[dx + 1][dy + 1] = (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
-            pv[dx + 1][dy + 1] =
+            N.R[dx + 1][dy + 1] = (dx == 0 && dy == 0) ? C : (in ? A[q] : 0ull);
+            N.pv[dx + 1][dy + 1] =
                 (in && need_prev) ? (uint32_t)(A[q - 1] >> 63) : 0u;
-            nx0[dx + 1][dy + 1] =
+            N.nx0[dx + 1][dy + 1] =
                 (in && need_next) ? (uint32_t)(A[q + 1] & 1ull) : 0u;
         }
-    // border filter: a voxel with all six face neighbours alive is never
-    // simple
-    const uint64_t zlo = (C << 1) | (uint64_t)pv[1][1];
-    const uint64_t zhi = (C >> 1) | ((uint64_t)nx0[1][1] << 63);
-    cand &= ~(R[0][1] & R[2][1] & R[1][0] & R[1][2] & zlo & zhi);
-    uint64_t removed = 0;
-    // four independent table lookups in flight per round
-    while (cand)
-    {
-        int bb[4];
-        uint32_t cc[4];
-        int nb = 0;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-        {
-            bb[jj] = 0;
-            cc[jj] = 0;
-            if (cand)
-            {
-                bb[jj] = __ffsll((unsigned long long)cand) - 1;
-                cand &= cand - 1;
-                cc[jj] = nb_code(R, pv, nx0, bb[jj]);
-                nb = jj + 1;
-            }
-        }
-        uint32_t lw[4];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-            lw[jj] = jj < nb ? lut[cc[jj] >> 5] : 0u;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
-            if ((lw[jj] >> (cc[jj] & 31)) & 1u)
-                removed |= 1ull << bb[jj];
-        *ntest += nb;
-    }
-    if (!removed)
-        return 0;
+    const uint64_t zlo = (C << 1) | (uint64_t)N.pv[1][1];
+    const uint64_t zhi = (C >> 1) | ((uint64_t)N.nx0[1][1] << 63);
+    return cand & ~(N.R[0][1] & N.R[2][1] & N.R[1][0] & N.R[1][2] & zlo & zhi);
+}
+
+// Removal of `removed` from word o (value C): the word, the labels, and the
+// revisit flags of every word holding a neighbour.
+//
+// Revisit flags: the flag of z-parity p (D0 / D1) is raised whenever a voxel
+// of the 26-neighbourhood of one of the word's parity-p voxels is removed.
+// All flag traffic is plain byte stores: in pass (sx, sy, sz) the flags of
+// parity sz are cleared only on rows of parity (sx, sy), and raised only on
+// other rows.
+__device__ __forceinline__ void apply_removal(const Plane& P, uint64_t* A,
+                                              uint8_t* __restrict__ label,
+                                              uint8_t* D0, uint8_t* D1,
+                                              uint8_t* Fo, int x, int y, int w,
+                                              int64_t o, uint64_t C,
+                                              uint64_t removed, int k)
+{
     A[o] = C & ~removed;
     uint8_t* row = label + ((int64_t)x * P.ny + y) * P.nz + 64 * w;
     uint64_t m = removed;
@@ -421,7 +460,6 @@ __device__ __forceinline__ uint64_t visit_word(
         m &= m - 1;
         row[bt] = (uint8_t)(2 + k);
     }
-    // raise the flags of every word holding a neighbour
     const bool lo = (removed & 1ull) && w > 0;
     const bool hi = (removed >> 63) && w + 1 < P.nwz;
 #pragma unroll
@@ -457,7 +495,86 @@ __device__ __forceinline__ uint64_t visit_word(
                 }
             }
         }
+}
+
+// Visit of one word by one thread (sparse pass; its flag was set and has
+// been cleared): candidates of the pass's z parity (alive, eligible, a
+// background face neighbour) go through the simple-point table, four lookups
+// in flight per round.  Returns the removed bits; *ntest counts the lookups.
+__device__ __forceinline__ uint64_t visit_word(
+    const Plane& P, uint64_t* A, const uint64_t* __restrict__ E,
+    uint8_t* __restrict__ label, uint8_t* D0, uint8_t* D1, uint8_t* Fo,
+    const uint32_t* __restrict__ lut, int x, int y, int w, int64_t o,
+    uint64_t zsub, int k, uint32_t* ntest)
+{
+    const uint64_t C = A[o];
+    uint64_t cand = C & E[o] & zsub;
+    if (!cand)
+        return 0;
+    Nbhd N;
+    cand = load_nbhd(P, A, C, cand, x, y, w, N);
+    uint64_t removed = 0;
+    while (cand)
+    {
+        int bb[4];
+        uint32_t cc[4];
+        int nb = 0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+        {
+            bb[jj] = 0;
+            cc[jj] = 0;
+            if (cand)
+            {
+                bb[jj] = __ffsll((unsigned long long)cand) - 1;
+                cand &= cand - 1;
+                cc[jj] = nb_code(N.R, N.pv, N.nx0, bb[jj]);
+                nb = jj + 1;
+            }
+        }
+        uint32_t lw[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+            lw[jj] = jj < nb ? lut[cc[jj] >> 5] : 0u;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+            if ((lw[jj] >> (cc[jj] & 31)) & 1u)
+                removed |= 1ull << bb[jj];
+        *ntest += nb;
+    }
+    if (removed)
+        apply_removal(P, A, label, D0, D1, Fo, x, y, w, o, C, removed, k);
     return removed;
+}
+
+// exclusive prefix sum of v over the workgroup (kBlock threads); *total gets
+// the sum.  `wsum` holds kBlock/64 entries; ends after a barrier, so the
+// caller must not rewrite wsum before its next barrier.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* wsum,
+                                               uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t t = __shfl_up(x, d);
+        if (lane >= d)
+            x += t;
+    }
+    if (lane == 63)
+        wsum[wv] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i)
+    {
+        const uint32_t t = wsum[i];
+        base += i < wv ? t : 0u;
+        tot += t;
+    }
+    *total = tot;
+    return base + x - v;
 }
 
 // class-relative word t of a pass -> (x, y, w); false past the last row
@@ -508,7 +625,15 @@ __device__ __forceinline__ PassCtl pass_ctl(const uint32_t* ctl, int j,
 }
 
 // One subfield pass, dense: an item is 256 words of one volume (thread =
-// word, words ordered (x/2, y/2, w) within the pass's row parity).
+// word, words ordered (x/2, y/2, w) within the pass's row parity).  Each
+// thread filters its word's candidates; the candidates of the whole item are
+// then queued in LDS (26-neighbourhood code and bit, owner word) and every
+// thread takes kQ/kBlock queue entries per round, so the table lookups of a
+// word with many candidates are spread over the workgroup and issued
+// together.  Hits are or-ed into the owner word's removal mask in LDS; each
+// owner applies its own removals.
+constexpr int kQ = 1024;
+
 template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     thin_pass(vc_work W, const uint32_t* __restrict__ lut, int s, int j,
@@ -520,6 +645,9 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t sweep = pc.sweep;
     const int k = pc.k;
     __shared__ uint32_t red[kBlock / 64];
+    __shared__ uint32_t qc[kQ];  // code << 6 | bit
+    __shared__ uint8_t qw[kQ];   // owner thread
+    __shared__ unsigned long long rmv[kBlock];
     const int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
     const uint64_t zsub = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
     for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
@@ -540,20 +668,65 @@ __global__ void __launch_bounds__(kBlock)
         const uint8_t fl = valid ? Fs[o] : (uint8_t)0;
         if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
             continue; // converged at this level (block-uniform)
-        uint64_t removed = 0;
-        uint32_t ntest = 0;
+        uint64_t C = 0, cand = 0;
+        Nbhd N;
         if (fl)
         {
             Fs[o] = 0;
-            removed = visit_word(P, d.alive, d.elig, d.label, d.d0, d.d1, Fo,
-                                 lut, x, y, w, o, zsub, k, &ntest);
+            C = d.alive[o];
+            cand = C & d.elig[o] & zsub;
+            if (cand)
+                cand = load_nbhd(P, d.alive, C, cand, x, y, w, N);
         }
-        // blocks that tested nothing skip the reductions (block-uniform)
-        if (!__syncthreads_or((removed != 0) | (ntest != 0)))
+        // items without candidates end here (block-uniform)
+        if (!__syncthreads_or(cand != 0))
             continue;
+        rmv[threadIdx.x] = 0;
+        uint32_t total;
+        const uint32_t off =
+            block_scan((uint32_t)__popcll(cand), red, &total);
+        uint32_t idx = off;
+        uint64_t rest = cand;
+        for (uint32_t c0 = 0; c0 < total; c0 += kQ)
+        {
+            // queue this thread's candidates that fall in [c0, c0 + kQ)
+            while (rest && idx < c0 + kQ)
+            {
+                const int bt = __ffsll((unsigned long long)rest) - 1;
+                rest &= rest - 1;
+                qc[idx - c0] = (nb_code(N.R, N.pv, N.nx0, bt) << 6) | (uint32_t)bt;
+                qw[idx - c0] = (uint8_t)threadIdx.x;
+                ++idx;
+            }
+            __syncthreads();
+            const uint32_t n = min((uint32_t)kQ, total - c0);
+            uint32_t qv[kQ / kBlock], lw[kQ / kBlock];
+#pragma unroll
+            for (int r = 0; r < kQ / kBlock; ++r)
+            {
+                const uint32_t e = threadIdx.x + r * kBlock;
+                qv[r] = e < n ? qc[e] : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < kQ / kBlock; ++r)
+            {
+                const uint32_t e = threadIdx.x + r * kBlock;
+                lw[r] = e < n ? lut[qv[r] >> 11] : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < kQ / kBlock; ++r)
+            {
+                const uint32_t e = threadIdx.x + r * kBlock;
+                if (e < n && ((lw[r] >> ((qv[r] >> 6) & 31u)) & 1u))
+                    atomicOr(&rmv[qw[e]], 1ull << (qv[r] & 63u));
+            }
+            __syncthreads();
+        }
+        const uint64_t removed = rmv[threadIdx.x];
+        if (removed)
+            apply_removal(P, d.alive, d.label, d.d0, d.d1, Fo, x, y, w, o, C,
+                          removed, k);
         const uint32_t nrem = block_sum((uint32_t)__popcll(removed), red);
-        __syncthreads();
-        const uint32_t ntst = block_sum(ntest, red);
         if (threadIdx.x == 0)
         {
             // counters spread over VC_CTR_SLOTS cache lines: same-address
@@ -566,10 +739,9 @@ __global__ void __launch_bounds__(kBlock)
                 vany[4 * vi + (sweep & 3)] = 1u;
                 atomicAdd(&c[8 + k], (unsigned long long)nrem);
             }
-            if (ntst)
-                atomicAdd(&c[0], (unsigned long long)ntst);
+            atomicAdd(&c[0], (unsigned long long)total);
         }
-        __syncthreads(); // red is rewritten by the next item
+        __syncthreads(); // red, rmv are rewritten by the next item
     }
 }
 
@@ -660,6 +832,341 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// ------------------------------------------------------------ tile sweeps
+// One launch per sweep for the late sweeps of a level.  A tile is 32 x 32
+// rows (x, y) of one word (64 z).  Its region -- the tile plus 8 rows on each
+// side in x and y and 32 bits on each side in z -- is loaded into LDS as
+// 128-bit row windows (bit i = z 64w - 32 + i) and the 8 subfield passes run
+// there.  A decision at pass p reaches the interior only from rows within
+// 7 - p of it, so pass p evaluates those rows only; rows and bits outside the
+// region read as background, and the errors that can cause stay at least
+// 8 - p rows away from the interior.  The interior is therefore exact and is
+// written back.
+//
+// Planes: sweep t reads alive (t even) or alive2 (t odd) and writes the other
+// one.  A tile that is not visited has equal content in both planes (it did
+// not change in the previous sweep, and the sweep before wrote it), so no
+// copy is needed; the entry kernel copies alive to alive2 once.  A tile is
+// visited iff a tile of its 3 x 3 x 3 block changed in the previous sweep
+// (changes travel at most 8 voxels per sweep, less than a tile); on entry
+// the tiles holding a revisit flag count as changed.  At the end of a level
+// (a sweep that changed nothing) both planes are equal again.
+constexpr int kTT = 32, kTH = 8, kTR = kTT + 2 * kTH, kTRows = kTR * kTR;
+#ifndef VC_TILE_BLOCK
+#define VC_TILE_BLOCK 512
+#endif
+constexpr int kTBlock = VC_TILE_BLOCK; // threads of a tile-sweep workgroup
+constexpr int kTQ = 1024;              // candidate queue entries per round
+// rows of one parity evaluated per pass: at most 23 x 23 (pass 0)
+constexpr int kTRowsPerThread = (23 * 23 + kTBlock - 1) / kTBlock;
+typedef unsigned __int128 u128;
+
+__device__ __forceinline__ u128 load_win(const uint64_t* __restrict__ P,
+                                         int64_t o, int w, int nwz)
+{
+    const uint64_t c = P[o];
+    const uint64_t p = w > 0 ? P[o - 1] : 0ull;
+    const uint64_t n = w + 1 < nwz ? P[o + 1] : 0ull;
+    const uint64_t lo = (p >> 32) | (c << 32);
+    const uint64_t hi = (c >> 32) | (n << 32);
+    return ((u128)hi << 64) | (u128)lo;
+}
+
+// 26-neighbourhood code of window bit i (1 <= i <= 126), the order of nb_code
+__device__ __forceinline__ uint32_t nb_code_w(const u128 (&R)[3][3], int i)
+{
+    uint32_t code = 0;
+    int pos = 0;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+        {
+            const uint32_t r3 = (uint32_t)(R[dx][dy] >> (i - 1)) & 7u;
+            if (dx == 1 && dy == 1)
+            {
+                code |= ((r3 & 1u) | ((r3 >> 1) & 2u)) << pos;
+                pos += 2;
+            }
+            else
+            {
+                code |= r3 << pos;
+                pos += 3;
+            }
+        }
+    return code;
+}
+
+__device__ __forceinline__ int ctz128(u128 v)
+{
+    const uint64_t lo = (uint64_t)v;
+    return lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll((uint64_t)(v >> 64));
+}
+
+// bits [a, b) of a 128-bit window
+__device__ __forceinline__ u128 bit_range(int a, int b)
+{
+    const u128 one = 1;
+    return ((one << b) - 1) & ~((one << a) - 1);
+}
+
+// entry: alive2 = alive on every tile, change flags (both parities) = the
+// tile holds a revisit flag
+template <bool kB>
+__global__ void __launch_bounds__(kBlock)
+    thin_tile_enter(vc_work W)
+{
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    {
+        int vi;
+        uint32_t lt;
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
+        const int nwz = d.nwz;
+        const int w = (int)(lt % (uint32_t)nwz);
+        const uint32_t r2 = lt / (uint32_t)nwz;
+        const int ty = (int)(r2 % (uint32_t)d.nty), tx = (int)(r2 / (uint32_t)d.nty);
+        bool f = false;
+        for (int q = threadIdx.x; q < kTT * kTT; q += kBlock)
+        {
+            const int x = tx * kTT + q / kTT, y = ty * kTT + q % kTT;
+            if (x >= d.nx || y >= d.ny)
+                continue;
+            const int64_t o = ((int64_t)x * d.ny + y) * nwz + w;
+            d.alive2[o] = d.alive[o];
+            f = f || d.d0[o] || d.d1[o];
+        }
+        const bool any = __syncthreads_or(f);
+        if (threadIdx.x == 0)
+        {
+            const int64_t nt = (int64_t)d.ntx * d.nty * nwz;
+            d.tchg[lt] = any ? 1 : 0;
+            d.tchg[nt + lt] = any ? 1 : 0;
+        }
+    }
+}
+
+// exclusive prefix sum over a tile-sweep workgroup (kTBlock threads)
+__device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t* wsum,
+                                              uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t t = __shfl_up(x, d);
+        if (lane >= d)
+            x += t;
+    }
+    if (lane == 63)
+        wsum[wv] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kTBlock / 64; ++i)
+    {
+        const uint32_t t = wsum[i];
+        base += i < wv ? t : 0u;
+        tot += t;
+    }
+    *total = tot;
+    return base + x - v;
+}
+
+// One sweep over the visited tiles.  Each pass: the threads filter their
+// rows' candidates (alive, eligible, z parity, range, a background face
+// neighbour), queue them in LDS (row << 7 | bit), and every thread then takes
+// kTQ / kTBlock queue entries: neighbourhood code from the LDS rows, table
+// lookup, and on a hit an LDS atomic clears the bit.  Same-subfield voxels
+// are not 26-adjacent, so a code never reads a bit that this pass clears.
+template <bool kB>
+__global__ void __launch_bounds__(kTBlock)
+    thin_tile_sweep(vc_work W, const uint32_t* __restrict__ lut, int j,
+                    int k_arg, int nvol, const uint32_t* __restrict__ ctl,
+                    uint32_t* gany, uint32_t* vany,
+                    unsigned long long* __restrict__ ctr)
+{
+    const PassCtl pc = pass_ctl(ctl, j, k_arg, 0, nvol, gany, vany);
+    const uint32_t sweep = pc.sweep;
+    const int k = pc.k;
+    __shared__ u128 SA[kTRows]; // alive
+    __shared__ u128 SE[kTRows]; // eligible (constant within the level)
+    __shared__ uint32_t tq[kTQ];
+    __shared__ uint32_t red[kTBlock / 64];
+    unsigned long long* SA64 = reinterpret_cast<unsigned long long*>(SA);
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    {
+        int vi;
+        uint32_t lt;
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
+        if (sweep > 0 && vany[4 * vi + ((sweep - 1) & 3)] == 0)
+            continue; // converged at this level (block-uniform)
+        const int nwz = d.nwz;
+        const int64_t nt = (int64_t)d.ntx * d.nty * nwz;
+        const int w = (int)(lt % (uint32_t)nwz);
+        const uint32_t r2 = lt / (uint32_t)nwz;
+        const int ty = (int)(r2 % (uint32_t)d.nty), tx = (int)(r2 / (uint32_t)d.nty);
+        const uint8_t* cprev = d.tchg + (int64_t)((sweep + 1) & 1) * nt;
+        uint8_t* ccur = d.tchg + (int64_t)(sweep & 1) * nt;
+        bool act = false;
+        if (threadIdx.x < 27)
+        {
+            const int ax = tx + (int)threadIdx.x / 9 - 1;
+            const int ay = ty + ((int)threadIdx.x / 3) % 3 - 1;
+            const int aw = w + (int)threadIdx.x % 3 - 1;
+            if ((unsigned)ax < (unsigned)d.ntx && (unsigned)ay < (unsigned)d.nty &&
+                (unsigned)aw < (unsigned)nwz)
+                act = cprev[((int64_t)ax * d.nty + ay) * nwz + aw] != 0;
+        }
+        if (!__syncthreads_or(act))
+        {
+            if (threadIdx.x == 0)
+                ccur[lt] = 0;
+            continue;
+        }
+        const uint64_t* Pin = (sweep & 1) ? d.alive2 : d.alive;
+        uint64_t* Pout = (sweep & 1) ? d.alive : d.alive2;
+        const int X0 = tx * kTT - kTH, Y0 = ty * kTT - kTH; // region origin
+        for (int r = threadIdx.x; r < kTRows; r += kTBlock)
+        {
+            const int x = X0 + r / kTR, y = Y0 + r % kTR;
+            u128 a = 0, e = 0;
+            if ((unsigned)x < (unsigned)d.nx && (unsigned)y < (unsigned)d.ny)
+            {
+                const int64_t o = ((int64_t)x * d.ny + y) * nwz + w;
+                a = load_win(Pin, o, w, nwz);
+                e = load_win(d.elig, o, w, nwz);
+            }
+            SA[r] = a;
+            SE[r] = e;
+        }
+        __syncthreads();
+        uint32_t nrem = 0, ntest = 0;
+        uint8_t* lrow0 = d.label + (int64_t)64 * w;
+        for (int p = 0; p < 8; ++p)
+        {
+            const int sx = p & 1, sy = (p >> 1) & 1, sz = (p >> 2) & 1;
+            const int D = 7 - p;
+            // region rows [kTH - D, kTH + kTT + D) of parity (sx, sy); the
+            // region origin is even, so a row's parity is its region index's
+            const int lo = kTH - D, hi = kTH + kTT + D;
+            const int rx0 = lo + ((lo & 1) != sx), ry0 = lo + ((lo & 1) != sy);
+            const int nrx = (hi - rx0 + 1) >> 1, nry = (hi - ry0 + 1) >> 1;
+            // window bit parity = z parity (64w - 32 is even)
+            const uint64_t zs64 = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+            const u128 zmask = (((u128)zs64 << 64) | (u128)zs64) &
+                               bit_range(32 - D, 96 + D);
+            u128 cnd[kTRowsPerThread];
+            int rr[kTRowsPerThread];
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int t = 0; t < kTRowsPerThread; ++t)
+            {
+                cnd[t] = 0;
+                rr[t] = 0;
+                const int q = threadIdx.x + t * kTBlock;
+                if (q >= nrx * nry)
+                    continue;
+                const int rx = rx0 + 2 * (q / nry), ry = ry0 + 2 * (q % nry);
+                const int r = rx * kTR + ry;
+                const u128 C = SA[r];
+                u128 c = C & SE[r] & zmask;
+                if (c)
+                    c &= ~(SA[r - kTR] & SA[r + kTR] & SA[r - 1] & SA[r + 1] &
+                           (C << 1) & (C >> 1));
+                cnd[t] = c;
+                rr[t] = r;
+                cnt += (uint32_t)(__popcll((uint64_t)c) +
+                                  __popcll((uint64_t)(c >> 64)));
+            }
+            uint32_t total;
+            uint32_t idx = tile_scan(cnt, red, &total);
+            ntest += total;
+            for (uint32_t c0 = 0; c0 < total; c0 += kTQ)
+            {
+#pragma unroll
+                for (int t = 0; t < kTRowsPerThread; ++t)
+                    while (cnd[t] && idx < c0 + kTQ)
+                    {
+                        const int i = ctz128(cnd[t]);
+                        cnd[t] &= cnd[t] - 1;
+                        tq[idx - c0] = ((uint32_t)rr[t] << 7) | (uint32_t)i;
+                        ++idx;
+                    }
+                __syncthreads();
+                const uint32_t n = min((uint32_t)kTQ, total - c0);
+                uint32_t ent[kTQ / kTBlock], code[kTQ / kTBlock],
+                    lw[kTQ / kTBlock];
+#pragma unroll
+                for (int t = 0; t < kTQ / kTBlock; ++t)
+                {
+                    const uint32_t e = threadIdx.x + t * kTBlock;
+                    ent[t] = e < n ? tq[e] : 0xFFFFFFFFu;
+                    code[t] = 0;
+                    if (e < n)
+                    {
+                        const int r = (int)(ent[t] >> 7);
+                        u128 R[3][3];
+#pragma unroll
+                        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                            for (int dy = 0; dy < 3; ++dy)
+                                R[dx][dy] = SA[r + (dx - 1) * kTR + (dy - 1)];
+                        code[t] = nb_code_w(R, (int)(ent[t] & 127u));
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < kTQ / kTBlock; ++t)
+                    lw[t] = ent[t] != 0xFFFFFFFFu ? lut[code[t] >> 5] : 0u;
+#pragma unroll
+                for (int t = 0; t < kTQ / kTBlock; ++t)
+                {
+                    if (ent[t] == 0xFFFFFFFFu || !((lw[t] >> (code[t] & 31)) & 1u))
+                        continue;
+                    const int r = (int)(ent[t] >> 7), i = (int)(ent[t] & 127u);
+                    atomicAnd(&SA64[2 * r + (i >> 6)], ~(1ull << (i & 63)));
+                    const int rx = r / kTR, ry = r % kTR;
+                    if (rx >= kTH && rx < kTH + kTT && ry >= kTH &&
+                        ry < kTH + kTT && i >= 32 && i < 96)
+                    {
+                        ++nrem;
+                        lrow0[((int64_t)(X0 + rx) * d.ny + (Y0 + ry)) * d.nz +
+                              (i - 32)] = (uint8_t)(2 + k);
+                    }
+                }
+                __syncthreads();
+            }
+            __syncthreads(); // red is rewritten by the next pass's scan
+        }
+        for (int q = threadIdx.x; q < kTT * kTT; q += kTBlock)
+        {
+            const int rx = kTH + q / kTT, ry = kTH + q % kTT;
+            const int x = X0 + rx, y = Y0 + ry;
+            if (x < d.nx && y < d.ny)
+                Pout[((int64_t)x * d.ny + y) * nwz + w] =
+                    (uint64_t)(SA[rx * kTR + ry] >> 32);
+        }
+        const uint32_t tot = block_sum(nrem, red);
+        if (threadIdx.x == 0)
+        {
+            ccur[lt] = tot ? 1 : 0;
+            unsigned long long* c =
+                ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
+            if (tot)
+            {
+                gany[sweep & (VC_RING - 1)] = 1u;
+                vany[4 * vi + (sweep & 3)] = 1u;
+                atomicAdd(&c[8 + k], (unsigned long long)tot);
+            }
+            if (ntest)
+                atomicAdd(&c[0], (unsigned long long)ntest);
+        }
+        __syncthreads(); // SA, SE, tq, red are rewritten by the next item
+    }
+}
+
 // the next segment's first sweep
 __global__ void seg_advance(uint32_t* ctl, int K)
 {
@@ -698,41 +1205,84 @@ bool tail_sweep_of(const vc_sched* s, int64_t sweep)
     return s->sc.tail_sweep >= 0 && sweep >= s->sc.tail_sweep;
 }
 
+// how a segment's sweeps run
+enum SegMode
+{
+    SEG_DENSE = 0,
+    SEG_SPARSE = 1,
+    SEG_TILE = 2,       // tile sweeps
+    SEG_TILE_ENTRY = 3  // tile sweeps, entry kernel first
+};
+
+template <bool kB>
+void launch_tile_sweep(hipStream_t S, unsigned g, const vc_work& W,
+                       const uint32_t* lut, int j, int k, int n,
+                       const uint32_t* ctl, vc_sched* s)
+{
+    thin_tile_sweep<kB><<<g, kTBlock, 0, S>>>(W, lut, j, k, n, ctl, s->d_gany,
+                                             s->d_vany, s->d_ctr);
+}
+
 int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
-                   int k, bool graph_tail)
+                   int k, int mode)
 {
     hipStream_t S = ctx->stream;
     const int pc = s->sc.thin_grid_per_cu;
-    for (int j = 0; j < K; ++j)
+    const uint32_t* ctl = graph ? s->d_ctl : nullptr;
+    if (mode >= SEG_TILE)
     {
-        const bool tail = graph ? graph_tail : tail_sweep_of(s, first + j);
-        for (int sub = 0; sub < 8; ++sub)
+        const unsigned g = vc_sched_grid(ctx, s, VK_TT, pc);
+        const vc_work W = vc_sched_work(s, VK_TT);
+        if (g && mode == SEG_TILE_ENTRY)
         {
-            const int kind = VK_TP + (sub & 3);
-            const unsigned g = vc_sched_grid(ctx, s, kind, pc);
-            if (!g)
-                continue;
-            const vc_work W = vc_sched_work(s, kind);
-            const int jj = graph ? j : (int)(first + j);
-            const uint32_t* ctl = graph ? s->d_ctl : nullptr;
             if (s->n > 1)
-            {
-                if (tail)
-                    launch_pass<true, true>(S, g, W, ctx->d_lut, sub, jj, k,
-                                            s->n, ctl, s);
-                else
-                    launch_pass<true, false>(S, g, W, ctx->d_lut, sub, jj, k,
-                                             s->n, ctl, s);
-            }
-            else if (tail)
-                launch_pass<false, true>(S, g, W, ctx->d_lut, sub, jj, k, s->n,
-                                         ctl, s);
+                thin_tile_enter<true><<<g, kBlock, 0, S>>>(W);
             else
-                launch_pass<false, false>(S, g, W, ctx->d_lut, sub, jj, k,
-                                          s->n, ctl, s);
+                thin_tile_enter<false><<<g, kBlock, 0, S>>>(W);
+            VC_CHECK_LAUNCH(ctx);
+        }
+        for (int j = 0; j < K && g; ++j)
+        {
+            const int jj = graph ? j : (int)(first + j);
+            if (s->n > 1)
+                launch_tile_sweep<true>(S, g, W, ctx->d_lut, jj, k, s->n, ctl, s);
+            else
+                launch_tile_sweep<false>(S, g, W, ctx->d_lut, jj, k, s->n, ctl,
+                                         s);
             VC_CHECK_LAUNCH(ctx);
         }
     }
+    else
+        for (int j = 0; j < K; ++j)
+        {
+            const bool tail =
+                graph ? mode == SEG_SPARSE : tail_sweep_of(s, first + j);
+            for (int sub = 0; sub < 8; ++sub)
+            {
+                const int kind = VK_TP + (sub & 3);
+                const unsigned g = vc_sched_grid(ctx, s, kind, pc);
+                if (!g)
+                    continue;
+                const vc_work W = vc_sched_work(s, kind);
+                const int jj = graph ? j : (int)(first + j);
+                if (s->n > 1)
+                {
+                    if (tail)
+                        launch_pass<true, true>(S, g, W, ctx->d_lut, sub, jj, k,
+                                                s->n, ctl, s);
+                    else
+                        launch_pass<true, false>(S, g, W, ctx->d_lut, sub, jj,
+                                                 k, s->n, ctl, s);
+                }
+                else if (tail)
+                    launch_pass<false, true>(S, g, W, ctx->d_lut, sub, jj, k,
+                                             s->n, ctl, s);
+                else
+                    launch_pass<false, false>(S, g, W, ctx->d_lut, sub, jj, k,
+                                              s->n, ctl, s);
+                VC_CHECK_LAUNCH(ctx);
+            }
+        }
     if (graph)
     {
         seg_advance<<<1, 64, 0, S>>>(s->d_ctl, K);
@@ -741,20 +1291,21 @@ int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
     return VC_OK;
 }
 
-// a segment as a captured graph (captured once per length, then replayed)
-// (graph segments use one variant, chosen by their first sweep)
-int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k)
+// a segment as a captured graph (captured once per length and mode, then
+// replayed); dense/sparse graph segments use one variant, chosen by their
+// first sweep
+int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k,
+                int mode)
 {
     if (!s->sc.use_graph)
-        return launch_segment(ctx, s, K, false, first, k, false);
+        return launch_segment(ctx, s, K, false, first, k, mode);
     hipStream_t S = ctx->stream;
-    const bool tail = tail_sweep_of(s, first);
-    const int key = K * 2 + (tail ? 1 : 0);
+    const int key = K * 4 + mode;
     auto it = s->graphs.find(key);
     if (it == s->graphs.end())
     {
         VC_HIP(ctx, hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
-        const int rc = launch_segment(ctx, s, K, true, 0, 0, tail);
+        const int rc = launch_segment(ctx, s, K, true, 0, 0, mode);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(S, &g);
         if (rc != VC_OK)
@@ -856,16 +1407,28 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
         }
         int64_t base = 0;
         int K = sc.sweeps_per_sync ? sc.sweeps_per_sync : VC_SEG_MIN;
+        bool in_tile = false;
         for (;;)
         {
             if (base > guard)
                 return vc_fail(ctx, VC_ERR_GUARD,
                                "thinning did not converge in %lld sweeps",
                                (long long)base);
-            rc = run_segment(ctx, s, K, base, k);
+            // tile sweeps from the first segment starting at >= tile_sweep
+            const bool tile = sc.tile_sweep >= 0 && base >= sc.tile_sweep &&
+                              s->total[VK_TT] != 0;
+            const int mode = tile ? (in_tile ? SEG_TILE : SEG_TILE_ENTRY)
+                                  : (tail_sweep_of(s, base) ? SEG_SPARSE
+                                                            : SEG_DENSE);
+            rc = run_segment(ctx, s, K, base, k, mode);
             if (rc != VC_OK)
                 return rc;
-            launches += (int64_t)K * (per_seg - 1) + (sc.use_graph ? 1 : 0);
+            if (tile)
+                launches += (int64_t)K + (in_tile ? 0 : 1) +
+                            (sc.use_graph ? 1 : 0);
+            else
+                launches += (int64_t)K * (per_seg - 1) + (sc.use_graph ? 1 : 0);
+            in_tile = tile;
             VC_HIP(ctx, hipMemcpyAsync(s->h_ring, s->d_gany, 4 * VC_RING,
                                        hipMemcpyDeviceToHost, S));
             VC_HIP(ctx, hipStreamSynchronize(S));

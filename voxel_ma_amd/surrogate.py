@@ -28,7 +28,9 @@ def make_volume(ctx, L, rng, blur_frac=0.0):
     the benchmark volume uses seed 3), else a torus / ball union
     (synth.training_specs); vc_edt + vc_measure run on it."""
     from voxel_ma_amd import synth
-    if rng.random() < blur_frac:
+    # no draw when blur_frac is 0: the generator stream (and so the volumes
+    # of a seed) is the one the committed weights were trained on
+    if blur_frac > 0 and rng.random() < blur_frac:
         buf, _ = ctx.synth_blurred_noise((L, L, L), sigma=12.0, fill=0.30,
                                          seed=int(rng.integers(1000, 1 << 40)))
         vol = ctx.from_device(buf.ptr, (L, L, L))
