@@ -833,15 +833,22 @@ __global__ void __launch_bounds__(64)
 }
 
 // ------------------------------------------------------------ tile sweeps
-// One launch per sweep for the late sweeps of a level.  A tile is 32 x 32
-// rows (x, y) of one word (64 z).  Its region -- the tile plus 8 rows on each
-// side in x and y and 32 bits on each side in z -- is loaded into LDS as
-// 128-bit row windows (bit i = z 64w - 32 + i) and the 8 subfield passes run
-// there.  A decision at pass p reaches the interior only from rows within
-// 7 - p of it, so pass p evaluates those rows only; rows and bits outside the
-// region read as background, and the errors that can cause stay at least
-// 8 - p rows away from the interior.  The interior is therefore exact and is
-// written back.
+// One launch per sweep.  A tile is 32 x 32 rows (x, y) of one word (64 z).
+// The sweep's 8 subfield passes run in LDS on the tile's region: the tile
+// plus a halo of 8 rows in x, 4 rows in y and 2 bits in z.  Why that halo:
+// the decision of voxel u at pass p = sx + 2 sy + 4 sz reaches the interior
+// only through a chain of neighbours decided at strictly later passes; along
+// such a chain x parity can flip at every pass, y parity at 1->2, 3->4, 5->6
+// and z parity at 3->4 only.  So u matters iff its distance from the interior
+// is at most Fx(p) = 7 - p in x, Fy(p) = {3,3,2,2,1,1,0,0}[p] in y and
+// Fz(p) = (p < 4) in z; exactly those voxels are evaluated, and they read
+// neighbours at most one further out, all inside the region.  A neighbour
+// that changes before the reading pass is itself within its own pass's
+// range (its F is larger by one in every axis in which it differs), so every
+// read sees the true state and the interior is exact.
+//
+// Rows are held as 68-bit windows (bit i = z 64w - 2 + i) in two parts, a
+// 64-bit low word and the 4 high bits.
 //
 // Planes: sweep t reads alive (t even) or alive2 (t odd) and writes the other
 // one.  A tile that is not visited has equal content in both planes (it did
@@ -851,28 +858,39 @@ __global__ void __launch_bounds__(64)
 // (changes travel at most 8 voxels per sweep, less than a tile); on entry
 // the tiles holding a revisit flag count as changed.  At the end of a level
 // (a sweep that changed nothing) both planes are equal again.
-constexpr int kTT = 32, kTH = 8, kTR = kTT + 2 * kTH, kTRows = kTR * kTR;
+constexpr int kTT = 32;                     // interior rows along x and y
+constexpr int kHX = 8, kHY = 4, kHZ = 2;    // halo: rows x, rows y, bits z
+constexpr int kRX = kTT + 2 * kHX, kRY = kTT + 2 * kHY, kTRows = kRX * kRY;
 #ifndef VC_TILE_BLOCK
 #define VC_TILE_BLOCK 512
 #endif
 constexpr int kTBlock = VC_TILE_BLOCK; // threads of a tile-sweep workgroup
+#ifndef VC_TILE_WAVES
+#define VC_TILE_WAVES 6
+#endif
+// waves per SIMD the tile sweep is compiled for (3 workgroups of 512 per CU,
+// which its 44 KB of LDS allows)
+constexpr int kTWaves = VC_TILE_WAVES;
 constexpr int kTQ = 1024;              // candidate queue entries per round
-// rows of one parity evaluated per pass: at most 23 x 23 (pass 0)
-constexpr int kTRowsPerThread = (23 * 23 + kTBlock - 1) / kTBlock;
+// rows of one parity evaluated per pass: at most 23 x 19 (pass 0)
+constexpr int kTRowsPerThread = (23 * 19 + kTBlock - 1) / kTBlock;
 typedef unsigned __int128 u128;
 
-__device__ __forceinline__ u128 load_win(const uint64_t* __restrict__ P,
-                                         int64_t o, int w, int nwz)
+__device__ __forceinline__ int fy_of(int p) { return (7 - p) >> 1; }
+
+// window of word (o, w): bits 62..63 of word w-1, word w, bits 0..1 of w+1
+__device__ __forceinline__ void load_win(const uint64_t* __restrict__ P,
+                                         int64_t o, int w, int nwz,
+                                         uint64_t& lo, uint32_t& hi)
 {
     const uint64_t c = P[o];
     const uint64_t p = w > 0 ? P[o - 1] : 0ull;
     const uint64_t n = w + 1 < nwz ? P[o + 1] : 0ull;
-    const uint64_t lo = (p >> 32) | (c << 32);
-    const uint64_t hi = (c >> 32) | (n << 32);
-    return ((u128)hi << 64) | (u128)lo;
+    lo = (p >> 62) | (c << 2);
+    hi = (uint32_t)(c >> 62) | ((uint32_t)(n & 3ull) << 2);
 }
 
-// 26-neighbourhood code of window bit i (1 <= i <= 126), the order of nb_code
+// 26-neighbourhood code of window bit i (1 <= i <= 66), the order of nb_code
 __device__ __forceinline__ uint32_t nb_code_w(const u128 (&R)[3][3], int i)
 {
     uint32_t code = 0;
@@ -981,7 +999,7 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t* wsum,
 // lookup, and on a hit an LDS atomic clears the bit.  Same-subfield voxels
 // are not 26-adjacent, so a code never reads a bit that this pass clears.
 template <bool kB>
-__global__ void __launch_bounds__(kTBlock)
+__global__ void __launch_bounds__(kTBlock, kTWaves)
     thin_tile_sweep(vc_work W, const uint32_t* __restrict__ lut, int j,
                     int k_arg, int nvol, const uint32_t* __restrict__ ctl,
                     uint32_t* gany, uint32_t* vany,
@@ -990,11 +1008,12 @@ __global__ void __launch_bounds__(kTBlock)
     const PassCtl pc = pass_ctl(ctl, j, k_arg, 0, nvol, gany, vany);
     const uint32_t sweep = pc.sweep;
     const int k = pc.k;
-    __shared__ u128 SA[kTRows]; // alive
-    __shared__ u128 SE[kTRows]; // eligible (constant within the level)
+    __shared__ unsigned long long SAlo[kTRows]; // alive, window bits 0..63
+    __shared__ uint32_t SAhi[kTRows];           // alive, window bits 64..67
+    __shared__ unsigned long long SElo[kTRows]; // eligible (constant in a level)
+    __shared__ uint8_t SEhi[kTRows];
     __shared__ uint32_t tq[kTQ];
     __shared__ uint32_t red[kTBlock / 64];
-    unsigned long long* SA64 = reinterpret_cast<unsigned long long*>(SA);
     for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
     {
         int vi;
@@ -1028,36 +1047,44 @@ __global__ void __launch_bounds__(kTBlock)
         }
         const uint64_t* Pin = (sweep & 1) ? d.alive2 : d.alive;
         uint64_t* Pout = (sweep & 1) ? d.alive : d.alive2;
-        const int X0 = tx * kTT - kTH, Y0 = ty * kTT - kTH; // region origin
+        const int X0 = tx * kTT - kHX, Y0 = ty * kTT - kHY; // region origin
         for (int r = threadIdx.x; r < kTRows; r += kTBlock)
         {
-            const int x = X0 + r / kTR, y = Y0 + r % kTR;
-            u128 a = 0, e = 0;
+            const int x = X0 + r / kRY, y = Y0 + r % kRY;
+            uint64_t alo = 0, elo = 0;
+            uint32_t ahi = 0, ehi = 0;
             if ((unsigned)x < (unsigned)d.nx && (unsigned)y < (unsigned)d.ny)
             {
                 const int64_t o = ((int64_t)x * d.ny + y) * nwz + w;
-                a = load_win(Pin, o, w, nwz);
-                e = load_win(d.elig, o, w, nwz);
+                load_win(Pin, o, w, nwz, alo, ahi);
+                load_win(d.elig, o, w, nwz, elo, ehi);
             }
-            SA[r] = a;
-            SE[r] = e;
+            SAlo[r] = alo;
+            SAhi[r] = ahi;
+            SElo[r] = elo;
+            SEhi[r] = (uint8_t)ehi;
         }
         __syncthreads();
+        auto row = [&](int r) -> u128 {
+            return ((u128)SAhi[r] << 64) | (u128)SAlo[r];
+        };
         uint32_t nrem = 0, ntest = 0;
         uint8_t* lrow0 = d.label + (int64_t)64 * w;
         for (int p = 0; p < 8; ++p)
         {
             const int sx = p & 1, sy = (p >> 1) & 1, sz = (p >> 2) & 1;
-            const int D = 7 - p;
-            // region rows [kTH - D, kTH + kTT + D) of parity (sx, sy); the
-            // region origin is even, so a row's parity is its region index's
-            const int lo = kTH - D, hi = kTH + kTT + D;
-            const int rx0 = lo + ((lo & 1) != sx), ry0 = lo + ((lo & 1) != sy);
-            const int nrx = (hi - rx0 + 1) >> 1, nry = (hi - ry0 + 1) >> 1;
-            // window bit parity = z parity (64w - 32 is even)
+            const int Fx = 7 - p, Fy = fy_of(p), Fz = p < 4 ? 1 : 0;
+            // rows [kHX - Fx, kHX + kTT + Fx) x [kHY - Fy, kHY + kTT + Fy) of
+            // parity (sx, sy); the region origin is even, so a row's parity
+            // is its region index's
+            const int xlo = kHX - Fx, xhi = kHX + kTT + Fx;
+            const int ylo = kHY - Fy, yhi = kHY + kTT + Fy;
+            const int rx0 = xlo + ((xlo & 1) != sx), ry0 = ylo + ((ylo & 1) != sy);
+            const int nrx = (xhi - rx0 + 1) >> 1, nry = (yhi - ry0 + 1) >> 1;
+            // window bit parity = z parity (64w - 2 is even)
             const uint64_t zs64 = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
             const u128 zmask = (((u128)zs64 << 64) | (u128)zs64) &
-                               bit_range(32 - D, 96 + D);
+                               bit_range(kHZ - Fz, kHZ + 64 + Fz);
             u128 cnd[kTRowsPerThread];
             int rr[kTRowsPerThread];
             uint32_t cnt = 0;
@@ -1070,11 +1097,12 @@ __global__ void __launch_bounds__(kTBlock)
                 if (q >= nrx * nry)
                     continue;
                 const int rx = rx0 + 2 * (q / nry), ry = ry0 + 2 * (q % nry);
-                const int r = rx * kTR + ry;
-                const u128 C = SA[r];
-                u128 c = C & SE[r] & zmask;
+                const int r = rx * kRY + ry;
+                const u128 C = row(r);
+                const u128 E = ((u128)SEhi[r] << 64) | (u128)SElo[r];
+                u128 c = C & E & zmask;
                 if (c)
-                    c &= ~(SA[r - kTR] & SA[r + kTR] & SA[r - 1] & SA[r + 1] &
+                    c &= ~(row(r - kRY) & row(r + kRY) & row(r - 1) & row(r + 1) &
                            (C << 1) & (C >> 1));
                 cnd[t] = c;
                 rr[t] = r;
@@ -1113,7 +1141,7 @@ __global__ void __launch_bounds__(kTBlock)
                         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                             for (int dy = 0; dy < 3; ++dy)
-                                R[dx][dy] = SA[r + (dx - 1) * kTR + (dy - 1)];
+                                R[dx][dy] = row(r + (dx - 1) * kRY + (dy - 1));
                         code[t] = nb_code_w(R, (int)(ent[t] & 127u));
                     }
                 }
@@ -1126,14 +1154,17 @@ __global__ void __launch_bounds__(kTBlock)
                     if (ent[t] == 0xFFFFFFFFu || !((lw[t] >> (code[t] & 31)) & 1u))
                         continue;
                     const int r = (int)(ent[t] >> 7), i = (int)(ent[t] & 127u);
-                    atomicAnd(&SA64[2 * r + (i >> 6)], ~(1ull << (i & 63)));
-                    const int rx = r / kTR, ry = r % kTR;
-                    if (rx >= kTH && rx < kTH + kTT && ry >= kTH &&
-                        ry < kTH + kTT && i >= 32 && i < 96)
+                    if (i < 64)
+                        atomicAnd(&SAlo[r], ~(1ull << i));
+                    else
+                        atomicAnd(&SAhi[r], ~(1u << (i - 64)));
+                    const int rx = r / kRY, ry = r % kRY;
+                    if (rx >= kHX && rx < kHX + kTT && ry >= kHY &&
+                        ry < kHY + kTT && i >= kHZ && i < kHZ + 64)
                     {
                         ++nrem;
                         lrow0[((int64_t)(X0 + rx) * d.ny + (Y0 + ry)) * d.nz +
-                              (i - 32)] = (uint8_t)(2 + k);
+                              (i - kHZ)] = (uint8_t)(2 + k);
                     }
                 }
                 __syncthreads();
@@ -1142,11 +1173,11 @@ __global__ void __launch_bounds__(kTBlock)
         }
         for (int q = threadIdx.x; q < kTT * kTT; q += kTBlock)
         {
-            const int rx = kTH + q / kTT, ry = kTH + q % kTT;
+            const int rx = kHX + q / kTT, ry = kHY + q % kTT;
             const int x = X0 + rx, y = Y0 + ry;
             if (x < d.nx && y < d.ny)
                 Pout[((int64_t)x * d.ny + y) * nwz + w] =
-                    (uint64_t)(SA[rx * kTR + ry] >> 32);
+                    (uint64_t)(row(rx * kRY + ry) >> kHZ);
         }
         const uint32_t tot = block_sum(nrem, red);
         if (threadIdx.x == 0)
@@ -1163,7 +1194,7 @@ __global__ void __launch_bounds__(kTBlock)
             if (ntest)
                 atomicAdd(&c[0], (unsigned long long)ntest);
         }
-        __syncthreads(); // SA, SE, tq, red are rewritten by the next item
+        __syncthreads(); // the LDS rows, tq, red are rewritten by the next item
     }
 }
 
