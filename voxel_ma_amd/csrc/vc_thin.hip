@@ -874,8 +874,6 @@ constexpr int kTWaves = VC_TILE_WAVES;
 constexpr int kTQ = 1024;              // candidate queue entries per round
 // rows of one parity evaluated per pass: at most 23 x 19 (pass 0)
 constexpr int kTRowsPerThread = (23 * 19 + kTBlock - 1) / kTBlock;
-typedef unsigned __int128 u128;
-
 __device__ __forceinline__ int fy_of(int p) { return (7 - p) >> 1; }
 
 // window of word (o, w): bits 62..63 of word w-1, word w, bits 0..1 of w+1
@@ -890,9 +888,28 @@ __device__ __forceinline__ void load_win(const uint64_t* __restrict__ P,
     hi = (uint32_t)(c >> 62) | ((uint32_t)(n & 3ull) << 2);
 }
 
-// 26-neighbourhood code of window bit i (1 <= i <= 66), the order of nb_code
-__device__ __forceinline__ uint32_t nb_code_w(const u128 (&R)[3][3], int i)
+// a region row: window bits 0..63 and 64..67 (pad keeps rows 16-byte
+// aligned, so one row is one 128-bit LDS access)
+struct TRow
 {
+    uint64_t lo;
+    uint32_t hi, pad;
+};
+
+// bits j, j+1, j+2 of a row given as dwords d0 (bits 0..31), d1, d2
+__device__ __forceinline__ uint32_t bits3(uint32_t d0, uint32_t d1, uint32_t d2,
+                                          int sel, int sh)
+{
+    const uint32_t lo = sel == 0 ? d0 : (sel == 1 ? d1 : d2);
+    const uint32_t hi = sel == 0 ? d1 : (sel == 1 ? d2 : 0u);
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)sh) & 7u;
+}
+
+// 26-neighbourhood code of window bit i (1 <= i <= 66) from the 3 x 3 rows
+// around it (R[dx][dy]), the order of nb_code
+__device__ __forceinline__ uint32_t nb_code_rows(const TRow (&R)[3][3], int i)
+{
+    const int j = i - 1, sel = j >> 5, sh = j & 31;
     uint32_t code = 0;
     int pos = 0;
 #pragma unroll
@@ -900,7 +917,9 @@ __device__ __forceinline__ uint32_t nb_code_w(const u128 (&R)[3][3], int i)
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
         {
-            const uint32_t r3 = (uint32_t)(R[dx][dy] >> (i - 1)) & 7u;
+            const TRow& t = R[dx][dy];
+            const uint32_t r3 = bits3((uint32_t)t.lo, (uint32_t)(t.lo >> 32),
+                                      t.hi, sel, sh);
             if (dx == 1 && dy == 1)
             {
                 code |= ((r3 & 1u) | ((r3 >> 1) & 2u)) << pos;
@@ -913,19 +932,6 @@ __device__ __forceinline__ uint32_t nb_code_w(const u128 (&R)[3][3], int i)
             }
         }
     return code;
-}
-
-__device__ __forceinline__ int ctz128(u128 v)
-{
-    const uint64_t lo = (uint64_t)v;
-    return lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll((uint64_t)(v >> 64));
-}
-
-// bits [a, b) of a 128-bit window
-__device__ __forceinline__ u128 bit_range(int a, int b)
-{
-    const u128 one = 1;
-    return ((one << b) - 1) & ~((one << a) - 1);
 }
 
 // entry: alive2 = alive on every tile, change flags (both parities) = the
@@ -1008,8 +1014,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
     const PassCtl pc = pass_ctl(ctl, j, k_arg, 0, nvol, gany, vany);
     const uint32_t sweep = pc.sweep;
     const int k = pc.k;
-    __shared__ unsigned long long SAlo[kTRows]; // alive, window bits 0..63
-    __shared__ uint32_t SAhi[kTRows];           // alive, window bits 64..67
+    __shared__ TRow SA[kTRows];                 // alive
     __shared__ unsigned long long SElo[kTRows]; // eligible (constant in a level)
     __shared__ uint8_t SEhi[kTRows];
     __shared__ uint32_t tq[kTQ];
@@ -1059,15 +1064,11 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                 load_win(Pin, o, w, nwz, alo, ahi);
                 load_win(d.elig, o, w, nwz, elo, ehi);
             }
-            SAlo[r] = alo;
-            SAhi[r] = ahi;
+            SA[r] = TRow{alo, ahi, 0u};
             SElo[r] = elo;
             SEhi[r] = (uint8_t)ehi;
         }
         __syncthreads();
-        auto row = [&](int r) -> u128 {
-            return ((u128)SAhi[r] << 64) | (u128)SAlo[r];
-        };
         uint32_t nrem = 0, ntest = 0;
         uint8_t* lrow0 = d.label + (int64_t)64 * w;
         for (int p = 0; p < 8; ++p)
@@ -1081,33 +1082,47 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             const int ylo = kHY - Fy, yhi = kHY + kTT + Fy;
             const int rx0 = xlo + ((xlo & 1) != sx), ry0 = ylo + ((ylo & 1) != sy);
             const int nrx = (xhi - rx0 + 1) >> 1, nry = (yhi - ry0 + 1) >> 1;
-            // window bit parity = z parity (64w - 2 is even)
+            // window bit parity = z parity (64w - 2 is even); bits
+            // [kHZ - Fz, kHZ + 64 + Fz): low word from bit 1 or 2 up, high
+            // bits 64..65 or 64..66
             const uint64_t zs64 = sz ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
-            const u128 zmask = (((u128)zs64 << 64) | (u128)zs64) &
-                               bit_range(kHZ - Fz, kHZ + 64 + Fz);
-            u128 cnd[kTRowsPerThread];
+            const uint64_t zlo = zs64 & (~0ull << (kHZ - Fz));
+            const uint32_t zhi = (uint32_t)zs64 & ((1u << (kHZ + Fz)) - 1u);
+            uint64_t clo[kTRowsPerThread];
+            uint32_t chi[kTRowsPerThread];
             int rr[kTRowsPerThread];
             uint32_t cnt = 0;
 #pragma unroll
             for (int t = 0; t < kTRowsPerThread; ++t)
             {
-                cnd[t] = 0;
+                clo[t] = 0;
+                chi[t] = 0;
                 rr[t] = 0;
                 const int q = threadIdx.x + t * kTBlock;
                 if (q >= nrx * nry)
                     continue;
                 const int rx = rx0 + 2 * (q / nry), ry = ry0 + 2 * (q % nry);
                 const int r = rx * kRY + ry;
-                const u128 C = row(r);
-                const u128 E = ((u128)SEhi[r] << 64) | (u128)SElo[r];
-                u128 c = C & E & zmask;
-                if (c)
-                    c &= ~(row(r - kRY) & row(r + kRY) & row(r - 1) & row(r + 1) &
-                           (C << 1) & (C >> 1));
-                cnd[t] = c;
+                const TRow C = SA[r];
+                uint64_t cl = C.lo & SElo[r] & zlo;
+                uint32_t ch = C.hi & (uint32_t)SEhi[r] & zhi;
+                if (cl | ch)
+                {
+                    // border filter: all six face neighbours alive -> never
+                    // simple
+                    const TRow a = SA[r - kRY], bb = SA[r + kRY], c = SA[r - 1],
+                               e = SA[r + 1];
+                    const uint64_t zdn = (C.lo << 1);
+                    const uint64_t zup = (C.lo >> 1) | ((uint64_t)C.hi << 63);
+                    cl &= ~(a.lo & bb.lo & c.lo & e.lo & zdn & zup);
+                    const uint32_t hdn = (C.hi << 1) | (uint32_t)(C.lo >> 63);
+                    const uint32_t hup = C.hi >> 1;
+                    ch &= ~(a.hi & bb.hi & c.hi & e.hi & hdn & hup);
+                }
+                clo[t] = cl;
+                chi[t] = ch;
                 rr[t] = r;
-                cnt += (uint32_t)(__popcll((uint64_t)c) +
-                                  __popcll((uint64_t)(c >> 64)));
+                cnt += (uint32_t)(__popcll(cl) + __popc(ch));
             }
             uint32_t total;
             uint32_t idx = tile_scan(cnt, red, &total);
@@ -1116,13 +1131,23 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             {
 #pragma unroll
                 for (int t = 0; t < kTRowsPerThread; ++t)
-                    while (cnd[t] && idx < c0 + kTQ)
+                {
+                    const uint32_t base = (uint32_t)rr[t] << 7;
+                    while (clo[t] && idx < c0 + kTQ)
                     {
-                        const int i = ctz128(cnd[t]);
-                        cnd[t] &= cnd[t] - 1;
-                        tq[idx - c0] = ((uint32_t)rr[t] << 7) | (uint32_t)i;
+                        const int i = __ffsll((unsigned long long)clo[t]) - 1;
+                        clo[t] &= clo[t] - 1;
+                        tq[idx - c0] = base | (uint32_t)i;
                         ++idx;
                     }
+                    while (chi[t] && idx < c0 + kTQ)
+                    {
+                        const int i = __ffs(chi[t]) - 1;
+                        chi[t] &= chi[t] - 1;
+                        tq[idx - c0] = base | (uint32_t)(64 + i);
+                        ++idx;
+                    }
+                }
                 __syncthreads();
                 const uint32_t n = min((uint32_t)kTQ, total - c0);
                 uint32_t ent[kTQ / kTBlock], code[kTQ / kTBlock],
@@ -1136,13 +1161,13 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                     if (e < n)
                     {
                         const int r = (int)(ent[t] >> 7);
-                        u128 R[3][3];
+                        TRow R[3][3];
 #pragma unroll
                         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                             for (int dy = 0; dy < 3; ++dy)
-                                R[dx][dy] = row(r + (dx - 1) * kRY + (dy - 1));
-                        code[t] = nb_code_w(R, (int)(ent[t] & 127u));
+                                R[dx][dy] = SA[r + (dx - 1) * kRY + (dy - 1)];
+                        code[t] = nb_code_rows(R, (int)(ent[t] & 127u));
                     }
                 }
 #pragma unroll
@@ -1155,9 +1180,10 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                         continue;
                     const int r = (int)(ent[t] >> 7), i = (int)(ent[t] & 127u);
                     if (i < 64)
-                        atomicAnd(&SAlo[r], ~(1ull << i));
+                        atomicAnd(reinterpret_cast<unsigned long long*>(&SA[r].lo),
+                                  ~(1ull << i));
                     else
-                        atomicAnd(&SAhi[r], ~(1u << (i - 64)));
+                        atomicAnd(&SA[r].hi, ~(1u << (i - 64)));
                     const int rx = r / kRY, ry = r % kRY;
                     if (rx >= kHX && rx < kHX + kTT && ry >= kHY &&
                         ry < kHY + kTT && i >= kHZ && i < kHZ + 64)
@@ -1176,8 +1202,11 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             const int rx = kHX + q / kTT, ry = kHY + q % kTT;
             const int x = X0 + rx, y = Y0 + ry;
             if (x < d.nx && y < d.ny)
+            {
+                const TRow t = SA[rx * kRY + ry];
                 Pout[((int64_t)x * d.ny + y) * nwz + w] =
-                    (uint64_t)(row(rx * kRY + ry) >> kHZ);
+                    (t.lo >> kHZ) | ((uint64_t)t.hi << (64 - kHZ));
+            }
         }
         const uint32_t tot = block_sum(nrem, red);
         if (threadIdx.x == 0)
