@@ -175,40 +175,27 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                         by = y;
                     }
                 }
-                for (int dd = 1;; ++dd)
+                // branch-free search body (clamped loads, selects): lower
+                // index first, so ties keep the lowest y
+                const int ymax = max(y, ny - 1 - y);
+                for (int dd = 1; dd <= ymax; ++dd)
                 {
                     const int d2 = dd * dd;
                     if (d2 > best)
                         break;
-                    const bool lo = y - dd >= 0, hi = y + dd < ny;
-                    if (!lo && !hi)
-                        break;
-                    if (lo)
-                    {
-                        const int q = colz[(y - dd) * TZ + tz];
-                        if (q >= 0)
-                        {
-                            const int v = (z - q) * (z - q) + d2;
-                            if (v <= best)
-                            {
-                                best = v;
-                                by = y - dd;
-                            }
-                        }
-                    }
-                    if (hi)
-                    {
-                        const int q = colz[(y + dd) * TZ + tz];
-                        if (q >= 0)
-                        {
-                            const int v = (z - q) * (z - q) + d2;
-                            if (v < best)
-                            {
-                                best = v;
-                                by = y + dd;
-                            }
-                        }
-                    }
+                    const int yl = y - dd, yh = y + dd;
+                    const int ql = colz[max(yl, 0) * TZ + tz];
+                    const int qh = colz[min(yh, ny - 1) * TZ + tz];
+                    const int vl =
+                        (yl >= 0 && ql >= 0) ? (z - ql) * (z - ql) + d2 : kInf;
+                    const int vh =
+                        (yh < ny && qh >= 0) ? (z - qh) * (z - qh) + d2 : kInf;
+                    const bool tl = vl <= best && vl < kInf;
+                    best = tl ? vl : best;
+                    by = tl ? yl : by;
+                    const bool th = vh < best;
+                    best = th ? vh : best;
+                    by = th ? yh : by;
                 }
                 uint32_t outv = VC_NONE;
                 if (by >= 0)
@@ -275,10 +262,9 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                 int bxv = -1;
                 auto cost = [&](int xx) -> int {
                     const uint32_t v = colyz[xx * TZ + tz];
-                    if (v == VC_NONE)
-                        return kInf;
                     const int yy = (int)(v >> 16), zz = (int)(v & 0xFFFFu);
-                    return (y - yy) * (y - yy) + (z - zz) * (z - zz);
+                    const int c = (y - yy) * (y - yy) + (z - zz) * (z - zz);
+                    return v == VC_NONE ? kInf : c;
                 };
                 {
                     const int c = cost(x);
@@ -288,32 +274,24 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                         bxv = x;
                     }
                 }
-                for (int dd = 1;; ++dd)
+                // branch-free search body (clamped loads, selects): lower
+                // index first, so ties keep the lowest x
+                const int xmax = max(x, nx - 1 - x);
+                for (int dd = 1; dd <= xmax; ++dd)
                 {
                     const int d2 = dd * dd;
                     if (d2 > best)
                         break;
-                    const bool lo = x - dd >= 0, hi = x + dd < nx;
-                    if (!lo && !hi)
-                        break;
-                    if (lo)
-                    {
-                        const int c = cost(x - dd);
-                        if (c < kInf && c + d2 <= best)
-                        {
-                            best = c + d2;
-                            bxv = x - dd;
-                        }
-                    }
-                    if (hi)
-                    {
-                        const int c = cost(x + dd);
-                        if (c < kInf && c + d2 < best)
-                        {
-                            best = c + d2;
-                            bxv = x + dd;
-                        }
-                    }
+                    const int xl = x - dd, xh = x + dd;
+                    const int cl = cost(max(xl, 0)), ch = cost(min(xh, nx - 1));
+                    const int vl = (xl >= 0 && cl < kInf) ? cl + d2 : kInf;
+                    const int vh = (xh < nx && ch < kInf) ? ch + d2 : kInf;
+                    const bool tl = vl <= best && vl < kInf;
+                    best = tl ? vl : best;
+                    bxv = tl ? xl : bxv;
+                    const bool th = vh < best;
+                    best = th ? vh : best;
+                    bxv = th ? xh : bxv;
                 }
                 const int64_t o = (int64_t)x * sx + off;
                 if (bxv < 0)
