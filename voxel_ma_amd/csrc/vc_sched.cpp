@@ -57,7 +57,7 @@ int64_t items_of(const vc_vdesc& d, int kind)
 
 bool xcd_kind(int kind)
 {
-    return kind == VK_EY || kind == VK_EX || kind == VK_MS;
+    return kind == VK_EY || kind == VK_EX || kind == VK_MS || kind == VK_TT;
 }
 
 void drop_graphs(vc_sched* s);

@@ -1019,8 +1019,14 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
     __shared__ uint8_t SEhi[kTRows];
     __shared__ uint32_t tq[kTQ];
     __shared__ uint32_t red[kTBlock / 64];
-    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    // XCD-mapped: the tiles of one row column (consecutive items) and their
+    // neighbours share rows, so they go to the same XCD's L2
+    const uint32_t per = (W.total + 7) / 8;
+    for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
     {
+        const uint32_t b = vc_xcd_item(L, per);
+        if (b >= W.total)
+            continue;
         int vi;
         uint32_t lt;
         vc_item<kB>(W, b, vi, lt);
