@@ -99,9 +99,21 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
         if el > target_s or reps >= 20:
             break
     mvox = crop.size * reps / el / 1e6
+    # the boundary row (sites, quads, Euler terms) on the same crop
+    t0 = time.perf_counter()
+    breps = 0
+    while True:
+        O.boundary(crop)
+        breps += 1
+        bel = time.perf_counter() - t0
+        if bel > 3.0 or breps >= 20:
+            break
     return {"value": round(mvox, 3), "unit": "Mvoxels/s", "cores": 1, "kind": "port",
             "sample": "oracle (single-thread C) EDT+measure+thinning on the central %d^3 crop "
-                      "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el)}
+                      "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el),
+            "rows": {"R3_R4_boundary": {"value": round(crop.size * breps / bel / 1e6, 3),
+                                        "unit": "Mvoxels/s", "sample": "oracle boundary + "
+                                        "Euler on the same crop, %d reps" % breps}}}
 
 
 def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
@@ -171,6 +183,164 @@ def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
     b.release()
     for v in vols:
         v.release()
+    return out
+
+
+def _grid_complex(m):
+    """An m x m grid of unit quads in the z = 0 plane (vertices (i, j, 0)),
+    each quad a face whose two sites sit at (i+.5, j+.5, +-h) with h varying
+    smoothly, so lambda = 2h varies over the complex."""
+    import numpy as np
+    i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    v = lambda a, b: (a * (m + 1) + b).ravel()
+    faces = np.stack([v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)], 1).astype(np.int32)
+    gi, gj = np.meshgrid(np.arange(m + 1), np.arange(m + 1), indexing="ij")
+    vts = np.stack([gi.ravel(), gj.ravel(), np.zeros(gi.size)], 1).astype(np.float32)
+    h = (1.0 + 0.5 * np.sin(i / 7.0) * np.cos(j / 5.0)).ravel().astype(np.float32)
+    c = np.stack([i.ravel() + 0.5, j.ravel() + 0.5], 1).astype(np.float32)
+    sites = np.concatenate([np.column_stack([c, h]), np.column_stack([c, -h])]).astype(np.float32)
+    nf = m * m
+    face_sites = np.stack([np.arange(nf), np.arange(nf) + nf], 1).astype(np.int32)
+    return vts, faces, sites, face_sites
+
+
+def _csr(n_rows, rows, cols):
+    """CSR of (row, col) pairs, cols in ascending order within a row."""
+    import numpy as np
+    order = np.lexsort((cols, rows))
+    off = np.zeros(n_rows + 1, np.int64)
+    np.add.at(off, rows + 1, 1)
+    return np.cumsum(off), np.ascontiguousarray(cols[order], np.int32)
+
+
+def _random_octree(R, seed):
+    """A .sof stream of resolution R: internal nodes split with probability
+    falling with depth, leaves carry random corner signs, empty nodes a
+    random sign (pre-order, children (i, j, k) with k fastest)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = bytearray(np.int32(R).tobytes())
+
+    def node(ln, depth):
+        if ln > 1 and rng.random() < (0.95 if depth < 3 else 0.45):
+            out.append(0)
+            for _ in range(8):
+                node(ln // 2, depth + 1)
+        elif ln == 1 or rng.random() < 0.5:
+            out.append(2)
+            out.append(int(rng.integers(1, 255)))
+        else:
+            out.append(1)
+            out.append(int(rng.integers(0, 2)))
+    node(R, 0)
+    return bytes(out)
+
+
+def rows_leg(ctx, occ_host, label_host, reps=3):
+    """The other rows of the path, each on its own input, timed on the
+    device stream (wall clock around the C-ABI call; the Voronoi-side calls
+    take host arrays, so their figures include the PCIe copies)."""
+    import numpy as np
+    import voxel_ma_amd as V
+    out = {}
+
+    def best_of(fn, k=reps):
+        ts = []
+        for _ in range(k):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), r
+
+    # R3/R4: boundary sites, quads and cube-complex Euler terms (computed once
+    # per volume: a fresh upload per repetition, the size query computes all)
+    n = occ_host.shape[0]
+    vols = [ctx.upload(occ_host) for _ in range(reps)]
+    it = iter(vols)
+
+    def bnd():
+        v = next(it)
+        L = ctx._lib
+        ns, nq = ctypes.c_int64(), ctypes.c_int64()
+        L.vc_boundary_extract(ctx.handle, v.handle, None, 0, ctypes.byref(ns), None, 0,
+                              ctypes.byref(nq), None)
+        return ns.value, nq.value
+    t_b, (ns, nq) = best_of(bnd)
+    eu = vols[0].boundary(with_euler=True)[2]
+    for v in vols:
+        v.release()
+    N = occ_host.size
+    algo_b = N * 1 + ns * 12 + nq * 16
+    out["R3_R4_boundary"] = {
+        "workload": "boundary sites + quads + Euler terms of the %d^3 headline volume" % n,
+        "ms": round(t_b * 1e3, 3), "sites": ns, "quads": nq, "euler": eu["euler"],
+        "chi": eu["chi"], "value": round(N / t_b / 1e6, 1), "unit": "Mvoxels/s",
+        "roofline": {"bound": "hbm", "achieved": round(algo_b / t_b / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(algo_b / t_b / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes": "1 B/voxel + 12 B/site + 16 B/quad"}}
+
+    # R2: dense occupancy from a .sof octree (R = 256)
+    stream = _random_octree(256, 9)
+    tree = V.Octree(stream)
+    t_o, vo = best_of(lambda: tree.volume(ctx))
+    vo.release()
+    n3 = 257 ** 3
+    out["R2_octree"] = {"workload": "random .sof octree, R = 256 (%d nodes), to a 257^3 volume"
+                        % (tree.info.n_internal + tree.info.n_empty + tree.info.n_leaf),
+                        "ms": round(t_o * 1e3, 3), "value": round(n3 / t_o / 1e6, 1),
+                        "unit": "Mvoxels/s"}
+    tree.free()
+
+    # R6/R7: tagging and lambda measures; R8/R9 extraction and merging;
+    # R10-R12 finalize + prune, on a 500 x 500 grid complex
+    m = 500
+    vts, faces, sites, fs = _grid_complex(m)
+    vol = ctx.upload(occ_host)
+    pts = (np.random.default_rng(5).random((8 << 20, 3)) * n).astype(np.float32)
+    t_t, _ = best_of(lambda: V.voro_tag_points(vol, pts))
+    vol.release()
+    out["R6_tag"] = {"workload": "%d random points in the headline volume" % len(pts),
+                     "ms": round(t_t * 1e3, 3), "value": round(len(pts) / t_t / 1e6, 1),
+                     "unit": "Mpoints/s", "note": "host arrays, PCIe copies included"}
+    edges, foff, fe = V.cc_finalize(len(vts), [], [list(f) for f in faces])
+    ne, nf = len(edges), len(faces)
+    fe_rows = np.repeat(np.arange(nf), np.diff(foff))
+    ef_off, ef = _csr(ne, fe.astype(np.int64), fe_rows.astype(np.int32))
+    ve_rows = edges.ravel().astype(np.int64)
+    ve_cols = np.repeat(np.arange(ne, dtype=np.int32), 2)
+    ve_off, ve = _csr(len(vts), ve_rows, ve_cols)
+    fvalid = np.ones(nf, np.uint8)
+    t_m, (lv, le, lf) = best_of(lambda: V.voro_measures(ctx, len(vts), edges, fs, sites, fvalid,
+                                                         ve_off, ve, ef_off, ef))
+    out["R7_measures"] = {"workload": "lambda of %d faces, %d edges, %d vertices" % (nf, ne, len(vts)),
+                          "ms": round(t_m * 1e3, 3), "value": round(nf / t_m / 1e6, 2),
+                          "unit": "Mfaces/s", "note": "host arrays, PCIe copies included"}
+    valid = np.ones(len(vts), np.uint8)
+    fl = [list(f) for f in faces]
+    t_x, _ = best_of(lambda: V.voro_extract_inside(len(vts), edges, fl, valid, le, lf), 1)
+    t_g, _ = best_of(lambda: V.voro_merge_close(vts, edges, fl, valid, 0.25, True), 1)
+    out["R8_R9_host"] = {"workload": "extract inside + merge close vertices on %d faces" % nf,
+                         "extract_ms": round(t_x * 1e3, 1), "merge_ms": round(t_g * 1e3, 1)}
+    t_f, _ = best_of(lambda: V.cc_finalize(len(vts), [], fl), 1)
+    t_p, _ = best_of(lambda: V.cc_prune(len(vts), edges, foff, fe, le, lf, 1.5, 1.5), 1)
+    out["R10_R12_cellcomplex"] = {"workload": "finalize + FIFO prune of %d quads" % nf,
+                                  "finalize_ms": round(t_f * 1e3, 1),
+                                  "prune_ms": round(t_p * 1e3, 1)}
+
+    # output files: the skeleton of the headline volume as PLY and .r
+    import tempfile
+    sk = np.argwhere(label_host == 1).astype(np.float32)
+    r = np.ones(len(sk), np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        t_w, _ = best_of(lambda: V.write_ply(os.path.join(td, "s.ply"), sk), 1)
+        sz = os.path.getsize(os.path.join(td, "s.ply"))
+        t_r, _ = best_of(lambda: V.write_radii(os.path.join(td, "s.r"), sk, r), 1)
+    out["writers"] = {"workload": "PLY + .r of the %d skeleton voxels" % len(sk),
+                      "ply_ms": round(t_w * 1e3, 1), "ply_MB": round(sz / 1e6, 1),
+                      "radii_ms": round(t_r * 1e3, 1)}
     return out
 
 
@@ -261,6 +431,9 @@ def main():
                     help="only the batched-solids leg (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate-guided pruning leg")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the per-row measurements (boundary, octree, Voronoi side, "
+                         "cell complex, writers)")
     ap.add_argument("--profile-steps", action="store_true",
                     help="no CPU baseline, no hashing (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -390,7 +563,11 @@ def main():
         print("[bench] surrogate-guided pruning", file=sys.stderr, flush=True)
         out["surrogate"] = surrogate_leg(ctx, ev, vol, levels, n, max(1, a.steps // 2),
                                          ms_step)
+    label_host = vol.get(voxel_ma_amd.BUF_LABEL)
     vol.release()
+    if not (a.no_rows or a.profile_steps):
+        print("[bench] other rows", file=sys.stderr, flush=True)
+        out["rows"] = rows_leg(ctx, occ_host, label_host)
     if not (a.no_batch or a.profile_steps):
         out["batch"] = batch_leg(ctx, ev, a.steps, a.warmup)
     if not (a.no_cpu_baseline or a.profile_steps):

@@ -82,3 +82,13 @@ def test_boundary_capacity_error(vc):
     vol.release()
     assert rc == voxel_ma_amd.VC_ERR_CAPACITY
     assert ns.value == 26 and nq.value == 24
+
+
+def test_boundary_vectorised_rows_match_oracle(vc, oracle):
+    """Volumes whose z extent is a multiple of 8 take the 8-voxels-per-thread
+    passes: odd x / y extents, objects touching every face of the grid."""
+    rng = np.random.default_rng(23)
+    for it in range(25):
+        shape = (int(rng.integers(1, 20)), int(rng.integers(1, 20)), 8 * int(rng.integers(1, 6)))
+        occ = (rng.random(shape) < rng.choice([0.2, 0.5, 0.8, 0.97])).astype(np.uint8)
+        _compare(vc, oracle, occ, "vec%d" % it)

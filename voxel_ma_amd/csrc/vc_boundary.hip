@@ -194,6 +194,22 @@ __device__ __forceinline__ Vox gather(const uint8_t* __restrict__ occ,
     return V;
 }
 
+// whether voxel (x, y, z) differs from one of its six face neighbours
+// (outside the grid reads 0): voxels without such an event emit nothing, so
+// the full 27-neighbour gather is skipped for them
+__device__ __forceinline__ bool face_event(const uint8_t* __restrict__ occ,
+                                           const Dims& D, int x, int y, int z)
+{
+    const int64_t sy = D.nz, sx = (int64_t)D.ny * D.nz;
+    const int64_t i = x * sx + y * sy + z;
+    const bool me = occ[i] != 0;
+    const bool xm = x > 0 && occ[i - sx] != 0, xp = x + 1 < D.nx && occ[i + sx] != 0;
+    const bool ym = y > 0 && occ[i - sy] != 0, yp = y + 1 < D.ny && occ[i + sy] != 0;
+    const bool zm = z > 0 && occ[i - 1] != 0, zp = z + 1 < D.nz && occ[i + 1] != 0;
+    return (xm != me) | (xp != me) | (ym != me) | (yp != me) | (zm != me) |
+           (zp != me);
+}
+
 __device__ __forceinline__ void decode(const Dims& D, int64_t i, int& x,
                                        int& y, int& z)
 {
@@ -252,9 +268,12 @@ __global__ void __launch_bounds__(kBlock)
     {
         int x, y, z;
         decode(D, i, x, y, z);
-        const Emit e = decide(gather(occ, D, x, y, z));
-        ns = (uint32_t)e.nsite;
-        nf = (uint32_t)e.nface;
+        if (face_event(occ, D, x, y, z))
+        {
+            const Emit e = decide(gather(occ, D, x, y, z));
+            ns = (uint32_t)e.nsite;
+            nf = (uint32_t)e.nface;
+        }
     }
     uint32_t tot;
     block_scan_excl((ns << 16) | nf, sh, &tot);
@@ -263,6 +282,199 @@ __global__ void __launch_bounds__(kBlock)
         tile_sites[blockIdx.x] = tot >> 16;
         tile_faces[blockIdx.x] = tot & 0xFFFFu;
     }
+}
+
+// ---- vectorised passes (nz a multiple of 8): one thread per 8 consecutive
+// z of a row, in linear order.  The 3 x 3 neighbouring rows are read as one
+// 8-byte load each plus the two edge bytes, packed into 10-bit windows (bit
+// b = occupancy of z0 - 1 + b, outside the grid = 0).
+constexpr int kVz = 8;
+
+__device__ __forceinline__ uint32_t row_win(const uint8_t* __restrict__ occ,
+                                            const Dims& D, int x, int y, int z0)
+{
+    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny)
+        return 0;
+    const uint8_t* r = occ + ((int64_t)x * D.ny + y) * D.nz;
+    const uint64_t mid = *reinterpret_cast<const uint64_t*>(r + z0);
+    // one bit per non-zero byte, gathered into the top byte
+    uint64_t t = mid | (mid >> 4);
+    t |= t >> 2;
+    t |= t >> 1;
+    t &= 0x0101010101010101ull;
+    const uint32_t m8 = (uint32_t)((t * 0x0102040810204080ull) >> 56);
+    const uint32_t lo = z0 > 0 && r[z0 - 1] != 0;
+    const uint32_t hi = z0 + kVz < D.nz && r[z0 + kVz] != 0;
+    return lo | (m8 << 1) | (hi << 9);
+}
+
+struct Chunk
+{
+    uint32_t w[3][3]; // windows of rows (x + dx - 1, y + dy - 1)
+    int x, y, z0;
+};
+
+__device__ __forceinline__ bool load_chunk(const uint8_t* __restrict__ occ,
+                                           const Dims& D, int64_t c, Chunk& K)
+{
+    const int nzc = D.nz / kVz;
+    if (c >= (int64_t)D.nx * D.ny * nzc)
+        return false;
+    const int64_t row = c / nzc;
+    K.z0 = (int)(c - row * nzc) * kVz;
+    K.x = (int)(row / D.ny);
+    K.y = (int)(row - (int64_t)K.x * D.ny);
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+            K.w[dx][dy] = row_win(occ, D, K.x + dx - 1, K.y + dy - 1, K.z0);
+    return true;
+}
+
+// the 3 x 3 x 3 neighbourhood of voxel z0 + j of the chunk; null if the voxel
+// has no face event (then it emits nothing)
+__device__ __forceinline__ bool chunk_vox(const Chunk& K, const Dims& D, int j,
+                                          Vox& V)
+{
+    const uint32_t c = K.w[1][1] >> j;
+    const uint32_t me = (c >> 1) & 1u;
+    const uint32_t f = (c & 1u) | ((c >> 2) & 1u) << 1 |
+                       ((K.w[0][1] >> (j + 1)) & 1u) << 2 |
+                       ((K.w[2][1] >> (j + 1)) & 1u) << 3 |
+                       ((K.w[1][0] >> (j + 1)) & 1u) << 4 |
+                       ((K.w[1][2] >> (j + 1)) & 1u) << 5;
+    if (f == (me ? 0x3Fu : 0u))
+        return false;
+    const int z = K.z0 + j;
+    uint32_t zin = 7u;
+    if (z == 0)
+        zin &= ~1u;
+    if (z + 1 >= D.nz)
+        zin &= ~4u;
+    V.occ = 0;
+    V.ing = 0;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+        {
+            const int b = nbit(dx - 1, dy - 1, -1);
+            V.occ |= ((K.w[dx][dy] >> j) & 7u) << b;
+            const bool rin = (unsigned)(K.x + dx - 1) < (unsigned)D.nx &&
+                             (unsigned)(K.y + dy - 1) < (unsigned)D.ny;
+            V.ing |= (rin ? zin : 0u) << b;
+        }
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bnd_count_v(const uint8_t* __restrict__ occ, Dims D,
+                uint32_t* __restrict__ tile_sites,
+                uint32_t* __restrict__ tile_faces)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ns = 0, nf = 0;
+    Chunk K;
+    if (load_chunk(occ, D, c, K))
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+        {
+            Vox V;
+            if (chunk_vox(K, D, j, V))
+            {
+                const Emit e = decide(V);
+                ns += (uint32_t)e.nsite;
+                nf += (uint32_t)e.nface;
+            }
+        }
+    uint32_t tot;
+    block_scan_excl((ns << 16) | nf, sh, &tot);
+    if (threadIdx.x == 0)
+    {
+        tile_sites[blockIdx.x] = tot >> 16;
+        tile_faces[blockIdx.x] = tot & 0xFFFFu;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bnd_emit_sites_v(const uint8_t* __restrict__ occ, Dims D,
+                     const uint32_t* __restrict__ tile_sites,
+                     float* __restrict__ sites, uint32_t* __restrict__ cmap)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Chunk K;
+    const bool ok = load_chunk(occ, D, c, K);
+    uint32_t ns = 0;
+    if (ok)
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+        {
+            Vox V;
+            if (chunk_vox(K, D, j, V))
+                ns += (uint32_t)decide(V).nsite;
+        }
+    uint32_t tot;
+    uint32_t s = tile_sites[blockIdx.x] + block_scan_excl(ns, sh, &tot);
+    if (ok && ns)
+        for (int j = 0; j < kVz; ++j)
+        {
+            Vox V;
+            if (!chunk_vox(K, D, j, V))
+                continue;
+            const Emit e = decide(V);
+            const int z = K.z0 + j;
+            for (int q = 0; q < e.nsite; ++q, ++s)
+            {
+                const int cc = e.corner[q];
+                sites[3 * (int64_t)s + 0] = (float)K.x + 0.5f * kSgn[cc][0];
+                sites[3 * (int64_t)s + 1] = (float)K.y + 0.5f * kSgn[cc][1];
+                sites[3 * (int64_t)s + 2] = (float)z + 0.5f * kSgn[cc][2];
+                cmap[corner_id(D, K.x, K.y, z, cc)] = s;
+            }
+        }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bnd_emit_quads_v(const uint8_t* __restrict__ occ, Dims D,
+                     const uint32_t* __restrict__ tile_faces,
+                     const uint32_t* __restrict__ cmap,
+                     int32_t* __restrict__ quads)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Chunk K;
+    const bool ok = load_chunk(occ, D, c, K);
+    uint32_t nf = 0;
+    if (ok)
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+        {
+            Vox V;
+            if (chunk_vox(K, D, j, V))
+                nf += (uint32_t)decide(V).nface;
+        }
+    uint32_t tot;
+    uint32_t f = tile_faces[blockIdx.x] + block_scan_excl(nf, sh, &tot);
+    if (ok && nf)
+        for (int j = 0; j < kVz; ++j)
+        {
+            Vox V;
+            if (!chunk_vox(K, D, j, V))
+                continue;
+            const Emit e = decide(V);
+            const int z = K.z0 + j;
+            for (int q = 0; q < e.nface; ++q, ++f)
+            {
+                const int fi = kFwrt[e.face_o[q]];
+                int32_t* qd = quads + 4 * (int64_t)f;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    qd[t] = (int32_t)cmap[corner_id(D, K.x, K.y, z, kFcrn[fi][t])];
+            }
+        }
 }
 
 // exclusive scan of a uint32 array in place, one block of 1024 per chunk;
@@ -320,7 +532,8 @@ __global__ void __launch_bounds__(kBlock)
     if (i < N)
     {
         decode(D, i, x, y, z);
-        e = decide(gather(occ, D, x, y, z));
+        if (face_event(occ, D, x, y, z))
+            e = decide(gather(occ, D, x, y, z));
     }
     uint32_t tot;
     const uint32_t base =
@@ -350,7 +563,8 @@ __global__ void __launch_bounds__(kBlock)
     if (i < N)
     {
         decode(D, i, x, y, z);
-        e = decide(gather(occ, D, x, y, z));
+        if (face_event(occ, D, x, y, z))
+            e = decide(gather(occ, D, x, y, z));
     }
     uint32_t tot;
     const uint32_t base =
@@ -457,7 +671,14 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
                     blk += occ_at(occ, D, cx - 1 + a, cy - 1 + b, cz - 1 + c);
-        if (blk != 0 && blk != 8)
+        if (blk == 0 || blk == 8)
+        {
+            // uniform block: no site, no boundary edge; every edge from this
+            // point inside the grid is interior iff the block is full
+            if (blk == 8)
+                ei = (cx < D.nx) + (cy < D.ny) + (cz < D.nz);
+        }
+        else
         {
             // site: mask around the (int)-truncated base voxel
             // (include/spaceinfo.h:140-153), bit (7 - i) for offset i
@@ -470,7 +691,6 @@ __global__ void __launch_bounds__(kBlock)
                                bz + (i & 1))
                         << (7 - i);
             vb = (unsigned long long)tbl[mask];
-        }
         // edges (+x, +y, +z) from this lattice point: the 4 voxels around
         // the edge, in cyclic order; boundary faces = value changes
 #pragma unroll
@@ -500,6 +720,7 @@ __global__ void __launch_bounds__(kBlock)
             if (v4[0] + v4[1] + v4[2] + v4[3] == 4)
                 ei += 1;
         }
+        }
     }
     const int slot = (blockIdx.x + blockIdx.y * 3 + blockIdx.z * 7) % kSlots;
     unsigned long long* a = acc + slot * kAcc;
@@ -520,22 +741,24 @@ __global__ void __launch_bounds__(kBlock)
     if (z < D.nz && occ_at(occ, D, x, y, z))
     {
         t = 1;
-        fi = occ_at(occ, D, x - 1, y, z) + occ_at(occ, D, x + 1, y, z) +
-             occ_at(occ, D, x, y - 1, z) + occ_at(occ, D, x, y + 1, z) +
-             occ_at(occ, D, x, y, z - 1) + occ_at(occ, D, x, y, z + 1);
-        // vox_corner_offset order; base voxel = (int)(coord +- 0.5)
+        // one gather of the 3 x 3 x 3 neighbourhood (outside the grid = 0)
+        const uint32_t m = gather(occ, D, x, y, z).occ;
+        fi = obit(m, -1, 0, 0) + obit(m, 1, 0, 0) + obit(m, 0, -1, 0) +
+             obit(m, 0, 1, 0) + obit(m, 0, 0, -1) + obit(m, 0, 0, 1);
+        // vox_corner_offset order; base voxel = (int)(coord +- 0.5): the
+        // block starts at offset -1, or 0 on the high side or at coordinate 0
 #pragma unroll
         for (int c = 0; c < 8; ++c)
         {
-            const int bx = ((c >> 2) & 1) ? x : (x > 0 ? x - 1 : 0);
-            const int by = ((c >> 1) & 1) ? y : (y > 0 ? y - 1 : 0);
-            const int bz = (c & 1) ? z : (z > 0 ? z - 1 : 0);
-            int n = 0;
+            const int ax = (((c >> 2) & 1) || x == 0) ? 0 : -1;
+            const int ay = (((c >> 1) & 1) || y == 0) ? 0 : -1;
+            const int az = ((c & 1) || z == 0) ? 0 : -1;
+            uint32_t bm = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                n += occ_at(occ, D, bx + ((i >> 2) & 1), by + ((i >> 1) & 1),
-                            bz + (i & 1));
-            vi += n == 8;
+                bm |= 1u << nbit(ax + ((i >> 2) & 1), ay + ((i >> 1) & 1),
+                                 az + (i & 1));
+            vi += (m & bm) == bm;
         }
     }
     const int slot = (blockIdx.x + blockIdx.y * 3 + blockIdx.z * 7) % kSlots;
@@ -743,25 +966,32 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
     const Dims D{v->nx, v->ny, v->nz};
     const int64_t N = v->N;
     const int64_t ntiles = (N + kBlock - 1) / kBlock;
+    // site / face tiles: 256 voxels, or 256 chunks of 8 z when nz allows
+    const bool vec = (v->nz % kVz) == 0;
+    const int64_t etiles =
+        vec ? (N / kVz + kBlock - 1) / kBlock : ntiles;
     int rc = VC_OK;
     if (!v->bnd_done)
     {
         uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr;
-        if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)ntiles)) != VC_OK)
+        if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)etiles)) != VC_OK)
             return rc;
-        if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)ntiles)) != VC_OK)
+        if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)etiles)) != VC_OK)
         {
             vc_free(ts);
             return rc;
         }
-        bnd_count<<<(unsigned)ntiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
+        if (vec)
+            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ts, tf);
+        else
+            bnd_count<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
         hipError_t e = hipGetLastError();
         uint32_t last_s = 0, last_f = 0, cnt_s = 0, cnt_f = 0;
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&last_s, ts + ntiles - 1, 4,
+            e = hipMemcpyAsync(&last_s, ts + etiles - 1, 4,
                                hipMemcpyDeviceToHost, S);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&last_f, tf + ntiles - 1, 4,
+            e = hipMemcpyAsync(&last_f, tf + etiles - 1, 4,
                                hipMemcpyDeviceToHost, S);
         if (e == hipSuccess)
             e = hipStreamSynchronize(S);
@@ -769,15 +999,15 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             rc = vc_fail(ctx, VC_ERR_HIP, "boundary count: %s",
                          hipGetErrorString(e));
         if (rc == VC_OK)
-            rc = scan_inplace(ctx, ts, ntiles);
+            rc = scan_inplace(ctx, ts, etiles);
         if (rc == VC_OK)
-            rc = scan_inplace(ctx, tf, ntiles);
+            rc = scan_inplace(ctx, tf, etiles);
         if (rc == VC_OK)
         {
-            e = hipMemcpyAsync(&cnt_s, ts + ntiles - 1, 4,
+            e = hipMemcpyAsync(&cnt_s, ts + etiles - 1, 4,
                                hipMemcpyDeviceToHost, S);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(&cnt_f, tf + ntiles - 1, 4,
+                e = hipMemcpyAsync(&cnt_f, tf + etiles - 1, 4,
                                    hipMemcpyDeviceToHost, S);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(S);
@@ -801,13 +1031,21 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
         }
         if (rc == VC_OK)
         {
-            bnd_emit_sites<<<(unsigned)ntiles, kBlock, 0, S>>>(
-                v->occ, D, N, ts, v->bnd_sites, cmap);
+            if (vec)
+                bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(
+                    v->occ, D, ts, v->bnd_sites, cmap);
+            else
+                bnd_emit_sites<<<(unsigned)etiles, kBlock, 0, S>>>(
+                    v->occ, D, N, ts, v->bnd_sites, cmap);
             e = hipGetLastError();
             if (e == hipSuccess)
             {
-                bnd_emit_quads<<<(unsigned)ntiles, kBlock, 0, S>>>(
-                    v->occ, D, N, tf, cmap, v->bnd_quads);
+                if (vec)
+                    bnd_emit_quads_v<<<(unsigned)etiles, kBlock, 0, S>>>(
+                        v->occ, D, tf, cmap, v->bnd_quads);
+                else
+                    bnd_emit_quads<<<(unsigned)etiles, kBlock, 0, S>>>(
+                        v->occ, D, N, tf, cmap, v->bnd_quads);
                 e = hipGetLastError();
             }
             // Euler terms

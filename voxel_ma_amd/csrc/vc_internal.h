@@ -122,6 +122,7 @@ enum vc_kind
     VK_LI,     // level set-up: 4 wavefronts (512 z each) per item
     VK_TP,     // subfield pass, (x, y) parity 0..3: 256 words per item
     VK_TT = VK_TP + 4, // tile sweep: one 32 x 32 x 64 (x, y, z) tile per item
+    VK_LN,             // level set-up for k > 0: 256 plane words per item
     VK_COUNT
 };
 

@@ -46,6 +46,8 @@ int64_t items_of(const vc_vdesc& d, int kind)
         return cdiv((int64_t)d.nx * d.ny * cdiv(d.nz, 512), kLiWaves);
     case VK_TT:
         return (int64_t)d.ntx * d.nty * d.nwz;
+    case VK_LN:
+        return cdiv((int64_t)d.nx * d.ny * d.nwz, kTpWords);
     default:
     {
         const int p = kind - VK_TP, sx = p & 1, sy = p >> 1;

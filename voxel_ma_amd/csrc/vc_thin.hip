@@ -347,6 +347,79 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Level set-up for k > 0, one thread per plane word: elig = (lvl <= k) of
+// its 64 voxels, and both revisit flags = an alive voxel becomes eligible
+// now (lvl == k).  The 64 level bytes are read as four 16-byte loads when the
+// rows are 16-byte aligned (nz a multiple of 16).  Also resets the level's
+// sweep control.
+template <bool kB>
+__global__ void __launch_bounds__(kBlock)
+    thin_level_next(vc_work W, int k, int nvol, uint32_t* __restrict__ ctl,
+                    uint32_t* __restrict__ gany, uint32_t* __restrict__ vany)
+{
+    {
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t gs = gridDim.x * blockDim.x;
+        if (g == 0)
+        {
+            ctl[0] = 0;
+            ctl[1] = (uint32_t)k;
+        }
+        for (uint32_t i = g; i < (uint32_t)VC_RING; i += gs)
+            gany[i] = 0;
+        for (uint32_t i = g; i < 4u * (uint32_t)nvol; i += gs)
+            vany[i] = 0;
+    }
+    for (uint32_t b = blockIdx.x; b < W.total; b += gridDim.x)
+    {
+        int vi;
+        uint32_t lt;
+        vc_item<kB>(W, b, vi, lt);
+        const vc_vdesc d = vc_desc<kB>(W, vi);
+        const int64_t o = (int64_t)lt * kBlock + threadIdx.x; // plane word
+        const int64_t words = (int64_t)d.nx * d.ny * d.nwz;
+        if (o >= words)
+            continue;
+        const int64_t rowi = o / d.nwz;
+        const int z0 = (int)(o - rowi * d.nwz) * 64;
+        const uint8_t* lv = d.lvl + rowi * d.nz + z0;
+        uint64_t e = 0, nw = 0;
+        if ((d.nz & 15) == 0 && z0 + 64 <= d.nz)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                const uint4 q = reinterpret_cast<const uint4*>(lv)[c];
+                const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                    {
+                        const uint32_t l = (wd[u] >> (8 * j)) & 0xFFu;
+                        const int bit = 16 * c + 4 * u + j;
+                        e |= (uint64_t)(l <= (uint32_t)k) << bit;
+                        nw |= (uint64_t)(l == (uint32_t)k) << bit;
+                    }
+            }
+        }
+        else
+        {
+            const int n = min(64, d.nz - z0);
+            for (int j = 0; j < n; ++j)
+            {
+                const uint32_t l = lv[j];
+                e |= (uint64_t)(l <= (uint32_t)k) << j;
+                nw |= (uint64_t)(l == (uint32_t)k) << j;
+            }
+        }
+        d.elig[o] = e;
+        const uint8_t f = (nw & d.alive[o]) ? 1 : 0;
+        d.d0[o] = f;
+        d.d1[o] = f;
+    }
+}
+
 // bits z-1, z, z+1 of a row (prev/next = the neighbouring words' edge bits)
 __device__ __forceinline__ uint32_t row3(uint64_t W, uint32_t prev63,
                                          uint32_t next0, int b)
@@ -1457,17 +1530,28 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
                                                  : (int64_t)0x7FFFFFFF - 64;
     for (int k = 0; k < nlev; ++k)
     {
-        const unsigned g = vc_sched_grid(ctx, s, VK_LI, sc.thin_grid_per_cu);
+        const int li_kind = k == 0 ? VK_LI : VK_LN;
+        const unsigned g = vc_sched_grid(ctx, s, li_kind, sc.thin_grid_per_cu);
         if (g)
         {
-            if (s->n > 1)
+            const vc_work W = vc_sched_work(s, li_kind);
+            if (k > 0)
+            {
+                if (s->n > 1)
+                    thin_level_next<true><<<g, kBlock, 0, S>>>(
+                        W, k, s->n, s->d_ctl, s->d_gany, s->d_vany);
+                else
+                    thin_level_next<false><<<g, kBlock, 0, S>>>(
+                        W, k, s->n, s->d_ctl, s->d_gany, s->d_vany);
+            }
+            else if (s->n > 1)
                 thin_level_init<true><<<g, kBlock, 0, S>>>(
-                    vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
-                    s->d_ctl, s->d_gany, s->d_vany);
+                    W, s->d_T, s->d_L, nlev, k, s->n, s->d_ctl, s->d_gany,
+                    s->d_vany);
             else
                 thin_level_init<false><<<g, kBlock, 0, S>>>(
-                    vc_sched_work(s, VK_LI), s->d_T, s->d_L, nlev, k, s->n,
-                    s->d_ctl, s->d_gany, s->d_vany);
+                    W, s->d_T, s->d_L, nlev, k, s->n, s->d_ctl, s->d_gany,
+                    s->d_vany);
             VC_CHECK_LAUNCH(ctx);
             launches++;
         }
