@@ -305,7 +305,7 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
     out["R6_tag"] = {"workload": "%d random points in the headline volume" % len(pts),
                      "ms": round(t_t * 1e3, 3), "value": round(len(pts) / t_t / 1e6, 1),
                      "unit": "Mpoints/s", "note": "host arrays, PCIe copies included"}
-    edges, foff, fe = V.cc_finalize(len(vts), [], [list(f) for f in faces])
+    edges, foff, fe = V.cc_finalize(len(vts), [], faces)
     ne, nf = len(edges), len(faces)
     fe_rows = np.repeat(np.arange(nf), np.diff(foff))
     ef_off, ef = _csr(ne, fe.astype(np.int64), fe_rows.astype(np.int32))
@@ -319,7 +319,7 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
                           "ms": round(t_m * 1e3, 3), "value": round(nf / t_m / 1e6, 2),
                           "unit": "Mfaces/s", "note": "host arrays, PCIe copies included"}
     valid = np.ones(len(vts), np.uint8)
-    fl = [list(f) for f in faces]
+    fl = faces
     t_x, _ = best_of(lambda: V.voro_extract_inside(len(vts), edges, fl, valid, le, lf), 1)
     t_g, _ = best_of(lambda: V.voro_merge_close(vts, edges, fl, valid, 0.25, True), 1)
     out["R8_R9_host"] = {"workload": "extract inside + merge close vertices on %d faces" % nf,

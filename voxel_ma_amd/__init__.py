@@ -605,11 +605,8 @@ def cc_finalize(nv, edges, faces):
     of (a, b); faces: list of vertex lists.  Returns (edges (ne, 2), face
     offsets, face side edge ids)."""
     e_in = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
-    foff = np.zeros(len(faces) + 1, np.int64)
-    for i, f in enumerate(faces):
-        foff[i + 1] = foff[i] + len(f)
-    fv = np.ascontiguousarray(np.concatenate([np.array(f, np.int32) for f in faces])
-                              if faces else np.zeros(0, np.int32))
+    foff, fv = _faces_csr(faces)
+    fv = fv[:foff[-1]] if foff[-1] < len(fv) else fv
     cap = len(e_in) + len(fv) + 1
     eout = np.zeros((cap, 2), np.int32)
     fe = np.zeros(max(len(fv), 1), np.int32)
@@ -640,13 +637,24 @@ def cc_prune(nv, edges, foff, fe, em, fm, f_t, l_t):
 
 
 # ---------------------------------------------------------------- Voronoi side
+def _faces_csr(faces):
+    """Face polygons as (offsets int64, vertex ids int32); a 2-D array of
+    equal-size polygons is converted without a Python loop."""
+    if isinstance(faces, np.ndarray) and faces.ndim == 2:
+        nf, k = faces.shape
+        foff = np.arange(nf + 1, dtype=np.int64) * k
+        fv = np.ascontiguousarray(faces, np.int32).ravel()
+    else:
+        foff = np.zeros(len(faces) + 1, np.int64)
+        foff[1:] = np.cumsum([len(f) for f in faces])
+        fv = (np.concatenate([np.asarray(f, np.int32) for f in faces])
+              if len(faces) else np.zeros(0, np.int32))
+    return foff, np.ascontiguousarray(fv if fv.size else np.zeros(1, np.int32))
+
+
 def _complex_arrays(nv, edges, faces_vts):
     e = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
-    foff = np.zeros(len(faces_vts) + 1, np.int64)
-    for i, f in enumerate(faces_vts):
-        foff[i + 1] = foff[i] + len(f)
-    fv = np.ascontiguousarray(np.concatenate([np.array(f, np.int32) for f in faces_vts])
-                              if len(faces_vts) else np.zeros(1, np.int32))
+    foff, fv = _faces_csr(faces_vts)
     return e, foff, fv
 
 
@@ -712,14 +720,14 @@ def voro_merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
     vts = np.ascontiguousarray(np.asarray(vts, np.float32).reshape(-1, 3))
     nv = len(vts)
     e, foff, fv = _complex_arrays(nv, edges, faces_vts)
+    # edges of each vertex in edge order (CSR)
+    rows = e.ravel().astype(np.int64)
+    cols = np.repeat(np.arange(len(e), dtype=np.int32), 2)
+    order = np.lexsort((cols, rows))
     ve_off = np.zeros(nv + 1, np.int64)
-    adj = [[] for _ in range(nv)]
-    for i, (a, b) in enumerate(np.array(edges, np.int32).reshape(-1, 2)):
-        adj[a].append(i)
-        adj[b].append(i)
-    for v in range(nv):
-        ve_off[v + 1] = ve_off[v] + len(adj[v])
-    ve = np.ascontiguousarray(np.array([i for a in adj for i in a] or [0], np.int32))
+    np.add.at(ve_off, rows + 1, 1)
+    ve_off = np.cumsum(ve_off)
+    ve = np.ascontiguousarray(cols[order] if len(cols) else np.zeros(1, np.int32), np.int32)
     vv = np.ascontiguousarray(np.asarray(v_valid, np.uint8)) if v_valid is not None else \
         np.ones(max(nv, 1), np.uint8)
     nf = len(faces_vts)
