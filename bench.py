@@ -176,6 +176,7 @@ def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
         "edt_achieved_gbs": round(EDT_BYTES_PER_VOXEL * voxels / (edt_ms * 1e-3) / 1e9, 1),
         "thin_stats": {"removed": [int(x) for x in st.removed[:levels.shape[1]]],
                        "passes_with_work": int(st.passes), "tests": int(st.tests),
+                       "voxel_passes": int(st.voxel_passes),
                        "launches": int(st.launches)},
         "schedule": sc.as_dict(), "tune_s": round(tune_s, 2),
         "tuned_ms": {"edt_measure": round(tt[0], 3), "thin": round(tt[1], 3)},
@@ -434,6 +435,9 @@ def main():
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the per-row measurements (boundary, octree, Voronoi side, "
                          "cell complex, writers)")
+    ap.add_argument("--no-sweep-events", action="store_true",
+                    help="time the tile sweeps by the thinning stage instead of HIP "
+                         "events around each launch")
     ap.add_argument("--profile-steps", action="store_true",
                     help="no CPU baseline, no hashing (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -509,6 +513,18 @@ def main():
     ctx.synchronize()
     wall = time.perf_counter() - t0
     print("[bench] timed %d steps in %.3f s" % (a.steps, wall), file=sys.stderr, flush=True)
+    # the thinning kernel's own launch durations: HIP events on the context
+    # stream around every tile-sweep launch, over a second timed region of
+    # the same steps (the event records add host gaps between launches, so
+    # they stay out of the region that gives `value`)
+    sweep_launches, sweep_kernel_ms = 0, 0.0
+    if not a.no_sweep_events:
+        ctx.thin_timing(True)
+        for k in range(a.steps):
+            step()
+        ctx.synchronize()
+        sweep_launches, sweep_kernel_ms = ctx.thin_timing_read()
+        ctx.thin_timing(False)
 
     edt_ms = sum(ev.ms(e[0], e[1]) for e in evs) / a.steps
     meas_ms = sum(ev.ms(e[1], e[2]) for e in evs) / a.steps
@@ -517,12 +533,27 @@ def main():
     value = N / (ms_step * 1e-3) / 1e6
 
     edt_gbs = EDT_BYTES_PER_VOXEL * N / (edt_ms * 1e-3) / 1e9
-    # thinning: average duration of one sweep (one tile-sweep launch: 8
-    # subfield passes) over the timed region, level set-ups and host waits
-    # between segments included; algorithmic bytes per sweep
+    # thinning: one tile-sweep launch runs all 8 subfield passes of a sweep
+    # over the tiles whose 3x3x3 tile neighbourhood changed in the sweep
+    # before.  Algorithmic bytes of a step = voxels touched, summed over the
+    # subfield passes (the library counts them: 8 x the voxels of each visited
+    # tile) x the per-pass model; launches past a level's convergence visit
+    # nothing.  Achieved = those bytes / the summed launch durations (HIP
+    # events), i.e. algorithmic bytes per launch / average launch duration.
     sweeps = max(1, stats.passes // 8)
-    sweep_ms = thin_ms / sweeps
-    thin_gbs = THIN_BYTES_PER_VOXEL_SWEEP * N / (sweep_ms * 1e-3) / 1e9
+    thin_bytes_step = THIN_BYTES_PER_VOXEL_PASS * stats.voxel_passes
+    launches_per_step = sweep_launches / a.steps if sweep_launches else 0
+    if sweep_launches:
+        kernel_ms_step = sweep_kernel_ms / a.steps
+        avg_launch_us = sweep_kernel_ms * 1e3 / sweep_launches
+    else:
+        # timing switched off: the thinning stage's time per sweep (level
+        # set-ups and host waits included) stands in for the kernel time
+        kernel_ms_step = thin_ms
+        avg_launch_us = thin_ms * 1e3 / sweeps
+    thin_gbs = thin_bytes_step / (kernel_ms_step * 1e-3) / 1e9
+    # the same launches credited with a dense sweep over all N voxels each
+    dense_gbs = THIN_BYTES_PER_VOXEL_SWEEP * N * sweeps / (kernel_ms_step * 1e-3) / 1e9
     out = {
         "metric": "Mvoxels/s EDT+thinning at 512^3",
         "value": round(value, 2),
@@ -545,13 +576,22 @@ def main():
                       "thin": round(thin_ms, 3)},
         "thin_stats": {"removed": [int(x) for x in stats.removed[:len(levels)]],
                        "passes_with_work": int(stats.passes), "tests": int(stats.tests),
+                       "voxel_passes": int(stats.voxel_passes),
                        "launches": int(stats.launches)},
         "roofline": {"bound": "hbm", "kernel": "thin_tile_sweep (8 subfield passes)",
                      "achieved": round(thin_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(thin_gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("thin_tile_sweep", n),
-                     "bytes_per_voxel_sweep": round(THIN_BYTES_PER_VOXEL_SWEEP, 4),
-                     "sweeps": sweeps, "avg_sweep_us": round(sweep_ms * 1e3, 2)},
+                     "bytes_per_voxel_pass": round(THIN_BYTES_PER_VOXEL_PASS, 4),
+                     "voxel_passes_per_step": int(stats.voxel_passes),
+                     "touched_frac": round(stats.voxel_passes / (8.0 * N * sweeps), 4),
+                     "dense_equivalent_gbs": round(dense_gbs, 1),
+                     "sweeps_per_step": sweeps,
+                     "launches_per_step": round(launches_per_step, 2),
+                     "avg_launch_us": round(avg_launch_us, 2),
+                     "kernel_ms_per_step": round(kernel_ms_step, 3),
+                     "timing": "hip events per launch" if sweep_launches
+                               else "thinning stage / sweeps"},
         "roofline_edt": {"bound": "hbm", "kernel": "edt (3 passes)", "achieved": round(edt_gbs, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21,

@@ -116,6 +116,10 @@ typedef struct vc_thin_stats {
     int64_t passes;                 /* subfield passes that had work        */
     int64_t tests;                  /* simple-point tests executed          */
     int64_t launches;               /* kernel launches issued               */
+    int64_t voxel_passes;           /* voxels visited, summed over subfield
+                                       passes: a tile sweep counts its tiles'
+                                       voxels 8 times, a word pass 4 x 64
+                                       voxels per flagged word             */
 } vc_thin_stats;
 /* levels: ascending, n_levels in [1, 64].  For level k (ascending) repeat
  * 8-subfield sweeps (subfield s = (x&1)|(y&1)<<1|(z&1)<<2, s ascending)
@@ -127,6 +131,17 @@ typedef struct vc_thin_stats {
  * stats may be NULL; when given, the call waits for the result. */
 int vc_thin_levels(vc_ctx* ctx, vc_vol* vol, const float* levels,
                    int n_levels, vc_thin_stats* stats);
+
+/* Tile-sweep timing, a measurement hook with no reference counterpart:
+ * while enabled, each thin_tile_sweep launch that vc_thin_levels or
+ * vc_batch_run issues on this context is bracketed by two HIP events on the
+ * context stream (with use_graph, one pair per graph segment, which then also
+ * holds the segment's entry and bookkeeping kernels); the elapsed times are
+ * read at the host wait that ends each segment.  Enabling or disabling
+ * resets the totals.  vc_thin_timing_read returns the launches timed and
+ * their summed duration in milliseconds. */
+int vc_thin_timing(vc_ctx* ctx, int enable);
+int vc_thin_timing_read(vc_ctx* ctx, int64_t* launches, double* total_ms);
 
 /* ---- (b) boundary sites + cube-complex Euler ---------------------------- */
 typedef struct vc_euler {

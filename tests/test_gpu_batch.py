@@ -312,3 +312,53 @@ def test_tile_sweeps_blobs_match_dense(vc):
     for lab, rem, passes in out[1:]:
         assert np.array_equal(lab, out[0][0])
         assert rem == out[0][1] and passes == out[0][2]
+
+
+def test_sweep_timing_and_visited_voxels(vc):
+    """The tile-sweep timing hook brackets each tile-sweep launch with HIP
+    events without changing any result, and the visited-voxel counter stays
+    within the dense bound (tiles of 32 x 32 x 64 voxels, 8 passes each; word
+    passes 256 voxels per visited word)."""
+    shape = (128, 96, 128)   # 4 x 3 x 2 whole tiles
+    occ, _ = vc.synth_blurred_noise(shape, sigma=4.0, fill=0.35, seed=6)
+    host = occ.download(np.uint8, shape)
+    occ.free()
+    from voxel_ma_amd import synth
+    levels = synth.levels_from_tt(synth.inside_size(host))
+    N = host.size
+    runs = {}
+    for name, sched, timed in (("tile", dict(tile_sweep=0), False),
+                               ("tile-timed", dict(tile_sweep=0), True),
+                               ("tile-graph-timed", dict(tile_sweep=0, use_graph=1,
+                                                         sweeps_per_sync=4), True),
+                               ("dense", dict(tile_sweep=-1, tail_sweep=-1), True)):
+        say("sweep timing", name)
+        m = vc.upload(host)
+        b = V.Batch(vc, [m])
+        b.set_schedule(**sched)
+        vc.thin_timing(timed)
+        st = b.run(levels)
+        launches, ms = vc.thin_timing_read()
+        vc.thin_timing(False)
+        runs[name] = (m.get(V.BUF_LABEL), list(st.removed[:3]), st.passes,
+                      st.voxel_passes)
+        sweeps = st.passes // 8
+        assert 0 < st.voxel_passes <= 8 * N * sweeps, name
+        if name.startswith("tile"):
+            assert st.voxel_passes % (8 * 32 * 32 * 64) == 0, name
+            if timed:
+                assert sweeps <= launches <= st.launches and ms > 0.0, name
+        else:
+            assert st.voxel_passes % 256 == 0
+            assert launches == 0 and ms == 0.0   # no tile sweeps ran
+        if not timed:
+            assert launches == 0 and ms == 0.0
+        b.release()
+        m.release()
+    ref = runs["tile"]
+    for name, r in runs.items():
+        assert np.array_equal(r[0], ref[0]), name
+        assert r[1] == ref[1] and r[2] == ref[2], name
+    # the tile schedules visit the same tiles whether timed or not
+    assert runs["tile-timed"][3] == ref[3]
+    assert runs["tile-graph-timed"][3] == ref[3]

@@ -31,7 +31,8 @@ _BUF_DTYPE = {BUF_OCC: np.uint8, BUF_EDT: np.int32, BUF_FT: np.int32,
 # every symbol declared in include/vc_api.h
 EXPORTS = [
     "vc_ctx_create", "vc_ctx_destroy", "vc_last_error", "vc_ctx_stream",
-    "vc_ctx_synchronize", "vc_simple_table_host", "vc_volume_upload",
+    "vc_ctx_synchronize", "vc_simple_table_host", "vc_thin_timing",
+    "vc_thin_timing_read", "vc_volume_upload",
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
     "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
@@ -67,7 +68,8 @@ class VcError(RuntimeError):
 
 class ThinStats(ctypes.Structure):
     _fields_ = [("removed", ctypes.c_int64 * VC_MAX_LEVELS), ("passes", ctypes.c_int64),
-                ("tests", ctypes.c_int64), ("launches", ctypes.c_int64)]
+                ("tests", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("voxel_passes", ctypes.c_int64)]
 
 
 class Euler(ctypes.Structure):
@@ -131,6 +133,9 @@ def library():
         "vc_ctx_stream": [P, PP],
         "vc_ctx_synchronize": [P],
         "vc_simple_table_host": [P, P],
+        "vc_thin_timing": [P, I],
+        "vc_thin_timing_read": [P, ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_double)],
         "vc_volume_upload": [P, P, I, I, I, PP],
         "vc_volume_from_device": [P, P, I, I, I, PP],
         "vc_volume_release": [P, P],
@@ -302,6 +307,18 @@ class Context:
 
     def synchronize(self):
         self.check(self._lib.vc_ctx_synchronize(self.handle))
+
+    def thin_timing(self, enable):
+        """Bracket every tile-sweep launch of this context with HIP events
+        (True) or stop (False); either resets the totals."""
+        self.check(self._lib.vc_thin_timing(self.handle, 1 if enable else 0))
+
+    def thin_timing_read(self):
+        """(launches timed, summed duration in ms) since thin_timing()."""
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        self.check(self._lib.vc_thin_timing_read(self.handle, ctypes.byref(n),
+                                                 ctypes.byref(ms)))
+        return int(n.value), float(ms.value)
 
     def simple_table(self):
         out = np.empty(1 << 21, np.uint32)

@@ -134,6 +134,13 @@ int vc_ctx_destroy(vc_ctx* ctx)
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     vc_free(ctx->d_lut);
+    for (hipEvent_t e : ctx->ev_free)
+        (void)hipEventDestroy(e);
+    for (auto& pr : ctx->ev_open)
+    {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;

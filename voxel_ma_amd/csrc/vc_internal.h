@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vc_api.h"
@@ -22,6 +23,15 @@ struct vc_ctx
     std::string err;
     // autotuned schedules by batch shape (volumes, voxels, largest axis)
     std::map<std::vector<int64_t>, vc_schedule> tuned;
+    // tile-sweep timing (vc_thin_timing): an event pair around each direct
+    // thin_tile_sweep launch, or around each graph segment, read back at the
+    // segment's host wait
+    bool time_sweeps = false;
+    int64_t sweep_launches = 0;
+    double sweep_ms = 0.0;
+    std::vector<hipEvent_t> ev_free;                         // reusable
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_open;  // not yet read
+    std::vector<int> ev_open_n;                              // launches per pair
 };
 
 // packed voxel coordinate: x << (by+bz) | y << bz | z, fits in 31 bits
@@ -52,7 +62,7 @@ struct vc_vol
     uint64_t* elig = nullptr;  // eligible at the current level (lvl <= k)
     uint8_t* dflag = nullptr;  // 2 x words: revisit flags per z parity
     uint64_t* alive2 = nullptr; // tile sweeps: the other alive plane
-    uint8_t* tchg = nullptr;    // tile sweeps: 2 x tiles change flags
+    uint32_t* tchg = nullptr;   // tile sweeps: 2 x tiles neighbour masks
     // boundary stage results (computed once per volume)
     bool bnd_done = false;
     float* bnd_sites = nullptr;   // 3 floats per site
@@ -108,7 +118,9 @@ struct vc_vdesc
     uint8_t* d0; // revisit flags, z parity 0
     uint8_t* d1; // revisit flags, z parity 1
     uint64_t* alive2; // second alive plane of the tile sweeps
-    uint8_t* tchg;    // 2 x tiles: tile changed in the last sweep (by parity)
+    uint32_t* tchg;   // 2 x tiles (by sweep parity): bit (dx+1)*9+(dy+1)*3+
+                      // (dz+1) = the tile removed a voxel in the halo of its
+                      // neighbour at (dx, dy, dz) (bit 13: anywhere)
     int ntx, nty;     // tiles along x and y (32 rows each)
 };
 

@@ -130,7 +130,7 @@ int vc_vol_ensure(vc_ctx* ctx, vc_vol* v)
         return rc;
     const size_t tiles = (size_t)cdiv(v->nx, kTileXY) * cdiv(v->ny, kTileXY) *
                          ((v->nz + 63) / 64);
-    if (!v->tchg && (rc = vc_alloc(ctx, (void**)&v->tchg, 2 * tiles)) != VC_OK)
+    if (!v->tchg && (rc = vc_alloc(ctx, (void**)&v->tchg, 2 * 4 * tiles)) != VC_OK)
         return rc;
     return VC_OK;
 }

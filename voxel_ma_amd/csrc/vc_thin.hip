@@ -751,9 +751,17 @@ __global__ void __launch_bounds__(kBlock)
             if (cand)
                 cand = load_nbhd(P, d.alive, C, cand, x, y, w, N);
         }
-        // items without candidates end here (block-uniform)
-        if (!__syncthreads_or(cand != 0))
+        // visited words (flagged), each standing for the 4 x 64 voxels of
+        // its (x/2, y/2, w) cell; items without candidates end here
+        const uint32_t nvis = (uint32_t)__syncthreads_count(fl != 0);
+        const bool anycand = __syncthreads_or(cand != 0);
+        if (!anycand)
+        {
+            if (threadIdx.x == 0 && nvis)
+                atomicAdd(&ctr[(size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS + 1],
+                          (unsigned long long)nvis * 256ull);
             continue;
+        }
         rmv[threadIdx.x] = 0;
         uint32_t total;
         const uint32_t off =
@@ -813,6 +821,7 @@ __global__ void __launch_bounds__(kBlock)
                 atomicAdd(&c[8 + k], (unsigned long long)nrem);
             }
             atomicAdd(&c[0], (unsigned long long)total);
+            atomicAdd(&c[1], (unsigned long long)nvis * 256ull);
         }
         __syncthreads(); // red, rmv are rewritten by the next item
     }
@@ -900,6 +909,8 @@ __global__ void __launch_bounds__(64)
             }
             if (ntest)
                 atomicAdd(&c[0], (unsigned long long)ntest);
+            if (n)
+                atomicAdd(&c[1], (unsigned long long)n * 256ull);
         }
         __syncthreads(); // list is rewritten by the next item
     }
@@ -934,6 +945,24 @@ __global__ void __launch_bounds__(64)
 constexpr int kTT = 32;                     // interior rows along x and y
 constexpr int kHX = 8, kHY = 4, kHZ = 2;    // halo: rows x, rows y, bits z
 constexpr int kRX = kTT + 2 * kHX, kRY = kTT + 2 * kHY, kTRows = kRX * kRY;
+// neighbour masks: bit (dx+1)*9 + (dy+1)*3 + (dz+1) for the tile at offset
+// (dx, dy, dz); bit 13 is the tile itself
+constexpr uint32_t kAllNbrs = (1u << 27) - 1u;
+
+// The tiles whose region (tile + halo) holds a voxel at tile-local
+// (x, y, z): the tile itself, plus the lower neighbour along an axis when the
+// voxel lies within that neighbour's halo reach (x < kHX, ...), the upper one
+// when x >= kTT - kHX, and the products of those along several axes.
+__device__ __forceinline__ uint32_t halo_nbrs(int x, int y, int z)
+{
+    const uint32_t xs = 2u | (x < kHX ? 1u : 0u) | (x >= kTT - kHX ? 4u : 0u);
+    const uint32_t ys = 2u | (y < kHY ? 1u : 0u) | (y >= kTT - kHY ? 4u : 0u);
+    const uint32_t zs = 2u | (z < kHZ ? 1u : 0u) | (z >= 64 - kHZ ? 4u : 0u);
+    const uint32_t yz = ((ys & 1u) ? zs : 0u) | ((ys & 2u) ? zs << 3 : 0u) |
+                        ((ys & 4u) ? zs << 6 : 0u);
+    return ((xs & 1u) ? yz : 0u) | ((xs & 2u) ? yz << 9 : 0u) |
+           ((xs & 4u) ? yz << 18 : 0u);
+}
 #ifndef VC_TILE_BLOCK
 #define VC_TILE_BLOCK 512
 #endif
@@ -1036,9 +1065,10 @@ __global__ void __launch_bounds__(kBlock)
         const bool any = __syncthreads_or(f);
         if (threadIdx.x == 0)
         {
+            // a tile with new candidates: itself and every neighbour
             const int64_t nt = (int64_t)d.ntx * d.nty * nwz;
-            d.tchg[lt] = any ? 1 : 0;
-            d.tchg[nt + lt] = any ? 1 : 0;
+            d.tchg[lt] = any ? kAllNbrs : 0u;
+            d.tchg[nt + lt] = any ? kAllNbrs : 0u;
         }
     }
 }
@@ -1092,6 +1122,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
     __shared__ uint8_t SEhi[kTRows];
     __shared__ uint32_t tq[kTQ];
     __shared__ uint32_t red[kTBlock / 64];
+    __shared__ uint32_t snbr; // neighbour mask of this tile's removals
     // XCD-mapped: the tiles of one row column (consecutive items) and their
     // neighbours share rows, so they go to the same XCD's L2
     const uint32_t per = (W.total + 7) / 8;
@@ -1111,8 +1142,11 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
         const int w = (int)(lt % (uint32_t)nwz);
         const uint32_t r2 = lt / (uint32_t)nwz;
         const int ty = (int)(r2 % (uint32_t)d.nty), tx = (int)(r2 / (uint32_t)d.nty);
-        const uint8_t* cprev = d.tchg + (int64_t)((sweep + 1) & 1) * nt;
-        uint8_t* ccur = d.tchg + (int64_t)(sweep & 1) * nt;
+        const uint32_t* cprev = d.tchg + (int64_t)((sweep + 1) & 1) * nt;
+        uint32_t* ccur = d.tchg + (int64_t)(sweep & 1) * nt;
+        // visit iff a tile of the 3x3x3 neighbourhood removed a voxel of this
+        // tile's region in the last sweep: a tile whose region is unchanged
+        // and that removed nothing would remove nothing again
         bool act = false;
         if (threadIdx.x < 27)
         {
@@ -1121,7 +1155,8 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             const int aw = w + (int)threadIdx.x % 3 - 1;
             if ((unsigned)ax < (unsigned)d.ntx && (unsigned)ay < (unsigned)d.nty &&
                 (unsigned)aw < (unsigned)nwz)
-                act = cprev[((int64_t)ax * d.nty + ay) * nwz + aw] != 0;
+                act = (cprev[((int64_t)ax * d.nty + ay) * nwz + aw] >>
+                       (26 - threadIdx.x)) & 1u;
         }
         if (!__syncthreads_or(act))
         {
@@ -1129,6 +1164,8 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                 ccur[lt] = 0;
             continue;
         }
+        if (threadIdx.x == 0)
+            snbr = 0u;
         const uint64_t* Pin = (sweep & 1) ? d.alive2 : d.alive;
         uint64_t* Pout = (sweep & 1) ? d.alive : d.alive2;
         const int X0 = tx * kTT - kHX, Y0 = ty * kTT - kHY; // region origin
@@ -1148,7 +1185,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             SEhi[r] = (uint8_t)ehi;
         }
         __syncthreads();
-        uint32_t nrem = 0, ntest = 0;
+        uint32_t nrem = 0, ntest = 0, nbr = 0;
         uint8_t* lrow0 = d.label + (int64_t)64 * w;
         for (int p = 0; p < 8; ++p)
         {
@@ -1268,6 +1305,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                         ry < kHY + kTT && i >= kHZ && i < kHZ + 64)
                     {
                         ++nrem;
+                        nbr |= halo_nbrs(rx - kHX, ry - kHY, i - kHZ);
                         lrow0[((int64_t)(X0 + rx) * d.ny + (Y0 + ry)) * d.nz +
                               (i - kHZ)] = (uint8_t)(2 + k);
                     }
@@ -1287,10 +1325,15 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                     (t.lo >> kHZ) | ((uint64_t)t.hi << (64 - kHZ));
             }
         }
-        const uint32_t tot = block_sum(nrem, red);
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1)
+            nbr |= __shfl_xor(nbr, dd);
+        if ((threadIdx.x & 63) == 0 && nbr)
+            atomicOr(&snbr, nbr);
+        const uint32_t tot = block_sum(nrem, red); // orders the atomics
         if (threadIdx.x == 0)
         {
-            ccur[lt] = tot ? 1 : 0;
+            ccur[lt] = snbr;
             unsigned long long* c =
                 ctr + (size_t)(b % VC_CTR_SLOTS) * VC_CTR_WORDS;
             if (tot)
@@ -1301,6 +1344,10 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             }
             if (ntest)
                 atomicAdd(&c[0], (unsigned long long)ntest);
+            // the tile's own voxels, once per subfield pass
+            const int64_t tv = (int64_t)min(kTT, d.nx - tx * kTT) *
+                               min(kTT, d.ny - ty * kTT) * min(64, d.nz - 64 * w);
+            atomicAdd(&c[1], (unsigned long long)(8 * tv));
         }
         __syncthreads(); // the LDS rows, tq, red are rewritten by the next item
     }
@@ -1362,6 +1409,73 @@ void launch_tile_sweep(hipStream_t S, unsigned g, const vc_work& W,
                                              s->d_vany, s->d_ctr);
 }
 
+// sweep timing: an event from the context's pool (nullptr when none can be
+// created)
+hipEvent_t ev_take(vc_ctx* ctx)
+{
+    if (ctx->ev_free.empty())
+    {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess)
+            return nullptr;
+        return e;
+    }
+    hipEvent_t e = ctx->ev_free.back();
+    ctx->ev_free.pop_back();
+    return e;
+}
+
+// records the first event of a timed region; e[0] = e[1] = nullptr when
+// timing is off
+int ev_begin(vc_ctx* ctx, hipEvent_t* e)
+{
+    e[0] = e[1] = nullptr;
+    if (!ctx->time_sweeps)
+        return VC_OK;
+    e[0] = ev_take(ctx);
+    e[1] = ev_take(ctx);
+    if (!e[0] || !e[1])
+    {
+        for (int i = 0; i < 2; ++i)
+            if (e[i])
+                ctx->ev_free.push_back(e[i]);
+        return vc_fail(ctx, VC_ERR_HIP, "sweep timing: hipEventCreate failed");
+    }
+    VC_HIP(ctx, hipEventRecord(e[0], ctx->stream));
+    return VC_OK;
+}
+
+int ev_end(vc_ctx* ctx, hipEvent_t* e, int launches)
+{
+    if (!e[0])
+        return VC_OK;
+    VC_HIP(ctx, hipEventRecord(e[1], ctx->stream));
+    ctx->ev_open.emplace_back(e[0], e[1]);
+    ctx->ev_open_n.push_back(launches);
+    return VC_OK;
+}
+
+// after a host wait on the context stream: add the closed regions
+int ev_collect(vc_ctx* ctx)
+{
+    for (size_t i = 0; i < ctx->ev_open.size(); ++i)
+    {
+        float ms = 0.0f;
+        const hipError_t e = hipEventElapsedTime(&ms, ctx->ev_open[i].first,
+                                                 ctx->ev_open[i].second);
+        if (e != hipSuccess)
+            return vc_fail(ctx, VC_ERR_HIP, "sweep timing: %s",
+                           hipGetErrorString(e));
+        ctx->sweep_ms += ms;
+        ctx->sweep_launches += ctx->ev_open_n[i];
+        ctx->ev_free.push_back(ctx->ev_open[i].first);
+        ctx->ev_free.push_back(ctx->ev_open[i].second);
+    }
+    ctx->ev_open.clear();
+    ctx->ev_open_n.clear();
+    return VC_OK;
+}
+
 int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
                    int k, int mode)
 {
@@ -1383,12 +1497,22 @@ int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
         for (int j = 0; j < K && g; ++j)
         {
             const int jj = graph ? j : (int)(first + j);
+            hipEvent_t ev[2] = {nullptr, nullptr};
+            if (!graph)
+            {
+                const int rc = ev_begin(ctx, ev);
+                if (rc != VC_OK)
+                    return rc;
+            }
             if (s->n > 1)
                 launch_tile_sweep<true>(S, g, W, ctx->d_lut, jj, k, s->n, ctl, s);
             else
                 launch_tile_sweep<false>(S, g, W, ctx->d_lut, jj, k, s->n, ctl,
                                          s);
             VC_CHECK_LAUNCH(ctx);
+            const int rc = ev_end(ctx, ev, 1);
+            if (rc != VC_OK)
+                return rc;
         }
     }
     else
@@ -1464,8 +1588,15 @@ int run_segment(vc_ctx* ctx, vc_sched* s, int K, int64_t first, int k,
                            hipGetErrorString(e2));
         it = s->graphs.emplace(key, ex).first;
     }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (mode >= SEG_TILE)
+    {
+        const int rc = ev_begin(ctx, ev);
+        if (rc != VC_OK)
+            return rc;
+    }
     VC_HIP(ctx, hipGraphLaunch(it->second, S));
-    return VC_OK;
+    return ev_end(ctx, ev, K);
 }
 
 int upload_levels(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev)
@@ -1497,6 +1628,34 @@ int upload_levels(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev)
 }
 
 } // namespace
+
+int vc_thin_timing(vc_ctx* ctx, int enable)
+{
+    if (!ctx || (enable != 0 && enable != 1))
+        return VC_ERR_INVALID;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    VC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& pr : ctx->ev_open)
+    {
+        ctx->ev_free.push_back(pr.first);
+        ctx->ev_free.push_back(pr.second);
+    }
+    ctx->ev_open.clear();
+    ctx->ev_open_n.clear();
+    ctx->time_sweeps = enable != 0;
+    ctx->sweep_launches = 0;
+    ctx->sweep_ms = 0.0;
+    return VC_OK;
+}
+
+int vc_thin_timing_read(vc_ctx* ctx, int64_t* launches, double* total_ms)
+{
+    if (!ctx || !launches || !total_ms)
+        return VC_ERR_INVALID;
+    *launches = ctx->sweep_launches;
+    *total_ms = ctx->sweep_ms;
+    return VC_OK;
+}
 
 int vc_lut_build(vc_ctx* ctx)
 {
@@ -1582,6 +1741,9 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
             VC_HIP(ctx, hipMemcpyAsync(s->h_ring, s->d_gany, 4 * VC_RING,
                                        hipMemcpyDeviceToHost, S));
             VC_HIP(ctx, hipStreamSynchronize(S));
+            rc = ev_collect(ctx);
+            if (rc != VC_OK)
+                return rc;
             int64_t conv = -1;
             for (int jj = 0; jj < K; ++jj)
                 if (s->h_ring[(base + jj) & (VC_RING - 1)] == 0)
@@ -1609,6 +1771,7 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
         {
             const unsigned long long* c = s->h_ctr + (size_t)sl * VC_CTR_WORDS;
             st->tests += (int64_t)c[0];
+            st->voxel_passes += (int64_t)c[1];
             for (int q = 0; q < nlev; ++q)
                 st->removed[q] += (int64_t)c[8 + q];
         }
