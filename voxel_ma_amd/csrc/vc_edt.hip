@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kBlock)
 template <bool kB>
 __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 {
-    extern __shared__ int16_t colz[]; // [ny][TZ]
+    extern __shared__ int16_t colz[]; // [ny][SY], SY >= TZ
     const uint32_t per = (W.total + 7) / 8;
     for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
     {
@@ -131,44 +131,61 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
         vc_item<kB>(W, b, vi, T);
         const vc_vdesc d = vc_desc<kB>(W, vi);
         const int ny = d.ny, nz = d.nz, TZ = d.tzy;
+        const int SY = vc_edt_ystride(TZ); // padded LDS row
         const int nzt = (nz + TZ - 1) / TZ;
         const int z0 = (int)(T % nzt) * TZ;
         const int x = (int)(T / nzt);
-        const int tz = threadIdx.x % TZ;
-        const int ty0 = threadIdx.x / TZ;
-        const int tys = blockDim.x / TZ;
-        const int z = z0 + tz;
-        const bool zin = z < nz;
         const int64_t base = (int64_t)x * ny * nz;
         const int16_t* zq = reinterpret_cast<const int16_t*>(d.edt);
-        // 16 independent loads in flight per thread before the LDS stores
-        constexpr int U = 16;
-        for (int y = ty0; y < ny; y += tys * U)
         {
-            int16_t v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
+            // staging: TZ-wide coalesced row segments, 16 independent loads
+            // in flight per thread before the LDS stores
+            const int tz = threadIdx.x % TZ;
+            const int ty0 = threadIdx.x / TZ;
+            const int tys = blockDim.x / TZ;
+            const int z = z0 + tz;
+            const bool zin = z < nz;
+            constexpr int U = 16;
+            for (int y = ty0; y < ny; y += tys * U)
             {
-                const int yy = y + u * tys;
-                v[u] = (zin && yy < ny) ? zq[base + (int64_t)yy * nz + z]
-                                        : (int16_t)-1;
-            }
+                int16_t v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-            {
-                const int yy = y + u * tys;
-                if (yy < ny)
-                    colz[yy * TZ + tz] = v[u];
+                for (int u = 0; u < U; ++u)
+                {
+                    const int yy = y + u * tys;
+                    v[u] = (zin && yy < ny) ? zq[base + (int64_t)yy * nz + z]
+                                            : (int16_t)-1;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const int yy = y + u * tys;
+                    if (yy < ny)
+                        colz[yy * SY + tz] = v[u];
+                }
             }
         }
         __syncthreads();
-        if (zin)
-            for (int y = ty0; y < ny; y += tys)
+        // search lanes: each wavefront takes blocks of ZG z x YG y (8 x 8
+        // for TZ >= 8); a compact block keeps the lanes' search lengths
+        // close, so fewer lanes idle while the longest search runs
+        const int lzg = min(__builtin_ctz((unsigned)TZ), 3);
+        const int ZG = 1 << lzg, YG = 64 >> lzg;
+        const int lngz = __builtin_ctz((unsigned)TZ) - lzg; // z blocks: 2^lngz
+        const int nblk = ((ny + YG - 1) / YG) << lngz;
+        const int lane = threadIdx.x & 63;
+        for (int B = (int)(threadIdx.x >> 6); B < nblk; B += (int)(blockDim.x >> 6))
+        {
+            const int y = (B >> lngz) * YG + (lane >> lzg);
+            const int tz = ((B & ((1 << lngz) - 1)) << lzg) + (lane & (ZG - 1));
+            const int z = z0 + tz;
+            if (y >= ny || z >= nz)
+                continue;
             {
                 int best = kInf;
                 int by = -1;
                 {
-                    const int q = colz[y * TZ + tz];
+                    const int q = colz[y * SY + tz];
                     if (q >= 0)
                     {
                         best = (z - q) * (z - q);
@@ -184,8 +201,8 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                     if (d2 > best)
                         break;
                     const int yl = y - dd, yh = y + dd;
-                    const int ql = colz[max(yl, 0) * TZ + tz];
-                    const int qh = colz[min(yh, ny - 1) * TZ + tz];
+                    const int ql = colz[max(yl, 0) * SY + tz];
+                    const int qh = colz[min(yh, ny - 1) * SY + tz];
                     const int vl =
                         (yl >= 0 && ql >= 0) ? (z - ql) * (z - ql) + d2 : kInf;
                     const int vh =
@@ -200,9 +217,10 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                 uint32_t outv = VC_NONE;
                 if (by >= 0)
                     outv = ((uint32_t)by << 16) |
-                           (uint32_t)(uint16_t)colz[by * TZ + tz];
+                           (uint32_t)(uint16_t)colz[by * SY + tz];
                 d.ft[base + (int64_t)y * nz + z] = outv;
             }
+        }
         __syncthreads(); // colz is rewritten by the next item
     }
 }

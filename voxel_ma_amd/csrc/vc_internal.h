@@ -41,6 +41,14 @@ struct vc_pack
     uint32_t my, mz;
 };
 
+// LDS row (int16 elements) of the EDT y pass for TZ staged z columns: padded
+// so that the 4 rows of a half-wavefront's 8 x 8 search block fall on
+// disjoint LDS banks
+inline __host__ __device__ int vc_edt_ystride(int tz)
+{
+    return tz + (tz >= 32 ? tz / 8 : 0);
+}
+
 struct vc_sched;
 
 struct vc_vol

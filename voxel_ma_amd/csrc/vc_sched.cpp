@@ -192,7 +192,8 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
     for (const auto& d : s->h_desc)
     {
         s->ez_stride = std::max(s->ez_stride, d.nwz);
-        s->lds[VK_EY] = std::max(s->lds[VK_EY], (size_t)d.tzy * d.ny * 2);
+        s->lds[VK_EY] = std::max(s->lds[VK_EY],
+                                 (size_t)vc_edt_ystride(d.tzy) * d.ny * 2);
         s->lds[VK_EX] = std::max(s->lds[VK_EX], (size_t)d.tzx * d.nx * 4);
     }
     s->lds[VK_EZ] = (size_t)kRowsZ * s->ez_stride * 8;
