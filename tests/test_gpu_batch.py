@@ -85,7 +85,7 @@ def test_batch_matches_oracle_per_volume(vc, ragged):
          tail_sweep=0, tile_sweep=-1),
     dict(edt_lds_kb=32, edt_grid_per_cu=8, thin_grid_per_cu=8, sweeps_per_sync=0, use_graph=1,
          tail_sweep=2, tile_sweep=-1),
-    dict(edt_lds_kb=16, edt_grid_per_cu=2, thin_grid_per_cu=0, sweeps_per_sync=3, use_graph=1,
+    dict(edt_lds_kb=8, edt_grid_per_cu=2, thin_grid_per_cu=0, sweeps_per_sync=3, use_graph=1,
          tail_sweep=1, tile_sweep=-1),
     dict(edt_lds_kb=32, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=0, use_graph=0,
          tail_sweep=3, tile_sweep=-1),
@@ -216,7 +216,7 @@ def test_batch_tune_installs_valid_schedule(vc, ragged):
     b = V.Batch(vc, members)
     b.edt().measure()
     sc, (t_front, t_thin) = b.tune(levels, reps=1)
-    assert sc.edt_lds_kb in (16, 32, 64) and t_front > 0 and t_thin > 0
+    assert sc.edt_lds_kb in (8, 16, 32, 64) and t_front > 0 and t_thin > 0
     assert b.schedule().as_dict() == sc.as_dict()
     _check_members(members, vols, levels, ref, "tuned")
     # same shape key: the cached schedule is reused

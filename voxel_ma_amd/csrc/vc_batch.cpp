@@ -272,8 +272,8 @@ int vc_batch_tune(vc_ctx* ctx, vc_batch* b, const float* levels, int n_levels,
     else
     {
         // EDT + measure candidates, incumbent first: (lds KiB, grid per CU)
-        static const int kEdt[][2] = {{32, 0}, {16, 0}, {64, 0},
-                                      {32, 8}, {16, 8}, {64, 8}};
+        static const int kEdt[][2] = {{16, 0}, {32, 0}, {8, 0},
+                                      {64, 0}, {16, 8}, {32, 8}};
         vc_schedule best = vc_schedule_default();
         float best_t = 0.0f;
         for (size_t c = 0; c < sizeof(kEdt) / sizeof(kEdt[0]); ++c)

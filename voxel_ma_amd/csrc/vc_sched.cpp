@@ -89,7 +89,7 @@ void drop_graphs(vc_sched* s)
 vc_schedule vc_schedule_default()
 {
     vc_schedule sc;
-    sc.edt_lds_kb = 32;
+    sc.edt_lds_kb = 16;
     sc.edt_grid_per_cu = 0;
     sc.thin_grid_per_cu = 0;
     sc.sweeps_per_sync = 0;
@@ -162,8 +162,9 @@ void vc_sched_free(vc_sched* s)
 
 int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
 {
-    if (sc.edt_lds_kb != 16 && sc.edt_lds_kb != 32 && sc.edt_lds_kb != 64)
-        return vc_fail(ctx, VC_ERR_INVALID, "edt_lds_kb must be 16, 32 or 64");
+    if (sc.edt_lds_kb != 8 && sc.edt_lds_kb != 16 && sc.edt_lds_kb != 32 &&
+        sc.edt_lds_kb != 64)
+        return vc_fail(ctx, VC_ERR_INVALID, "edt_lds_kb must be 8, 16, 32 or 64");
     if (sc.edt_grid_per_cu < 0 || sc.edt_grid_per_cu > 64 ||
         sc.thin_grid_per_cu < 0 || sc.thin_grid_per_cu > 64)
         return vc_fail(ctx, VC_ERR_INVALID, "grid_per_cu must be in [0, 64]");
