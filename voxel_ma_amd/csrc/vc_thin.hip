@@ -976,6 +976,9 @@ constexpr int kTWaves = VC_TILE_WAVES;
 constexpr int kTQ = 1024;              // candidate queue entries per round
 // rows of one parity evaluated per pass: at most 23 x 19 (pass 0)
 constexpr int kTRowsPerThread = (23 * 19 + kTBlock - 1) / kTBlock;
+// a row holds at most 34 candidates of one z parity (68-bit window), and the
+// scan's ballot prefix takes per-thread counts below 64
+static_assert(kTRowsPerThread * 34 < 64, "tile_scan counts fit 6 bits");
 __device__ __forceinline__ int fy_of(int p) { return (7 - p) >> 1; }
 
 // window of word (o, w): bits 62..63 of word w-1, word w, bits 0..1 of w+1
@@ -1073,32 +1076,38 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// exclusive prefix sum over a tile-sweep workgroup (kTBlock threads)
+// exclusive prefix sum over a tile-sweep workgroup (kTBlock threads) of
+// v < 64: the wavefront's prefix from one ballot per bit plane of v (no
+// cross-lane data movement), then the wave totals through wsum (kTBlock/64
+// words; callers alternate two arrays so that no trailing barrier is needed
+// before the next scan)
 __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t* wsum,
                                               uint32_t* total)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint32_t x = v;
+    const int wv = threadIdx.x >> 6;
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1)
+    for (int bit = 0; bit < 6; ++bit)
     {
-        const uint32_t t = __shfl_up(x, d);
-        if (lane >= d)
-            x += t;
+        const uint64_t m = __ballot((v >> bit) & 1u);
+        pre += __builtin_amdgcn_mbcnt_hi(
+                   (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
+               << bit;
+        tot += (uint32_t)__popcll(m) << bit;
     }
-    if (lane == 63)
-        wsum[wv] = x;
+    if ((threadIdx.x & 63) == 0)
+        wsum[wv] = tot;
     __syncthreads();
-    uint32_t base = 0, tot = 0;
+    uint32_t base = 0, all = 0;
 #pragma unroll
     for (int i = 0; i < kTBlock / 64; ++i)
     {
         const uint32_t t = wsum[i];
         base += i < wv ? t : 0u;
-        tot += t;
+        all += t;
     }
-    *total = tot;
-    return base + x - v;
+    *total = all;
+    return base + pre;
 }
 
 // One sweep over the visited tiles.  Each pass: the threads filter their
@@ -1121,7 +1130,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
     __shared__ unsigned long long SElo[kTRows]; // eligible (constant in a level)
     __shared__ uint8_t SEhi[kTRows];
     __shared__ uint32_t tq[kTQ];
-    __shared__ uint32_t red[kTBlock / 64];
+    __shared__ uint32_t red[2 * (kTBlock / 64)]; // scan totals, by pass parity
     __shared__ uint32_t snbr; // neighbour mask of this tile's removals
     // XCD-mapped: the tiles of one row column (consecutive items) and their
     // neighbours share rows, so they go to the same XCD's L2
@@ -1241,7 +1250,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                 cnt += (uint32_t)(__popcll(cl) + __popc(ch));
             }
             uint32_t total;
-            uint32_t idx = tile_scan(cnt, red, &total);
+            uint32_t idx = tile_scan(cnt, red + (p & 1) * (kTBlock / 64), &total);
             ntest += total;
             for (uint32_t c0 = 0; c0 < total; c0 += kTQ)
             {
@@ -1312,7 +1321,9 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                 }
                 __syncthreads();
             }
-            __syncthreads(); // red is rewritten by the next pass's scan
+            // no barrier here: a pass with candidates ended with its last
+            // round's barrier (after the atomics the next pass reads); one
+            // without changed nothing, and the next scan uses the other red
         }
         for (int q = threadIdx.x; q < kTT * kTT; q += kTBlock)
         {
