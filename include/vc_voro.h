@@ -25,6 +25,9 @@
  *                        and of the TetGen loader (src/voroinfo.cpp:290-325,
  *                        :404-441)
  *   vc_voro_part_inside  VoroInfo::getPartInside (src/voroinfo.cpp:573-620)
+ *   vc_voro_from_ft      computeVD's TetGen Voronoi diagram of the boundary
+ *                        sites (src/highlevelalgo.cpp:487-529), as the
+ *                        discrete diagram of the feature transform
  */
 #ifndef VC_VORO_H
 #define VC_VORO_H
@@ -134,6 +137,34 @@ int vc_voro_part_inside(int32_t nv, int32_t nf, const int64_t* face_off,
                         int32_t* vts_idx, int64_t vts_cap, int64_t* n_vts,
                         int32_t* edges_out, int64_t edge_cap, int64_t* n_edges,
                         uint32_t* tris, int64_t tri_cap, int64_t* n_tris);
+
+/* Discrete Voronoi diagram of the background voxels, restricted to the
+ * object, from the feature transform (vc_edt must have run on vol): the
+ * voxel-core construction that replaces TetGen in computeVD
+ * (src/highlevelalgo.cpp:487-529, src/voroinfo.cpp:113-325), whose output
+ * feeds the steps above unchanged.  Every pair of 6-adjacent inside voxels
+ * p, q = p + e_a whose nearest background voxels (feature transform, lowest
+ * linear index among the nearest) differ contributes the voxel face between
+ * them, a quad on the bisector region of the two sites.
+ *   faces       4 int32 vertex ids per face; faces ordered by p (linear
+ *               index), then axis x, y, z; corners (p_b + db, p_c + dc) of
+ *               the face at p_a + 1/2, b = a+1, c = a+2 (mod 3), in the order
+ *               (0,0), (1,0), (1,1), (0,1): counter-clockwise seen from +a;
+ *   face_sites  2 int32 site ids per face: p's site, then q's;
+ *   vts         3 floats per vertex: the face corners used, numbered in
+ *               corner-lattice order ((i, j, k) at (i-1/2, j-1/2, k-1/2),
+ *               i slowest), in voxel coordinates;
+ *   sites       3 floats per site: the background voxels that are a face
+ *               site, numbered in linear order (voxel centres).
+ * Lambda of a face is |s1 - s2| (vc_voro_measures).  Host outputs; with any
+ * of vts / faces / face_sites / sites NULL only the counts are written (size
+ * query); VC_ERR_CAPACITY with the counts written when a capacity is short;
+ * VC_ERR_STATE before vc_edt; VC_ERR_INVALID when 3 * voxels reaches 2^32 or
+ * the corner lattice 2^31 points.  Runs on the device; waits. */
+int vc_voro_from_ft(vc_ctx* ctx, vc_vol* vol, float* vts, int64_t vts_cap,
+                    int64_t* n_vts, int32_t* faces, int32_t* face_sites,
+                    int64_t face_cap, int64_t* n_faces, float* sites,
+                    int64_t site_cap, int64_t* n_sites);
 
 #ifdef __cplusplus
 }

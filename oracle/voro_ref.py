@@ -17,7 +17,13 @@ float32 arithmetic:
                     in order, each vertex takes its distance to the face's
                     first site, the last assignment stays
   part_inside       getPartInside (src/voroinfo.cpp:573-620)
-Pinned by hand-derived known answers in tests/test_voro.py.
+  voronoi_from_ft   the discrete Voronoi diagram of the feature transform
+                    (the construction that replaces computeVD's TetGen step,
+                    src/highlevelalgo.cpp:487-529): a face per pair of
+                    6-adjacent inside voxels with different nearest
+                    background voxels
+Pinned by hand-derived known answers in tests/test_voro.py; voronoi_from_ft
+also against a brute-force nearest-background search and a per-voxel loop.
 """
 import collections
 
@@ -219,3 +225,58 @@ def part_inside(nv, faces_vts, v_valid):
                     a, b = c[k], c[(k + 1) % 3]
                     es.add((min(a, b), max(a, b)))
     return vi, sorted(es), tris
+
+
+def voronoi_from_ft(occ, ft):
+    """occ: (nx, ny, nz) uint8; ft: the same shape, linear index of each
+    voxel's nearest background voxel (lowest index among the nearest).
+    Returns (vts (V, 3) float32, faces (F, 4) int32, face_sites (F, 2)
+    int32, sites (S, 3) float32) in the order of vc_voro_from_ft: faces by
+    lower voxel (linear index) then axis; corners (0,0), (1,0), (1,1), (0,1)
+    over the axes a+1, a+2; vertices in corner-lattice order; sites in
+    linear order."""
+    occ = np.asarray(occ) != 0
+    ft = np.asarray(ft, np.int64)
+    nx, ny, nz = occ.shape
+    shape = np.array(occ.shape)
+    rec = []  # (p linear, axis, corner lattice ids (4), site p, site q)
+    for a in range(3):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[a] = slice(0, shape[a] - 1)
+        hi[a] = slice(1, shape[a])
+        m = occ[tuple(lo)] & occ[tuple(hi)] & (ft[tuple(lo)] != ft[tuple(hi)])
+        p = np.argwhere(m)  # voxel coordinates of the lower voxel, C order
+        if not len(p):
+            continue
+        b, c = (a + 1) % 3, (a + 2) % 3
+        corners = []
+        for db, dc in ((0, 0), (1, 0), (1, 1), (0, 1)):
+            q = p.copy()
+            q[:, a] += 1
+            q[:, b] += db
+            q[:, c] += dc
+            corners.append((q[:, 0] * (ny + 1) + q[:, 1]) * (nz + 1) + q[:, 2])
+        lin = (p[:, 0] * ny + p[:, 1]) * nz + p[:, 2]
+        qn = p.copy()
+        qn[:, a] += 1
+        lin_q = (qn[:, 0] * ny + qn[:, 1]) * nz + qn[:, 2]
+        rec.append((lin * 3 + a, np.stack(corners, 1),
+                    ft.reshape(-1)[lin], ft.reshape(-1)[lin_q]))
+    if not rec:
+        return (np.zeros((0, 3), np.float32), np.zeros((0, 4), np.int32),
+                np.zeros((0, 2), np.int32), np.zeros((0, 3), np.float32))
+    key = np.concatenate([r[0] for r in rec])
+    order = np.argsort(key, kind="stable")
+    corners = np.concatenate([r[1] for r in rec])[order]
+    s_p = np.concatenate([r[2] for r in rec])[order]
+    s_q = np.concatenate([r[3] for r in rec])[order]
+    lat, faces = np.unique(corners.reshape(-1), return_inverse=True)
+    faces = faces.reshape(-1, 4).astype(np.int32)
+    k = lat % (nz + 1)
+    r = lat // (nz + 1)
+    vts = np.stack([r // (ny + 1), r % (ny + 1), k], 1).astype(np.float32) - np.float32(0.5)
+    used, sid = np.unique(np.concatenate([s_p, s_q]), return_inverse=True)
+    face_sites = np.stack([sid[:len(s_p)], sid[len(s_p):]], 1).astype(np.int32)
+    sites = np.stack([used // (ny * nz), (used // nz) % ny, used % nz], 1).astype(np.float32)
+    return vts, faces, face_sites, sites

@@ -91,3 +91,66 @@ def test_cli_vol2skel_matches_oracle(tmp_path, oracle):
             assert fh.read() == export_ref.radii(exp, rad)
         with open("%s_thinned%g.ma" % (prefix, t), "rb") as fh:
             assert fh.read() == export_ref.dotma(exp, rad)
+
+
+def _read_ply(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    counts = {}
+    for l in lines:
+        if l.startswith("element "):
+            _, name, n = l.split()
+            counts[name] = int(n)
+        if l == "end_header":
+            break
+    body = lines[lines.index("end_header") + 1:]
+    nv, ne, nf = counts["vertex"], counts["edge"], counts["face"]
+    v = np.array([list(map(float, l.split())) for l in body[:nv]], np.float32).reshape(-1, 3)
+    e = np.array([list(map(int, l.split())) for l in body[nv:nv + ne]], np.int64).reshape(-1, 2)
+    t = np.array([list(map(int, l.split()))[1:] for l in body[nv + ne:nv + ne + nf]],
+                 np.int64).reshape(-1, 3)
+    return v, e, t
+
+
+def _chi(v, e, t):
+    sides = {tuple(sorted(p)) for p in e.tolist()}
+    for a, b, c in t.tolist():
+        sides |= {tuple(sorted((a, b))), tuple(sorted((b, c))), tuple(sorted((c, a)))}
+    return len(v) - len(sides) + len(t)
+
+
+def test_cli_vol2ma_voxel_core(tmp_path, vc):
+    """-md=vol2ma on a ball: the discrete Voronoi diagram of the feature
+    transform, tagged, merged, its inside part written in full and
+    FIFO-pruned per threshold.  The full complex holds exactly the valid
+    vertices and faces (all four corners inside) of vc_voro_from_ft, each
+    quad as two triangles; a collapse is a homotopy equivalence, so every
+    pruned complex has the full complex's Euler characteristic."""
+    import voxel_ma_amd as V
+    n = 24
+    g = np.indices((n, n, n)) - (n - 1) / 2.0
+    occ = ((g ** 2).sum(0) <= 9.0 ** 2).astype(np.uint8)
+    raw = str(tmp_path / "ball.raw")
+    occ.tofile(raw)
+    out = str(tmp_path / "ball.ply")
+    say("cli vol2ma ball")
+    r = subprocess.run([CLI, "-md=vol2ma", raw, out, "-dims=%d,%d,%d" % (n, n, n),
+                        "-tt=0.04,0.2", "-fullOrPruned=2", "-outToDotma"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vol = vc.upload(occ)
+    vol.edt()
+    vts, faces, fs, sites = V.voro_from_ft(vol)
+    valid = V.voro_tag_points(vol, vts).astype(bool)
+    vol.release()
+    vf = faces[valid[faces].all(1)]
+    v, e, t = _read_ply(out)
+    assert len(v) == int(valid.sum()) and len(t) == 2 * len(vf)
+    assert np.array_equal(np.sort(v, axis=0), np.sort(vts[valid], axis=0))
+    chi_full = _chi(v, e, t)
+    for tt in ("0.04", "0.2"):
+        vt, et, tt_ = _read_ply(str(tmp_path / ("ball_thinned%s.ply" % tt)))
+        assert 0 < len(vt) <= len(v)
+        assert _chi(vt, et, tt_) == chi_full, tt
+        assert os.path.exists(str(tmp_path / ("ball_thinned%s.r" % tt)))
+        assert os.path.exists(str(tmp_path / ("ball_thinned%s.ma" % tt)))

@@ -82,3 +82,58 @@ def test_radii_match_oracle(vc):
     assert got[0] == np.float32(np.sqrt(np.float32(41))) and got[1] == 13.0
     assert got[2] == 4.0 and got[3] == 0.0
     assert len(V.voro_radii(vc, [], [], [], [(0, 0, 0)])) == 0
+
+
+def test_voronoi_from_ft_matches_oracle(vc, oracle):
+    """vc_voro_from_ft against the oracle's discrete Voronoi diagram of the
+    oracle feature transform: vertices, faces, face sites and sites
+    bit-exact, on random volumes (odd shapes, sparse and dense), a torus and
+    a full block (no background: no face)."""
+    from voxel_ma_amd import synth
+    rng = np.random.default_rng(31)
+    cases = []
+    for i in range(8):
+        shape = tuple(int(s) for s in rng.integers(1, 24, size=3))
+        cases.append((rng.random(shape) < rng.choice([0.3, 0.6, 0.9])).astype(np.uint8))
+    cases.append(synth.torus_c1())
+    cases.append(np.ones((5, 6, 7), np.uint8))
+    cases.append(np.zeros((4, 4, 4), np.uint8))
+    for i, occ in enumerate(cases):
+        say("voronoi from ft", i, occ.shape)
+        vol = vc.upload(occ)
+        vol.edt()
+        got = V.voro_from_ft(vol)
+        _, ft = oracle.edt_ft(occ)
+        want = R.voronoi_from_ft(occ, ft)
+        for name, g, w in zip(("vts", "faces", "face_sites", "sites"), got, want):
+            assert g.shape == w.shape and np.array_equal(g, w), (i, name)
+        vol.release()
+
+
+def test_voronoi_from_ft_errors_and_measures(vc):
+    """Before vc_edt the call fails with VC_ERR_STATE; short capacities give
+    VC_ERR_CAPACITY with the counts; the two sites of a face are different
+    voxel centres, so its lambda |s1 - s2| is at least 1."""
+    import ctypes
+    rng = np.random.default_rng(32)
+    occ = (rng.random((20, 18, 22)) < 0.7).astype(np.uint8)
+    vol = vc.upload(occ)
+    with pytest.raises(V.VcError):
+        V.voro_from_ft(vol)
+    vol.edt()
+    vts, faces, fs, sites = V.voro_from_ft(vol)
+    assert len(faces) > 0
+    L = V.library()
+    nv, nf, ns = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    bv = np.zeros((len(vts), 3), np.float32)
+    bf = np.zeros((len(faces), 4), np.int32)
+    bs = np.zeros((len(faces), 2), np.int32)
+    bst = np.zeros((len(sites), 3), np.float32)
+    rc = L.vc_voro_from_ft(vc.handle, vol.handle, bv.ctypes.data, len(vts), ctypes.byref(nv),
+                           bf.ctypes.data, bs.ctypes.data, len(faces) - 1, ctypes.byref(nf),
+                           bst.ctypes.data, len(sites), ctypes.byref(ns))
+    assert rc == V.VC_ERR_CAPACITY
+    assert (nv.value, nf.value, ns.value) == (len(vts), len(faces), len(sites))
+    lam_f = np.linalg.norm(sites[fs[:, 0]] - sites[fs[:, 1]], axis=1)
+    assert np.all(lam_f >= 1.0)
+    vol.release()

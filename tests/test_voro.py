@@ -161,3 +161,90 @@ def test_radii_oracle_known_answer():
     assert r[2] == 4.0 and r[3] == 0.0
     r2 = R.radii(vts, faces, fs, sites)
     assert r2[1] == np.float32(np.sqrt(np.float32(9 + 16 + 64)))
+
+
+# ---- discrete Voronoi diagram of the feature transform (oracle pinning)
+
+def _brute_ft(occ):
+    """Nearest background voxel of every voxel by exhaustive search: squared
+    distance to every background voxel, first (lowest linear index) minimum;
+    -1 when there is no background."""
+    shape = occ.shape
+    coords = np.indices(shape).reshape(3, -1).T
+    bg = np.flatnonzero(occ.reshape(-1) == 0)
+    if not len(bg):
+        return np.full(shape, -1, np.int64)
+    d = ((coords[:, None, :] - coords[bg][None, :, :]) ** 2).sum(-1)
+    return bg[np.argmin(d, axis=1)].reshape(shape)
+
+
+def _loop_voronoi(occ, ft):
+    """Per-voxel loop restatement: faces as corner coordinate tuples, then
+    vertices numbered by sorting the distinct corners."""
+    nx, ny, nz = occ.shape
+    faces, fsites = [], []
+    for x in range(nx):
+        for y in range(ny):
+            for z in range(nz):
+                if not occ[x, y, z]:
+                    continue
+                for a, (dx, dy, dz) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    q = (x + dx, y + dy, z + dz)
+                    if q[a] >= occ.shape[a] or not occ[q] or ft[q] == ft[x, y, z]:
+                        continue
+                    b, c = (a + 1) % 3, (a + 2) % 3
+                    quad = []
+                    for db, dc in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        v = [x - 0.5, y - 0.5, z - 0.5]
+                        v[a] += 1
+                        v[b] += db
+                        v[c] += dc
+                        quad.append(tuple(v))
+                    faces.append(quad)
+                    fsites.append((int(ft[x, y, z]), int(ft[q])))
+    corners = sorted({v for f in faces for v in f})
+    vid = {v: i for i, v in enumerate(corners)}
+    used = sorted({s for p in fsites for s in p})
+    sid = {s: i for i, s in enumerate(used)}
+    sites = [(s // (ny * nz), (s // nz) % ny, s % nz) for s in used]
+    return (np.array(corners, np.float32).reshape(-1, 3),
+            np.array([[vid[v] for v in f] for f in faces], np.int32).reshape(-1, 4),
+            np.array([[sid[a], sid[b]] for a, b in fsites], np.int32).reshape(-1, 2),
+            np.array(sites, np.float32).reshape(-1, 3))
+
+
+def test_voronoi_from_ft_known_answer():
+    """Two inside voxels (1,1,1), (1,1,2) in a 3 x 3 x 4 grid: each has five
+    background face neighbours at distance 1, the lowest-index ones are
+    (0,1,1) (index 5) and (0,1,2) (index 6), so the pair is split by one face
+    at z = 1.5 with corners (x, y) = (0.5, 0.5), (1.5, 0.5), (1.5, 1.5),
+    (0.5, 1.5); lattice points (1,1,2), (1,2,2), (2,1,2), (2,2,2) number
+    0..3, so the quad is [0, 2, 3, 1]."""
+    occ = np.zeros((3, 3, 4), np.uint8)
+    occ[1, 1, 1] = occ[1, 1, 2] = 1
+    ft = _brute_ft(occ)
+    assert ft[1, 1, 1] == 5 and ft[1, 1, 2] == 6
+    vts, faces, fs, sites = R.voronoi_from_ft(occ, ft)
+    assert vts.tolist() == [[0.5, 0.5, 1.5], [0.5, 1.5, 1.5], [1.5, 0.5, 1.5],
+                            [1.5, 1.5, 1.5]]
+    assert faces.tolist() == [[0, 2, 3, 1]]
+    assert fs.tolist() == [[0, 1]]
+    assert sites.tolist() == [[0, 1, 1], [0, 1, 2]]
+
+
+def test_voronoi_from_ft_matches_brute_force():
+    """The oracle against a per-voxel loop on a brute-force feature
+    transform, on small random volumes (both empty-face and dense cases)."""
+    rng = np.random.default_rng(12)
+    for i in range(12):
+        shape = tuple(int(s) for s in rng.integers(1, 7, size=3))
+        occ = (rng.random(shape) < rng.choice([0.3, 0.7, 0.9, 1.0])).astype(np.uint8)
+        ft = _brute_ft(occ)
+        got = R.voronoi_from_ft(occ, ft)
+        want = _loop_voronoi(occ, ft)
+        for g, w in zip(got, want):
+            assert g.shape == w.shape and np.array_equal(g, w), (i, shape)
+        # every face separates two different sites; every vertex is used
+        if len(got[1]):
+            assert np.all(got[2][:, 0] != got[2][:, 1])
+            assert set(np.unique(got[1])) == set(range(len(got[0])))

@@ -48,7 +48,7 @@ EXPORTS = [
     "vc_surrogate_measure", "vc_surrogate_activation",
     # include/vc_voro.h
     "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside", "vc_voro_merge_close",
-    "vc_voro_radii", "vc_voro_part_inside",
+    "vc_voro_radii", "vc_voro_part_inside", "vc_voro_from_ft",
     # include/vc_octree.h
     "vc_octree_parse", "vc_octree_free", "vc_octree_value", "vc_volume_from_octree",
     "vc_octree_leaf_centers",
@@ -187,6 +187,8 @@ def library():
         "vc_voro_merge_close": [I, I, I, P, P, P, P, P, P, P, ctypes.c_float, I, P, P,
                                 ctypes.POINTER(I), P, P, P, P, I64, ctypes.POINTER(I64), P],
         "vc_voro_radii": [P, I, I, P, P, P, P, I, P, P],
+        "vc_voro_from_ft": [P, P, P, I64, ctypes.POINTER(I64), P, P, I64,
+                            ctypes.POINTER(I64), P, I64, ctypes.POINTER(I64)],
         "vc_voro_part_inside": [I, I, P, P, P, P, I64, ctypes.POINTER(I64), P, I64,
                                 ctypes.POINTER(I64), P, I64, ctypes.POINTER(I64)],
         "vc_write_ply": [ctypes.c_char_p, P, I64, P, I64, P, I64],
@@ -784,6 +786,25 @@ def voro_radii(ctx, vts, faces_vts, face_sites, sites):
     ctx.check(library().vc_voro_radii(ctx.handle, nv, len(faces_vts), _p(vts), _p(foff), _p(fv),
                                       _p(st1), len(st), _p(fs), _p(r)))
     return r[:nv]
+
+
+def voro_from_ft(vol):
+    """vc_voro_from_ft: the discrete Voronoi diagram of the feature transform
+    (vol.edt() first): (vts (V, 3) float32, faces (F, 4) int32, face_sites
+    (F, 2) int32, sites (S, 3) float32)."""
+    ctx = vol.ctx
+    L = library()
+    nv, nf, ns = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    ctx.check(L.vc_voro_from_ft(ctx.handle, vol.handle, None, 0, ctypes.byref(nv), None,
+                                None, 0, ctypes.byref(nf), None, 0, ctypes.byref(ns)))
+    vts = np.zeros((max(nv.value, 1), 3), np.float32)
+    faces = np.zeros((max(nf.value, 1), 4), np.int32)
+    fs = np.zeros((max(nf.value, 1), 2), np.int32)
+    sites = np.zeros((max(ns.value, 1), 3), np.float32)
+    ctx.check(L.vc_voro_from_ft(ctx.handle, vol.handle, _p(vts), len(vts), ctypes.byref(nv),
+                                _p(faces), _p(fs), len(faces), ctypes.byref(nf), _p(sites),
+                                len(sites), ctypes.byref(ns)))
+    return vts[:nv.value], faces[:nf.value], fs[:nf.value], sites[:ns.value]
 
 
 def voro_part_inside(nv, faces_vts, v_valid):

@@ -957,6 +957,11 @@ int scan_inplace(vc_ctx* ctx, uint32_t* a, int64_t n)
 
 } // namespace
 
+int vc_scan_u32(vc_ctx* ctx, uint32_t* a, int64_t n)
+{
+    return scan_inplace(ctx, a, n);
+}
+
 int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                     int64_t* n_sites, int32_t* quads, int64_t quad_cap,
                     int64_t* n_quads, vc_euler* eu)

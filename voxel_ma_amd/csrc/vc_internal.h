@@ -218,6 +218,9 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* vol, float* sites, int64_t site_cap,
                     int64_t* n_sites, int32_t* quads, int64_t quad_cap,
                     int64_t* n_quads, vc_euler* eu);
 int vc_ft_to_linear(vc_ctx* ctx, vc_vol* vol, int32_t* d_dst);
+// exclusive scan of n uint32 in device memory, in place, on the context
+// stream (waits)
+int vc_scan_u32(vc_ctx* ctx, uint32_t* a, int64_t n);
 
 // host helpers
 int64_t vc_level_threshold(float level);

@@ -3,7 +3,9 @@
 //
 // Both are gathers: one thread per point / face / edge / vertex, the measure
 // of an edge or vertex a maximum over its CSR neighbourhood.
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "vc_internal.h"
@@ -167,6 +169,199 @@ struct Scratch
         return rc;
     }
 };
+
+// ---- discrete Voronoi 2-complex from the feature transform
+// A face for every pair of 6-adjacent inside voxels p, q = p + e_a whose
+// nearest background voxels differ; faces ordered by p (linear), then axis
+// x, y, z.  The face lies at lattice coordinate p_a + 1 along axis a; its
+// corners are the lattice points (p_b + db, p_c + dc) in the cyclic order
+// (0,0), (1,0), (1,1), (0,1) of the two other axes b = a+1, c = a+2 (mod 3),
+// so every face is counter-clockwise seen from +a.  Lattice point (i, j, k)
+// is the voxel corner at (i - 1/2, j - 1/2, k - 1/2).
+struct FtGrid
+{
+    const uint8_t* occ;
+    const uint32_t* ft;
+    int nx, ny, nz;
+    vc_pack pk;
+};
+
+__device__ __forceinline__ uint32_t vft_axes(const FtGrid& G, int x, int y,
+                                             int z, int64_t i)
+{
+    if (!G.occ[i])
+        return 0u;
+    const uint32_t f = G.ft[i];
+    uint32_t m = 0;
+    const int64_t sy = G.nz, sx = (int64_t)G.ny * G.nz;
+    if (x + 1 < G.nx && G.occ[i + sx] && G.ft[i + sx] != f)
+        m |= 1u;
+    if (y + 1 < G.ny && G.occ[i + sy] && G.ft[i + sy] != f)
+        m |= 2u;
+    if (z + 1 < G.nz && G.occ[i + 1] && G.ft[i + 1] != f)
+        m |= 4u;
+    return m;
+}
+
+__device__ __forceinline__ int64_t vft_site(const FtGrid& G, uint32_t f)
+{
+    int x, y, z;
+    vc_unpack_xyz(G.pk, f, x, y, z);
+    return ((int64_t)x * G.ny + y) * G.nz + z;
+}
+
+// corner lattice index of corner q (0..3) of the face of voxel (x, y, z)
+// towards +a
+__device__ __forceinline__ int64_t vft_corner(const FtGrid& G, int x, int y,
+                                              int z, int a, int q)
+{
+    int c[3] = {x, y, z};
+    c[a] += 1;
+    c[(a + 1) % 3] += (q == 1 || q == 2) ? 1 : 0;
+    c[(a + 2) % 3] += (q >= 2) ? 1 : 0;
+    return ((int64_t)c[0] * (G.ny + 1) + c[1]) * (G.nz + 1) + c[2];
+}
+
+__device__ __forceinline__ void vft_xyz(const FtGrid& G, int64_t i, int& x,
+                                        int& y, int& z)
+{
+    z = (int)(i % G.nz);
+    const int64_t r = i / G.nz;
+    y = (int)(r % G.ny);
+    x = (int)(r / G.ny);
+}
+
+// pass 1: faces per 256-voxel tile; flags the used lattice points and sites
+__global__ void __launch_bounds__(kBlock)
+    vft_count(FtGrid G, int64_t N, uint32_t* __restrict__ tile_n,
+              uint32_t* __restrict__ lat_flag, uint32_t* __restrict__ site_flag)
+{
+    __shared__ uint32_t red[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t m = 0;
+    if (i < N)
+    {
+        int x, y, z;
+        vft_xyz(G, i, x, y, z);
+        m = vft_axes(G, x, y, z, i);
+        if (m)
+        {
+            site_flag[vft_site(G, G.ft[i])] = 1u;
+            const int64_t sy = G.nz, sx = (int64_t)G.ny * G.nz;
+            const int64_t nb[3] = {i + sx, i + sy, i + 1};
+            for (int a = 0; a < 3; ++a)
+            {
+                if (!((m >> a) & 1u))
+                    continue;
+                site_flag[vft_site(G, G.ft[nb[a]])] = 1u;
+                for (int q = 0; q < 4; ++q)
+                    lat_flag[vft_corner(G, x, y, z, a, q)] = 1u;
+            }
+        }
+    }
+    uint32_t c = (uint32_t)__popc(m);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        c += __shfl_xor(c, d);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            t += red[w];
+        tile_n[blockIdx.x] = t;
+    }
+}
+
+// pass 2: faces (4 lattice ids through the scanned lattice flags) and their
+// two site ids (scanned site flags), at the tile offset + the voxel's rank
+__global__ void __launch_bounds__(kBlock)
+    vft_emit(FtGrid G, int64_t N, const uint32_t* __restrict__ tile_off,
+             const uint32_t* __restrict__ lat_id,
+             const uint32_t* __restrict__ site_id, int32_t* __restrict__ faces,
+             int32_t* __restrict__ face_sites)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t m = 0;
+    int x = 0, y = 0, z = 0;
+    if (i < N)
+    {
+        vft_xyz(G, i, x, y, z);
+        m = vft_axes(G, x, y, z, i);
+    }
+    // exclusive prefix of the per-voxel counts (0..3) over the tile, from
+    // one ballot per bit plane
+    const uint32_t c = (uint32_t)__popc(m);
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; bit < 2; ++bit)
+    {
+        const uint64_t b = __ballot((c >> bit) & 1u);
+        pre += __builtin_amdgcn_mbcnt_hi(
+                   (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))
+               << bit;
+        tot += (uint32_t)__popcll(b) << bit;
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        wsum[wv] = tot;
+    __syncthreads();
+    for (int w = 0; w < wv; ++w)
+        pre += wsum[w];
+    if (!m)
+        return;
+    int64_t f = (int64_t)tile_off[blockIdx.x] + pre;
+    const int64_t sy = G.nz, sx = (int64_t)G.ny * G.nz;
+    const int64_t nb[3] = {i + sx, i + sy, i + 1};
+    const int32_t s0 = (int32_t)site_id[vft_site(G, G.ft[i])];
+    for (int a = 0; a < 3; ++a)
+    {
+        if (!((m >> a) & 1u))
+            continue;
+        for (int q = 0; q < 4; ++q)
+            faces[4 * f + q] = (int32_t)lat_id[vft_corner(G, x, y, z, a, q)];
+        face_sites[2 * f] = s0;
+        face_sites[2 * f + 1] = (int32_t)site_id[vft_site(G, G.ft[nb[a]])];
+        ++f;
+    }
+}
+
+// pass 3: coordinates of the used lattice points (scanned flags: point l is
+// used iff id[l + 1] - id[l] == 1)
+__global__ void __launch_bounds__(kBlock)
+    vft_vertices(int nx, int ny, int nz, const uint32_t* __restrict__ lat_id,
+                 float* __restrict__ vts)
+{
+    const int64_t L = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= L || lat_id[l + 1] == lat_id[l])
+        return;
+    const int k = (int)(l % (nz + 1));
+    const int64_t r = l / (nz + 1);
+    const int j = (int)(r % (ny + 1)), ii = (int)(r / (ny + 1));
+    const int64_t v = lat_id[l];
+    vts[3 * v] = (float)ii - 0.5f;
+    vts[3 * v + 1] = (float)j - 0.5f;
+    vts[3 * v + 2] = (float)k - 0.5f;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    vft_sites(FtGrid G, int64_t N, const uint32_t* __restrict__ site_id,
+              float* __restrict__ sites)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N || site_id[i + 1] == site_id[i])
+        return;
+    int x, y, z;
+    vft_xyz(G, i, x, y, z);
+    const int64_t s = site_id[i];
+    sites[3 * s] = (float)x;
+    sites[3 * s + 1] = (float)y;
+    sites[3 * s + 2] = (float)z;
+}
 
 } // namespace
 
@@ -372,6 +567,98 @@ int vc_voro_radii(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
     vertex_radius<<<blocks_for(nv), kBlock, 0, S>>>(dv, ds, dfs, dlast, nv, dr);
     VC_CHECK_LAUNCH(ctx);
     VC_HIP(ctx, hipMemcpyAsync(radii, dr, 4 * (size_t)nv, hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+int vc_voro_from_ft(vc_ctx* ctx, vc_vol* vol, float* vts, int64_t vts_cap,
+                    int64_t* n_vts, int32_t* faces, int32_t* face_sites,
+                    int64_t face_cap, int64_t* n_faces, float* sites,
+                    int64_t site_cap, int64_t* n_sites)
+{
+    if (!ctx || !vol || vol->ctx != ctx || !n_vts || !n_faces || !n_sites ||
+        vts_cap < 0 || face_cap < 0 || site_cap < 0)
+        return VC_ERR_INVALID;
+    if (!vol->edt_done)
+        return vc_fail(ctx, VC_ERR_STATE, "vc_voro_from_ft: run vc_edt first");
+    const int64_t N = vol->N;
+    const int64_t L = (int64_t)(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1);
+    // face counts are scanned in uint32 (at most 3 per voxel), vertex ids are
+    // int32
+    if (3 * N >= (int64_t)UINT32_MAX || L >= (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID,
+                       "vc_voro_from_ft: volume too large (%lld voxels)",
+                       (long long)N);
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    const int64_t tiles = (N + kBlock - 1) / kBlock;
+    uint32_t *tile = nullptr, *lat = nullptr, *sid = nullptr;
+    int rc = sc.get(ctx, &tile, (size_t)tiles + 1);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &lat, (size_t)L + 1);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &sid, (size_t)N + 1);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipMemsetAsync(tile, 0, 4 * ((size_t)tiles + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(lat, 0, 4 * ((size_t)L + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(sid, 0, 4 * ((size_t)N + 1), S));
+    const FtGrid G{vol->occ, vol->ft, vol->nx, vol->ny, vol->nz, vol->pk};
+    vft_count<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, tile, tiles + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, lat, L + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, sid, N + 1)) != VC_OK)
+        return rc;
+    uint32_t tot[3];
+    VC_HIP(ctx, hipMemcpy(&tot[0], tile + tiles, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&tot[1], lat + L, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&tot[2], sid + N, 4, hipMemcpyDeviceToHost));
+    const int64_t F = tot[0], V = tot[1], NS = tot[2];
+    if (F > (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID,
+                       "vc_voro_from_ft: %lld faces exceed int32 ids",
+                       (long long)F);
+    *n_faces = F;
+    *n_vts = V;
+    *n_sites = NS;
+    if (!vts || !faces || !face_sites || !sites)
+        return VC_OK; // size query
+    if (V > vts_cap || F > face_cap || NS > site_cap)
+        return VC_ERR_CAPACITY;
+    int32_t *dfaces = nullptr, *dfs = nullptr;
+    float *dv = nullptr, *ds = nullptr;
+    rc = sc.get(ctx, &dfaces, 4 * (size_t)F);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dfs, 2 * (size_t)F);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &dv, 3 * (size_t)V);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &ds, 3 * (size_t)NS);
+    if (rc != VC_OK)
+        return rc;
+    vft_emit<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid, dfaces,
+                                                 dfs);
+    VC_CHECK_LAUNCH(ctx);
+    vft_vertices<<<blocks_for(L), kBlock, 0, S>>>(vol->nx, vol->ny, vol->nz,
+                                                   lat, dv);
+    VC_CHECK_LAUNCH(ctx);
+    vft_sites<<<blocks_for(N), kBlock, 0, S>>>(G, N, sid, ds);
+    VC_CHECK_LAUNCH(ctx);
+    if (F > 0)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(faces, dfaces, 16 * (size_t)F,
+                                   hipMemcpyDeviceToHost, S));
+        VC_HIP(ctx, hipMemcpyAsync(face_sites, dfs, 8 * (size_t)F,
+                                   hipMemcpyDeviceToHost, S));
+    }
+    if (V > 0)
+        VC_HIP(ctx, hipMemcpyAsync(vts, dv, 12 * (size_t)V,
+                                   hipMemcpyDeviceToHost, S));
+    if (NS > 0)
+        VC_HIP(ctx, hipMemcpyAsync(sites, ds, 12 * (size_t)NS,
+                                   hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }

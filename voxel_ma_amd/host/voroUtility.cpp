@@ -9,6 +9,13 @@
 //                                                  per threshold t:
 //                                                  <prefix>_thinned<t>.ply
 //                                                  and .r (and .ma)
+//   -md=vol2ma <in> <out.ply|out.off> [-tt=a,b,c] [-fullOrPruned=0|1|2]
+//              [-outToDotma] [-needEuler]          voxel core: the discrete
+//                                                  Voronoi diagram of the
+//                                                  feature transform (in
+//                                                  place of TetGen's), tagged,
+//                                                  merged, its inside part
+//                                                  FIFO-pruned per threshold
 //   -md=topo -meshToCheck=<m.off>                  closed pockets of a mesh
 //
 // Volumes: .mrc (modes 0, 1, 2, 6), .sof / .sog octrees, or raw uint8 with
@@ -168,6 +175,8 @@ int usage()
                  "[-dims=nx,ny,nz]\n"
                  "       voroUtility -md=vol2skel <in> <out_prefix> "
                  "[-tt=0.01,0.02,0.04] [-outToDotma] [-dims=...]\n"
+                 "       voroUtility -md=vol2ma <in> <out.ply|out.off> "
+                 "[-tt=0.04] [-fullOrPruned=0|1|2] [-outToDotma] [-needEuler]\n"
                  "       voroUtility -md=topo -meshToCheck=<mesh.off>\n";
     return 1;
 }
@@ -299,6 +308,54 @@ int main(int argc, char** argv)
                                  voxelvoro::ExportErrCode::SUCCESS)
                     return 1;
             }
+            return 0;
+        }
+        if (md == "vol2ma")
+        {
+            // src/voroUtility.cpp:454-525
+            if (pos.size() < 2)
+                return usage();
+            auto vol = load(pos[0]);
+            if (!vol)
+            {
+                std::cerr << "cannot read volume " << pos[0] << "\n";
+                return 1;
+            }
+            const bool need_euler = flag_on(flags, "needEuler", false);
+            std::vector<float> tt = {0.04f};
+            if (flags.count("tt"))
+            {
+                // the reference parses each threshold with std::stof
+                tt.clear();
+                std::stringstream ss(flags["tt"]);
+                std::string tok;
+                while (std::getline(ss, tok, ','))
+                    tt.push_back(std::stof(tok));
+            }
+            const int full_or_pruned =
+                flags.count("fullOrPruned") ? std::stoi(flags["fullOrPruned"]) : 0;
+            const auto t0 = Clock::now();
+            voxelvoro::VoroInfo voro;
+            voxelvoro::computeVD(vol, voro);
+            std::cout << "time -> discrete voronoi diagram: " << ms_since(t0)
+                      << " ms\n";
+            if (!voxelvoro::preprocessVoro(voro, vol, need_euler))
+            {
+                std::cerr << "Error processing voro!\n";
+                return 1;
+            }
+            const auto t1 = Clock::now();
+            if (voxelvoro::exportInsideVoroMesh(
+                    voro, vol, pos[1].c_str(), need_euler,
+                    flag_on(flags, "collapseZeroLenEdges", false), true, false,
+                    full_or_pruned, true, flag_on(flags, "outToDotma", false),
+                    tt) != voxelvoro::ExportErrCode::SUCCESS)
+            {
+                std::cerr << "Error: couldn't write inside voro info.\n";
+                return 1;
+            }
+            std::cout << "time -> extract + prune + write: " << ms_since(t1)
+                      << " ms\n";
             return 0;
         }
         if (md == "topo")

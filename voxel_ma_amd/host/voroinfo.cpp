@@ -207,6 +207,18 @@ bool VoroInfo::tagVtsUsingUniformVol(const std::shared_ptr<Volume3DScalar>& vol)
     return true;
 }
 
+bool VoroInfo::isFaceFinite(int fi) const
+{
+    if (onlyHasInside())
+        return true;
+    std::vector<int> poly;
+    m_geom.getFaceVRep(fi, poly);
+    for (int v : poly)
+        if (!m_is_finite_v[v])
+            return false;
+    return true;
+}
+
 float VoroInfo::getInsidePartSize() const
 {
     if (!m_bb_valid)

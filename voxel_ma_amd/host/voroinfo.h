@@ -57,6 +57,9 @@ public:
     void getValidVts(std::vector<int>& idx) const;
     void getValidEdges(std::vector<int>& idx) const;
     void getValidFaces(std::vector<int>& idx) const;
+    // include/voroinfo_imp.h:57-60: a face is finite iff all its vertices
+    // are (the diagram loader's per-face flag)
+    bool isFaceFinite(int fi) const;
 
     // src/voroinfo.cpp:1432-1574 (LAMBDA)
     void computeVertexMeasure(MeasureForMA::meassuretype tp,
