@@ -81,6 +81,7 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
     same volume: EDT + measure + levels + thinning."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_lib as O
 
     n = occ_host.shape[0]
@@ -108,12 +109,32 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
         bel = time.perf_counter() - t0
         if bel > 3.0 or breps >= 20:
             break
+    # the discrete Voronoi diagram of the feature transform (numpy oracle) on
+    # a 64^3 crop
+    import voro_ref as VR
+    vside = min(n, 64)
+    v0 = (n - vside) // 2
+    vcrop = np.ascontiguousarray(occ_host[v0:v0 + vside, v0:v0 + vside, v0:v0 + vside])
+    _, vft = O.edt_ft(vcrop)
+    t0 = time.perf_counter()
+    vreps = 0
+    while True:
+        VR.voronoi_from_ft(vcrop, vft)
+        vreps += 1
+        vel = time.perf_counter() - t0
+        if vel > 3.0 or vreps >= 10:
+            break
     return {"value": round(mvox, 3), "unit": "Mvoxels/s", "cores": 1, "kind": "port",
             "sample": "oracle (single-thread C) EDT+measure+thinning on the central %d^3 crop "
                       "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el),
             "rows": {"R3_R4_boundary": {"value": round(crop.size * breps / bel / 1e6, 3),
                                         "unit": "Mvoxels/s", "sample": "oracle boundary + "
-                                        "Euler on the same crop, %d reps" % breps}}}
+                                        "Euler on the same crop, %d reps" % breps},
+                     "voronoi_from_ft": {"value": round(vcrop.size * vreps / vel / 1e6, 3),
+                                         "unit": "Mvoxels/s",
+                                         "sample": "oracle (numpy) discrete Voronoi diagram "
+                                                   "on the central %d^3 crop, %d reps"
+                                                   % (vside, vreps)}}}
 
 
 def batch_leg(ctx, ev, steps, warmup, count=256, seed=4):
@@ -294,6 +315,24 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
                         "ms": round(t_o * 1e3, 3), "value": round(n3 / t_o / 1e6, 1),
                         "unit": "Mvoxels/s"}
     tree.free()
+
+    # the voxel-core Voronoi construction (discrete diagram of the feature
+    # transform) on the central 256^3 crop; host outputs, so PCIe included
+    side = min(n, 256)
+    c0 = (n - side) // 2
+    crop = np.ascontiguousarray(occ_host[c0:c0 + side, c0:c0 + side, c0:c0 + side])
+    vcrop = ctx.upload(crop)
+    vcrop.edt()
+    t_v, (vv, vf, vfs, vs) = best_of(lambda: V.voro_from_ft(vcrop))
+    vcrop.release()
+    out["voronoi_from_ft"] = {
+        "workload": "discrete Voronoi diagram of the feature transform, central %d^3 crop "
+                    "of the headline volume" % side,
+        "ms": round(t_v * 1e3, 2), "faces": int(len(vf)), "vertices": int(len(vv)),
+        "sites": int(len(vs)), "value": round(crop.size / t_v / 1e6, 1),
+        "unit": "Mvoxels/s", "note": "size query + fill (two passes over the volume), "
+                                     "outputs copied to host memory"}
+    del vv, vf, vfs, vs
 
     # R6/R7: tagging and lambda measures; R8/R9 extraction and merging;
     # R10-R12 finalize + prune, on a 500 x 500 grid complex
