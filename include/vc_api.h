@@ -34,6 +34,7 @@
  *                          src/highlevelalgo.cpp:815-961), on voxels
  *   vc_cc_finalize        cellcomplex::finalize (src/cellcomplex.cpp:364-491)
  *   vc_cc_prune           CellComplexThinning::prune (src/ccthin.cpp:201-771)
+ *   vc_cc_prune_device    the same, on the device
  *   vc_boundary_extract   Surfacer::extractBoundaryVts /
  *                         extractBoundaryVtsAndFaces / computeEulerChar
  *                         (src/surfacing.cpp:49-440, :619-660)
@@ -192,6 +193,20 @@ int vc_cc_prune(const vc_cc_view* cc, const float* edge_measure,
                 const float* face_measure, float f_t, float l_t,
                 uint8_t* removed_v, uint8_t* removed_e, uint8_t* removed_f,
                 int32_t* ref_e);
+
+/* vc_cc_prune on the device: the same collapse and the same outputs,
+ * computed one generation of the FIFO queue at a time (the pairs the
+ * previous generation's pops pushed, in push order), with the pops of a
+ * generation in parallel and their pushes ordered by (pop index, step).
+ * Adjacency is built on the device.  VC_ERR_STATE where vc_cc_prune
+ * returns it; VC_ERR_INVALID for ids out of range, an edge with equal ends
+ * or a face listing a side twice (vc_cc_prune handles those).  generations
+ * (may be NULL) receives the number of queue generations.  Waits. */
+int vc_cc_prune_device(vc_ctx* ctx, const vc_cc_view* cc,
+                       const float* edge_measure, const float* face_measure,
+                       float f_t, float l_t, uint8_t* removed_v,
+                       uint8_t* removed_e, uint8_t* removed_f, int32_t* ref_e,
+                       int64_t* generations);
 
 /* ---- results ------------------------------------------------------------- */
 /* which: */

@@ -36,7 +36,7 @@ EXPORTS = [
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
     "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
-    "vc_cc_prune",
+    "vc_cc_prune", "vc_cc_prune_device",
     # include/vc_batch.h
     "vc_batch_create", "vc_batch_destroy", "vc_batch_size", "vc_batch_edt", "vc_batch_measure",
     "vc_batch_thin", "vc_batch_run", "vc_schedule_defaults", "vc_batch_get_schedule",
@@ -153,6 +153,8 @@ def library():
         "vc_synth_solids": [P, P, I, I, I, P, I],
         "vc_cc_finalize": [I, I, P, I, P, P, P, I, ctypes.POINTER(I), P],
         "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
+        "vc_cc_prune_device": [P, ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float,
+                               P, P, P, P, ctypes.POINTER(ctypes.c_int64)],
         "vc_batch_create": [P, P, I, PP],
         "vc_batch_destroy": [P, P],
         "vc_batch_size": [P, ctypes.POINTER(I), ctypes.POINTER(I64)],
@@ -653,6 +655,26 @@ def cc_prune(nv, edges, foff, fe, em, fm, f_t, l_t):
     rc = library().vc_cc_prune(ctypes.byref(view), _p(em), _p(fm), float(f_t), float(l_t),
                                _p(rv), _p(re), _p(rf), _p(ref))
     return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne]
+
+
+def cc_prune_device(ctx, nv, edges, foff, fe, em, fm, f_t, l_t):
+    """The same collapse on the device (vc_cc_prune_device).  Returns (rc,
+    removed_v, removed_e, removed_f, edge ref counts, queue generations)."""
+    edges = np.ascontiguousarray(edges, np.int32)
+    foff = np.ascontiguousarray(foff, np.int64)
+    fe = np.ascontiguousarray(fe, np.int32)
+    ne, nf = len(edges), len(foff) - 1
+    em = np.ascontiguousarray(em, np.float32)
+    fm = np.ascontiguousarray(fm, np.float32)
+    rv, re, rf = (np.zeros(max(nv, 1), np.uint8), np.zeros(max(ne, 1), np.uint8),
+                  np.zeros(max(nf, 1), np.uint8))
+    ref = np.zeros(max(ne, 1), np.int32)
+    gens = ctypes.c_int64()
+    view = CcView(nv, ne, nf, edges.ctypes.data, foff.ctypes.data, fe.ctypes.data)
+    rc = library().vc_cc_prune_device(ctx.handle, ctypes.byref(view), _p(em), _p(fm),
+                                      float(f_t), float(l_t), _p(rv), _p(re), _p(rf),
+                                      _p(ref), ctypes.byref(gens))
+    return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne], gens.value
 
 
 # ---------------------------------------------------------------- Voronoi side
