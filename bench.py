@@ -264,6 +264,7 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
     take host arrays, so their figures include the PCIe copies)."""
     import numpy as np
     import voxel_ma_amd as V
+    from voxel_ma_amd import synth
     out = {}
 
     def best_of(fn, k=reps):
@@ -365,10 +366,46 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
     out["R8_R9_host"] = {"workload": "extract inside + merge close vertices on %d faces" % nf,
                          "extract_ms": round(t_x * 1e3, 1), "merge_ms": round(t_g * 1e3, 1)}
     t_f, _ = best_of(lambda: V.cc_finalize(len(vts), [], fl), 1)
-    t_p, _ = best_of(lambda: V.cc_prune(len(vts), edges, foff, fe, le, lf, 1.5, 1.5), 1)
+    t_p, rh = best_of(lambda: V.cc_prune(len(vts), edges, foff, fe, le, lf, 1.5, 1.5), 1)
+    t_pd, rd = best_of(lambda: V.cc_prune_device(ctx, len(vts), edges, foff, fe, le, lf,
+                                                 1.5, 1.5))
+    same = all(np.array_equal(a, b) for a, b in zip(rh[1:], rd[1:5]))
     out["R10_R12_cellcomplex"] = {"workload": "finalize + FIFO prune of %d quads" % nf,
                                   "finalize_ms": round(t_f * 1e3, 1),
-                                  "prune_ms": round(t_p * 1e3, 1)}
+                                  "prune_ms": round(t_p * 1e3, 1),
+                                  "prune_device_ms": round(t_pd * 1e3, 1),
+                                  "generations": int(rd[5]), "device_equals_host": same}
+
+    # the voxel core of a 128^3 crop: discrete Voronoi diagram, inside part
+    # triangulated, FIFO collapse at t = 0.04 on the host and on the device
+    side = min(n, 128)
+    c0 = (n - side) // 2
+    crop = np.ascontiguousarray(occ_host[c0:c0 + side, c0:c0 + side, c0:c0 + side])
+    vc_ = ctx.upload(crop)
+    vc_.edt()
+    cv, cf, cfs, cs = V.voro_from_ft(vc_)
+    cvalid = V.voro_tag_points(vc_, cv)
+    vc_.release()
+    clam = np.linalg.norm(cs[cfs[:, 0]] - cs[cfs[:, 1]], axis=1).astype(np.float32)
+    vi, eo, em_, tr, _, fm_ = V.voro_extract_inside(len(cv), [], cf, cvalid,
+                                                    np.zeros(0, np.float32), clam)
+    tris = np.asarray(tr, np.int32).reshape(-1, 3)
+    ce, cfo, cfe = V.cc_finalize(len(vi), np.asarray(eo, np.int32).reshape(-1, 2), tris)
+    cem = np.zeros(len(ce), np.float32)
+    cem[:len(em_)] = em_
+    cfm = np.asarray(fm_, np.float32)
+    thr = float(synth.inside_size(crop)) * 0.04
+    t_ch, rch = best_of(lambda: V.cc_prune(len(vi), ce, cfo, cfe, cem, cfm, thr, thr), 1)
+    t_cd, rcd = best_of(lambda: V.cc_prune_device(ctx, len(vi), ce, cfo, cfe, cem, cfm,
+                                                  thr, thr))
+    out["voxel_core_collapse"] = {
+        "workload": "inside part of the discrete Voronoi diagram of the central %d^3 crop "
+                    "(%d triangles, %d edges), FIFO collapse at t = 0.04" % (side, len(tris),
+                                                                            len(ce)),
+        "host_ms": round(t_ch * 1e3, 1), "device_ms": round(t_cd * 1e3, 1),
+        "generations": int(rcd[5]), "faces_removed": int(rcd[3].sum()),
+        "device_equals_host": all(np.array_equal(a, b) for a, b in zip(rch[1:], rcd[1:5])),
+        "note": "host arrays in and out; device time includes the adjacency build"}
 
     # output files: the skeleton of the headline volume as PLY and .r
     import tempfile
