@@ -361,10 +361,25 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
                           "unit": "Mfaces/s", "note": "host arrays, PCIe copies included"}
     valid = np.ones(len(vts), np.uint8)
     fl = faces
-    t_x, _ = best_of(lambda: V.voro_extract_inside(len(vts), edges, fl, valid, le, lf), 1)
-    t_g, _ = best_of(lambda: V.voro_merge_close(vts, edges, fl, valid, 0.25, True), 1)
-    out["R8_R9_host"] = {"workload": "extract inside + merge close vertices on %d faces" % nf,
-                         "extract_ms": round(t_x * 1e3, 1), "merge_ms": round(t_g * 1e3, 1)}
+    t_x, xh = best_of(lambda: V.voro_extract_inside(len(vts), edges, fl, valid, le, lf), 1)
+    t_xd, xd = best_of(lambda: V.voro_extract_inside_device(ctx, len(vts), edges, fl, valid,
+                                                            le, lf))
+    # the merge's C-ABI calls alone (arrays prepared once)
+    call_h, out_h = V.merge_close_call(vts, edges, fl, valid, 0.25, True, None)
+    call_d, out_d = V.merge_close_call(vts, edges, fl, valid, 0.25, True, ctx)
+    t_g, _ = best_of(call_h, 1)
+    t_gd, _ = best_of(call_d)
+    gh, gd = out_h(), out_d()
+    out["R8_R9"] = {"workload": "extract inside + merge close vertices on %d faces" % nf,
+                    "extract_host_ms": round(t_x * 1e3, 1),
+                    "extract_device_ms": round(t_xd * 1e3, 1),
+                    "merge_host_ms": round(t_g * 1e3, 1), "merge_device_ms": round(t_gd * 1e3, 1),
+                    "merge_on_device": bool(gd[6]),
+                    "device_equals_host": all(np.array_equal(np.asarray(a), np.asarray(b))
+                                              for a, b in zip(xh, xd)) and
+                                          all(gd[k] == gh[k] for k in range(6)),
+                    "note": "host arrays in and out; extraction through its wrapper, the merge "
+                            "as its C-ABI call alone"}
     t_f, _ = best_of(lambda: V.cc_finalize(len(vts), [], fl), 1)
     t_p, rh = best_of(lambda: V.cc_prune(len(vts), edges, foff, fe, le, lf, 1.5, 1.5), 1)
     t_pd, rd = best_of(lambda: V.cc_prune_device(ctx, len(vts), edges, foff, fe, le, lf,

@@ -19,8 +19,10 @@
  *                        (src/voroinfo.cpp:1432-1574,
  *                        include/measureforMA_imp.h:1-4)
  *   vc_voro_extract_inside  VoroInfo::extractInsideWithMeasure
- *                        (src/voroinfo.cpp:624-729)
- *   vc_voro_merge_close  VoroInfo::mergeCloseVts (src/voroinfo.cpp:1064-1384)
+ *                        (src/voroinfo.cpp:624-729); also on the device
+ *                        (vc_voro_extract_inside_device)
+ *   vc_voro_merge_close  VoroInfo::mergeCloseVts (src/voroinfo.cpp:1064-1384);
+ *                        vc_voro_merge_close_device when nothing merges
  *   vc_voro_radii        the radius step of VoroInfo::computeInfoRelatedtoSites
  *                        and of the TetGen loader (src/voroinfo.cpp:290-325,
  *                        :404-441)
@@ -86,6 +88,20 @@ int vc_voro_extract_inside(int32_t nv, int32_t ne, int32_t nf,
                            int64_t* n_edges, int32_t* tris, int32_t* from_fi,
                            float* fm, int64_t tri_cap, int64_t* n_tris);
 
+/* vc_voro_extract_inside on the device: the same outputs in the same order
+ * (valid vertices by compaction, valid edges by compaction, triangles at
+ * scanned offsets, each appended side kept iff it is the first of its exact
+ * pair in face / triangle / side order and no valid edge is stored as that
+ * pair: a hash table keeps the smallest candidate index per pair).  Host
+ * arrays; waits. */
+int vc_voro_extract_inside_device(
+    vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const int32_t* edges,
+    const int64_t* face_off, const int32_t* face_vts, const uint8_t* v_valid,
+    const float* lam_e, const float* lam_f, int32_t* vts_idx, int64_t vts_cap,
+    int64_t* n_vts, int32_t* edges_out, float* em, int64_t edge_cap,
+    int64_t* n_edges, int32_t* tris, int32_t* from_fi, float* fm,
+    int64_t tri_cap, int64_t* n_tris);
+
 /* Close-vertex merge (host): for each vertex vi in order (skipping visited,
  * and, with only_inside, invalid ones) start a new vertex and absorb by BFS
  * over the vertex-vertex adjacency (ve_off / ve in edge order) every
@@ -115,6 +131,24 @@ int vc_voro_merge_close(int32_t nv, int32_t ne, int32_t nf, const float* vts,
                         int32_t* new_face_vts, uint8_t* remove_face,
                         int32_t* new_edges, int64_t edge_cap,
                         int64_t* n_new_edges, uint8_t* keep_edge);
+
+/* vc_voro_merge_close with the work on the device when no edge joins two
+ * vertices within eps (no BFS absorbs anything: every eligible vertex is its
+ * own cluster, faces keep their distinct new ids, the new edge set is sorted
+ * and deduplicated on the device); otherwise the order-defined BFS runs on
+ * the host (vc_voro_merge_close).  Same arguments and outputs; *on_device
+ * (may be NULL) is 1 when the device path ran.  Waits. */
+int vc_voro_merge_close_device(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                               const float* vts, const int32_t* edges,
+                               const int64_t* face_off, const int32_t* face_vts,
+                               const int64_t* ve_off, const int32_t* ve,
+                               const uint8_t* v_valid, float eps, int only_inside,
+                               int32_t* old_to_new, int32_t* new_vts,
+                               int32_t* n_new_vts, int64_t* new_face_off,
+                               int32_t* new_face_vts, uint8_t* remove_face,
+                               int32_t* new_edges, int64_t edge_cap,
+                               int64_t* n_new_edges, uint8_t* keep_edge,
+                               int* on_device);
 
 /* Radius of every vertex (host arrays, computed on the device, waits): faces
  * are visited in order and every vertex v of face f is given

@@ -137,3 +137,81 @@ def test_voronoi_from_ft_errors_and_measures(vc):
     lam_f = np.linalg.norm(sites[fs[:, 0]] - sites[fs[:, 1]], axis=1)
     assert np.all(lam_f >= 1.0)
     vol.release()
+
+
+def test_extract_inside_device_matches_host(vc):
+    """vc_voro_extract_inside_device against the host extraction (pinned to
+    the oracle in tests/test_voro.py): the hand-derived square (a stored
+    edge (3,0) does not hide the side (0,3)), random polygon complexes with
+    stored edges in both orientations and random validity, and the discrete
+    Voronoi diagram of a ball."""
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0)]
+    got = V.voro_extract_inside_device(vc, 4, edges, [[0, 1, 2, 3]], [1, 1, 1, 1],
+                                       [1.0, 2.0, 3.0, 4.0], [7.0])
+    assert [tuple(e) for e in got[1]] == [(0, 1), (1, 2), (2, 3), (3, 0), (0, 3), (1, 3)]
+    assert list(got[2]) == [1.0, 2.0, 3.0, 4.0, 7.0, 7.0]
+    assert [tuple(t) for t in got[3]] == [(3, 0, 1), (3, 1, 2)]
+    rng = np.random.default_rng(41)
+    cases = []
+    for it in range(6):
+        nv, edges, faces, fs, sites = _complex(rng)
+        edges = [(b, a) if rng.random() < 0.3 else (a, b) for a, b in edges]
+        edges = [e for e in edges if rng.random() < 0.6]
+        valid = (rng.random(nv) < rng.choice([0.5, 0.9, 1.0])).astype(np.uint8)
+        cases.append((nv, edges, faces, valid, rng.random(len(edges)).astype(np.float32),
+                      rng.random(len(faces)).astype(np.float32)))
+    n = 24
+    g = np.indices((n, n, n)) - (n - 1) / 2.0
+    occ = ((g ** 2).sum(0) <= 9.5 ** 2).astype(np.uint8)
+    vol = vc.upload(occ)
+    vol.edt()
+    vts, faces, fs, sites = V.voro_from_ft(vol)
+    valid = V.voro_tag_points(vol, vts)
+    vol.release()
+    lam = np.linalg.norm(sites[fs[:, 0]] - sites[fs[:, 1]], axis=1).astype(np.float32)
+    cases.append((len(vts), [], faces, valid, np.zeros(0, np.float32), lam))
+    for i, (nv, edges, faces, valid, le, lf) in enumerate(cases):
+        say("extract inside device", i, nv, len(edges), len(faces))
+        want = V.voro_extract_inside(nv, edges, faces, valid, le, lf)
+        got = V.voro_extract_inside_device(vc, nv, edges, faces, valid, le, lf)
+        for k, (a, b) in enumerate(zip(got, want)):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), (i, k)
+
+
+def test_merge_close_device_matches_host(vc):
+    """vc_voro_merge_close_device against the host merge: random complexes
+    whose adjacent vertices are apart (the device path, with polygons that
+    repeat a vertex: deduplicated, or turned into an edge when two distinct
+    ids remain), with and without only_inside; the discrete Voronoi diagram
+    of a ball; and complexes with coincident neighbours (the host path)."""
+    rng = np.random.default_rng(52)
+    cases = []
+    for it in range(6):
+        nv, edges, faces, fs, sites = _complex(rng)
+        vts = (rng.random((nv, 3)) * 100).astype(np.float32)
+        faces = faces + [[0, 1, 0, 2], [3, 4, 3], [5, 6, 5, 6, 7]]
+        valid = (rng.random(nv) < 0.8).astype(np.uint8)
+        cases.append((vts, edges, faces, valid, 1e-3, it % 2 == 0, True))
+    n = 20
+    g = np.indices((n, n, n)) - (n - 1) / 2.0
+    occ = ((g ** 2).sum(0) <= 8.0 ** 2).astype(np.uint8)
+    vol = vc.upload(occ)
+    vol.edt()
+    vts, faces, fs, sites = V.voro_from_ft(vol)
+    valid = V.voro_tag_points(vol, vts)
+    vol.release()
+    cases.append((vts, [], [list(f) for f in faces], valid, 1e-6, True, True))
+    # coincident neighbours: vertex 1 on top of vertex 0, 5 near 4
+    vts = (rng.random((12, 3)) * 10).astype(np.float32)
+    vts[1] = vts[0]
+    vts[5] = vts[4] + np.float32(1e-5)
+    edges = [(0, 1), (1, 2), (2, 3), (4, 5), (5, 6), (0, 7), (8, 9), (10, 11)]
+    faces = [[0, 1, 2], [4, 5, 6, 3], [7, 8, 9, 10]]
+    cases.append((vts, edges, faces, np.ones(12, np.uint8), 1e-3, False, False))
+    for i, (vts, edges, faces, valid, eps, oi, dev_expected) in enumerate(cases):
+        say("merge close device", i, len(vts), len(edges), len(faces), oi)
+        want = V.voro_merge_close(vts, edges, faces, valid, eps, oi)
+        got = V.voro_merge_close_device(vc, vts, edges, faces, valid, eps, oi)
+        assert got[6] == dev_expected, i
+        for k in range(6):
+            assert got[k] == want[k], (i, k)

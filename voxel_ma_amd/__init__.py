@@ -47,7 +47,8 @@ EXPORTS = [
     "vc_surrogate_gradients", "vc_surrogate_train_step", "vc_surrogate_infer",
     "vc_surrogate_measure", "vc_surrogate_activation",
     # include/vc_voro.h
-    "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside", "vc_voro_merge_close",
+    "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside",
+    "vc_voro_extract_inside_device", "vc_voro_merge_close", "vc_voro_merge_close_device",
     "vc_voro_radii", "vc_voro_part_inside", "vc_voro_from_ft",
     # include/vc_octree.h
     "vc_octree_parse", "vc_octree_free", "vc_octree_value", "vc_volume_from_octree",
@@ -186,8 +187,13 @@ def library():
         "vc_voro_measures": [P, I, I, I, P, I, P, P, P, P, P, P, P, P, P],
         "vc_voro_extract_inside": [I, I, I, P, P, P, P, P, P, P, I64, ctypes.POINTER(I64), P, P,
                                    I64, ctypes.POINTER(I64), P, P, P, I64, ctypes.POINTER(I64)],
+        "vc_voro_extract_inside_device": [P, I, I, I, P, P, P, P, P, P, P, I64,
+                                          ctypes.POINTER(I64), P, P, I64, ctypes.POINTER(I64),
+                                          P, P, P, I64, ctypes.POINTER(I64)],
         "vc_voro_merge_close": [I, I, I, P, P, P, P, P, P, P, ctypes.c_float, I, P, P,
                                 ctypes.POINTER(I), P, P, P, P, I64, ctypes.POINTER(I64), P],
+        "vc_voro_merge_close_device": [P, I, I, I, P, P, P, P, P, P, P, ctypes.c_float, I, P, P,
+                                ctypes.POINTER(I), P, P, P, P, I64, ctypes.POINTER(I64), P, ctypes.POINTER(I)],
         "vc_voro_radii": [P, I, I, P, P, P, P, I, P, P],
         "vc_voro_from_ft": [P, P, P, I64, ctypes.POINTER(I64), P, P, I64,
                             ctypes.POINTER(I64), P, I64, ctypes.POINTER(I64)],
@@ -755,9 +761,57 @@ def voro_extract_inside(nv, edges, faces_vts, v_valid, lam_e, lam_f):
             fm[:n3.value])
 
 
+def voro_extract_inside_device(ctx, nv, edges, faces_vts, v_valid, lam_e, lam_f):
+    """vc_voro_extract_inside_device: the same outputs as voro_extract_inside,
+    computed on the device (one call: the capacities are the upper bounds)."""
+    e, foff, fv = _complex_arrays(nv, edges, faces_vts)
+    vv = np.ascontiguousarray(np.asarray(v_valid, np.uint8))
+    vv = vv if vv.size else np.zeros(1, np.uint8)
+    le = np.ascontiguousarray(np.asarray(lam_e, np.float32))
+    lf = np.ascontiguousarray(np.asarray(lam_f, np.float32))
+    le = le if le.size else np.zeros(1, np.float32)
+    lf = lf if lf.size else np.zeros(1, np.float32)
+    nt_max = int(np.maximum(np.diff(foff) - 2, 0).sum()) if len(foff) > 1 else 0
+    ne_max = len(e) + 3 * nt_max
+    vi = np.zeros(max(nv, 1), np.int32)
+    eo = np.zeros((max(ne_max, 1), 2), np.int32)
+    em = np.zeros(max(ne_max, 1), np.float32)
+    tr = np.zeros((max(nt_max, 1), 3), np.int32)
+    ff = np.zeros(max(nt_max, 1), np.int32)
+    fm = np.zeros(max(nt_max, 1), np.float32)
+    n1, n2, n3 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    ctx.check(library().vc_voro_extract_inside_device(
+        ctx.handle, nv, len(e), len(foff) - 1, _p(e), _p(foff), _p(fv), _p(vv), _p(le), _p(lf),
+        _p(vi), len(vi), ctypes.byref(n1), _p(eo), _p(em), len(eo), ctypes.byref(n2), _p(tr),
+        _p(ff), _p(fm), len(tr), ctypes.byref(n3)))
+    return (vi[:n1.value], eo[:n2.value], em[:n2.value], tr[:n3.value], ff[:n3.value],
+            fm[:n3.value])
+
+
 def voro_merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
     """vc_voro_merge_close: (old_to_new, new_vts, new_faces, remove_face,
     new_edges, keep_edge)."""
+    return _merge_close(vts, edges, faces_vts, v_valid, eps, only_inside, None)[:6]
+
+
+def voro_merge_close_device(ctx, vts, edges, faces_vts, v_valid, eps, only_inside):
+    """vc_voro_merge_close_device: the same six outputs plus whether the
+    device path ran (nothing within eps along an edge)."""
+    return _merge_close(vts, edges, faces_vts, v_valid, eps, only_inside, ctx)
+
+
+def _merge_close(vts, edges, faces_vts, v_valid, eps, only_inside, ctx):
+    call, out = merge_close_call(vts, edges, faces_vts, v_valid, eps, only_inside, ctx)
+    rc = call()
+    if rc != VC_OK:
+        raise VcError(rc, "vc_voro_merge_close")
+    return out()
+
+
+def merge_close_call(vts, edges, faces_vts, v_valid, eps, only_inside, ctx):
+    """(call, outputs): call() runs vc_voro_merge_close (ctx None) or
+    vc_voro_merge_close_device on arrays prepared once and returns the
+    status; outputs() converts the results (the wrapper's lists)."""
     vts = np.ascontiguousarray(np.asarray(vts, np.float32).reshape(-1, 3))
     nv = len(vts)
     e, foff, fv = _complex_arrays(nv, edges, faces_vts)
@@ -782,17 +836,22 @@ def voro_merge_close(vts, edges, faces_vts, v_valid, eps, only_inside):
     cap = len(e) + nf + 1
     ne_out = np.zeros((cap, 2), np.int32)
     nne = ctypes.c_int64()
-    rc = library().vc_voro_merge_close(nv, len(e), nf, _p(vts), _p(e), _p(foff), _p(fv),
-                                       _p(ve_off), _p(ve), _p(vv), float(eps),
-                                       1 if only_inside else 0, _p(o2n), _p(nvts),
-                                       ctypes.byref(nn), _p(nfo), _p(nfv), _p(rem), _p(ne_out),
-                                       cap, ctypes.byref(nne), _p(keep))
-    if rc != VC_OK:
-        raise VcError(rc, "vc_voro_merge_close")
-    kept = int((rem[:nf] == 0).sum())
-    faces = [list(nfv[nfo[i]:nfo[i + 1]]) for i in range(kept)]
-    return (list(o2n[:nv]), list(nvts[:nn.value]), faces, list(rem[:nf]),
-            [tuple(x) for x in ne_out[:nne.value]], list(keep[:len(e)]))
+    args = (nv, len(e), nf, _p(vts), _p(e), _p(foff), _p(fv), _p(ve_off), _p(ve), _p(vv),
+            float(eps), 1 if only_inside else 0, _p(o2n), _p(nvts), ctypes.byref(nn), _p(nfo),
+            _p(nfv), _p(rem), _p(ne_out), cap, ctypes.byref(nne), _p(keep))
+    dev = ctypes.c_int(0)
+
+    def call():
+        if ctx is None:
+            return library().vc_voro_merge_close(*args)
+        return library().vc_voro_merge_close_device(ctx.handle, *args, ctypes.byref(dev))
+
+    def outputs():
+        kept = int((rem[:nf] == 0).sum())
+        faces = [list(nfv[nfo[i]:nfo[i + 1]]) for i in range(kept)]
+        return (list(o2n[:nv]), list(nvts[:nn.value]), faces, list(rem[:nf]),
+                [tuple(x) for x in ne_out[:nne.value]], list(keep[:len(e)]), bool(dev.value))
+    return call, outputs
 
 
 def voro_radii(ctx, vts, faces_vts, face_sites, sites):

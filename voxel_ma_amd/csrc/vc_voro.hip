@@ -3,6 +3,7 @@
 //
 // Both are gathers: one thread per point / face / edge / vertex, the measure
 // of an edge or vertex a maximum over its CSR neighbourhood.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -363,6 +364,174 @@ __global__ void __launch_bounds__(kBlock)
     sites[3 * s + 2] = (float)z;
 }
 
+// ---- inside extraction on the device (the order of vc_voro_extract_inside)
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__device__ __forceinline__ uint64_t pair_key(int32_t a, int32_t b)
+{
+    return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return k;
+}
+
+// insert key with value v (the table keeps the smallest value per key)
+__device__ void hash_min(uint64_t* keys, unsigned long long* vals, uint64_t mask,
+                         uint64_t key, unsigned long long v)
+{
+    uint64_t h = mix64(key) & mask;
+    for (;;)
+    {
+        const uint64_t prev = atomicCAS(
+            reinterpret_cast<unsigned long long*>(&keys[h]),
+            (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (prev == kEmptyKey || prev == key)
+        {
+            atomicMin(&vals[h], v);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ unsigned long long hash_get(const uint64_t* keys,
+                                       const unsigned long long* vals,
+                                       uint64_t mask, uint64_t key)
+{
+    uint64_t h = mix64(key) & mask;
+    while (keys[h] != key)
+        h = (h + 1) & mask;
+    return vals[h];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    xi_flags(int32_t nv, int32_t ne, int32_t nf, const int32_t* __restrict__ E,
+             const int64_t* __restrict__ fo, const int32_t* __restrict__ fv,
+             const uint8_t* __restrict__ valid, uint32_t* __restrict__ vflag,
+             uint32_t* __restrict__ eflag, uint32_t* __restrict__ tcount)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < nv)
+        vflag[i] = valid[i] ? 1u : 0u;
+    if (i < ne)
+        eflag[i] = valid[E[2 * i]] && valid[E[2 * i + 1]] ? 1u : 0u;
+    if (i < nf)
+    {
+        bool ok = true;
+        for (int64_t j = fo[i]; j < fo[i + 1] && ok; ++j)
+            ok = valid[fv[j]] != 0;
+        const int64_t n = fo[i + 1] - fo[i];
+        tcount[i] = ok && n >= 3 ? (uint32_t)(n - 2) : 0u;
+    }
+}
+
+// valid vertex list, valid edges (renumbered, measures) and their exact
+// pairs in the table with value 0
+__global__ void __launch_bounds__(kBlock)
+    xi_emit_ve(int32_t nv, int32_t ne, const int32_t* __restrict__ E,
+               const uint8_t* __restrict__ valid,
+               const uint32_t* __restrict__ vpos, const uint32_t* __restrict__ epos,
+               const float* __restrict__ lam_e, int32_t* __restrict__ vts_idx,
+               int32_t* __restrict__ e_out, float* __restrict__ em,
+               uint64_t* keys, unsigned long long* vals, uint64_t mask)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < nv && valid[i])
+        vts_idx[vpos[i]] = (int32_t)i;
+    if (i < ne && epos[i + 1] != epos[i])
+    {
+        const int32_t a = E[2 * i], b = E[2 * i + 1];
+        const uint32_t k = epos[i];
+        e_out[2 * k] = (int32_t)vpos[a];
+        e_out[2 * k + 1] = (int32_t)vpos[b];
+        em[k] = lam_e[i];
+        hash_min(keys, vals, mask, pair_key(a, b), 0ull);
+    }
+}
+
+// triangles of the valid faces (zig-zag: triangle t of an n-gon is
+// (p[i], p[j], p[j+1]) for even t and (p[i], p[j], p[i-1]) for odd t, with
+// i = n-1-floor(t/2), j = ceil(t/2)); their normalised sides as candidates
+// 3 t + s, inserted with value candidate + 1
+__global__ void __launch_bounds__(kBlock)
+    xi_tris(int32_t nf, const int64_t* __restrict__ fo,
+            const int32_t* __restrict__ fv, const uint32_t* __restrict__ toff,
+            const uint32_t* __restrict__ vpos, const float* __restrict__ lam_f,
+            int32_t* __restrict__ tris, int32_t* __restrict__ from_fi,
+            float* __restrict__ fm, uint64_t* __restrict__ ckey, uint64_t* keys,
+            unsigned long long* vals, uint64_t mask)
+{
+    const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf || toff[f + 1] == toff[f])
+        return;
+    const int32_t* p = fv + fo[f];
+    const int n = (int)(fo[f + 1] - fo[f]);
+    for (int t = 0; t < n - 2; ++t)
+    {
+        const int i = n - 1 - t / 2, j = (t + 1) / 2;
+        const int32_t v[3] = {p[i], p[j], (t & 1) ? p[i - 1] : p[j + 1]};
+        const int64_t T = (int64_t)toff[f] + t;
+        for (int s = 0; s < 3; ++s)
+        {
+            tris[3 * T + s] = (int32_t)vpos[v[s]];
+            int32_t a = v[s], b = v[(s + 1) % 3];
+            if (a > b)
+            {
+                const int32_t x = a;
+                a = b;
+                b = x;
+            }
+            const uint64_t k = pair_key(a, b);
+            ckey[3 * T + s] = k;
+            hash_min(keys, vals, mask, k, (unsigned long long)(3 * T + s) + 1ull);
+        }
+        from_fi[T] = (int32_t)f;
+        fm[T] = lam_f[f];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    xi_keep(int64_t nc, const uint64_t* __restrict__ ckey, const uint64_t* keys,
+            const unsigned long long* vals, uint64_t mask,
+            uint32_t* __restrict__ keep)
+{
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c < nc)
+        keep[c] = hash_get(keys, vals, mask, ckey[c]) == (unsigned long long)c + 1ull
+                      ? 1u
+                      : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    xi_emit_sides(int64_t nc, const uint64_t* __restrict__ ckey,
+                  const uint32_t* __restrict__ kpos,
+                  const uint32_t* __restrict__ vpos,
+                  const int32_t* __restrict__ from_fi,
+                  const float* __restrict__ lam_f, int64_t base,
+                  int32_t* __restrict__ e_out, float* __restrict__ em)
+{
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc || kpos[c + 1] == kpos[c])
+        return;
+    const int64_t k = base + kpos[c];
+    const uint64_t key = ckey[c];
+    e_out[2 * k] = (int32_t)vpos[(int32_t)(key >> 32)];
+    e_out[2 * k + 1] = (int32_t)vpos[(int32_t)(key & 0xFFFFFFFFull)];
+    em[k] = lam_f[from_fi[c / 3]];
+}
+
+__global__ void __launch_bounds__(kBlock) fill_u64(uint64_t* a, int64_t n, uint64_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
 } // namespace
 
 extern "C" {
@@ -567,6 +736,155 @@ int vc_voro_radii(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
     vertex_radius<<<blocks_for(nv), kBlock, 0, S>>>(dv, ds, dfs, dlast, nv, dr);
     VC_CHECK_LAUNCH(ctx);
     VC_HIP(ctx, hipMemcpyAsync(radii, dr, 4 * (size_t)nv, hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+int vc_voro_extract_inside_device(
+    vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const int32_t* edges,
+    const int64_t* face_off, const int32_t* face_vts, const uint8_t* v_valid,
+    const float* lam_e, const float* lam_f, int32_t* vts_idx, int64_t vts_cap,
+    int64_t* n_vts, int32_t* edges_out, float* em, int64_t edge_cap,
+    int64_t* n_edges, int32_t* tris, int32_t* from_fi, float* fm,
+    int64_t tri_cap, int64_t* n_tris)
+{
+    if (!ctx || nv < 0 || ne < 0 || nf < 0 || (ne > 0 && (!edges || !lam_e)) ||
+        (nf > 0 && (!face_off || !face_vts || !lam_f)) || (nv > 0 && !v_valid) ||
+        !n_vts || !n_edges || !n_tris)
+        return VC_ERR_INVALID;
+    const int64_t nfv = nf > 0 ? face_off[nf] : 0;
+    if ((nf > 0 && face_off[0] != 0) || nfv < 0 || 3 * nfv >= (int64_t)INT32_MAX)
+        return VC_ERR_INVALID;
+    for (int64_t i = 0; i < 2LL * ne; ++i)
+        if (edges[i] < 0 || edges[i] >= nv)
+            return VC_ERR_INVALID;
+    for (int32_t f = 0; f < nf; ++f)
+        if (face_off[f + 1] < face_off[f])
+            return VC_ERR_INVALID;
+    for (int64_t j = 0; j < nfv; ++j)
+        if (face_vts[j] < 0 || face_vts[j] >= nv)
+            return VC_ERR_INVALID;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    int32_t *dE, *dfv, *dvi, *deo, *dtr, *dff;
+    int64_t* dfo;
+    uint8_t* dval;
+    float *dle, *dlf, *dem, *dfm;
+    uint32_t *vpos, *epos, *toff;
+    int rc = VC_OK;
+    auto get = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = sc.get(ctx, p, n);
+    };
+    get(&dE, 2 * (size_t)ne + 2);
+    get(&dfo, (size_t)nf + 1);
+    get(&dfv, (size_t)nfv + 1);
+    get(&dval, (size_t)nv + 1);
+    get(&dle, (size_t)ne + 1);
+    get(&dlf, (size_t)nf + 1);
+    get(&vpos, (size_t)nv + 1);
+    get(&epos, (size_t)ne + 1);
+    get(&toff, (size_t)nf + 1);
+    if (rc != VC_OK)
+        return rc;
+    if (ne)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(dE, edges, 8 * (size_t)ne, hipMemcpyHostToDevice, S));
+        VC_HIP(ctx, hipMemcpyAsync(dle, lam_e, 4 * (size_t)ne, hipMemcpyHostToDevice, S));
+    }
+    if (nf)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(dfo, face_off, 8 * ((size_t)nf + 1),
+                                   hipMemcpyHostToDevice, S));
+        VC_HIP(ctx, hipMemcpyAsync(dlf, lam_f, 4 * (size_t)nf, hipMemcpyHostToDevice, S));
+        if (nfv)
+            VC_HIP(ctx, hipMemcpyAsync(dfv, face_vts, 4 * (size_t)nfv,
+                                       hipMemcpyHostToDevice, S));
+    }
+    if (nv)
+        VC_HIP(ctx, hipMemcpyAsync(dval, v_valid, (size_t)nv, hipMemcpyHostToDevice, S));
+    VC_HIP(ctx, hipMemsetAsync(vpos, 0, 4 * ((size_t)nv + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(epos, 0, 4 * ((size_t)ne + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(toff, 0, 4 * ((size_t)nf + 1), S));
+    const int64_t mx = std::max<int64_t>(std::max<int64_t>(nv, ne), nf);
+    if (mx)
+        xi_flags<<<blocks_for(mx), kBlock, 0, S>>>(nv, ne, nf, dE, dfo, dfv, dval,
+                                                   vpos, epos, toff);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, vpos, (int64_t)nv + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, epos, (int64_t)ne + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, toff, (int64_t)nf + 1)) != VC_OK)
+        return rc;
+    uint32_t cnt[3];
+    VC_HIP(ctx, hipMemcpy(&cnt[0], vpos + nv, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&cnt[1], epos + ne, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&cnt[2], toff + nf, 4, hipMemcpyDeviceToHost));
+    const int64_t NV = cnt[0], NE = cnt[1], NT = cnt[2], NC = 3 * NT;
+    // table: a power of two >= 2 x the keys
+    uint64_t H = 1024;
+    while (H < 2 * (uint64_t)(NE + NC) + 1)
+        H <<= 1;
+    uint64_t* keys;
+    unsigned long long* vals;
+    uint64_t* ckey;
+    uint32_t* keep;
+    get(&dvi, (size_t)NV + 1);
+    get(&deo, 2 * (size_t)(NE + NC) + 2);
+    get(&dem, (size_t)(NE + NC) + 1);
+    get(&dtr, 3 * (size_t)NT + 3);
+    get(&dff, (size_t)NT + 1);
+    get(&dfm, (size_t)NT + 1);
+    get(&keys, (size_t)H);
+    get(&vals, (size_t)H);
+    get(&ckey, (size_t)NC + 1);
+    get(&keep, (size_t)NC + 1);
+    if (rc != VC_OK)
+        return rc;
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
+        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
+    VC_CHECK_LAUNCH(ctx);
+    const int64_t mve = std::max<int64_t>(nv, ne);
+    if (mve)
+        xi_emit_ve<<<blocks_for(mve), kBlock, 0, S>>>(nv, ne, dE, dval, vpos, epos,
+                                                      dle, dvi, deo, dem, keys, vals,
+                                                      H - 1);
+    if (nf)
+        xi_tris<<<blocks_for(nf), kBlock, 0, S>>>(nf, dfo, dfv, toff, vpos, dlf, dtr,
+                                                  dff, dfm, ckey, keys, vals, H - 1);
+    VC_CHECK_LAUNCH(ctx);
+    VC_HIP(ctx, hipMemsetAsync(keep, 0, 4 * ((size_t)NC + 1), S));
+    if (NC)
+        xi_keep<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, keep);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, keep, NC + 1)) != VC_OK)
+        return rc;
+    uint32_t nk = 0;
+    VC_HIP(ctx, hipMemcpy(&nk, keep + NC, 4, hipMemcpyDeviceToHost));
+    if (NC)
+        xi_emit_sides<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keep, vpos, dff, dlf,
+                                                        NE, deo, dem);
+    VC_CHECK_LAUNCH(ctx);
+    *n_vts = NV;
+    *n_edges = NE + nk;
+    *n_tris = NT;
+    if ((vts_idx && vts_cap < *n_vts) || (edges_out && edge_cap < *n_edges) ||
+        (tris && tri_cap < *n_tris))
+        return VC_ERR_CAPACITY;
+    if (vts_idx && NV)
+        VC_HIP(ctx, hipMemcpyAsync(vts_idx, dvi, 4 * (size_t)NV, hipMemcpyDeviceToHost, S));
+    if (edges_out && *n_edges)
+        VC_HIP(ctx, hipMemcpyAsync(edges_out, deo, 8 * (size_t)*n_edges,
+                                   hipMemcpyDeviceToHost, S));
+    if (em && *n_edges)
+        VC_HIP(ctx, hipMemcpyAsync(em, dem, 4 * (size_t)*n_edges, hipMemcpyDeviceToHost, S));
+    if (tris && NT)
+        VC_HIP(ctx, hipMemcpyAsync(tris, dtr, 12 * (size_t)NT, hipMemcpyDeviceToHost, S));
+    if (from_fi && NT)
+        VC_HIP(ctx, hipMemcpyAsync(from_fi, dff, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
+    if (fm && NT)
+        VC_HIP(ctx, hipMemcpyAsync(fm, dfm, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }

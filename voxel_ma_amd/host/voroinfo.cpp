@@ -364,23 +364,21 @@ void VoroInfo::extractInsideWithMeasure(
     std::vector<uint8_t> valid((size_t)nv + 1, 0);
     for (int v = 0; v < nv; ++v)
         valid[v] = isVertexValid(v) ? 1 : 0;
-    int64_t n1 = 0, n2 = 0, n3 = 0;
-    int rc = vc_voro_extract_inside(nv, ne, nf, edges.data(), foff.data(),
-                                    fvts.data(), valid.data(), le.data(),
-                                    lf.data(), nullptr, 0, &n1, nullptr,
-                                    nullptr, 0, &n2, nullptr, nullptr, nullptr,
-                                    0, &n3);
-    if (rc != VC_OK && rc != VC_ERR_CAPACITY)
-        vcdev::check(rc, "vc_voro_extract_inside");
+    // on the device, with the upper bounds as capacities (one pass): every
+    // vertex, every triangle of the zig-zag triangulations, every edge plus
+    // three sides per triangle
+    int64_t nt_max = 0;
+    for (int32_t f = 0; f < nf; ++f)
+        nt_max += std::max<int64_t>(foff[f + 1] - foff[f] - 2, 0);
+    int64_t n1 = nv, n2 = ne + 3 * nt_max, n3 = nt_max;
     std::vector<int32_t> vi(n1 + 1), eo(2 * n2 + 2), tr(3 * n3 + 3), ff(n3 + 1);
     std::vector<float> em(n2 + 1), fm(n3 + 1);
-    vcdev::check(vc_voro_extract_inside(nv, ne, nf, edges.data(), foff.data(),
-                                        fvts.data(), valid.data(), le.data(),
-                                        lf.data(), vi.data(), n1, &n1,
-                                        eo.data(), em.data(), n2, &n2,
-                                        tr.data(), ff.data(), fm.data(), n3,
-                                        &n3),
-                 "vc_voro_extract_inside");
+    vcdev::check(vc_voro_extract_inside_device(
+                     vcdev::context(), nv, ne, nf, edges.data(), foff.data(),
+                     fvts.data(), valid.data(), le.data(), lf.data(), vi.data(),
+                     n1, &n1, eo.data(), em.data(), n2, &n2, tr.data(), ff.data(),
+                     fm.data(), n3, &n3),
+                 "vc_voro_extract_inside_device");
     for (int64_t i = 0; i < n1; ++i)
     {
         out_vts.push_back(m_geom.getVert(vi[i]));
@@ -434,14 +432,15 @@ void VoroInfo::mergeCloseVts(float eps, bool only_inside)
     std::vector<uint8_t> rem((size_t)nf + 1), keep((size_t)ne + 1);
     int32_t n_new = 0;
     int64_t n_e = 0;
-    vcdev::check(vc_voro_merge_close(nv, ne, nf, vts.data(), edges.data(),
-                                     foff.data(), fvts.data(), ve_off.data(),
-                                     ve.data(), valid.data(), eps,
-                                     only_inside ? 1 : 0, o2n.data(),
-                                     nvts.data(), &n_new, nfo.data(),
-                                     nfv.data(), rem.data(), ned.data(),
-                                     (int64_t)ne + nf, &n_e, keep.data()),
-                 "vc_voro_merge_close");
+    // on the device when nothing lies within eps along an edge, else the
+    // order-defined BFS on the host
+    vcdev::check(vc_voro_merge_close_device(
+                     vcdev::context(), nv, ne, nf, vts.data(), edges.data(),
+                     foff.data(), fvts.data(), ve_off.data(), ve.data(),
+                     valid.data(), eps, only_inside ? 1 : 0, o2n.data(),
+                     nvts.data(), &n_new, nfo.data(), nfv.data(), rem.data(),
+                     ned.data(), (int64_t)ne + nf, &n_e, keep.data(), nullptr),
+                 "vc_voro_merge_close_device");
     // the new geometry: seeds, surviving edges, kept faces, then finalize
     std::vector<point> new_v;
     std::vector<bool> new_valid, new_finite;
