@@ -380,13 +380,17 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
                                           all(gd[k] == gh[k] for k in range(6)),
                     "note": "host arrays in and out; extraction through its wrapper, the merge "
                             "as its C-ABI call alone"}
-    t_f, _ = best_of(lambda: V.cc_finalize(len(vts), [], fl), 1)
+    t_f, fh = best_of(lambda: V.cc_finalize(len(vts), [], fl), 1)
+    t_fd, fd = best_of(lambda: V.cc_finalize(len(vts), [], fl, ctx=ctx))
     t_p, rh = best_of(lambda: V.cc_prune(len(vts), edges, foff, fe, le, lf, 1.5, 1.5), 1)
     t_pd, rd = best_of(lambda: V.cc_prune_device(ctx, len(vts), edges, foff, fe, le, lf,
                                                  1.5, 1.5))
     same = all(np.array_equal(a, b) for a, b in zip(rh[1:], rd[1:5]))
     out["R10_R12_cellcomplex"] = {"workload": "finalize + FIFO prune of %d quads" % nf,
                                   "finalize_ms": round(t_f * 1e3, 1),
+                                  "finalize_device_ms": round(t_fd * 1e3, 1),
+                                  "finalize_device_equals_host":
+                                      all(np.array_equal(a, b) for a, b in zip(fh, fd)),
                                   "prune_ms": round(t_p * 1e3, 1),
                                   "prune_device_ms": round(t_pd * 1e3, 1),
                                   "generations": int(rd[5]), "device_equals_host": same}

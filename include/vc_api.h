@@ -33,6 +33,7 @@
  *                         (src/ccthin.cpp:201-424,
  *                          src/highlevelalgo.cpp:815-961), on voxels
  *   vc_cc_finalize        cellcomplex::finalize (src/cellcomplex.cpp:364-491)
+ *   vc_cc_finalize_device the same, on the device
  *   vc_cc_prune           CellComplexThinning::prune (src/ccthin.cpp:201-771)
  *   vc_cc_prune_device    the same, on the device
  *   vc_boundary_extract   Surfacer::extractBoundaryVts /
@@ -184,6 +185,16 @@ int vc_cc_finalize(int32_t nv, int32_t ne_in, const int32_t* edges_in,
                    int32_t nf, const int64_t* face_off,
                    const int32_t* face_vts, int32_t* edges_out,
                    int32_t edge_cap, int32_t* ne_out, int32_t* face_edges_out);
+/* vc_cc_finalize on the device, the same outputs: the given edges (exact
+ * pairs) then the face sides (normalised) are candidates in that order, the
+ * first candidate of each pair (a device hash table keeps the smallest
+ * index) becomes an edge, numbered by a scan.  Host arrays; waits.  With
+ * edges_out NULL only *ne_out is written. */
+int vc_cc_finalize_device(vc_ctx* ctx, int32_t nv, int32_t ne_in,
+                          const int32_t* edges_in, int32_t nf,
+                          const int64_t* face_off, const int32_t* face_vts,
+                          int32_t* edges_out, int32_t edge_cap, int32_t* ne_out,
+                          int32_t* face_edges_out);
 /* CellComplexThinning::prune(f_t, l_t) (src/ccthin.cpp:201-424, :514-771):
  * queue every face-edge pair with face measure < f_t (edge order), then every
  * edge-vertex pair with edge measure < l_t (vertex order), then pop FIFO.

@@ -115,3 +115,28 @@ def test_device_rejects_degenerate_complexes(vc):
     fe = np.array([0, 1, 2, 1], np.int32)
     rc = V.cc_prune_device(vc, 3, e, foff, fe, np.zeros(3), np.zeros(1), 1.0, 1.0)[0]
     assert rc == V.VC_ERR_INVALID
+
+
+def test_finalize_device_matches_host(vc):
+    """vc_cc_finalize_device against the host finalize: the reference's
+    hand-built complex (tests/test_voro_merge.cpp:10-47: duplicate given
+    edges collapse, face sides reuse given ids), random complexes with given
+    edges in both orientations (an edge stored (b, a) does not absorb the
+    normalised side (a, b)), and a voxel-core complex."""
+    edges = [(0, 1), (1, 2), (2, 3), (0, 3), (2, 4), (1, 4), (1, 5), (4, 5), (5, 7), (6, 7),
+             (4, 6), (0, 3), (3, 8), (0, 8), (0, 8), (8, 9), (0, 9)]
+    faces = [[0, 1, 2, 3], [1, 2, 4], [1, 4, 5], [4, 5, 7, 6], [0, 3, 8], [0, 8, 9]]
+    cases = [(10, edges, faces)]
+    rng = np.random.default_rng(61)
+    for _ in range(20):
+        nv, extra, fs = _random_complex(rng)
+        extra = extra + [(b, a) for a, b in extra[:2]]
+        cases.append((nv, extra, fs))
+    nv, faces = _grid(150, 120)
+    cases.append((nv, [(1, 0), (0, 151), (3, 2)], faces))
+    for i, (nv, e_in, fs) in enumerate(cases):
+        say("finalize device", i, nv, len(e_in), len(fs))
+        h = V.cc_finalize(nv, e_in, fs)
+        d = V.cc_finalize(nv, e_in, fs, ctx=vc)
+        for a, b in zip(h, d):
+            assert np.array_equal(a, b), i

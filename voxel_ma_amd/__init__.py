@@ -36,7 +36,7 @@ EXPORTS = [
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
     "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
-    "vc_cc_prune", "vc_cc_prune_device",
+    "vc_cc_prune", "vc_cc_prune_device", "vc_cc_finalize_device",
     # include/vc_batch.h
     "vc_batch_create", "vc_batch_destroy", "vc_batch_size", "vc_batch_edt", "vc_batch_measure",
     "vc_batch_thin", "vc_batch_run", "vc_schedule_defaults", "vc_batch_get_schedule",
@@ -153,6 +153,7 @@ def library():
                                    ctypes.c_uint64, I, ctypes.POINTER(ctypes.c_float)],
         "vc_synth_solids": [P, P, I, I, I, P, I],
         "vc_cc_finalize": [I, I, P, I, P, P, P, I, ctypes.POINTER(I), P],
+        "vc_cc_finalize_device": [P, I, I, P, I, P, P, P, I, ctypes.POINTER(I), P],
         "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
         "vc_cc_prune_device": [P, ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float,
                                P, P, P, P, ctypes.POINTER(ctypes.c_int64)],
@@ -627,10 +628,11 @@ class Surrogate:
 
 
 # ---------------------------------------------------------------- cell complex
-def cc_finalize(nv, edges, faces):
-    """cellcomplex::finalize through vc_cc_finalize (host side).  edges: list
-    of (a, b); faces: list of vertex lists.  Returns (edges (ne, 2), face
-    offsets, face side edge ids)."""
+def cc_finalize(nv, edges, faces, ctx=None):
+    """cellcomplex::finalize through vc_cc_finalize (host), or through
+    vc_cc_finalize_device when a context is given.  edges: list of (a, b);
+    faces: list of vertex lists.  Returns (edges (ne, 2), face offsets, face
+    side edge ids)."""
     e_in = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
     foff, fv = _faces_csr(faces)
     fv = fv[:foff[-1]] if foff[-1] < len(fv) else fv
@@ -638,8 +640,13 @@ def cc_finalize(nv, edges, faces):
     eout = np.zeros((cap, 2), np.int32)
     fe = np.zeros(max(len(fv), 1), np.int32)
     ne = ctypes.c_int()
-    rc = library().vc_cc_finalize(nv, len(e_in), _p(e_in), len(faces), _p(foff), _p(fv),
-                                  _p(eout), cap, ctypes.byref(ne), _p(fe))
+    if ctx is None:
+        rc = library().vc_cc_finalize(nv, len(e_in), _p(e_in), len(faces), _p(foff), _p(fv),
+                                      _p(eout), cap, ctypes.byref(ne), _p(fe))
+    else:
+        rc = library().vc_cc_finalize_device(ctx.handle, nv, len(e_in), _p(e_in), len(faces),
+                                             _p(foff), _p(fv), _p(eout), cap, ctypes.byref(ne),
+                                             _p(fe))
     if rc != VC_OK:
         raise VcError(rc, "vc_cc_finalize")
     return eout[:ne.value].copy(), foff, fe[:len(fv)].copy()

@@ -525,6 +525,72 @@ __global__ void __launch_bounds__(kBlock)
     em[k] = lam_f[from_fi[c / 3]];
 }
 
+// ---- cellcomplex::finalize on the device: candidates are the given edges
+// (exact pairs) then the face sides (normalised), in that order; the first
+// candidate of each pair becomes an edge, numbered in candidate order
+__global__ void __launch_bounds__(kBlock)
+    fz_keys(int32_t ne, const int32_t* __restrict__ E, int32_t nf,
+            const int64_t* __restrict__ fo, const int32_t* __restrict__ fv,
+            uint64_t* __restrict__ ckey, uint64_t* keys,
+            unsigned long long* vals, uint64_t mask)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < ne)
+    {
+        const uint64_t k = pair_key(E[2 * i], E[2 * i + 1]);
+        ckey[i] = k;
+        hash_min(keys, vals, mask, k, (unsigned long long)i);
+    }
+    if (i < nf)
+    {
+        const int64_t n = fo[i + 1] - fo[i];
+        for (int64_t j = 0; j < n; ++j)
+        {
+            int32_t a = fv[fo[i] + j], b = fv[fo[i] + (j + 1) % n];
+            if (a > b)
+            {
+                const int32_t t = a;
+                a = b;
+                b = t;
+            }
+            const uint64_t k = pair_key(a, b);
+            const int64_t c = ne + fo[i] + j;
+            ckey[c] = k;
+            hash_min(keys, vals, mask, k, (unsigned long long)c);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    fz_first(int64_t nc, const uint64_t* __restrict__ ckey, const uint64_t* keys,
+             const unsigned long long* vals, uint64_t mask,
+             uint32_t* __restrict__ first)
+{
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c < nc)
+        first[c] = hash_get(keys, vals, mask, ckey[c]) == (unsigned long long)c ? 1u
+                                                                                : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    fz_emit(int64_t nc, int32_t ne, const uint64_t* __restrict__ ckey,
+            const uint32_t* __restrict__ pos, const uint64_t* keys,
+            const unsigned long long* vals, uint64_t mask,
+            int32_t* __restrict__ e_out, int32_t* __restrict__ fe)
+{
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc)
+        return;
+    const uint64_t k = ckey[c];
+    if (pos[c + 1] != pos[c])
+    {
+        e_out[2 * (int64_t)pos[c]] = (int32_t)(k >> 32);
+        e_out[2 * (int64_t)pos[c] + 1] = (int32_t)(k & 0xFFFFFFFFull);
+    }
+    if (c >= ne)
+        fe[c - ne] = (int32_t)pos[hash_get(keys, vals, mask, k)];
+}
+
 __global__ void __launch_bounds__(kBlock) fill_u64(uint64_t* a, int64_t n, uint64_t v)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -885,6 +951,98 @@ int vc_voro_extract_inside_device(
         VC_HIP(ctx, hipMemcpyAsync(from_fi, dff, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
     if (fm && NT)
         VC_HIP(ctx, hipMemcpyAsync(fm, dfm, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
+int vc_cc_finalize_device(vc_ctx* ctx, int32_t nv, int32_t ne_in,
+                          const int32_t* edges_in, int32_t nf,
+                          const int64_t* face_off, const int32_t* face_vts,
+                          int32_t* edges_out, int32_t edge_cap, int32_t* ne_out,
+                          int32_t* face_edges_out)
+{
+    if (!ctx || nv < 0 || ne_in < 0 || nf < 0 || !ne_out ||
+        (ne_in && !edges_in) || (nf && (!face_off || !face_vts || !face_edges_out)))
+        return VC_ERR_INVALID;
+    const int64_t nfv = nf ? face_off[nf] : 0;
+    if ((nf && face_off[0] != 0) || nfv < 0 || (int64_t)ne_in + nfv >= (int64_t)INT32_MAX)
+        return VC_ERR_INVALID;
+    for (int32_t f = 0; f < nf; ++f)
+        if (face_off[f + 1] < face_off[f])
+            return VC_ERR_INVALID;
+    for (int64_t i = 0; i < 2LL * ne_in; ++i)
+        if (edges_in[i] < 0 || edges_in[i] >= nv)
+            return VC_ERR_INVALID;
+    for (int64_t j = 0; j < nfv; ++j)
+        if (face_vts[j] < 0 || face_vts[j] >= nv)
+            return VC_ERR_INVALID;
+    VC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    const int64_t NC = (int64_t)ne_in + nfv;
+    uint64_t H = 1024;
+    while (H < 2 * (uint64_t)NC + 1)
+        H <<= 1;
+    int32_t *dE, *dfv, *deo, *dfe;
+    int64_t* dfo;
+    uint64_t *keys, *ckey;
+    unsigned long long* vals;
+    uint32_t* pos;
+    int rc = VC_OK;
+    auto get = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = sc.get(ctx, p, n);
+    };
+    get(&dE, 2 * (size_t)ne_in + 2);
+    get(&dfo, (size_t)nf + 1);
+    get(&dfv, (size_t)nfv + 1);
+    get(&deo, 2 * (size_t)NC + 2);
+    get(&dfe, (size_t)nfv + 1);
+    get(&keys, (size_t)H);
+    get(&vals, (size_t)H);
+    get(&ckey, (size_t)NC + 1);
+    get(&pos, (size_t)NC + 1);
+    if (rc != VC_OK)
+        return rc;
+    if (ne_in)
+        VC_HIP(ctx, hipMemcpyAsync(dE, edges_in, 8 * (size_t)ne_in,
+                                   hipMemcpyHostToDevice, S));
+    if (nf)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(dfo, face_off, 8 * ((size_t)nf + 1),
+                                   hipMemcpyHostToDevice, S));
+        if (nfv)
+            VC_HIP(ctx, hipMemcpyAsync(dfv, face_vts, 4 * (size_t)nfv,
+                                       hipMemcpyHostToDevice, S));
+    }
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
+        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
+    VC_HIP(ctx, hipMemsetAsync(pos, 0, 4 * ((size_t)NC + 1), S));
+    const int64_t mx = std::max<int64_t>(ne_in, nf);
+    if (mx)
+        fz_keys<<<blocks_for(mx), kBlock, 0, S>>>(ne_in, dE, nf, dfo, dfv, ckey, keys,
+                                                  vals, H - 1);
+    if (NC)
+        fz_first<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, pos);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, pos, NC + 1)) != VC_OK)
+        return rc;
+    uint32_t ne = 0;
+    VC_HIP(ctx, hipMemcpy(&ne, pos + NC, 4, hipMemcpyDeviceToHost));
+    *ne_out = (int32_t)ne;
+    if ((int64_t)ne > edge_cap || !edges_out)
+        return edges_out ? VC_ERR_CAPACITY : VC_OK;
+    if (NC)
+        fz_emit<<<blocks_for(NC), kBlock, 0, S>>>(NC, ne_in, ckey, pos, keys, vals,
+                                                  H - 1, deo, dfe);
+    VC_CHECK_LAUNCH(ctx);
+    if (ne)
+        VC_HIP(ctx, hipMemcpyAsync(edges_out, deo, 8 * (size_t)ne,
+                                   hipMemcpyDeviceToHost, S));
+    if (nfv)
+        VC_HIP(ctx, hipMemcpyAsync(face_edges_out, dfe, 4 * (size_t)nfv,
+                                   hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }
