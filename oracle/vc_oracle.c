@@ -91,6 +91,23 @@ void or_envelope_1d(const int64_t* f, int n, int64_t* D, int32_t* arg)
     free(t);
 }
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* threads of the OpenMP loops (1 = the sequential restatement); returns
+ * the number in use */
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0)
+        omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
               int32_t* ft)
 {
@@ -102,10 +119,15 @@ int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
     int32_t* zq = (int32_t*)malloc(sizeof(int32_t) * N);
     int32_t* yq = (int32_t*)malloc(sizeof(int32_t) * N);
     int32_t* zz = (int32_t*)malloc(sizeof(int32_t) * N);
+    /* rows and columns are independent: with OpenMP each thread takes whole
+     * rows with its own scratch */
+#pragma omp parallel
+    {
     int64_t* f = (int64_t*)malloc(sizeof(int64_t) * nmax);
     int64_t* D = (int64_t*)malloc(sizeof(int64_t) * nmax);
     int32_t* a = (int32_t*)malloc(sizeof(int32_t) * nmax);
     /* pass z: nearest background along the row */
+#pragma omp for schedule(static)
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y)
         {
@@ -116,6 +138,7 @@ int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
                 zq[IDX(x, y, z)] = a[z];
         }
     /* pass y */
+#pragma omp for schedule(static)
     for (int x = 0; x < nx; ++x)
         for (int z = 0; z < nz; ++z)
         {
@@ -132,6 +155,7 @@ int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
             }
         }
     /* pass x */
+#pragma omp for schedule(static)
     for (int y = 0; y < ny; ++y)
         for (int z = 0; z < nz; ++z)
         {
@@ -159,12 +183,13 @@ int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
                 }
             }
         }
-    free(zq);
-    free(yq);
-    free(zz);
     free(f);
     free(D);
     free(a);
+    }
+    free(zq);
+    free(yq);
+    free(zz);
     return 0;
 }
 
@@ -207,6 +232,7 @@ void or_lambda2(const uint8_t* occ, const int32_t* ft, int nx, int ny, int nz,
 {
     static const int O6[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+#pragma omp parallel for schedule(static)
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int z = 0; z < nz; ++z)
@@ -256,6 +282,7 @@ int64_t or_level_threshold(float level)
 void or_levels(const uint8_t* occ, const int32_t* lam2, int64_t n,
                const int64_t* T, int nlev, uint8_t* lvl)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i)
     {
         uint8_t v = 255;
@@ -555,17 +582,16 @@ void or_simple_lut(uint32_t* lut_words)
  * ascending order and removes every alive voxel of the subfield whose level
  * index is <= k and which is simple in the current image.
  * ==================================================================== */
-static uint8_t* g_memo = NULL; /* 2 bits per config: 0 unknown, 1 no, 2 yes */
+static uint8_t* g_memo = NULL; /* per config: 0 unknown, 1 no, 2 yes */
 
+/* threads that meet the same configuration store the same value */
 static int memo_simple(uint32_t nb)
 {
-    if (!g_memo)
-        g_memo = (uint8_t*)calloc((size_t)1 << 26, 1);
-    uint8_t v = g_memo[nb];
+    uint8_t v = __atomic_load_n(&g_memo[nb], __ATOMIC_RELAXED);
     if (!v)
     {
         v = or_is_simple(nb) ? 2 : 1;
-        g_memo[nb] = v;
+        __atomic_store_n(&g_memo[nb], v, __ATOMIC_RELAXED);
     }
     return v == 2;
 }
@@ -598,6 +624,8 @@ int or_thin(uint8_t* label, const uint8_t* lvl, int nx, int ny, int nz,
         return -1;
     if (st)
         memset(st, 0, sizeof(*st));
+    if (!g_memo)
+        g_memo = (uint8_t*)calloc((size_t)1 << 26, 1);
     for (int k = 0; k < nlev; ++k)
     {
         int changed;
@@ -609,6 +637,11 @@ int or_thin(uint8_t* label, const uint8_t* lvl, int nx, int ny, int nz,
             for (int s = 0; s < 8; ++s)
             {
                 int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+                int64_t tests = 0, removed = 0;
+                /* voxels of one subfield are never 26-adjacent: a pass reads
+                 * no label it writes, so its voxels are independent */
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : changed) \
+    reduction(+ : tests, removed)
                 for (int x = sx; x < nx; x += 2)
                     for (int y = sy; y < ny; y += 2)
                         for (int z = sz; z < nz; z += 2)
@@ -617,16 +650,19 @@ int or_thin(uint8_t* label, const uint8_t* lvl, int nx, int ny, int nz,
                             if (label[i] != 1 || lvl[i] > k)
                                 continue;
                             uint32_t nb = gather26(label, nx, ny, nz, x, y, z);
-                            if (st)
-                                st->tests++;
+                            tests++;
                             if (memo_simple(nb))
                             {
                                 label[i] = (uint8_t)(2 + k);
                                 changed = 1;
-                                if (st)
-                                    st->removed[k]++;
+                                removed++;
                             }
                         }
+                if (st)
+                {
+                    st->tests += tests;
+                    st->removed[k] += removed;
+                }
             }
         } while (changed);
     }

@@ -27,6 +27,10 @@ extern "C" {
 /* d2[p] = min over background q of |p-q|^2 (0 on background); ft[p] = linear
  * index of the lowest-index minimiser; -1 in both when the volume has no
  * background voxel.  Separable lower envelopes, passes z, y, x. */
+/* threads of the restatement's OpenMP loops (1 = sequential; results do
+ * not depend on it); returns the number in use */
+int or_set_threads(int n);
+
 int or_edt_ft(const uint8_t* occ, int nx, int ny, int nz, int32_t* d2,
               int32_t* ft);
 /* O(N*B) brute force (tiny volumes only). */

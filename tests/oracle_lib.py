@@ -32,6 +32,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.c_void_p
         i32, i64 = ctypes.c_int, ctypes.c_int64
+        L.or_set_threads.argtypes = [i32]
+        L.or_set_threads.restype = i32
         L.or_edt_ft.argtypes = [P, i32, i32, i32, P, P]
         L.or_edt_brute.argtypes = [P, i32, i32, i32, P, P]
         L.or_envelope_1d.argtypes = [P, i32, P, P]
@@ -213,3 +215,9 @@ def simple_triangulate(poly):
     out = np.zeros((max(len(p) - 2, 1), 3), np.int32)
     n = lib().or_simple_triangulate(_p(p), len(p), _p(out))
     return out[:n]
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (1 = sequential); returns the
+    number in use.  Results do not depend on it."""
+    return int(lib().or_set_threads(int(n)))

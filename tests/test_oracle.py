@@ -249,3 +249,23 @@ def test_thinning_preserves_topology_and_is_a_fixed_point(oracle, simple_lut_cpu
     elig = alive & (lvl <= len(levels) - 1)
     c = code[elig]
     assert not ((simple_lut_cpu[c >> 5] >> (c & 31)) & 1).any()
+
+
+def test_oracle_threads_do_not_change_results(oracle):
+    """The OpenMP loops of the restatement (EDT rows, measure, levels, the
+    voxels of one subfield pass) give the sequential results."""
+    rng = np.random.default_rng(77)
+    occ = (rng.random((30, 26, 34)) < 0.75).astype(np.uint8)
+    levels = [1.5, 3.0]
+    out = []
+    n0 = oracle.set_threads(0)
+    for n in (1, 4):
+        oracle.set_threads(n)
+        d, f = oracle.edt_ft(occ)
+        lam = oracle.lambda2(occ, f)
+        lvl = oracle.levels(occ, lam, levels)
+        lab, st = oracle.thin(occ, lvl, len(levels))
+        out.append((d, f, lam, lvl, lab, list(st.removed[:2]), st.tests))
+    oracle.set_threads(n0)
+    for a, b in zip(*out):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
