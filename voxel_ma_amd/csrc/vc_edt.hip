@@ -386,12 +386,17 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
         auto load = [&](int xx) -> uint32_t {
             return (valid && xx >= 0 && xx < nx) ? ft[xx * sx + row] : kOob;
         };
-        uint32_t prev = load(xa - 1), cur = load(xa);
+        auto load_halo = [&](int xx) -> uint32_t {
+            return (hin && xx < xb) ? ft[xx * sx + (int64_t)gy * nz + gz] : kOob;
+        };
+        // two slices ahead in flight: the loads for x + 2 (and the halo of
+        // x + 1) are issued before the work on slice x
+        uint32_t prev = load(xa - 1), cur = load(xa), next = load(xa + 1);
+        uint32_t halo = load_halo(xa);
         for (int x = xa; x < xb; ++x)
         {
-            const uint32_t next = load(x + 1);
-            const uint32_t halo =
-                hin ? ft[x * sx + (int64_t)gy * nz + gz] : kOob;
+            const uint32_t next2 = load(x + 2);
+            const uint32_t halo_n = load_halo(x + 1);
             S[ty + 1][tz + 1] = valid ? cur : kOob;
             if (hact)
                 S[hy + 1][hz + 1] = halo;
@@ -427,6 +432,8 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
             __syncthreads();
             prev = cur;
             cur = next;
+            next = next2;
+            halo = halo_n;
         }
     }
 }
