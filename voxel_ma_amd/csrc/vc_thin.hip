@@ -968,10 +968,10 @@ __device__ __forceinline__ uint32_t halo_nbrs(int x, int y, int z)
 #endif
 constexpr int kTBlock = VC_TILE_BLOCK; // threads of a tile-sweep workgroup
 #ifndef VC_TILE_WAVES
-#define VC_TILE_WAVES 6
+#define VC_TILE_WAVES 8
 #endif
-// waves per SIMD the tile sweep is compiled for (3 workgroups of 512 per CU,
-// which its 44 KB of LDS allows)
+// waves per SIMD the tile sweep is compiled for (4 workgroups of 512 per CU,
+// which its 37 KB of LDS allows: 64 VGPRs)
 constexpr int kTWaves = VC_TILE_WAVES;
 constexpr int kTQ = 1024;              // candidate queue entries per round
 // rows of one parity evaluated per pass: at most 23 x 19 (pass 0)
@@ -993,8 +993,8 @@ __device__ __forceinline__ void load_win(const uint64_t* __restrict__ P,
     hi = (uint32_t)(c >> 62) | ((uint32_t)(n & 3ull) << 2);
 }
 
-// a region row: window bits 0..63 and 64..67 (pad keeps rows 16-byte
-// aligned, so one row is one 128-bit LDS access)
+// a region row: window bits 0..63 and 64..67 (in LDS the low words and the
+// high nibbles live in separate arrays, which keeps the tile sweep at 37 KB)
 struct TRow
 {
     uint64_t lo;
@@ -1126,9 +1126,14 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
     const PassCtl pc = pass_ctl(ctl, j, k_arg, 0, nvol, gany, vany);
     const uint32_t sweep = pc.sweep;
     const int k = pc.k;
-    __shared__ TRow SA[kTRows];                 // alive
+    __shared__ unsigned long long SAL[kTRows];  // alive, window bits 0..63
     __shared__ unsigned long long SElo[kTRows]; // eligible (constant in a level)
-    __shared__ uint8_t SEhi[kTRows];
+    __shared__ uint32_t SH4[kTRows / 4];        // per row byte: alive bits 64..67
+                                                // (low nibble), eligible (high)
+    uint8_t* const SHB = reinterpret_cast<uint8_t*>(SH4);
+    auto row = [&](int r) -> TRow {
+        return TRow{SAL[r], (uint32_t)(SHB[r] & 0xFu), 0u};
+    };
     __shared__ uint32_t tq[kTQ];
     __shared__ uint32_t red[2 * (kTBlock / 64)]; // scan totals, by pass parity
     __shared__ uint32_t snbr; // neighbour mask of this tile's removals
@@ -1189,9 +1194,9 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                 load_win(Pin, o, w, nwz, alo, ahi);
                 load_win(d.elig, o, w, nwz, elo, ehi);
             }
-            SA[r] = TRow{alo, ahi, 0u};
+            SAL[r] = alo;
             SElo[r] = elo;
-            SEhi[r] = (uint8_t)ehi;
+            SHB[r] = (uint8_t)(ahi | (ehi << 4));
         }
         __syncthreads();
         uint32_t nrem = 0, ntest = 0, nbr = 0;
@@ -1228,15 +1233,16 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                     continue;
                 const int rx = rx0 + 2 * (q / nry), ry = ry0 + 2 * (q % nry);
                 const int r = rx * kRY + ry;
-                const TRow C = SA[r];
+                const uint32_t hb = SHB[r];
+                const TRow C{SAL[r], hb & 0xFu, 0u};
                 uint64_t cl = C.lo & SElo[r] & zlo;
-                uint32_t ch = C.hi & (uint32_t)SEhi[r] & zhi;
+                uint32_t ch = C.hi & (hb >> 4) & zhi;
                 if (cl | ch)
                 {
                     // border filter: all six face neighbours alive -> never
                     // simple
-                    const TRow a = SA[r - kRY], bb = SA[r + kRY], c = SA[r - 1],
-                               e = SA[r + 1];
+                    const TRow a = row(r - kRY), bb = row(r + kRY), c = row(r - 1),
+                               e = row(r + 1);
                     const uint64_t zdn = (C.lo << 1);
                     const uint64_t zup = (C.lo >> 1) | ((uint64_t)C.hi << 63);
                     cl &= ~(a.lo & bb.lo & c.lo & e.lo & zdn & zup);
@@ -1291,7 +1297,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                             for (int dy = 0; dy < 3; ++dy)
-                                R[dx][dy] = SA[r + (dx - 1) * kRY + (dy - 1)];
+                                R[dx][dy] = row(r + (dx - 1) * kRY + (dy - 1));
                         code[t] = nb_code_rows(R, (int)(ent[t] & 127u));
                     }
                 }
@@ -1305,10 +1311,9 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
                         continue;
                     const int r = (int)(ent[t] >> 7), i = (int)(ent[t] & 127u);
                     if (i < 64)
-                        atomicAnd(reinterpret_cast<unsigned long long*>(&SA[r].lo),
-                                  ~(1ull << i));
+                        atomicAnd(&SAL[r], ~(1ull << i));
                     else
-                        atomicAnd(&SA[r].hi, ~(1u << (i - 64)));
+                        atomicAnd(&SH4[r >> 2], ~(1u << (8 * (r & 3) + (i - 64))));
                     const int rx = r / kRY, ry = r % kRY;
                     if (rx >= kHX && rx < kHX + kTT && ry >= kHY &&
                         ry < kHY + kTT && i >= kHZ && i < kHZ + 64)
@@ -1331,7 +1336,7 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
             const int x = X0 + rx, y = Y0 + ry;
             if (x < d.nx && y < d.ny)
             {
-                const TRow t = SA[rx * kRY + ry];
+                const TRow t = row(rx * kRY + ry);
                 Pout[((int64_t)x * d.ny + y) * nwz + w] =
                     (t.lo >> kHZ) | ((uint64_t)t.hi << (64 - kHZ));
             }
