@@ -75,3 +75,19 @@ def test_edt_config1_torus(vc, oracle):
 def test_edt_config2_spheres(vc, oracle):
     from voxel_ma_amd import synth
     _check(vc, oracle, synth.spheres_c2(), "c2")
+
+
+def test_edt_rows_multiple_of_8(vc, oracle):
+    """Rows whose length is a multiple of 8 take the 8-voxels-per-lane row
+    pass: lengths that are and are not multiples of 64, sparse background
+    (searches across words), rows without background, and consecutive rows
+    of very different content (the LDS row masks are reused)."""
+    rng = np.random.default_rng(17)
+    for nz in (8, 16, 24, 64, 72, 136, 520, 1032):
+        for fill in (0.5, 0.9, 0.995):
+            occ = (rng.random((5, 6, nz)) < fill).astype(np.uint8)
+            occ[1, 2, :] = 1          # a row without background
+            occ[3, 0, :] = 0          # a row of background only
+            occ[4, 5, :] = 1
+            occ[4, 5, nz - 1] = 0     # background only at the row's end
+            _check(vc, oracle, occ, "z8 %d %.3f" % (nz, fill))

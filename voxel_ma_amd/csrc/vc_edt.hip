@@ -30,8 +30,9 @@ constexpr uint32_t kOob = 0xFFFFFFFEu; // neighbour outside the grid
 // whose squared diagonal exceeds INT32_MAX
 constexpr int kInf = 0x7FFFFFFF;
 
-// rows along z: wavefront = row; the row's background bitmask (one ballot
-// per 64 voxels) sits in the wavefront's LDS slice of `stride` words
+// rows along z: wavefront = row; the row's background bitmask sits in the
+// wavefront's LDS slice of `stride` words (rows whose length is a multiple
+// of 8: 8 voxels per lane and access; others: one ballot per 64 voxels)
 template <bool kB>
 __global__ void __launch_bounds__(kBlock)
     edt_pass_z(vc_work W, int stride)
@@ -51,6 +52,91 @@ __global__ void __launch_bounds__(kBlock)
         const bool active = row < (int64_t)d.nx * d.ny;
         const uint8_t* r = d.occ + row * nz;
         const int nw = (nz + 63) >> 6;
+        if ((nz & 7) == 0)
+        {
+            // 8 voxels per lane: one 8-byte load, a background byte by SWAR
+            // zero-byte detection, one nearest-background search per side of
+            // the lane's block, one 16-byte store
+            uint8_t* bgb = reinterpret_cast<uint8_t*>(bgmask);
+            if (active)
+            {
+                // the bytes of the last word past the row end hold no
+                // background
+                for (int q = (nz >> 3) + lane; q < nw * 8; q += 64)
+                    bgb[q] = 0;
+                for (int z0 = lane * 8; z0 < nz; z0 += 512)
+                {
+                    const uint64_t v =
+                        *reinterpret_cast<const uint64_t*>(r + z0);
+                    constexpr uint64_t k7f = 0x7F7F7F7F7F7F7F7Full;
+                    const uint64_t zb = ~(((v & k7f) + k7f) | v | k7f);
+                    bgb[z0 >> 3] = (uint8_t)(((zb >> 7) * 0x0102040810204080ull) >> 56);
+                }
+            }
+            __syncthreads();
+            if (active)
+            {
+                int16_t* out = reinterpret_cast<int16_t*>(d.edt) + row * nz;
+                for (int z0 = lane * 8; z0 < nz; z0 += 512)
+                {
+                    const int w = z0 >> 6, k = (z0 >> 3) & 7;
+                    const uint64_t m = bgmask[w];
+                    const uint32_t b = (uint32_t)(m >> (8 * k)) & 0xFFu;
+                    int Lb = -1, Rb = -1;
+                    const uint64_t lm = k ? m & ((1ull << (8 * k)) - 1ull) : 0ull;
+                    if (lm)
+                        Lb = (w << 6) + 63 - __clzll(lm);
+                    else
+                        for (int ww = w - 1; ww >= 0; --ww)
+                        {
+                            const uint64_t q = bgmask[ww];
+                            if (q)
+                            {
+                                Lb = (ww << 6) + 63 - __clzll(q);
+                                break;
+                            }
+                        }
+                    const uint64_t rm = k < 7 ? m & ~((1ull << (8 * k + 8)) - 1ull) : 0ull;
+                    if (rm)
+                        Rb = (w << 6) + __ffsll((unsigned long long)rm) - 1;
+                    else
+                        for (int ww = w + 1; ww < nw; ++ww)
+                        {
+                            const uint64_t q = bgmask[ww];
+                            if (q)
+                            {
+                                Rb = (ww << 6) + __ffsll((unsigned long long)q) - 1;
+                                break;
+                            }
+                        }
+                    uint32_t packed[4];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                    {
+                        const int z = z0 + j;
+                        int res;
+                        if ((b >> j) & 1u)
+                            res = z;
+                        else
+                        {
+                            const uint32_t below = b & ((1u << j) - 1u);
+                            const uint32_t above = b & ~((2u << j) - 1u);
+                            const int L = below ? z0 + 31 - __clz(below) : Lb;
+                            const int R = above ? z0 + __ffs(above) - 1 : Rb;
+                            res = L < 0 ? R : (R < 0 ? L : (z - L <= R - z ? L : R));
+                        }
+                        if (j & 1)
+                            packed[j >> 1] |= (uint32_t)(uint16_t)res << 16;
+                        else
+                            packed[j >> 1] = (uint32_t)(uint16_t)res;
+                    }
+                    *reinterpret_cast<uint4*>(out + z0) =
+                        make_uint4(packed[0], packed[1], packed[2], packed[3]);
+                }
+            }
+            __syncthreads(); // the slice is rewritten by the next item
+            continue;
+        }
         if (active)
             for (int w = 0; w < nw; ++w)
             {
