@@ -92,3 +92,30 @@ def test_boundary_vectorised_rows_match_oracle(vc, oracle):
         shape = (int(rng.integers(1, 20)), int(rng.integers(1, 20)), 8 * int(rng.integers(1, 6)))
         occ = (rng.random(shape) < rng.choice([0.2, 0.5, 0.8, 0.97])).astype(np.uint8)
         _compare(vc, oracle, occ, "vec%d" % it)
+
+
+def test_boundary_vectorised_edge_cases(vc, oracle):
+    """Euler terms and thin-layer components on the 8-voxels-per-thread path:
+    full and empty grids, voxels at the two far corners (the lattice points
+    at z = nz and the truncated base voxel at coordinate 0), runs of layer
+    voxels that cross 8-voxel chunks, and blurred blobs with many layer
+    components of several sizes."""
+    import scipy.ndimage as nd
+    cases = {"full8": np.ones((5, 6, 8), np.uint8), "empty8": np.zeros((3, 4, 16), np.uint8),
+             "single8": np.zeros((1, 1, 8), np.uint8)}
+    cases["single8"][0, 0, 7] = 1
+    for name, idx in (("corner0", (0, 0, 0)), ("corner1", (-1, -1, -1)), ("mid7", (2, 2, 7))):
+        occ = np.zeros((4, 5, 16), np.uint8)
+        occ[idx] = 1
+        cases[name] = occ
+    shell = np.zeros((12, 12, 24), np.uint8)
+    shell[1:11, 1:11, 3:21] = 1
+    shell[4:8, 4:8, 7:17] = 0
+    cases["shell"] = shell
+    rng = np.random.default_rng(29)
+    for i, (shape, sig, fill) in enumerate((((40, 33, 64), 2.0, 0.5), ((17, 48, 40), 1.2, 0.3),
+                                            ((32, 32, 96), 3.0, 0.7))):
+        v = nd.gaussian_filter(rng.standard_normal(shape), sig)
+        cases["blob%d" % i] = (v > np.quantile(v, 1 - fill)).astype(np.uint8)
+    for name, occ in cases.items():
+        _compare(vc, oracle, occ, name)

@@ -31,24 +31,76 @@ namespace
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------- reference tables
-// face offsets o: -x, +x, -y, +y, -z, +z (include/surfacing.h:172-178)
-__constant__ int8_t kOff[6][3] = {{-1, 0, 0}, {1, 0, 0},  {0, -1, 0},
-                                  {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
-// corners shared with the neighbour across offset o (surfacing.h:185-190)
-__constant__ int8_t kCwrt[6][4] = {{2, 3, 7, 6}, {0, 1, 5, 4}, {4, 6, 2, 0},
-                                   {1, 3, 7, 5}, {0, 2, 3, 1}, {5, 7, 6, 4}};
-// face index of offset o (surfacing.h:191-197)
-__constant__ int8_t kFwrt[6] = {3, 2, 4, 5, 0, 1};
-// corner slots of face fi (surfacing.h:144-149)
-__constant__ int8_t kFcrn[6][4] = {{0, 1, 3, 2}, {4, 5, 7, 6}, {0, 4, 5, 1},
-                                   {2, 6, 7, 3}, {4, 0, 2, 6}, {5, 1, 3, 7}};
-// corner c of voxel (i,j,k) is at (i,j,k) + 0.5 * kSgn[c] (surfacing.h:97-119)
-__constant__ int8_t kSgn[8][3] = {{1, -1, -1}, {1, 1, -1},  {-1, -1, -1},
-                                  {-1, 1, -1}, {1, -1, 1},  {1, 1, 1},
-                                  {-1, -1, 1}, {-1, 1, 1}};
+// Tables of the reference Surfacer, as compile-time functions (run-time
+// lookups go through packed integer constants, never through local arrays).
+struct Tab
+{
+    int8_t off[6][3];  // face offsets o: -x, +x, -y, +y, -z, +z (include/surfacing.h:172-178)
+    int8_t cwrt[6][4]; // corners shared with the neighbour across offset o (surfacing.h:185-190)
+    int8_t fwrt[6];    // face index of offset o (surfacing.h:191-197)
+    int8_t fcrn[6][4]; // corner slots of face fi (surfacing.h:144-149)
+    int8_t sgn[8][3];  // corner c of voxel (i,j,k) is at (i,j,k) + 0.5 * sgn[c] (surfacing.h:97-119)
+};
+constexpr Tab kT = {
+    {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}},
+    {{2, 3, 7, 6}, {0, 1, 5, 4}, {4, 6, 2, 0}, {1, 3, 7, 5}, {0, 2, 3, 1}, {5, 7, 6, 4}},
+    {3, 2, 4, 5, 0, 1},
+    {{0, 1, 3, 2}, {4, 5, 7, 6}, {0, 4, 5, 1}, {2, 6, 7, 3}, {4, 0, 2, 6}, {5, 1, 3, 7}},
+    {{1, -1, -1}, {1, 1, -1}, {-1, -1, -1}, {-1, 1, -1}, {1, -1, 1}, {1, 1, 1}, {-1, -1, 1},
+     {-1, 1, 1}}};
+
+// corner signs in closed form: x is + for slots {0,1,4,5}, y for odd slots,
+// z for slots >= 4
+__host__ __device__ constexpr int csx(int c) { return (c & 2) ? -1 : 1; }
+__host__ __device__ constexpr int csy(int c) { return (c & 1) ? 1 : -1; }
+__host__ __device__ constexpr int csz(int c) { return (c & 4) ? 1 : -1; }
+constexpr bool signs_ok()
+{
+    for (int c = 0; c < 8; ++c)
+        if (csx(c) != kT.sgn[c][0] || csy(c) != kT.sgn[c][1] || csz(c) != kT.sgn[c][2])
+            return false;
+    return true;
+}
+static_assert(signs_ok(), "corner sign formulas");
+
+// slot ii of corner c in cwrt[o] (every corner on the side of o is listed)
+constexpr int cwrt_pos(int o, int c)
+{
+    for (int ii = 0; ii < 4; ++ii)
+        if (kT.cwrt[o][ii] == c)
+            return ii;
+    return -1;
+}
+// first-visit key 4 o + ii -> corner slot, 3 bits per key (keys 0-15, 16-23)
+constexpr uint64_t key_bits(int from)
+{
+    uint64_t r = 0;
+    for (int k = from; k < 24 && k < from + 16; ++k)
+        r |= (uint64_t)kT.cwrt[k >> 2][k & 3] << (3 * (k - from));
+    return r;
+}
+__device__ __forceinline__ int key_corner(int k)
+{
+    constexpr uint64_t lo = key_bits(0), hi = key_bits(16);
+    return (int)(((k < 16) ? (lo >> (3 * k)) : (hi >> (3 * (k - 16)))) & 7u);
+}
+// the 4 corner slots of the face across offset o, 12 bits per offset
+constexpr uint64_t face_bits(int from)
+{
+    uint64_t r = 0;
+    for (int o = from; o < 6 && o < from + 5; ++o)
+        for (int t = 0; t < 4; ++t)
+            r |= (uint64_t)kT.fcrn[kT.fwrt[o]][t] << (12 * (o - from) + 3 * t);
+    return r;
+}
+__device__ __forceinline__ uint32_t face_corners(int o)
+{
+    constexpr uint64_t lo = face_bits(0), hi = face_bits(5);
+    return (uint32_t)((o < 5 ? (lo >> (12 * o)) : hi) & 0xFFFu);
+}
 
 // neighbourhood bit of offset (dx, dy, dz): dz+1 + 3(dy+1) + 9(dx+1)
-__device__ __forceinline__ int nbit(int dx, int dy, int dz)
+__host__ __device__ constexpr int nbit(int dx, int dy, int dz)
 {
     return (dz + 1) + 3 * (dy + 1) + 9 * (dx + 1);
 }
@@ -64,51 +116,47 @@ __device__ __forceinline__ bool obit(uint32_t m, int dx, int dy, int dz)
     return (m >> nbit(dx, dy, dz)) & 1u;
 }
 
-// Decisions of one voxel: new corners (slot order by first visit) and new
-// faces (by offset).
+// Decisions of one voxel as masks: new corners by first-visit key 4 o + ii
+// (emitted in ascending key order; distinct corners have distinct keys) and
+// new faces by offset o (emitted in ascending o).
 struct Emit
 {
-    int nsite;
-    int8_t corner[8]; // corner slots in emission order
-    int nface;
-    int8_t face_o[6]; // offsets in emission order
+    uint32_t keys;
+    uint32_t faces;
 };
 
-__device__ Emit decide(const Vox& V)
+__device__ __forceinline__ Emit decide(const Vox& V)
 {
-    Emit r;
-    r.nsite = 0;
-    r.nface = 0;
+    Emit r{0u, 0u};
     const bool me = obit(V.occ, 0, 0, 0);
     // events of this voxel
     uint32_t ev = 0;
 #pragma unroll
     for (int o = 0; o < 6; ++o)
-        if (obit(V.occ, kOff[o][0], kOff[o][1], kOff[o][2]) != me)
+        if (obit(V.occ, kT.off[o][0], kT.off[o][1], kT.off[o][2]) != me)
             ev |= 1u << o;
     if (!ev)
         return r;
-    // corners: key = 4*o + ii of the first touching offset, if first visitor
-    int key[8];
-    int cnt = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {
-        const int sx = kSgn[c][0], sy = kSgn[c][1], sz = kSgn[c][2];
+        const int sx = csx(c), sy = csy(c), sz = csz(c);
         const int ox = sx > 0 ? 1 : 0, oy = sy > 0 ? 3 : 2, oz = sz > 0 ? 5 : 4;
-        int o = -1;
+        // key of the first touching offset
+        int key;
         if (ev & (1u << ox))
-            o = ox;
+            key = 4 * ox + cwrt_pos(ox, c);
         else if (ev & (1u << oy))
-            o = oy;
+            key = 4 * oy + cwrt_pos(oy, c);
         else if (ev & (1u << oz))
-            o = oz;
-        if (o < 0)
+            key = 4 * oz + cwrt_pos(oz, c);
+        else
             continue;
         // an earlier (linear order = lexicographic block order) in-grid
         // block voxel touching the corner visits it first
         bool earlier = false;
         const int lx = sx > 0 ? 0 : -1, ly = sy > 0 ? 0 : -1, lz = sz > 0 ? 0 : -1;
+#pragma unroll
         for (int q = 0; q < 8; ++q)
         {
             const int a = q >> 2, b = (q >> 1) & 1, e = q & 1;
@@ -119,46 +167,20 @@ __device__ Emit decide(const Vox& V)
                 continue;
             const bool u = obit(V.occ, ux, uy, uz);
             const int vx = lx + (1 - a), vy = ly + (1 - b), vz = lz + (1 - e);
-            if (obit(V.occ, vx, uy, uz) != u || obit(V.occ, ux, vy, uz) != u ||
-                obit(V.occ, ux, uy, vz) != u)
-            {
-                earlier = true;
-                break;
-            }
+            earlier |= obit(V.occ, vx, uy, uz) != u || obit(V.occ, ux, vy, uz) != u ||
+                       obit(V.occ, ux, uy, vz) != u;
         }
-        if (earlier)
-            continue;
-        int ii = 0;
-        while (kCwrt[o][ii] != c)
-            ++ii;
-        key[cnt] = 4 * o + ii;
-        r.corner[cnt] = (int8_t)c;
-        ++cnt;
+        if (!earlier)
+            r.keys |= 1u << key;
     }
-    // insertion sort by key (at most 8)
-    for (int i = 1; i < cnt; ++i)
-    {
-        const int kk = key[i];
-        const int8_t cc = r.corner[i];
-        int j = i - 1;
-        while (j >= 0 && key[j] > kk)
-        {
-            key[j + 1] = key[j];
-            r.corner[j + 1] = r.corner[j];
-            --j;
-        }
-        key[j + 1] = kk;
-        r.corner[j + 1] = cc;
-    }
-    r.nsite = cnt;
     // faces created here: +offsets, and -offsets whose neighbour is outside
 #pragma unroll
     for (int o = 0; o < 6; ++o)
         if ((ev >> o) & 1u)
         {
-            const bool nb_in = obit(V.ing, kOff[o][0], kOff[o][1], kOff[o][2]);
+            const bool nb_in = obit(V.ing, kT.off[o][0], kT.off[o][1], kT.off[o][2]);
             if ((o & 1) || !nb_in)
-                r.face_o[r.nface++] = (int8_t)o;
+                r.faces |= 1u << o;
         }
     return r;
 }
@@ -225,8 +247,7 @@ __device__ __forceinline__ void decode(const Dims& D, int64_t i, int& x,
 __device__ __forceinline__ int64_t corner_id(const Dims& D, int x, int y,
                                              int z, int c)
 {
-    const int cx = x + (kSgn[c][0] > 0), cy = y + (kSgn[c][1] > 0),
-              cz = z + (kSgn[c][2] > 0);
+    const int cx = x + (csx(c) > 0), cy = y + (csy(c) > 0), cz = z + (csz(c) > 0);
     return ((int64_t)cx * (D.ny + 1) + cy) * (D.nz + 1) + cz;
 }
 
@@ -271,8 +292,8 @@ __global__ void __launch_bounds__(kBlock)
         if (face_event(occ, D, x, y, z))
         {
             const Emit e = decide(gather(occ, D, x, y, z));
-            ns = (uint32_t)e.nsite;
-            nf = (uint32_t)e.nface;
+            ns = (uint32_t)__builtin_popcount(e.keys);
+            nf = (uint32_t)__builtin_popcount(e.faces);
         }
     }
     uint32_t tot;
@@ -368,8 +389,44 @@ __device__ __forceinline__ bool chunk_vox(const Chunk& K, const Dims& D, int j,
     return true;
 }
 
+// emission of one voxel's sites (coordinates + corner map) and quads
+__device__ __forceinline__ void put_sites(const Emit& e, const Dims& D, int x, int y,
+                                          int z, uint32_t s, float* __restrict__ sites,
+                                          uint32_t* __restrict__ cmap)
+{
+    for (uint32_t k = e.keys; k; k &= k - 1, ++s)
+    {
+        const int cc = key_corner(__builtin_ctz(k));
+        float* p = sites + 3 * (int64_t)s;
+        p[0] = (float)x + 0.5f * (float)csx(cc);
+        p[1] = (float)y + 0.5f * (float)csy(cc);
+        p[2] = (float)z + 0.5f * (float)csz(cc);
+        cmap[corner_id(D, x, y, z, cc)] = s;
+    }
+}
+
+__device__ __forceinline__ void put_quads(const Emit& e, const Dims& D, int x, int y,
+                                          int z, uint32_t f,
+                                          const uint32_t* __restrict__ cmap,
+                                          int32_t* __restrict__ quads)
+{
+    for (uint32_t k = e.faces; k; k &= k - 1, ++f)
+    {
+        const uint32_t fc = face_corners(__builtin_ctz(k));
+        int4 q;
+        q.x = (int32_t)cmap[corner_id(D, x, y, z, fc & 7u)];
+        q.y = (int32_t)cmap[corner_id(D, x, y, z, (fc >> 3) & 7u)];
+        q.z = (int32_t)cmap[corner_id(D, x, y, z, (fc >> 6) & 7u)];
+        q.w = (int32_t)cmap[corner_id(D, x, y, z, (fc >> 9) & 7u)];
+        *reinterpret_cast<int4*>(quads + 4 * (int64_t)f) = q;
+    }
+}
+
+// pass 1 (vectorised): per-chunk counts (sites << 16 | faces) and per-tile
+// totals; the emit passes read the chunk counts and skip empty chunks
 __global__ void __launch_bounds__(kBlock)
     bnd_count_v(const uint8_t* __restrict__ occ, Dims D,
+                uint32_t* __restrict__ chunk_cnt,
                 uint32_t* __restrict__ tile_sites,
                 uint32_t* __restrict__ tile_faces)
 {
@@ -378,6 +435,7 @@ __global__ void __launch_bounds__(kBlock)
     uint32_t ns = 0, nf = 0;
     Chunk K;
     if (load_chunk(occ, D, c, K))
+    {
 #pragma unroll
         for (int j = 0; j < kVz; ++j)
         {
@@ -385,10 +443,12 @@ __global__ void __launch_bounds__(kBlock)
             if (chunk_vox(K, D, j, V))
             {
                 const Emit e = decide(V);
-                ns += (uint32_t)e.nsite;
-                nf += (uint32_t)e.nface;
+                ns += (uint32_t)__builtin_popcount(e.keys);
+                nf += (uint32_t)__builtin_popcount(e.faces);
             }
         }
+        chunk_cnt[c] = (ns << 16) | nf;
+    }
     uint32_t tot;
     block_scan_excl((ns << 16) | nf, sh, &tot);
     if (threadIdx.x == 0)
@@ -400,80 +460,52 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock)
     bnd_emit_sites_v(const uint8_t* __restrict__ occ, Dims D,
+                     const uint32_t* __restrict__ chunk_cnt,
                      const uint32_t* __restrict__ tile_sites,
                      float* __restrict__ sites, uint32_t* __restrict__ cmap)
 {
     __shared__ uint32_t sh[kBlock / 64];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Chunk K;
-    const bool ok = load_chunk(occ, D, c, K);
-    uint32_t ns = 0;
-    if (ok)
-#pragma unroll
-        for (int j = 0; j < kVz; ++j)
-        {
-            Vox V;
-            if (chunk_vox(K, D, j, V))
-                ns += (uint32_t)decide(V).nsite;
-        }
+    const int64_t nch = (int64_t)D.nx * D.ny * (D.nz / kVz);
+    const uint32_t ns = c < nch ? chunk_cnt[c] >> 16 : 0u;
     uint32_t tot;
     uint32_t s = tile_sites[blockIdx.x] + block_scan_excl(ns, sh, &tot);
-    if (ok && ns)
+    Chunk K;
+    if (ns && load_chunk(occ, D, c, K))
         for (int j = 0; j < kVz; ++j)
         {
             Vox V;
             if (!chunk_vox(K, D, j, V))
                 continue;
             const Emit e = decide(V);
-            const int z = K.z0 + j;
-            for (int q = 0; q < e.nsite; ++q, ++s)
-            {
-                const int cc = e.corner[q];
-                sites[3 * (int64_t)s + 0] = (float)K.x + 0.5f * kSgn[cc][0];
-                sites[3 * (int64_t)s + 1] = (float)K.y + 0.5f * kSgn[cc][1];
-                sites[3 * (int64_t)s + 2] = (float)z + 0.5f * kSgn[cc][2];
-                cmap[corner_id(D, K.x, K.y, z, cc)] = s;
-            }
+            put_sites(e, D, K.x, K.y, K.z0 + j, s, sites, cmap);
+            s += (uint32_t)__builtin_popcount(e.keys);
         }
 }
 
 __global__ void __launch_bounds__(kBlock)
     bnd_emit_quads_v(const uint8_t* __restrict__ occ, Dims D,
+                     const uint32_t* __restrict__ chunk_cnt,
                      const uint32_t* __restrict__ tile_faces,
                      const uint32_t* __restrict__ cmap,
                      int32_t* __restrict__ quads)
 {
     __shared__ uint32_t sh[kBlock / 64];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Chunk K;
-    const bool ok = load_chunk(occ, D, c, K);
-    uint32_t nf = 0;
-    if (ok)
-#pragma unroll
-        for (int j = 0; j < kVz; ++j)
-        {
-            Vox V;
-            if (chunk_vox(K, D, j, V))
-                nf += (uint32_t)decide(V).nface;
-        }
+    const int64_t nch = (int64_t)D.nx * D.ny * (D.nz / kVz);
+    const uint32_t nf = c < nch ? chunk_cnt[c] & 0xFFFFu : 0u;
     uint32_t tot;
     uint32_t f = tile_faces[blockIdx.x] + block_scan_excl(nf, sh, &tot);
-    if (ok && nf)
+    Chunk K;
+    if (nf && load_chunk(occ, D, c, K))
         for (int j = 0; j < kVz; ++j)
         {
             Vox V;
             if (!chunk_vox(K, D, j, V))
                 continue;
             const Emit e = decide(V);
-            const int z = K.z0 + j;
-            for (int q = 0; q < e.nface; ++q, ++f)
-            {
-                const int fi = kFwrt[e.face_o[q]];
-                int32_t* qd = quads + 4 * (int64_t)f;
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    qd[t] = (int32_t)cmap[corner_id(D, K.x, K.y, z, kFcrn[fi][t])];
-            }
+            put_quads(e, D, K.x, K.y, K.z0 + j, f, cmap, quads);
+            f += (uint32_t)__builtin_popcount(e.faces);
         }
 }
 
@@ -526,8 +558,7 @@ __global__ void __launch_bounds__(kBlock)
 {
     __shared__ uint32_t sh[kBlock / 64];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Emit e;
-    e.nsite = 0;
+    Emit e{0u, 0u};
     int x = 0, y = 0, z = 0;
     if (i < N)
     {
@@ -536,17 +567,9 @@ __global__ void __launch_bounds__(kBlock)
             e = decide(gather(occ, D, x, y, z));
     }
     uint32_t tot;
-    const uint32_t base =
-        tile_sites[blockIdx.x] + block_scan_excl((uint32_t)e.nsite, sh, &tot);
-    for (int j = 0; j < e.nsite; ++j)
-    {
-        const int c = e.corner[j];
-        const uint32_t s = base + j;
-        sites[3 * (int64_t)s + 0] = (float)x + 0.5f * kSgn[c][0];
-        sites[3 * (int64_t)s + 1] = (float)y + 0.5f * kSgn[c][1];
-        sites[3 * (int64_t)s + 2] = (float)z + 0.5f * kSgn[c][2];
-        cmap[corner_id(D, x, y, z, c)] = s;
-    }
+    const uint32_t base = tile_sites[blockIdx.x] +
+                          block_scan_excl((uint32_t)__builtin_popcount(e.keys), sh, &tot);
+    put_sites(e, D, x, y, z, base, sites, cmap);
 }
 
 // pass 4: quads through the corner map
@@ -557,8 +580,7 @@ __global__ void __launch_bounds__(kBlock)
 {
     __shared__ uint32_t sh[kBlock / 64];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Emit e;
-    e.nface = 0;
+    Emit e{0u, 0u};
     int x = 0, y = 0, z = 0;
     if (i < N)
     {
@@ -567,16 +589,9 @@ __global__ void __launch_bounds__(kBlock)
             e = decide(gather(occ, D, x, y, z));
     }
     uint32_t tot;
-    const uint32_t base =
-        tile_faces[blockIdx.x] + block_scan_excl((uint32_t)e.nface, sh, &tot);
-    for (int j = 0; j < e.nface; ++j)
-    {
-        const int fi = kFwrt[e.face_o[j]];
-        int32_t* q = quads + 4 * (int64_t)(base + j);
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            q[t] = (int32_t)cmap[corner_id(D, x, y, z, kFcrn[fi][t])];
-    }
+    const uint32_t base = tile_faces[blockIdx.x] +
+                          block_scan_excl((uint32_t)__builtin_popcount(e.faces), sh, &tot);
+    put_quads(e, D, x, y, z, base, cmap, quads);
 }
 
 // ------------------------------------------------------------- Euler terms
@@ -878,6 +893,261 @@ __global__ void __launch_bounds__(kBlock)
     block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
 }
 
+// ---- vectorised Euler terms and thin layer (nz a multiple of 8).  The same
+// 10-bit row windows as the site / quad passes; one thread per 8 consecutive
+// z of a row, so every neighbour read of the per-voxel kernels above becomes
+// a bit of one of nine windows (27 loads per 8 voxels instead of up to 27
+// per voxel).
+
+// block sum of the eight Euler terms, one atomic per term and block
+__device__ __forceinline__ void block_add8(const uint32_t (&v)[8],
+                                           unsigned long long (*sh)[kBlock / 64],
+                                           unsigned long long* a)
+{
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        uint32_t s = v[k];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+            s += __shfl_xor(s, d);
+        if ((threadIdx.x & 63) == 0)
+            sh[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8)
+    {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            t += sh[threadIdx.x][w];
+        // accumulator of term k: V_bnd, E_bnd, Vint_sum, E_int, F_int, T,
+        // chi positive, chi negative
+        const int dst = threadIdx.x < 6 ? (int)threadIdx.x : (int)threadIdx.x + 2;
+        if (t)
+            atomicAdd(&a[dst], t);
+    }
+}
+
+// Lattice row (cx, cy) in [0, nx] x [0, ny], lattice points z0 .. z0 + 7 (and
+// nz on the row's last chunk); voxel (cx, cy, z) for the same z when the row
+// is a voxel row.  Window w[dx][dy] is voxel row (cx + dx - 1, cy + dy - 1)
+// and its bit b is voxel z0 - 1 + b, so lattice point cz = z0 + j sees
+// voxels cz - 1, cz at bits j, j + 1.  Terms and conventions as chi_lattice,
+// euler_lattice and euler_voxels.
+__global__ void __launch_bounds__(kBlock)
+    euler_rows_v(const uint8_t* __restrict__ occ, Dims D,
+                 const int32_t* __restrict__ tbl, unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[8][kBlock / 64];
+    const int nzc = D.nz / kVz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // V_bnd, E_bnd, Vint_sum, E_int, F_int, T, chi+, chi-
+    uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (c < (int64_t)(D.nx + 1) * (D.ny + 1) * nzc)
+    {
+        const int64_t row = c / nzc;
+        const int z0 = (int)(c - row * nzc) * kVz;
+        const int cx = (int)(row / (D.ny + 1));
+        const int cy = (int)(row - (int64_t)cx * (D.ny + 1));
+        uint32_t w[3][3];
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                w[dx][dy] = row_win(occ, D, cx + dx - 1, cy + dy - 1, z0);
+        const bool vrow = cx < D.nx && cy < D.ny;
+        const int jn = z0 + kVz == D.nz ? kVz + 1 : kVz;
+        // rows of the site mask: base voxel (int)-truncated at coordinate 0,
+        // so the block is rows cx - 1, cx (cx > 0) or 0, 1 (cx = 0), same in y
+        uint32_t sr[2][2];
+#pragma unroll
+        for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+            for (int iy = 0; iy < 2; ++iy)
+            {
+                const uint32_t a0 = cy > 0 ? w[ix][iy] : w[ix][iy + 1];
+                const uint32_t a1 = cy > 0 ? w[ix + 1][iy] : w[ix + 1][iy + 1];
+                sr[ix][iy] = cx > 0 ? a0 : a1;
+            }
+        for (int j = 0; j < jn; ++j)
+        {
+            const int cz = z0 + j;
+            // the 8 voxels (cx - 1 + a, cy - 1 + b, cz - 1 + e), bit 4a + 2b + e
+            const uint32_t m = ((w[0][0] >> j) & 3u) | ((w[0][1] >> j) & 3u) << 2 |
+                               ((w[1][0] >> j) & 3u) << 4 | ((w[1][1] >> j) & 3u) << 6;
+            s[6] += (m != 0) + ((m & 0xC0u) != 0) + ((m & 0xA0u) != 0) +
+                    ((m & 0x88u) != 0);
+            s[7] += ((m & 0xF0u) != 0) + ((m & 0xCCu) != 0) + ((m & 0xAAu) != 0) +
+                    ((m & 0x80u) != 0);
+            const int blk = __builtin_popcount(m);
+            if (blk == 8)
+                s[3] += (cx < D.nx) + (cy < D.ny) + (cz < D.nz);
+            else if (blk != 0)
+            {
+                const int sz0 = cz > 0 ? j : j + 1;
+                uint32_t mask = 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                {
+                    const int ix = (i >> 2) & 1, iy = (i >> 1) & 1, iz = i & 1;
+                    mask |= ((sr[ix][iy] >> (sz0 + iz)) & 1u) << (7 - i);
+                }
+                s[0] += (uint32_t)tbl[mask];
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax)
+                {
+                    const int lim = ax == 0 ? D.nx : (ax == 1 ? D.ny : D.nz);
+                    const int cc = ax == 0 ? cx : (ax == 1 ? cy : cz);
+                    if (cc >= lim)
+                        continue;
+                    uint32_t v4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int u = (q == 1 || q == 2) ? 0 : -1; // cyclic square
+                        const int wq = (q >= 2) ? 0 : -1;
+                        int p[3] = {0, 0, 0};
+                        p[(ax + 1) % 3] += u;
+                        p[(ax + 2) % 3] += wq;
+                        v4[q] = (w[p[0] + 1][p[1] + 1] >> (j + p[2] + 1)) & 1u;
+                    }
+                    const uint32_t ch = (v4[0] != v4[1]) + (v4[1] != v4[2]) +
+                                        (v4[2] != v4[3]) + (v4[3] != v4[0]);
+                    s[1] += ch == 2 ? 1u : (ch == 4 ? 2u : 0u);
+                    s[3] += (v4[0] & v4[1] & v4[2] & v4[3]);
+                }
+            }
+            if (!vrow || j >= kVz || !((w[1][1] >> (j + 1)) & 1u))
+                continue;
+            // inside voxel (cx, cy, cz): T, F_int and the interior-corner sum
+            uint32_t m27 = 0;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+                    m27 |= ((w[dx][dy] >> j) & 7u) << nbit(dx - 1, dy - 1, -1);
+            s[5] += 1;
+            s[4] += obit(m27, -1, 0, 0) + obit(m27, 1, 0, 0) + obit(m27, 0, -1, 0) +
+                    obit(m27, 0, 1, 0) + obit(m27, 0, 0, -1) + obit(m27, 0, 0, 1);
+            // corner block of slot c starts at offset -1, or 0 on the high
+            // side or at coordinate 0; block (-1,-1,-1) is bits 0x361B
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                const int ax = (((q >> 2) & 1) || cx == 0) ? 0 : -1;
+                const int ay = (((q >> 1) & 1) || cy == 0) ? 0 : -1;
+                const int az = ((q & 1) || cz == 0) ? 0 : -1;
+                const uint32_t bm = 0x361Bu << ((az + 1) + 3 * (ay + 1) + 9 * (ax + 1));
+                s[2] += (m27 & bm) == bm;
+            }
+        }
+    }
+    block_add8(s, sh, acc + (blockIdx.x % kSlots) * kAcc);
+}
+
+// boundary voxels of 8 consecutive z: inside with a non-inside face
+// neighbour (outside the grid counts as non-inside), one 8-byte store
+__global__ void __launch_bounds__(kBlock)
+    layer_boundary_v(const uint8_t* __restrict__ occ, Dims D,
+                     uint8_t* __restrict__ bnd)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Chunk K;
+    if (!load_chunk(occ, D, c, K))
+        return;
+    const uint32_t me = K.w[1][1] >> 1;
+    const uint32_t all6 = (K.w[1][1] & (K.w[1][1] >> 2)) & (K.w[0][1] >> 1) &
+                          (K.w[2][1] >> 1) & (K.w[1][0] >> 1) & (K.w[1][2] >> 1);
+    const uint32_t b = me & ~all6 & 0xFFu;
+    uint64_t bytes = 0; // bit j -> byte j
+#pragma unroll
+    for (int j = 0; j < kVz; ++j)
+        bytes |= (uint64_t)((b >> j) & 1u) << (8 * j);
+    *reinterpret_cast<uint64_t*>(
+        bnd + ((int64_t)K.x * D.ny + K.y) * D.nz + K.z0) = bytes;
+}
+
+// thin-layer labels of 8 consecutive z: -1 outside the layer, else the
+// index of the first voxel of its run of layer voxels within the 8 (a run
+// is z-connected, so it starts as one union-find tree)
+__global__ void __launch_bounds__(kBlock)
+    layer_init_v(const uint8_t* __restrict__ occ,
+                 const uint8_t* __restrict__ bnd, Dims D,
+                 int32_t* __restrict__ lab)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Chunk B;
+    if (!load_chunk(bnd, D, c, B))
+        return;
+    const int64_t base = ((int64_t)B.x * D.ny + B.y) * D.nz + B.z0;
+    const uint64_t o8 = *reinterpret_cast<const uint64_t*>(occ + base);
+    uint32_t near = 0; // bit b: a boundary voxel within z0 - 1 + b +- 1
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+            near |= B.w[dx][dy] | (B.w[dx][dy] >> 1) | (B.w[dx][dy] << 1);
+    int32_t out[kVz];
+    int32_t start = -1;
+#pragma unroll
+    for (int j = 0; j < kVz; ++j)
+    {
+        const bool in = ((o8 >> (8 * j)) & 0xFFu) && ((near >> (j + 1)) & 1u);
+        start = in ? (start < 0 ? (int32_t)(base + j) : start) : -1;
+        out[j] = start;
+    }
+    int4* dst = reinterpret_cast<int4*>(lab + base);
+    dst[0] = make_int4(out[0], out[1], out[2], out[3]);
+    dst[1] = make_int4(out[4], out[5], out[6], out[7]);
+}
+
+// unions of 8 consecutive z: runs are already trees, so z unions happen only
+// across chunks; the x / y union of a voxel is skipped when the voxel before
+// it in z and that voxel's x / y neighbour are both in the layer (the same
+// trees were joined through them)
+__global__ void __launch_bounds__(kBlock)
+    layer_union_v(int32_t* lab, Dims D, unsigned long long* guard_hits)
+{
+    const int nzc = D.nz / kVz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (int64_t)D.nx * D.ny * nzc)
+        return;
+    const int64_t row = c / nzc;
+    const int z0 = (int)(c - row * nzc) * kVz;
+    const int x = (int)(row / D.ny);
+    const int y = (int)(row - (int64_t)x * D.ny);
+    const int64_t sy = D.nz, sx = (int64_t)D.ny * D.nz;
+    const int64_t base = row * D.nz + z0;
+    auto members = [&](int64_t at) -> uint32_t {
+        const int4* p = reinterpret_cast<const int4*>(lab + at);
+        const int4 a = p[0], b = p[1];
+        return (uint32_t)(a.x >= 0) | (uint32_t)(a.y >= 0) << 1 |
+               (uint32_t)(a.z >= 0) << 2 | (uint32_t)(a.w >= 0) << 3 |
+               (uint32_t)(b.x >= 0) << 4 | (uint32_t)(b.y >= 0) << 5 |
+               (uint32_t)(b.z >= 0) << 6 | (uint32_t)(b.w >= 0) << 7;
+    };
+    const uint32_t me = members(base);
+    if (!me)
+        return;
+    const uint32_t mx = x + 1 < D.nx ? members(base + sx) & me : 0u;
+    const uint32_t my = y + 1 < D.ny ? members(base + sy) & me : 0u;
+    // pairs whose predecessor pair (z - 1) is also a layer pair
+    const uint32_t ux = mx & ~(mx << 1), uy = my & ~(my << 1);
+    for (uint32_t u = ux; u; u &= u - 1)
+    {
+        const int j = __builtin_ctz(u);
+        uf_union(lab, (int32_t)(base + j), (int32_t)(base + j + sx), guard_hits);
+    }
+    for (uint32_t u = uy; u; u &= u - 1)
+    {
+        const int j = __builtin_ctz(u);
+        uf_union(lab, (int32_t)(base + j), (int32_t)(base + j + sy), guard_hits);
+    }
+    if ((me >> 7) & 1u && z0 + kVz < D.nz && lab[base + kVz] >= 0)
+        uf_union(lab, (int32_t)(base + kVz - 1), (int32_t)(base + kVz), guard_hits);
+}
+
 // the reference's per-mask surface-component table, derived from first
 // principles (6-components of the inside corners + 18-components of the
 // outside corners - 1; four antipodal hole masks and 0/255 give 0)
@@ -978,7 +1248,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
     int rc = VC_OK;
     if (!v->bnd_done)
     {
-        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr;
+        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr, *ccnt = nullptr;
         if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)etiles)) != VC_OK)
             return rc;
         if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)etiles)) != VC_OK)
@@ -986,8 +1256,15 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             vc_free(ts);
             return rc;
         }
+        // per-chunk (sites, faces) counts of the vectorised passes
+        if (vec && (rc = vc_alloc(ctx, (void**)&ccnt, 4 * (size_t)(N / kVz))) != VC_OK)
+        {
+            vc_free(ts);
+            vc_free(tf);
+            return rc;
+        }
         if (vec)
-            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ts, tf);
+            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ccnt, ts, tf);
         else
             bnd_count<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
         hipError_t e = hipGetLastError();
@@ -1038,7 +1315,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
         {
             if (vec)
                 bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                    v->occ, D, ts, v->bnd_sites, cmap);
+                    v->occ, D, ccnt, ts, v->bnd_sites, cmap);
             else
                 bnd_emit_sites<<<(unsigned)etiles, kBlock, 0, S>>>(
                     v->occ, D, N, ts, v->bnd_sites, cmap);
@@ -1047,7 +1324,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             {
                 if (vec)
                     bnd_emit_quads_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                        v->occ, D, tf, cmap, v->bnd_quads);
+                        v->occ, D, ccnt, tf, cmap, v->bnd_quads);
                 else
                     bnd_emit_quads<<<(unsigned)etiles, kBlock, 0, S>>>(
                         v->occ, D, N, tf, cmap, v->bnd_quads);
@@ -1075,7 +1352,20 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                                    hipMemcpyHostToDevice, S);
             if (rc == VC_OK && e == hipSuccess)
                 e = hipMemsetAsync(acc, 0, 8 * kSlots * kAcc, S);
-            if (rc == VC_OK && e == hipSuccess)
+            if (rc == VC_OK && e == hipSuccess && vec)
+            {
+                const int64_t nzc = v->nz / kVz;
+                const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
+                const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
+                const unsigned gv = (unsigned)etiles;
+                euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
+                layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
+                layer_init_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
+                layer_union_v<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
+                layer_count<<<(unsigned)ntiles, kBlock, 0, S>>>(lab, N, acc);
+                e = hipGetLastError();
+            }
+            else if (rc == VC_OK && e == hipSuccess)
             {
                 const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
                               (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
@@ -1131,6 +1421,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
         vc_free(ts);
         vc_free(tf);
         vc_free(cmap);
+        vc_free(ccnt);
         if (rc != VC_OK)
             return rc;
         v->bnd_nsites = ns;
