@@ -91,3 +91,14 @@ def test_edt_rows_multiple_of_8(vc, oracle):
             occ[4, 5, :] = 1
             occ[4, 5, nz - 1] = 0     # background only at the row's end
             _check(vc, oracle, occ, "z8 %d %.3f" % (nz, fill))
+
+
+def test_edt_long_axes(vc, oracle):
+    """Axes longer than 16384: the y pass falls back from its 24-bit
+    offset arithmetic to the plain one (both must give the same result);
+    sparse background so searches run long."""
+    rng = np.random.default_rng(19)
+    for shape in ((2, 3, 16500), (2, 16500, 3), (3, 2, 20000)):
+        occ = (rng.random(shape) < 0.999).astype(np.uint8)
+        occ[0, 0, 0] = 0
+        _check(vc, oracle, occ, "long %s" % (shape,))

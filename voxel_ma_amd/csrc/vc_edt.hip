@@ -202,7 +202,14 @@ __global__ void __launch_bounds__(kBlock)
 
 // columns along y: item = TZ consecutive z at fixed x, all y.  z' comes from
 // the first half of the edt buffer, (y', z') goes to the ft buffer.
-template <bool kB>
+//
+// kM (every volume has ny, nz <= 16384): the staged element is the signed
+// offset z - z' (int16), so a candidate's cost is one 24-bit multiply-add,
+// (z - z')^2 + dy^2; a row without background is stored as -32768, whose
+// cost 2^30 + dy^2 is at least kNoBg, above every real cost (< 2^29).
+constexpr int kNoBg = 1 << 30;
+
+template <bool kB, bool kM>
 __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 {
     extern __shared__ int16_t colz[]; // [ny][SY], SY >= TZ
@@ -246,8 +253,11 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                 for (int u = 0; u < U; ++u)
                 {
                     const int yy = y + u * tys;
+                    int16_t e = v[u];
+                    if (kM)
+                        e = v[u] >= 0 ? (int16_t)(z - v[u]) : (int16_t)-32768;
                     if (yy < ny)
-                        colz[yy * SY + tz] = v[u];
+                        colz[yy * SY + tz] = e;
                 }
             }
         }
@@ -267,6 +277,44 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
             const int z = z0 + tz;
             if (y >= ny || z >= nz)
                 continue;
+            if (kM)
+            {
+                int best = kNoBg;
+                int by = -1;
+                {
+                    const int e = colz[y * SY + tz];
+                    const int c = __mul24(e, e);
+                    if (c < kNoBg)
+                    {
+                        best = c;
+                        by = y;
+                    }
+                }
+                const int ymax = max(y, ny - 1 - y);
+                for (int dd = 1; dd <= ymax; ++dd)
+                {
+                    const int d2 = dd * dd;
+                    if (d2 > best)
+                        break;
+                    const int yl = y - dd, yh = y + dd;
+                    const int el = colz[max(yl, 0) * SY + tz];
+                    const int eh = colz[min(yh, ny - 1) * SY + tz];
+                    const int vl = yl >= 0 ? __mul24(el, el) + d2 : kNoBg;
+                    const int vh = yh < ny ? __mul24(eh, eh) + d2 : kNoBg;
+                    const bool tl = vl <= best && vl < kNoBg;
+                    best = tl ? vl : best;
+                    by = tl ? yl : by;
+                    const bool th = vh < best;
+                    best = th ? vh : best;
+                    by = th ? yh : by;
+                }
+                uint32_t outv = VC_NONE;
+                if (by >= 0)
+                    outv = ((uint32_t)by << 16) |
+                           (uint32_t)(uint16_t)(z - colz[by * SY + tz]);
+                d.ft[base + (int64_t)y * nz + z] = outv;
+            }
+            else
             {
                 int best = kInf;
                 int by = -1;
@@ -314,6 +362,24 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
 // columns along x: item = TZ consecutive z at fixed y, all x.  Rewrites ft in
 // place (the workgroup stages exactly the elements it later overwrites) and
 // overwrites edt once every z' of the volume has been consumed by pass y.
+//
+// The staged element is the offset pair (y - y', z - z') as two int16 (axes
+// are at most VC_MAX_AXIS = 32767 long), so a candidate's cost dx^2 + dy^2 +
+// dz^2 is one saturating dot product, sdot2(p, p, dx^2).  A column without
+// background is stored as (-32768, -32768): its cost 2^31 + dx^2 saturates
+// to kInf.  Real costs are squared grid distances, below the squared
+// diagonal (<= INT32_MAX, checked at volume creation; INT32_MAX itself is
+// 7 mod 8, not a sum of three squares), so they never reach kInf.
+static_assert(VC_MAX_AXIS <= 32767, "offset pairs must fit int16");
+typedef short vc_s16x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kNoFeat = 0x80008000u;
+
+__device__ __forceinline__ int dot_cost(uint32_t p, int d2)
+{
+    const vc_s16x2 a = __builtin_bit_cast(vc_s16x2, p);
+    return __builtin_amdgcn_sdot2(a, a, d2, true);
+}
+
 template <bool kB>
 __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
 {
@@ -354,8 +420,13 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
             for (int u = 0; u < U; ++u)
             {
                 const int xx = x + u * txs;
+                const uint32_t e =
+                    v[u] == VC_NONE
+                        ? kNoFeat
+                        : ((uint32_t)(uint16_t)(y - (int)(v[u] >> 16)) << 16) |
+                              (uint32_t)(uint16_t)(z - (int)(v[u] & 0xFFFFu));
                 if (xx < nx)
-                    colyz[xx * TZ + tz] = v[u];
+                    colyz[xx * TZ + tz] = e;
             }
         }
         __syncthreads();
@@ -364,14 +435,12 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
             {
                 int best = kInf;
                 int bxv = -1;
-                auto cost = [&](int xx) -> int {
-                    const uint32_t v = colyz[xx * TZ + tz];
-                    const int yy = (int)(v >> 16), zz = (int)(v & 0xFFFFu);
-                    const int c = (y - yy) * (y - yy) + (z - zz) * (z - zz);
-                    return v == VC_NONE ? kInf : c;
+                // cost of candidate xx plus d2 (kInf without background)
+                auto cost = [&](int xx, int d2) -> int {
+                    return dot_cost(colyz[xx * TZ + tz], d2);
                 };
                 {
-                    const int c = cost(x);
+                    const int c = cost(x, 0);
                     if (c < kInf)
                     {
                         best = c;
@@ -387,9 +456,9 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                     if (d2 > best)
                         break;
                     const int xl = x - dd, xh = x + dd;
-                    const int cl = cost(max(xl, 0)), ch = cost(min(xh, nx - 1));
-                    const int vl = (xl >= 0 && cl < kInf) ? cl + d2 : kInf;
-                    const int vh = (xh < nx && ch < kInf) ? ch + d2 : kInf;
+                    const int cl = cost(max(xl, 0), d2), ch = cost(min(xh, nx - 1), d2);
+                    const int vl = (xl >= 0) ? cl : kInf;
+                    const int vh = (xh < nx) ? ch : kInf;
                     const bool tl = vl <= best && vl < kInf;
                     best = tl ? vl : best;
                     bxv = tl ? xl : bxv;
@@ -406,8 +475,9 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                 else
                 {
                     const uint32_t v = colyz[bxv * TZ + tz];
-                    ft[o] = vc_pack_xyz(d.pk, bxv, (int)(v >> 16),
-                                        (int)(v & 0xFFFFu));
+                    const int yy = y - (int)(int16_t)(v >> 16);
+                    const int zz = z - (int)(int16_t)(v & 0xFFFFu);
+                    ft[o] = vc_pack_xyz(d.pk, bxv, yy, zz);
                     d.edt[o] = (int32_t)best;
                 }
             }
@@ -565,12 +635,19 @@ int vc_edt_sched(vc_ctx* ctx, vc_sched* s)
     g = vc_sched_grid(ctx, s, VK_EY, pc);
     if (g)
     {
-        if (s->n > 1)
-            edt_pass_y<true><<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
-                vc_sched_work(s, VK_EY));
+        bool m24 = true; // signed offsets and their squares fit
+        for (const vc_vol* v : s->vols)
+            m24 = m24 && v->ny <= 16384 && v->nz <= 16384;
+        const size_t ly = s->lds[VK_EY];
+        const vc_work wy = vc_sched_work(s, VK_EY);
+        if (s->n > 1 && m24)
+            edt_pass_y<true, true><<<g, kBlock, ly, ctx->stream>>>(wy);
+        else if (s->n > 1)
+            edt_pass_y<true, false><<<g, kBlock, ly, ctx->stream>>>(wy);
+        else if (m24)
+            edt_pass_y<false, true><<<g, kBlock, ly, ctx->stream>>>(wy);
         else
-            edt_pass_y<false><<<g, kBlock, s->lds[VK_EY], ctx->stream>>>(
-                vc_sched_work(s, VK_EY));
+            edt_pass_y<false, false><<<g, kBlock, ly, ctx->stream>>>(wy);
         VC_CHECK_LAUNCH(ctx);
     }
     g = vc_sched_grid(ctx, s, VK_EX, pc);

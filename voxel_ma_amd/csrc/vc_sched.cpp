@@ -185,7 +185,10 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
     const size_t budget = (size_t)sc.edt_lds_kb * 1024;
     for (auto& d : s->h_desc)
     {
-        d.tzy = pick_tile(d.ny, 2, budget, d.nz);
+        // the y pass stages int16 but is sized like the x pass (4 bytes per
+        // element): at 512^3 an 8-wide y tile (8 KB of a 16 KB budget, more
+        // resident workgroups) runs 10% faster than a 16-wide one
+        d.tzy = pick_tile(d.ny, 4, budget, d.nz);
         d.tzx = pick_tile(d.nx, 4, budget, d.nz);
     }
     std::memset(s->lds, 0, sizeof(s->lds));
