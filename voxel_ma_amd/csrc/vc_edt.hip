@@ -290,12 +290,9 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                         by = y;
                     }
                 }
-                const int ymax = max(y, ny - 1 - y);
-                for (int dd = 1; dd <= ymax; ++dd)
-                {
+                // distance dd: lower index first (ties keep the lowest y)
+                auto step = [&](int dd) {
                     const int d2 = dd * dd;
-                    if (d2 > best)
-                        break;
                     const int yl = y - dd, yh = y + dd;
                     const int el = colz[max(yl, 0) * SY + tz];
                     const int eh = colz[min(yh, ny - 1) * SY + tz];
@@ -307,6 +304,18 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                     const bool th = vh < best;
                     best = th ? vh : best;
                     by = th ? yh : by;
+                };
+                // two distances per trip: every candidate at distance dd
+                // costs >= dd^2, so one evaluated past the stopping distance
+                // (or past the column ends, where both sides are out of
+                // range) never wins
+                const int ymax = max(y, ny - 1 - y);
+                for (int dd = 1; dd <= ymax; dd += 2)
+                {
+                    if (dd * dd > best)
+                        break;
+                    step(dd);
+                    step(dd + 1);
                 }
                 uint32_t outv = VC_NONE;
                 if (by >= 0)
@@ -449,12 +458,8 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                 }
                 // branch-free search body (clamped loads, selects): lower
                 // index first, so ties keep the lowest x
-                const int xmax = max(x, nx - 1 - x);
-                for (int dd = 1; dd <= xmax; ++dd)
-                {
+                auto step = [&](int dd) {
                     const int d2 = dd * dd;
-                    if (d2 > best)
-                        break;
                     const int xl = x - dd, xh = x + dd;
                     const int cl = cost(max(xl, 0), d2), ch = cost(min(xh, nx - 1), d2);
                     const int vl = (xl >= 0) ? cl : kInf;
@@ -465,6 +470,15 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
                     const bool th = vh < best;
                     best = th ? vh : best;
                     bxv = th ? xh : bxv;
+                };
+                // two distances per trip, as in the y pass
+                const int xmax = max(x, nx - 1 - x);
+                for (int dd = 1; dd <= xmax; dd += 2)
+                {
+                    if (dd * dd > best)
+                        break;
+                    step(dd);
+                    step(dd + 1);
                 }
                 const int64_t o = (int64_t)x * sx + off;
                 if (bxv < 0)
