@@ -505,6 +505,10 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
 // value is read about once.  A voxel q is background iff FT(q) == q, so no
 // occupancy is read.
 constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
+#ifndef VC_MEAS_AHEAD
+#define VC_MEAS_AHEAD 2
+#endif
+constexpr int kMeasAhead = VC_MEAS_AHEAD; // >= 2
 
 template <bool kB>
 __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
@@ -553,20 +557,40 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
         const int64_t row = (int64_t)y * nz + z;
         const int gy = y0 + hy, gz = z0 + hz;
         const bool hin = hact && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+        // loads are unconditional from clamped, always in-bounds addresses,
+        // and the ring keeps the raw values: out-of-grid neighbours are
+        // replaced by kOob only where a slice is used.  A load under a
+        // branch, or a select right after it, makes the compiler wait for
+        // every outstanding load (the prefetched slices too).
+        const int64_t rowc = valid ? row : 0;
+        const int64_t hrow = hin ? (int64_t)gy * nz + gz : rowc;
+        // slices past xb (the chunk end, whose next neighbour is read) are
+        // clamped to it: prefetches beyond the chunk fetch nothing new
+        const int xlast = min(xb, nx - 1);
         auto load = [&](int xx) -> uint32_t {
-            return (valid && xx >= 0 && xx < nx) ? ft[xx * sx + row] : kOob;
+            return ft[min(max(xx, 0), xlast) * sx + rowc];
         };
         auto load_halo = [&](int xx) -> uint32_t {
-            return (hin && xx < xb) ? ft[xx * sx + (int64_t)gy * nz + gz] : kOob;
+            return ft[min(xx, xb - 1) * sx + hrow];
         };
-        // two slices ahead in flight: the loads for x + 2 (and the halo of
-        // x + 1) are issued before the work on slice x
-        uint32_t prev = load(xa - 1), cur = load(xa), next = load(xa + 1);
-        uint32_t halo = load_halo(xa);
+        // kMeasAhead slices in flight: the loads of slice x + kMeasAhead (and
+        // the halo of x + kMeasAhead - 1) are issued before the work on slice
+        // x.  fr[i] holds slice x - 1 + i, hr[i] the halo of slice x + i.
+        uint32_t fr[kMeasAhead + 1], hr[kMeasAhead - 1];
+#pragma unroll
+        for (int i = 0; i <= kMeasAhead; ++i)
+            fr[i] = load(xa - 1 + i);
+#pragma unroll
+        for (int i = 0; i < kMeasAhead - 1; ++i)
+            hr[i] = load_halo(xa + i);
         for (int x = xa; x < xb; ++x)
         {
-            const uint32_t next2 = load(x + 2);
-            const uint32_t halo_n = load_halo(x + 1);
+            const uint32_t fnew = load(x + kMeasAhead);
+            const uint32_t hnew = load_halo(x + kMeasAhead - 1);
+            const uint32_t prev = (valid && x > 0) ? fr[0] : kOob;
+            const uint32_t cur = fr[1];
+            const uint32_t next = (valid && x + 1 < nx) ? fr[2] : kOob;
+            const uint32_t halo = hin ? hr[0] : kOob;
             S[ty + 1][tz + 1] = valid ? cur : kOob;
             if (hact)
                 S[hy + 1][hz + 1] = halo;
@@ -600,10 +624,14 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
                 d.lam2[x * sx + row] = m;
             }
             __syncthreads();
-            prev = cur;
-            cur = next;
-            next = next2;
-            halo = halo_n;
+#pragma unroll
+            for (int i = 0; i < kMeasAhead; ++i)
+                fr[i] = fr[i + 1];
+            fr[kMeasAhead] = fnew;
+#pragma unroll
+            for (int i = 0; i + 1 < kMeasAhead - 1; ++i)
+                hr[i] = hr[i + 1];
+            hr[kMeasAhead - 2] = hnew;
         }
     }
 }
