@@ -499,41 +499,27 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
     }
 }
 
-// Per-voxel measure with an x-sliding window: a 64(z) x 8(y) workgroup walks
-// a chunk of kMeasXC slices; FT(x-1), FT(x), FT(x+1) stay in registers and
-// the y/z neighbours of slice x come from a 10 x 66 LDS window, so each FT
-// value is read about once.  A voxel q is background iff FT(q) == q, so no
-// occupancy is read.
-constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
-#ifndef VC_MEAS_AHEAD
-#define VC_MEAS_AHEAD 2
-#endif
-constexpr int kMeasAhead = VC_MEAS_AHEAD; // >= 2
+// Per-voxel measure, one wavefront per strip of kMeasZ = 62 z x kMeasY y x
+// kMeasXC x: lanes hold z0 - 1 .. z0 + 62 (lanes 0 and 63 are the z halo),
+// registers hold slices x - 1, x, x + 1 (and x + 2 in flight) of the
+// kMeasY + 2 rows y0 - 1 .. y0 + kMeasY, and z neighbours come from the
+// adjacent lanes: no LDS and no barrier.  Loads are unconditional from clamped, in-bounds addresses and
+// the registers keep the raw values; out-of-grid neighbours become kOob
+// where they are used (a load under a branch, or a select right after it,
+// makes the compiler wait for every outstanding load).  A voxel q is
+// background iff FT(q) == q, so no occupancy is read.
+constexpr int kMeasR = kMeasY + 2; // rows held: the strip's and y -+ 1
+constexpr int kMeasWaves = 4; // independent strips per workgroup
 
 template <bool kB>
-__global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
+__global__ void __launch_bounds__(64 * kMeasWaves) measure_lam2(vc_work W)
 {
-    __shared__ uint32_t S[kMeasY + 2][kMeasZ + 2];
-    const int tz = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * kMeasZ + tz;
-    // halo slots: rows y0-1 and y0+8 (66 each), then columns z0-1 and z0+64
-    int hy = 0, hz = 0;
-    bool hact = false;
-    if (tid < 2 * (kMeasZ + 2))
-    {
-        hy = tid < kMeasZ + 2 ? -1 : kMeasY;
-        hz = (tid % (kMeasZ + 2)) - 1;
-        hact = true;
-    }
-    else if (tid < 2 * (kMeasZ + 2) + 2 * kMeasY)
-    {
-        const int u = tid - 2 * (kMeasZ + 2);
-        hy = u >> 1;
-        hz = (u & 1) ? kMeasZ : -1;
-        hact = true;
-    }
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const uint32_t per = (W.total + 7) / 8;
-    for (uint32_t L = blockIdx.x; L < 8 * per; L += gridDim.x)
+    // slot of wave w of workgroup g: g + w * gridDim.x (+ rounds), which
+    // keeps g's XCD item group (gridDim.x is a multiple of 8)
+    for (uint32_t L = blockIdx.x + gridDim.x * (uint32_t)wave; L < 8 * per;
+         L += kMeasWaves * gridDim.x)
     {
         const uint32_t b = vc_xcd_item(L, per);
         if (b >= W.total)
@@ -551,87 +537,81 @@ __global__ void __launch_bounds__(kMeasZ * kMeasY) measure_lam2(vc_work W)
         const int y0 = (int)((T / nzt) % nyt) * kMeasY;
         const int xa = (int)(T / ((uint32_t)nzt * nyt)) * kMeasXC;
         const int xb = min(nx, xa + kMeasXC);
-        const int z = z0 + tz, y = y0 + ty;
-        const bool valid = z < nz && y < ny;
-        const int64_t sx = (int64_t)ny * nz;
-        const int64_t row = (int64_t)y * nz + z;
-        const int gy = y0 + hy, gz = z0 + hz;
-        const bool hin = hact && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
-        // loads are unconditional from clamped, always in-bounds addresses,
-        // and the ring keeps the raw values: out-of-grid neighbours are
-        // replaced by kOob only where a slice is used.  A load under a
-        // branch, or a select right after it, makes the compiler wait for
-        // every outstanding load (the prefetched slices too).
-        const int64_t rowc = valid ? row : 0;
-        const int64_t hrow = hin ? (int64_t)gy * nz + gz : rowc;
-        // slices past xb (the chunk end, whose next neighbour is read) are
-        // clamped to it: prefetches beyond the chunk fetch nothing new
+        const int z = z0 - 1 + lane;
+        const bool zok = lane >= 1 && lane <= kMeasZ && z < nz; // computed lanes
+        // element indices fit 32 bits (N < 2^31)
+        const uint32_t sx = (uint32_t)ny * (uint32_t)nz;
+        const int zc = min(max(z, 0), nz - 1);
+        // row i is y0 - 1 + i; slices past xb (whose next neighbour is read)
+        // clamp to it, so prefetches beyond the chunk fetch nothing new
+        uint32_t roff[kMeasR];
+#pragma unroll
+        for (int i = 0; i < kMeasR; ++i)
+            roff[i] = (uint32_t)min(max(y0 - 1 + i, 0), ny - 1) * (uint32_t)nz + (uint32_t)zc;
         const int xlast = min(xb, nx - 1);
-        auto load = [&](int xx) -> uint32_t {
-            return ft[min(max(xx, 0), xlast) * sx + rowc];
+        auto ld = [&](int xx, int i) -> uint32_t {
+            return ft[(uint32_t)min(max(xx, 0), xlast) * sx + roff[i]];
         };
-        auto load_halo = [&](int xx) -> uint32_t {
-            return ft[min(xx, xb - 1) * sx + hrow];
-        };
-        // kMeasAhead slices in flight: the loads of slice x + kMeasAhead (and
-        // the halo of x + kMeasAhead - 1) are issued before the work on slice
-        // x.  fr[i] holds slice x - 1 + i, hr[i] the halo of slice x + i.
-        uint32_t fr[kMeasAhead + 1], hr[kMeasAhead - 1];
+        uint32_t P[kMeasR], C[kMeasR], Q[kMeasR]; // slices x - 1, x, x + 1
 #pragma unroll
-        for (int i = 0; i <= kMeasAhead; ++i)
-            fr[i] = load(xa - 1 + i);
-#pragma unroll
-        for (int i = 0; i < kMeasAhead - 1; ++i)
-            hr[i] = load_halo(xa + i);
+        for (int i = 0; i < kMeasR; ++i)
+        {
+            P[i] = ld(xa - 1, i);
+            C[i] = ld(xa, i);
+            Q[i] = ld(xa + 1, i);
+        }
+        const uint32_t sxp = 1u << (pk.by + pk.bz), syp = 1u << pk.bz;
         for (int x = xa; x < xb; ++x)
         {
-            const uint32_t fnew = load(x + kMeasAhead);
-            const uint32_t hnew = load_halo(x + kMeasAhead - 1);
-            const uint32_t prev = (valid && x > 0) ? fr[0] : kOob;
-            const uint32_t cur = fr[1];
-            const uint32_t next = (valid && x + 1 < nx) ? fr[2] : kOob;
-            const uint32_t halo = hin ? hr[0] : kOob;
-            S[ty + 1][tz + 1] = valid ? cur : kOob;
-            if (hact)
-                S[hy + 1][hz + 1] = halo;
-            __syncthreads();
-            if (valid)
+            uint32_t F[kMeasR];
+#pragma unroll
+            for (int i = 0; i < kMeasR; ++i)
+                F[i] = ld(x + 2, i);
+#pragma unroll
+            for (int r = 1; r <= kMeasY; ++r)
             {
+                const int y = y0 - 1 + r;
+                const uint32_t cur = C[r];
+                // z neighbours from the adjacent lanes (all lanes shuffle)
+                const uint32_t zm = __shfl_up(cur, 1), zp = __shfl_down(cur, 1);
+                if (!zok || y >= ny)
+                    continue;
                 uint32_t m = 0;
                 const uint32_t me = vc_pack_xyz(pk, x, y, z);
                 if (cur != VC_NONE && cur != me)
                 {
                     int fx, fy, fz;
                     vc_unpack_xyz(pk, cur, fx, fy, fz);
+                    // branch-free: the distance of a skipped neighbour
+                    // (outside, no feature, background) is discarded
                     auto visit = [&](uint32_t g, uint32_t q) {
-                        if (g == kOob || g == VC_NONE || g == q)
-                            return;
                         int gx, gy2, gz2;
                         vc_unpack_xyz(pk, g, gx, gy2, gz2);
                         const uint32_t dd =
                             (uint32_t)((fx - gx) * (fx - gx)) +
                             (uint32_t)((fy - gy2) * (fy - gy2)) +
                             (uint32_t)((fz - gz2) * (fz - gz2));
-                        m = dd > m ? dd : m;
+                        const bool ok = g != kOob && g != VC_NONE && g != q;
+                        m = (ok && dd > m) ? dd : m;
                     };
-                    visit(prev, vc_pack_xyz(pk, x - 1, y, z));
-                    visit(next, vc_pack_xyz(pk, x + 1, y, z));
-                    visit(S[ty][tz + 1], vc_pack_xyz(pk, x, y - 1, z));
-                    visit(S[ty + 2][tz + 1], vc_pack_xyz(pk, x, y + 1, z));
-                    visit(S[ty + 1][tz], vc_pack_xyz(pk, x, y, z - 1));
-                    visit(S[ty + 1][tz + 2], vc_pack_xyz(pk, x, y, z + 1));
+                    // neighbour indices: me -+ one step of a field (an
+                    // out-of-grid neighbour is kOob and never compared)
+                    visit(x > 0 ? P[r] : kOob, me - sxp);
+                    visit(x + 1 < nx ? Q[r] : kOob, me + sxp);
+                    visit(y > 0 ? C[r - 1] : kOob, me - syp);
+                    visit(y + 1 < ny ? C[r + 1] : kOob, me + syp);
+                    visit(z > 0 ? zm : kOob, me - 1u);
+                    visit(z + 1 < nz ? zp : kOob, me + 1u);
                 }
-                d.lam2[x * sx + row] = m;
+                d.lam2[(uint32_t)x * sx + roff[r]] = m;
             }
-            __syncthreads();
 #pragma unroll
-            for (int i = 0; i < kMeasAhead; ++i)
-                fr[i] = fr[i + 1];
-            fr[kMeasAhead] = fnew;
-#pragma unroll
-            for (int i = 0; i + 1 < kMeasAhead - 1; ++i)
-                hr[i] = hr[i + 1];
-            hr[kMeasAhead - 2] = hnew;
+            for (int i = 0; i < kMeasR; ++i)
+            {
+                P[i] = C[i];
+                C[i] = Q[i];
+                Q[i] = F[i];
+            }
         }
     }
 }
@@ -711,14 +691,17 @@ int vc_edt_sched(vc_ctx* ctx, vc_sched* s)
 int vc_measure_sched(vc_ctx* ctx, vc_sched* s)
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
-    const unsigned g = vc_sched_grid(ctx, s, VK_MS, s->sc.edt_grid_per_cu);
-    if (g)
+    // the scheduler counts one slot per strip; a workgroup runs kMeasWaves
+    // strips (grid a multiple of 8 for the XCD mapping)
+    const unsigned gs = vc_sched_grid(ctx, s, VK_MS, s->sc.edt_grid_per_cu);
+    const unsigned g = 8 * ((gs + 8 * kMeasWaves - 1) / (8 * kMeasWaves));
+    if (gs)
     {
         if (s->n > 1)
-            measure_lam2<true><<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+            measure_lam2<true><<<g, 64 * kMeasWaves, 0, ctx->stream>>>(
                 vc_sched_work(s, VK_MS));
         else
-            measure_lam2<false><<<g, dim3(kMeasZ, kMeasY), 0, ctx->stream>>>(
+            measure_lam2<false><<<g, 64 * kMeasWaves, 0, ctx->stream>>>(
                 vc_sched_work(s, VK_MS));
         VC_CHECK_LAUNCH(ctx);
     }

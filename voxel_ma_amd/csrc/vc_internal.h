@@ -49,6 +49,13 @@ inline __host__ __device__ int vc_edt_ystride(int tz)
     return tz + (tz >= 32 ? tz / 8 : 0);
 }
 
+// measure strip (z, y, x), one wavefront each: 62 computed z (lanes 1..62
+// of a wavefront; lanes 0 and 63 hold the z halo), VC_MEAS_Y rows, 32 slices
+#ifndef VC_MEAS_Y
+#define VC_MEAS_Y 4
+#endif
+constexpr int kMeasZ = 62, kMeasY = VC_MEAS_Y, kMeasXC = 32;
+
 struct vc_sched;
 
 struct vc_vol
@@ -138,7 +145,7 @@ enum vc_kind
     VK_EZ = 0, // EDT pass z: 4 rows per item
     VK_EY,     // EDT pass y: one (x, z-tile) column tile per item
     VK_EX,     // EDT pass x: one (y, z-tile) column tile per item
-    VK_MS,     // measure: one 64 x 8 x 32 (z, y, x) tile per item
+    VK_MS,     // measure: one kMeasZ x kMeasY x kMeasXC wavefront strip per item
     VK_LI,     // level set-up: 4 wavefronts (512 z each) per item
     VK_TP,     // subfield pass, (x, y) parity 0..3: 256 words per item
     VK_TT = VK_TP + 4, // tile sweep: one 32 x 32 x 64 (x, y, z) tile per item

@@ -16,7 +16,6 @@ namespace
 constexpr int kRowsZ = 4;       // EDT pass z: rows (wavefronts) per item
 constexpr int kLiWaves = 4;     // level set-up: wavefronts per item
 constexpr int kTpWords = 256;   // subfield pass: words per item
-constexpr int kMeasZ = 64, kMeasY = 8, kMeasXC = 32;
 constexpr int kTileXY = 32;     // tile sweep: rows per tile along x and y
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
