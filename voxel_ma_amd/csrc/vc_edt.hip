@@ -552,63 +552,95 @@ __global__ void __launch_bounds__(64 * kMeasWaves) measure_lam2(vc_work W)
         auto ld = [&](int xx, int i) -> uint32_t {
             return ft[(uint32_t)min(max(xx, 0), xlast) * sx + roff[i]];
         };
-        uint32_t P[kMeasR], C[kMeasR], Q[kMeasR]; // slices x - 1, x, x + 1
+        // a voxel's feature, unpacked, and whether the voxel is inside (in
+        // the grid, with a feature, not background: FT(v) != v).  A pair of
+        // 6-neighbours counts for both voxels' measure iff both are inside,
+        // so each pair is priced once, at its lower voxel (+x, +y, +z), and
+        // reaches the upper voxel from the previous slice, the row below or
+        // the lane below.
+        struct Ft
+        {
+            int x, y, z;
+            bool in;
+        };
+        auto feat = [&](uint32_t g, uint32_t self, bool grid) -> Ft {
+            Ft f;
+            vc_unpack_xyz(pk, g, f.x, f.y, f.z);
+            f.in = grid && g != VC_NONE && g != self;
+            return f;
+        };
+        auto dist = [&](const Ft& a, const Ft& b) -> uint32_t {
+            const uint32_t dd = (uint32_t)((a.x - b.x) * (a.x - b.x)) +
+                                (uint32_t)((a.y - b.y) * (a.y - b.y)) +
+                                (uint32_t)((a.z - b.z) * (a.z - b.z));
+            return (a.in && b.in) ? dd : 0u;
+        };
+        const bool zin = z >= 0 && z < nz;
+        bool rin[kMeasR];
 #pragma unroll
         for (int i = 0; i < kMeasR; ++i)
-        {
-            P[i] = ld(xa - 1, i);
-            C[i] = ld(xa, i);
-            Q[i] = ld(xa + 1, i);
-        }
+            rin[i] = (unsigned)(y0 - 1 + i) < (unsigned)ny;
         const uint32_t sxp = 1u << (pk.by + pk.bz), syp = 1u << pk.bz;
+        // packed index of (x, y0 - 1 + r, z) = pack(x, y0, zc) + (r - 1) syp
+        // (only compared where the voxel is in the grid)
+        const uint32_t me0 = vc_pack_xyz(pk, xa, y0, zc);
+        uint32_t C[kMeasR], Q[kMeasR]; // raw slices x, x + 1
+        uint32_t pxb[kMeasR];          // pair (x - 1, x) of each strip row
+        {
+            uint32_t Pv[kMeasR];
+#pragma unroll
+            for (int i = 0; i < kMeasR; ++i)
+            {
+                Pv[i] = ld(xa - 1, i);
+                C[i] = ld(xa, i);
+                Q[i] = ld(xa + 1, i);
+            }
+#pragma unroll
+            for (int r = 1; r <= kMeasY; ++r)
+            {
+                const uint32_t self = me0 + (uint32_t)(r - 1) * syp;
+                pxb[r] = dist(feat(Pv[r], self - sxp, xa > 0 && rin[r] && zin),
+                              feat(C[r], self, rin[r] && zin));
+            }
+        }
         for (int x = xa; x < xb; ++x)
         {
             uint32_t F[kMeasR];
 #pragma unroll
             for (int i = 0; i < kMeasR; ++i)
                 F[i] = ld(x + 2, i);
+            const uint32_t meX = me0 + (uint32_t)(x - xa) * sxp;
+            Ft c[kMeasR];
+#pragma unroll
+            for (int r = 0; r < kMeasR; ++r)
+                c[r] = feat(C[r], meX + (uint32_t)(r - 1) * syp, rin[r] && zin);
+            uint32_t py[kMeasR - 1]; // pair (row r, row r + 1)
+#pragma unroll
+            for (int r = 0; r + 1 < kMeasR; ++r)
+                py[r] = dist(c[r], c[r + 1]);
+            const bool xnext = x + 1 < nx;
 #pragma unroll
             for (int r = 1; r <= kMeasY; ++r)
             {
                 const int y = y0 - 1 + r;
-                const uint32_t cur = C[r];
-                // z neighbours from the adjacent lanes (all lanes shuffle)
-                const uint32_t zm = __shfl_up(cur, 1), zp = __shfl_down(cur, 1);
-                if (!zok || y >= ny)
-                    continue;
-                uint32_t m = 0;
-                const uint32_t me = vc_pack_xyz(pk, x, y, z);
-                if (cur != VC_NONE && cur != me)
-                {
-                    int fx, fy, fz;
-                    vc_unpack_xyz(pk, cur, fx, fy, fz);
-                    // branch-free: the distance of a skipped neighbour
-                    // (outside, no feature, background) is discarded
-                    auto visit = [&](uint32_t g, uint32_t q) {
-                        int gx, gy2, gz2;
-                        vc_unpack_xyz(pk, g, gx, gy2, gz2);
-                        const uint32_t dd =
-                            (uint32_t)((fx - gx) * (fx - gx)) +
-                            (uint32_t)((fy - gy2) * (fy - gy2)) +
-                            (uint32_t)((fz - gz2) * (fz - gz2));
-                        const bool ok = g != kOob && g != VC_NONE && g != q;
-                        m = (ok && dd > m) ? dd : m;
-                    };
-                    // neighbour indices: me -+ one step of a field (an
-                    // out-of-grid neighbour is kOob and never compared)
-                    visit(x > 0 ? P[r] : kOob, me - sxp);
-                    visit(x + 1 < nx ? Q[r] : kOob, me + sxp);
-                    visit(y > 0 ? C[r - 1] : kOob, me - syp);
-                    visit(y + 1 < ny ? C[r + 1] : kOob, me + syp);
-                    visit(z > 0 ? zm : kOob, me - 1u);
-                    visit(z + 1 < nz ? zp : kOob, me + 1u);
-                }
-                d.lam2[(uint32_t)x * sx + roff[r]] = m;
+                // pair (z, z + 1) with the lane above, (z - 1, z) from the
+                // lane below (all lanes shuffle)
+                Ft zn;
+                vc_unpack_xyz(pk, __shfl_down(C[r], 1), zn.x, zn.y, zn.z);
+                zn.in = __shfl_down((int)c[r].in, 1) != 0;
+                const uint32_t pz = dist(c[r], zn);
+                const uint32_t pzb = __shfl_up(pz, 1);
+                const uint32_t self = meX + (uint32_t)(r - 1) * syp;
+                const uint32_t px = dist(c[r], feat(Q[r], self + sxp, xnext && rin[r] && zin));
+                const uint32_t m =
+                    max(max(px, pxb[r]), max(max(py[r - 1], py[r]), max(pz, pzb)));
+                pxb[r] = px;
+                if (zok && y < ny)
+                    d.lam2[(uint32_t)x * sx + roff[r]] = m;
             }
 #pragma unroll
             for (int i = 0; i < kMeasR; ++i)
             {
-                P[i] = C[i];
                 C[i] = Q[i];
                 Q[i] = F[i];
             }

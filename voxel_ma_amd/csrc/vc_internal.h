@@ -52,7 +52,7 @@ inline __host__ __device__ int vc_edt_ystride(int tz)
 // measure strip (z, y, x), one wavefront each: 62 computed z (lanes 1..62
 // of a wavefront; lanes 0 and 63 hold the z halo), VC_MEAS_Y rows, 32 slices
 #ifndef VC_MEAS_Y
-#define VC_MEAS_Y 4
+#define VC_MEAS_Y 2
 #endif
 constexpr int kMeasZ = 62, kMeasY = VC_MEAS_Y, kMeasXC = 32;
 
