@@ -33,7 +33,8 @@ typedef struct vc_schedule {
                             workgroup per work item                       */
     int thin_grid_per_cu;/* subfield passes: the same                     */
     int sweeps_per_sync; /* sweeps launched between two convergence
-                            checks: 0 = adaptive (4, 8, 12, then 16), or
+                            checks: 0 = adaptive (4, 8, 12, then 16),
+                            -1 = adaptive doubling (4, 8, then 16), or
                             1-32                                          */
     int use_graph;       /* 1: each segment of sweeps is a captured
                             hipGraph replayed on the context stream       */

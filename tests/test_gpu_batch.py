@@ -97,6 +97,10 @@ def test_batch_matches_oracle_per_volume(vc, ragged):
          tail_sweep=-1, tile_sweep=4),
     dict(edt_lds_kb=32, edt_grid_per_cu=1, thin_grid_per_cu=1, sweeps_per_sync=2, use_graph=1,
          tail_sweep=-1, tile_sweep=0),
+    dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=-1, use_graph=1,
+         tail_sweep=-1, tile_sweep=0),
+    dict(edt_lds_kb=16, edt_grid_per_cu=0, thin_grid_per_cu=0, sweeps_per_sync=-1, use_graph=0,
+         tail_sweep=2, tile_sweep=-1),
 ])
 def test_batch_every_schedule_same_result(vc, ragged, sched):
     vols, levels, ref = ragged

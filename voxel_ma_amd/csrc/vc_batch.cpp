@@ -294,8 +294,9 @@ int vc_batch_tune(vc_ctx* ctx, vc_batch* b, const float* levels, int n_levels,
         }
         t_front = best_t;
         // thinning candidates: (grid per CU, graph, sweeps per sync, first
-        // tile sweep); tile sweeps hold 78 KB of LDS, two workgroups per CU
-        static const int kThin[][4] = {{0, 0, 0, 0},  {0, 1, 0, 0},
+        // tile sweep); tile sweeps hold 37 KB of LDS, four workgroups per CU
+        static const int kThin[][4] = {{0, 0, 0, 0},  {0, 0, -1, 0},
+                                       {0, 1, 0, 0},
                                        {0, 0, 16, 0}, {2, 0, 0, 0},
                                        {0, 0, 0, 4},  {0, 0, 0, -1},
                                        {0, 1, 8, -1}, {8, 0, 0, -1},

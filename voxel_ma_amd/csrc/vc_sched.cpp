@@ -169,9 +169,9 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
         return vc_fail(ctx, VC_ERR_INVALID, "grid_per_cu must be in [0, 64]");
     // the host reads a segment's flags from a ring of VC_RING entries whose
     // entries are cleared VC_RING / 2 sweeps ahead
-    if (sc.sweeps_per_sync < 0 || sc.sweeps_per_sync > VC_RING / 2)
+    if (sc.sweeps_per_sync < -1 || sc.sweeps_per_sync > VC_RING / 2)
         return vc_fail(ctx, VC_ERR_INVALID,
-                       "sweeps_per_sync must be in [0, %d]", VC_RING / 2);
+                       "sweeps_per_sync must be in [-1, %d]", VC_RING / 2);
     if (sc.use_graph != 0 && sc.use_graph != 1)
         return vc_fail(ctx, VC_ERR_INVALID, "use_graph must be 0 or 1");
     if (sc.tail_sweep < -1 || sc.tail_sweep > 1000000)

@@ -1731,7 +1731,7 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
             launches++;
         }
         int64_t base = 0;
-        int K = sc.sweeps_per_sync ? sc.sweeps_per_sync : VC_SEG_MIN;
+        int K = sc.sweeps_per_sync > 0 ? sc.sweeps_per_sync : VC_SEG_MIN;
         bool in_tile = false;
         for (;;)
         {
@@ -1773,13 +1773,16 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
                 passes_total += 8 * (conv + 1);
                 break;
             }
-            // segments grow by VC_SEG_MIN sweeps (4, 8, 12, 16): a level
-            // converges inside its last segment, whose remaining sweeps
-            // launch and find nothing, so arithmetic growth overshoots less
-            // than doubling at the same number of host checks on typical
-            // levels (512^3 headline: 110 launches instead of 122, -1.6 %)
-            if (!sc.sweeps_per_sync)
+            // adaptive segments: by default they grow by VC_SEG_MIN sweeps
+            // (4, 8, 12, 16) — a level converges inside its last segment,
+            // whose remaining sweeps launch and find nothing, so arithmetic
+            // growth overshoots less than doubling (512^3 headline: 110
+            // launches instead of 122, -1.6 %); sweeps_per_sync = -1 doubles
+            // (4, 8, 16: fewer host checks for levels of ~25 sweeps)
+            if (sc.sweeps_per_sync == 0)
                 K = K + VC_SEG_MIN < VC_SEG_MAX ? K + VC_SEG_MIN : VC_SEG_MAX;
+            else if (sc.sweeps_per_sync < 0)
+                K = K * 2 < VC_SEG_MAX ? K * 2 : VC_SEG_MAX;
         }
     }
     if (st)
