@@ -27,7 +27,9 @@ typedef struct vc_batch vc_batch;
 
 /* launch parameters (autotuned by vc_batch_tune; all results identical) */
 typedef struct vc_schedule {
-    int edt_lds_kb;      /* LDS per EDT column tile: 8, 16, 32 or 64 KiB   */
+    int edt_lds_kb;      /* LDS budget of the EDT column tiles: 8, 16, 32
+                            or 64 KiB (the x pass gets twice it, at most
+                            64 KiB)                                       */
     int edt_grid_per_cu; /* EDT + measure launches: persistent grid of
                             this many workgroups per CU; 0 = one
                             workgroup per work item                       */

@@ -188,7 +188,10 @@ int vc_sched_apply(vc_ctx* ctx, vc_sched* s, const vc_schedule& sc)
         // element): at 512^3 an 8-wide y tile (8 KB of a 16 KB budget, more
         // resident workgroups) runs 10% faster than a 16-wide one
         d.tzy = pick_tile(d.ny, 4, budget, d.nz);
-        d.tzx = pick_tile(d.nx, 4, budget, d.nz);
+        // the x pass gets twice the budget, at most 64 KB: 16-wide x tiles
+        // (32 KB at 512^3, 64-byte row segments) measured 3.5 % faster for
+        // the whole EDT than 8-wide ones
+        d.tzx = pick_tile(d.nx, 4, std::min(2 * budget, (size_t)65536), d.nz);
     }
     std::memset(s->lds, 0, sizeof(s->lds));
     s->ez_stride = 0;
