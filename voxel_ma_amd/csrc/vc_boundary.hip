@@ -422,11 +422,13 @@ __device__ __forceinline__ void put_quads(const Emit& e, const Dims& D, int x, i
     }
 }
 
-// pass 1 (vectorised): per-chunk counts (sites << 16 | faces) and per-tile
-// totals; the emit passes read the chunk counts and skip empty chunks
+// pass 1 (vectorised): per-chunk counts (sites << 16 | faces), per-tile
+// totals, and for chunks that emit anything each voxel's decision masks
+// (corner keys in bits 0..23, face offsets in bits 24..29; 0 without an
+// event), so the emit passes only scatter; chunks without output are skipped
 __global__ void __launch_bounds__(kBlock)
     bnd_count_v(const uint8_t* __restrict__ occ, Dims D,
-                uint32_t* __restrict__ chunk_cnt,
+                uint32_t* __restrict__ chunk_cnt, uint32_t* __restrict__ vmask,
                 uint32_t* __restrict__ tile_sites,
                 uint32_t* __restrict__ tile_faces)
 {
@@ -436,18 +438,27 @@ __global__ void __launch_bounds__(kBlock)
     Chunk K;
     if (load_chunk(occ, D, c, K))
     {
+        uint32_t em[kVz];
 #pragma unroll
         for (int j = 0; j < kVz; ++j)
         {
+            em[j] = 0;
             Vox V;
             if (chunk_vox(K, D, j, V))
             {
                 const Emit e = decide(V);
                 ns += (uint32_t)__builtin_popcount(e.keys);
                 nf += (uint32_t)__builtin_popcount(e.faces);
+                em[j] = e.keys | (e.faces << 24);
             }
         }
         chunk_cnt[c] = (ns << 16) | nf;
+        if (ns | nf)
+        {
+            uint4* dst = reinterpret_cast<uint4*>(vmask + (int64_t)c * kVz);
+            dst[0] = make_uint4(em[0], em[1], em[2], em[3]);
+            dst[1] = make_uint4(em[4], em[5], em[6], em[7]);
+        }
     }
     uint32_t tot;
     block_scan_excl((ns << 16) | nf, sh, &tot);
@@ -458,9 +469,20 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// chunk c of the vectorised passes: row (x, y) and first z
+__device__ __forceinline__ void chunk_at(const Dims& D, int64_t c, int& x, int& y,
+                                         int& z0)
+{
+    const int nzc = D.nz / kVz;
+    const int64_t row = c / nzc;
+    z0 = (int)(c - row * nzc) * kVz;
+    x = (int)(row / D.ny);
+    y = (int)(row - (int64_t)x * D.ny);
+}
+
 __global__ void __launch_bounds__(kBlock)
-    bnd_emit_sites_v(const uint8_t* __restrict__ occ, Dims D,
-                     const uint32_t* __restrict__ chunk_cnt,
+    bnd_emit_sites_v(Dims D, const uint32_t* __restrict__ chunk_cnt,
+                     const uint32_t* __restrict__ vmask,
                      const uint32_t* __restrict__ tile_sites,
                      float* __restrict__ sites, uint32_t* __restrict__ cmap)
 {
@@ -470,22 +492,25 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t ns = c < nch ? chunk_cnt[c] >> 16 : 0u;
     uint32_t tot;
     uint32_t s = tile_sites[blockIdx.x] + block_scan_excl(ns, sh, &tot);
-    Chunk K;
-    if (ns && load_chunk(occ, D, c, K))
+    if (ns)
+    {
+        int x, y, z0;
+        chunk_at(D, c, x, y, z0);
+        const uint4* src = reinterpret_cast<const uint4*>(vmask + c * kVz);
+        const uint4 a = src[0], b = src[1];
+        const uint32_t em[kVz] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
         for (int j = 0; j < kVz; ++j)
         {
-            Vox V;
-            if (!chunk_vox(K, D, j, V))
-                continue;
-            const Emit e = decide(V);
-            put_sites(e, D, K.x, K.y, K.z0 + j, s, sites, cmap);
+            const Emit e{em[j] & 0xFFFFFFu, 0u};
+            put_sites(e, D, x, y, z0 + j, s, sites, cmap);
             s += (uint32_t)__builtin_popcount(e.keys);
         }
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
-    bnd_emit_quads_v(const uint8_t* __restrict__ occ, Dims D,
-                     const uint32_t* __restrict__ chunk_cnt,
+    bnd_emit_quads_v(Dims D, const uint32_t* __restrict__ chunk_cnt,
+                     const uint32_t* __restrict__ vmask,
                      const uint32_t* __restrict__ tile_faces,
                      const uint32_t* __restrict__ cmap,
                      int32_t* __restrict__ quads)
@@ -496,17 +521,20 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t nf = c < nch ? chunk_cnt[c] & 0xFFFFu : 0u;
     uint32_t tot;
     uint32_t f = tile_faces[blockIdx.x] + block_scan_excl(nf, sh, &tot);
-    Chunk K;
-    if (nf && load_chunk(occ, D, c, K))
+    if (nf)
+    {
+        int x, y, z0;
+        chunk_at(D, c, x, y, z0);
+        const uint4* src = reinterpret_cast<const uint4*>(vmask + c * kVz);
+        const uint4 a = src[0], b = src[1];
+        const uint32_t em[kVz] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
         for (int j = 0; j < kVz; ++j)
         {
-            Vox V;
-            if (!chunk_vox(K, D, j, V))
-                continue;
-            const Emit e = decide(V);
-            put_quads(e, D, K.x, K.y, K.z0 + j, f, cmap, quads);
+            const Emit e{0u, em[j] >> 24};
+            put_quads(e, D, x, y, z0 + j, f, cmap, quads);
             f += (uint32_t)__builtin_popcount(e.faces);
         }
+    }
 }
 
 // exclusive scan of a uint32 array in place, one block of 1024 per chunk;
@@ -1248,7 +1276,8 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
     int rc = VC_OK;
     if (!v->bnd_done)
     {
-        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr, *ccnt = nullptr;
+        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr, *ccnt = nullptr,
+                 *vmask = nullptr;
         if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)etiles)) != VC_OK)
             return rc;
         if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)etiles)) != VC_OK)
@@ -1256,15 +1285,19 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             vc_free(ts);
             return rc;
         }
-        // per-chunk (sites, faces) counts of the vectorised passes
-        if (vec && (rc = vc_alloc(ctx, (void**)&ccnt, 4 * (size_t)(N / kVz))) != VC_OK)
+        // per-chunk (sites, faces) counts and per-voxel decision masks of the
+        // vectorised passes
+        if (vec && ((rc = vc_alloc(ctx, (void**)&ccnt, 4 * (size_t)(N / kVz))) != VC_OK ||
+                    (rc = vc_alloc(ctx, (void**)&vmask, 4 * (size_t)N)) != VC_OK))
         {
             vc_free(ts);
             vc_free(tf);
+            vc_free(ccnt);
             return rc;
         }
         if (vec)
-            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ccnt, ts, tf);
+            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ccnt, vmask, ts,
+                                                            tf);
         else
             bnd_count<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
         hipError_t e = hipGetLastError();
@@ -1315,7 +1348,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
         {
             if (vec)
                 bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                    v->occ, D, ccnt, ts, v->bnd_sites, cmap);
+                    D, ccnt, vmask, ts, v->bnd_sites, cmap);
             else
                 bnd_emit_sites<<<(unsigned)etiles, kBlock, 0, S>>>(
                     v->occ, D, N, ts, v->bnd_sites, cmap);
@@ -1324,7 +1357,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             {
                 if (vec)
                     bnd_emit_quads_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                        v->occ, D, ccnt, tf, cmap, v->bnd_quads);
+                        D, ccnt, vmask, tf, cmap, v->bnd_quads);
                 else
                     bnd_emit_quads<<<(unsigned)etiles, kBlock, 0, S>>>(
                         v->occ, D, N, tf, cmap, v->bnd_quads);
@@ -1422,6 +1455,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
         vc_free(tf);
         vc_free(cmap);
         vc_free(ccnt);
+        vc_free(vmask);
         if (rc != VC_OK)
             return rc;
         v->bnd_nsites = ns;
