@@ -153,24 +153,25 @@ __device__ __forceinline__ Emit decide(const Vox& V)
         else
             continue;
         // an earlier (linear order = lexicographic block order) in-grid
-        // block voxel touching the corner visits it first
-        bool earlier = false;
+        // block voxel touching the corner visits it first.  Block slot
+        // q = 4a + 2b + e holds voxel (lx + a, ly + b, lz + e); it touches the
+        // corner iff it differs from one of its three block neighbours
+        // (slots q ^ 4, q ^ 2, q ^ 1); this voxel is slot q0.
         const int lx = sx > 0 ? 0 : -1, ly = sy > 0 ? 0 : -1, lz = sz > 0 ? 0 : -1;
+        const int q0 = 4 * -lx + 2 * -ly + -lz;
+        uint32_t m8 = 0, g8 = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
         {
-            const int a = q >> 2, b = (q >> 1) & 1, e = q & 1;
-            const int ux = lx + a, uy = ly + b, uz = lz + e;
-            if (ux == 0 && uy == 0 && uz == 0)
-                break; // reached this voxel
-            if (!obit(V.ing, ux, uy, uz))
-                continue;
-            const bool u = obit(V.occ, ux, uy, uz);
-            const int vx = lx + (1 - a), vy = ly + (1 - b), vz = lz + (1 - e);
-            earlier |= obit(V.occ, vx, uy, uz) != u || obit(V.occ, ux, vy, uz) != u ||
-                       obit(V.occ, ux, uy, vz) != u;
+            const int ux = lx + (q >> 2), uy = ly + ((q >> 1) & 1), uz = lz + (q & 1);
+            m8 |= (uint32_t)obit(V.occ, ux, uy, uz) << q;
+            g8 |= (uint32_t)obit(V.ing, ux, uy, uz) << q;
         }
-        if (!earlier)
+        const uint32_t f4 = ((m8 >> 4) | (m8 << 4)) & 0xFFu;
+        const uint32_t f2 = ((m8 >> 2) & 0x33u) | ((m8 << 2) & 0xCCu);
+        const uint32_t f1 = ((m8 >> 1) & 0x55u) | ((m8 << 1) & 0xAAu);
+        const uint32_t touch = (m8 ^ f4) | (m8 ^ f2) | (m8 ^ f1);
+        if (!(touch & g8 & ((1u << q0) - 1u)))
             r.keys |= 1u << key;
     }
     // faces created here: +offsets, and -offsets whose neighbour is outside
