@@ -1076,10 +1076,11 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // boundary voxels of 8 consecutive z: inside with a non-inside face
-// neighbour (outside the grid counts as non-inside), one 8-byte store
+// neighbour (outside the grid counts as non-inside), one byte per chunk
+// (bit j = voxel z0 + j)
 __global__ void __launch_bounds__(kBlock)
     layer_boundary_v(const uint8_t* __restrict__ occ, Dims D,
-                     uint8_t* __restrict__ bnd)
+                     uint8_t* __restrict__ bnd8)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     Chunk K;
@@ -1088,13 +1089,20 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t me = K.w[1][1] >> 1;
     const uint32_t all6 = (K.w[1][1] & (K.w[1][1] >> 2)) & (K.w[0][1] >> 1) &
                           (K.w[2][1] >> 1) & (K.w[1][0] >> 1) & (K.w[1][2] >> 1);
-    const uint32_t b = me & ~all6 & 0xFFu;
-    uint64_t bytes = 0; // bit j -> byte j
-#pragma unroll
-    for (int j = 0; j < kVz; ++j)
-        bytes |= (uint64_t)((b >> j) & 1u) << (8 * j);
-    *reinterpret_cast<uint64_t*>(
-        bnd + ((int64_t)K.x * D.ny + K.y) * D.nz + K.z0) = bytes;
+    bnd8[c] = (uint8_t)(me & ~all6 & 0xFFu);
+}
+
+// 10-bit window of the boundary marks of row (x, y) at chunk z0 (bit b =
+// voxel z0 - 1 + b, outside the grid = 0), as row_win for occupancy
+__device__ __forceinline__ uint32_t bnd_win(const uint8_t* __restrict__ bnd8,
+                                            const Dims& D, int x, int y, int z0)
+{
+    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny)
+        return 0;
+    const int64_t c = ((int64_t)x * D.ny + y) * (D.nz / kVz) + z0 / kVz;
+    const uint32_t lo = z0 > 0 ? (uint32_t)bnd8[c - 1] >> 7 : 0u;
+    const uint32_t hi = z0 + kVz < D.nz ? (uint32_t)bnd8[c + 1] & 1u : 0u;
+    return lo | ((uint32_t)bnd8[c] << 1) | (hi << 9);
 }
 
 // thin-layer labels of 8 consecutive z: -1 outside the layer, else the
@@ -1102,21 +1110,25 @@ __global__ void __launch_bounds__(kBlock)
 // is z-connected, so it starts as one union-find tree)
 __global__ void __launch_bounds__(kBlock)
     layer_init_v(const uint8_t* __restrict__ occ,
-                 const uint8_t* __restrict__ bnd, Dims D,
+                 const uint8_t* __restrict__ bnd8, Dims D,
                  int32_t* __restrict__ lab)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Chunk B;
-    if (!load_chunk(bnd, D, c, B))
+    if (c >= (int64_t)D.nx * D.ny * (D.nz / kVz))
         return;
-    const int64_t base = ((int64_t)B.x * D.ny + B.y) * D.nz + B.z0;
+    int x, y, z0;
+    chunk_at(D, c, x, y, z0);
+    const int64_t base = ((int64_t)x * D.ny + y) * D.nz + z0;
     const uint64_t o8 = *reinterpret_cast<const uint64_t*>(occ + base);
     uint32_t near = 0; // bit b: a boundary voxel within z0 - 1 + b +- 1
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
-            near |= B.w[dx][dy] | (B.w[dx][dy] >> 1) | (B.w[dx][dy] << 1);
+        {
+            const uint32_t w = bnd_win(bnd8, D, x + dx - 1, y + dy - 1, z0);
+            near |= w | (w >> 1) | (w << 1);
+        }
     int32_t out[kVz];
     int32_t start = -1;
 #pragma unroll
@@ -1175,6 +1187,25 @@ __global__ void __launch_bounds__(kBlock)
     }
     if ((me >> 7) & 1u && z0 + kVz < D.nz && lab[base + kVz] >= 0)
         uf_union(lab, (int32_t)(base + kVz - 1), (int32_t)(base + kVz), guard_hits);
+}
+
+// thin-layer components: roots (lab[i] == i) of 8 consecutive labels
+__global__ void __launch_bounds__(kBlock)
+    layer_count_v(const int32_t* __restrict__ lab, int64_t nchunk,
+                  unsigned long long* acc)
+{
+    __shared__ unsigned long long sh[kBlock / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long r = 0;
+    if (c < nchunk)
+    {
+        const int4* p = reinterpret_cast<const int4*>(lab + c * kVz);
+        const int4 a = p[0], b = p[1];
+        const int32_t i = (int32_t)(c * kVz);
+        r = (a.x == i) + (a.y == i + 1) + (a.z == i + 2) + (a.w == i + 3) +
+            (b.x == i + 4) + (b.y == i + 5) + (b.z == i + 6) + (b.w == i + 7);
+    }
+    block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
 }
 
 // the reference's per-mask surface-component table, derived from first
@@ -1396,7 +1427,7 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
                 layer_init_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
                 layer_union_v<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
-                layer_count<<<(unsigned)ntiles, kBlock, 0, S>>>(lab, N, acc);
+                layer_count_v<<<gv, kBlock, 0, S>>>(lab, N / kVz, acc);
                 e = hipGetLastError();
             }
             else if (rc == VC_OK && e == hipSuccess)
