@@ -282,6 +282,47 @@ def _random_octree(R, seed):
     return bytes(out)
 
 
+def boundary_row(ctx, occ_host, reps=3):
+    """R3/R4: boundary sites, quads and cube-complex Euler terms of the
+    headline volume (computed once per volume: a fresh upload per
+    repetition, the size query computes all), best of `reps`."""
+    def best_of(fn, k=reps):
+        ts = []
+        for _ in range(k):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), r
+
+    n = occ_host.shape[0]
+    vols = [ctx.upload(occ_host) for _ in range(reps)]
+    it = iter(vols)
+
+    def bnd():
+        v = next(it)
+        L = ctx._lib
+        ns, nq = ctypes.c_int64(), ctypes.c_int64()
+        L.vc_boundary_extract(ctx.handle, v.handle, None, 0, ctypes.byref(ns), None, 0,
+                              ctypes.byref(nq), None)
+        return ns.value, nq.value
+    t_b, (ns, nq) = best_of(bnd)
+    eu = vols[0].boundary(with_euler=True)[2]
+    for v in vols:
+        v.release()
+    N = occ_host.size
+    algo_b = N * 1 + ns * 12 + nq * 16
+    return {
+        "workload": "boundary sites + quads + Euler terms of the %d^3 headline volume" % n,
+        "ms": round(t_b * 1e3, 3), "sites": ns, "quads": nq, "euler": eu["euler"],
+        "chi": eu["chi"], "value": round(N / t_b / 1e6, 1), "unit": "Mvoxels/s",
+        "roofline": {"bound": "hbm", "achieved": round(algo_b / t_b / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(algo_b / t_b / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes": "1 B/voxel + 12 B/site + 16 B/quad"}}
+
+
 def rows_leg(ctx, occ_host, label_host, reps=3):
     """The other rows of the path, each on its own input, timed on the
     device stream (wall clock around the C-ABI call; the Voronoi-side calls
@@ -301,33 +342,7 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
             ts.append(time.perf_counter() - t0)
         return min(ts), r
 
-    # R3/R4: boundary sites, quads and cube-complex Euler terms (computed once
-    # per volume: a fresh upload per repetition, the size query computes all)
-    n = occ_host.shape[0]
-    vols = [ctx.upload(occ_host) for _ in range(reps)]
-    it = iter(vols)
-
-    def bnd():
-        v = next(it)
-        L = ctx._lib
-        ns, nq = ctypes.c_int64(), ctypes.c_int64()
-        L.vc_boundary_extract(ctx.handle, v.handle, None, 0, ctypes.byref(ns), None, 0,
-                              ctypes.byref(nq), None)
-        return ns.value, nq.value
-    t_b, (ns, nq) = best_of(bnd)
-    eu = vols[0].boundary(with_euler=True)[2]
-    for v in vols:
-        v.release()
-    N = occ_host.size
-    algo_b = N * 1 + ns * 12 + nq * 16
-    out["R3_R4_boundary"] = {
-        "workload": "boundary sites + quads + Euler terms of the %d^3 headline volume" % n,
-        "ms": round(t_b * 1e3, 3), "sites": ns, "quads": nq, "euler": eu["euler"],
-        "chi": eu["chi"], "value": round(N / t_b / 1e6, 1), "unit": "Mvoxels/s",
-        "roofline": {"bound": "hbm", "achieved": round(algo_b / t_b / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(algo_b / t_b / 1e9 / HBM_PEAK_GBS, 4),
-                     "bytes": "1 B/voxel + 12 B/site + 16 B/quad"}}
+    out["R3_R4_boundary"] = boundary_row(ctx, occ_host, reps)
 
     # R2: dense occupancy from a .sof octree (R = 256)
     stream = _random_octree(256, 9)
@@ -549,6 +564,8 @@ def main():
                     help="skip the batched-solids leg (256 volumes)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched-solids leg (for profiling)")
+    ap.add_argument("--boundary-only", action="store_true",
+                    help="only the boundary row on the headline volume (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate-guided pruning leg")
     ap.add_argument("--no-rows", action="store_true",
@@ -596,6 +613,11 @@ def main():
     size = synth.inside_size(occ_host)
     levels = synth.levels_from_tt(size)
     vol_hash = hashlib.sha256(occ_host.tobytes()).hexdigest()[:16]
+    if a.boundary_only:
+        occ_dev.free()
+        print(json.dumps({"R3_R4_boundary": boundary_row(ctx, occ_host)}), flush=True)
+        ctx.close()
+        return
 
     vol = ctx.from_device(occ_dev.ptr, (n, n, n))
     ctx.synchronize()
