@@ -343,6 +343,7 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
         return min(ts), r
 
     out["R3_R4_boundary"] = boundary_row(ctx, occ_host, reps)
+    n = occ_host.shape[0]
 
     # R2: dense occupancy from a .sof octree (R = 256)
     stream = _random_octree(256, 9)
