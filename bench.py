@@ -76,50 +76,103 @@ def measured_traffic(kernel, n):
     return None if v is None else round(v)
 
 
-def cpu_baseline(occ_host, levels, target_s=15.0):
-    """The oracle (single-threaded C restatement) on a bounded cubic crop of the
-    same volume: EDT + measure + levels + thinning."""
+def _host_cpu():
+    """Host CPU description from lscpu (model, sockets, cores per socket,
+    threads per core), /proc/cpuinfo as a fallback."""
+    import subprocess
+    info = {"nproc_visible": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                             text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+                "CPU(s)": "cpus", "CPU max MHz": "max_mhz"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "model" not in info:
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
+
+
+def _native_oracle():
+    """Builds the oracle with -O3 -march=native on this machine (the
+    Makefile's `native` target, into a fresh temporary directory, so no
+    build made for another CPU is picked up); returns (path, flags), or the
+    portable build and its flags if the native build fails."""
+    import subprocess
+    import tempfile
+    odir = os.path.join(ROOT, "oracle")
+    out = tempfile.mkdtemp(prefix="vc_oracle_native_")
+    r = subprocess.run(["make", "-B", "-s", "-C", odir, "native", "OUT=" + out],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    flags = subprocess.run(["make", "-s", "-C", odir, "native-flags"], stdout=subprocess.PIPE,
+                           stderr=subprocess.DEVNULL, text=True).stdout.strip()
+    path = os.path.join(out, "native", "liboracle.so")
+    if r.returncode == 0 and os.path.exists(path):
+        return path, flags
+    print("[bench] native oracle build failed, using the portable one:\n" + r.stdout,
+          file=sys.stderr, flush=True)
+    return os.path.join(odir, "_build", "liboracle.so"), "-O2 -march=x86-64-v2 -fopenmp"
+
+
+def cpu_baseline(occ_host, levels, omp_runs=5):
+    """The oracle (the C restatement, built -O3 -march=native on this host)
+    on the full volume: EDT + measure + levels + thinning.  The OpenMP leg on
+    all host threads is the median of `omp_runs` runs after one warm-up; the
+    sequential leg (the reference's single-threaded design) runs once."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_lib as O
 
-    n = occ_host.shape[0]
+    path, flags = _native_oracle()
+    O.LIB_PATH = path
+    O._lib = None
+    O.lib()
+    if "vc_oracle_native_" in path:
+        import shutil
+        shutil.rmtree(os.path.dirname(os.path.dirname(path)), ignore_errors=True)
+    occ = np.ascontiguousarray(occ_host)
+    n_vox = occ.size
+
+    def once():
+        t0 = time.perf_counter()
+        d, f = O.edt_ft(occ)
+        del d
+        lam = O.lambda2(occ, f)
+        del f
+        lvl = O.levels(occ, lam, levels)
+        del lam
+        O.thin(occ, lvl, len(levels))
+        return time.perf_counter() - t0
+
+    threads = O.set_threads(0)
+    print("[bench] cpu baseline: oracle on %d threads, %d runs + warm-up"
+          % (threads, omp_runs), file=sys.stderr, flush=True)
+    once()  # warm-up; also fills the oracle's simple-point memo
+    ts = sorted(once() for _ in range(omp_runs))
+    med = ts[len(ts) // 2]
+    O.set_threads(1)
+    print("[bench] cpu baseline: oracle on 1 thread", file=sys.stderr, flush=True)
+    t1 = once()
+    O.set_threads(threads)
+    shape = "x".join(str(s) for s in occ.shape)
+    # the boundary row (sites, quads, Euler terms) on the central 128^3 crop
+    n = occ.shape[0]
     side = min(n, 128)
     c0 = (n - side) // 2
-    crop = np.ascontiguousarray(occ_host[c0:c0 + side, c0:c0 + side, c0:c0 + side])
-
-    def run(limit_s, max_reps):
-        t0 = time.perf_counter()
-        reps = 0
-        while True:
-            d, f = O.edt_ft(crop)
-            lam = O.lambda2(crop, f)
-            lvl = O.levels(crop, lam, levels)
-            O.thin(crop, lvl, len(levels))
-            reps += 1
-            el = time.perf_counter() - t0
-            if el > limit_s or reps >= max_reps:
-                return reps, el
-    # the sequential restatement (the reference's single-threaded design),
-    # then its OpenMP loops on the host's cores
-    threads = O.set_threads(0)
-    O.set_threads(1)
-    run(0.0, 1)  # untimed: fills the oracle's simple-point memo for both runs
-    reps, el = run(target_s, 20)
-    mvox = crop.size * reps / el / 1e6
-    O.set_threads(threads)
-    mreps, mel = run(target_s / 3, 40)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    # the boundary row (sites, quads, Euler terms) on the same crop
+    crop = np.ascontiguousarray(occ[c0:c0 + side, c0:c0 + side, c0:c0 + side])
     t0 = time.perf_counter()
     breps = 0
     while True:
@@ -133,7 +186,7 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
     import voro_ref as VR
     vside = min(n, 64)
     v0 = (n - vside) // 2
-    vcrop = np.ascontiguousarray(occ_host[v0:v0 + vside, v0:v0 + vside, v0:v0 + vside])
+    vcrop = np.ascontiguousarray(occ[v0:v0 + vside, v0:v0 + vside, v0:v0 + vside])
     _, vft = O.edt_ft(vcrop)
     t0 = time.perf_counter()
     vreps = 0
@@ -143,17 +196,23 @@ def cpu_baseline(occ_host, levels, target_s=15.0):
         vel = time.perf_counter() - t0
         if vel > 3.0 or vreps >= 10:
             break
-    return {"value": round(mvox, 3), "unit": "Mvoxels/s", "cores": 1, "kind": "port",
-            "sample": "oracle (single-thread C) EDT+measure+thinning on the central %d^3 crop "
-                      "of the same volume, same levels, %d reps in %.1f s" % (side, reps, el),
-            "openmp": {"value": round(crop.size * mreps / mel / 1e6, 3), "unit": "Mvoxels/s",
-                       "threads": threads,
-                       "sample": "the same oracle with its OpenMP loops, %d reps in %.1f s"
-                                 % (mreps, mel)},
-            "host": {"nproc": os.cpu_count(), "model": model},
+    return {"value": round(n_vox / med / 1e6, 3), "unit": "Mvoxels/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle (C restatement, %s) EDT+measure+levels+thinning on the full %s "
+                      "volume with the same levels, OpenMP on %d threads: median of %d runs "
+                      "(%s s) after one warm-up" % (flags, shape, threads, omp_runs,
+                                                    ", ".join("%.2f" % t for t in ts), ),
+            "single_thread": {"value": round(n_vox / t1 / 1e6, 3), "unit": "Mvoxels/s",
+                              "cores": 1, "seconds": round(t1, 2),
+                              "sample": "the same oracle on 1 thread (the reference's "
+                                        "single-threaded design), the full %s volume, "
+                                        "one run" % shape},
+            "build": flags,
+            "host": _host_cpu(),
             "rows": {"R3_R4_boundary": {"value": round(crop.size * breps / bel / 1e6, 3),
                                         "unit": "Mvoxels/s", "sample": "oracle boundary + "
-                                        "Euler on the same crop, %d reps" % breps},
+                                        "Euler (sequential) on the central %d^3 crop, %d reps"
+                                        % (side, breps)},
                      "voronoi_from_ft": {"value": round(vcrop.size * vreps / vel / 1e6, 3),
                                          "unit": "Mvoxels/s",
                                          "sample": "oracle (numpy) discrete Voronoi diagram "

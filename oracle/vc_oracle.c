@@ -315,44 +315,59 @@ static uint32_t nb26_to_cube(uint32_t nb26)
     return lo | (hi << 14);
 }
 
-/* BFS component count of `set` (27-bit) under adjacency `adj` (6 or 26);
- * if `must` != 0 only components meeting `must` are counted. */
+/* Adjacency of the 27 cells as one 27-bit mask per cell, from coordinate
+ * differences (26: Chebyshev distance 1; 6: L1 distance 1).  Built before
+ * any parallel loop that tests simplicity. */
+static uint32_t g_adj26[27], g_adj6[27];
+static int g_adj_ready = 0;
+
+static void init_adj(void)
+{
+    if (g_adj_ready)
+        return;
+    for (int u = 0; u < 27; ++u)
+    {
+        int ux, uy, uz;
+        cell_xyz(u, &ux, &uy, &uz);
+        uint32_t a26 = 0, a6 = 0;
+        for (int v = 0; v < 27; ++v)
+        {
+            int vx, vy, vz;
+            cell_xyz(v, &vx, &vy, &vz);
+            int ax = abs(ux - vx), ay = abs(uy - vy), az = abs(uz - vz);
+            if (v != u && ax <= 1 && ay <= 1 && az <= 1)
+                a26 |= 1u << v;
+            if (ax + ay + az == 1)
+                a6 |= 1u << v;
+        }
+        g_adj26[u] = a26;
+        g_adj6[u] = a6;
+    }
+    g_adj_ready = 1;
+}
+
+/* BFS component count of `set` (27-bit) under adjacency `adj` (6 or 26),
+ * one cell popped at a time; if `must` != 0 only components meeting `must`
+ * are counted. */
 static int bfs_components(uint32_t set, int adj, uint32_t must)
 {
-    int seen[27] = {0};
-    int queue[27];
+    const uint32_t* nbr = adj == 26 ? g_adj26 : g_adj6;
     int count = 0;
-    for (int s = 0; s < 27; ++s)
+    uint32_t rest = set;
+    while (rest)
     {
-        if (!((set >> s) & 1u) || seen[s])
-            continue;
-        int head = 0, tail = 0, hits = 0;
-        queue[tail++] = s;
-        seen[s] = 1;
-        while (head < tail)
+        uint32_t comp = rest & (~rest + 1u);
+        uint32_t queue = comp;
+        while (queue)
         {
-            int u = queue[head++];
-            if ((must >> u) & 1u)
-                hits = 1;
-            int ux, uy, uz;
-            cell_xyz(u, &ux, &uy, &uz);
-            for (int v = 0; v < 27; ++v)
-            {
-                if (!((set >> v) & 1u) || seen[v])
-                    continue;
-                int vx, vy, vz;
-                cell_xyz(v, &vx, &vy, &vz);
-                int ax = abs(ux - vx), ay = abs(uy - vy), az = abs(uz - vz);
-                int ok = adj == 26 ? (ax <= 1 && ay <= 1 && az <= 1)
-                                   : (ax + ay + az == 1);
-                if (ok)
-                {
-                    seen[v] = 1;
-                    queue[tail++] = v;
-                }
-            }
+            int u = __builtin_ctz(queue);
+            queue &= queue - 1u;
+            uint32_t fresh = nbr[u] & set & ~comp;
+            comp |= fresh;
+            queue |= fresh;
         }
-        if (!must || hits)
+        rest &= ~comp;
+        if (!must || (comp & must))
             count++;
     }
     return count;
@@ -360,6 +375,7 @@ static int bfs_components(uint32_t set, int adj, uint32_t must)
 
 int or_is_simple(uint32_t nb26)
 {
+    init_adj();
     uint32_t X = nb26_to_cube(nb26); /* centre bit 13 clear */
     uint32_t n18 = 0, n6 = 0;
     for (int j = 0; j < 27; ++j)
@@ -575,6 +591,42 @@ void or_simple_lut(uint32_t* lut_words)
     }
 }
 
+/* All 2^26 neighbourhood codes: the BFS definition (or_is_simple) against
+ * the bit-parallel one that fills the table (fast_simple).  Returns the
+ * number of codes where they differ; *n_simple gets the number of simple
+ * codes by the BFS definition.  With `topo_stride` > 0, every
+ * topo_stride-th code (starting at topo_offset) is also checked against the
+ * topological definition on a 5x5x5 box (or_is_simple_topo) and the
+ * disagreements are added to *topo_mismatch. */
+int64_t or_simple_exhaustive(int64_t* n_simple, int64_t topo_stride,
+                             int64_t topo_offset, int64_t* topo_mismatch,
+                             int64_t* topo_checked)
+{
+    init_masks();
+    init_adj();
+    int64_t bad = 0, simple = 0, tbad = 0, tchk = 0;
+#pragma omp parallel for schedule(dynamic, 65536) reduction(+ : bad, simple, tbad, tchk)
+    for (int64_t c = 0; c < ((int64_t)1 << 26); ++c)
+    {
+        int a = or_is_simple((uint32_t)c);
+        int b = fast_simple((uint32_t)c);
+        simple += a;
+        bad += a != b;
+        if (topo_stride > 0 && c % topo_stride == topo_offset % topo_stride)
+        {
+            tchk++;
+            tbad += a != or_is_simple_topo((uint32_t)c);
+        }
+    }
+    if (n_simple)
+        *n_simple = simple;
+    if (topo_mismatch)
+        *topo_mismatch = tbad;
+    if (topo_checked)
+        *topo_checked = tchk;
+    return bad;
+}
+
 /* ======================================================================
  * Measure-ordered subfield thinning (project-defined R15).
  * For each level k (ascending): repeat full sweeps until a sweep removes
@@ -626,6 +678,7 @@ int or_thin(uint8_t* label, const uint8_t* lvl, int nx, int ny, int nz,
         memset(st, 0, sizeof(*st));
     if (!g_memo)
         g_memo = (uint8_t*)calloc((size_t)1 << 26, 1);
+    init_adj();
     for (int k = 0; k < nlev; ++k)
     {
         int changed;
@@ -684,6 +737,7 @@ int64_t or_euler_voxels(const uint8_t* occ, int nx, int ny, int nz)
 {
     int64_t V = 0, E = 0, F = 0, C = 0;
     /* vertices: lattice points with any of the 8 incident voxels inside */
+#pragma omp parallel for schedule(static) reduction(+ : V)
     for (int x = 0; x <= nx; ++x)
         for (int y = 0; y <= ny; ++y)
             for (int z = 0; z <= nz; ++z)
@@ -696,6 +750,7 @@ int64_t or_euler_voxels(const uint8_t* occ, int nx, int ny, int nz)
                 V += any;
             }
     /* edges along each axis: 4 incident voxels */
+#pragma omp parallel for schedule(static) reduction(+ : E)
     for (int x = 0; x < nx; ++x) /* x-edges at (x, y, z) lattice, y<=ny, z<=nz */
         for (int y = 0; y <= ny; ++y)
             for (int z = 0; z <= nz; ++z)
@@ -706,6 +761,7 @@ int64_t or_euler_voxels(const uint8_t* occ, int nx, int ny, int nz)
                         any |= occ_at(occ, nx, ny, nz, x, y + b, z + c);
                 E += any;
             }
+#pragma omp parallel for schedule(static) reduction(+ : E)
     for (int x = 0; x <= nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int z = 0; z <= nz; ++z)
@@ -716,6 +772,7 @@ int64_t or_euler_voxels(const uint8_t* occ, int nx, int ny, int nz)
                         any |= occ_at(occ, nx, ny, nz, x + a, y, z + c);
                 E += any;
             }
+#pragma omp parallel for schedule(static) reduction(+ : E)
     for (int x = 0; x <= nx; ++x)
         for (int y = 0; y <= ny; ++y)
             for (int z = 0; z < nz; ++z)
@@ -727,50 +784,101 @@ int64_t or_euler_voxels(const uint8_t* occ, int nx, int ny, int nz)
                 E += any;
             }
     /* faces: 2 incident voxels */
+#pragma omp parallel for schedule(static) reduction(+ : F)
     for (int x = 0; x <= nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int z = 0; z < nz; ++z)
                 F += occ_at(occ, nx, ny, nz, x - 1, y, z) |
                      occ_at(occ, nx, ny, nz, x, y, z);
+#pragma omp parallel for schedule(static) reduction(+ : F)
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y <= ny; ++y)
             for (int z = 0; z < nz; ++z)
                 F += occ_at(occ, nx, ny, nz, x, y - 1, z) |
                      occ_at(occ, nx, ny, nz, x, y, z);
+#pragma omp parallel for schedule(static) reduction(+ : F)
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int z = 0; z <= nz; ++z)
                 F += occ_at(occ, nx, ny, nz, x, y, z - 1) |
                      occ_at(occ, nx, ny, nz, x, y, z);
+#pragma omp parallel for schedule(static) reduction(+ : C)
     for (int64_t i = 0; i < (int64_t)nx * ny * nz; ++i)
         C += occ[i] != 0;
     return V - E + F - C;
 }
 
-static int64_t label_components(const uint8_t* occ, int nx, int ny, int nz,
-                                int want, int adj, int64_t* n_border)
+/* Connected components of the voxels whose occupancy equals `want`, by BFS
+ * over a copy of the grid padded by one voxel on every side: the pad holds
+ * `pad` (1 = member, 0 = not), so no neighbour needs a bounds check.  With
+ * pad = 1 every component touching the grid border is joined through the
+ * pad into one component (and the pad alone counts as one when none does). */
+static int64_t padded_components(const uint8_t* occ, int nx, int ny, int nz,
+                                 int want, int adj, int pad)
 {
-    int64_t N = (int64_t)nx * ny * nz;
-    uint8_t* seen = (uint8_t*)calloc((size_t)N, 1);
-    int64_t* q = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N > 0 ? N : 1));
-    int64_t count = 0, border = 0;
-    for (int64_t s = 0; s < N; ++s)
+    const int64_t px = nx + 2, py = ny + 2, pz = nz + 2;
+    const int64_t NP = px * py * pz;
+    uint8_t* m = (uint8_t*)malloc((size_t)NP);
+    int64_t* q = (int64_t*)malloc(sizeof(int64_t) * (size_t)NP);
+    if (!m || !q)
     {
-        if ((occ[s] != 0) != want || seen[s])
+        free(m);
+        free(q);
+        return -1;
+    }
+    memset(m, pad ? 1 : 0, (size_t)NP);
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+        {
+            uint8_t* row = m + ((x + 1) * py + (y + 1)) * pz + 1;
+            const uint8_t* src = occ + IDX(x, y, 0);
+            for (int z = 0; z < nz; ++z)
+                row[z] = (uint8_t)((src[z] != 0) == want);
+        }
+    int64_t off[26];
+    int noff = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
+            {
+                int l1 = abs(dx) + abs(dy) + abs(dz);
+                if (l1 == 0 || (adj == 6 && l1 != 1))
+                    continue;
+                off[noff++] = (dx * py + dy) * pz + dz;
+            }
+    /* members are 1; a visited member becomes 2.  Neighbours of a
+     * non-pad cell stay inside the padded grid; pad cells (pad = 1) have
+     * out-of-range neighbours, so their offsets are checked against the
+     * padded bounds. */
+    int64_t count = 0;
+    for (int64_t s = 0; s < NP; ++s)
+    {
+        if (m[s] != 1)
             continue;
         count++;
-        int touches = 0;
         int64_t h = 0, t = 0;
         q[t++] = s;
-        seen[s] = 1;
+        m[s] = 2;
         while (h < t)
         {
             int64_t u = q[h++];
-            int uz = (int)(u % nz), uy = (int)((u / nz) % ny),
-                ux = (int)(u / ((int64_t)nz * ny));
-            if (ux == 0 || uy == 0 || uz == 0 || ux == nx - 1 ||
-                uy == ny - 1 || uz == nz - 1)
-                touches = 1;
+            int64_t uz = u % pz, uy = (u / pz) % py, ux = u / (pz * py);
+            int edge = ux == 0 || uy == 0 || uz == 0 || ux == px - 1 ||
+                       uy == py - 1 || uz == pz - 1;
+            if (!edge)
+            {
+                for (int k = 0; k < noff; ++k)
+                {
+                    int64_t v = u + off[k];
+                    if (m[v] == 1)
+                    {
+                        m[v] = 2;
+                        q[t++] = v;
+                    }
+                }
+                continue;
+            }
             for (int dx = -1; dx <= 1; ++dx)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dz = -1; dz <= 1; ++dz)
@@ -778,37 +886,34 @@ static int64_t label_components(const uint8_t* occ, int nx, int ny, int nz,
                         int l1 = abs(dx) + abs(dy) + abs(dz);
                         if (l1 == 0 || (adj == 6 && l1 != 1))
                             continue;
-                        int vx = ux + dx, vy = uy + dy, vz = uz + dz;
-                        if (vx < 0 || vy < 0 || vz < 0 || vx >= nx ||
-                            vy >= ny || vz >= nz)
+                        int64_t vx = ux + dx, vy = uy + dy, vz = uz + dz;
+                        if (vx < 0 || vy < 0 || vz < 0 || vx >= px ||
+                            vy >= py || vz >= pz)
                             continue;
-                        int64_t v = IDX(vx, vy, vz);
-                        if ((occ[v] != 0) != want || seen[v])
-                            continue;
-                        seen[v] = 1;
-                        q[t++] = v;
+                        int64_t v = (vx * py + vy) * pz + vz;
+                        if (m[v] == 1)
+                        {
+                            m[v] = 2;
+                            q[t++] = v;
+                        }
                     }
         }
-        border += touches;
     }
-    free(seen);
+    free(m);
     free(q);
-    if (n_border)
-        *n_border = border;
     return count;
 }
 
 int64_t or_components26(const uint8_t* occ, int nx, int ny, int nz)
 {
-    return label_components(occ, nx, ny, nz, 1, 26, NULL);
+    return padded_components(occ, nx, ny, nz, 1, 26, 0);
 }
 
 int64_t or_bg_components6(const uint8_t* occ, int nx, int ny, int nz)
 {
-    int64_t border = 0;
-    int64_t c = label_components(occ, nx, ny, nz, 0, 6, &border);
-    /* every component touching the grid border joins the outer shell */
-    return c - border + 1;
+    /* the outside of the grid is background: components touching the grid
+     * border are one (the unbounded one) */
+    return padded_components(occ, nx, ny, nz, 0, 6, 1);
 }
 
 /* ======================================================================
@@ -1070,7 +1175,7 @@ int or_boundary(const uint8_t* occ, int nx, int ny, int nz, float* sites,
         Ei /= 4;
         Fi /= 2;
         /* C: 6-components of the thin inside layer (:619-660) */
-        int64_t C = label_components(thin, nx, ny, nz, 1, 6, NULL);
+        int64_t C = padded_components(thin, nx, ny, nz, 1, 6, 0);
         eu->V_bnd = V;
         eu->E_bnd = E;
         eu->V_int = Vi;

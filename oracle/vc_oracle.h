@@ -60,6 +60,11 @@ void or_levels(const uint8_t* occ, const int32_t* lam2, int64_t n,
 int or_is_simple(uint32_t nb26);            /* Bertrand-Malandain T26/T6 */
 int or_is_simple_topo(uint32_t nb26);       /* b0(X), b0(~X), chi preserved */
 void or_simple_lut(uint32_t* lut_words);    /* 2^21 words, bit = simple */
+/* all 2^26 codes: BFS definition vs the bit-parallel one (mismatches);
+ * every topo_stride-th code also vs the 5x5x5 topological definition */
+int64_t or_simple_exhaustive(int64_t* n_simple, int64_t topo_stride,
+                             int64_t topo_offset, int64_t* topo_mismatch,
+                             int64_t* topo_checked);
 
 /* ---------------- measure-ordered subfield thinning (project-defined) --- */
 typedef struct or_thin_stats {

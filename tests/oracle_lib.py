@@ -44,6 +44,9 @@ def lib():
         L.or_is_simple.argtypes = [ctypes.c_uint32]
         L.or_is_simple_topo.argtypes = [ctypes.c_uint32]
         L.or_simple_lut.argtypes = [P]
+        L.or_simple_exhaustive.argtypes = [ctypes.POINTER(i64), i64, i64, ctypes.POINTER(i64),
+                                           ctypes.POINTER(i64)]
+        L.or_simple_exhaustive.restype = i64
         L.or_thin.argtypes = [P, P, i32, i32, i32, i32, ctypes.POINTER(OrThinStats)]
         L.or_euler_voxels.argtypes = [P, i32, i32, i32]
         L.or_euler_voxels.restype = i64
@@ -156,6 +159,15 @@ def betti(occ):
     b0 = components26(occ)
     b2 = bg_components6(occ) - 1
     return b0, b0 + b2 - chi, b2, chi
+
+
+def simple_exhaustive(topo_stride=0, topo_offset=0):
+    """(BFS-vs-table mismatches over all 2^26 codes, simple codes,
+    topological-definition mismatches, codes checked topologically)."""
+    ns, tb, tc = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    bad = lib().or_simple_exhaustive(ctypes.byref(ns), int(topo_stride), int(topo_offset),
+                                     ctypes.byref(tb), ctypes.byref(tc))
+    return int(bad), ns.value, tb.value, tc.value
 
 
 def cube_components():

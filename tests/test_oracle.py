@@ -84,6 +84,60 @@ def test_lambda_closed_form(oracle):
     assert lam[2, 2, 3] == 9 and lam[2, 2, 4] == 9
 
 
+def _lambda2_numpy(occ, ft):
+    """lambda^2 by array shifts: for each of the 6 face neighbours, the
+    squared distance between the two voxels' feature coordinates where both
+    are inside with a feature; the maximum over the 6 (0 elsewhere)."""
+    nx, ny, nz = occ.shape
+    ok = (occ > 0) & (ft >= 0)
+    fx, fy, fz = np.unravel_index(np.where(ok, ft, 0), occ.shape)
+    cx = np.stack([fx, fy, fz]).astype(np.int64)
+    out = np.zeros(occ.shape, np.int64)
+    for ax in range(3):
+        for sgn in (-1, 1):
+            sl_a = [slice(None)] * 3
+            sl_b = [slice(None)] * 3
+            if sgn > 0:
+                sl_a[ax], sl_b[ax] = slice(0, -1), slice(1, None)
+            else:
+                sl_a[ax], sl_b[ax] = slice(1, None), slice(0, -1)
+            a, b = tuple(sl_a), tuple(sl_b)
+            both = ok[a] & ok[b]
+            d = ((cx[(slice(None),) + a] - cx[(slice(None),) + b]) ** 2).sum(axis=0)
+            out[a] = np.maximum(out[a], np.where(both, d, 0))
+    return out
+
+
+def test_lambda_against_brute_force_features(oracle):
+    """lambda^2 of the oracle against an independent restatement: features
+    from the brute-force nearest-background search (or_edt_brute, O(N * B))
+    and the 6-neighbour maximum by array shifts, on random volumes of every
+    fill (including a volume with no background, where every feature is
+    -1 and lambda^2 = 0)."""
+    rng = np.random.default_rng(41)
+    for i in range(60):
+        shape = tuple(int(s) for s in rng.integers(1, 16, size=3))
+        occ = (rng.random(shape) < rng.choice([0.3, 0.6, 0.85, 0.97])).astype(np.uint8)
+        if i == 0:
+            occ[:] = 1
+        db, fb = oracle.edt_brute(occ)
+        d, f = oracle.edt_ft(occ)
+        assert np.array_equal(f, fb), i
+        assert np.array_equal(oracle.lambda2(occ, fb), _lambda2_numpy(occ, fb)), i
+    # slab of even width 2k between two background planes: the middle pair's
+    # features lie on opposite planes, (2k + 1)^2 apart; every other inside
+    # voxel sees features one apart across y or z, or identical ones
+    for k in (1, 2, 5):
+        occ = np.zeros((2 * k + 2, 5, 6), np.uint8)
+        occ[1:2 * k + 1] = 1
+        d, f = oracle.edt_ft(occ)
+        lam = oracle.lambda2(occ, f)
+        assert (lam[k] == (2 * k + 1) ** 2).all() and (lam[k + 1] == (2 * k + 1) ** 2).all()
+        rest = np.ones(occ.shape[0], bool)
+        rest[[0, k, k + 1, 2 * k + 1]] = False
+        assert set(np.unique(lam[rest])) <= {0, 1}
+
+
 def test_level_thresholds(oracle):
     for l, T in [(1.5, 3), (2.0, 4), (0.0, 0), (-1.0, 0), (3.0000002, 10), (0.55, 1)]:
         assert oracle.level_thresholds([l])[0] == T, l
@@ -269,3 +323,45 @@ def test_oracle_threads_do_not_change_results(oracle):
     oracle.set_threads(n0)
     for a, b in zip(*out):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_component_counts_match_scipy_label(oracle):
+    """The Betti helpers' component counts against scipy.ndimage.label (an
+    independent library routine): 26-components of the object, and
+    6-components of the background with the outside of the grid counted as
+    background (so every component touching the border is the one
+    unbounded component); chi of a union of closed cubes is additive over
+    disjoint pieces."""
+    from scipy import ndimage
+    rng = np.random.default_rng(31)
+    s26 = np.ones((3, 3, 3), bool)
+    s6 = ndimage.generate_binary_structure(3, 1)
+    for i in range(40):
+        shape = tuple(int(s) for s in rng.integers(1, 24, size=3))
+        occ = (rng.random(shape) < rng.choice([0.2, 0.5, 0.7, 0.9])).astype(np.uint8)
+        assert oracle.components26(occ) == ndimage.label(occ, structure=s26)[1], i
+        bg = np.pad(occ == 0, 1, constant_values=True)
+        assert oracle.bg_components6(occ) == ndimage.label(bg, structure=s6)[1], i
+    # chi adds over pieces that share no lattice point
+    a = (rng.random((9, 9, 9)) < 0.6).astype(np.uint8)
+    b = (rng.random((9, 9, 9)) < 0.4).astype(np.uint8)
+    both = np.zeros((9, 20, 9), np.uint8)
+    both[:, :9] = a
+    both[:, 11:] = b
+    assert oracle.euler_voxels(both) == oracle.euler_voxels(a) + oracle.euler_voxels(b)
+
+
+def test_simple_point_definitions_agree_on_all_codes(oracle, simple_lut_cpu):
+    """Every one of the 2^26 neighbourhood codes: the BFS definition
+    (T26 = 1 and T6 = 1 by per-cell adjacency lists) equals the
+    bit-parallel definition that fills the table; every 61st code is also
+    checked against the topological definition (b0 of the object, b0 of the
+    background and chi of a padded 5x5x5 box unchanged when the centre is
+    removed).  The table the GPU builds is compared with this one bit for
+    bit in the GPU suite."""
+    bad, n_simple, tbad, tchk = oracle.simple_exhaustive(topo_stride=61, topo_offset=17)
+    assert bad == 0
+    assert tchk == len(range(17, 1 << 26, 61)) and tbad == 0
+    # the table holds exactly the simple codes the BFS definition counts
+    pop = int(np.unpackbits(simple_lut_cpu.view(np.uint8)).sum())
+    assert pop == n_simple
