@@ -372,11 +372,18 @@ __global__ void colsum_reduce(const float* __restrict__ part, int nb, int C,
 }
 
 // ------------------------------------------------------------ elementwise
+// The work-items of one dispatch along x are counted in 32 bits, so a launch
+// of one thread per (voxel, channel) of a 512^3 activation (up to 48 x 2^27
+// elements) cannot cover its range: these kernels loop over their range with
+// the grid's stride and are launched with at most kMaxGrid workgroups.
+#define VC_GRID_STRIDE(t, n)                                                  \
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (n); \
+         t += (int64_t)gridDim.x * blockDim.x)
 __global__ void __launch_bounds__(kBlock)
     relu_mask(bf16* __restrict__ g, const bf16* __restrict__ y, int64_t n)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n && !((float)y[t] > 0.0f))
+    VC_GRID_STRIDE(t, n)
+    if (!((float)y[t] > 0.0f))
         g[t] = (bf16)0.0f;
 }
 
@@ -395,25 +402,25 @@ __global__ void __launch_bounds__(kBlock)
              int Y, int Z)
 {
     const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)Xp * Yp * Zp * C;
-    if (t >= n)
-        return;
-    const int c = (int)(t % C);
-    const int64_t v = t / C;
-    const int z = (int)(v % Zp);
-    const int64_t r = v / Zp;
-    const int y = (int)(r % Yp), x = (int)(r / Yp);
-    float s = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
+    VC_GRID_STRIDE(t, n)
     {
-        const int64_t cv =
-            (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
-            (2 * z + (q & 1));
-        s += (float)a[cv * C + c];
+        const int c = (int)(t % C);
+        const int64_t v = t / C;
+        const int z = (int)(v % Zp);
+        const int64_t r = v / Zp;
+        const int y = (int)(r % Yp), x = (int)(r / Yp);
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            const int64_t cv =
+                (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
+                (2 * z + (q & 1));
+            s += (float)a[cv * C + c];
+        }
+        p[t] = (bf16)(s * 0.125f);
     }
-    p[t] = (bf16)(s * 0.125f);
 }
 
 // g_child += g_parent / 8
@@ -421,14 +428,14 @@ __global__ void __launch_bounds__(kBlock)
     pool_bwd_acc(bf16* __restrict__ ga, const bf16* __restrict__ gp, int C,
                  int X, int Y, int Z)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)X * Y * Z * C;
-    if (t >= n)
-        return;
-    const int c = (int)(t % C);
-    const int64_t v = t / C;
-    const int64_t pv = parent_of(v, X, Y, Z);
-    ga[t] = (bf16)((float)ga[t] + 0.125f * (float)gp[pv * C + c]);
+    VC_GRID_STRIDE(t, n)
+    {
+        const int c = (int)(t % C);
+        const int64_t v = t / C;
+        const int64_t pv = parent_of(v, X, Y, Z);
+        ga[t] = (bf16)((float)ga[t] + 0.125f * (float)gp[pv * C + c]);
+    }
 }
 
 // cat[v] = [up[parent(v)] (C1 channels), skip[v] (C2)] at child dims
@@ -437,14 +444,14 @@ __global__ void __launch_bounds__(kBlock)
                int C2, bf16* __restrict__ cat, int X, int Y, int Z)
 {
     const int C = C1 + C2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)X * Y * Z * C;
-    if (t >= n)
-        return;
-    const int c = (int)(t % C);
-    const int64_t v = t / C;
-    cat[t] = c < C1 ? up[parent_of(v, X, Y, Z) * C1 + c]
-                    : sk[v * C2 + (c - C1)];
+    VC_GRID_STRIDE(t, n)
+    {
+        const int c = (int)(t % C);
+        const int64_t v = t / C;
+        cat[t] = c < C1 ? up[parent_of(v, X, Y, Z) * C1 + c]
+                        : sk[v * C2 + (c - C1)];
+    }
 }
 
 // g_up[parent][c] = sum over the 8 children of g_cat[child][c] (c < C1)
@@ -453,25 +460,25 @@ __global__ void __launch_bounds__(kBlock)
              bf16* __restrict__ gup, int X, int Y, int Z)
 {
     const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)Xp * Yp * Zp * C1;
-    if (t >= n)
-        return;
-    const int c = (int)(t % C1);
-    const int64_t v = t / C1;
-    const int z = (int)(v % Zp);
-    const int64_t r = v / Zp;
-    const int y = (int)(r % Yp), x = (int)(r / Yp);
-    float s = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
+    VC_GRID_STRIDE(t, n)
     {
-        const int64_t cv =
-            (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
-            (2 * z + (q & 1));
-        s += (float)gcat[cv * C + c];
+        const int c = (int)(t % C1);
+        const int64_t v = t / C1;
+        const int z = (int)(v % Zp);
+        const int64_t r = v / Zp;
+        const int y = (int)(r % Yp), x = (int)(r / Yp);
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            const int64_t cv =
+                (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
+                (2 * z + (q & 1));
+            s += (float)gcat[cv * C + c];
+        }
+        gup[t] = (bf16)s;
     }
-    gup[t] = (bf16)s;
 }
 
 // g_skip[v][c] = g_cat[v][C1 + c]
@@ -480,12 +487,12 @@ __global__ void __launch_bounds__(kBlock)
                bf16* __restrict__ gsk, int64_t V)
 {
     const int C2 = C - C1;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= V * C2)
-        return;
-    const int c = (int)(t % C2);
-    const int64_t v = t / C2;
-    gsk[t] = gcat[v * C + C1 + c];
+    VC_GRID_STRIDE(t, V * C2)
+    {
+        const int c = (int)(t % C2);
+        const int64_t v = t / C2;
+        gsk[t] = gcat[v * C + C1 + c];
+    }
 }
 
 // network input over a region of dims (X, Y, Z) whose voxel (0,0,0) is
@@ -697,17 +704,15 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     fill_f32(float* __restrict__ p, int64_t n, float v)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n)
-        p[t] = v;
+    VC_GRID_STRIDE(t, n)
+    p[t] = v;
 }
 
 __global__ void __launch_bounds__(kBlock)
     bf16_to_f32(const bf16* __restrict__ a, float* __restrict__ b, int64_t n)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n)
-        b[t] = (float)a[t];
+    VC_GRID_STRIDE(t, n)
+    b[t] = (float)a[t];
 }
 
 // core of a region's head output -> volume outputs (lambda, relative error)
@@ -775,6 +780,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// grid of a VC_GRID_STRIDE kernel: one thread per element up to kMaxGrid
+// workgroups (2^28 threads), fewer trips of the loop otherwise
+constexpr int64_t kMaxGrid = int64_t(1) << 20;
+unsigned grid_for(int64_t n)
+{
+    const int64_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)(b < kMaxGrid ? (b > 0 ? b : 1) : kMaxGrid);
+}
 
 } // namespace
 
@@ -1064,7 +1077,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s)
     {
         dims(lv, X, Y, Z);
         const int64_t n = level_voxels(s, lv + 1) * pc[lv];
-        pool_fwd<<<blocks_for(n), kBlock, 0, S>>>(s->act[2 * lv + 1], pools[lv],
+        pool_fwd<<<grid_for(n), kBlock, 0, S>>>(s->act[2 * lv + 1], pools[lv],
                                                    pc[lv], X, Y, Z);
         VC_CHECK_LAUNCH(ctx);
         if ((rc = conv(2 * lv + 2, pools[lv])) != VC_OK ||
@@ -1081,7 +1094,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s)
         const int lv = 2 - d;
         dims(lv, X, Y, Z);
         const int64_t n = level_voxels(s, lv) * (c1s[d] + c2s[d]);
-        concat_fwd<<<blocks_for(n), kBlock, 0, S>>>(ups[d], c1s[d], skips[d],
+        concat_fwd<<<grid_for(n), kBlock, 0, S>>>(ups[d], c1s[d], skips[d],
                                                      c2s[d], cats[d], X, Y, Z);
         VC_CHECK_LAUNCH(ctx);
         if ((rc = conv(8 + 2 * d, cats[d])) != VC_OK ||
@@ -1105,7 +1118,7 @@ int layer_back(vc_ctx* ctx, vc_surrogate* s, int l, bf16* gy, const bf16* x,
     const int X = s->X >> L.level, Y = s->Y >> L.level, Z = s->Z >> L.level;
     const int64_t V = (int64_t)X * Y * Z;
     hipStream_t S = ctx->stream;
-    relu_mask<<<blocks_for(V * L.cout), kBlock, 0, S>>>(gy, s->act[l],
+    relu_mask<<<grid_for(V * L.cout), kBlock, 0, S>>>(gy, s->act[l],
                                                          V * L.cout);
     VC_CHECK_LAUNCH(ctx);
     int rc = layer_wgrad(ctx, s, l, x, gy);
@@ -1203,11 +1216,11 @@ int backward(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
             return rc;
         dims(d.lv, X, Y, Z);
         const int64_t nu = level_voxels(s, d.lv + 1) * d.C1;
-        split_up<<<blocks_for(nu), kBlock, 0, S>>>(d.gcat, d.C, d.C1, d.gup, X,
+        split_up<<<grid_for(nu), kBlock, 0, S>>>(d.gcat, d.C, d.C1, d.gup, X,
                                                     Y, Z);
         VC_CHECK_LAUNCH(ctx);
         const int64_t Vl = level_voxels(s, d.lv);
-        split_skip<<<blocks_for(Vl * (d.C - d.C1)), kBlock, 0, S>>>(
+        split_skip<<<grid_for(Vl * (d.C - d.C1)), kBlock, 0, S>>>(
             d.gcat, d.C, d.C1, d.gskip, Vl);
         VC_CHECK_LAUNCH(ctx);
     }
@@ -1227,7 +1240,7 @@ int backward(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
             return rc;
         dims(lv - 1, X, Y, Z);
         const int64_t n = level_voxels(s, lv - 1) * pc[lv - 1];
-        pool_bwd_acc<<<blocks_for(n), kBlock, 0, S>>>(
+        pool_bwd_acc<<<grid_for(n), kBlock, 0, S>>>(
             s->gact[2 * lv - 1], gpools[lv - 1], pc[lv - 1], X, Y, Z);
         VC_CHECK_LAUNCH(ctx);
     }
@@ -1298,8 +1311,15 @@ int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
     const int dims[3] = {v->nx, v->ny, v->nz};
     int amax = dims[0] > dims[1] ? dims[0] : dims[1];
     amax = amax > dims[2] ? amax : dims[2];
-    const bool whole = tile_core <= 0 || tile_core >= amax;
-    const int T = whole ? 0 : round8(tile_core);
+    // a conv launch covers its region with 2 threads per voxel, counted in
+    // 32 bits: a whole volume beyond that runs as tiles of 512 (tiled and
+    // whole-volume passes give the same outputs bit for bit)
+    const int64_t whole_vox =
+        (int64_t)round8(dims[0]) * round8(dims[1]) * round8(dims[2]);
+    const bool fits = 2 * whole_vox < (int64_t(1) << 32);
+    const bool whole = (tile_core <= 0 || tile_core >= amax) && fits;
+    const int T = whole ? 0
+                        : round8(tile_core > 0 && tile_core < amax ? tile_core : 512);
     int R[3];
     for (int a = 0; a < 3; ++a)
         R[a] = whole ? round8(dims[a]) : T + 2 * kHalo;
@@ -1522,7 +1542,7 @@ int vc_surrogate_gradients(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
         return rc;
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
-    fill_f32<<<blocks_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
+    fill_f32<<<grid_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
     VC_CHECK_LAUNCH(ctx);
     if ((rc = train_volume(ctx, s, vol, s->lossbuf, s->ctr)) != VC_OK)
         return rc;
@@ -1552,7 +1572,7 @@ int vc_surrogate_train_step(vc_ctx* ctx, vc_surrogate* s, vc_vol* const* vols,
     }
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
-    fill_f32<<<blocks_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
+    fill_f32<<<grid_for(s->np), kBlock, 0, S>>>(s->grad, s->np, 0.0f);
     VC_CHECK_LAUNCH(ctx);
     for (int i = 0; i < n; ++i)
     {
@@ -1711,7 +1731,7 @@ int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
         int rc = vc_alloc(ctx, (void**)&tmp, 4 * (size_t)n);
         if (rc != VC_OK)
             return rc;
-        bf16_to_f32<<<blocks_for(n), kBlock, 0, S>>>(s->act[which], tmp, n);
+        bf16_to_f32<<<grid_for(n), kBlock, 0, S>>>(s->act[which], tmp, n);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(host, tmp, 4 * (size_t)n, hipMemcpyDeviceToHost,
