@@ -1,7 +1,12 @@
-"""CPU: the voroUtility front end, host-side mode (topo: closed pockets of a
-mesh via the cell-complex collapse, src/highlevelalgo.cpp:993-1020)."""
+"""GPU: the voroUtility front end's topo mode (closed pockets of a mesh via
+the cell-complex collapse, src/highlevelalgo.cpp:993-1020), which finalizes,
+collapses and counts components on the device."""
 import os
 import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "voxel_ma_amd", "voroUtility")

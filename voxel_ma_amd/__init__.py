@@ -37,6 +37,11 @@ EXPORTS = [
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
     "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
     "vc_cc_prune", "vc_cc_prune_device", "vc_cc_finalize_device",
+    # include/vc_cc.h
+    "vc_cc_create", "vc_cc_release", "vc_cc_sizes", "vc_cc_get", "vc_cc_topology",
+    "vc_cc_subset_components", "vc_ccthin_create", "vc_ccthin_release", "vc_ccthin_assign",
+    "vc_ccthin_prune", "vc_ccthin_removed", "vc_ccthin_remaining", "vc_ccthin_remained",
+    "vc_ccthin_remaining_euler", "vc_cc_closed_components", "vc_euler_char",
     # include/vc_batch.h
     "vc_batch_create", "vc_batch_destroy", "vc_batch_size", "vc_batch_edt", "vc_batch_measure",
     "vc_batch_thin", "vc_batch_run", "vc_schedule_defaults", "vc_batch_get_schedule",
@@ -102,6 +107,14 @@ class SurrogateStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class CcTopo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("V", "E", "F", "T", "C", "euler",
+                                              "face_components")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 class OctreeInfo(ctypes.Structure):
     _fields_ = [("resolution", ctypes.c_int32), ("n_internal", ctypes.c_int64),
                 ("n_empty", ctypes.c_int64), ("n_leaf", ctypes.c_int64),
@@ -157,6 +170,25 @@ def library():
         "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
         "vc_cc_prune_device": [P, ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float,
                                P, P, P, P, ctypes.POINTER(ctypes.c_int64)],
+        "vc_cc_create": [P, I, I, P, I, P, P, PP],
+        "vc_cc_release": [P, P],
+        "vc_cc_sizes": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
+                        ctypes.POINTER(I64)],
+        "vc_cc_get": [P, P, P, P],
+        "vc_cc_topology": [P, P, ctypes.POINTER(CcTopo)],
+        "vc_cc_subset_components": [P, P, P, ctypes.POINTER(I64)],
+        "vc_ccthin_create": [P, P, PP],
+        "vc_ccthin_release": [P, P],
+        "vc_ccthin_assign": [P, P, P, P],
+        "vc_ccthin_prune": [P, P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(I64)],
+        "vc_ccthin_removed": [P, P, P, P, P, P],
+        "vc_ccthin_remaining": [P, P, P, I64, ctypes.POINTER(I64), P, I64,
+                                ctypes.POINTER(I64), P, P, I64, ctypes.POINTER(I64)],
+        "vc_ccthin_remained": [P, P, P, I64, ctypes.POINTER(I64), P, I64, ctypes.POINTER(I64),
+                               P, I64, ctypes.POINTER(I64)],
+        "vc_ccthin_remaining_euler": [P, P, ctypes.POINTER(CcTopo)],
+        "vc_cc_closed_components": [P, P, ctypes.POINTER(I64)],
+        "vc_euler_char": [P, I, P, I64, P, I64, ctypes.POINTER(CcTopo)],
         "vc_batch_create": [P, P, I, PP],
         "vc_batch_destroy": [P, P],
         "vc_batch_size": [P, ctypes.POINTER(I), ctypes.POINTER(I64)],
@@ -688,6 +720,168 @@ def cc_prune_device(ctx, nv, edges, foff, fe, em, fm, f_t, l_t):
                                       float(f_t), float(l_t), _p(rv), _p(re), _p(rf),
                                       _p(ref), ctypes.byref(gens))
     return rc, rv[:nv], re[:ne], rf[:nf], ref[:ne], gens.value
+
+
+class CellComplex:
+    """A 2-complex finalized on the device and kept there (include/vc_cc.h):
+    cellcomplex(vts, edges, faces) + finalize, its topology checks, and the
+    collapse states over it (CellComplexThin)."""
+
+    def __init__(self, ctx, nv, edges, faces):
+        self.ctx = ctx
+        self.nv = int(nv)
+        e_in = np.ascontiguousarray(np.array(edges, np.int32).reshape(-1, 2))
+        foff, fv = _faces_csr(faces)
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.vc_cc_create(ctx.handle, self.nv, len(e_in), _p(e_in),
+                                        len(foff) - 1, _p(foff), _p(fv), ctypes.byref(h)))
+        self.handle = h
+        self.face_off = foff
+        self.face_vts = fv[:foff[-1]]
+
+    def sizes(self):
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        self.ctx.check(self.ctx._lib.vc_cc_sizes(self.handle, ctypes.byref(a), ctypes.byref(b),
+                                                 ctypes.byref(c), ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def finalized(self):
+        """(edges (ne, 2), face side edge ids)"""
+        _, ne, _, ns = self.sizes()
+        e = np.zeros((max(ne, 1), 2), np.int32)
+        fe = np.zeros(max(ns, 1), np.int32)
+        self.ctx.check(self.ctx._lib.vc_cc_get(self.ctx.handle, self.handle, _p(e), _p(fe)))
+        return e[:ne], fe[:ns]
+
+    def topology(self):
+        t = CcTopo()
+        self.ctx.check(self.ctx._lib.vc_cc_topology(self.ctx.handle, self.handle,
+                                                    ctypes.byref(t)))
+        return t.as_dict()
+
+    def subset_components(self, face_in):
+        f = np.ascontiguousarray(np.asarray(face_in, np.uint8))
+        n = ctypes.c_int64()
+        self.ctx.check(self.ctx._lib.vc_cc_subset_components(self.ctx.handle, self.handle,
+                                                             _p(f), ctypes.byref(n)))
+        return n.value
+
+    def closed_components(self):
+        n = ctypes.c_int64()
+        self.ctx.check(self.ctx._lib.vc_cc_closed_components(self.ctx.handle, self.handle,
+                                                             ctypes.byref(n)))
+        return n.value
+
+    def thinning(self):
+        return CellComplexThin(self)
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_cc_release(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class CellComplexThin:
+    """CellComplexThinning over a CellComplex on the device (vc_ccthin)."""
+
+    def __init__(self, cc):
+        self.cc = cc
+        self.ctx = cc.ctx
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._lib.vc_ccthin_create(self.ctx.handle, cc.handle,
+                                                      ctypes.byref(h)))
+        self.handle = h
+
+    def assign(self, em, fm):
+        em = np.ascontiguousarray(em, np.float32)
+        fm = np.ascontiguousarray(fm, np.float32)
+        self.ctx.check(self.ctx._lib.vc_ccthin_assign(self.ctx.handle, self.handle, _p(em),
+                                                      _p(fm)))
+
+    def prune(self, f_t, l_t):
+        """Returns (status, queue generations); status VC_ERR_INVALID for a
+        degenerate complex."""
+        g = ctypes.c_int64()
+        rc = self.ctx._lib.vc_ccthin_prune(self.ctx.handle, self.handle, float(f_t),
+                                           float(l_t), ctypes.byref(g))
+        if rc not in (VC_OK, VC_ERR_INVALID, VC_ERR_STATE):
+            self.ctx.check(rc)
+        return rc, g.value
+
+    def removed(self):
+        nv, ne, nf, _ = self.cc.sizes()
+        rv, re, rf = (np.zeros(max(nv, 1), np.uint8), np.zeros(max(ne, 1), np.uint8),
+                      np.zeros(max(nf, 1), np.uint8))
+        ref = np.zeros(max(ne, 1), np.int32)
+        self.ctx.check(self.ctx._lib.vc_ccthin_removed(self.ctx.handle, self.handle, _p(rv),
+                                                       _p(re), _p(rf), _p(ref)))
+        return rv[:nv], re[:ne], rf[:nf], ref[:ne]
+
+    def remaining(self):
+        """(kept vertex ids, edges (n, 2), triangles (n, 3), from_fi), sized by
+        a count query first"""
+        L = self.ctx._lib
+        n = [ctypes.c_int64() for _ in range(3)]
+        self.ctx.check(L.vc_ccthin_remaining(self.ctx.handle, self.handle, None, 0,
+                                             ctypes.byref(n[0]), None, 0, ctypes.byref(n[1]),
+                                             None, None, 0, ctypes.byref(n[2])))
+        nv, ne, nt = (x.value for x in n)
+        vi = np.zeros(max(nv, 1), np.int32)
+        e = np.zeros((max(ne, 1), 2), np.int32)
+        t = np.zeros((max(nt, 1), 3), np.int32)
+        ff = np.zeros(max(nt, 1), np.int32)
+        self.ctx.check(L.vc_ccthin_remaining(self.ctx.handle, self.handle, _p(vi), nv,
+                                             ctypes.byref(n[0]), _p(e), ne, ctypes.byref(n[1]),
+                                             _p(t), _p(ff), nt, ctypes.byref(n[2])))
+        return vi[:nv], e[:ne], t[:nt], ff[:nt]
+
+    def remained(self):
+        """(getRemainedVts, getRemainedEdges, getRemainedFaces)"""
+        L = self.ctx._lib
+        n = [ctypes.c_int64() for _ in range(3)]
+        self.ctx.check(L.vc_ccthin_remained(self.ctx.handle, self.handle, None, 0,
+                                            ctypes.byref(n[0]), None, 0, ctypes.byref(n[1]),
+                                            None, 0, ctypes.byref(n[2])))
+        a = [np.zeros(max(x.value, 1), np.int32) for x in n]
+        c = [x.value for x in n]
+        self.ctx.check(L.vc_ccthin_remained(self.ctx.handle, self.handle, _p(a[0]), c[0],
+                                            ctypes.byref(n[0]), _p(a[1]), c[1],
+                                            ctypes.byref(n[1]), _p(a[2]), c[2],
+                                            ctypes.byref(n[2])))
+        return tuple(x[:k] for x, k in zip(a, c))
+
+    def remaining_euler(self):
+        t = CcTopo()
+        self.ctx.check(self.ctx._lib.vc_ccthin_remaining_euler(self.ctx.handle, self.handle,
+                                                               ctypes.byref(t)))
+        return t.as_dict()
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_ccthin_release(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def euler_char(ctx, nv, edges, tris):
+    """util::computeEulerChar(vts, edges, tris) on the device (vc_euler_char)."""
+    e = np.ascontiguousarray(np.asarray(edges, np.int32).reshape(-1, 2))
+    t = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+    out = CcTopo()
+    ctx.check(ctx._lib.vc_euler_char(ctx.handle, int(nv), _p(e), len(e), _p(t), len(t),
+                                     ctypes.byref(out)))
+    return out.as_dict()
 
 
 # ---------------------------------------------------------------- Voronoi side

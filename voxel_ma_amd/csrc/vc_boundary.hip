@@ -1301,6 +1301,21 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
     const Dims D{v->nx, v->ny, v->nz};
     const int64_t N = v->N;
     const int64_t ntiles = (N + kBlock - 1) / kBlock;
+    // site and face counts are scanned in 32 bits and quads index sites in
+    // int32: a volume has at most 3N + 2(nx ny + ny nz + nx nz) boundary faces
+    // and (nx+1)(ny+1)(nz+1) corners
+    {
+        const int64_t faces_max =
+            3 * N + 2 * ((int64_t)v->nx * v->ny + (int64_t)v->ny * v->nz +
+                         (int64_t)v->nx * v->nz);
+        const int64_t corners =
+            (int64_t)(v->nx + 1) * (v->ny + 1) * (v->nz + 1);
+        if (faces_max > (int64_t)UINT32_MAX || corners > (int64_t)INT32_MAX)
+            return vc_fail(ctx, VC_ERR_INVALID,
+                           "boundary: %d x %d x %d may have more faces than 32-bit "
+                           "counts or more corners than int32 site indices hold",
+                           v->nx, v->ny, v->nz);
+    }
     // site / face tiles: 256 voxels, or 256 chunks of 8 z when nz allows
     const bool vec = (v->nz % kVz) == 0;
     const int64_t etiles =
@@ -1407,8 +1422,10 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 rc = vc_alloc(ctx, (void**)&acc, 8 * kSlots * kAcc);
                 if (rc == VC_OK)
                     rc = vc_alloc(ctx, (void**)&dtbl, sizeof(htbl));
+                // boundary marks: one byte per 8-voxel chunk on the
+                // vectorised path, one per voxel otherwise
                 if (rc == VC_OK)
-                    rc = vc_alloc(ctx, (void**)&bnd, (size_t)N);
+                    rc = vc_alloc(ctx, (void**)&bnd, (size_t)(vec ? N / kVz : N));
                 if (rc == VC_OK)
                     rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)N);
             }

@@ -14,6 +14,8 @@
 #include "../../include/vc_batch.h"
 
 // ---------------------------------------------------------------- context
+struct vc_sched;
+
 struct vc_ctx
 {
     int device = 0;
@@ -228,6 +230,14 @@ int vc_ft_to_linear(vc_ctx* ctx, vc_vol* vol, int32_t* d_dst);
 // exclusive scan of n uint32 in device memory, in place, on the context
 // stream (waits)
 int vc_scan_u32(vc_ctx* ctx, uint32_t* a, int64_t n);
+// cellcomplex::finalize: host-side validation of array inputs, and the
+// device computation on device arrays (vc_voro.hip)
+int vc_cc_check_host(vc_ctx* ctx, int32_t nv, int32_t ne_in,
+                     const int32_t* edges_in, int32_t nf,
+                     const int64_t* face_off, const int32_t* face_vts);
+int vc_cc_finalize_dev(vc_ctx* ctx, int32_t ne_in, const int32_t* dE, int32_t nf,
+                       const int64_t* dfo, const int32_t* dfv, int64_t nfv,
+                       int32_t* deo, int32_t* dfe, int32_t* ne_out);
 
 // host helpers
 int64_t vc_level_threshold(float level);

@@ -600,6 +600,80 @@ __global__ void __launch_bounds__(kBlock) fill_u64(uint64_t* a, int64_t n, uint6
 
 } // namespace
 
+// host-side validation of a complex given as arrays (ids in range,
+// monotone face offsets starting at 0, total size within int32)
+int vc_cc_check_host(vc_ctx* ctx, int32_t nv, int32_t ne_in,
+                     const int32_t* edges_in, int32_t nf,
+                     const int64_t* face_off, const int32_t* face_vts)
+{
+    const int64_t nfv = nf ? face_off[nf] : 0;
+    if ((nf && face_off[0] != 0) || nfv < 0 || (int64_t)ne_in + nfv >= (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID, "complex: bad face offsets or too large");
+    for (int32_t f = 0; f < nf; ++f)
+        if (face_off[f + 1] < face_off[f])
+            return vc_fail(ctx, VC_ERR_INVALID, "complex: face offsets decrease");
+    for (int64_t i = 0; i < 2LL * ne_in; ++i)
+        if (edges_in[i] < 0 || edges_in[i] >= nv)
+            return vc_fail(ctx, VC_ERR_INVALID, "complex: edge end out of range");
+    for (int64_t j = 0; j < nfv; ++j)
+        if (face_vts[j] < 0 || face_vts[j] >= nv)
+            return vc_fail(ctx, VC_ERR_INVALID, "complex: face vertex out of range");
+    return VC_OK;
+}
+
+// cellcomplex::finalize on device arrays: deo receives the edges (capacity
+// 2 * (ne_in + nfv) ints), dfe each face side's edge id (nfv ints); *ne_out
+// the edge count.  Waits for the count.
+int vc_cc_finalize_dev(vc_ctx* ctx, int32_t ne_in, const int32_t* dE, int32_t nf,
+                       const int64_t* dfo, const int32_t* dfv, int64_t nfv,
+                       int32_t* deo, int32_t* dfe, int32_t* ne_out)
+{
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    const int64_t NC = (int64_t)ne_in + nfv;
+    uint64_t H = 1024;
+    while (H < 2 * (uint64_t)NC + 1)
+        H <<= 1;
+    uint64_t *keys, *ckey;
+    unsigned long long* vals;
+    uint32_t* pos;
+    int rc = VC_OK;
+    auto get = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = sc.get(ctx, p, n);
+    };
+    get(&keys, (size_t)H);
+    get(&vals, (size_t)H);
+    get(&ckey, (size_t)NC + 1);
+    get(&pos, (size_t)NC + 1);
+    if (rc != VC_OK)
+        return rc;
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
+        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
+    VC_HIP(ctx, hipMemsetAsync(pos, 0, 4 * ((size_t)NC + 1), S));
+    const int64_t mx = std::max<int64_t>(ne_in, nf);
+    if (mx)
+        fz_keys<<<blocks_for(mx), kBlock, 0, S>>>(ne_in, dE, nf, dfo, dfv, ckey, keys,
+                                                  vals, H - 1);
+    if (NC)
+        fz_first<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, pos);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, pos, NC + 1)) != VC_OK)
+        return rc;
+    uint32_t ne = 0;
+    VC_HIP(ctx, hipMemcpyAsync(&ne, pos + NC, 4, hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    *ne_out = (int32_t)ne;
+    if (NC)
+        fz_emit<<<blocks_for(NC), kBlock, 0, S>>>(NC, ne_in, ckey, pos, keys, vals,
+                                                  H - 1, deo, dfe);
+    VC_CHECK_LAUNCH(ctx);
+    // the temporaries are freed on return: wait for the kernels reading them
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    return VC_OK;
+}
+
 extern "C" {
 
 int vc_voro_tag_points(vc_ctx* ctx, vc_vol* vol, const float* pts, int64_t n,
@@ -965,30 +1039,15 @@ int vc_cc_finalize_device(vc_ctx* ctx, int32_t nv, int32_t ne_in,
         (ne_in && !edges_in) || (nf && (!face_off || !face_vts || !face_edges_out)))
         return VC_ERR_INVALID;
     const int64_t nfv = nf ? face_off[nf] : 0;
-    if ((nf && face_off[0] != 0) || nfv < 0 || (int64_t)ne_in + nfv >= (int64_t)INT32_MAX)
-        return VC_ERR_INVALID;
-    for (int32_t f = 0; f < nf; ++f)
-        if (face_off[f + 1] < face_off[f])
-            return VC_ERR_INVALID;
-    for (int64_t i = 0; i < 2LL * ne_in; ++i)
-        if (edges_in[i] < 0 || edges_in[i] >= nv)
-            return VC_ERR_INVALID;
-    for (int64_t j = 0; j < nfv; ++j)
-        if (face_vts[j] < 0 || face_vts[j] >= nv)
-            return VC_ERR_INVALID;
+    int rc = vc_cc_check_host(ctx, nv, ne_in, edges_in, nf, face_off, face_vts);
+    if (rc != VC_OK)
+        return rc;
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
     Scratch sc;
     const int64_t NC = (int64_t)ne_in + nfv;
-    uint64_t H = 1024;
-    while (H < 2 * (uint64_t)NC + 1)
-        H <<= 1;
     int32_t *dE, *dfv, *deo, *dfe;
     int64_t* dfo;
-    uint64_t *keys, *ckey;
-    unsigned long long* vals;
-    uint32_t* pos;
-    int rc = VC_OK;
     auto get = [&](auto** p, size_t n) {
         if (rc == VC_OK)
             rc = sc.get(ctx, p, n);
@@ -998,10 +1057,6 @@ int vc_cc_finalize_device(vc_ctx* ctx, int32_t nv, int32_t ne_in,
     get(&dfv, (size_t)nfv + 1);
     get(&deo, 2 * (size_t)NC + 2);
     get(&dfe, (size_t)nfv + 1);
-    get(&keys, (size_t)H);
-    get(&vals, (size_t)H);
-    get(&ckey, (size_t)NC + 1);
-    get(&pos, (size_t)NC + 1);
     if (rc != VC_OK)
         return rc;
     if (ne_in)
@@ -1015,28 +1070,13 @@ int vc_cc_finalize_device(vc_ctx* ctx, int32_t nv, int32_t ne_in,
             VC_HIP(ctx, hipMemcpyAsync(dfv, face_vts, 4 * (size_t)nfv,
                                        hipMemcpyHostToDevice, S));
     }
-    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
-    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
-        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
-    VC_HIP(ctx, hipMemsetAsync(pos, 0, 4 * ((size_t)NC + 1), S));
-    const int64_t mx = std::max<int64_t>(ne_in, nf);
-    if (mx)
-        fz_keys<<<blocks_for(mx), kBlock, 0, S>>>(ne_in, dE, nf, dfo, dfv, ckey, keys,
-                                                  vals, H - 1);
-    if (NC)
-        fz_first<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, pos);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, pos, NC + 1)) != VC_OK)
+    int32_t ne = 0;
+    if ((rc = vc_cc_finalize_dev(ctx, ne_in, dE, nf, dfo, dfv, nfv, deo, dfe, &ne)) !=
+        VC_OK)
         return rc;
-    uint32_t ne = 0;
-    VC_HIP(ctx, hipMemcpy(&ne, pos + NC, 4, hipMemcpyDeviceToHost));
-    *ne_out = (int32_t)ne;
+    *ne_out = ne;
     if ((int64_t)ne > edge_cap || !edges_out)
         return edges_out ? VC_ERR_CAPACITY : VC_OK;
-    if (NC)
-        fz_emit<<<blocks_for(NC), kBlock, 0, S>>>(NC, ne_in, ckey, pos, keys, vals,
-                                                  H - 1, deo, dfe);
-    VC_CHECK_LAUNCH(ctx);
     if (ne)
         VC_HIP(ctx, hipMemcpyAsync(edges_out, deo, 8 * (size_t)ne,
                                    hipMemcpyDeviceToHost, S));

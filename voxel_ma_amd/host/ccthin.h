@@ -1,8 +1,13 @@
 // ccthin.h -- CellComplexThinning, the reference's measure-thresholded
 // simple-pair collapse of a 2-complex (include/ccthin.h:13-167).  prune()
-// runs through the C ABI (vc_cc_prune) in the reference's FIFO order.
+// runs on the device over the complex cellcomplex::finalize left there
+// (vc_ccthin_prune, include/vc_cc.h), in the reference's FIFO order, and so
+// do remainingCC and getRemained*; a complex the device collapse refuses (an
+// edge with equal ends, a face listing a side twice) is collapsed on the
+// host in the same order (vc_cc_prune).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "cellcomplex.h"
@@ -17,6 +22,9 @@ public:
         VERTEX = 2
     };
     CellComplexThinning() { reset(); }
+    ~CellComplexThinning() { release(); }
+    CellComplexThinning(const CellComplexThinning&) = delete;
+    CellComplexThinning& operator=(const CellComplexThinning&) = delete;
     // non-owning, as in the reference (the caller keeps the complex alive)
     void setup(cellcomplex* cc) { m_cc = cc; }
     void reset();
@@ -44,10 +52,18 @@ public:
                      std::vector<int>* from_fi = nullptr) const;
 
 private:
+    void release();
+    void prune_host(float f_t, float l_t);
+
     cellcomplex* m_cc = nullptr;
     std::vector<float> m_measure[3];
     std::vector<bool> m_removed[3];
     std::vector<int> m_ref_edge;
+    // the collapse state on the device, over the complex it was made for
+    std::shared_ptr<vc_cc> m_cc_dev;
+    vc_ccthin* m_dev = nullptr;
+    bool m_measures_sent = false;
+    bool m_host = false; // the last prune ran on the host
     float m_face_degen_thresh = 0.0f;
     int m_cmpnt_num_faces_thresh = 0;
 };

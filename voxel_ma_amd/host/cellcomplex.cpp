@@ -21,7 +21,7 @@ void cellcomplex::appendFace(const std::vector<int>& f)
 {
     m_fv.insert(m_fv.end(), f.begin(), f.end());
     m_face_off.push_back((int64_t)m_fv.size());
-    m_finalized = false;
+    invalidate();
 }
 
 void cellcomplex::appendFace(const uTriFace& t)
@@ -44,20 +44,25 @@ void cellcomplex::finalize()
 {
     if (m_finalized)
         return;
+    vc_ctx* ctx = vcdev::context();
     const int32_t nv = (int32_t)m_vts.size(), nf = (int32_t)numFaces();
-    std::vector<int32_t> ein = edgeArray();
-    std::vector<int32_t> eout(ein.size() + 2 * m_fv.size());
-    std::vector<int32_t> fe(m_fv.size());
+    const std::vector<int32_t> ein = edgeArray();
+    vc_cc* h = nullptr;
+    vcdev::check(vc_cc_create(ctx, nv, (int32_t)m_edges.size(), ein.data(), nf,
+                              m_face_off.data(), m_fv.data(), &h),
+                 "vc_cc_create");
+    m_dev = std::shared_ptr<vc_cc>(h, [ctx](vc_cc* p) { vc_cc_release(ctx, p); });
     int32_t ne = 0;
-    vcdev::check(vc_cc_finalize(nv, (int32_t)m_edges.size(), ein.data(), nf,
-                                m_face_off.data(), m_fv.data(), eout.data(),
-                                (int32_t)(eout.size() / 2), &ne, fe.data()),
-                 "vc_cc_finalize");
+    int64_t ns = 0;
+    vcdev::check(vc_cc_sizes(h, nullptr, &ne, nullptr, &ns), "vc_cc_sizes");
+    std::vector<int32_t> eout(2 * (size_t)ne + 2);
+    m_fe.assign((size_t)ns, 0);
+    vcdev::check(vc_cc_get(ctx, h, eout.data(), m_fe.data()), "vc_cc_get");
     m_edges.resize(ne);
     for (int32_t e = 0; e < ne; ++e)
         m_edges[e] = ivec2(eout[2 * e], eout[2 * e + 1]);
-    m_fe.swap(fe);
-    // adjacency: vertex -> edges in edge order, edge -> faces in face order
+    // host adjacency for the accessors: vertex -> edges in edge order,
+    // edge -> faces in face order (the device holds the same lists)
     m_ve_off.assign((size_t)nv + 1, 0);
     m_ef_off.assign((size_t)ne + 1, 0);
     for (const auto& e : m_edges)
@@ -114,45 +119,62 @@ std::vector<int> cellcomplex::refCntPerEdge() const
 
 void cellcomplex::eulerChar(eulerchar& ec) const
 {
-    const int C = util::findNumConnComponents(m_vts, m_edges);
+    vc_ctx* ctx = vcdev::context();
+    vc_cc_topo t{};
+    if (m_finalized && m_dev)
+        vcdev::check(vc_cc_topology(ctx, m_dev.get(), &t), "vc_cc_topology");
+    else
+    {
+        // not finalized: the edges as given (the reference's const
+        // overload, src/cellcomplex.cpp:493-502)
+        const std::vector<int32_t> e = edgeArray();
+        vcdev::check(vc_euler_char(ctx, (int32_t)m_vts.size(), e.data(),
+                                   (int64_t)m_edges.size(), nullptr, 0, &t),
+                     "vc_euler_char");
+    }
     ec.V = (int)numVts();
     ec.E = (int)numEdges();
     ec.F = (int)numFaces();
     ec.T = 0;
     ec.euler = ec.V - ec.E + ec.F;
-    ec.C = C;
+    ec.C = (int)t.C;
 }
 
 int cellcomplex::compute_conn_cmpnts() const
 {
-    const size_t nf = numFaces();
-    std::vector<bool> fvis(nf, false), evis(numEdges(), false);
-    std::vector<int> stack, fe;
-    int ncc = 0;
-    for (size_t s = 0; s < nf; ++s)
+    if (!m_finalized || !m_dev)
     {
-        if (fvis[s])
-            continue;
-        ++ncc;
-        stack.assign(1, (int)s);
-        while (!stack.empty())
-        {
-            const int f = stack.back();
-            stack.pop_back();
-            if (fvis[f])
-                continue;
-            fvis[f] = true;
-            getFaceERep(f, fe);
-            for (int e : fe)
-            {
-                if (evis[e])
-                    continue;
-                evis[e] = true;
-                for (int k = 0; k < cntNbFacesofEdge(e); ++k)
-                    if (!fvis[nbFaceofEdge(e, k)])
-                        stack.push_back(nbFaceofEdge(e, k));
-            }
-        }
+        cellcomplex c = *this;
+        c.finalize();
+        return c.compute_conn_cmpnts();
     }
-    return ncc;
+    vc_cc_topo t{};
+    vcdev::check(vc_cc_topology(vcdev::context(), m_dev.get(), &t), "vc_cc_topology");
+    return (int)t.face_components;
+}
+
+int cellcomplex::compute_conn_cmpnts(const std::vector<int>* subset, int) const
+{
+    if (!m_finalized || !m_dev)
+    {
+        cellcomplex c = *this;
+        c.finalize();
+        return c.compute_conn_cmpnts(subset, 1);
+    }
+    if (!subset)
+    {
+        // every vertex, joined through the edges (src/cellcomplex.cpp:203-213)
+        vc_cc_topo t{};
+        vcdev::check(vc_cc_topology(vcdev::context(), m_dev.get(), &t),
+                     "vc_cc_topology");
+        return (int)t.C;
+    }
+    std::vector<uint8_t> in(numFaces(), 0);
+    for (int f : *subset)
+        if (f >= 0 && (size_t)f < in.size())
+            in[f] = 1;
+    int64_t n = 0;
+    vcdev::check(vc_cc_subset_components(vcdev::context(), m_dev.get(), in.data(), &n),
+                 "vc_cc_subset_components");
+    return (int)n;
 }

@@ -1,5 +1,8 @@
 #include "geomalgo.h"
 
+#include "../../include/vc_cc.h"
+#include "vc_device.h"
+
 #include <algorithm>
 #include <set>
 #include <unordered_map>
@@ -121,26 +124,22 @@ void computeEulerChar(const vector<uTriFace>& tris, eulerchar& ec)
 void computeEulerChar(const vector<point>& vts, const vector<ivec2>& e_in,
                       const vector<uTriFace>& tris, eulerchar& ec)
 {
-    std::set<std::pair<int, int>> edges;
-    for (const auto& e : e_in)
-        edges.insert({e[0], e[1]});
-    for (const auto& f : tris)
-        for (int k = 0; k < 3; ++k)
-        {
-            const ivec2 e = makeEdge((int)f[k], (int)f[(k + 1) % 3]);
-            edges.insert({e[0], e[1]});
-        }
-    const int V = (int)vts.size();
-    vector<vector<int>> adj(V);
-    for (const auto& e : edges)
+    // on the device (vc_euler_char): V = |vts|, E = distinct pairs among the
+    // edges as given and the normalised triangle sides, C by union-find
+    vector<int32_t> e(2 * e_in.size()), t(3 * tris.size());
+    for (size_t i = 0; i < e_in.size(); ++i)
     {
-        adj[e.first].push_back(e.second);
-        adj[e.second].push_back(e.first);
+        e[2 * i] = e_in[i][0];
+        e[2 * i + 1] = e_in[i][1];
     }
-    vector<bool> visited(V, false);
-    const int C = findNumConnComponents(adj, V, visited);
-    const int E = (int)edges.size(), F = (int)tris.size();
-    ec = eulerchar(V, E, F, 0, C, V - E + F);
+    for (size_t i = 0; i < tris.size(); ++i)
+        for (int k = 0; k < 3; ++k)
+            t[3 * i + k] = (int32_t)tris[i][k];
+    vc_cc_topo r{};
+    vcdev::check(vc_euler_char(vcdev::context(), (int32_t)vts.size(), e.data(),
+                               (int64_t)e_in.size(), t.data(), (int64_t)tris.size(), &r),
+                 "vc_euler_char");
+    ec = eulerchar((int)r.V, (int)r.E, (int)r.F, 0, (int)r.C, (int)r.euler);
 }
 
 } // namespace util

@@ -17,6 +17,15 @@ int nClosedComponents(const cellcomplex& cc_in)
 {
     cellcomplex cc = cc_in;
     cc.finalize();
+    int64_t n = 0;
+    const int rc = vc_cc_closed_components(vcdev::context(), cc.device().get(), &n);
+    if (rc != VC_ERR_INVALID)
+    {
+        vcdev::check(rc, "vc_cc_closed_components");
+        return (int)n;
+    }
+    // a complex the device collapse refuses (an edge with equal ends, a face
+    // listing a side twice): the same steps through the host collapse
     CellComplexThinning thin;
     thin.setup(&cc);
     thin.assignElementValues(std::vector<float>(cc.numVts(), 0.0f),

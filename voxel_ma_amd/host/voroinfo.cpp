@@ -488,4 +488,28 @@ void VoroInfo::mergeCloseVts(float eps, bool only_inside)
     recompute_flags();
 }
 
+void VoroInfo::computeEulerChar(eulerchar& ec, bool do_prune_first) const
+{
+    int V, E, F, C;
+    if (do_prune_first)
+    {
+        std::vector<int> vi, ei, fi;
+        getValidVts(vi);
+        getValidEdges(ei);
+        getValidFaces(fi);
+        V = (int)vi.size();
+        E = (int)ei.size();
+        F = (int)fi.size();
+        C = m_geom.compute_conn_cmpnts(&fi, 1);
+    }
+    else
+    {
+        V = (int)m_geom.numVts();
+        E = (int)m_geom.numEdges();
+        F = (int)m_geom.numFaces();
+        C = m_geom.compute_conn_cmpnts();
+    }
+    ec = eulerchar(V, E, F, 0, C, V - E + F);
+}
+
 } // namespace voxelvoro

@@ -95,6 +95,12 @@ public:
     ivec2 getSitesOfFace(int fi) const { return m_face_sites[fi]; }
     const std::vector<point>& getSitesPosition() const { return m_site_positions; }
 
+    // src/voroinfo.cpp:1386-1430: V, E, F of the valid part (do_prune_first)
+    // or of the whole diagram; C = components of the valid faces' vertices
+    // through their sides, or the face components of the whole complex
+    // (cellcomplex::compute_conn_cmpnts, on the device)
+    void computeEulerChar(eulerchar& ec, bool do_prune_first) const;
+
     // valid vertices, the valid triangles of every face's triangulation and
     // their sides, sorted (src/voroinfo.cpp:573-620)
     void getPartInside(std::vector<int>& vts_indices, std::vector<ivec2>& edges,
