@@ -219,6 +219,33 @@ int vc_cc_prune_device(vc_ctx* ctx, const vc_cc_view* cc,
                        uint8_t* removed_e, uint8_t* removed_f, int32_t* ref_e,
                        int64_t* generations);
 
+/* ---- one call that allocates nothing ------------------------------------ */
+/* vc_skeleton_noalloc has no counterpart in the reference.  It is for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid hipMalloc): vc_edt + vc_measure + vc_thin_levels of one volume that
+ * the caller holds in device memory, with the same results, and no hipMalloc,
+ * hipHostMalloc or hipFree in the call.  Every intermediate (squared EDT,
+ * feature transform, lambda^2, level bytes, thinning planes and control
+ * words) lives in a fixed device array of VC_NOALLOC_SCRATCH_BYTES that each
+ * GPU the library runs on holds until the process ends; vc_ctx_create takes
+ * its address.  Every call on one GPU shares that GPU's array (it keeps the
+ * last call's contents; the call initialises what it reads), so calls that
+ * overlap on one GPU -- from two host threads or two contexts of the same
+ * device -- race and must be serialised by the caller.
+ *   d_occ    N bytes of device memory, C order [x][y][z], non-zero = inside;
+ *   d_label  N bytes of device memory, written as vc_get(VC_BUF_LABEL);
+ *   levels   host array, as vc_thin_levels; stats may be NULL.
+ * Limits: N = nx*ny*nz <= VC_NOALLOC_MAX_VOXELS, nx <= 16384, ny <= 32768,
+ * and the layout (13 bytes per voxel + 26 bytes per 64-voxel z word + small
+ * control blocks) within VC_NOALLOC_SCRATCH_BYTES; beyond them VC_ERR_INVALID
+ * with the sizes in the message.  Enqueued on the context stream; returns
+ * when the thinning has converged (it waits for its convergence checks). */
+#define VC_NOALLOC_SCRATCH_BYTES (64LL << 20)
+#define VC_NOALLOC_MAX_VOXELS (1LL << 22)
+int vc_skeleton_noalloc(vc_ctx* ctx, const uint8_t* d_occ, int nx, int ny,
+                        int nz, const float* levels, int n_levels,
+                        uint8_t* d_label, vc_thin_stats* stats);
+
 /* ---- results ------------------------------------------------------------- */
 /* which: */
 #define VC_BUF_OCC 0    /* uint8  N */

@@ -36,7 +36,7 @@ EXPORTS = [
     "vc_volume_from_device", "vc_volume_release", "vc_volume_dims", "vc_edt",
     "vc_measure", "vc_set_measure_f32", "vc_thin_levels", "vc_boundary_extract",
     "vc_get", "vc_get_device", "vc_synth_blurred_noise", "vc_synth_solids", "vc_cc_finalize",
-    "vc_cc_prune", "vc_cc_prune_device", "vc_cc_finalize_device",
+    "vc_cc_prune", "vc_cc_prune_device", "vc_cc_finalize_device", "vc_skeleton_noalloc",
     # include/vc_cc.h
     "vc_cc_create", "vc_cc_release", "vc_cc_sizes", "vc_cc_get", "vc_cc_topology",
     "vc_cc_subset_components", "vc_ccthin_create", "vc_ccthin_release", "vc_ccthin_assign",
@@ -62,6 +62,8 @@ EXPORTS = [
     "vc_write_ply", "vc_write_radii", "vc_write_dotma", "vc_write_off", "vc_write_tetnodes",
     "vc_xform_apply",
 ]
+NOALLOC_SCRATCH_BYTES = 64 << 20
+NOALLOC_MAX_VOXELS = 1 << 22
 SURROGATE_PARAMS = 678786
 SURROGATE_HALO = 56
 
@@ -170,6 +172,7 @@ def library():
         "vc_cc_prune": [ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float, P, P, P, P],
         "vc_cc_prune_device": [P, ctypes.POINTER(CcView), P, P, ctypes.c_float, ctypes.c_float,
                                P, P, P, P, ctypes.POINTER(ctypes.c_int64)],
+        "vc_skeleton_noalloc": [P, P, I, I, I, P, I, P, ctypes.POINTER(ThinStats)],
         "vc_cc_create": [P, I, I, P, I, P, P, PP],
         "vc_cc_release": [P, P],
         "vc_cc_sizes": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),

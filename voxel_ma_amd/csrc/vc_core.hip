@@ -115,6 +115,8 @@ int vc_ctx_create(int device, vc_ctx** out)
         rc = VC_ERR_HIP;
     if (rc == VC_OK)
         rc = vc_lut_build(ctx);
+    if (rc == VC_OK)
+        rc = vc_noalloc_setup(ctx);
     if (rc == VC_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = VC_ERR_HIP;
     if (rc != VC_OK)
@@ -134,6 +136,7 @@ int vc_ctx_destroy(vc_ctx* ctx)
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     vc_free(ctx->d_lut);
+    vc_noalloc_teardown(ctx);
     for (hipEvent_t e : ctx->ev_free)
         (void)hipEventDestroy(e);
     for (auto& pr : ctx->ev_open)

@@ -34,6 +34,11 @@ struct vc_ctx
     std::vector<hipEvent_t> ev_free;                         // reusable
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_open;  // not yet read
     std::vector<int> ev_open_n;                              // launches per pair
+    // vc_skeleton_noalloc: this device's scratch array (device address) and
+    // the host records of the call, built at context creation
+    uint8_t* na_scratch = nullptr;
+    vc_vol* na_vol = nullptr;
+    vc_sched* na_sched = nullptr;
 };
 
 // packed voxel coordinate: x << (by+bz) | y << bz | z, fits in 31 bits
@@ -218,6 +223,9 @@ int vc_vol_sched(vc_ctx* ctx, vc_vol* v);  // build v->self if missing
 
 // ---------------------------------------------------------------- stages
 int vc_lut_build(vc_ctx* ctx);
+// vc_skeleton_noalloc set-up at context creation / release at destruction
+int vc_noalloc_setup(vc_ctx* ctx);
+void vc_noalloc_teardown(vc_ctx* ctx);
 int vc_edt_sched(vc_ctx* ctx, vc_sched* s);
 int vc_measure_sched(vc_ctx* ctx, vc_sched* s);
 // levels: n x n_levels (row i = ascending levels of volume i)
