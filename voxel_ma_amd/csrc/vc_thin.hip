@@ -937,11 +937,14 @@ __global__ void __launch_bounds__(64)
 // Planes: sweep t reads alive (t even) or alive2 (t odd) and writes the other
 // one.  A tile that is not visited has equal content in both planes (it did
 // not change in the previous sweep, and the sweep before wrote it), so no
-// copy is needed; the entry kernel copies alive to alive2 once.  A tile is
-// visited iff a tile of its 3 x 3 x 3 block changed in the previous sweep
-// (changes travel at most 8 voxels per sweep, less than a tile); on entry
-// the tiles holding a revisit flag count as changed.  At the end of a level
-// (a sweep that changed nothing) both planes are equal again.
+// copy is needed.  A level's first sweep visits every tile, so it writes
+// every tile of alive2; when tile sweeps start later in a level (after dense
+// passes) the entry kernel copies alive to alive2 instead.  After the first
+// sweep a tile is visited iff a tile of its 3 x 3 x 3 block changed in the
+// previous sweep (changes travel at most 8 voxels per sweep, less than a
+// tile); after an entry kernel the tiles holding a revisit flag count as
+// changed.  At the end of a level (a sweep that changed nothing) both planes
+// are equal again.
 constexpr int kTT = 32;                     // interior rows along x and y
 constexpr int kHX = 8, kHY = 4, kHZ = 2;    // halo: rows x, rows y, bits z
 constexpr int kRX = kTT + 2 * kHX, kRY = kTT + 2 * kHY, kTRows = kRX * kRY;
@@ -1160,9 +1163,12 @@ __global__ void __launch_bounds__(kTBlock, kTWaves)
         uint32_t* ccur = d.tchg + (int64_t)(sweep & 1) * nt;
         // visit iff a tile of the 3x3x3 neighbourhood removed a voxel of this
         // tile's region in the last sweep: a tile whose region is unchanged
-        // and that removed nothing would remove nothing again
-        bool act = false;
-        if (threadIdx.x < 27)
+        // and that removed nothing would remove nothing again.  The first
+        // sweep of a level visits every tile: it writes both planes' tiles
+        // (reads alive, writes alive2), which stands in for the entry copy,
+        // and its masks start the chain
+        bool act = sweep == 0;
+        if (sweep > 0 && threadIdx.x < 27)
         {
             const int ax = tx + (int)threadIdx.x / 9 - 1;
             const int ay = ty + ((int)threadIdx.x / 3) % 3 - 1;
@@ -1502,7 +1508,9 @@ int launch_segment(vc_ctx* ctx, vc_sched* s, int K, bool graph, int64_t first,
     {
         const unsigned g = vc_sched_grid(ctx, s, VK_TT, pc);
         const vc_work W = vc_sched_work(s, VK_TT);
-        if (g && mode == SEG_TILE_ENTRY)
+        // tile sweeps from the level's first sweep need no entry kernel: that
+        // sweep visits every tile (see thin_tile_sweep)
+        if (g && mode == SEG_TILE_ENTRY && s->sc.tile_sweep != 0)
         {
             if (s->n > 1)
                 thin_tile_enter<true><<<g, kBlock, 0, S>>>(W);
@@ -1749,7 +1757,7 @@ int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels, int nlev,
             if (rc != VC_OK)
                 return rc;
             if (tile)
-                launches += (int64_t)K + (in_tile ? 0 : 1) +
+                launches += (int64_t)K + (in_tile || sc.tile_sweep == 0 ? 0 : 1) +
                             (sc.use_graph ? 1 : 0);
             else
                 launches += (int64_t)K * (per_seg - 1) + (sc.use_graph ? 1 : 0);
