@@ -312,21 +312,34 @@ __global__ void __launch_bounds__(kBlock)
 // b = occupancy of z0 - 1 + b, outside the grid = 0).
 constexpr int kVz = 8;
 
+// The thread of chunk c + 1 is the next lane (threads are chunks in linear
+// order), and chunk c + 1 is the next 8 z of the same row unless c ends its
+// row: the two edge bits come from the neighbouring lanes' middle bytes, and
+// only lanes 0 and 63 load them.  Every lane of the wavefront calls it (the
+// shuffles); `live` is false for threads past the last chunk, which load
+// nothing (they are the highest lanes, so no live lane needs their bits: a
+// live chunk whose successor is past the end ends its row).
 __device__ __forceinline__ uint32_t row_win(const uint8_t* __restrict__ occ,
-                                            const Dims& D, int x, int y, int z0)
+                                            const Dims& D, int x, int y, int z0,
+                                            bool live)
 {
-    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny)
-        return 0;
-    const uint8_t* r = occ + ((int64_t)x * D.ny + y) * D.nz;
-    const uint64_t mid = *reinterpret_cast<const uint64_t*>(r + z0);
+    const bool in = live && (unsigned)x < (unsigned)D.nx && (unsigned)y < (unsigned)D.ny;
+    const uint8_t* r = occ + (in ? ((int64_t)x * D.ny + y) * D.nz : 0);
+    const uint64_t mid = in ? *reinterpret_cast<const uint64_t*>(r + z0) : 0ull;
     // one bit per non-zero byte, gathered into the top byte
     uint64_t t = mid | (mid >> 4);
     t |= t >> 2;
     t |= t >> 1;
     t &= 0x0101010101010101ull;
     const uint32_t m8 = (uint32_t)((t * 0x0102040810204080ull) >> 56);
-    const uint32_t lo = z0 > 0 && r[z0 - 1] != 0;
-    const uint32_t hi = z0 + kVz < D.nz && r[z0 + kVz] != 0;
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(m8, 1u);
+    const uint32_t next = __shfl_down(m8, 1u);
+    uint32_t lo = 0, hi = 0;
+    if (in && z0 > 0)
+        lo = lane > 0 ? (prev >> 7) & 1u : (uint32_t)(r[z0 - 1] != 0);
+    if (in && z0 + kVz < D.nz)
+        hi = lane < 63 ? next & 1u : (uint32_t)(r[z0 + kVz] != 0);
     return lo | (m8 << 1) | (hi << 9);
 }
 
@@ -336,22 +349,22 @@ struct Chunk
     int x, y, z0;
 };
 
+// every lane of the wavefront calls it (row_win); false past the last chunk
 __device__ __forceinline__ bool load_chunk(const uint8_t* __restrict__ occ,
                                            const Dims& D, int64_t c, Chunk& K)
 {
     const int nzc = D.nz / kVz;
-    if (c >= (int64_t)D.nx * D.ny * nzc)
-        return false;
-    const int64_t row = c / nzc;
-    K.z0 = (int)(c - row * nzc) * kVz;
+    const bool live = c < (int64_t)D.nx * D.ny * nzc;
+    const int64_t row = (live ? c : 0) / nzc;
+    K.z0 = (int)((live ? c : 0) - row * nzc) * kVz;
     K.x = (int)(row / D.ny);
     K.y = (int)(row - (int64_t)K.x * D.ny);
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
-            K.w[dx][dy] = row_win(occ, D, K.x + dx - 1, K.y + dy - 1, K.z0);
-    return true;
+            K.w[dx][dy] = row_win(occ, D, K.x + dx - 1, K.y + dy - 1, K.z0, live);
+    return live;
 }
 
 // the 3 x 3 x 3 neighbourhood of voxel z0 + j of the chunk; null if the voxel
@@ -973,18 +986,21 @@ __global__ void __launch_bounds__(kBlock)
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // V_bnd, E_bnd, Vint_sum, E_int, F_int, T, chi+, chi-
     uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (c < (int64_t)(D.nx + 1) * (D.ny + 1) * nzc)
+    // lattice rows in linear order, chunks of a row consecutive: the windows
+    // are read by every lane (row_win shuffles), the terms by live ones
+    const bool live = c < (int64_t)(D.nx + 1) * (D.ny + 1) * nzc;
+    const int64_t row = (live ? c : 0) / nzc;
+    const int z0 = (int)((live ? c : 0) - row * nzc) * kVz;
+    const int cx = (int)(row / (D.ny + 1));
+    const int cy = (int)(row - (int64_t)cx * (D.ny + 1));
+    uint32_t w[3][3];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+            w[dx][dy] = row_win(occ, D, cx + dx - 1, cy + dy - 1, z0, live);
+    if (live)
     {
-        const int64_t row = c / nzc;
-        const int z0 = (int)(c - row * nzc) * kVz;
-        const int cx = (int)(row / (D.ny + 1));
-        const int cy = (int)(row - (int64_t)cx * (D.ny + 1));
-        uint32_t w[3][3];
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-                w[dx][dy] = row_win(occ, D, cx + dx - 1, cy + dy - 1, z0);
         const bool vrow = cx < D.nx && cy < D.ny;
         const int jn = z0 + kVz == D.nz ? kVz + 1 : kVz;
         // rows of the site mask: base voxel (int)-truncated at coordinate 0,
@@ -1092,119 +1108,148 @@ __global__ void __launch_bounds__(kBlock)
     bnd8[c] = (uint8_t)(me & ~all6 & 0xFFu);
 }
 
+// ---- thin layer with compact labels (vectorised path): only the layer
+// voxels get a union-find label.  lay8[c] marks the layer voxels of chunk c
+// (inside and within one voxel, 26-wise, of a boundary voxel); their labels
+// are numbered in linear order, lay_off[c] being the exclusive prefix count
+// of the chunks before c, so voxel j of chunk c has label
+// lay_off[c] + popcount(lay8[c] & ((1 << j) - 1)).
+
 // 10-bit window of the boundary marks of row (x, y) at chunk z0 (bit b =
-// voxel z0 - 1 + b, outside the grid = 0), as row_win for occupancy
-__device__ __forceinline__ uint32_t bnd_win(const uint8_t* __restrict__ bnd8,
-                                            const Dims& D, int x, int y, int z0)
+// voxel z0 - 1 + b, outside the grid = 0); the edge bits come from the
+// neighbouring lanes' chunks, as in row_win, so every lane calls it
+__device__ __forceinline__ uint32_t bnd_win_s(const uint8_t* __restrict__ bnd8,
+                                              const Dims& D, int x, int y, int z0,
+                                              bool live)
 {
-    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny)
-        return 0;
-    const int64_t c = ((int64_t)x * D.ny + y) * (D.nz / kVz) + z0 / kVz;
-    const uint32_t lo = z0 > 0 ? (uint32_t)bnd8[c - 1] >> 7 : 0u;
-    const uint32_t hi = z0 + kVz < D.nz ? (uint32_t)bnd8[c + 1] & 1u : 0u;
-    return lo | ((uint32_t)bnd8[c] << 1) | (hi << 9);
+    const bool in = live && (unsigned)x < (unsigned)D.nx && (unsigned)y < (unsigned)D.ny;
+    const int64_t c = in ? ((int64_t)x * D.ny + y) * (D.nz / kVz) + z0 / kVz : 0;
+    const uint32_t mid = in ? (uint32_t)bnd8[c] : 0u;
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(mid, 1u);
+    const uint32_t next = __shfl_down(mid, 1u);
+    uint32_t lo = 0, hi = 0;
+    if (in && z0 > 0)
+        lo = lane > 0 ? prev >> 7 : (uint32_t)bnd8[c - 1] >> 7;
+    if (in && z0 + kVz < D.nz)
+        hi = lane < 63 ? next & 1u : (uint32_t)bnd8[c + 1] & 1u;
+    return lo | (mid << 1) | (hi << 9);
 }
 
-// thin-layer labels of 8 consecutive z: -1 outside the layer, else the
-// index of the first voxel of its run of layer voxels within the 8 (a run
-// is z-connected, so it starts as one union-find tree)
+// layer marks of chunk c and their count (scanned into label offsets)
 __global__ void __launch_bounds__(kBlock)
-    layer_init_v(const uint8_t* __restrict__ occ,
+    layer_mark_v(const uint8_t* __restrict__ occ,
                  const uint8_t* __restrict__ bnd8, Dims D,
-                 int32_t* __restrict__ lab)
+                 uint8_t* __restrict__ lay8, uint32_t* __restrict__ lay_cnt)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int64_t)D.nx * D.ny * (D.nz / kVz))
-        return;
+    const int nzc = D.nz / kVz;
+    const bool live = c < (int64_t)D.nx * D.ny * nzc;
     int x, y, z0;
-    chunk_at(D, c, x, y, z0);
-    const int64_t base = ((int64_t)x * D.ny + y) * D.nz + z0;
-    const uint64_t o8 = *reinterpret_cast<const uint64_t*>(occ + base);
+    chunk_at(D, live ? c : 0, x, y, z0);
     uint32_t near = 0; // bit b: a boundary voxel within z0 - 1 + b +- 1
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
         {
-            const uint32_t w = bnd_win(bnd8, D, x + dx - 1, y + dy - 1, z0);
+            const uint32_t w = bnd_win_s(bnd8, D, x + dx - 1, y + dy - 1, z0, live);
             near |= w | (w >> 1) | (w << 1);
         }
-    int32_t out[kVz];
-    int32_t start = -1;
+    if (!live)
+        return;
+    const uint64_t o8 = *reinterpret_cast<const uint64_t*>(
+        occ + ((int64_t)x * D.ny + y) * D.nz + z0);
+    uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < kVz; ++j)
-    {
-        const bool in = ((o8 >> (8 * j)) & 0xFFu) && ((near >> (j + 1)) & 1u);
-        start = in ? (start < 0 ? (int32_t)(base + j) : start) : -1;
-        out[j] = start;
-    }
-    int4* dst = reinterpret_cast<int4*>(lab + base);
-    dst[0] = make_int4(out[0], out[1], out[2], out[3]);
-    dst[1] = make_int4(out[4], out[5], out[6], out[7]);
+        m |= (((o8 >> (8 * j)) & 0xFFu) != 0 && ((near >> (j + 1)) & 1u)) ? 1u << j : 0u;
+    lay8[c] = (uint8_t)m;
+    lay_cnt[c] = (uint32_t)__builtin_popcount(m);
 }
 
-// unions of 8 consecutive z: runs are already trees, so z unions happen only
-// across chunks; the x / y union of a voxel is skipped when the voxel before
-// it in z and that voxel's x / y neighbour are both in the layer (the same
-// trees were joined through them)
+__device__ __forceinline__ int32_t lay_id(const uint8_t* __restrict__ lay8,
+                                          const uint32_t* __restrict__ off,
+                                          int64_t c, int j)
+{
+    return (int32_t)(off[c] + (uint32_t)__builtin_popcount(lay8[c] & ((1u << j) - 1u)));
+}
+
+// initial labels: each z-run of layer voxels within a chunk is one tree
+// rooted at its first voxel
 __global__ void __launch_bounds__(kBlock)
-    layer_union_v(int32_t* lab, Dims D, unsigned long long* guard_hits)
+    layer_init_c(const uint8_t* __restrict__ lay8,
+                 const uint32_t* __restrict__ off, int64_t nchunk,
+                 int32_t* __restrict__ lab)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunk)
+        return;
+    const uint32_t m = lay8[c];
+    if (!m)
+        return;
+    const int32_t base = (int32_t)off[c];
+    int32_t start = -1, k = 0;
+    for (int j = 0; j < kVz; ++j)
+    {
+        if (!((m >> j) & 1u))
+        {
+            start = -1;
+            continue;
+        }
+        if (start < 0)
+            start = base + k;
+        lab[base + k] = start;
+        ++k;
+    }
+}
+
+// unions on the compact labels: runs are already trees, so z unions happen
+// only across chunks; the x / y union of a voxel is skipped when the voxel
+// before it in z and that voxel's x / y neighbour are both in the layer
+// (the same trees were joined through them)
+__global__ void __launch_bounds__(kBlock)
+    layer_union_c(const uint8_t* __restrict__ lay8,
+                  const uint32_t* __restrict__ off, Dims D, int32_t* lab,
+                  unsigned long long* guard_hits)
 {
     const int nzc = D.nz / kVz;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (int64_t)D.nx * D.ny * nzc)
         return;
+    const uint32_t me = lay8[c];
+    if (!me)
+        return;
     const int64_t row = c / nzc;
     const int z0 = (int)(c - row * nzc) * kVz;
     const int x = (int)(row / D.ny);
     const int y = (int)(row - (int64_t)x * D.ny);
-    const int64_t sy = D.nz, sx = (int64_t)D.ny * D.nz;
-    const int64_t base = row * D.nz + z0;
-    auto members = [&](int64_t at) -> uint32_t {
-        const int4* p = reinterpret_cast<const int4*>(lab + at);
-        const int4 a = p[0], b = p[1];
-        return (uint32_t)(a.x >= 0) | (uint32_t)(a.y >= 0) << 1 |
-               (uint32_t)(a.z >= 0) << 2 | (uint32_t)(a.w >= 0) << 3 |
-               (uint32_t)(b.x >= 0) << 4 | (uint32_t)(b.y >= 0) << 5 |
-               (uint32_t)(b.z >= 0) << 6 | (uint32_t)(b.w >= 0) << 7;
-    };
-    const uint32_t me = members(base);
-    if (!me)
-        return;
-    const uint32_t mx = x + 1 < D.nx ? members(base + sx) & me : 0u;
-    const uint32_t my = y + 1 < D.ny ? members(base + sy) & me : 0u;
-    // pairs whose predecessor pair (z - 1) is also a layer pair
+    const int64_t cx = c + (int64_t)D.ny * nzc, cy = c + nzc;
+    const uint32_t mx = x + 1 < D.nx ? lay8[cx] & me : 0u;
+    const uint32_t my = y + 1 < D.ny ? lay8[cy] & me : 0u;
     const uint32_t ux = mx & ~(mx << 1), uy = my & ~(my << 1);
     for (uint32_t u = ux; u; u &= u - 1)
     {
         const int j = __builtin_ctz(u);
-        uf_union(lab, (int32_t)(base + j), (int32_t)(base + j + sx), guard_hits);
+        uf_union(lab, lay_id(lay8, off, c, j), lay_id(lay8, off, cx, j), guard_hits);
     }
     for (uint32_t u = uy; u; u &= u - 1)
     {
         const int j = __builtin_ctz(u);
-        uf_union(lab, (int32_t)(base + j), (int32_t)(base + j + sy), guard_hits);
+        uf_union(lab, lay_id(lay8, off, c, j), lay_id(lay8, off, cy, j), guard_hits);
     }
-    if ((me >> 7) & 1u && z0 + kVz < D.nz && lab[base + kVz] >= 0)
-        uf_union(lab, (int32_t)(base + kVz - 1), (int32_t)(base + kVz), guard_hits);
+    if (((me >> 7) & 1u) && z0 + kVz < D.nz && (lay8[c + 1] & 1u))
+        uf_union(lab, lay_id(lay8, off, c, kVz - 1), (int32_t)off[c + 1], guard_hits);
 }
 
-// thin-layer components: roots (lab[i] == i) of 8 consecutive labels
+// layer components: roots among the n compact labels
 __global__ void __launch_bounds__(kBlock)
-    layer_count_v(const int32_t* __restrict__ lab, int64_t nchunk,
+    layer_count_c(const int32_t* __restrict__ lab, int64_t n,
                   unsigned long long* acc)
 {
     __shared__ unsigned long long sh[kBlock / 64];
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long r = 0;
-    if (c < nchunk)
-    {
-        const int4* p = reinterpret_cast<const int4*>(lab + c * kVz);
-        const int4 a = p[0], b = p[1];
-        const int32_t i = (int32_t)(c * kVz);
-        r = (a.x == i) + (a.y == i + 1) + (a.z == i + 2) + (a.w == i + 3) +
-            (b.x == i + 4) + (b.y == i + 5) + (b.z == i + 6) + (b.w == i + 7);
-    }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long r = i < n && lab[i] == (int32_t)i ? 1ull : 0ull;
     block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
 }
 
@@ -1415,6 +1460,8 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             int32_t* dtbl = nullptr;
             uint8_t* bnd = nullptr;
             int32_t* lab = nullptr;
+            uint8_t* lay8 = nullptr;
+            uint32_t* lay_off = nullptr;
             int32_t htbl[256];
             cube_table(htbl);
             if (e == hipSuccess)
@@ -1426,7 +1473,13 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 // vectorised path, one per voxel otherwise
                 if (rc == VC_OK)
                     rc = vc_alloc(ctx, (void**)&bnd, (size_t)(vec ? N / kVz : N));
-                if (rc == VC_OK)
+                // labels: every voxel on the plain path; the vectorised path
+                // labels only the layer voxels (allocated once counted)
+                if (rc == VC_OK && vec)
+                    rc = vc_alloc(ctx, (void**)&lay8, (size_t)(N / kVz));
+                if (rc == VC_OK && vec)
+                    rc = vc_alloc(ctx, (void**)&lay_off, 4 * (size_t)(N / kVz + 1));
+                if (rc == VC_OK && !vec)
                     rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)N);
             }
             if (rc == VC_OK && e == hipSuccess)
@@ -1440,12 +1493,33 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
                 const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
                 const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
                 const unsigned gv = (unsigned)etiles;
+                const int64_t nch = N / kVz;
                 euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
                 layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
-                layer_init_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
-                layer_union_v<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
-                layer_count_v<<<gv, kBlock, 0, S>>>(lab, N / kVz, acc);
+                layer_mark_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lay8, lay_off);
                 e = hipGetLastError();
+                uint32_t nlay = 0;
+                if (e == hipSuccess)
+                    e = hipMemsetAsync(lay_off + nch, 0, 4, S);
+                if (e == hipSuccess)
+                {
+                    rc = scan_inplace(ctx, lay_off, nch + 1);
+                    if (rc == VC_OK)
+                        e = hipMemcpyAsync(&nlay, lay_off + nch, 4,
+                                           hipMemcpyDeviceToHost, S);
+                    if (rc == VC_OK && e == hipSuccess)
+                        e = hipStreamSynchronize(S);
+                }
+                if (rc == VC_OK && e == hipSuccess && nlay)
+                    rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)nlay);
+                if (rc == VC_OK && e == hipSuccess && nlay)
+                {
+                    const unsigned gn = (unsigned)(((int64_t)nlay + kBlock - 1) / kBlock);
+                    layer_init_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, nch, lab);
+                    layer_union_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, D, lab, acc + 7);
+                    layer_count_c<<<gn, kBlock, 0, S>>>(lab, (int64_t)nlay, acc);
+                    e = hipGetLastError();
+                }
             }
             else if (rc == VC_OK && e == hipSuccess)
             {
@@ -1496,6 +1570,8 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
             vc_free(dtbl);
             vc_free(bnd);
             vc_free(lab);
+            vc_free(lay8);
+            vc_free(lay_off);
             if (rc == VC_OK && e != hipSuccess)
                 rc = vc_fail(ctx, VC_ERR_HIP, "boundary emit/Euler: %s",
                              hipGetErrorString(e));
