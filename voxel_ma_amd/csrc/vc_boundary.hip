@@ -20,6 +20,7 @@
 // base voxel at coordinate -0.5.  C is the number of 6-components of the
 // "thin inside layer" (inside voxels within Chebyshev distance 1 of an inside
 // voxel with a non-inside face neighbour), by union-find.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -367,40 +368,102 @@ __device__ __forceinline__ bool load_chunk(const uint8_t* __restrict__ occ,
     return live;
 }
 
-// the 3 x 3 x 3 neighbourhood of voxel z0 + j of the chunk; null if the voxel
-// has no face event (then it emits nothing)
-__device__ __forceinline__ bool chunk_vox(const Chunk& K, const Dims& D, int j,
-                                          Vox& V)
+// 8 x 8 bit transpose: bit 8 r + c of x becomes bit 8 c + r
+__device__ __forceinline__ uint64_t transpose8(uint64_t x)
 {
-    const uint32_t c = K.w[1][1] >> j;
-    const uint32_t me = (c >> 1) & 1u;
-    const uint32_t f = (c & 1u) | ((c >> 2) & 1u) << 1 |
-                       ((K.w[0][1] >> (j + 1)) & 1u) << 2 |
-                       ((K.w[2][1] >> (j + 1)) & 1u) << 3 |
-                       ((K.w[1][0] >> (j + 1)) & 1u) << 4 |
-                       ((K.w[1][2] >> (j + 1)) & 1u) << 5;
-    if (f == (me ? 0x3Fu : 0u))
-        return false;
-    const int z = K.z0 + j;
-    uint32_t zin = 7u;
-    if (z == 0)
-        zin &= ~1u;
-    if (z + 1 >= D.nz)
-        zin &= ~4u;
-    V.occ = 0;
-    V.ing = 0;
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x ^= t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x ^= t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    x ^= t ^ (t << 28);
+    return x;
+}
+
+// The decisions of `decide` for the 8 voxels of a chunk at once, bit-sliced:
+// every quantity is an 8-bit word whose bit j belongs to voxel z0 + j, so
+// voxels with events cost the wavefront no divergent work.  em[j] gets
+// voxel j's corner keys (bits 0..23) and face offsets (bits 24..29); returns
+// the chunk's (sites << 16) | faces.
+__device__ __forceinline__ uint32_t decide_chunk(const Chunk& K, const Dims& D,
+                                                 uint32_t em[kVz])
+{
+    // occupancy of offset (dx, dy, dz) per voxel, and its in-grid mask
+    auto O = [&](int dx, int dy, int dz) -> uint32_t {
+        return (K.w[dx + 1][dy + 1] >> (1 + dz)) & 0xFFu;
+    };
+    const uint32_t zlo = K.z0 == 0 ? 0xFEu : 0xFFu;          // z - 1 in grid
+    const uint32_t zhi = K.z0 + kVz >= D.nz ? 0x7Fu : 0xFFu; // z + 1 in grid
+    auto G = [&](int dx, int dy, int dz) -> uint32_t {
+        const bool rin = (unsigned)(K.x + dx) < (unsigned)D.nx &&
+                         (unsigned)(K.y + dy) < (unsigned)D.ny;
+        return rin ? (dz < 0 ? zlo : (dz > 0 ? zhi : 0xFFu)) : 0u;
+    };
+    const uint32_t me = O(0, 0, 0);
+    uint32_t ev[6];
 #pragma unroll
-    for (int dx = 0; dx < 3; ++dx)
+    for (int o = 0; o < 6; ++o)
+        ev[o] = me ^ O(kT.off[o][0], kT.off[o][1], kT.off[o][2]);
+    uint32_t key[24];
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-        {
-            const int b = nbit(dx - 1, dy - 1, -1);
-            V.occ |= ((K.w[dx][dy] >> j) & 7u) << b;
-            const bool rin = (unsigned)(K.x + dx - 1) < (unsigned)D.nx &&
-                             (unsigned)(K.y + dy - 1) < (unsigned)D.ny;
-            V.ing |= (rin ? zin : 0u) << b;
-        }
-    return true;
+    for (int k = 0; k < 24; ++k)
+        key[k] = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const int sx = csx(c), sy = csy(c), sz = csz(c);
+        const int ox = sx > 0 ? 1 : 0, oy = sy > 0 ? 3 : 2, oz = sz > 0 ? 5 : 4;
+        const int lx = sx > 0 ? 0 : -1, ly = sy > 0 ? 0 : -1, lz = sz > 0 ? 0 : -1;
+        const int q0 = 4 * -lx + 2 * -ly + -lz;
+        uint32_t W[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            W[q] = O(lx + (q >> 2), ly + ((q >> 1) & 1), lz + (q & 1));
+        // no earlier in-grid block voxel touches the corner
+        uint32_t first = 0xFFu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q < q0)
+            {
+                const uint32_t touch = (W[q] ^ W[q ^ 4]) | (W[q] ^ W[q ^ 2]) |
+                                       (W[q] ^ W[q ^ 1]);
+                first &= ~(touch & G(lx + (q >> 2), ly + ((q >> 1) & 1), lz + (q & 1)));
+            }
+        // keyed at the first touching offset among ox, oy, oz
+        const uint32_t ax = first & ev[ox];
+        const uint32_t ay = first & ~ev[ox] & ev[oy];
+        const uint32_t az = first & ~ev[ox] & ~ev[oy] & ev[oz];
+        key[4 * ox + cwrt_pos(ox, c)] |= ax;
+        key[4 * oy + cwrt_pos(oy, c)] |= ay;
+        key[4 * oz + cwrt_pos(oz, c)] |= az;
+    }
+    // faces: +offsets, and -offsets whose neighbour is outside the grid
+    uint64_t F = 0;
+#pragma unroll
+    for (int o = 0; o < 6; ++o)
+    {
+        const uint32_t f = (o & 1) ? ev[o]
+                                   : ev[o] & ~G(kT.off[o][0], kT.off[o][1], kT.off[o][2]);
+        F |= (uint64_t)(f & 0xFFu) << (8 * o);
+    }
+    uint64_t M[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 24; ++k)
+        M[k >> 3] |= (uint64_t)(key[k] & 0xFFu) << (8 * (k & 7));
+    const uint32_t ns = (uint32_t)(__popcll(M[0]) + __popcll(M[1]) + __popcll(M[2]));
+    const uint32_t nf = (uint32_t)__popcll(F);
+    if (ns | nf)
+    {
+        const uint64_t T0 = transpose8(M[0]), T1 = transpose8(M[1]),
+                       T2 = transpose8(M[2]), T3 = transpose8(F);
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+            em[j] = (uint32_t)((T0 >> (8 * j)) & 0xFFu) |
+                    (uint32_t)((T1 >> (8 * j)) & 0xFFu) << 8 |
+                    (uint32_t)((T2 >> (8 * j)) & 0xFFu) << 16 |
+                    (uint32_t)((T3 >> (8 * j)) & 0xFFu) << 24;
+    }
+    return (ns << 16) | nf;
 }
 
 // emission of one voxel's sites (coordinates + corner map) and quads
@@ -453,21 +516,11 @@ __global__ void __launch_bounds__(kBlock)
     if (load_chunk(occ, D, c, K))
     {
         uint32_t em[kVz];
-#pragma unroll
-        for (int j = 0; j < kVz; ++j)
-        {
-            em[j] = 0;
-            Vox V;
-            if (chunk_vox(K, D, j, V))
-            {
-                const Emit e = decide(V);
-                ns += (uint32_t)__builtin_popcount(e.keys);
-                nf += (uint32_t)__builtin_popcount(e.faces);
-                em[j] = e.keys | (e.faces << 24);
-            }
-        }
-        chunk_cnt[c] = (ns << 16) | nf;
-        if (ns | nf)
+        const uint32_t cnt = decide_chunk(K, D, em);
+        ns = cnt >> 16;
+        nf = cnt & 0xFFFFu;
+        chunk_cnt[c] = cnt;
+        if (cnt)
         {
             uint4* dst = reinterpret_cast<uint4*>(vmask + (int64_t)c * kVz);
             dst[0] = make_uint4(em[0], em[1], em[2], em[3]);
@@ -976,12 +1029,19 @@ __device__ __forceinline__ void block_add8(const uint32_t (&v)[8],
 // is a voxel row.  Window w[dx][dy] is voxel row (cx + dx - 1, cy + dy - 1)
 // and its bit b is voxel z0 - 1 + b, so lattice point cz = z0 + j sees
 // voxels cz - 1, cz at bits j, j + 1.  Terms and conventions as chi_lattice,
-// euler_lattice and euler_voxels.
+// euler_lattice and euler_voxels, evaluated bit-sliced: each quantity is a
+// word whose bit j belongs to lattice point (or voxel) z0 + j, so all points
+// of the chunk are done by the same few bitwise operations; only the
+// boundary-corner table lookups run per point.
 __global__ void __launch_bounds__(kBlock)
     euler_rows_v(const uint8_t* __restrict__ occ, Dims D,
                  const int32_t* __restrict__ tbl, unsigned long long* acc)
 {
     __shared__ unsigned long long sh[8][kBlock / 64];
+    __shared__ int32_t stbl[256];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x)
+        stbl[i] = tbl[i];
+    __syncthreads();
     const int nzc = D.nz / kVz;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // V_bnd, E_bnd, Vint_sum, E_int, F_int, T, chi+, chi-
@@ -1001,90 +1061,91 @@ __global__ void __launch_bounds__(kBlock)
             w[dx][dy] = row_win(occ, D, cx + dx - 1, cy + dy - 1, z0, live);
     if (live)
     {
-        const bool vrow = cx < D.nx && cy < D.ny;
-        const int jn = z0 + kVz == D.nz ? kVz + 1 : kVz;
-        // rows of the site mask: base voxel (int)-truncated at coordinate 0,
-        // so the block is rows cx - 1, cx (cx > 0) or 0, 1 (cx = 0), same in y
-        uint32_t sr[2][2];
+        // lattice points of the chunk (a ninth, cz = nz, on a row's last)
+        const uint32_t JM = z0 + kVz == D.nz ? 0x1FFu : 0xFFu;
+        const uint32_t zf = z0 == 0 ? 1u : 0u; // point / voxel at z = 0
+        // voxel (cx - 1 + a, cy - 1 + b, cz - 1 + e) of lattice point cz
+        auto L = [&](int a, int b, int e) -> uint32_t { return (w[a][b] >> e) & JM; };
+        const uint32_t l000 = L(0, 0, 0), l001 = L(0, 0, 1), l010 = L(0, 1, 0),
+                       l011 = L(0, 1, 1), l100 = L(1, 0, 0), l101 = L(1, 0, 1),
+                       l110 = L(1, 1, 0), l111 = L(1, 1, 1);
+        const uint32_t any = l000 | l001 | l010 | l011 | l100 | l101 | l110 | l111;
+        const uint32_t all = l000 & l001 & l010 & l011 & l100 & l101 & l110 & l111;
+        s[6] = __builtin_popcount(any) + __builtin_popcount(l110 | l111) +
+               __builtin_popcount(l101 | l111) + __builtin_popcount(l011 | l111);
+        s[7] = __builtin_popcount(l100 | l101 | l110 | l111) +
+               __builtin_popcount(l010 | l011 | l110 | l111) +
+               __builtin_popcount(l001 | l011 | l101 | l111) + __builtin_popcount(l111);
+        // edges from the point along +x, +y, +z (inside the grid): the four
+        // voxels around each, in cyclic order
+        const uint32_t lim[3] = {cx < D.nx ? JM : 0u, cy < D.ny ? JM : 0u, 0xFFu};
+        const uint32_t ring[3][4] = {{l100, l110, l111, l101},
+                                     {l010, l011, l111, l110},
+                                     {l001, l101, l111, l011}};
 #pragma unroll
-        for (int ix = 0; ix < 2; ++ix)
-#pragma unroll
-            for (int iy = 0; iy < 2; ++iy)
-            {
-                const uint32_t a0 = cy > 0 ? w[ix][iy] : w[ix][iy + 1];
-                const uint32_t a1 = cy > 0 ? w[ix + 1][iy] : w[ix + 1][iy + 1];
-                sr[ix][iy] = cx > 0 ? a0 : a1;
-            }
-        for (int j = 0; j < jn; ++j)
+        for (int ax = 0; ax < 3; ++ax)
         {
-            const int cz = z0 + j;
-            // the 8 voxels (cx - 1 + a, cy - 1 + b, cz - 1 + e), bit 4a + 2b + e
-            const uint32_t m = ((w[0][0] >> j) & 3u) | ((w[0][1] >> j) & 3u) << 2 |
-                               ((w[1][0] >> j) & 3u) << 4 | ((w[1][1] >> j) & 3u) << 6;
-            s[6] += (m != 0) + ((m & 0xC0u) != 0) + ((m & 0xA0u) != 0) +
-                    ((m & 0x88u) != 0);
-            s[7] += ((m & 0xF0u) != 0) + ((m & 0xCCu) != 0) + ((m & 0xAAu) != 0) +
-                    ((m & 0x80u) != 0);
-            const int blk = __builtin_popcount(m);
-            if (blk == 8)
-                s[3] += (cx < D.nx) + (cy < D.ny) + (cz < D.nz);
-            else if (blk != 0)
+            const uint32_t v0 = ring[ax][0], v1 = ring[ax][1], v2 = ring[ax][2],
+                           v3 = ring[ax][3];
+            const uint32_t d0 = v0 ^ v1, d1 = v1 ^ v2, d2 = v2 ^ v3, d3 = v3 ^ v0;
+            const uint32_t ch4 = d0 & d1 & d2 & d3;
+            const uint32_t ch2 = (d0 | d1 | d2 | d3) & ~ch4;
+            s[1] += __builtin_popcount(ch2 & lim[ax]) + 2 * __builtin_popcount(ch4 & lim[ax]);
+            s[3] += __builtin_popcount(v0 & v1 & v2 & v3 & lim[ax]);
+        }
+        // boundary corners (some but not all of the 8 voxels inside): the
+        // table of the site mask, whose block starts at voxel 0 where the
+        // coordinate is 0
+        const uint32_t part = any & ~all;
+        if (part)
+        {
+            const int bx = cx == 0 ? 1 : 0, by = cy == 0 ? 1 : 0;
+            uint64_t X = 0;
+            uint32_t last = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
             {
-                const int sz0 = cz > 0 ? j : j + 1;
-                uint32_t mask = 0;
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                {
-                    const int ix = (i >> 2) & 1, iy = (i >> 1) & 1, iz = i & 1;
-                    mask |= ((sr[ix][iy] >> (sz0 + iz)) & 1u) << (7 - i);
-                }
-                s[0] += (uint32_t)tbl[mask];
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax)
-                {
-                    const int lim = ax == 0 ? D.nx : (ax == 1 ? D.ny : D.nz);
-                    const int cc = ax == 0 ? cx : (ax == 1 ? cy : cz);
-                    if (cc >= lim)
-                        continue;
-                    uint32_t v4[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int u = (q == 1 || q == 2) ? 0 : -1; // cyclic square
-                        const int wq = (q >= 2) ? 0 : -1;
-                        int p[3] = {0, 0, 0};
-                        p[(ax + 1) % 3] += u;
-                        p[(ax + 2) % 3] += wq;
-                        v4[q] = (w[p[0] + 1][p[1] + 1] >> (j + p[2] + 1)) & 1u;
-                    }
-                    const uint32_t ch = (v4[0] != v4[1]) + (v4[1] != v4[2]) +
-                                        (v4[2] != v4[3]) + (v4[3] != v4[0]);
-                    s[1] += ch == 2 ? 1u : (ch == 4 ? 2u : 0u);
-                    s[3] += (v4[0] & v4[1] & v4[2] & v4[3]);
-                }
+                const int ix = (i >> 2) & 1, iy = (i >> 1) & 1, iz = i & 1;
+                const uint32_t r = w[ix + bx][iy + by];
+                const uint32_t si = (((r >> iz) & ~zf) | ((r >> (iz + 1)) & zf)) & JM;
+                X |= (uint64_t)(si & 0xFFu) << (8 * (7 - i));
+                last |= ((si >> 8) & 1u) << (7 - i);
             }
-            if (!vrow || j >= kVz || !((w[1][1] >> (j + 1)) & 1u))
-                continue;
-            // inside voxel (cx, cy, cz): T, F_int and the interior-corner sum
-            uint32_t m27 = 0;
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-                    m27 |= ((w[dx][dy] >> j) & 7u) << nbit(dx - 1, dy - 1, -1);
-            s[5] += 1;
-            s[4] += obit(m27, -1, 0, 0) + obit(m27, 1, 0, 0) + obit(m27, 0, -1, 0) +
-                    obit(m27, 0, 1, 0) + obit(m27, 0, 0, -1) + obit(m27, 0, 0, 1);
-            // corner block of slot c starts at offset -1, or 0 on the high
-            // side or at coordinate 0; block (-1,-1,-1) is bits 0x361B
+            const uint64_t T = transpose8(X);
+            for (uint32_t p = part; p; p &= p - 1)
+            {
+                const int j = __builtin_ctz(p);
+                const uint32_t mask = j < 8 ? (uint32_t)(T >> (8 * j)) & 0xFFu : last;
+                s[0] += (uint32_t)stbl[mask];
+            }
+        }
+        // inside voxels (cx, cy, z0 + j): T, F_int and the interior-corner sum
+        if (cx < D.nx && cy < D.ny)
+        {
+            auto V = [&](int dx, int dy, int dz) -> uint32_t {
+                return (w[dx + 1][dy + 1] >> (dz + 1)) & 0xFFu;
+            };
+            const uint32_t in = V(0, 0, 0);
+            s[5] = __builtin_popcount(in);
+            s[4] = __builtin_popcount(in & V(-1, 0, 0)) + __builtin_popcount(in & V(1, 0, 0)) +
+                   __builtin_popcount(in & V(0, -1, 0)) + __builtin_popcount(in & V(0, 1, 0)) +
+                   __builtin_popcount(in & V(0, 0, -1)) + __builtin_popcount(in & V(0, 0, 1));
+            // 2x2x2 block (ax..ax+1, ay..ay+1, az..az+1) all inside
+            auto B = [&](int ax, int ay, int az) -> uint32_t {
+                return V(ax, ay, az) & V(ax, ay, az + 1) & V(ax, ay + 1, az) &
+                       V(ax, ay + 1, az + 1) & V(ax + 1, ay, az) & V(ax + 1, ay, az + 1) &
+                       V(ax + 1, ay + 1, az) & V(ax + 1, ay + 1, az + 1);
+            };
+            // corner slot q: block offset -1, or 0 on its high side or at
+            // coordinate 0 (in z only the voxel at z = 0)
 #pragma unroll
             for (int q = 0; q < 8; ++q)
             {
                 const int ax = (((q >> 2) & 1) || cx == 0) ? 0 : -1;
                 const int ay = (((q >> 1) & 1) || cy == 0) ? 0 : -1;
-                const int az = ((q & 1) || cz == 0) ? 0 : -1;
-                const uint32_t bm = 0x361Bu << ((az + 1) + 3 * (ay + 1) + 9 * (ax + 1));
-                s[2] += (m27 & bm) == bm;
+                const uint32_t full = (q & 1) ? B(ax, ay, 0)
+                                              : ((B(ax, ay, -1) & ~zf) | (B(ax, ay, 0) & zf));
+                s[2] += __builtin_popcount(in & full);
             }
         }
     }
@@ -1242,14 +1303,18 @@ __global__ void __launch_bounds__(kBlock)
         uf_union(lab, lay_id(lay8, off, c, kVz - 1), (int32_t)off[c + 1], guard_hits);
 }
 
-// layer components: roots among the n compact labels
+// layer components: roots among the *n_lab compact labels (the count stays
+// on the device: a fixed grid strides over it)
 __global__ void __launch_bounds__(kBlock)
-    layer_count_c(const int32_t* __restrict__ lab, int64_t n,
-                  unsigned long long* acc)
+    layer_count_c(const int32_t* __restrict__ lab,
+                  const uint32_t* __restrict__ n_lab, unsigned long long* acc)
 {
     __shared__ unsigned long long sh[kBlock / 64];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long r = i < n && lab[i] == (int32_t)i ? 1ull : 0ull;
+    const int64_t n = *n_lab;
+    unsigned long long r = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        r += lab[i] == (int32_t)i ? 1ull : 0ull;
     block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
 }
 
@@ -1330,6 +1395,207 @@ int scan_inplace(vc_ctx* ctx, uint32_t* a, int64_t n)
     return VC_OK;
 }
 
+// words of chunk totals scan_ws needs for n elements (all levels)
+int64_t scan_tmp_words(int64_t n)
+{
+    int64_t t = 0;
+    while (n > 1)
+    {
+        n = (n + 1023) / 1024;
+        t += n;
+    }
+    return t > 0 ? t : 1;
+}
+
+// exclusive scan in place, queued on S without allocating or waiting; tmp
+// holds scan_tmp_words(n) words
+hipError_t scan_ws(hipStream_t S, uint32_t* a, int64_t n, uint32_t* tmp)
+{
+    const int64_t chunks = (n + 1023) / 1024;
+    scan_chunks<<<(unsigned)chunks, 1024, 0, S>>>(a, n, tmp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && chunks > 1)
+    {
+        e = scan_ws(S, tmp, chunks, tmp + chunks);
+        if (e == hipSuccess)
+        {
+            scan_add<<<(unsigned)chunks, 1024, 0, S>>>(a, n, tmp);
+            e = hipGetLastError();
+        }
+    }
+    return e;
+}
+
+template <typename T>
+int ws(vc_ctx* ctx, int slot, size_t bytes, T** p)
+{
+    return vc_ws_get(ctx, slot, bytes, reinterpret_cast<void**>(p));
+}
+
+// sites, quads and Euler terms of v into the volume's records.  Scratch
+// comes from the context's slots; one wait for the totals (the Euler
+// kernels are queued before it), the emit kernels are left queued.
+int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
+{
+    hipStream_t S = ctx->stream;
+    const Dims D{v->nx, v->ny, v->nz};
+    const int64_t N = v->N;
+    const int64_t ntiles = (N + kBlock - 1) / kBlock;
+    const int64_t nch = vec ? N / kVz : 0;
+    const size_t ncorner = (size_t)(v->nx + 1) * (v->ny + 1) * (v->nz + 1);
+    uint32_t *ts, *tf, *cmap, *tmp, *ccnt = nullptr, *vmask = nullptr,
+             *lay_off = nullptr;
+    unsigned long long* acc;
+    uint8_t *bnd, *lay8 = nullptr;
+    int32_t* lab;
+    const int64_t tmpw = std::max(scan_tmp_words(etiles + 1),
+                                  vec ? scan_tmp_words(nch + 1) : (int64_t)1);
+    int rc;
+    // tile totals carry one more word: the scan leaves the grand total there
+    if ((rc = ws(ctx, VC_WS_BND_TS, 4 * (size_t)(etiles + 1), &ts)) != VC_OK ||
+        (rc = ws(ctx, VC_WS_BND_TF, 4 * (size_t)(etiles + 1), &tf)) != VC_OK ||
+        (rc = ws(ctx, VC_WS_BND_CMAP, 4 * ncorner, &cmap)) != VC_OK ||
+        (rc = ws(ctx, VC_WS_SCAN, 4 * (size_t)tmpw, &tmp)) != VC_OK ||
+        (rc = ws(ctx, VC_WS_BND_ACC, 8 * kSlots * kAcc + 4 * 256, &acc)) != VC_OK ||
+        // boundary marks: one byte per 8-voxel chunk on the vectorised path,
+        // one per voxel otherwise; labels: at most one per voxel
+        (rc = ws(ctx, VC_WS_BND_MARK, (size_t)(vec ? nch : N), &bnd)) != VC_OK ||
+        (rc = ws(ctx, VC_WS_BND_LAB, 4 * (size_t)N, &lab)) != VC_OK)
+        return rc;
+    // per-chunk counts and per-voxel decision masks, thin-layer marks and
+    // label offsets of the vectorised passes
+    if (vec && ((rc = ws(ctx, VC_WS_BND_CCNT, 4 * (size_t)nch, &ccnt)) != VC_OK ||
+                (rc = ws(ctx, VC_WS_BND_VMASK, 4 * (size_t)N, &vmask)) != VC_OK ||
+                (rc = ws(ctx, VC_WS_BND_LAY8, (size_t)nch, &lay8)) != VC_OK ||
+                (rc = ws(ctx, VC_WS_BND_LAYOFF, 4 * (size_t)(nch + 1), &lay_off)) != VC_OK))
+        return rc;
+    int32_t* dtbl = reinterpret_cast<int32_t*>(acc + kSlots * kAcc);
+    int32_t htbl[256];
+    cube_table(htbl);
+
+    hipError_t e = hipMemsetAsync(ts + etiles, 0, 4, S);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(tf + etiles, 0, 4, S);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(acc, 0, 8 * kSlots * kAcc, S);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dtbl, htbl, sizeof(htbl), hipMemcpyHostToDevice, S);
+    if (e == hipSuccess)
+    {
+        if (vec)
+            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ccnt, vmask, ts, tf);
+        else
+            bnd_count<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = scan_ws(S, ts, etiles + 1, tmp);
+    if (e == hipSuccess)
+        e = scan_ws(S, tf, etiles + 1, tmp);
+    uint32_t tot[2] = {0, 0};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&tot[0], ts + etiles, 4, hipMemcpyDeviceToHost, S);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&tot[1], tf + etiles, 4, hipMemcpyDeviceToHost, S);
+    // Euler terms
+    if (e == hipSuccess && vec)
+    {
+        const int64_t nzc = v->nz / kVz;
+        const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
+        const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
+        const unsigned gv = (unsigned)etiles;
+        const unsigned gn = (unsigned)std::min<int64_t>(etiles, 2048);
+        euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
+        layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
+        layer_mark_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lay8, lay_off);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemsetAsync(lay_off + nch, 0, 4, S);
+        if (e == hipSuccess)
+            e = scan_ws(S, lay_off, nch + 1, tmp);
+        if (e == hipSuccess)
+        {
+            layer_init_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, nch, lab);
+            layer_union_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, D, lab, acc + 7);
+            layer_count_c<<<gn, kBlock, 0, S>>>(lab, lay_off + nch, acc);
+            e = hipGetLastError();
+        }
+    }
+    else if (e == hipSuccess)
+    {
+        const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
+                      (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
+        euler_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
+        chi_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, acc);
+        const dim3 gv((unsigned)((v->nz + kBlock - 1) / kBlock),
+                      (unsigned)v->ny, (unsigned)v->nx);
+        euler_voxels<<<gv, kBlock, 0, S>>>(v->occ, D, acc);
+        layer_boundary<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
+        layer_init<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
+        layer_union<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
+        layer_count<<<(unsigned)ntiles, kBlock, 0, S>>>(lab, N, acc);
+        e = hipGetLastError();
+    }
+    unsigned long long hacc[kSlots * kAcc];
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost, S);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(S);
+    if (e != hipSuccess)
+        return vc_fail(ctx, VC_ERR_HIP, "boundary count/Euler: %s", hipGetErrorString(e));
+    const int64_t ns = tot[0], nf = tot[1];
+    long long sum[kAcc] = {0};
+    for (int sl = 0; sl < kSlots; ++sl)
+        for (int q = 0; q < kAcc; ++q)
+            sum[q] += (long long)hacc[sl * kAcc + q];
+    if (sum[7] != 0)
+        return vc_fail(ctx, VC_ERR_GUARD, "boundary: union-find retry bound reached");
+
+    vc_free(v->bnd_sites);
+    vc_free(v->bnd_quads);
+    v->bnd_sites = nullptr;
+    v->bnd_quads = nullptr;
+    if ((rc = vc_alloc(ctx, (void**)&v->bnd_sites, 12 * (size_t)ns)) != VC_OK ||
+        (rc = vc_alloc(ctx, (void**)&v->bnd_quads, 16 * (size_t)nf)) != VC_OK)
+        return rc;
+    if (vec)
+    {
+        bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(D, ccnt, vmask, ts,
+                                                             v->bnd_sites, cmap);
+        bnd_emit_quads_v<<<(unsigned)etiles, kBlock, 0, S>>>(D, ccnt, vmask, tf, cmap,
+                                                             v->bnd_quads);
+    }
+    else
+    {
+        bnd_emit_sites<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts,
+                                                           v->bnd_sites, cmap);
+        bnd_emit_quads<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, tf, cmap,
+                                                           v->bnd_quads);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return vc_fail(ctx, VC_ERR_HIP, "boundary emit: %s", hipGetErrorString(e));
+
+    vc_euler& E = v->bnd_euler;
+    std::memset(&E, 0, sizeof(E));
+    E.V_bnd = sum[0];
+    E.E_bnd = sum[1];
+    E.V_int = sum[2] / 8;
+    E.E_int = sum[3];
+    E.F_int = sum[4] / 2;
+    E.T = sum[5];
+    E.C = sum[6];
+    E.V = E.V_bnd + E.V_int;
+    E.E = E.E_bnd + E.E_int;
+    E.F = nf + E.F_int;
+    E.euler = (int64_t)(int32_t)(E.V - E.E + E.F - E.T);
+    E.chi = sum[8] - sum[9];
+    v->bnd_nsites = ns;
+    v->bnd_nquads = nf;
+    v->bnd_done = true;
+    return VC_OK;
+}
+
 } // namespace
 
 int vc_scan_u32(vc_ctx* ctx, uint32_t* a, int64_t n)
@@ -1343,7 +1609,6 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
 {
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
-    const Dims D{v->nx, v->ny, v->nz};
     const int64_t N = v->N;
     const int64_t ntiles = (N + kBlock - 1) / kBlock;
     // site and face counts are scanned in 32 bits and quads index sites in
@@ -1365,227 +1630,11 @@ int vc_boundary_run(vc_ctx* ctx, vc_vol* v, float* sites, int64_t site_cap,
     const bool vec = (v->nz % kVz) == 0;
     const int64_t etiles =
         vec ? (N / kVz + kBlock - 1) / kBlock : ntiles;
-    int rc = VC_OK;
     if (!v->bnd_done)
     {
-        uint32_t *ts = nullptr, *tf = nullptr, *cmap = nullptr, *ccnt = nullptr,
-                 *vmask = nullptr;
-        if ((rc = vc_alloc(ctx, (void**)&ts, 4 * (size_t)etiles)) != VC_OK)
-            return rc;
-        if ((rc = vc_alloc(ctx, (void**)&tf, 4 * (size_t)etiles)) != VC_OK)
-        {
-            vc_free(ts);
-            return rc;
-        }
-        // per-chunk (sites, faces) counts and per-voxel decision masks of the
-        // vectorised passes
-        if (vec && ((rc = vc_alloc(ctx, (void**)&ccnt, 4 * (size_t)(N / kVz))) != VC_OK ||
-                    (rc = vc_alloc(ctx, (void**)&vmask, 4 * (size_t)N)) != VC_OK))
-        {
-            vc_free(ts);
-            vc_free(tf);
-            vc_free(ccnt);
-            return rc;
-        }
-        if (vec)
-            bnd_count_v<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, ccnt, vmask, ts,
-                                                            tf);
-        else
-            bnd_count<<<(unsigned)etiles, kBlock, 0, S>>>(v->occ, D, N, ts, tf);
-        hipError_t e = hipGetLastError();
-        uint32_t last_s = 0, last_f = 0, cnt_s = 0, cnt_f = 0;
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&last_s, ts + etiles - 1, 4,
-                               hipMemcpyDeviceToHost, S);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&last_f, tf + etiles - 1, 4,
-                               hipMemcpyDeviceToHost, S);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(S);
-        if (e != hipSuccess)
-            rc = vc_fail(ctx, VC_ERR_HIP, "boundary count: %s",
-                         hipGetErrorString(e));
-        if (rc == VC_OK)
-            rc = scan_inplace(ctx, ts, etiles);
-        if (rc == VC_OK)
-            rc = scan_inplace(ctx, tf, etiles);
-        if (rc == VC_OK)
-        {
-            e = hipMemcpyAsync(&cnt_s, ts + etiles - 1, 4,
-                               hipMemcpyDeviceToHost, S);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(&cnt_f, tf + etiles - 1, 4,
-                                   hipMemcpyDeviceToHost, S);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(S);
-            if (e != hipSuccess)
-                rc = vc_fail(ctx, VC_ERR_HIP, "boundary scan readback: %s",
-                             hipGetErrorString(e));
-        }
-        const int64_t ns = (int64_t)cnt_s + last_s, nf = (int64_t)cnt_f + last_f;
-        const size_t ncorner = (size_t)(v->nx + 1) * (v->ny + 1) * (v->nz + 1);
-        if (rc == VC_OK)
-        {
-            vc_free(v->bnd_sites);
-            vc_free(v->bnd_quads);
-            v->bnd_sites = nullptr;
-            v->bnd_quads = nullptr;
-            rc = vc_alloc(ctx, (void**)&v->bnd_sites, 12 * (size_t)ns);
-            if (rc == VC_OK)
-                rc = vc_alloc(ctx, (void**)&v->bnd_quads, 16 * (size_t)nf);
-            if (rc == VC_OK)
-                rc = vc_alloc(ctx, (void**)&cmap, 4 * ncorner);
-        }
-        if (rc == VC_OK)
-        {
-            if (vec)
-                bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                    D, ccnt, vmask, ts, v->bnd_sites, cmap);
-            else
-                bnd_emit_sites<<<(unsigned)etiles, kBlock, 0, S>>>(
-                    v->occ, D, N, ts, v->bnd_sites, cmap);
-            e = hipGetLastError();
-            if (e == hipSuccess)
-            {
-                if (vec)
-                    bnd_emit_quads_v<<<(unsigned)etiles, kBlock, 0, S>>>(
-                        D, ccnt, vmask, tf, cmap, v->bnd_quads);
-                else
-                    bnd_emit_quads<<<(unsigned)etiles, kBlock, 0, S>>>(
-                        v->occ, D, N, tf, cmap, v->bnd_quads);
-                e = hipGetLastError();
-            }
-            // Euler terms
-            unsigned long long* acc = nullptr;
-            int32_t* dtbl = nullptr;
-            uint8_t* bnd = nullptr;
-            int32_t* lab = nullptr;
-            uint8_t* lay8 = nullptr;
-            uint32_t* lay_off = nullptr;
-            int32_t htbl[256];
-            cube_table(htbl);
-            if (e == hipSuccess)
-            {
-                rc = vc_alloc(ctx, (void**)&acc, 8 * kSlots * kAcc);
-                if (rc == VC_OK)
-                    rc = vc_alloc(ctx, (void**)&dtbl, sizeof(htbl));
-                // boundary marks: one byte per 8-voxel chunk on the
-                // vectorised path, one per voxel otherwise
-                if (rc == VC_OK)
-                    rc = vc_alloc(ctx, (void**)&bnd, (size_t)(vec ? N / kVz : N));
-                // labels: every voxel on the plain path; the vectorised path
-                // labels only the layer voxels (allocated once counted)
-                if (rc == VC_OK && vec)
-                    rc = vc_alloc(ctx, (void**)&lay8, (size_t)(N / kVz));
-                if (rc == VC_OK && vec)
-                    rc = vc_alloc(ctx, (void**)&lay_off, 4 * (size_t)(N / kVz + 1));
-                if (rc == VC_OK && !vec)
-                    rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)N);
-            }
-            if (rc == VC_OK && e == hipSuccess)
-                e = hipMemcpyAsync(dtbl, htbl, sizeof(htbl),
-                                   hipMemcpyHostToDevice, S);
-            if (rc == VC_OK && e == hipSuccess)
-                e = hipMemsetAsync(acc, 0, 8 * kSlots * kAcc, S);
-            if (rc == VC_OK && e == hipSuccess && vec)
-            {
-                const int64_t nzc = v->nz / kVz;
-                const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
-                const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
-                const unsigned gv = (unsigned)etiles;
-                const int64_t nch = N / kVz;
-                euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
-                layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
-                layer_mark_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lay8, lay_off);
-                e = hipGetLastError();
-                uint32_t nlay = 0;
-                if (e == hipSuccess)
-                    e = hipMemsetAsync(lay_off + nch, 0, 4, S);
-                if (e == hipSuccess)
-                {
-                    rc = scan_inplace(ctx, lay_off, nch + 1);
-                    if (rc == VC_OK)
-                        e = hipMemcpyAsync(&nlay, lay_off + nch, 4,
-                                           hipMemcpyDeviceToHost, S);
-                    if (rc == VC_OK && e == hipSuccess)
-                        e = hipStreamSynchronize(S);
-                }
-                if (rc == VC_OK && e == hipSuccess && nlay)
-                    rc = vc_alloc(ctx, (void**)&lab, 4 * (size_t)nlay);
-                if (rc == VC_OK && e == hipSuccess && nlay)
-                {
-                    const unsigned gn = (unsigned)(((int64_t)nlay + kBlock - 1) / kBlock);
-                    layer_init_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, nch, lab);
-                    layer_union_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, D, lab, acc + 7);
-                    layer_count_c<<<gn, kBlock, 0, S>>>(lab, (int64_t)nlay, acc);
-                    e = hipGetLastError();
-                }
-            }
-            else if (rc == VC_OK && e == hipSuccess)
-            {
-                const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
-                              (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
-                euler_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
-                chi_lattice<<<gl, kBlock, 0, S>>>(v->occ, D, acc);
-                const dim3 gv((unsigned)((v->nz + kBlock - 1) / kBlock),
-                              (unsigned)v->ny, (unsigned)v->nx);
-                euler_voxels<<<gv, kBlock, 0, S>>>(v->occ, D, acc);
-                layer_boundary<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
-                layer_init<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lab);
-                layer_union<<<gv, kBlock, 0, S>>>(lab, D, acc + 7);
-                layer_count<<<(unsigned)ntiles, kBlock, 0, S>>>(lab, N, acc);
-                e = hipGetLastError();
-            }
-            unsigned long long hacc[kSlots * kAcc];
-            if (rc == VC_OK && e == hipSuccess)
-                e = hipMemcpyAsync(hacc, acc, sizeof(hacc),
-                                   hipMemcpyDeviceToHost, S);
-            if (rc == VC_OK && e == hipSuccess)
-                e = hipStreamSynchronize(S);
-            if (rc == VC_OK && e == hipSuccess)
-            {
-                long long s[kAcc] = {0};
-                for (int sl = 0; sl < kSlots; ++sl)
-                    for (int q = 0; q < kAcc; ++q)
-                        s[q] += (long long)hacc[sl * kAcc + q];
-                if (s[7] != 0)
-                    rc = vc_fail(ctx, VC_ERR_GUARD,
-                                 "boundary: union-find retry bound reached");
-                vc_euler& E = v->bnd_euler;
-                std::memset(&E, 0, sizeof(E));
-                E.V_bnd = s[0];
-                E.E_bnd = s[1];
-                E.V_int = s[2] / 8;
-                E.E_int = s[3];
-                E.F_int = s[4] / 2;
-                E.T = s[5];
-                E.C = s[6];
-                E.V = E.V_bnd + E.V_int;
-                E.E = E.E_bnd + E.E_int;
-                E.F = nf + E.F_int;
-                E.euler = (int64_t)(int32_t)(E.V - E.E + E.F - E.T);
-                E.chi = s[8] - s[9];
-            }
-            vc_free(acc);
-            vc_free(dtbl);
-            vc_free(bnd);
-            vc_free(lab);
-            vc_free(lay8);
-            vc_free(lay_off);
-            if (rc == VC_OK && e != hipSuccess)
-                rc = vc_fail(ctx, VC_ERR_HIP, "boundary emit/Euler: %s",
-                             hipGetErrorString(e));
-        }
-        vc_free(ts);
-        vc_free(tf);
-        vc_free(cmap);
-        vc_free(ccnt);
-        vc_free(vmask);
+        const int rc = boundary_compute(ctx, v, vec, etiles);
         if (rc != VC_OK)
             return rc;
-        v->bnd_nsites = ns;
-        v->bnd_nquads = nf;
-        v->bnd_done = true;
     }
     *n_sites = v->bnd_nsites;
     *n_quads = v->bnd_nquads;

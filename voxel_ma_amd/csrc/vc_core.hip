@@ -43,6 +43,29 @@ void vc_free(void* p)
         (void)hipFree(p);
 }
 
+int vc_ws_get(vc_ctx* ctx, int slot, size_t bytes, void** p)
+{
+    *p = nullptr;
+    if (slot < 0 || slot >= VC_WS_SLOTS)
+        return vc_fail(ctx, VC_ERR_INVALID, "scratch slot %d", slot);
+    if (ctx->ws_bytes[slot] < bytes || !ctx->ws[slot])
+    {
+        // hipFree waits for the device, so the buffer's last user is done
+        vc_free(ctx->ws[slot]);
+        ctx->ws[slot] = nullptr;
+        ctx->ws_bytes[slot] = 0;
+        // a little headroom, so slowly growing inputs do not reallocate
+        // every call
+        const size_t want = bytes + bytes / 8;
+        const int rc = vc_alloc(ctx, &ctx->ws[slot], want);
+        if (rc != VC_OK)
+            return rc;
+        ctx->ws_bytes[slot] = want;
+    }
+    *p = ctx->ws[slot];
+    return VC_OK;
+}
+
 int64_t vc_level_threshold(float level)
 {
     double l = (double)level;
@@ -136,6 +159,8 @@ int vc_ctx_destroy(vc_ctx* ctx)
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     vc_free(ctx->d_lut);
+    for (int k = 0; k < VC_WS_SLOTS; ++k)
+        vc_free(ctx->ws[k]);
     vc_noalloc_teardown(ctx);
     for (hipEvent_t e : ctx->ev_free)
         (void)hipEventDestroy(e);

@@ -16,6 +16,24 @@
 // ---------------------------------------------------------------- context
 struct vc_sched;
 
+// scratch slots of a context (vc_ws_get); one per buffer of a call that
+// uses them, no two live buffers share a slot
+enum vc_ws_slot
+{
+    VC_WS_BND_TS,     // boundary: per-tile site counts
+    VC_WS_BND_TF,     // boundary: per-tile face counts
+    VC_WS_BND_CCNT,   // boundary: per-chunk counts
+    VC_WS_BND_VMASK,  // boundary: per-voxel decision masks
+    VC_WS_BND_CMAP,   // boundary: corner lattice -> site index
+    VC_WS_BND_ACC,    // boundary: Euler accumulators and cube table
+    VC_WS_BND_MARK,   // boundary: boundary marks
+    VC_WS_BND_LAY8,   // boundary: thin-layer marks
+    VC_WS_BND_LAYOFF, // boundary: thin-layer label offsets
+    VC_WS_BND_LAB,    // boundary: thin-layer labels
+    VC_WS_SCAN,       // chunk totals of the scans
+    VC_WS_SLOTS
+};
+
 struct vc_ctx
 {
     int device = 0;
@@ -39,6 +57,10 @@ struct vc_ctx
     uint8_t* na_scratch = nullptr;
     vc_vol* na_vol = nullptr;
     vc_sched* na_sched = nullptr;
+    // reusable device scratch by slot (vc_ws_get): grown on demand, kept until
+    // the context is destroyed, so repeated calls allocate nothing
+    void* ws[VC_WS_SLOTS] = {};
+    size_t ws_bytes[VC_WS_SLOTS] = {};
 };
 
 // packed voxel coordinate: x << (by+bz) | y << bz | z, fits in 31 bits
@@ -114,6 +136,10 @@ int vc_fail(vc_ctx* ctx, int code, const char* fmt, ...);
 
 int vc_alloc(vc_ctx* ctx, void** p, size_t bytes);
 void vc_free(void* p);
+// device scratch of slot `slot` with at least `bytes` bytes, kept by the
+// context: contents are not preserved when it grows (the old buffer is freed
+// after the device finishes with it); calls on one context run one at a time
+int vc_ws_get(vc_ctx* ctx, int slot, size_t bytes, void** p);
 
 // ---------------------------------------------------------------- schedules
 // Every volume kernel walks a work table: item b of a launch is one tile of
