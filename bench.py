@@ -64,6 +64,13 @@ class HipEvents:
         return out.value
 
 
+# `traffic` is not counted in the bench run itself: hardware counters need a
+# profiler pass of their own, so it is read from the committed summary of
+# such passes over this same workload
+TRAFFIC_SOURCE = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, "
+                  "one run each over this workload, scaled by a copy kernel's reading")
+
+
 def measured_traffic(kernel, n):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (512^3 workload only), else None."""
@@ -783,6 +790,7 @@ def main():
                      "achieved": round(thin_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(thin_gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("thin_tile_sweep", n),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "bytes_per_voxel_pass": round(THIN_BYTES_PER_VOXEL_PASS, 4),
                      "voxel_passes_per_step": int(stats.voxel_passes),
                      "touched_frac": round(stats.voxel_passes / (8.0 * N * sweeps), 4),
@@ -798,7 +806,8 @@ def main():
                          "frac": round(edt_gbs / HBM_PEAK_GBS, 4), "bytes_per_voxel": 21,
                          "traffic": sum(filter(None, (measured_traffic(k, n) for k in
                                                       ("edt_pass_z", "edt_pass_y",
-                                                       "edt_pass_x")))) or None},
+                                                       "edt_pass_x")))) or None,
+                         "traffic_source": TRAFFIC_SOURCE},
     }
     if not (a.no_surrogate or a.profile_steps):
         print("[bench] surrogate-guided pruning", file=sys.stderr, flush=True)

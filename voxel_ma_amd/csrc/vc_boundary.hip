@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(kBlock)
 // [5] T, [6] thin-layer components, [7] union-find guard, [8] / [9] positive
 // / negative terms of the cubical Euler characteristic, spread over 64 slots
 constexpr int kSlots = 64;
-constexpr int kAcc = 10;
+constexpr int kAcc = 11;
 
 __device__ __forceinline__ void block_add(unsigned long long v,
                                           unsigned long long* sh,
@@ -1078,35 +1078,32 @@ __global__ void __launch_bounds__(kBlock)
                __builtin_popcount(l001 | l011 | l101 | l111) + __builtin_popcount(l111);
         // edges from the point along +x, +y, +z (inside the grid): the four
         // voxels around each, in cyclic order
-        const uint32_t lim[3] = {cx < D.nx ? JM : 0u, cy < D.ny ? JM : 0u, 0xFFu};
-        const uint32_t ring[3][4] = {{l100, l110, l111, l101},
-                                     {l010, l011, l111, l110},
-                                     {l001, l101, l111, l011}};
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-        {
-            const uint32_t v0 = ring[ax][0], v1 = ring[ax][1], v2 = ring[ax][2],
-                           v3 = ring[ax][3];
+        auto edge = [&](uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3, uint32_t lim) {
             const uint32_t d0 = v0 ^ v1, d1 = v1 ^ v2, d2 = v2 ^ v3, d3 = v3 ^ v0;
             const uint32_t ch4 = d0 & d1 & d2 & d3;
             const uint32_t ch2 = (d0 | d1 | d2 | d3) & ~ch4;
-            s[1] += __builtin_popcount(ch2 & lim[ax]) + 2 * __builtin_popcount(ch4 & lim[ax]);
-            s[3] += __builtin_popcount(v0 & v1 & v2 & v3 & lim[ax]);
-        }
+            s[1] += __builtin_popcount(ch2 & lim) + 2 * __builtin_popcount(ch4 & lim);
+            s[3] += __builtin_popcount(v0 & v1 & v2 & v3 & lim);
+        };
+        edge(l100, l110, l111, l101, cx < D.nx ? JM : 0u);
+        edge(l010, l011, l111, l110, cy < D.ny ? JM : 0u);
+        edge(l001, l101, l111, l011, 0xFFu);
         // boundary corners (some but not all of the 8 voxels inside): the
         // table of the site mask, whose block starts at voxel 0 where the
         // coordinate is 0
         const uint32_t part = any & ~all;
         if (part)
         {
-            const int bx = cx == 0 ? 1 : 0, by = cy == 0 ? 1 : 0;
+            const bool bx = cx == 0, by = cy == 0;
             uint64_t X = 0;
             uint32_t last = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
             {
                 const int ix = (i >> 2) & 1, iy = (i >> 1) & 1, iz = i & 1;
-                const uint32_t r = w[ix + bx][iy + by];
+                // selects, not a run-time index (which would put w in memory)
+                const uint32_t r = bx ? (by ? w[ix + 1][iy + 1] : w[ix + 1][iy])
+                                      : (by ? w[ix][iy + 1] : w[ix][iy]);
                 const uint32_t si = (((r >> iz) & ~zf) | ((r >> (iz + 1)) & zf)) & JM;
                 X |= (uint64_t)(si & 0xFFu) << (8 * (7 - i));
                 last |= ((si >> 8) & 1u) << (7 - i);
@@ -1136,15 +1133,25 @@ __global__ void __launch_bounds__(kBlock)
                        V(ax, ay + 1, az + 1) & V(ax + 1, ay, az) & V(ax + 1, ay, az + 1) &
                        V(ax + 1, ay + 1, az) & V(ax + 1, ay + 1, az + 1);
             };
+            const uint32_t b000 = B(-1, -1, -1), b001 = B(-1, -1, 0), b010 = B(-1, 0, -1),
+                           b011 = B(-1, 0, 0), b100 = B(0, -1, -1), b101 = B(0, -1, 0),
+                           b110 = B(0, 0, -1), b111 = B(0, 0, 0);
+            // block with offset 0 (true) or -1 (false) per axis: selects of
+            // values, so the windows stay in registers
+            auto blk = [&](bool X0, bool Y0, bool Z0) -> uint32_t {
+                return X0 ? (Y0 ? (Z0 ? b111 : b110) : (Z0 ? b101 : b100))
+                          : (Y0 ? (Z0 ? b011 : b010) : (Z0 ? b001 : b000));
+            };
             // corner slot q: block offset -1, or 0 on its high side or at
             // coordinate 0 (in z only the voxel at z = 0)
 #pragma unroll
             for (int q = 0; q < 8; ++q)
             {
-                const int ax = (((q >> 2) & 1) || cx == 0) ? 0 : -1;
-                const int ay = (((q >> 1) & 1) || cy == 0) ? 0 : -1;
-                const uint32_t full = (q & 1) ? B(ax, ay, 0)
-                                              : ((B(ax, ay, -1) & ~zf) | (B(ax, ay, 0) & zf));
+                const bool X0 = ((q >> 2) & 1) || cx == 0;
+                const bool Y0 = ((q >> 1) & 1) || cy == 0;
+                const uint32_t full = (q & 1) ? blk(X0, Y0, true)
+                                              : ((blk(X0, Y0, false) & ~zf) |
+                                                 (blk(X0, Y0, true) & zf));
                 s[2] += __builtin_popcount(in & full);
             }
         }
@@ -1169,153 +1176,231 @@ __global__ void __launch_bounds__(kBlock)
     bnd8[c] = (uint8_t)(me & ~all6 & 0xFFu);
 }
 
-// ---- thin layer with compact labels (vectorised path): only the layer
-// voxels get a union-find label.  lay8[c] marks the layer voxels of chunk c
-// (inside and within one voxel, 26-wise, of a boundary voxel); their labels
-// are numbered in linear order, lay_off[c] being the exclusive prefix count
-// of the chunks before c, so voxel j of chunk c has label
-// lay_off[c] + popcount(lay8[c] & ((1 << j) - 1)).
+// ---- thin layer, vectorised path: two-level labelling over voxel positions.
+// lay8[c] marks the layer voxels of chunk c (inside and within one voxel,
+// 26-wise, of a boundary voxel).  A tile of kTX x kTY x kTZ chunks (8 x 8 x 32
+// voxels, one workgroup of 256) is labelled in LDS: local ids are positions in the tile (chunk
+// t, bit j -> 8 t + j), runs within a chunk start as trees, and the pairs
+// inside the tile are joined by a lock-free union-find on LDS; every layer
+// voxel's label (its linear voxel index in lab) is then the linear index of
+// its tile-local root.  A second pass joins the pairs that cross tile faces
+// on those labels.  Components = tile-local roots - successful cross-tile
+// links (each link joins two components), so no pass counts roots.
+constexpr int kTX = 8, kTY = 8, kTZ = 4, kTile = kTX * kTY * kTZ;
 
 // 10-bit window of the boundary marks of row (x, y) at chunk z0 (bit b =
-// voxel z0 - 1 + b, outside the grid = 0); the edge bits come from the
-// neighbouring lanes' chunks, as in row_win, so every lane calls it
-__device__ __forceinline__ uint32_t bnd_win_s(const uint8_t* __restrict__ bnd8,
-                                              const Dims& D, int x, int y, int z0,
-                                              bool live)
+// voxel z0 - 1 + b, outside the grid = 0)
+__device__ __forceinline__ uint32_t bnd_win(const uint8_t* __restrict__ bnd8,
+                                            const Dims& D, int x, int y, int z0)
 {
-    const bool in = live && (unsigned)x < (unsigned)D.nx && (unsigned)y < (unsigned)D.ny;
-    const int64_t c = in ? ((int64_t)x * D.ny + y) * (D.nz / kVz) + z0 / kVz : 0;
-    const uint32_t mid = in ? (uint32_t)bnd8[c] : 0u;
-    const int lane = threadIdx.x & 63;
-    const uint32_t prev = __shfl_up(mid, 1u);
-    const uint32_t next = __shfl_down(mid, 1u);
-    uint32_t lo = 0, hi = 0;
-    if (in && z0 > 0)
-        lo = lane > 0 ? prev >> 7 : (uint32_t)bnd8[c - 1] >> 7;
-    if (in && z0 + kVz < D.nz)
-        hi = lane < 63 ? next & 1u : (uint32_t)bnd8[c + 1] & 1u;
-    return lo | (mid << 1) | (hi << 9);
+    if ((unsigned)x >= (unsigned)D.nx || (unsigned)y >= (unsigned)D.ny)
+        return 0;
+    const int64_t c = ((int64_t)x * D.ny + y) * (D.nz / kVz) + z0 / kVz;
+    const uint32_t lo = z0 > 0 ? (uint32_t)bnd8[c - 1] >> 7 : 0u;
+    const uint32_t hi = z0 + kVz < D.nz ? (uint32_t)bnd8[c + 1] & 1u : 0u;
+    return lo | ((uint32_t)bnd8[c] << 1) | (hi << 9);
 }
 
-// layer marks of chunk c and their count (scanned into label offsets)
-__global__ void __launch_bounds__(kBlock)
-    layer_mark_v(const uint8_t* __restrict__ occ,
-                 const uint8_t* __restrict__ bnd8, Dims D,
-                 uint8_t* __restrict__ lay8, uint32_t* __restrict__ lay_cnt)
+__device__ __forceinline__ int32_t lds_find(int32_t* par, int32_t i)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nzc = D.nz / kVz;
-    const bool live = c < (int64_t)D.nx * D.ny * nzc;
-    int x, y, z0;
-    chunk_at(D, live ? c : 0, x, y, z0);
-    uint32_t near = 0; // bit b: a boundary voxel within z0 - 1 + b +- 1
-#pragma unroll
-    for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
+    int32_t p = par[i];
+    while (p != i)
+    {
+        const int32_t g = par[p];
+        if (g == p)
+            break;
+        par[i] = g;
+        i = p;
+        p = g;
+    }
+    return p;
+}
+
+__device__ __forceinline__ void lds_union(int32_t* par, int32_t a, int32_t b)
+{
+    // terminates: every failed CAS is another thread's successful link, and
+    // there are fewer links than tile voxels
+    for (;;)
+    {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b)
+            return;
+        if (a > b)
         {
-            const uint32_t w = bnd_win_s(bnd8, D, x + dx - 1, y + dy - 1, z0, live);
-            near |= w | (w >> 1) | (w << 1);
+            const int32_t t = a;
+            a = b;
+            b = t;
         }
-    if (!live)
-        return;
-    const uint64_t o8 = *reinterpret_cast<const uint64_t*>(
-        occ + ((int64_t)x * D.ny + y) * D.nz + z0);
-    uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < kVz; ++j)
-        m |= (((o8 >> (8 * j)) & 0xFFu) != 0 && ((near >> (j + 1)) & 1u)) ? 1u << j : 0u;
-    lay8[c] = (uint8_t)m;
-    lay_cnt[c] = (uint32_t)__builtin_popcount(m);
-}
-
-__device__ __forceinline__ int32_t lay_id(const uint8_t* __restrict__ lay8,
-                                          const uint32_t* __restrict__ off,
-                                          int64_t c, int j)
-{
-    return (int32_t)(off[c] + (uint32_t)__builtin_popcount(lay8[c] & ((1u << j) - 1u)));
-}
-
-// initial labels: each z-run of layer voxels within a chunk is one tree
-// rooted at its first voxel
-__global__ void __launch_bounds__(kBlock)
-    layer_init_c(const uint8_t* __restrict__ lay8,
-                 const uint32_t* __restrict__ off, int64_t nchunk,
-                 int32_t* __restrict__ lab)
-{
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunk)
-        return;
-    const uint32_t m = lay8[c];
-    if (!m)
-        return;
-    const int32_t base = (int32_t)off[c];
-    int32_t start = -1, k = 0;
-    for (int j = 0; j < kVz; ++j)
-    {
-        if (!((m >> j) & 1u))
-        {
-            start = -1;
-            continue;
-        }
-        if (start < 0)
-            start = base + k;
-        lab[base + k] = start;
-        ++k;
+        if (atomicCAS(&par[b], b, a) == b)
+            return;
     }
 }
 
-// unions on the compact labels: runs are already trees, so z unions happen
-// only across chunks; the x / y union of a voxel is skipped when the voxel
-// before it in z and that voxel's x / y neighbour are both in the layer
-// (the same trees were joined through them)
-__global__ void __launch_bounds__(kBlock)
-    layer_union_c(const uint8_t* __restrict__ lay8,
-                  const uint32_t* __restrict__ off, Dims D, int32_t* lab,
-                  unsigned long long* guard_hits)
+// the x / y pairs of chunks `me` and `nb` (same z) to join: a pair whose
+// predecessor in z is also a pair is already joined through it (runs within
+// a chunk are trees)
+__device__ __forceinline__ uint32_t pair_starts(uint32_t me, uint32_t nb)
 {
+    const uint32_t m = me & nb;
+    return m & ~(m << 1);
+}
+
+__global__ void __launch_bounds__(kTile)
+    layer_tile(const uint8_t* __restrict__ occ, const uint8_t* __restrict__ bnd8,
+               Dims D, int tiles_y, int tiles_z, int64_t n_tiles,
+               uint8_t* __restrict__ lay8, int32_t* __restrict__ lab,
+               unsigned long long* acc)
+{
+    __shared__ int32_t par[kTile * kVz];
+    __shared__ uint8_t m8[kTile];
+    __shared__ unsigned long long sh[kTile / 64];
+    // XCD-mapped: tiles next to each other in z (which read the same lines)
+    // go to one XCD's L2
+    const uint32_t item = vc_xcd_item(blockIdx.x, gridDim.x / 8);
+    if (item >= (uint32_t)n_tiles)
+        return;
+    const int t = threadIdx.x;
+    const int tz = t % kTZ, ty = (t / kTZ) % kTY, tx = t / (kTZ * kTY);
+    int blk = (int)item;
+    const int bz = blk % tiles_z;
+    blk /= tiles_z;
+    const int by = blk % tiles_y, bx = blk / tiles_y;
     const int nzc = D.nz / kVz;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int64_t)D.nx * D.ny * nzc)
-        return;
-    const uint32_t me = lay8[c];
-    if (!me)
-        return;
-    const int64_t row = c / nzc;
-    const int z0 = (int)(c - row * nzc) * kVz;
-    const int x = (int)(row / D.ny);
-    const int y = (int)(row - (int64_t)x * D.ny);
-    const int64_t cx = c + (int64_t)D.ny * nzc, cy = c + nzc;
-    const uint32_t mx = x + 1 < D.nx ? lay8[cx] & me : 0u;
-    const uint32_t my = y + 1 < D.ny ? lay8[cy] & me : 0u;
-    const uint32_t ux = mx & ~(mx << 1), uy = my & ~(my << 1);
-    for (uint32_t u = ux; u; u &= u - 1)
+    const int x = bx * kTX + tx, y = by * kTY + ty, zc = bz * kTZ + tz;
+    const bool in = x < D.nx && y < D.ny && zc < nzc;
+    uint32_t me = 0;
+    if (in)
     {
-        const int j = __builtin_ctz(u);
-        uf_union(lab, lay_id(lay8, off, c, j), lay_id(lay8, off, cx, j), guard_hits);
+        const int z0 = zc * kVz;
+        uint32_t near = 0; // bit b: a boundary voxel within z0 - 1 + b +- 1
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+            {
+                const uint32_t w = bnd_win(bnd8, D, x + dx, y + dy, z0);
+                near |= w | (w >> 1) | (w << 1);
+            }
+        const uint64_t o8 = *reinterpret_cast<const uint64_t*>(
+            occ + ((int64_t)x * D.ny + y) * D.nz + z0);
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+            me |= (((o8 >> (8 * j)) & 0xFFu) != 0 && ((near >> (j + 1)) & 1u)) ? 1u << j : 0u;
+        lay8[((int64_t)x * D.ny + y) * nzc + zc] = (uint8_t)me;
     }
-    for (uint32_t u = uy; u; u &= u - 1)
+    m8[t] = (uint8_t)me;
+    if (me)
     {
-        const int j = __builtin_ctz(u);
-        uf_union(lab, lay_id(lay8, off, c, j), lay_id(lay8, off, cy, j), guard_hits);
+        // runs within the chunk rooted at their first voxel (entries of
+        // voxels outside the layer are never read)
+        int32_t start = t * kVz;
+#pragma unroll
+        for (int j = 0; j < kVz; ++j)
+        {
+            if ((me >> j) & 1u)
+                par[t * kVz + j] = start;
+            else
+                start = t * kVz + j + 1;
+        }
     }
-    if (((me >> 7) & 1u) && z0 + kVz < D.nz && (lay8[c + 1] & 1u))
-        uf_union(lab, lay_id(lay8, off, c, kVz - 1), (int32_t)off[c + 1], guard_hits);
+    __syncthreads();
+    if (me)
+    {
+        if (tz + 1 < kTZ && ((me >> 7) & 1u) && (m8[t + 1] & 1u))
+            lds_union(par, t * kVz + kVz - 1, (t + 1) * kVz);
+        if (tx + 1 < kTX)
+            for (uint32_t u = pair_starts(me, m8[t + kTZ * kTY]); u; u &= u - 1)
+            {
+                const int j = __builtin_ctz(u);
+                lds_union(par, t * kVz + j, (t + kTZ * kTY) * kVz + j);
+            }
+        if (ty + 1 < kTY)
+            for (uint32_t u = pair_starts(me, m8[t + kTZ]); u; u &= u - 1)
+            {
+                const int j = __builtin_ctz(u);
+                lds_union(par, t * kVz + j, (t + kTZ) * kVz + j);
+            }
+    }
+    __syncthreads();
+    unsigned long long roots = 0;
+    if (me)
+    {
+        const int64_t base = ((int64_t)x * D.ny + y) * D.nz + zc * kVz;
+        for (uint32_t u = me; u; u &= u - 1)
+        {
+            const int j = __builtin_ctz(u);
+            const int32_t r = lds_find(par, t * kVz + j);
+            roots += r == t * kVz + j ? 1ull : 0ull;
+            const int rt = r / kVz, rj = r % kVz;
+            const int rx = bx * kTX + rt / (kTZ * kTY), ry = by * kTY + (rt / kTZ) % kTY;
+            const int rz = (bz * kTZ + rt % kTZ) * kVz + rj;
+            lab[base + j] = (int32_t)(((int64_t)rx * D.ny + ry) * D.nz + rz);
+        }
+    }
+    block_add(roots, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
 }
 
-// layer components: roots among the *n_lab compact labels (the count stays
-// on the device: a fixed grid strides over it)
+// union that reports whether it joined two components
+__device__ uint32_t uf_link(int32_t* lab, int32_t a, int32_t b,
+                            unsigned long long* guard_hits)
+{
+    for (int guard = 0; guard < (1 << 20); ++guard)
+    {
+        a = uf_find(lab, a);
+        b = uf_find(lab, b);
+        if (a == b)
+            return 0;
+        if (a > b)
+        {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&lab[b], b, a) == b)
+            return 1;
+    }
+    atomicAdd(guard_hits, 1ull);
+    return 0;
+}
+
+// the pairs that cross a tile face (+x, +y, +z); successful links counted
 __global__ void __launch_bounds__(kBlock)
-    layer_count_c(const int32_t* __restrict__ lab,
-                  const uint32_t* __restrict__ n_lab, unsigned long long* acc)
+    layer_border(const uint8_t* __restrict__ lay8, Dims D, int32_t* lab,
+                 unsigned long long* acc)
 {
     __shared__ unsigned long long sh[kBlock / 64];
-    const int64_t n = *n_lab;
-    unsigned long long r = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        r += lab[i] == (int32_t)i ? 1ull : 0ull;
-    block_add(r, sh, &acc[(blockIdx.x % kSlots) * kAcc + 6]);
+    const int nzc = D.nz / kVz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t me = c < (int64_t)D.nx * D.ny * nzc ? lay8[c] : 0u;
+    unsigned long long links = 0;
+    if (me)
+    {
+        unsigned long long* guard = acc + 7;
+        const int64_t row = c / nzc;
+        const int zc = (int)(c - row * nzc);
+        const int x = (int)(row / D.ny);
+        const int y = (int)(row - (int64_t)x * D.ny);
+        const int32_t p = (int32_t)(row * D.nz + zc * kVz);
+        if ((x + 1) % kTX == 0 && x + 1 < D.nx)
+        {
+            const int32_t sx = D.ny * D.nz;
+            for (uint32_t u = pair_starts(me, lay8[c + (int64_t)D.ny * nzc]); u; u &= u - 1)
+            {
+                const int j = __builtin_ctz(u);
+                links += uf_link(lab, p + j, p + j + sx, guard);
+            }
+        }
+        if ((y + 1) % kTY == 0 && y + 1 < D.ny)
+            for (uint32_t u = pair_starts(me, lay8[c + nzc]); u; u &= u - 1)
+            {
+                const int j = __builtin_ctz(u);
+                links += uf_link(lab, p + j, p + j + D.nz, guard);
+            }
+        if ((zc + 1) % kTZ == 0 && zc + 1 < nzc && ((me >> 7) & 1u) && (lay8[c + 1] & 1u))
+            links += uf_link(lab, p + kVz - 1, p + kVz, guard);
+    }
+    block_add(links, sh, &acc[(blockIdx.x % kSlots) * kAcc + 10]);
 }
 
 // the reference's per-mask surface-component table, derived from first
@@ -1443,13 +1528,11 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
     const int64_t ntiles = (N + kBlock - 1) / kBlock;
     const int64_t nch = vec ? N / kVz : 0;
     const size_t ncorner = (size_t)(v->nx + 1) * (v->ny + 1) * (v->nz + 1);
-    uint32_t *ts, *tf, *cmap, *tmp, *ccnt = nullptr, *vmask = nullptr,
-             *lay_off = nullptr;
+    uint32_t *ts, *tf, *cmap, *tmp, *ccnt = nullptr, *vmask = nullptr;
     unsigned long long* acc;
     uint8_t *bnd, *lay8 = nullptr;
     int32_t* lab;
-    const int64_t tmpw = std::max(scan_tmp_words(etiles + 1),
-                                  vec ? scan_tmp_words(nch + 1) : (int64_t)1);
+    const int64_t tmpw = scan_tmp_words(etiles + 1);
     int rc;
     // tile totals carry one more word: the scan leaves the grand total there
     if ((rc = ws(ctx, VC_WS_BND_TS, 4 * (size_t)(etiles + 1), &ts)) != VC_OK ||
@@ -1458,16 +1541,16 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
         (rc = ws(ctx, VC_WS_SCAN, 4 * (size_t)tmpw, &tmp)) != VC_OK ||
         (rc = ws(ctx, VC_WS_BND_ACC, 8 * kSlots * kAcc + 4 * 256, &acc)) != VC_OK ||
         // boundary marks: one byte per 8-voxel chunk on the vectorised path,
-        // one per voxel otherwise; labels: at most one per voxel
+        // one per voxel otherwise; labels: one per voxel (only those of
+        // layer voxels are written and read)
         (rc = ws(ctx, VC_WS_BND_MARK, (size_t)(vec ? nch : N), &bnd)) != VC_OK ||
         (rc = ws(ctx, VC_WS_BND_LAB, 4 * (size_t)N, &lab)) != VC_OK)
         return rc;
-    // per-chunk counts and per-voxel decision masks, thin-layer marks and
-    // label offsets of the vectorised passes
+    // per-chunk counts, per-voxel decision masks and thin-layer marks of the
+    // vectorised passes
     if (vec && ((rc = ws(ctx, VC_WS_BND_CCNT, 4 * (size_t)nch, &ccnt)) != VC_OK ||
                 (rc = ws(ctx, VC_WS_BND_VMASK, 4 * (size_t)N, &vmask)) != VC_OK ||
-                (rc = ws(ctx, VC_WS_BND_LAY8, (size_t)nch, &lay8)) != VC_OK ||
-                (rc = ws(ctx, VC_WS_BND_LAYOFF, 4 * (size_t)(nch + 1), &lay_off)) != VC_OK))
+                (rc = ws(ctx, VC_WS_BND_LAY8, (size_t)nch, &lay8)) != VC_OK))
         return rc;
     int32_t* dtbl = reinterpret_cast<int32_t*>(acc + kSlots * kAcc);
     int32_t htbl[256];
@@ -1504,22 +1587,15 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
         const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
         const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
         const unsigned gv = (unsigned)etiles;
-        const unsigned gn = (unsigned)std::min<int64_t>(etiles, 2048);
         euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
         layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
-        layer_mark_v<<<gv, kBlock, 0, S>>>(v->occ, bnd, D, lay8, lay_off);
+        const int tiles_x = (v->nx + kTX - 1) / kTX, tiles_y = (v->ny + kTY - 1) / kTY,
+                  tiles_z = (int)((nzc + kTZ - 1) / kTZ);
+        const int64_t n_tiles = (int64_t)tiles_x * tiles_y * tiles_z;
+        layer_tile<<<(unsigned)(8 * ((n_tiles + 7) / 8)), kTile, 0, S>>>(
+            v->occ, bnd, D, tiles_y, tiles_z, n_tiles, lay8, lab, acc);
+        layer_border<<<gv, kBlock, 0, S>>>(lay8, D, lab, acc);
         e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemsetAsync(lay_off + nch, 0, 4, S);
-        if (e == hipSuccess)
-            e = scan_ws(S, lay_off, nch + 1, tmp);
-        if (e == hipSuccess)
-        {
-            layer_init_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, nch, lab);
-            layer_union_c<<<gv, kBlock, 0, S>>>(lay8, lay_off, D, lab, acc + 7);
-            layer_count_c<<<gn, kBlock, 0, S>>>(lab, lay_off + nch, acc);
-            e = hipGetLastError();
-        }
     }
     else if (e == hipSuccess)
     {
@@ -1584,7 +1660,9 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
     E.E_int = sum[3];
     E.F_int = sum[4] / 2;
     E.T = sum[5];
-    E.C = sum[6];
+    // vectorised path: tile-local roots - cross-tile links (slot 10 is 0
+    // otherwise)
+    E.C = sum[6] - sum[10];
     E.V = E.V_bnd + E.V_int;
     E.E = E.E_bnd + E.E_int;
     E.F = nf + E.F_int;

@@ -28,7 +28,6 @@ enum vc_ws_slot
     VC_WS_BND_ACC,    // boundary: Euler accumulators and cube table
     VC_WS_BND_MARK,   // boundary: boundary marks
     VC_WS_BND_LAY8,   // boundary: thin-layer marks
-    VC_WS_BND_LAYOFF, // boundary: thin-layer label offsets
     VC_WS_BND_LAB,    // boundary: thin-layer labels
     VC_WS_SCAN,       // chunk totals of the scans
     VC_WS_SLOTS
