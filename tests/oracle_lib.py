@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# VC_ORACLE_LIBRARY: another build of the oracle (the sanitizer build)
+LIB_PATH = os.environ.get("VC_ORACLE_LIBRARY",
+                          os.path.join(ROOT, "oracle", "_build", "liboracle.so"))
 
 _lib = None
 

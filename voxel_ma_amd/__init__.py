@@ -137,9 +137,13 @@ def library():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libvoxelcore.so is not built (run __graft_entry__.build())")
-    L = ctypes.CDLL(LIB_PATH)
+    # VC_LIBRARY names another build of the C ABI to load instead (the
+    # sanitizer build of the host-only entry points); only the entry points
+    # it has are bound
+    path = os.environ.get("VC_LIBRARY", LIB_PATH)
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built (run __graft_entry__.build())" % path)
+    L = ctypes.CDLL(path)
     P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     PP = ctypes.POINTER(ctypes.c_void_p)
     sig = {
@@ -243,6 +247,8 @@ def library():
         "vc_xform_apply": [P, P, I64, P],
     }
     for name, args in sig.items():
+        if path != LIB_PATH and not hasattr(L, name):
+            continue
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int

@@ -52,6 +52,26 @@ def build_library(verbose=False):
     return LIB
 
 
+# The host-only sources (no device code): also built with AddressSanitizer
+# and UndefinedBehaviorSanitizer into a library of their entry points, which
+# the sanitizer test loads in place of libvoxelcore.so (VC_LIBRARY) with the
+# ASan runtime preloaded
+HOST_ONLY_SOURCES = ["vc_ccthin.cpp", "vc_voro.cpp", "vc_export.cpp"]
+ASAN_DIR = os.path.join(PKG, "_asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libvchost_asan.so")
+
+
+def build_host_sanitized():
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    tmp = ASAN_LIB + ".tmp"
+    _run(["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fPIC", "-shared",
+          "-fsanitize=address,undefined", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I", "/opt/rocm/include", "-I", os.path.join(ROOT, "include")]
+         + [os.path.join(CSRC, s) for s in HOST_ONLY_SOURCES] + ["-o", tmp, "-lpthread"])
+    os.replace(tmp, ASAN_LIB)
+    return ASAN_LIB
+
+
 def build_cli():
     """voroUtility, linked against libvoxelcore.so (rpath = its directory)."""
     tmp = CLI + ".tmp"
@@ -67,6 +87,11 @@ def build_cli():
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+def build_oracle_sanitized():
+    _run(["make", "-C", os.path.join(ROOT, "oracle"), "sanitize"])
+    return os.path.join(ROOT, "oracle", "_build", "asan", "liboracle.so")
 
 
 if __name__ == "__main__":
