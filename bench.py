@@ -532,6 +532,8 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
         "device_equals_host": all(np.array_equal(a, b) for a, b in zip(rch[1:], rcd[1:5])),
         "note": "host arrays in and out; device time includes the adjacency build"}
 
+    out["vol2ma"] = vol2ma_row(crop, side)
+
     # output files: the skeleton of the headline volume as PLY and .r
     import tempfile
     sk = np.argwhere(label_host == 1).astype(np.float32)
@@ -544,6 +546,47 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
                       "ply_ms": round(t_w * 1e3, 1), "ply_MB": round(sz / 1e6, 1),
                       "radii_ms": round(t_r * 1e3, 1)}
     return out
+
+
+def vol2ma_row(crop, side, tt="0.04,0.2"):
+    """`voroUtility -md=vol2ma` (src/voroUtility.cpp:454-525) on the crop as a
+    user runs it: a child process reading the raw volume and writing the full
+    inside part (PLY + .r) and each pruned complex (PLY + .r + .ma).  Wall time
+    of the process (start-up included), the program's own stage times, and a
+    digest of every output file (so runs of different builds can be compared
+    byte for byte)."""
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "voxel_ma_amd", "voroUtility")
+    with tempfile.TemporaryDirectory() as td:
+        raw = os.path.join(td, "crop.raw")
+        crop.tofile(raw)
+        outp = os.path.join(td, "core.ply")
+        cmd = [cli, "-md=vol2ma", raw, outp, "-dims=%d,%d,%d" % crop.shape, "-tt=" + tt,
+               "-fullOrPruned=2", "-outToDotma"]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"error": r.stdout[-2000:]}
+        stages = {}
+        for line in r.stdout.splitlines():
+            if line.startswith("time -> "):
+                k, v = line[len("time -> "):].rsplit(":", 1)
+                stages[k.strip()] = float(v.split()[0])
+        h = hashlib.sha256()
+        files = sorted(f for f in os.listdir(td) if f != "crop.raw")
+        sizes = 0
+        for f in files:
+            with open(os.path.join(td, f), "rb") as fh:
+                b = fh.read()
+            sizes += len(b)
+            h.update(f.encode() + b"\0" + hashlib.sha256(b).digest())
+    return {"workload": "voroUtility -md=vol2ma on the central %d^3 crop, -tt=%s, full + pruned "
+                        "outputs with .ma" % (side, tt),
+            "wall_ms": round(wall * 1e3, 1), "stages_ms": stages, "files": files,
+            "bytes": sizes, "outputs_sha256": h.hexdigest()[:16]}
 
 
 def surrogate_leg(ctx, ev, vol, levels, n, steps, plain_ms, tau=0.01):
@@ -631,6 +674,8 @@ def main():
                     help="skip the batched-solids leg (256 volumes)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched-solids leg (for profiling)")
+    ap.add_argument("--vol2ma-only", action="store_true",
+                    help="only the vol2ma row on the central 128^3 crop of the headline volume")
     ap.add_argument("--boundary-only", action="store_true",
                     help="only the boundary row on the headline volume (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -684,6 +729,14 @@ def main():
         occ_dev.free()
         print(json.dumps({"R3_R4_boundary": boundary_row(ctx, occ_host)}), flush=True)
         ctx.close()
+        return
+    if a.vol2ma_only:
+        occ_dev.free()
+        side = min(n, 128)
+        c0 = (n - side) // 2
+        crop = np.ascontiguousarray(occ_host[c0:c0 + side, c0:c0 + side, c0:c0 + side])
+        ctx.close()
+        print(json.dumps({"vol2ma": vol2ma_row(crop, side)}), flush=True)
         return
 
     vol = ctx.from_device(occ_dev.ptr, (n, n, n))

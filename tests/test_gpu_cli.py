@@ -154,3 +154,33 @@ def test_cli_vol2ma_voxel_core(tmp_path, vc):
         assert _chi(vt, et, tt_) == chi_full, tt
         assert os.path.exists(str(tmp_path / ("ball_thinned%s.r" % tt)))
         assert os.path.exists(str(tmp_path / ("ball_thinned%s.ma" % tt)))
+
+
+def test_cli_vol2ma_resident_equals_host_chain(tmp_path):
+    """-md=vol2ma with the diagram resident on the device (computeVD keeps
+    it there through tagging, merging, extraction and the collapse) writes
+    the same bytes as the chain over host arrays (-hostVoro): the full
+    inside part, each pruned complex, the radii and the .ma files."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(11)
+    f = gaussian_filter(rng.standard_normal((48, 44, 40)), 3.0)
+    occ = (f > np.quantile(f, 0.55)).astype(np.uint8)
+    raw = str(tmp_path / "blobs.raw")
+    occ.tofile(raw)
+    outs = {}
+    for mode in ("device", "host"):
+        d = tmp_path / mode
+        d.mkdir()
+        cmd = [CLI, "-md=vol2ma", raw, str(d / "core.ply"), "-dims=48,44,40",
+               "-tt=0.04,0.2", "-fullOrPruned=2", "-outToDotma"]
+        if mode == "host":
+            cmd.append("-hostVoro")
+        say("cli vol2ma", mode)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[mode] = {p.name: p.read_bytes() for p in sorted(d.iterdir())}
+    assert sorted(outs["device"]) == sorted(outs["host"])
+    assert len(outs["device"]) == 8
+    for name in outs["device"]:
+        assert outs["device"][name] == outs["host"][name], name
+

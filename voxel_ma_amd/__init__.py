@@ -55,6 +55,9 @@ EXPORTS = [
     "vc_voro_tag_points", "vc_voro_measures", "vc_voro_extract_inside",
     "vc_voro_extract_inside_device", "vc_voro_merge_close", "vc_voro_merge_close_device",
     "vc_voro_radii", "vc_voro_part_inside", "vc_voro_from_ft",
+    # include/vc_vdiag.h
+    "vc_vdiag_from_ft", "vc_vdiag_release", "vc_vdiag_info_get", "vc_vdiag_radii",
+    "vc_vdiag_tag", "vc_vdiag_merge_close", "vc_vdiag_get", "vc_vdiag_extract_inside",
     # include/vc_octree.h
     "vc_octree_parse", "vc_octree_free", "vc_octree_value", "vc_volume_from_octree",
     "vc_octree_leaf_centers",
@@ -107,6 +110,14 @@ class SurrogateStats(ctypes.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class VdiagInfo(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int64), ("ne", ctypes.c_int64), ("nf", ctypes.c_int64),
+                ("ns", ctypes.c_int64), ("n_sides", ctypes.c_int64),
+                ("tagged", ctypes.c_int32), ("radii", ctypes.c_int32),
+                ("n_valid", ctypes.c_int64), ("bb_min", ctypes.c_float * 3),
+                ("bb_max", ctypes.c_float * 3)]
 
 
 class CcTopo(ctypes.Structure):
@@ -218,6 +229,16 @@ def library():
         "vc_surrogate_infer": [P, P, P, I, P, P],
         "vc_surrogate_measure": [P, P, P, ctypes.c_float, I, ctypes.POINTER(SurrogateStats)],
         "vc_surrogate_activation": [P, P, I, P, I64],
+        "vc_vdiag_from_ft": [P, P, PP],
+        "vc_vdiag_release": [P, P],
+        "vc_vdiag_info_get": [P, ctypes.POINTER(VdiagInfo)],
+        "vc_vdiag_radii": [P, P],
+        "vc_vdiag_tag": [P, P, P],
+        "vc_vdiag_merge_close": [P, P, ctypes.c_float, I, ctypes.POINTER(I)],
+        "vc_vdiag_get": [P, P, P, P, P, P, P, P, P, P],
+        "vc_vdiag_extract_inside": [P, P, I, P, P, I64, ctypes.POINTER(I64), P, P, I64,
+                                    ctypes.POINTER(I64), P, P, P, I64, ctypes.POINTER(I64),
+                                    PP],
         "vc_octree_parse": [P, ctypes.c_size_t, I, PP, ctypes.POINTER(OctreeInfo)],
         "vc_octree_free": [P],
         "vc_octree_value": [P, I, I, I, ctypes.POINTER(ctypes.c_double)],
@@ -787,6 +808,101 @@ class CellComplex:
     def release(self):
         if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.vc_cc_release(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class VoroDiagram:
+    """The discrete Voronoi diagram of a volume kept on the device
+    (include/vc_vdiag.h): built from the feature transform, given radii,
+    tagged, merged and its inside part extracted without leaving the device."""
+
+    def __init__(self, vol):
+        self.ctx = vol.ctx
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._lib.vc_vdiag_from_ft(self.ctx.handle, vol.handle,
+                                                      ctypes.byref(h)))
+        self.handle = h
+
+    def info(self):
+        i = VdiagInfo()
+        self.ctx.check(self.ctx._lib.vc_vdiag_info_get(self.handle, ctypes.byref(i)))
+        return {"nv": i.nv, "ne": i.ne, "nf": i.nf, "ns": i.ns, "n_sides": i.n_sides,
+                "tagged": i.tagged, "radii": i.radii, "n_valid": i.n_valid,
+                "bb_min": list(i.bb_min), "bb_max": list(i.bb_max)}
+
+    def radii(self):
+        self.ctx.check(self.ctx._lib.vc_vdiag_radii(self.ctx.handle, self.handle))
+        return self
+
+    def tag(self, vol):
+        self.ctx.check(self.ctx._lib.vc_vdiag_tag(self.ctx.handle, self.handle, vol.handle))
+        return self
+
+    def merge_close(self, eps, only_inside):
+        on = ctypes.c_int()
+        self.ctx.check(self.ctx._lib.vc_vdiag_merge_close(self.ctx.handle, self.handle,
+                                                          float(eps), int(bool(only_inside)),
+                                                          ctypes.byref(on)))
+        return bool(on.value)
+
+    def get(self):
+        """dict of host copies: vts (nv, 3), edges (ne, 2), face_off, face_vts,
+        face_sites (nf, 2), sites (ns, 3), radii (or None), valid"""
+        i = self.info()
+        nv, ne, nf, ns, nsd = i["nv"], i["ne"], i["nf"], i["ns"], i["n_sides"]
+        vts = np.zeros((nv + 1, 3), np.float32)
+        e = np.zeros((ne + 1, 2), np.int32)
+        fo = np.zeros(nf + 1, np.int64)
+        fv = np.zeros(nsd + 1, np.int32)
+        fs = np.zeros((nf + 1, 2), np.int32)
+        st = np.zeros((ns + 1, 3), np.float32)
+        r = np.zeros(nv + 1, np.float32)
+        val = np.zeros(nv + 1, np.uint8)
+        self.ctx.check(self.ctx._lib.vc_vdiag_get(
+            self.ctx.handle, self.handle, _p(vts), _p(e), _p(fo), _p(fv), _p(fs), _p(st),
+            _p(r) if i["radii"] else None, _p(val)))
+        return {"vts": vts[:nv], "edges": e[:ne], "face_off": fo, "face_vts": fv[:nsd],
+                "face_sites": fs[:nf], "sites": st[:ns],
+                "radii": r[:nv] if i["radii"] else None, "valid": val[:nv]}
+
+    def extract_inside(self, all_valid=False, keep_complex=False):
+        """(vts_idx, vm, edges (n, 2), em, tris (n, 3), from_fi, fm[, CellComplex
+        handle]) of extractInsideWithMeasure(LAMBDA) on the device"""
+        i = self.info()
+        tcap = max(i["n_sides"] - 2 * i["nf"], 0)
+        vcap, ecap = i["nv"], i["ne"] + 3 * tcap
+        vi = np.zeros(vcap + 1, np.int32)
+        vm = np.zeros(vcap + 1, np.float32)
+        eo = np.zeros((ecap + 1, 2), np.int32)
+        em = np.zeros(ecap + 1, np.float32)
+        tr = np.zeros((tcap + 1, 3), np.int32)
+        ff = np.zeros(tcap + 1, np.int32)
+        fm = np.zeros(tcap + 1, np.float32)
+        n1, n2, n3 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        cc = ctypes.c_void_p()
+        self.ctx.check(self.ctx._lib.vc_vdiag_extract_inside(
+            self.ctx.handle, self.handle, int(bool(all_valid)), _p(vi), _p(vm), vcap,
+            ctypes.byref(n1), _p(eo), _p(em), ecap, ctypes.byref(n2), _p(tr), _p(ff), _p(fm),
+            tcap, ctypes.byref(n3), ctypes.byref(cc) if keep_complex else None))
+        a, b, c = n1.value, n2.value, n3.value
+        out = (vi[:a], vm[:a], eo[:b], em[:b], tr[:c], ff[:c], fm[:c])
+        if keep_complex:
+            h = CellComplex.__new__(CellComplex)
+            h.ctx, h.nv, h.handle = self.ctx, a, cc
+            h.face_off = np.arange(c + 1, dtype=np.int64) * 3
+            h.face_vts = tr[:c].reshape(-1)
+            out = out + (h,)
+        return out
+
+    def release(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            self.ctx._lib.vc_vdiag_release(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):

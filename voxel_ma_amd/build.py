@@ -14,7 +14,7 @@ LIB = os.path.join(PKG, "libvoxelcore.so")
 HIP_SOURCES = ["vc_core.hip", "vc_sched.cpp", "vc_edt.hip", "vc_thin.hip", "vc_batch.cpp",
                "vc_boundary.hip", "vc_synth.hip", "vc_ccthin.cpp", "vc_unet.hip",
                "vc_voro.hip", "vc_voro.cpp", "vc_octree.hip", "vc_export.cpp",
-               "vc_ccthin_dev.hip", "vc_voro_merge.hip", "vc_noalloc.hip"]
+               "vc_ccthin_dev.hip", "vc_voro_merge.hip", "vc_noalloc.hip", "vc_vdiag.hip"]
 HOST = os.path.join(PKG, "host")
 # the drop-in C++ layer (reference class names) linked into the same library
 HOST_SOURCES = ["vc_device.cpp", "geomalgo.cpp", "cellcomplex.cpp", "ccthin.cpp",

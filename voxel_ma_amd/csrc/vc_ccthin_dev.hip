@@ -2097,3 +2097,70 @@ int vc_cc_prune_device(vc_ctx* ctx, const vc_cc_view* cc, const float* em,
 }
 
 } // extern "C"
+
+// ---- a resident complex from device arrays (vc_internal.h)
+
+int vc_cc_create_dev(vc_ctx* ctx, int32_t nv, int32_t ne_in, const int32_t* dE,
+                     int32_t nf, const int64_t* dfo, const int32_t* dfv, int64_t nfv,
+                     vc_cc** out)
+{
+    *out = nullptr;
+    if (nfv >= (int64_t)UINT32_MAX / 2 || 2 * ((int64_t)ne_in + nfv) >= (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID, "vc_cc_create: complex too large");
+    hipStream_t S = ctx->stream;
+    vc_cc* c = new (std::nothrow) vc_cc();
+    if (!c)
+        return VC_ERR_NOMEM;
+    c->ctx = ctx;
+    c->nv = nv;
+    c->nf = nf;
+    c->nfe = nfv;
+    int rc = VC_OK;
+    auto get = [&](Mem& m, auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = m.get(ctx, p, n);
+    };
+    get(c->mem, &c->fo, (size_t)nf + 1);
+    get(c->mem, &c->fv, (size_t)nfv + 1);
+    get(c->mem, &c->fe, (size_t)nfv + 1);
+    get(c->mem, &c->E, 2 * ((size_t)ne_in + nfv) + 2);
+    auto fail = [&](int code) {
+        (void)hipStreamSynchronize(S);
+        delete c;
+        return code;
+    };
+    if (rc != VC_OK)
+        return fail(rc);
+    hipError_t e = hipSuccess;
+    if (nf)
+        e = hipMemcpyAsync(c->fo, dfo, 8 * ((size_t)nf + 1), hipMemcpyDeviceToDevice, S);
+    if (e == hipSuccess && nfv)
+        e = hipMemcpyAsync(c->fv, dfv, 4 * (size_t)nfv, hipMemcpyDeviceToDevice, S);
+    if (e != hipSuccess)
+        return fail(vc_fail(ctx, VC_ERR_HIP, "vc_cc_create: %s", hipGetErrorString(e)));
+    int32_t ne = 0;
+    if ((rc = vc_cc_finalize_dev(ctx, ne_in, dE, nf, c->fo, c->fv, nfv, c->E, c->fe,
+                                 &ne)) != VC_OK)
+        return fail(rc);
+    c->ne = ne;
+    if ((rc = cc_adjacency(ctx, c)) != VC_OK)
+        return fail(rc);
+    *out = c;
+    return VC_OK;
+}
+
+void vc_cc_dview_get(const vc_cc* c, vc_cc_dview* v)
+{
+    v->nv = c->nv;
+    v->ne = c->ne;
+    v->nf = c->nf;
+    v->nfe = c->nfe;
+    v->E = c->E;
+    v->fo = c->fo;
+    v->fv = c->fv;
+    v->fe = c->fe;
+    v->ve_off = c->ve_off;
+    v->ve = c->ve;
+    v->ef_off = c->ef_off;
+    v->ef = c->ef;
+}

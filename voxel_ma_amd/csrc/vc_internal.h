@@ -342,3 +342,97 @@ __device__ __forceinline__ uint32_t vc_xcd_item(uint32_t L, uint32_t per)
 {
     return (L & 7u) * per + (L >> 3);
 }
+
+// ---- the Voronoi-side steps on device arrays (vc_voro.hip,
+// vc_voro_merge.hip) and resident complexes built from device arrays
+// (vc_ccthin_dev.hip): the pieces of the resident voxel-core chain
+
+// the discrete Voronoi diagram of a feature transform in device arrays
+// (vc_alloc'ed; freed by vc_voro_dev_free)
+struct vc_voro_dev
+{
+    int64_t nv = 0, nf = 0, ns = 0;
+    float* vts = nullptr;     // 3 nv
+    int32_t* faces = nullptr; // 4 nf (quads)
+    int32_t* fs = nullptr;    // 2 nf
+    float* sites = nullptr;   // 3 ns
+};
+void vc_voro_dev_free(vc_voro_dev* d);
+// vc_voro_from_ft into device arrays; waits
+int vc_voro_from_ft_dev(vc_ctx* ctx, vc_vol* vol, vc_voro_dev* out);
+// vc_voro_tag_points / vc_voro_radii / vc_voro_measures on device arrays,
+// queued on the context stream (no wait); radii needs an nv-int scratch,
+// measures take the adjacency of a resident complex (uint32 offsets)
+int vc_voro_tag_dev(vc_ctx* ctx, vc_vol* vol, const float* pts, int64_t n,
+                    uint8_t* valid);
+int vc_voro_radii_dev(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
+                      const int64_t* face_off, const int32_t* face_vts,
+                      const float* sites, const int32_t* face_sites, int32_t* last,
+                      float* r);
+int vc_voro_measures_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                         const float* sites, const int32_t* face_sites,
+                         const uint8_t* face_valid, const uint32_t* ve_off,
+                         const int32_t* ve, const uint32_t* ef_off, const int32_t* ef,
+                         float* lam_v, float* lam_e, float* lam_f);
+// vc_voro_extract_inside on device arrays: outputs vc_alloc'ed (freed by
+// vc_voro_xi_free): vi (nv), eo (2 ne), em (ne), tr (3 nt), ff, fm (nt); waits
+struct vc_voro_xi
+{
+    int64_t nv = 0, ne = 0, nt = 0;
+    int32_t* vi = nullptr;
+    int32_t* eo = nullptr;
+    float* em = nullptr;
+    int32_t* tr = nullptr;
+    int32_t* ff = nullptr;
+    float* fm = nullptr;
+};
+void vc_voro_xi_free(vc_voro_xi* x);
+int vc_voro_extract_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                        const int32_t* dE, const int64_t* dfo, const int32_t* dfv,
+                        const uint8_t* dval, const float* dle, const float* dlf,
+                        vc_voro_xi* out);
+// the device path of vc_voro_merge_close_device on device arrays: *merges = 1
+// (nothing written) when some edge joins two vertices within eps; otherwise
+// old_to_new (o2n), the seeds (nvts, n_new), the kept faces' new polygons
+// (nfo / nfv, nf_kept faces, nfv_new ids), remove / keep flags, each face's
+// exclusive kept-face position (fpos, nf + 1) and the sorted new edges
+// (edges, ne_new); vc_alloc'ed, freed by vc_voro_mg_free; waits
+struct vc_voro_mg
+{
+    int32_t n_new = 0, nf_kept = 0;
+    int64_t nfv_new = 0, ne_new = 0;
+    int32_t* o2n = nullptr;
+    int32_t* nvts = nullptr;
+    int64_t* nfo = nullptr;
+    int32_t* nfv = nullptr;
+    uint8_t* keep = nullptr;
+    uint8_t* rem = nullptr;
+    uint32_t* fpos = nullptr;
+    int32_t* edges = nullptr;
+};
+void vc_voro_mg_free(vc_voro_mg* m);
+int vc_voro_merge_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const float* dv,
+                      const int32_t* dE, const int64_t* dfo, const int32_t* dfv,
+                      int64_t nfv, const uint8_t* dval, float eps, int only_inside,
+                      vc_voro_mg* out, int* merges);
+
+struct vc_cc;
+// vc_cc_create from device arrays (copied; the inputs may be freed after)
+int vc_cc_create_dev(vc_ctx* ctx, int32_t nv, int32_t ne_in, const int32_t* dE,
+                     int32_t nf, const int64_t* dfo, const int32_t* dfv, int64_t nfv,
+                     struct vc_cc** out);
+// the device arrays of a resident complex
+struct vc_cc_dview
+{
+    int32_t nv, ne, nf;
+    int64_t nfe;
+    const int32_t* E;
+    const int64_t* fo;
+    const int32_t* fv;
+    const int32_t* fe;
+    const uint32_t* ve_off;
+    const int32_t* ve;
+    const uint32_t* ef_off;
+    const int32_t* ef;
+};
+void vc_cc_dview_get(const struct vc_cc* c, vc_cc_dview* v);

@@ -64,10 +64,12 @@ __global__ void __launch_bounds__(kBlock)
     lam_f[f] = site_dist(sites, face_sites[2 * f], face_sites[2 * f + 1]);
 }
 
+// OffT: int64_t offsets (host arrays) or uint32_t (a resident vc_cc)
+template <typename OffT>
 __global__ void __launch_bounds__(kBlock)
     edge_lambda(const float* __restrict__ lam_f,
                 const uint8_t* __restrict__ face_valid,
-                const int64_t* __restrict__ ef_off,
+                const OffT* __restrict__ ef_off,
                 const int32_t* __restrict__ ef, int32_t ne,
                 float* __restrict__ lam_e)
 {
@@ -75,7 +77,7 @@ __global__ void __launch_bounds__(kBlock)
     if (e >= ne)
         return;
     float m = 0.0f;
-    for (int64_t j = ef_off[e]; j < ef_off[e + 1]; ++j)
+    for (int64_t j = (int64_t)ef_off[e]; j < (int64_t)ef_off[e + 1]; ++j)
     {
         const int32_t f = ef[j];
         if (face_valid[f])
@@ -84,12 +86,13 @@ __global__ void __launch_bounds__(kBlock)
     lam_e[e] = m;
 }
 
+template <typename OffT>
 __global__ void __launch_bounds__(kBlock)
     vertex_lambda(const float* __restrict__ lam_f,
                   const uint8_t* __restrict__ face_valid,
-                  const int64_t* __restrict__ ve_off,
+                  const OffT* __restrict__ ve_off,
                   const int32_t* __restrict__ ve,
-                  const int64_t* __restrict__ ef_off,
+                  const OffT* __restrict__ ef_off,
                   const int32_t* __restrict__ ef, int32_t nv,
                   float* __restrict__ lam_v)
 {
@@ -97,10 +100,10 @@ __global__ void __launch_bounds__(kBlock)
     if (v >= nv)
         return;
     float m = 0.0f;
-    for (int64_t i = ve_off[v]; i < ve_off[v + 1]; ++i)
+    for (int64_t i = (int64_t)ve_off[v]; i < (int64_t)ve_off[v + 1]; ++i)
     {
         const int32_t e = ve[i];
-        for (int64_t j = ef_off[e]; j < ef_off[e + 1]; ++j)
+        for (int64_t j = (int64_t)ef_off[e]; j < (int64_t)ef_off[e + 1]; ++j)
         {
             const int32_t f = ef[j];
             if (face_valid[f])
@@ -598,6 +601,65 @@ __global__ void __launch_bounds__(kBlock) fill_u64(uint64_t* a, int64_t n, uint6
         a[i] = v;
 }
 
+// counting pass of the discrete diagram: faces per tile, used lattice points
+// and sites flagged, all three scanned; tot = faces, vertices, sites
+int ft_counts(vc_ctx* ctx, vc_vol* vol, Scratch& sc, uint32_t** tile_out,
+              uint32_t** lat_out, uint32_t** sid_out, int64_t tot[3])
+{
+    const int64_t N = vol->N;
+    const int64_t L = (int64_t)(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1);
+    // face counts are scanned in uint32 (at most 3 per voxel), vertex ids are
+    // int32
+    if (3 * N >= (int64_t)UINT32_MAX || L >= (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID,
+                       "vc_voro_from_ft: volume too large (%lld voxels)",
+                       (long long)N);
+    hipStream_t S = ctx->stream;
+    const int64_t tiles = (N + kBlock - 1) / kBlock;
+    uint32_t *tile = nullptr, *lat = nullptr, *sid = nullptr;
+    int rc = sc.get(ctx, &tile, (size_t)tiles + 1);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &lat, (size_t)L + 1);
+    if (rc == VC_OK)
+        rc = sc.get(ctx, &sid, (size_t)N + 1);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipMemsetAsync(tile, 0, 4 * ((size_t)tiles + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(lat, 0, 4 * ((size_t)L + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(sid, 0, 4 * ((size_t)N + 1), S));
+    const FtGrid G{vol->occ, vol->ft, vol->nx, vol->ny, vol->nz, vol->pk};
+    vft_count<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, tile, tiles + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, lat, L + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, sid, N + 1)) != VC_OK)
+        return rc;
+    uint32_t t[3];
+    VC_HIP(ctx, hipMemcpy(&t[0], tile + tiles, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&t[1], lat + L, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&t[2], sid + N, 4, hipMemcpyDeviceToHost));
+    tot[0] = t[0];
+    tot[1] = t[1];
+    tot[2] = t[2];
+    if (tot[0] > (int64_t)INT32_MAX)
+        return vc_fail(ctx, VC_ERR_INVALID,
+                       "vc_voro_from_ft: %lld faces exceed int32 ids",
+                       (long long)tot[0]);
+    *tile_out = tile;
+    *lat_out = lat;
+    *sid_out = sid;
+    return VC_OK;
+}
+
+template <typename T>
+int own_alloc(vc_ctx* ctx, T** p, size_t n)
+{
+    void* q = nullptr;
+    const int rc = vc_alloc(ctx, &q, n * sizeof(T) + 16);
+    *p = static_cast<T*>(q);
+    return rc;
+}
+
 } // namespace
 
 // host-side validation of a complex given as arrays (ids in range,
@@ -907,11 +969,10 @@ int vc_voro_extract_inside_device(
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
     Scratch sc;
-    int32_t *dE, *dfv, *dvi, *deo, *dtr, *dff;
+    int32_t *dE, *dfv;
     int64_t* dfo;
     uint8_t* dval;
-    float *dle, *dlf, *dem, *dfm;
-    uint32_t *vpos, *epos, *toff;
+    float *dle, *dlf;
     int rc = VC_OK;
     auto get = [&](auto** p, size_t n) {
         if (rc == VC_OK)
@@ -923,9 +984,6 @@ int vc_voro_extract_inside_device(
     get(&dval, (size_t)nv + 1);
     get(&dle, (size_t)ne + 1);
     get(&dlf, (size_t)nf + 1);
-    get(&vpos, (size_t)nv + 1);
-    get(&epos, (size_t)ne + 1);
-    get(&toff, (size_t)nf + 1);
     if (rc != VC_OK)
         return rc;
     if (ne)
@@ -944,87 +1002,34 @@ int vc_voro_extract_inside_device(
     }
     if (nv)
         VC_HIP(ctx, hipMemcpyAsync(dval, v_valid, (size_t)nv, hipMemcpyHostToDevice, S));
-    VC_HIP(ctx, hipMemsetAsync(vpos, 0, 4 * ((size_t)nv + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(epos, 0, 4 * ((size_t)ne + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(toff, 0, 4 * ((size_t)nf + 1), S));
-    const int64_t mx = std::max<int64_t>(std::max<int64_t>(nv, ne), nf);
-    if (mx)
-        xi_flags<<<blocks_for(mx), kBlock, 0, S>>>(nv, ne, nf, dE, dfo, dfv, dval,
-                                                   vpos, epos, toff);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, vpos, (int64_t)nv + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, epos, (int64_t)ne + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, toff, (int64_t)nf + 1)) != VC_OK)
+    vc_voro_xi x;
+    struct Free
+    {
+        vc_voro_xi* x;
+        ~Free() { vc_voro_xi_free(x); }
+    } fr{&x};
+    if ((rc = vc_voro_extract_dev(ctx, nv, ne, nf, dE, dfo, dfv, dval, dle, dlf, &x)) !=
+        VC_OK)
         return rc;
-    uint32_t cnt[3];
-    VC_HIP(ctx, hipMemcpy(&cnt[0], vpos + nv, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&cnt[1], epos + ne, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&cnt[2], toff + nf, 4, hipMemcpyDeviceToHost));
-    const int64_t NV = cnt[0], NE = cnt[1], NT = cnt[2], NC = 3 * NT;
-    // table: a power of two >= 2 x the keys
-    uint64_t H = 1024;
-    while (H < 2 * (uint64_t)(NE + NC) + 1)
-        H <<= 1;
-    uint64_t* keys;
-    unsigned long long* vals;
-    uint64_t* ckey;
-    uint32_t* keep;
-    get(&dvi, (size_t)NV + 1);
-    get(&deo, 2 * (size_t)(NE + NC) + 2);
-    get(&dem, (size_t)(NE + NC) + 1);
-    get(&dtr, 3 * (size_t)NT + 3);
-    get(&dff, (size_t)NT + 1);
-    get(&dfm, (size_t)NT + 1);
-    get(&keys, (size_t)H);
-    get(&vals, (size_t)H);
-    get(&ckey, (size_t)NC + 1);
-    get(&keep, (size_t)NC + 1);
-    if (rc != VC_OK)
-        return rc;
-    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
-    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
-        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
-    VC_CHECK_LAUNCH(ctx);
-    const int64_t mve = std::max<int64_t>(nv, ne);
-    if (mve)
-        xi_emit_ve<<<blocks_for(mve), kBlock, 0, S>>>(nv, ne, dE, dval, vpos, epos,
-                                                      dle, dvi, deo, dem, keys, vals,
-                                                      H - 1);
-    if (nf)
-        xi_tris<<<blocks_for(nf), kBlock, 0, S>>>(nf, dfo, dfv, toff, vpos, dlf, dtr,
-                                                  dff, dfm, ckey, keys, vals, H - 1);
-    VC_CHECK_LAUNCH(ctx);
-    VC_HIP(ctx, hipMemsetAsync(keep, 0, 4 * ((size_t)NC + 1), S));
-    if (NC)
-        xi_keep<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, keep);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, keep, NC + 1)) != VC_OK)
-        return rc;
-    uint32_t nk = 0;
-    VC_HIP(ctx, hipMemcpy(&nk, keep + NC, 4, hipMemcpyDeviceToHost));
-    if (NC)
-        xi_emit_sides<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keep, vpos, dff, dlf,
-                                                        NE, deo, dem);
-    VC_CHECK_LAUNCH(ctx);
-    *n_vts = NV;
-    *n_edges = NE + nk;
-    *n_tris = NT;
+    *n_vts = x.nv;
+    *n_edges = x.ne;
+    *n_tris = x.nt;
     if ((vts_idx && vts_cap < *n_vts) || (edges_out && edge_cap < *n_edges) ||
         (tris && tri_cap < *n_tris))
         return VC_ERR_CAPACITY;
-    if (vts_idx && NV)
-        VC_HIP(ctx, hipMemcpyAsync(vts_idx, dvi, 4 * (size_t)NV, hipMemcpyDeviceToHost, S));
-    if (edges_out && *n_edges)
-        VC_HIP(ctx, hipMemcpyAsync(edges_out, deo, 8 * (size_t)*n_edges,
+    if (vts_idx && x.nv)
+        VC_HIP(ctx, hipMemcpyAsync(vts_idx, x.vi, 4 * (size_t)x.nv, hipMemcpyDeviceToHost, S));
+    if (edges_out && x.ne)
+        VC_HIP(ctx, hipMemcpyAsync(edges_out, x.eo, 8 * (size_t)x.ne,
                                    hipMemcpyDeviceToHost, S));
-    if (em && *n_edges)
-        VC_HIP(ctx, hipMemcpyAsync(em, dem, 4 * (size_t)*n_edges, hipMemcpyDeviceToHost, S));
-    if (tris && NT)
-        VC_HIP(ctx, hipMemcpyAsync(tris, dtr, 12 * (size_t)NT, hipMemcpyDeviceToHost, S));
-    if (from_fi && NT)
-        VC_HIP(ctx, hipMemcpyAsync(from_fi, dff, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
-    if (fm && NT)
-        VC_HIP(ctx, hipMemcpyAsync(fm, dfm, 4 * (size_t)NT, hipMemcpyDeviceToHost, S));
+    if (em && x.ne)
+        VC_HIP(ctx, hipMemcpyAsync(em, x.em, 4 * (size_t)x.ne, hipMemcpyDeviceToHost, S));
+    if (tris && x.nt)
+        VC_HIP(ctx, hipMemcpyAsync(tris, x.tr, 12 * (size_t)x.nt, hipMemcpyDeviceToHost, S));
+    if (from_fi && x.nt)
+        VC_HIP(ctx, hipMemcpyAsync(from_fi, x.ff, 4 * (size_t)x.nt, hipMemcpyDeviceToHost, S));
+    if (fm && x.nt)
+        VC_HIP(ctx, hipMemcpyAsync(fm, x.fm, 4 * (size_t)x.nt, hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }
@@ -1097,86 +1102,276 @@ int vc_voro_from_ft(vc_ctx* ctx, vc_vol* vol, float* vts, int64_t vts_cap,
         return VC_ERR_INVALID;
     if (!vol->edt_done)
         return vc_fail(ctx, VC_ERR_STATE, "vc_voro_from_ft: run vc_edt first");
-    const int64_t N = vol->N;
-    const int64_t L = (int64_t)(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1);
-    // face counts are scanned in uint32 (at most 3 per voxel), vertex ids are
-    // int32
-    if (3 * N >= (int64_t)UINT32_MAX || L >= (int64_t)INT32_MAX)
-        return vc_fail(ctx, VC_ERR_INVALID,
-                       "vc_voro_from_ft: volume too large (%lld voxels)",
-                       (long long)N);
     VC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t S = ctx->stream;
-    Scratch sc;
-    const int64_t tiles = (N + kBlock - 1) / kBlock;
-    uint32_t *tile = nullptr, *lat = nullptr, *sid = nullptr;
-    int rc = sc.get(ctx, &tile, (size_t)tiles + 1);
-    if (rc == VC_OK)
-        rc = sc.get(ctx, &lat, (size_t)L + 1);
-    if (rc == VC_OK)
-        rc = sc.get(ctx, &sid, (size_t)N + 1);
-    if (rc != VC_OK)
-        return rc;
-    VC_HIP(ctx, hipMemsetAsync(tile, 0, 4 * ((size_t)tiles + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(lat, 0, 4 * ((size_t)L + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(sid, 0, 4 * ((size_t)N + 1), S));
-    const FtGrid G{vol->occ, vol->ft, vol->nx, vol->ny, vol->nz, vol->pk};
-    vft_count<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, tile, tiles + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, lat, L + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, sid, N + 1)) != VC_OK)
-        return rc;
-    uint32_t tot[3];
-    VC_HIP(ctx, hipMemcpy(&tot[0], tile + tiles, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&tot[1], lat + L, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&tot[2], sid + N, 4, hipMemcpyDeviceToHost));
-    const int64_t F = tot[0], V = tot[1], NS = tot[2];
-    if (F > (int64_t)INT32_MAX)
-        return vc_fail(ctx, VC_ERR_INVALID,
-                       "vc_voro_from_ft: %lld faces exceed int32 ids",
-                       (long long)F);
-    *n_faces = F;
-    *n_vts = V;
-    *n_sites = NS;
+    int64_t tot[3];
+    int rc;
+    {
+        Scratch sc;
+        uint32_t *tile, *lat, *sid;
+        if ((rc = ft_counts(ctx, vol, sc, &tile, &lat, &sid, tot)) != VC_OK)
+            return rc;
+    }
+    *n_faces = tot[0];
+    *n_vts = tot[1];
+    *n_sites = tot[2];
     if (!vts || !faces || !face_sites || !sites)
         return VC_OK; // size query
-    if (V > vts_cap || F > face_cap || NS > site_cap)
+    if (tot[1] > vts_cap || tot[0] > face_cap || tot[2] > site_cap)
         return VC_ERR_CAPACITY;
-    int32_t *dfaces = nullptr, *dfs = nullptr;
-    float *dv = nullptr, *ds = nullptr;
-    rc = sc.get(ctx, &dfaces, 4 * (size_t)F);
-    if (rc == VC_OK)
-        rc = sc.get(ctx, &dfs, 2 * (size_t)F);
-    if (rc == VC_OK)
-        rc = sc.get(ctx, &dv, 3 * (size_t)V);
-    if (rc == VC_OK)
-        rc = sc.get(ctx, &ds, 3 * (size_t)NS);
-    if (rc != VC_OK)
-        return rc;
-    vft_emit<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid, dfaces,
-                                                 dfs);
-    VC_CHECK_LAUNCH(ctx);
-    vft_vertices<<<blocks_for(L), kBlock, 0, S>>>(vol->nx, vol->ny, vol->nz,
-                                                   lat, dv);
-    VC_CHECK_LAUNCH(ctx);
-    vft_sites<<<blocks_for(N), kBlock, 0, S>>>(G, N, sid, ds);
-    VC_CHECK_LAUNCH(ctx);
-    if (F > 0)
+    vc_voro_dev d;
+    struct Free
     {
-        VC_HIP(ctx, hipMemcpyAsync(faces, dfaces, 16 * (size_t)F,
+        vc_voro_dev* d;
+        ~Free() { vc_voro_dev_free(d); }
+    } fr{&d};
+    if ((rc = vc_voro_from_ft_dev(ctx, vol, &d)) != VC_OK)
+        return rc;
+    if (d.nf > 0)
+    {
+        VC_HIP(ctx, hipMemcpyAsync(faces, d.faces, 16 * (size_t)d.nf,
                                    hipMemcpyDeviceToHost, S));
-        VC_HIP(ctx, hipMemcpyAsync(face_sites, dfs, 8 * (size_t)F,
+        VC_HIP(ctx, hipMemcpyAsync(face_sites, d.fs, 8 * (size_t)d.nf,
                                    hipMemcpyDeviceToHost, S));
     }
-    if (V > 0)
-        VC_HIP(ctx, hipMemcpyAsync(vts, dv, 12 * (size_t)V,
+    if (d.nv > 0)
+        VC_HIP(ctx, hipMemcpyAsync(vts, d.vts, 12 * (size_t)d.nv,
                                    hipMemcpyDeviceToHost, S));
-    if (NS > 0)
-        VC_HIP(ctx, hipMemcpyAsync(sites, ds, 12 * (size_t)NS,
+    if (d.ns > 0)
+        VC_HIP(ctx, hipMemcpyAsync(sites, d.sites, 12 * (size_t)d.ns,
                                    hipMemcpyDeviceToHost, S));
     VC_HIP(ctx, hipStreamSynchronize(S));
     return VC_OK;
 }
 
 } // extern "C"
+
+// ---- device-array cores (vc_internal.h)
+
+void vc_voro_dev_free(vc_voro_dev* d)
+{
+    vc_free(d->vts);
+    vc_free(d->faces);
+    vc_free(d->fs);
+    vc_free(d->sites);
+    *d = vc_voro_dev{};
+}
+
+int vc_voro_from_ft_dev(vc_ctx* ctx, vc_vol* vol, vc_voro_dev* out)
+{
+    *out = vc_voro_dev{};
+    if (!vol->edt_done)
+        return vc_fail(ctx, VC_ERR_STATE, "vc_voro_from_ft: run vc_edt first");
+    hipStream_t S = ctx->stream;
+    const int64_t N = vol->N;
+    const int64_t L = (int64_t)(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1);
+    Scratch sc;
+    uint32_t *tile, *lat, *sid;
+    int64_t tot[3];
+    int rc = ft_counts(ctx, vol, sc, &tile, &lat, &sid, tot);
+    if (rc != VC_OK)
+        return rc;
+    const int64_t F = tot[0], V = tot[1], NS = tot[2];
+    if ((rc = own_alloc(ctx, &out->faces, 4 * (size_t)F)) != VC_OK ||
+        (rc = own_alloc(ctx, &out->fs, 2 * (size_t)F)) != VC_OK ||
+        (rc = own_alloc(ctx, &out->vts, 3 * (size_t)V)) != VC_OK ||
+        (rc = own_alloc(ctx, &out->sites, 3 * (size_t)NS)) != VC_OK)
+    {
+        vc_voro_dev_free(out);
+        return rc;
+    }
+    out->nf = F;
+    out->nv = V;
+    out->ns = NS;
+    const int64_t tiles = (N + kBlock - 1) / kBlock;
+    const FtGrid G{vol->occ, vol->ft, vol->nx, vol->ny, vol->nz, vol->pk};
+    vft_emit<<<(unsigned)tiles, kBlock, 0, S>>>(G, N, tile, lat, sid, out->faces,
+                                                 out->fs);
+    vft_vertices<<<blocks_for(L), kBlock, 0, S>>>(vol->nx, vol->ny, vol->nz, lat,
+                                                   out->vts);
+    vft_sites<<<blocks_for(N), kBlock, 0, S>>>(G, N, sid, out->sites);
+    const hipError_t e = hipGetLastError();
+    // the scratch (tile / lattice / site flags) is freed on return
+    const hipError_t e2 = hipStreamSynchronize(S);
+    if (e != hipSuccess || e2 != hipSuccess)
+    {
+        vc_voro_dev_free(out);
+        return vc_fail(ctx, VC_ERR_HIP, "vc_voro_from_ft: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    return VC_OK;
+}
+
+int vc_voro_tag_dev(vc_ctx* ctx, vc_vol* vol, const float* pts, int64_t n,
+                    uint8_t* valid)
+{
+    if (n > 0)
+        tag_points<<<blocks_for(n), kBlock, 0, ctx->stream>>>(pts, n, vol->occ, vol->nx,
+                                                               vol->ny, vol->nz, valid);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+int vc_voro_radii_dev(vc_ctx* ctx, int32_t nv, int32_t nf, const float* vts,
+                      const int64_t* face_off, const int32_t* face_vts,
+                      const float* sites, const int32_t* face_sites, int32_t* last,
+                      float* r)
+{
+    hipStream_t S = ctx->stream;
+    if (nv == 0)
+        return VC_OK;
+    VC_HIP(ctx, hipMemsetAsync(last, 0xFF, 4 * (size_t)nv, S));
+    if (nf > 0)
+        face_last<<<blocks_for(nf), kBlock, 0, S>>>(face_off, face_vts, nf, last);
+    vertex_radius<<<blocks_for(nv), kBlock, 0, S>>>(vts, sites, face_sites, last, nv, r);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+int vc_voro_measures_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                         const float* sites, const int32_t* face_sites,
+                         const uint8_t* face_valid, const uint32_t* ve_off,
+                         const int32_t* ve, const uint32_t* ef_off, const int32_t* ef,
+                         float* lam_v, float* lam_e, float* lam_f)
+{
+    hipStream_t S = ctx->stream;
+    if (nf > 0)
+        face_lambda<<<blocks_for(nf), kBlock, 0, S>>>(sites, face_sites, nf, lam_f);
+    if (ne > 0)
+        edge_lambda<<<blocks_for(ne), kBlock, 0, S>>>(lam_f, face_valid, ef_off, ef, ne,
+                                                       lam_e);
+    if (nv > 0 && ne > 0)
+        vertex_lambda<<<blocks_for(nv), kBlock, 0, S>>>(lam_f, face_valid, ve_off, ve,
+                                                         ef_off, ef, nv, lam_v);
+    else if (nv > 0)
+        VC_HIP(ctx, hipMemsetAsync(lam_v, 0, 4 * (size_t)nv, S));
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
+
+void vc_voro_xi_free(vc_voro_xi* x)
+{
+    vc_free(x->vi);
+    vc_free(x->eo);
+    vc_free(x->em);
+    vc_free(x->tr);
+    vc_free(x->ff);
+    vc_free(x->fm);
+    *x = vc_voro_xi{};
+}
+
+int vc_voro_extract_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                        const int32_t* dE, const int64_t* dfo, const int32_t* dfv,
+                        const uint8_t* dval, const float* dle, const float* dlf,
+                        vc_voro_xi* out)
+{
+    *out = vc_voro_xi{};
+    hipStream_t S = ctx->stream;
+    Scratch sc;
+    uint32_t *vpos, *epos, *toff;
+    int rc = VC_OK;
+    auto get = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = sc.get(ctx, p, n);
+    };
+    auto own = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = own_alloc(ctx, p, n);
+    };
+    get(&vpos, (size_t)nv + 1);
+    get(&epos, (size_t)ne + 1);
+    get(&toff, (size_t)nf + 1);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipMemsetAsync(vpos, 0, 4 * ((size_t)nv + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(epos, 0, 4 * ((size_t)ne + 1), S));
+    VC_HIP(ctx, hipMemsetAsync(toff, 0, 4 * ((size_t)nf + 1), S));
+    const int64_t mx = std::max<int64_t>(std::max<int64_t>(nv, ne), nf);
+    if (mx)
+        xi_flags<<<blocks_for(mx), kBlock, 0, S>>>(nv, ne, nf, dE, dfo, dfv, dval,
+                                                   vpos, epos, toff);
+    VC_CHECK_LAUNCH(ctx);
+    if ((rc = vc_scan_u32(ctx, vpos, (int64_t)nv + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, epos, (int64_t)ne + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, toff, (int64_t)nf + 1)) != VC_OK)
+        return rc;
+    uint32_t cnt[3];
+    VC_HIP(ctx, hipMemcpy(&cnt[0], vpos + nv, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&cnt[1], epos + ne, 4, hipMemcpyDeviceToHost));
+    VC_HIP(ctx, hipMemcpy(&cnt[2], toff + nf, 4, hipMemcpyDeviceToHost));
+    const int64_t NV = cnt[0], NE = cnt[1], NT = cnt[2], NC = 3 * NT;
+    // table: a power of two >= 2 x the keys
+    uint64_t H = 1024;
+    while (H < 2 * (uint64_t)(NE + NC) + 1)
+        H <<= 1;
+    uint64_t* keys;
+    unsigned long long* vals;
+    uint64_t* ckey;
+    uint32_t* keep;
+    own(&out->vi, (size_t)NV + 1);
+    own(&out->eo, 2 * (size_t)(NE + NC) + 2);
+    own(&out->em, (size_t)(NE + NC) + 1);
+    own(&out->tr, 3 * (size_t)NT + 3);
+    own(&out->ff, (size_t)NT + 1);
+    own(&out->fm, (size_t)NT + 1);
+    get(&keys, (size_t)H);
+    get(&vals, (size_t)H);
+    get(&ckey, (size_t)NC + 1);
+    get(&keep, (size_t)NC + 1);
+    if (rc != VC_OK)
+    {
+        vc_voro_xi_free(out);
+        return rc;
+    }
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(keys, (int64_t)H, kEmptyKey);
+    fill_u64<<<blocks_for((int64_t)H), kBlock, 0, S>>>(
+        reinterpret_cast<uint64_t*>(vals), (int64_t)H, ~0ull);
+    const int64_t mve = std::max<int64_t>(nv, ne);
+    if (mve)
+        xi_emit_ve<<<blocks_for(mve), kBlock, 0, S>>>(nv, ne, dE, dval, vpos, epos, dle,
+                                                      out->vi, out->eo, out->em, keys,
+                                                      vals, H - 1);
+    if (nf)
+        xi_tris<<<blocks_for(nf), kBlock, 0, S>>>(nf, dfo, dfv, toff, vpos, dlf, out->tr,
+                                                  out->ff, out->fm, ckey, keys, vals,
+                                                  H - 1);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemsetAsync(keep, 0, 4 * ((size_t)NC + 1), S);
+    if (e == hipSuccess && NC)
+    {
+        xi_keep<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keys, vals, H - 1, keep);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess)
+    {
+        (void)hipStreamSynchronize(S);
+        vc_voro_xi_free(out);
+        return vc_fail(ctx, VC_ERR_HIP, "inside extraction: %s", hipGetErrorString(e));
+    }
+    if ((rc = vc_scan_u32(ctx, keep, NC + 1)) != VC_OK)
+    {
+        vc_voro_xi_free(out);
+        return rc;
+    }
+    uint32_t nk = 0;
+    e = hipMemcpy(&nk, keep + NC, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && NC)
+    {
+        xi_emit_sides<<<blocks_for(NC), kBlock, 0, S>>>(NC, ckey, keep, vpos, out->ff,
+                                                        dlf, NE, out->eo, out->em);
+        e = hipGetLastError();
+    }
+    // the scratch is freed on return
+    const hipError_t e2 = hipStreamSynchronize(S);
+    if (e != hipSuccess || e2 != hipSuccess)
+    {
+        vc_voro_xi_free(out);
+        return vc_fail(ctx, VC_ERR_HIP, "inside extraction: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    out->nv = NV;
+    out->ne = NE + nk;
+    out->nt = NT;
+    return VC_OK;
+}

@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "vc_internal.h"
@@ -212,6 +213,170 @@ struct Mem
 
 } // namespace
 
+void vc_voro_mg_free(vc_voro_mg* m)
+{
+    vc_free(m->o2n);
+    vc_free(m->nvts);
+    vc_free(m->nfo);
+    vc_free(m->nfv);
+    vc_free(m->keep);
+    vc_free(m->rem);
+    vc_free(m->fpos);
+    vc_free(m->edges);
+    *m = vc_voro_mg{};
+}
+
+int vc_voro_merge_dev(vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const float* dv,
+                      const int32_t* dE, const int64_t* dfo, const int32_t* dfv,
+                      int64_t nfv, const uint8_t* dval, float eps, int only_inside,
+                      vc_voro_mg* out, int* merges)
+{
+    *out = vc_voro_mg{};
+    *merges = 0;
+    hipStream_t S = ctx->stream;
+    Mem m;
+    int* any;
+    uint32_t *vflag, *kflag, *fsize, *upos;
+    uint64_t *key, *kin, *kout;
+    int rc = VC_OK;
+    auto get = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+            rc = m.get(ctx, p, n);
+    };
+    auto own = [&](auto** p, size_t n) {
+        if (rc == VC_OK)
+        {
+            void* q = nullptr;
+            rc = vc_alloc(ctx, &q, n * sizeof(**p) + 16);
+            *p = static_cast<std::remove_reference_t<decltype(*p)>>(q);
+        }
+    };
+    get(&any, 1);
+    if (rc != VC_OK)
+        return rc;
+    VC_HIP(ctx, hipMemsetAsync(any, 0, sizeof(int), S));
+    if (ne)
+        close_edges<<<nblk(ne), kB, 0, S>>>(dv, dE, ne, eps * eps, any);
+    VC_CHECK_LAUNCH(ctx);
+    int hany = 0;
+    VC_HIP(ctx, hipMemcpyAsync(&hany, any, sizeof(int), hipMemcpyDeviceToHost, S));
+    VC_HIP(ctx, hipStreamSynchronize(S));
+    if (hany)
+    {
+        // some BFS absorbs a neighbour: the merge depends on the visit order
+        *merges = 1;
+        return VC_OK;
+    }
+    const size_t NK = (size_t)ne + nf;
+    own(&out->o2n, (size_t)nv + 1);
+    own(&out->nvts, (size_t)nv + 1);
+    own(&out->nfo, (size_t)nf + 2);
+    own(&out->nfv, (size_t)nfv + 1);
+    own(&out->keep, (size_t)ne + 1);
+    own(&out->rem, (size_t)nf + 1);
+    own(&out->fpos, (size_t)nf + 1);
+    get(&vflag, (size_t)nv + 1);
+    get(&kflag, NK + 1);
+    get(&fsize, (size_t)nf + 1);
+    get(&key, NK + 1);
+    get(&kin, NK + 1);
+    get(&kout, NK + 1);
+    get(&upos, NK + 1);
+    auto fail = [&](int code) {
+        (void)hipStreamSynchronize(S);
+        vc_voro_mg_free(out);
+        return code;
+    };
+    if (rc != VC_OK)
+        return fail(rc);
+    hipError_t e = hipSuccess;
+    auto hip = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    hip(hipMemsetAsync(vflag, 0, 4 * ((size_t)nv + 1), S));
+    hip(hipMemsetAsync(kflag, 0, 4 * (NK + 1), S));
+    hip(hipMemsetAsync(fsize, 0, 4 * ((size_t)nf + 1), S));
+    hip(hipMemsetAsync(out->fpos, 0, 4 * ((size_t)nf + 1), S));
+    hip(hipMemsetAsync(out->nfo, 0, 8, S));
+    if (nv)
+        elig<<<nblk(nv), kB, 0, S>>>(nv, dval, only_inside, vflag);
+    hip(hipGetLastError());
+    if (e != hipSuccess)
+        return fail(vc_fail(ctx, VC_ERR_HIP, "merge: %s", hipGetErrorString(e)));
+    if ((rc = vc_scan_u32(ctx, vflag, (int64_t)nv + 1)) != VC_OK)
+        return fail(rc);
+    if (nv)
+        relabel<<<nblk(nv), kB, 0, S>>>(nv, vflag, out->o2n, out->nvts);
+    if (ne)
+        edge_keys<<<nblk(ne), kB, 0, S>>>(dE, ne, dval, only_inside, out->o2n, out->keep,
+                                           key, kflag);
+    if (nf)
+        face_pass<<<nblk(nf), kB, 0, S>>>(dfo, dfv, nf, ne, dval, only_inside, out->o2n,
+                                           out->rem, fsize, out->fpos, key, kflag);
+    hip(hipGetLastError());
+    if (e != hipSuccess)
+        return fail(vc_fail(ctx, VC_ERR_HIP, "merge: %s", hipGetErrorString(e)));
+    if ((rc = vc_scan_u32(ctx, fsize, (int64_t)nf + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, out->fpos, (int64_t)nf + 1)) != VC_OK ||
+        (rc = vc_scan_u32(ctx, kflag, (int64_t)NK + 1)) != VC_OK)
+        return fail(rc);
+    if (nf)
+        face_write<<<nblk(nf), kB, 0, S>>>(dfo, dfv, nf, out->o2n, fsize, out->fpos,
+                                           out->nfo, out->nfv);
+    if (NK)
+        gather_keys<<<nblk((int64_t)NK), kB, 0, S>>>((int64_t)NK, key, kflag, kin);
+    hip(hipGetLastError());
+    uint32_t cnt[3] = {0, 0, 0}; // new vertices, kept faces, keys
+    hip(hipMemcpy(&cnt[0], vflag + nv, 4, hipMemcpyDeviceToHost));
+    hip(hipMemcpy(&cnt[1], out->fpos + nf, 4, hipMemcpyDeviceToHost));
+    hip(hipMemcpy(&cnt[2], kflag + NK, 4, hipMemcpyDeviceToHost));
+    int64_t nfv_new = 0;
+    hip(hipMemcpy(&nfv_new, out->nfo + cnt[1], 8, hipMemcpyDeviceToHost));
+    if (e != hipSuccess)
+        return fail(vc_fail(ctx, VC_ERR_HIP, "merge: %s", hipGetErrorString(e)));
+    const int64_t nk = cnt[2];
+    int64_t nuniq = 0;
+    if (nk > 0)
+    {
+        size_t tmp_bytes = 0;
+        hip(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, kin, kout, (int)nk, 0,
+                                              64, S));
+        void* tmp = nullptr;
+        if (e == hipSuccess &&
+            (rc = m.get(ctx, reinterpret_cast<uint8_t**>(&tmp), tmp_bytes)) != VC_OK)
+            return fail(rc);
+        hip(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, kin, kout, (int)nk, 0, 64,
+                                              S));
+        hip(hipMemsetAsync(upos + nk, 0, 4, S));
+        if (e == hipSuccess)
+            uniq_flags<<<nblk(nk), kB, 0, S>>>(nk, kout, upos);
+        hip(hipGetLastError());
+        if (e != hipSuccess)
+            return fail(vc_fail(ctx, VC_ERR_HIP, "merge: %s", hipGetErrorString(e)));
+        if ((rc = vc_scan_u32(ctx, upos, nk + 1)) != VC_OK)
+            return fail(rc);
+        uint32_t u = 0;
+        hip(hipMemcpy(&u, upos + nk, 4, hipMemcpyDeviceToHost));
+        nuniq = u;
+    }
+    own(&out->edges, 2 * (size_t)nuniq + 2);
+    if (rc != VC_OK)
+        return fail(rc);
+    if (nuniq && e == hipSuccess)
+        uniq_write<<<nblk(nk), kB, 0, S>>>(nk, kout, upos, out->edges);
+    hip(hipGetLastError());
+    // the scratch is freed on return
+    hip(hipStreamSynchronize(S));
+    if (e != hipSuccess)
+        return fail(vc_fail(ctx, VC_ERR_HIP, "merge: %s", hipGetErrorString(e)));
+    out->n_new = (int32_t)cnt[0];
+    out->nf_kept = (int32_t)cnt[1];
+    out->nfv_new = nfv_new;
+    out->ne_new = nuniq;
+    return VC_OK;
+}
+
 extern "C" int vc_voro_merge_close_device(
     vc_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const float* vts,
     const int32_t* edges, const int64_t* face_off, const int32_t* face_vts,
@@ -246,60 +411,25 @@ extern "C" int vc_voro_merge_close_device(
     hipStream_t S = ctx->stream;
     Mem m;
     float* dv;
-    int32_t *dE, *dfv, *o2n, *nvts, *nfv2, *dne;
-    int64_t *dfo, *nfo;
-    uint8_t *dval, *keep, *rem;
-    uint32_t *vflag, *kflag, *fsize, *fkeep, *upos;
-    uint64_t *key, *kin, *kout;
-    int* any;
+    int32_t *dE, *dfv;
+    int64_t* dfo;
+    uint8_t* dval;
     int rc = VC_OK;
     auto get = [&](auto** p, size_t n) {
         if (rc == VC_OK)
             rc = m.get(ctx, p, n);
     };
-    const size_t NK = (size_t)ne + nf;
     get(&dv, 3 * (size_t)nv + 3);
     get(&dE, 2 * (size_t)ne + 2);
     get(&dfo, (size_t)nf + 1);
     get(&dfv, (size_t)nfv + 1);
     get(&dval, (size_t)nv + 1);
-    get(&any, 1);
     if (rc != VC_OK)
         return rc;
     if (nv)
         VC_HIP(ctx, hipMemcpyAsync(dv, vts, 12 * (size_t)nv, hipMemcpyHostToDevice, S));
     if (ne)
         VC_HIP(ctx, hipMemcpyAsync(dE, edges, 8 * (size_t)ne, hipMemcpyHostToDevice, S));
-    VC_HIP(ctx, hipMemsetAsync(any, 0, sizeof(int), S));
-    if (ne)
-        close_edges<<<nblk(ne), kB, 0, S>>>(dv, dE, ne, eps * eps, any);
-    VC_CHECK_LAUNCH(ctx);
-    int hany = 0;
-    VC_HIP(ctx, hipMemcpyAsync(&hany, any, sizeof(int), hipMemcpyDeviceToHost, S));
-    VC_HIP(ctx, hipStreamSynchronize(S));
-    if (hany)
-        // some BFS absorbs a neighbour: the merge depends on the visit order
-        return vc_voro_merge_close(nv, ne, nf, vts, edges, face_off, face_vts,
-                                   ve_off, ve, v_valid, eps, only_inside,
-                                   old_to_new, new_vts, n_new_vts, new_face_off,
-                                   new_face_vts, remove_face, new_edges,
-                                   edge_cap, n_new_edges, keep_edge);
-    get(&o2n, (size_t)nv + 1);
-    get(&nvts, (size_t)nv + 1);
-    get(&nfo, (size_t)nf + 2);
-    get(&nfv2, (size_t)nfv + 1);
-    get(&keep, (size_t)ne + 1);
-    get(&rem, (size_t)nf + 1);
-    get(&vflag, (size_t)nv + 1);
-    get(&kflag, NK + 1);
-    get(&fsize, (size_t)nf + 1);
-    get(&fkeep, (size_t)nf + 1);
-    get(&key, NK + 1);
-    get(&kin, NK + 1);
-    get(&kout, NK + 1);
-    get(&upos, NK + 1);
-    if (rc != VC_OK)
-        return rc;
     if (nf)
     {
         VC_HIP(ctx, hipMemcpyAsync(dfo, face_off, 8 * ((size_t)nf + 1),
@@ -310,95 +440,52 @@ extern "C" int vc_voro_merge_close_device(
     }
     if (only_inside && nv)
         VC_HIP(ctx, hipMemcpyAsync(dval, v_valid, (size_t)nv, hipMemcpyHostToDevice, S));
-    VC_HIP(ctx, hipMemsetAsync(vflag, 0, 4 * ((size_t)nv + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(kflag, 0, 4 * (NK + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(fsize, 0, 4 * ((size_t)nf + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(fkeep, 0, 4 * ((size_t)nf + 1), S));
-    VC_HIP(ctx, hipMemsetAsync(nfo, 0, 8, S));
-    if (nv)
-        elig<<<nblk(nv), kB, 0, S>>>(nv, dval, only_inside, vflag);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, vflag, (int64_t)nv + 1)) != VC_OK)
-        return rc;
-    if (nv)
-        relabel<<<nblk(nv), kB, 0, S>>>(nv, vflag, o2n, nvts);
-    if (ne)
-        edge_keys<<<nblk(ne), kB, 0, S>>>(dE, ne, dval, only_inside, o2n, keep, key,
-                                           kflag);
-    if (nf)
-        face_pass<<<nblk(nf), kB, 0, S>>>(dfo, dfv, nf, ne, dval, only_inside, o2n,
-                                           rem, fsize, fkeep, key, kflag);
-    VC_CHECK_LAUNCH(ctx);
-    if ((rc = vc_scan_u32(ctx, fsize, (int64_t)nf + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, fkeep, (int64_t)nf + 1)) != VC_OK ||
-        (rc = vc_scan_u32(ctx, kflag, (int64_t)NK + 1)) != VC_OK)
-        return rc;
-    if (nf)
-        face_write<<<nblk(nf), kB, 0, S>>>(dfo, dfv, nf, o2n, fsize, fkeep, nfo, nfv2);
-    if (NK)
-        gather_keys<<<nblk((int64_t)NK), kB, 0, S>>>((int64_t)NK, key, kflag, kin);
-    VC_CHECK_LAUNCH(ctx);
-    uint32_t cnt[3] = {0, 0, 0}; // new vertices, kept faces, keys
-    VC_HIP(ctx, hipMemcpy(&cnt[0], vflag + nv, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&cnt[1], fkeep + nf, 4, hipMemcpyDeviceToHost));
-    VC_HIP(ctx, hipMemcpy(&cnt[2], kflag + NK, 4, hipMemcpyDeviceToHost));
-    const int64_t nk = cnt[2];
-    int64_t nuniq = 0;
-    if (nk > 0)
+    vc_voro_mg g;
+    struct Free
     {
-        size_t tmp_bytes = 0;
-        VC_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, kin, kout,
-                                                      (int)nk, 0, 64, S));
-        void* tmp = nullptr;
-        if ((rc = m.get(ctx, reinterpret_cast<uint8_t**>(&tmp), tmp_bytes)) != VC_OK)
-            return rc;
-        VC_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, kin, kout,
-                                                      (int)nk, 0, 64, S));
-        VC_HIP(ctx, hipMemsetAsync(upos + nk, 0, 4, S));
-        uniq_flags<<<nblk(nk), kB, 0, S>>>(nk, kout, upos);
-        VC_CHECK_LAUNCH(ctx);
-        if ((rc = vc_scan_u32(ctx, upos, nk + 1)) != VC_OK)
-            return rc;
-        uint32_t u = 0;
-        VC_HIP(ctx, hipMemcpy(&u, upos + nk, 4, hipMemcpyDeviceToHost));
-        nuniq = u;
-    }
-    *n_new_vts = (int32_t)cnt[0];
-    *n_new_edges = nuniq;
+        vc_voro_mg* g;
+        ~Free() { vc_voro_mg_free(g); }
+    } fr{&g};
+    int merges = 0;
+    if ((rc = vc_voro_merge_dev(ctx, nv, ne, nf, dv, dE, dfo, dfv, nfv, dval, eps,
+                                only_inside, &g, &merges)) != VC_OK)
+        return rc;
+    if (merges)
+        // some BFS absorbs a neighbour: the merge depends on the visit order
+        return vc_voro_merge_close(nv, ne, nf, vts, edges, face_off, face_vts,
+                                   ve_off, ve, v_valid, eps, only_inside,
+                                   old_to_new, new_vts, n_new_vts, new_face_off,
+                                   new_face_vts, remove_face, new_edges,
+                                   edge_cap, n_new_edges, keep_edge);
+    *n_new_vts = g.n_new;
+    *n_new_edges = g.ne_new;
     if (nv)
     {
-        VC_HIP(ctx, hipMemcpyAsync(old_to_new, o2n, 4 * (size_t)nv,
+        VC_HIP(ctx, hipMemcpyAsync(old_to_new, g.o2n, 4 * (size_t)nv,
                                    hipMemcpyDeviceToHost, S));
-        if (cnt[0])
-            VC_HIP(ctx, hipMemcpyAsync(new_vts, nvts, 4 * (size_t)cnt[0],
+        if (g.n_new)
+            VC_HIP(ctx, hipMemcpyAsync(new_vts, g.nvts, 4 * (size_t)g.n_new,
                                        hipMemcpyDeviceToHost, S));
     }
     if (ne)
-        VC_HIP(ctx, hipMemcpyAsync(keep_edge, keep, (size_t)ne, hipMemcpyDeviceToHost, S));
+        VC_HIP(ctx, hipMemcpyAsync(keep_edge, g.keep, (size_t)ne, hipMemcpyDeviceToHost, S));
     if (nf)
     {
-        VC_HIP(ctx, hipMemcpyAsync(remove_face, rem, (size_t)nf, hipMemcpyDeviceToHost, S));
-        VC_HIP(ctx, hipMemcpyAsync(new_face_off, nfo, 8 * ((size_t)cnt[1] + 1),
+        VC_HIP(ctx, hipMemcpyAsync(remove_face, g.rem, (size_t)nf, hipMemcpyDeviceToHost, S));
+        VC_HIP(ctx, hipMemcpyAsync(new_face_off, g.nfo, 8 * ((size_t)g.nf_kept + 1),
                                    hipMemcpyDeviceToHost, S));
-        VC_HIP(ctx, hipStreamSynchronize(S));
-        const int64_t nfv_new = new_face_off[cnt[1]];
-        if (nfv_new)
-            VC_HIP(ctx, hipMemcpyAsync(new_face_vts, nfv2, 4 * (size_t)nfv_new,
+        if (g.nfv_new)
+            VC_HIP(ctx, hipMemcpyAsync(new_face_vts, g.nfv, 4 * (size_t)g.nfv_new,
                                        hipMemcpyDeviceToHost, S));
     }
-    if (new_edges && nuniq)
+    if (new_edges && g.ne_new)
     {
-        if (edge_cap < nuniq)
+        if (edge_cap < g.ne_new)
         {
             VC_HIP(ctx, hipStreamSynchronize(S));
             return VC_ERR_CAPACITY;
         }
-        int32_t* dout;
-        if ((rc = m.get(ctx, &dout, 2 * (size_t)nuniq)) != VC_OK)
-            return rc;
-        uniq_write<<<nblk(nk), kB, 0, S>>>(nk, kout, upos, dout);
-        VC_CHECK_LAUNCH(ctx);
-        VC_HIP(ctx, hipMemcpyAsync(new_edges, dout, 8 * (size_t)nuniq,
+        VC_HIP(ctx, hipMemcpyAsync(new_edges, g.edges, 8 * (size_t)g.ne_new,
                                    hipMemcpyDeviceToHost, S));
     }
     VC_HIP(ctx, hipStreamSynchronize(S));

@@ -11,6 +11,8 @@ cellcomplex::cellcomplex(const std::vector<point>& vts,
 {
     m_vts = vts;
     m_edges = edges;
+    m_fv.reserve(3 * faces.size());
+    m_face_off.reserve(faces.size() + 1);
     for (const auto& t : faces)
         appendFace(t);
     if (perform_finalize)
@@ -26,7 +28,11 @@ void cellcomplex::appendFace(const std::vector<int>& f)
 
 void cellcomplex::appendFace(const uTriFace& t)
 {
-    appendFace(std::vector<int>{(int)t[0], (int)t[1], (int)t[2]});
+    m_fv.push_back((int32_t)t[0]);
+    m_fv.push_back((int32_t)t[1]);
+    m_fv.push_back((int32_t)t[2]);
+    m_face_off.push_back((int64_t)m_fv.size());
+    invalidate();
 }
 
 std::vector<int32_t> cellcomplex::edgeArray() const
@@ -51,10 +57,20 @@ void cellcomplex::finalize()
     vcdev::check(vc_cc_create(ctx, nv, (int32_t)m_edges.size(), ein.data(), nf,
                               m_face_off.data(), m_fv.data(), &h),
                  "vc_cc_create");
-    m_dev = std::shared_ptr<vc_cc>(h, [ctx](vc_cc* p) { vc_cc_release(ctx, p); });
-    int32_t ne = 0;
+    adoptFinalized(std::shared_ptr<vc_cc>(h, [ctx](vc_cc* p) { vc_cc_release(ctx, p); }));
+}
+
+void cellcomplex::adoptFinalized(std::shared_ptr<vc_cc> dev)
+{
+    vc_ctx* ctx = vcdev::context();
+    const int32_t nv = (int32_t)m_vts.size(), nf = (int32_t)numFaces();
+    vc_cc* h = dev.get();
+    int32_t dnv = 0, dnf = 0, ne = 0;
     int64_t ns = 0;
-    vcdev::check(vc_cc_sizes(h, nullptr, &ne, nullptr, &ns), "vc_cc_sizes");
+    vcdev::check(vc_cc_sizes(h, &dnv, &ne, &dnf, &ns), "vc_cc_sizes");
+    if (dnv != nv || dnf != nf || ns != m_face_off.back())
+        throw std::runtime_error("cellcomplex: the device complex has other sizes");
+    m_dev = std::move(dev);
     std::vector<int32_t> eout(2 * (size_t)ne + 2);
     m_fe.assign((size_t)ns, 0);
     vcdev::check(vc_cc_get(ctx, h, eout.data(), m_fe.data()), "vc_cc_get");

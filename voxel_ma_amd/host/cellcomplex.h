@@ -61,6 +61,10 @@ public:
     // the finalized complex on the device (finalize() first); shared by
     // copies of this complex
     const std::shared_ptr<vc_cc>& device() const { return m_dev; }
+    // finalize with a complex already finalized on the device from this
+    // complex's vertices, edges and faces (the inside part extracted on the
+    // device), instead of creating it again
+    void adoptFinalized(std::shared_ptr<vc_cc> dev);
 
     // flat views for the C ABI
     const std::vector<int64_t>& faceOffsets() const { return m_face_off; }

@@ -135,7 +135,7 @@ ExportErrCode writeInsideVoroToDotMA(const VoroInfo& voro, const char* filename)
     std::vector<float> radii;
     for (int i : vi)
     {
-        vts.push_back(voro.geom().getVert(i));
+        vts.push_back(voro.vertex(i));
         radii.push_back(i < (int)r.size() ? r[i] : 0.0f);
     }
     return writeToDotma(filename, vts, edges, tris, radii);
@@ -181,7 +181,7 @@ ExportErrCode writeRadiiToFile(const VoroInfo& voro, const char* r_file,
     std::vector<float> v(3 * sel.size() + 3), r(sel.size() + 1);
     for (size_t k = 0; k < sel.size(); ++k)
     {
-        const point& p = voro.geom().getVert(sel[k]);
+        const point& p = voro.vertex(sel[k]);
         for (int a = 0; a < 3; ++a)
             v[3 * k + a] = p[a];
         r[k] = R[sel[k]];

@@ -6,6 +6,7 @@
 #include <sstream>
 
 #include "../../include/vc_export.h"
+#include "../../include/vc_vdiag.h"
 #include "../../include/vc_voro.h"
 #include "geomalgo.h"
 #include "vc_device.h"
@@ -101,6 +102,33 @@ void computeVD(const std::shared_ptr<Volume3DScalar>& vol, VoroInfo& voro)
         ~Release() { vc_volume_release(c, v); }
     } release{ctx, v};
     vcdev::check(vc_edt(ctx, v, 0), "vc_edt");
+    // the diagram stays on the device; VoroInfo copies parts of it to the
+    // host only where a caller asks for them
+    vc_vdiag* d = nullptr;
+    vcdev::check(vc_vdiag_from_ft(ctx, v, &d), "vc_vdiag_from_ft");
+    voro.setDevice(std::shared_ptr<vc_vdiag>(d, [ctx](vc_vdiag* p) { vc_vdiag_release(ctx, p); }));
+    vc_vdiag_info in;
+    vcdev::check(vc_vdiag_info_get(d, &in), "vc_vdiag_info_get");
+    std::cout << "discrete voronoi diagram: " << in.nv << " vts, " << in.nf
+              << " faces, " << in.ns << " sites" << std::endl;
+    voro.computeRadii();
+}
+
+void computeVDHost(const std::shared_ptr<Volume3DScalar>& vol, VoroInfo& voro)
+{
+    vc_ctx* ctx = vcdev::context();
+    const std::vector<uint8_t> occ = vol->occupancy();
+    vc_vol* v = nullptr;
+    vcdev::check(vc_volume_upload(ctx, occ.data(), vol->getSizeX(),
+                                  vol->getSizeY(), vol->getSizeZ(), &v),
+                 "vc_volume_upload");
+    struct Release
+    {
+        vc_ctx* c;
+        vc_vol* v;
+        ~Release() { vc_volume_release(c, v); }
+    } release{ctx, v};
+    vcdev::check(vc_edt(ctx, v, 0), "vc_edt");
     int64_t nv = 0, nf = 0, ns = 0;
     vcdev::check(vc_voro_from_ft(ctx, v, nullptr, 0, &nv, nullptr, nullptr, 0,
                                  &nf, nullptr, 0, &ns),
@@ -145,12 +173,12 @@ bool preprocessVoro(VoroInfo& voro, const std::shared_ptr<Volume3DScalar>& vol,
     const int max_len =
         std::max(std::max(vol->getSizeX(), vol->getSizeY()), vol->getSizeZ());
     const float eps = voro.getInsidePartSize() * (1.0 / max_len) * 0.000001;
-    std::cout << "before merging voro size: " << voro.geom().numVts() << "/"
-              << voro.geom().numEdges() << "/" << voro.geom().numFaces()
+    std::cout << "before merging voro size: " << voro.numVts() << "/"
+              << voro.numEdges() << "/" << voro.numFaces()
               << std::endl;
     voro.mergeCloseVts(eps, true);
-    std::cout << "after merging voro size: " << voro.geom().numVts() << "/"
-              << voro.geom().numEdges() << "/" << voro.geom().numFaces()
+    std::cout << "after merging voro size: " << voro.numVts() << "/"
+              << voro.numEdges() << "/" << voro.numFaces()
               << std::endl;
     if (need_euler)
     {
@@ -226,8 +254,8 @@ ExportErrCode exportInsideVoroMesh(VoroInfo& voro,
     voro.extractInsideWithMeasure(MeasureForMA::LAMBDA, vts, edges, tris,
                                   from_fi, vm, em, fm);
     std::cout << "inside part extracted. # vts/edges/faces: "
-              << voro.geom().numVts() << "/" << voro.geom().numEdges() << "/"
-              << voro.geom().numFaces() << std::endl;
+              << voro.numVts() << "/" << voro.numEdges() << "/"
+              << voro.numFaces() << std::endl;
     if (vts.empty())
     {
         std::cout << "IVD: 0 vts; IO skipped." << std::endl;
@@ -254,7 +282,12 @@ ExportErrCode exportInsideVoroMesh(VoroInfo& voro,
         return ExportErrCode::SUCCESS;
 
     CellComplexThinning ccthin;
-    cellcomplex cc(vts_trans, edges, tris);
+    cellcomplex cc(vts_trans, edges, tris, false);
+    // the extraction on the device already finalized this complex there
+    if (auto dev = voro.takeInsideComplex())
+        cc.adoptFinalized(std::move(dev));
+    else
+        cc.finalize();
     ccthin.setup(&cc);
     ccthin.assignElementValues(vm, em, fm);
     ccthin.preprocess();
@@ -317,16 +350,21 @@ ExportErrCode exportInsideVoroMesh(VoroInfo& voro,
             }
             // all triangle sides, deduplicated, ascending (the reference
             // appends an unordered_set, whose order is implementation-defined)
-            std::set<std::pair<int, int>> sides;
+            // (as packed (min, max) keys, sorted and made unique)
+            std::vector<uint64_t> sides;
+            sides.reserve(3 * tris.size());
             for (const auto& f : tris)
                 for (int k = 0; k < 3; ++k)
                 {
-                    const int a = (int)f[k], b = (int)f[(k + 1) % 3];
-                    sides.insert({std::min(a, b), std::max(a, b)});
+                    const uint32_t a = f[k], b = f[(k + 1) % 3];
+                    sides.push_back((uint64_t)std::min(a, b) << 32 | std::max(a, b));
                 }
+            std::sort(sides.begin(), sides.end());
+            sides.erase(std::unique(sides.begin(), sides.end()), sides.end());
             std::vector<ivec2> all_edges = edges;
-            for (const auto& e : sides)
-                all_edges.push_back(ivec2(e.first, e.second));
+            all_edges.reserve(edges.size() + sides.size());
+            for (const uint64_t k : sides)
+                all_edges.push_back(ivec2((int)(k >> 32), (int)(k & 0xFFFFFFFFu)));
             writeToDotma(thin_file.substr(0, thin_file.find_last_of('.')) + ".ma",
                          vts_trans, all_edges, tris, radii);
         }

@@ -35,6 +35,9 @@ void thinVolume(const std::shared_ptr<Volume3DScalar>& vol,
 // place of TetGen's diagram of the boundary sites; radii computed as the
 // reference's diagram loader does
 void computeVD(const std::shared_ptr<Volume3DScalar>& vol, VoroInfo& voro);
+// computeVD with the diagram copied to the host (VoroInfo::setGeometry), so
+// the later steps run on host arrays (each through its own device call)
+void computeVDHost(const std::shared_ptr<Volume3DScalar>& vol, VoroInfo& voro);
 
 // src/highlevelalgo.cpp:530-583: tag the vertices, merge those closer than
 // insideSize / max axis * 1e-6 (generateMeasure is a no-op on the

@@ -15,7 +15,10 @@
 //                                                  feature transform (in
 //                                                  place of TetGen's), tagged,
 //                                                  merged, its inside part
-//                                                  FIFO-pruned per threshold
+//                                                  FIFO-pruned per threshold;
+//                                                  the diagram stays on the
+//                                                  device (-hostVoro: copied
+//                                                  to host arrays between steps)
 //   -md=topo -meshToCheck=<m.off>                  closed pockets of a mesh
 //
 // Volumes: .mrc (modes 0, 1, 2, 6), .sof / .sog octrees, or raw uint8 with
@@ -176,7 +179,8 @@ int usage()
                  "       voroUtility -md=vol2skel <in> <out_prefix> "
                  "[-tt=0.01,0.02,0.04] [-outToDotma] [-dims=...]\n"
                  "       voroUtility -md=vol2ma <in> <out.ply|out.off> "
-                 "[-tt=0.04] [-fullOrPruned=0|1|2] [-outToDotma] [-needEuler]\n"
+                 "[-tt=0.04] [-fullOrPruned=0|1|2] [-outToDotma] [-needEuler] "
+                 "[-hostVoro]\n"
                  "       voroUtility -md=topo -meshToCheck=<mesh.off>\n";
     return 1;
 }
@@ -336,7 +340,11 @@ int main(int argc, char** argv)
                 flags.count("fullOrPruned") ? std::stoi(flags["fullOrPruned"]) : 0;
             const auto t0 = Clock::now();
             voxelvoro::VoroInfo voro;
-            voxelvoro::computeVD(vol, voro);
+            // -hostVoro: the diagram on host arrays between the steps
+            if (flag_on(flags, "hostVoro", false))
+                voxelvoro::computeVDHost(vol, voro);
+            else
+                voxelvoro::computeVD(vol, voro);
             std::cout << "time -> discrete voronoi diagram: " << ms_since(t0)
                       << " ms\n";
             if (!voxelvoro::preprocessVoro(voro, vol, need_euler))

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <stdexcept>
 
+#include "../../include/vc_cc.h"
+#include "../../include/vc_vdiag.h"
 #include "../../include/vc_voro.h"
 #include "vc_device.h"
 
@@ -29,6 +31,8 @@ void VoroInfo::setGeometry(const std::vector<point>& vts,
 {
     if (face_sites.size() != faces.size())
         throw std::runtime_error("VoroInfo: one site pair per face");
+    m_dev.reset();
+    drop_device_copies();
     m_geom = cellcomplex(vts, {}, {}, false);
     for (const auto& f : faces)
         m_geom.appendFace(f);
@@ -42,6 +46,185 @@ void VoroInfo::setGeometry(const std::vector<point>& vts,
     m_r_per_v.clear();
     m_r_valid = false;
     recompute_flags();
+}
+
+void VoroInfo::setDevice(std::shared_ptr<vc_vdiag> d)
+{
+    m_dev = std::move(d);
+    drop_device_copies();
+    m_geom = cellcomplex();
+    m_face_sites.clear();
+    m_site_positions.clear();
+    m_vts_valid.clear();
+    m_is_finite_v.clear();
+    m_edge_valid.clear();
+    m_face_valid.clear();
+    m_tagged = false;
+    m_only_inside = -1;
+    m_bb_valid = false;
+    m_r_per_v.clear();
+    m_r_valid = false;
+}
+
+namespace
+{
+vc_vdiag_info dev_info(vc_vdiag* d)
+{
+    vc_vdiag_info in;
+    vcdev::check(vc_vdiag_info_get(d, &in), "vc_vdiag_info_get");
+    return in;
+}
+} // namespace
+
+void VoroInfo::drop_device_copies() const
+{
+    m_dev_vts.clear();
+    m_got_vts = m_got_fs = m_got_sites = m_got_radii = false;
+    m_inside_cc.reset();
+}
+
+void VoroInfo::own_device()
+{
+    // another VoroInfo shares the resident diagram: keep it unchanged for
+    // that one and continue on the host
+    if (m_dev && m_dev.use_count() > 1)
+        materialize();
+}
+
+void VoroInfo::materialize()
+{
+    if (!m_dev)
+        return;
+    vc_ctx* ctx = vcdev::context();
+    const vc_vdiag_info in = dev_info(m_dev.get());
+    std::vector<float> vts(3 * (size_t)in.nv + 3), sites(3 * (size_t)in.ns + 3),
+        radii((size_t)in.nv + 1);
+    std::vector<int32_t> edges(2 * (size_t)in.ne + 2), fv((size_t)in.n_sides + 1),
+        fs(2 * (size_t)in.nf + 2);
+    std::vector<int64_t> fo((size_t)in.nf + 1);
+    std::vector<uint8_t> valid((size_t)in.nv + 1);
+    vcdev::check(vc_vdiag_get(ctx, m_dev.get(), vts.data(), edges.data(), fo.data(),
+                              fv.data(), fs.data(), sites.data(),
+                              in.radii ? radii.data() : nullptr, valid.data()),
+                 "vc_vdiag_get");
+    std::vector<point> pv((size_t)in.nv);
+    for (int64_t i = 0; i < in.nv; ++i)
+        pv[i] = point(vts[3 * i], vts[3 * i + 1], vts[3 * i + 2]);
+    std::vector<ivec2> pe((size_t)in.ne);
+    for (int64_t e = 0; e < in.ne; ++e)
+        pe[e] = ivec2(edges[2 * e], edges[2 * e + 1]);
+    // the finalized edges come first, so finalizing again keeps the numbering
+    cellcomplex g(pv, pe, {}, false);
+    for (int64_t f = 0; f < in.nf; ++f)
+        g.appendFace(std::vector<int>(fv.begin() + fo[f], fv.begin() + fo[f + 1]));
+    g.finalize();
+    m_geom = std::move(g);
+    m_face_sites.resize((size_t)in.nf);
+    for (int64_t f = 0; f < in.nf; ++f)
+        m_face_sites[f] = ivec2(fs[2 * f], fs[2 * f + 1]);
+    m_site_positions.resize((size_t)in.ns);
+    for (int64_t i = 0; i < in.ns; ++i)
+        m_site_positions[i] = point(sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]);
+    m_is_finite_v.assign((size_t)in.nv, true);
+    m_vts_valid.resize((size_t)in.nv);
+    for (int64_t i = 0; i < in.nv; ++i)
+        m_vts_valid[i] = valid[i] != 0;
+    if (in.radii)
+        m_r_per_v.assign(radii.begin(), radii.begin() + in.nv);
+    m_dev.reset();
+    drop_device_copies();
+    recompute_flags();
+}
+
+size_t VoroInfo::numVts() const
+{
+    return m_dev ? (size_t)dev_info(m_dev.get()).nv : m_geom.numVts();
+}
+size_t VoroInfo::numEdges() const
+{
+    return m_dev ? (size_t)dev_info(m_dev.get()).ne : m_geom.numEdges();
+}
+size_t VoroInfo::numFaces() const
+{
+    return m_dev ? (size_t)dev_info(m_dev.get()).nf : m_geom.numFaces();
+}
+
+const point& VoroInfo::vertex(int vi) const
+{
+    if (!m_dev)
+        return m_geom.getVert(vi);
+    if (!m_got_vts)
+    {
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        std::vector<float> v(3 * (size_t)in.nv + 3);
+        vcdev::check(vc_vdiag_get(vcdev::context(), m_dev.get(), v.data(), nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr),
+                     "vc_vdiag_get");
+        m_dev_vts.resize((size_t)in.nv);
+        for (int64_t i = 0; i < in.nv; ++i)
+            m_dev_vts[i] = point(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        m_got_vts = true;
+    }
+    return m_dev_vts[vi];
+}
+
+const std::vector<ivec2>& VoroInfo::faceSites() const
+{
+    if (m_dev && !m_got_fs)
+    {
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        std::vector<int32_t> fs(2 * (size_t)in.nf + 2);
+        vcdev::check(vc_vdiag_get(vcdev::context(), m_dev.get(), nullptr, nullptr, nullptr,
+                                  nullptr, fs.data(), nullptr, nullptr, nullptr),
+                     "vc_vdiag_get");
+        auto& out = const_cast<std::vector<ivec2>&>(m_face_sites);
+        out.resize((size_t)in.nf);
+        for (int64_t f = 0; f < in.nf; ++f)
+            out[f] = ivec2(fs[2 * f], fs[2 * f + 1]);
+        m_got_fs = true;
+    }
+    return m_face_sites;
+}
+
+const std::vector<point>& VoroInfo::getSitesPosition() const
+{
+    if (m_dev && !m_got_sites)
+    {
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        std::vector<float> s(3 * (size_t)in.ns + 3);
+        vcdev::check(vc_vdiag_get(vcdev::context(), m_dev.get(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, s.data(), nullptr, nullptr),
+                     "vc_vdiag_get");
+        auto& out = const_cast<std::vector<point>&>(m_site_positions);
+        out.resize((size_t)in.ns);
+        for (int64_t i = 0; i < in.ns; ++i)
+            out[i] = point(s[3 * i], s[3 * i + 1], s[3 * i + 2]);
+        m_got_sites = true;
+    }
+    return m_site_positions;
+}
+
+const std::vector<float>& VoroInfo::getRadii() const
+{
+    if (m_dev && m_r_valid && !m_got_radii)
+    {
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        auto& out = const_cast<std::vector<float>&>(m_r_per_v);
+        out.assign((size_t)in.nv + 1, 0.0f);
+        vcdev::check(vc_vdiag_get(vcdev::context(), m_dev.get(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, out.data(), nullptr),
+                     "vc_vdiag_get");
+        out.resize((size_t)in.nv);
+        m_got_radii = true;
+    }
+    return m_r_per_v;
+}
+
+std::shared_ptr<vc_cc> VoroInfo::takeInsideComplex() const
+{
+    std::shared_ptr<vc_cc> r;
+    r.swap(m_inside_cc);
+    return r;
 }
 
 namespace
@@ -65,6 +248,17 @@ void face_csr(const cellcomplex& g, std::vector<int64_t>& foff,
 
 void VoroInfo::computeRadii()
 {
+    if (m_dev)
+    {
+        own_device();
+    }
+    if (m_dev)
+    {
+        vcdev::check(vc_vdiag_radii(vcdev::context(), m_dev.get()), "vc_vdiag_radii");
+        m_r_valid = true;
+        m_got_radii = false;
+        return;
+    }
     const int nv = (int)m_geom.numVts(), nf = (int)m_geom.numFaces();
     std::vector<float> vts(3 * (size_t)nv + 3), sites(3 * m_site_positions.size() + 3);
     for (int v = 0; v < nv; ++v)
@@ -95,6 +289,7 @@ void VoroInfo::getPartInside(std::vector<int>& vts_indices,
                              std::vector<ivec2>& edges,
                              std::vector<uTriFace>& tri_faces) const
 {
+    ensure_host();
     const int nv = (int)m_geom.numVts(), nf = (int)m_geom.numFaces();
     std::vector<int64_t> foff;
     std::vector<int32_t> fvts;
@@ -123,14 +318,17 @@ void VoroInfo::getPartInside(std::vector<int>& vts_indices,
 
 bool VoroInfo::isVertexValid(int vi) const
 {
+    ensure_host();
     return onlyHasInside() || (m_vts_valid[vi] && m_is_finite_v[vi]);
 }
 bool VoroInfo::isEdgeValid(int ei) const
 {
+    ensure_host();
     return onlyHasInside() || m_edge_valid[ei];
 }
 bool VoroInfo::isFaceValid(int fi) const
 {
+    ensure_host();
     return onlyHasInside() || m_face_valid[fi];
 }
 
@@ -173,6 +371,29 @@ bool VoroInfo::tagVtsUsingUniformVol(const std::shared_ptr<Volume3DScalar>& vol)
     vcdev::check(vc_volume_upload(ctx, occ.data(), vol->getSizeX(),
                                   vol->getSizeY(), vol->getSizeZ(), &dv),
                  "vc_volume_upload");
+    own_device();
+    if (m_dev)
+    {
+        const int rc = vc_vdiag_tag(ctx, m_dev.get(), dv);
+        vc_volume_release(ctx, dv);
+        vcdev::check(rc, "vc_vdiag_tag");
+        m_tagged = true;
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        // the valid vertices' box (every vertex is finite here)
+        m_bb_valid = onlyHasInside() ? in.nv > 0 : in.n_valid > 0;
+        if (onlyHasInside() && in.nv > 0 && in.n_valid != in.nv)
+        {
+            // every vertex counts as valid: the box of all of them
+            materialize();
+            return tagVtsUsingUniformVol(vol);
+        }
+        for (int a = 0; a < 3; ++a)
+        {
+            m_bb_min[a] = in.bb_min[a];
+            m_bb_max[a] = in.bb_max[a];
+        }
+        return true;
+    }
     const size_t n = m_geom.numVts();
     std::vector<float> pts(3 * n);
     for (size_t i = 0; i < n; ++i)
@@ -209,7 +430,8 @@ bool VoroInfo::tagVtsUsingUniformVol(const std::shared_ptr<Volume3DScalar>& vol)
 
 bool VoroInfo::isFaceFinite(int fi) const
 {
-    if (onlyHasInside())
+    // a resident diagram is the discrete one, whose vertices are all finite
+    if (onlyHasInside() || m_dev)
         return true;
     std::vector<int> poly;
     m_geom.getFaceVRep(fi, poly);
@@ -232,12 +454,25 @@ float VoroInfo::getInsidePartSize() const
 void VoroInfo::getValidVts(std::vector<int>& idx) const
 {
     idx.clear();
+    if (m_dev)
+    {
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        std::vector<uint8_t> valid((size_t)in.nv + 1);
+        vcdev::check(vc_vdiag_get(vcdev::context(), m_dev.get(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, valid.data()),
+                     "vc_vdiag_get");
+        for (int64_t v = 0; v < in.nv; ++v)
+            if (onlyHasInside() || valid[v])
+                idx.push_back((int)v);
+        return;
+    }
     for (size_t v = 0; v < m_geom.numVts(); ++v)
         if (isVertexValid((int)v))
             idx.push_back((int)v);
 }
 void VoroInfo::getValidEdges(std::vector<int>& idx) const
 {
+    ensure_host();
     idx.clear();
     for (size_t e = 0; e < m_geom.numEdges(); ++e)
         if (isEdgeValid((int)e))
@@ -245,6 +480,7 @@ void VoroInfo::getValidEdges(std::vector<int>& idx) const
 }
 void VoroInfo::getValidFaces(std::vector<int>& idx) const
 {
+    ensure_host();
     idx.clear();
     for (size_t f = 0; f < m_geom.numFaces(); ++f)
         if (isFaceValid((int)f))
@@ -254,6 +490,7 @@ void VoroInfo::getValidFaces(std::vector<int>& idx) const
 void VoroInfo::all_measures(std::vector<float>& lv, std::vector<float>& le,
                             std::vector<float>& lf) const
 {
+    ensure_host();
     const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
               nf = (int)m_geom.numFaces();
     std::vector<int64_t> ve_off(nv + 1, 0), ef_off(ne + 1, 0);
@@ -347,6 +584,43 @@ void VoroInfo::extractInsideWithMeasure(
     faces_msure.clear();
     if (tp != MeasureForMA::LAMBDA)
         return;
+    if (m_dev)
+    {
+        // on the resident diagram; the output's finalized complex is kept
+        // for the collapse (takeInsideComplex)
+        const vc_vdiag_info in = dev_info(m_dev.get());
+        // faces have >= 3 vertices: sum(size - 2) triangles
+        const int64_t tcap = std::max<int64_t>(in.n_sides - 2 * in.nf, 0);
+        const int64_t vcap = in.nv, ecap = in.ne + 3 * tcap;
+        std::unique_ptr<int32_t[]> vi(new int32_t[vcap + 1]), eo(new int32_t[2 * ecap + 2]),
+            tr(new int32_t[3 * tcap + 3]), ff(new int32_t[tcap + 1]);
+        std::unique_ptr<float[]> vm(new float[vcap + 1]), em(new float[ecap + 1]),
+            fm(new float[tcap + 1]);
+        int64_t n1 = 0, n2 = 0, n3 = 0;
+        vc_cc* inside = nullptr;
+        vc_ctx* ctx = vcdev::context();
+        vcdev::check(vc_vdiag_extract_inside(ctx, m_dev.get(), onlyHasInside() ? 1 : 0,
+                                             vi.get(), vm.get(), vcap, &n1, eo.get(),
+                                             em.get(), ecap, &n2, tr.get(), ff.get(),
+                                             fm.get(), tcap, &n3, &inside),
+                     "vc_vdiag_extract_inside");
+        m_inside_cc = std::shared_ptr<vc_cc>(inside, [ctx](vc_cc* p) { vc_cc_release(ctx, p); });
+        out_vts.reserve(n1);
+        vts_msure.assign(vm.get(), vm.get() + n1);
+        for (int64_t i = 0; i < n1; ++i)
+            out_vts.push_back(vertex(vi[i]));
+        out_edges.reserve(n2);
+        for (int64_t i = 0; i < n2; ++i)
+            out_edges.push_back(ivec2(eo[2 * i], eo[2 * i + 1]));
+        edges_msure.assign(em.get(), em.get() + n2);
+        out_tris.reserve(n3);
+        for (int64_t i = 0; i < n3; ++i)
+            out_tris.push_back(uTriFace((unsigned)tr[3 * i], (unsigned)tr[3 * i + 1],
+                                        (unsigned)tr[3 * i + 2]));
+        from_fi.assign(ff.get(), ff.get() + n3);
+        faces_msure.assign(fm.get(), fm.get() + n3);
+        return;
+    }
     std::vector<float> lv, le, lf;
     all_measures(lv, le, lf);
     const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
@@ -400,6 +674,23 @@ void VoroInfo::extractInsideWithMeasure(
 
 void VoroInfo::mergeCloseVts(float eps, bool only_inside)
 {
+    own_device();
+    if (m_dev)
+    {
+        int on_device = 0;
+        vcdev::check(vc_vdiag_merge_close(vcdev::context(), m_dev.get(), eps,
+                                          only_inside ? 1 : 0, &on_device),
+                     "vc_vdiag_merge_close");
+        if (on_device)
+        {
+            // the resident diagram is the merged one: seeds keep their
+            // validity and radius (the valid part only, with only_inside)
+            drop_device_copies();
+            return;
+        }
+        // vertices merge: the order-defined BFS on the host
+        materialize();
+    }
     const int nv = (int)m_geom.numVts(), ne = (int)m_geom.numEdges(),
               nf = (int)m_geom.numFaces();
     std::vector<float> vts(3 * (size_t)nv + 3);
@@ -490,6 +781,7 @@ void VoroInfo::mergeCloseVts(float eps, bool only_inside)
 
 void VoroInfo::computeEulerChar(eulerchar& ec, bool do_prune_first) const
 {
+    ensure_host();
     int V, E, F, C;
     if (do_prune_first)
     {

@@ -3,9 +3,11 @@
 // tagging, lambda measures, inside extraction with measures, close-vertex
 // merging.  The diagram is given as data (setGeometry: vertices, face
 // polygons, the two sites of every face, site positions, finite flags), not
-// built from TetGen output.  Tagging and measures run on the device
-// (include/vc_voro.h); extraction and merging on the host, in the
-// reference's order.
+// built from TetGen output, or kept resident on the device (setDevice, as
+// computeVD does: include/vc_vdiag.h), where the steps of vol2ma run without
+// copying it; the host geometry is then filled in from the device on the
+// first call that needs it.  Tagging, measures, extraction and the
+// nothing-merges case of merging run on the device (include/vc_voro.h).
 #pragma once
 
 #include <memory>
@@ -14,6 +16,9 @@
 #include "Volume3DScalar.h"
 #include "cellcomplex.h"
 #include "commondefs.h"
+
+struct vc_vdiag;
+struct vc_cc;
 
 namespace MeasureForMA
 {
@@ -40,9 +45,26 @@ public:
                      const std::vector<ivec2>& face_sites,
                      const std::vector<point>& sites,
                      const std::vector<bool>& finite_v = {});
-    const cellcomplex& geom() const { return m_geom; }
-    const std::vector<ivec2>& faceSites() const { return m_face_sites; }
-    const std::vector<point>& getSites() const { return m_site_positions; }
+    const cellcomplex& geom() const
+    {
+        ensure_host();
+        return m_geom;
+    }
+    const std::vector<ivec2>& faceSites() const;
+    const std::vector<point>& getSites() const { return getSitesPosition(); }
+
+    // the diagram resident on the device (computeVD); copies of a VoroInfo
+    // share it until one of them changes it
+    void setDevice(std::shared_ptr<vc_vdiag> d);
+    bool onDevice() const { return (bool)m_dev; }
+    size_t numVts() const;
+    size_t numEdges() const;
+    size_t numFaces() const;
+    // position of vertex vi (the resident diagram's, fetched once)
+    const point& vertex(int vi) const;
+    // the finalized complex of the output of the last
+    // extractInsideWithMeasure on the device, handed over once (else null)
+    std::shared_ptr<vc_cc> takeInsideComplex() const;
 
     // src/voroinfo.cpp:447-496
     bool tagVert(const point& p, const std::shared_ptr<Volume3DScalar>& vol) const;
@@ -91,9 +113,9 @@ public:
     // it (src/voroinfo.cpp:290-325), computed on the device
     void computeRadii();
     bool radiiValid() const { return m_r_valid; }
-    const std::vector<float>& getRadii() const { return m_r_per_v; }
-    ivec2 getSitesOfFace(int fi) const { return m_face_sites[fi]; }
-    const std::vector<point>& getSitesPosition() const { return m_site_positions; }
+    const std::vector<float>& getRadii() const;
+    ivec2 getSitesOfFace(int fi) const { return faceSites()[fi]; }
+    const std::vector<point>& getSitesPosition() const;
 
     // src/voroinfo.cpp:1386-1430: V, E, F of the valid part (do_prune_first)
     // or of the whole diagram; C = components of the valid faces' vertices
@@ -110,6 +132,16 @@ private:
     void all_measures(std::vector<float>& lv, std::vector<float>& le,
                       std::vector<float>& lf) const;
     void recompute_flags();
+    // the host geometry from the resident diagram (which is then dropped)
+    void materialize();
+    void ensure_host() const
+    {
+        if (m_dev)
+            const_cast<VoroInfo*>(this)->materialize();
+    }
+    // before changing a resident diagram another VoroInfo shares
+    void own_device();
+    void drop_device_copies() const;
 
     cellcomplex m_geom;
     std::vector<ivec2> m_face_sites;
@@ -121,6 +153,12 @@ private:
     bool m_bb_valid = false;
     std::vector<float> m_r_per_v;
     bool m_r_valid = false;
+    // resident diagram and the host copies fetched from it
+    std::shared_ptr<vc_vdiag> m_dev;
+    mutable std::vector<point> m_dev_vts;
+    mutable bool m_got_vts = false, m_got_fs = false, m_got_sites = false,
+                 m_got_radii = false;
+    mutable std::shared_ptr<vc_cc> m_inside_cc;
 };
 
 } // namespace voxelvoro
