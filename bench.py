@@ -589,6 +589,16 @@ def vol2ma_row(crop, side, tt="0.04,0.2"):
             "bytes": sizes, "outputs_sha256": h.hexdigest()[:16]}
 
 
+def _betti(occ, chi):
+    """(b0, b1, b2) of a voxel set as 26-connected object / 6-connected
+    background, with its Euler characteristic chi"""
+    import numpy as np
+    from scipy import ndimage
+    b0 = int(ndimage.label(occ, structure=np.ones((3, 3, 3), np.uint8))[1])
+    b2 = int(ndimage.label(np.pad(occ == 0, 1, constant_values=True))[1]) - 1
+    return [b0, b0 + b2 - int(chi), b2]
+
+
 def surrogate_leg(ctx, ev, vol, levels, n, steps, plain_ms, tau=0.01):
     """Surrogate-guided pruning on the same volume: EDT + surrogate measure
     (inference, gate at relative error tau, exact fallback) + thinning,
@@ -641,6 +651,11 @@ def surrogate_leg(ctx, ev, vol, levels, n, steps, plain_ms, tau=0.01):
         eul.append(int(v2.boundary(with_euler=True)[2]["chi"]))
         v2.release()
     eul.append(int(vol.boundary(with_euler=True)[2]["chi"]))
+    # Betti numbers of the (26, 6) voxel sets: b0 = 26-components, b2 =
+    # 6-components of the background (padded: the outside is one) - 1, b1 =
+    # b0 + b2 - chi (chi of the union of closed voxel cubes, computed above)
+    betti = [_betti(sk, chi) for sk, chi in ((sk_e, eul[0]), (sk_s, eul[1]))]
+    betti.append(_betti(occ, eul[2]))
     infer_ms = st.ms_infer
     tflops = 116e3 * N / (infer_ms * 1e-3) / 1e12
     net.release()
@@ -657,6 +672,8 @@ def surrogate_leg(ctx, ev, vol, levels, n, steps, plain_ms, tau=0.01):
         "level_decision_agreement": round(agree, 4),
         "skeleton_iou_pure_surrogate": round(iou, 4),
         "euler_chi": {"exact_skeleton": eul[0], "surrogate_skeleton": eul[1], "object": eul[2]},
+        "betti_b0_b1_b2": {"exact_skeleton": betti[0], "surrogate_skeleton": betti[1],
+                           "object": betti[2]},
         "roofline_infer": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0,
                            "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4),
                            "flop_per_voxel": 116000},

@@ -3,13 +3,14 @@
 // Tensors are channels last, [X][Y][Z][C] with C a multiple of 8, BF16.
 // A 3^3 convolution is an implicit GEMM: M = voxels, N = output channels,
 // K = (tap, 8-channel chunk) pairs.  One v_mfma_f32_16x16x32_bf16 consumes
-// four chunks (lane group g = lane / 16 holds chunk kk + g), so any channel
-// count that is a multiple of 8 tiles K exactly (padded to 4 chunks with
-// zero weights).  The input-gradient pass is the same kernel with the taps
-// flipped and the channel roles swapped; the weight gradient stages 64-voxel
+// four pairs (lane group g = lane / 16 holds one), so any channel count that
+// is a multiple of 8 tiles K exactly; the input is staged in LDS per output
+// brick (conv_fwd_lds).  The input-gradient pass is the same kernel with the
+// taps flipped and the channel roles swapped; the weight gradient stages 64-voxel
 // blocks of the layer input and output gradient in LDS and reduces over
 // voxels on MFMA into per-chunk partials, summed by a second kernel (fixed
 // order: the gradients are deterministic).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,18 @@ constexpr float kEpsT = 1.0f / 32.0f;
 constexpr float kErrW = 0.1f;
 constexpr float kErrFloor = 1e-4f;
 constexpr int kHalo = VC_SURROGATE_HALO;
+#ifndef VC_CONV_BX
+#define VC_CONV_BX 4
+#endif
+#ifndef VC_CONV_BY
+#define VC_CONV_BY 4
+#endif
+#ifndef VC_CONV_NZT
+#define VC_CONV_NZT 1
+#endif
+#ifndef VC_CONV_SC
+#define VC_CONV_SC 2
+#endif
 
 struct LayerSpec
 {
@@ -79,86 +92,129 @@ struct Box
     int ox, oy, oz, px, py, pz;
 };
 
-template <int NT>
+// The convolution, with the input staged in LDS.  A workgroup (4
+// wavefronts) owns an output brick of BX x BY rows (x, y) by 16 NZT voxels
+// (z), i.e. BX BY NZT tiles of 16 voxels; wavefront w owns tiles
+// w MT .. w MT + MT - 1 (tile t: row t / NZT, z segment t % NZT).  The
+// brick's input region (one voxel of halo) is staged SC chunks of 8
+// channels at a time, chunk-major (16 B per voxel and chunk, so the 16 lanes
+// of a tile read consecutive entries).  Within a slice the K pairs (tap,
+// chunk) run tap-major, four per MFMA (lane group g takes pair 4 j + g), so a
+// layer with fewer than 4 chunks still fills K; A comes from LDS, B (the
+// weights of that tap and chunk, shared by every workgroup) from global
+// memory, each B reused by the wavefront's MT tiles.
+//
+// kCat: the input is the decoder's concatenation [up | skip] without the
+// concatenated tensor: channels below c1 are the coarse level's (nearest
+// upsampled: voxel (x, y, z) reads (x/2, y/2, z/2) of `in`, c1 channels),
+// the rest the skip tensor `in2` (cinp - c1 channels).
+template <int NT, int BX, int BY, int NZT, int SC, bool kCat>
 __global__ void __launch_bounds__(kBlock)
-    conv_fwd(const bf16* __restrict__ in, int cinp, bf16* __restrict__ out,
-             const bf16x8* __restrict__ W, const float* __restrict__ bias,
-             int X, int Y, int Z, int64_t V, int kc, int relu, Box box)
+    conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
+                 int cinp, bf16* __restrict__ out, const bf16x8* __restrict__ W,
+                 const float* __restrict__ bias, int X, int Y, int Z, int relu, Box box)
 {
-    constexpr int MT = 2; // 16-voxel tiles per wavefront
     constexpr int Cout = NT * 16;
+    constexpr int MT = BX * BY * NZT / 4;
+    static_assert(MT * 4 == BX * BY * NZT, "tiles per wavefront");
+    constexpr int BZ = 16 * NZT, RY = BY + 2, RZ = BZ + 2;
+    constexpr int NR = (BX + 2) * RY * RZ;
+    __shared__ bf16x8 reg[SC * NR];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 15, g = lane >> 4;
-    const int cin8 = cinp >> 3;
-    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wv) * (16 * MT);
-    int vx[MT], vy[MT], vz[MT];
-    bool vin[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-    {
-        const int64_t v = vbase + m * 16 + i;
-        vin[m] = v < V;
-        const int64_t t = vin[m] ? v : 0;
-        vz[m] = (int)(t % Z);
-        const int64_t r = t / Z;
-        vy[m] = (int)(r % Y);
-        vx[m] = (int)(r / Y);
-    }
+    const int cin8 = cinp >> 3, c18 = c1 >> 3, c2 = cinp - c1;
+    const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
+    int blk = blockIdx.x;
+    const int bz = blk % nbz;
+    blk /= nbz;
+    const int by = blk % nby, bx = blk / nby;
+    const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < NT; ++n)
             acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const bf16x8 zero8 = {};
-    int tap = g / cin8, c8 = g % cin8; // chunk kk + g
-    for (int kk = 0; kk < kc; kk += 4)
-    {
-        const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-        bf16x8 a[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-        {
-            const int xx = vx[m] + dx, yy = vy[m] + dy, zz = vz[m] + dz;
-            const bool ok = tap < 27 && vin[m] && (unsigned)xx < (unsigned)X &&
-                            (unsigned)yy < (unsigned)Y &&
-                            (unsigned)zz < (unsigned)Z;
-            a[m] = ok ? *reinterpret_cast<const bf16x8*>(
-                            in + ((((int64_t)xx * Y + yy) * Z + zz) * cinp +
-                                  c8 * 8))
-                      : zero8;
-        }
-#pragma unroll
-        for (int n = 0; n < NT; ++n)
-        {
-            const bf16x8 b = W[(int64_t)(kk + g) * Cout + n * 16 + i];
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[m], b, acc[m][n], 0, 0, 0);
-        }
-        c8 += 4;
-        while (c8 >= cin8)
-        {
-            c8 -= cin8;
-            ++tap;
-        }
-    }
-    // lane holds D[4g + r][i]: voxel vbase + m*16 + 4g + r, channel n*16 + i
+    // region offset of each tile's first voxel (row + 1, z segment + 1)
+    int tb[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        tb[m] = ((row / BY + 1) * RY + (row % BY + 1)) * RZ + 1 + 16 * seg;
+    }
+    const bf16x8 zero8 = {};
+    for (int s0 = 0; s0 < cin8; s0 += SC)
+    {
+        const int S = min(SC, cin8 - s0);
+        __syncthreads(); // the previous slice's reads are done
+        for (int e = threadIdx.x; e < NR * S; e += kBlock)
+        {
+            const int c = e % S, r = e / S;
+            const int rz = r % RZ, rxy = r / RZ;
+            const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
+            bf16x8 v = zero8;
+            if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
+                (unsigned)zz < (unsigned)Z)
+            {
+                const int ch = s0 + c;
+                if (kCat && ch < c18)
+                    v = *reinterpret_cast<const bf16x8*>(
+                        in + ((((int64_t)(xx >> 1) * (Y >> 1)) + (yy >> 1)) * (Z >> 1) +
+                              (zz >> 1)) * c1 + ch * 8);
+                else if (kCat)
+                    v = *reinterpret_cast<const bf16x8*>(
+                        in2 + (((int64_t)xx * Y + yy) * Z + zz) * c2 + (ch - c18) * 8);
+                else
+                    v = *reinterpret_cast<const bf16x8*>(
+                        in + (((int64_t)xx * Y + yy) * Z + zz) * cinp + ch * 8);
+            }
+            reg[c * NR + r] = v;
+        }
+        __syncthreads();
+        const int npairs = 27 * S;
+        for (int p0 = 0; p0 < npairs; p0 += 4)
+        {
+            const int p = p0 + g;
+            const bool pv = p < npairs;
+            const int tap = pv ? p / S : 13, c = pv ? p % S : 0;
+            const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+            const int ro = c * NR + (dx * RY + dy) * RZ + dz + i;
+            bf16x8 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                a[m] = pv ? reg[ro + tb[m]] : zero8;
+            const int64_t krow = (int64_t)(tap * cin8 + s0 + c) * Cout;
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+            {
+                const bf16x8 b = pv ? W[krow + n * 16 + i] : zero8;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][n],
+                                                                        0, 0, 0);
+            }
+        }
+    }
+    // lane holds D[4g + r][i] of tile m: voxel 4 g + r of the tile, channel
+    // n * 16 + i
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        const int x = x0 + row / BY, y = y0 + row % BY;
+        if (x >= X || y >= Y)
+            continue;
+        const bool inxy = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                          (unsigned)(box.oy + y) < (unsigned)box.py;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
         {
-            const int64_t v = vbase + m * 16 + 4 * g + r;
-            if (v >= V)
+            const int z = z0 + 16 * seg + 4 * g + r;
+            if (z >= Z)
                 continue;
-            const int z = (int)(v % Z);
-            const int64_t q = v / Z;
-            const int y = (int)(q % Y), x = (int)(q / Y);
-            const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
-                               (unsigned)(box.oy + y) < (unsigned)box.py &&
-                               (unsigned)(box.oz + z) < (unsigned)box.pz;
+            const int64_t v = ((int64_t)x * Y + y) * Z + z;
+            const bool inbox = inxy && (unsigned)(box.oz + z) < (unsigned)box.pz;
 #pragma unroll
             for (int n = 0; n < NT; ++n)
             {
@@ -169,6 +225,7 @@ __global__ void __launch_bounds__(kBlock)
                 out[v * Cout + co] = inbox ? (bf16)val : (bf16)0.0f;
             }
         }
+    }
 }
 
 // Weight gradient partials.  Block (chunk, tg): taps (dx, dy) = (tg/3 - 1,
@@ -747,21 +804,47 @@ __global__ void __launch_bounds__(kBlock)
     gate_count(const uint8_t* __restrict__ occ, const float* __restrict__ rel,
                float tau, int64_t N, unsigned long long* __restrict__ ctr)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t acc = 0, fb = 0;
-    if (i < N && occ[i])
+    // a capped grid strides over the volume; counts are reduced per
+    // workgroup, so each workgroup adds once per counter (one add per
+    // wavefront and counter serialised millions of atomics on 3 addresses)
+    __shared__ unsigned long long sh[2][kBlock / 64];
+    unsigned long long acc = 0, fb = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (occ[i])
+        {
+            if (rel[i] <= tau)
+                ++acc;
+            else
+                ++fb;
+        }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
     {
-        if (rel[i] <= tau)
-            acc = 1;
-        else
-            fb = 1;
+        acc += __shfl_xor(acc, d);
+        fb += __shfl_xor(fb, d);
     }
-    uint64_t a = __ballot(acc), f = __ballot(fb);
-    if ((threadIdx.x & 63) == 0 && (a | f))
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
     {
-        atomicAdd(&ctr[0], (unsigned long long)__popcll(a));
-        atomicAdd(&ctr[1], (unsigned long long)__popcll(f));
-        atomicAdd(&ctr[2], (unsigned long long)__popcll(a | f));
+        sh[0][wv] = acc;
+        sh[1][wv] = fb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long a = 0, f = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+        {
+            a += sh[0][w];
+            f += sh[1][w];
+        }
+        if (a | f)
+        {
+            atomicAdd(&ctr[0], a);
+            atomicAdd(&ctr[1], f);
+            atomicAdd(&ctr[2], a + f);
+        }
     }
 }
 
@@ -969,35 +1052,51 @@ int pack_weights(vc_ctx* ctx, vc_surrogate* s)
     return VC_OK;
 }
 
-template <int NT>
-void launch_conv_t(hipStream_t S, const bf16* in, int cinp, bf16* out,
-                   const bf16* W, const float* bias, int X, int Y, int Z,
-                   int kc, int relu, const Box& box)
+// brick of the LDS convolution: BX x BY rows, NZT tiles of 16 voxels in z,
+// input staged SC chunks at a time
+constexpr int kCBX = VC_CONV_BX, kCBY = VC_CONV_BY, kCNZT = VC_CONV_NZT, kCSC = VC_CONV_SC;
+
+template <int NT, bool kCat>
+void launch_conv_t(hipStream_t S, const bf16* in, const bf16* in2, int c1, int cinp,
+                   bf16* out, const bf16* W, const float* bias, int X, int Y, int Z,
+                   int relu, const Box& box)
 {
-    const int64_t V = (int64_t)X * Y * Z;
-    const unsigned g = (unsigned)((V + 127) / 128);
-    conv_fwd<NT><<<g, kBlock, 0, S>>>(in, cinp, out,
-                                      reinterpret_cast<const bf16x8*>(W), bias,
-                                      X, Y, Z, V, kc, relu, box);
+    const int64_t nb = (int64_t)((X + kCBX - 1) / kCBX) * ((Y + kCBY - 1) / kCBY) *
+                       ((Z + 16 * kCNZT - 1) / (16 * kCNZT));
+    conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat><<<(unsigned)nb, kBlock, 0, S>>>(
+        in, in2, c1, cinp, out, reinterpret_cast<const bf16x8*>(W), bias, X, Y, Z, relu,
+        box);
 }
 
-// box: the region's origin and the padded-volume extent at this level
+// box: the region's origin and the padded-volume extent at this level.  With
+// in2 the input is the concatenation [in (coarse level, c1 channels,
+// upsampled) | in2 (cinp - c1 channels)].
 int launch_conv(vc_ctx* ctx, const bf16* in, int cinp, bf16* out, int cout,
                 const bf16* W, const float* bias, int X, int Y, int Z, int kc,
-                int relu, const Box& b)
+                int relu, const Box& b, const bf16* in2 = nullptr, int c1 = 0)
 {
     hipStream_t S = ctx->stream;
+    (void)kc;
+#define VC_CONV_CASE(nt)                                                              \
+    case nt:                                                                          \
+        if (in2)                                                                      \
+            launch_conv_t<nt, true>(S, in, in2, c1, cinp, out, W, bias, X, Y, Z, relu, b); \
+        else                                                                          \
+            launch_conv_t<nt, false>(S, in, nullptr, 0, cinp, out, W, bias, X, Y, Z, relu, \
+                                     b);                                              \
+        break;
     switch (cout / 16)
     {
-    case 1: launch_conv_t<1>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
-    case 2: launch_conv_t<2>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
-    case 3: launch_conv_t<3>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
-    case 4: launch_conv_t<4>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
-    case 5: launch_conv_t<5>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
-    case 7: launch_conv_t<7>(S, in, cinp, out, W, bias, X, Y, Z, kc, relu, b); break;
+        VC_CONV_CASE(1)
+        VC_CONV_CASE(2)
+        VC_CONV_CASE(3)
+        VC_CONV_CASE(4)
+        VC_CONV_CASE(5)
+        VC_CONV_CASE(7)
     default:
         return vc_fail(ctx, VC_ERR_INVALID, "conv: %d output channels", cout);
     }
+#undef VC_CONV_CASE
     VC_CHECK_LAUNCH(ctx);
     return VC_OK;
 }
@@ -1049,8 +1148,11 @@ int layer_wgrad(vc_ctx* ctx, vc_surrogate* s, int l, const bf16* x,
     return VC_OK;
 }
 
-// forward over the current workspace region (input x0 filled)
-int forward(vc_ctx* ctx, vc_surrogate* s)
+// forward over the current workspace region (input x0 filled); keep_cat
+// materializes the decoder's concatenated inputs (the weight gradients read
+// them), otherwise the first decoder convolution of a level reads the coarse
+// level and the skip tensor directly
+int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
 {
     hipStream_t S = ctx->stream;
     const bf16* in = s->x0;
@@ -1093,12 +1195,23 @@ int forward(vc_ctx* ctx, vc_surrogate* s)
     {
         const int lv = 2 - d;
         dims(lv, X, Y, Z);
-        const int64_t n = level_voxels(s, lv) * (c1s[d] + c2s[d]);
-        concat_fwd<<<grid_for(n), kBlock, 0, S>>>(ups[d], c1s[d], skips[d],
-                                                     c2s[d], cats[d], X, Y, Z);
-        VC_CHECK_LAUNCH(ctx);
-        if ((rc = conv(8 + 2 * d, cats[d])) != VC_OK ||
-            (rc = conv(9 + 2 * d, s->act[8 + 2 * d])) != VC_OK)
+        if (keep_cat)
+        {
+            const int64_t n = level_voxels(s, lv) * (c1s[d] + c2s[d]);
+            concat_fwd<<<grid_for(n), kBlock, 0, S>>>(ups[d], c1s[d], skips[d],
+                                                         c2s[d], cats[d], X, Y, Z);
+            VC_CHECK_LAUNCH(ctx);
+            rc = conv(8 + 2 * d, cats[d]);
+        }
+        else
+        {
+            const int l = 8 + 2 * d;
+            const LayerSpec& L = kLayers[l];
+            rc = launch_conv(ctx, ups[d], L.cin, s->act[l], L.cout, s->Wf[l],
+                             s->theta + param_offset(l) + 27LL * L.cin * L.cout, X, Y,
+                             Z, s->kc[l], 1, level_box(s, L.level), skips[d], c1s[d]);
+        }
+        if (rc != VC_OK || (rc = conv(9 + 2 * d, s->act[8 + 2 * d])) != VC_OK)
             return rc;
     }
     const int64_t V0 = level_voxels(s, 0);
@@ -1128,14 +1241,30 @@ int layer_back(vc_ctx* ctx, vc_surrogate* s, int l, bf16* gy, const bf16* x,
                        X, Y, Z, s->kcd[l], 0, level_box(s, L.level));
 }
 
+// inside voxels, one add per workgroup (a capped grid strides over them)
 __global__ void __launch_bounds__(kBlock)
     count_occ(const uint8_t* __restrict__ occ, int64_t N,
               unsigned long long* __restrict__ cnt)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t b = __ballot(i < N && occ[i] != 0);
-    if ((threadIdx.x & 63) == 0 && b)
-        atomicAdd(cnt, (unsigned long long)__popcll(b));
+    __shared__ unsigned long long sh[kBlock / 64];
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += occ[i] != 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        c += __shfl_xor(c, d);
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            t += sh[w];
+        if (t)
+            atomicAdd(cnt, t);
+    }
 }
 
 // dst = sum of part[0..n) (fixed order) / max(count, 1)
@@ -1172,7 +1301,8 @@ int backward(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
     const int64_t V0 = level_voxels(s, 0);
     const unsigned nb = blocks_for(V0);
     VC_HIP(ctx, hipMemsetAsync(cnt, 0, 8, S));
-    count_occ<<<blocks_for(v->N), kBlock, 0, S>>>(v->occ, v->N, cnt);
+    count_occ<<<(unsigned)std::min<int64_t>(blocks_for(v->N), 4096), kBlock, 0, S>>>(
+        v->occ, v->N, cnt);
     VC_CHECK_LAUNCH(ctx);
     loss_bwd<<<nb, kBlock, 0, S>>>(s->out, v->occ, v->lam2, v->nx, v->ny,
                                     v->nz, s->X, s->Y, s->Z, cnt, s->dout,
@@ -1268,7 +1398,7 @@ int train_volume(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
     prep_input<<<blocks_for(level_voxels(s, 0)), kBlock, 0, S>>>(
         v->occ, v->edt, v->nx, v->ny, v->nz, 0, 0, 0, s->x0, s->X, s->Y, s->Z);
     VC_CHECK_LAUNCH(ctx);
-    if ((rc = forward(ctx, s)) != VC_OK)
+    if ((rc = forward(ctx, s, true)) != VC_OK)
         return rc;
     return backward(ctx, s, v, loss_dev, cnt);
 }
@@ -1343,7 +1473,7 @@ int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
                     v->occ, v->edt, v->nx, v->ny, v->nz, ox, oy, oz, s->x0,
                     s->X, s->Y, s->Z);
                 VC_CHECK_LAUNCH(ctx);
-                if ((rc = forward(ctx, s)) != VC_OK)
+                if ((rc = forward(ctx, s, false)) != VC_OK)
                     return rc;
                 copy_core<<<blocks_for(VR), kBlock, 0, S>>>(
                     s->out, s->X, s->Y, s->Z, ox, oy, oz, tx, ty, tz,
@@ -1648,7 +1778,7 @@ int vc_surrogate_measure(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
             e = hipMemsetAsync(s->ctr, 0, 3 * 8, S);
         if (e == hipSuccess)
         {
-            gate_count<<<blocks_for(vol->N), kBlock, 0, S>>>(
+            gate_count<<<(unsigned)std::min<int64_t>(blocks_for(vol->N), 4096), kBlock, 0, S>>>(
                 vol->occ, s->vrel, max_rel_err, vol->N, s->ctr);
             e = hipGetLastError();
         }

@@ -93,19 +93,32 @@ __global__ void __launch_bounds__(kB)
     valid_box(const float* __restrict__ vts, const uint8_t* __restrict__ valid,
               int32_t nv, unsigned long long* cnt, uint32_t* box)
 {
+    // wavefront reductions first: one add and six min / max per wavefront
     const int32_t v = blockIdx.x * kB + threadIdx.x;
     const bool ok = v < nv && valid[v];
     const unsigned long long c = __popcll(__ballot(ok));
-    if ((threadIdx.x & 63) == 0 && c)
-        atomicAdd(cnt, c);
-    if (!ok)
-        return;
+    uint32_t lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
-        const uint32_t u = ord(vts[3 * v + a]);
-        atomicMin(&box[a], u);
-        atomicMax(&box[3 + a], u);
+        const uint32_t u = ok ? ord(vts[3 * v + a]) : 0u;
+        lo[a] = ok ? u : ~0u;
+        hi[a] = ok ? u : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+        {
+            lo[a] = min(lo[a], (uint32_t)__shfl_xor(lo[a], d));
+            hi[a] = max(hi[a], (uint32_t)__shfl_xor(hi[a], d));
+        }
+    }
+    if ((threadIdx.x & 63) != 0 || !c)
+        return;
+    atomicAdd(cnt, c);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        atomicMin(&box[a], lo[a]);
+        atomicMax(&box[3 + a], hi[a]);
     }
 }
 
