@@ -389,6 +389,37 @@ def boundary_row(ctx, occ_host, reps=3):
                      "bytes": "1 B/voxel + 12 B/site + 16 B/quad"}}
 
 
+def edt_ball_row(ctx, n, reps=3):
+    """EDT + FT of an n^3 grid holding a ball of radius n * 250 / 512 about
+    the grid centre, timed with HIP events on the library's stream (median of
+    `reps` after a warm-up); the same 21 bytes per voxel as the headline."""
+    import numpy as np
+    g = (np.arange(n) - (n - 1) / 2.0) ** 2
+    rad = n * 250.0 / 512.0
+    occ = ((g[:, None, None] + g[None, :, None] + g[None, None, :]) <= rad * rad)
+    vol = ctx.upload(occ.astype(np.uint8))
+    del occ
+    ev = HipEvents()
+    stream = ctx.stream()
+    vol.edt()
+    ctx.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream)
+        vol.edt()
+        ev.record(b, stream)
+        ts.append(ev.ms(a, b))
+    vol.release()
+    ms = sorted(ts)[len(ts) // 2]
+    gbs = EDT_BYTES_PER_VOXEL * n ** 3 / (ms * 1e-3) / 1e9
+    return {"workload": "ball of radius %.0f centred in a %d^3 grid" % (rad, n),
+            "ms": round(ms, 3), "value": round(n ** 3 / ms / 1e3, 1), "unit": "Mvoxels/s",
+            "achieved_gbs": round(gbs, 1), "frac": round(gbs / 8000.0, 4),
+            "note": "column passes search outward until the squared distance passes the "
+                    "best cost, so their work grows with the distance to the surface"}
+
+
 def rows_leg(ctx, occ_host, label_host, reps=3):
     """The other rows of the path, each on its own input, timed on the
     device stream (wall clock around the C-ABI call; the Voronoi-side calls
@@ -410,6 +441,11 @@ def rows_leg(ctx, occ_host, label_host, reps=3):
 
     out["R3_R4_boundary"] = boundary_row(ctx, occ_host, reps)
     n = occ_host.shape[0]
+
+    # R5 on a thick solid: a ball of radius 250 in the 512^3 grid (half the
+    # volume inside, mean depth ~62 voxels), the worst case of the column
+    # passes' outward searches, whose length follows the distance
+    out["R5_edt_thick_ball"] = edt_ball_row(ctx, n, reps)
 
     # R2: dense occupancy from a .sof octree (R = 256)
     stream = _random_octree(256, 9)

@@ -102,3 +102,30 @@ def test_edt_long_axes(vc, oracle):
         occ = (rng.random(shape) < 0.999).astype(np.uint8)
         occ[0, 0, 0] = 0
         _check(vc, oracle, occ, "long %s" % (shape,))
+
+
+def test_edt_thick_solids(vc, oracle):
+    """Thick solids: column searches run past their outward stretch of 16
+    distances into the block-by-block continuation, in both column passes.
+    A ball centred on a voxel corner (many equal-distance nearest
+    background voxels: the lowest index must win), an off-centre ellipsoid
+    (lopsided searches), a slab, and a block with a few background voxels
+    (searches across the whole column)."""
+    n = 96
+    g = np.arange(n) - (n - 1) / 2.0
+    r2 = g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2
+    cases = {"ball": (r2 <= 44.0 ** 2).astype(np.uint8)}
+    h = np.arange(n) - 40.0
+    e2 = (h[:, None, None] ** 2 + (1.3 * h[None, :, None]) ** 2 +
+          (0.7 * h[None, None, :]) ** 2)
+    cases["ellipsoid"] = (e2 <= 38.0 ** 2).astype(np.uint8)
+    slab = np.zeros((n, n, n), np.uint8)
+    slab[2:-2, 2:-2, 20:76] = 1
+    cases["slab"] = slab
+    rng = np.random.default_rng(23)
+    holes = np.ones((80, 90, 100), np.uint8)
+    for _ in range(6):
+        holes[tuple(int(rng.integers(0, s)) for s in holes.shape)] = 0
+    cases["holes"] = holes
+    for name, occ in cases.items():
+        _check(vc, oracle, occ, "thick " + name)
