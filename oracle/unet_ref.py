@@ -23,6 +23,18 @@ per voxel.  Every definition below is the one the HIP kernels implement:
 
 Parity of this restatement is pinned by central finite differences of the
 loss in float64 (tests/test_unet_ref.py), not by the GPU path.
+
+BF16 variant (forward / gradients with bf16=True): the same algebra with
+every tensor the kernels store in BF16 rounded to the nearest BF16 (ties to
+even) of its float32 value where they store it -- the network input, the
+weights the convolutions read, each conv output after bias and ReLU, each
+pooled tensor, the head's input gradient, each input gradient of a conv,
+the up-sampling gradient sums and the pooled-gradient accumulations; sums
+stay in float64 (the kernels accumulate in float32), the head output and
+dL/d(out) in float32.  It reproduces the kernels' rounding points so their
+gradients can be held to a percent-level bound.  round_bf16 is pinned by
+exact bit patterns (tests/test_unet_ref.py); the algebra is the float64
+path's.
 """
 import numpy as np
 
@@ -62,6 +74,19 @@ def pack(grads):
 
 
 TAPS = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
+
+
+def round_bf16(x):
+    """Nearest bfloat16 of float32(x), ties to even, as float64 (finite
+    values)."""
+    f = np.ascontiguousarray(np.asarray(x, np.float64).astype(np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _ident(x):
+    return x
 
 
 def shift(x, d):
@@ -120,24 +145,32 @@ def inputs(occ, edt):
     return x
 
 
-def forward(theta, x0, keep=False):
+def forward(theta, x0, keep=False, bf16=False):
+    rb = round_bf16 if bf16 else _ident
     P = unpack(theta)
     acts = {}
+    x0 = rb(x0)
 
     def cr(name, x):
-        y = np.maximum(conv(x, *P[name]), 0.0)
+        W, b = P[name]
+        y = rb(np.maximum(conv(x, rb(W), b), 0.0))
         acts[name] = (x, y)
         return y
 
+    def pl(x):
+        return rb(pool(x))
+
     b0 = cr("L0b", cr("L0a", x0))
-    b1 = cr("L1b", cr("L1a", pool(b0)))
-    b2 = cr("L2b", cr("L2a", pool(b1)))
-    b3 = cr("L3b", cr("L3a", pool(b2)))
+    b1 = cr("L1b", cr("L1a", pl(b0)))
+    b2 = cr("L2b", cr("L2a", pl(b1)))
+    b3 = cr("L3b", cr("L3a", pl(b2)))
     d2 = cr("L4b", cr("L4a", np.concatenate([up(b3), b2], axis=3)))
     d1 = cr("L5b", cr("L5a", np.concatenate([up(d2), b1], axis=3)))
     d0 = cr("L6b", cr("L6a", np.concatenate([up(d1), b0], axis=3)))
     Wh, bh = P["head"]
     out = d0 @ Wh + bh
+    if bf16:
+        out = out.astype(np.float32).astype(np.float64)
     if keep:
         return out, acts, P
     return out
@@ -166,26 +199,30 @@ def loss_and_dout(out, occ, lam2, e=None):
     return L, dout
 
 
-def gradients(theta, occ, edt, lam2):
-    """(loss, flat float64 gradient) of the network on one volume."""
+def gradients(theta, occ, edt, lam2, bf16=False):
+    """(loss, flat float64 gradient) of the network on one volume; with
+    bf16, the BF16 variant (the kernels' rounding points)."""
+    rb = round_bf16 if bf16 else _ident
     x0 = inputs(occ, edt)
-    out, acts, P = forward(theta, x0, keep=True)
+    out, acts, P = forward(theta, x0, keep=True, bf16=bf16)
     L, dout = loss_and_dout(out, occ, lam2)
+    if bf16:
+        dout = dout.astype(np.float32).astype(np.float64)
     G = {}
     Wh, _ = P["head"]
     d0 = acts["L6b"][1]
     G["head"] = (np.einsum("xyzc,xyzo->co", d0, dout), dout.sum(axis=(0, 1, 2)))
-    dy = dout @ Wh.T
+    dy = rb(dout @ Wh.T)
 
     def back(name, dy):
         x, y = acts[name]
         dz = dy * (y > 0)
-        dW, db, dx = conv_back(x, P[name][0], dz)
+        dW, db, dx = conv_back(x, rb(P[name][0]), dz)
         G[name] = (dW, db)
-        return dx
+        return rb(dx)
 
     def split(dcat, c_up):
-        return up_back(dcat[..., :c_up]), dcat[..., c_up:]
+        return rb(up_back(dcat[..., :c_up])), dcat[..., c_up:]
 
     dcat0 = back("L6a", back("L6b", dy))
     dd1, db0 = split(dcat0, 32)
@@ -193,9 +230,9 @@ def gradients(theta, occ, edt, lam2):
     dd2, db1 = split(dcat1, 48)
     dcat2 = back("L4a", back("L4b", dd2))
     db3, db2 = split(dcat2, 64)
-    db2 = db2 + pool_back(back("L3a", back("L3b", db3)))
-    db1 = db1 + pool_back(back("L2a", back("L2b", db2)))
-    db0 = db0 + pool_back(back("L1a", back("L1b", db1)))
+    db2 = rb(db2 + pool_back(back("L3a", back("L3b", db3))))
+    db1 = rb(db1 + pool_back(back("L2a", back("L2b", db2))))
+    db0 = rb(db0 + pool_back(back("L1a", back("L1b", db1))))
     back("L0a", back("L0b", db0))
     return L, pack(G)
 

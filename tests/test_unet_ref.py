@@ -67,3 +67,58 @@ def test_init_is_he_uniform():
     a = np.sqrt(6.0 / (27 * 16))
     assert np.all(np.abs(W) <= a) and np.all(b == 0)
     assert abs(W.std() - a / np.sqrt(3)) < 0.05 * a
+
+
+def test_round_bf16_known_patterns():
+    """Ties to even and carries into the exponent, at exact bit patterns."""
+    two = 2.0
+    cases = [
+        (1.0, 1.0),
+        (1.0 + two ** -8, 1.0),                      # tie, 1.0 is even
+        (1.0 + 3 * two ** -8, 1.0 + two ** -6),      # tie, odd -> up
+        (1.0 + two ** -8 + two ** -20, 1.0 + two ** -7),
+        (1.0 + two ** -8 - two ** -20, 1.0),
+        (-3.0, -3.0),
+        (0.0, 0.0),
+        (2.0 - two ** -9, 2.0),                      # carry into the exponent
+        (255.5, 256.0),                              # 0x437F8000: tie, odd -> up
+        (1e-30, np.float32(1e-30).view(np.uint32) & 0xFFFF0000),
+    ]
+    for x, want in cases:
+        if isinstance(want, (int, np.integer)):
+            u = int(want)
+            if (np.float32(x).view(np.uint32) & 0xFFFF) > 0x8000:
+                u += 0x10000
+            want = float(np.uint32(u).view(np.float32))
+        assert U.round_bf16(x) == want, (x, U.round_bf16(x), want)
+
+
+def test_round_bf16_is_nearest_tie_even():
+    """Against a direct choice between the two BF16 neighbours of each
+    float32 (the truncation and the next BF16 up in magnitude): the nearer
+    one, the even mantissa on a tie."""
+    rng = np.random.default_rng(9)
+    f = (rng.standard_normal(20000) * 10.0 ** rng.integers(-6, 6, 20000)).astype(np.float32)
+    f[:64] = np.float32(1.0) + np.arange(64, dtype=np.float32) * np.float32(2.0 ** -13)
+    u = f.view(np.uint32)
+    lo = (u & 0xFFFF0000).view(np.float32).astype(np.float64)
+    hi = ((u & 0xFFFF0000) + 0x10000).astype(np.uint32).view(np.float32).astype(np.float64)
+    x = f.astype(np.float64)
+    dl, dh = np.abs(x - lo), np.abs(hi - x)
+    lo_even = ((u >> 16) & 1) == 0
+    want = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(lo_even, lo, hi)))
+    assert np.array_equal(U.round_bf16(x), want)
+
+
+def test_bf16_variant_close_to_float64():
+    """The BF16 variant follows the float64 path up to BF16 rounding: loss
+    within 2 %, gradient cosine above 0.99 for the whole vector."""
+    occ, edt, lam2 = _volume(8, 3)
+    rng = np.random.default_rng(4)
+    th = U.init_params(5).astype(np.float64) + rng.normal(0, 0.05, size=U.num_params())
+    th = th.astype(np.float32).astype(np.float64)
+    L, g = U.gradients(th, occ, edt, lam2)
+    Lb, gb = U.gradients(th, occ, edt, lam2, bf16=True)
+    assert abs(Lb - L) <= 0.02 * abs(L)
+    assert g @ gb / (np.linalg.norm(g) * np.linalg.norm(gb)) > 0.99
+    assert not np.array_equal(g, gb)
