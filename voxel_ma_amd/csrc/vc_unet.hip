@@ -46,6 +46,10 @@ constexpr int kHalo = VC_SURROGATE_HALO;
 #ifndef VC_CONV_SC
 #define VC_CONV_SC 2
 #endif
+#ifndef VC_CONV_UNROLL
+#define VC_CONV_UNROLL 4
+#endif
+
 
 struct LayerSpec
 {
@@ -108,6 +112,47 @@ struct Box
 // concatenated tensor: channels below c1 are the coarse level's (nearest
 // upsampled: voxel (x, y, z) reads (x/2, y/2, z/2) of `in`, c1 channels),
 // the rest the skip tensor `in2` (cinp - c1 channels).
+// The MFMAs of one staged slice of S input chunks: K pairs p = tap * S + c,
+// four per MFMA (lane group g takes pair p0 + g).  S is a compile-time
+// constant here, so a pair's tap and chunk, and the tap's offsets, are
+// constant-divisor arithmetic (the run-time division by S had cost ~30 VALU
+// instructions per MFMA); the loop stays rolled, which keeps the registers
+// of the kernel, and with them its occupancy, where they were.
+template <int NT, int MT, int NR, int RY, int RZ, int S>
+__device__ __forceinline__ void conv_slice(const bf16x8* __restrict__ reg,
+                                           const int (&tb)[MT],
+                                           const bf16x8* __restrict__ W, int s0,
+                                           int cin8, f32x4 (&acc)[MT][NT])
+{
+    constexpr int Cout = NT * 16;
+    constexpr int NP = 27 * S;
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const bf16x8 zero8 = {};
+#pragma unroll VC_CONV_UNROLL
+    for (int p0 = 0; p0 < NP; p0 += 4)
+    {
+        const int p = p0 + g;
+        const bool v = p < NP;
+        const int tap = v ? p / S : 13, c = v ? p % S : 0;
+        const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+        const int ro = c * NR + (dx * RY + dy) * RZ + dz + i;
+        const int wo = (tap * cin8 + s0 + c) * Cout + i;
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+            a[m] = v ? reg[ro + tb[m]] : zero8;
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+        {
+            const bf16x8 b = v ? W[wo + n * 16] : zero8;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][n], 0,
+                                                                    0, 0);
+        }
+    }
+}
+
 template <int NT, int BX, int BY, int NZT, int SC, bool kCat>
 __global__ void __launch_bounds__(kBlock)
     conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
@@ -150,7 +195,8 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads(); // the previous slice's reads are done
         for (int e = threadIdx.x; e < NR * S; e += kBlock)
         {
-            const int c = e % S, r = e / S;
+            // S is SC or (an odd chunk count's last slice) 1: constant divisors
+            const int c = S == SC ? e % SC : 0, r = S == SC ? e / SC : e;
             const int rz = r % RZ, rxy = r / RZ;
             const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
             bf16x8 v = zero8;
@@ -172,29 +218,10 @@ __global__ void __launch_bounds__(kBlock)
             reg[c * NR + r] = v;
         }
         __syncthreads();
-        const int npairs = 27 * S;
-        for (int p0 = 0; p0 < npairs; p0 += 4)
-        {
-            const int p = p0 + g;
-            const bool pv = p < npairs;
-            const int tap = pv ? p / S : 13, c = pv ? p % S : 0;
-            const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-            const int ro = c * NR + (dx * RY + dy) * RZ + dz + i;
-            bf16x8 a[MT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-                a[m] = pv ? reg[ro + tb[m]] : zero8;
-            const int64_t krow = (int64_t)(tap * cin8 + s0 + c) * Cout;
-#pragma unroll
-            for (int n = 0; n < NT; ++n)
-            {
-                const bf16x8 b = pv ? W[krow + n * 16 + i] : zero8;
-#pragma unroll
-                for (int m = 0; m < MT; ++m)
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][n],
-                                                                        0, 0, 0);
-            }
-        }
+        if (S == SC)
+            conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
+        else
+            conv_slice<NT, MT, NR, RY, RZ, 1>(reg, tb, W, s0, cin8, acc);
     }
     // lane holds D[4g + r][i] of tile m: voxel 4 g + r of the tile, channel
     // n * 16 + i
