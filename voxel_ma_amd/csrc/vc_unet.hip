@@ -742,6 +742,113 @@ __global__ void __launch_bounds__(kBlock)
     w[t] -= lr * (mm / bc1) / (sqrtf(ss / bc2) + 1e-8f);
 }
 
+// The first layer (2 input channels, 16 outputs): a K pair is 4 taps x 2
+// channels (taps 4p .. 4p + 3) instead of one tap's 8-channel chunk with 6
+// zero channels, so a 16-voxel tile takes 2 MFMAs (7 pairs) instead of 7
+// (27 pairs), and the brick's input is staged as its 2 channels, 4 bytes per
+// voxel.  Weights: W4[pair][cout][8], pair p slot e = tap 4p + e/2, channel
+// e % 2 (zero past tap 26).  Bias + ReLU + zeroing outside the padded
+// volume as conv_fwd_lds.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <int BX, int BY, int NZT>
+__global__ void __launch_bounds__(kBlock)
+    conv_first_lds(const bf16* __restrict__ in, bf16* __restrict__ out,
+                   const bf16x8* __restrict__ W4, const float* __restrict__ bias, int X,
+                   int Y, int Z, Box box)
+{
+    constexpr int MT = BX * BY * NZT / 4;
+    static_assert(MT * 4 == BX * BY * NZT, "tiles per wavefront");
+    constexpr int BZ = 16 * NZT, RY = BY + 2, RZ = BZ + 2;
+    constexpr int NR = (BX + 2) * RY * RZ;
+    __shared__ uint32_t reg2[NR]; // channels 0 and 1 of each region voxel
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
+    int blk = blockIdx.x;
+    const int bz = blk % nbz;
+    blk /= nbz;
+    const int by = blk % nby, bx = blk / nby;
+    const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
+    int tb[MT];
+    f32x4 acc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        tb[m] = ((row / BY + 1) * RY + (row % BY + 1)) * RZ + 1 + 16 * seg;
+        acc[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    for (int e = threadIdx.x; e < NR; e += kBlock)
+    {
+        const int rz = e % RZ, rxy = e / RZ;
+        const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
+        uint32_t v = 0u;
+        if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
+            (unsigned)zz < (unsigned)Z)
+            v = *reinterpret_cast<const uint32_t*>(in + (((int64_t)xx * Y + yy) * Z + zz) * 8);
+        reg2[e] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p0 = 0; p0 < 8; p0 += 4)
+    {
+        const int p = p0 + g; // taps 4p .. 4p + 3 (pair 7 is all zero)
+        int ro[4];
+        bool tv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int tap = 4 * p + j;
+            tv[j] = tap < 27;
+            const int tt = tv[j] ? tap : 13;
+            ro[j] = ((tt / 9 - 1) * RY + (tt / 3) % 3 - 1) * RZ + tt % 3 - 1 + i;
+        }
+        const bf16x8 b = W4[p * 16 + i];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+        {
+            const u32x4 w = {tv[0] ? reg2[ro[0] + tb[m]] : 0u, tv[1] ? reg2[ro[1] + tb[m]] : 0u,
+                             tv[2] ? reg2[ro[2] + tb[m]] : 0u, tv[3] ? reg2[ro[3] + tb[m]] : 0u};
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w), b,
+                                                             acc[m], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        const int x = x0 + row / BY, y = y0 + row % BY;
+        if (x >= X || y >= Y)
+            continue;
+        const bool inxy = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                          (unsigned)(box.oy + y) < (unsigned)box.py;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const int z = z0 + 16 * seg + 4 * g + r;
+            if (z >= Z)
+                continue;
+            const int64_t v = ((int64_t)x * Y + y) * Z + z;
+            const bool inbox = inxy && (unsigned)(box.oz + z) < (unsigned)box.pz;
+            const float val = fmaxf(acc[m][r] + bias[i], 0.0f);
+            out[v * 16 + i] = inbox ? (bf16)val : (bf16)0.0f;
+        }
+    }
+}
+
+// first layer's W[27][2][16] -> W4[8][16][8] (pair p slot e: tap 4p + e/2,
+// channel e % 2)
+__global__ void __launch_bounds__(kBlock)
+    pack_first(const float* __restrict__ W, bf16* __restrict__ W4)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= 8 * 16 * 8)
+        return;
+    const int e = t & 7, co = (t >> 3) & 15, p = t >> 7;
+    const int tap = 4 * p + (e >> 1), ch = e & 1;
+    W4[t] = (bf16)(tap < 27 ? W[(tap * 2 + ch) * 16 + co] : 0.0f);
+}
+
 // FP32 master W[27][cin][cout] -> forward chunks Wf[kc][cout][8] and the
 // flipped, transposed input-gradient chunks Wd[kcd][cinp][8]
 __global__ void __launch_bounds__(kBlock)
@@ -913,6 +1020,7 @@ struct vc_surrogate
     int64_t step = 0;
     bf16* Wf[14] = {};
     bf16* Wd[14] = {};
+    bf16* W4 = nullptr; // the first layer's tap-packed weights (conv_first_lds)
     int kc[14] = {}, kcd[14] = {};
     bool packed = false;
     // workspace for one region
@@ -1075,6 +1183,8 @@ int pack_weights(vc_ctx* ctx, vc_surrogate* s)
             VC_CHECK_LAUNCH(ctx);
         }
     }
+    pack_first<<<blocks_for(8 * 16 * 8), kBlock, 0, S>>>(s->theta + param_offset(0), s->W4);
+    VC_CHECK_LAUNCH(ctx);
     s->packed = true;
     return VC_OK;
 }
@@ -1186,6 +1296,18 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
     int rc;
     auto conv = [&](int l, const bf16* x) {
         const LayerSpec& L = kLayers[l];
+        if (l == 0)
+        {
+            const int64_t nb = (int64_t)((s->X + kCBX - 1) / kCBX) *
+                               ((s->Y + kCBY - 1) / kCBY) *
+                               ((s->Z + 16 * kCNZT - 1) / (16 * kCNZT));
+            conv_first_lds<kCBX, kCBY, kCNZT><<<(unsigned)nb, kBlock, 0, S>>>(
+                x, s->act[0], reinterpret_cast<const bf16x8*>(s->W4),
+                s->theta + param_offset(0) + 27LL * L.cin * L.cout, s->X, s->Y, s->Z,
+                level_box(s, 0));
+            VC_CHECK_LAUNCH(ctx);
+            return VC_OK;
+        }
         return launch_conv(ctx, x, cinp_of(L.cin), s->act[l], L.cout, s->Wf[l],
                            s->theta + param_offset(l) + 27LL * L.cin * L.cout,
                            s->X >> L.level, s->Y >> L.level, s->Z >> L.level,
@@ -1557,6 +1679,8 @@ int vc_surrogate_create(vc_ctx* ctx, uint64_t seed, vc_surrogate** out)
         }
     }
     if (rc == VC_OK)
+        rc = vc_alloc(ctx, (void**)&s->W4, 2 * (size_t)8 * 16 * 8);
+    if (rc == VC_OK)
     {
         // He-uniform initial weights (oracle/unet_ref.py init_params)
         std::vector<float> th((size_t)s->np, 0.0f);
@@ -1620,6 +1744,7 @@ int vc_surrogate_destroy(vc_ctx* ctx, vc_surrogate* s)
         vc_free(s->Wf[l]);
         vc_free(s->Wd[l]);
     }
+    vc_free(s->W4);
     delete s;
     return VC_OK;
 }
