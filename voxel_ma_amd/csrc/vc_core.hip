@@ -320,6 +320,7 @@ int vc_volume_release(vc_ctx* ctx, vc_vol* v)
     vc_free(v->dflag);
     vc_free(v->alive2);
     vc_free(v->tchg);
+    vc_free(v->eflag);
     vc_sched_free(v->self);
     vc_free(v->bnd_sites);
     vc_free(v->bnd_quads);

@@ -18,6 +18,8 @@
 // Every kernel walks a work table (one volume or a batch of volumes, one
 // launch per pass over all of them); with a persistent grid each workgroup
 // loops over its share of the items.
+#include <algorithm>
+
 #include "vc_internal.h"
 
 namespace
@@ -29,6 +31,21 @@ constexpr uint32_t kOob = 0xFFFFFFFEu; // neighbour outside the grid
 // every candidate cost fits in int32: the volume constructor rejects shapes
 // whose squared diagonal exceeds INT32_MAX
 constexpr int kInf = 0x7FFFFFFF;
+
+// Thick solids.  A column pass's outward search for a voxel ends within the
+// distance of the voxel's own staged candidate (its cost bounds the best),
+// which the previous pass knows: pass z flags the y-pass tiles holding a
+// voxel more than kDeepY from the background along z (or with none in its
+// row), pass y the x-pass tiles holding a voxel whose (y, z) cost exceeds
+// kDeepX^2 (or has none).  A flagged tile is computed by divide and conquer
+// (column_argmins), whose work does not grow with the distance, and clears
+// its flag; the others run the outward search.  Either way the result is the
+// leftmost minimiser, so a stale flag costs time, never correctness.  On the
+// 512^3 blurred-noise volume no tile is flagged (its longest z run of inside
+// voxels has a half-length below 128; at a y threshold of 64, 13 % of the
+// y-pass tiles were, and the EDT took 1.44 instead of 1.26 ms); in a ball of
+// radius 250 most are (EDT 10.2 -> 4.1 ms).
+constexpr int kDeepY = 128, kDeepX = 48;
 
 // rows along z: wavefront = row; the row's background bitmask sits in the
 // wavefront's LDS slice of `stride` words (rows whose length is a multiple
@@ -124,6 +141,8 @@ __global__ void __launch_bounds__(kBlock)
                             const int L = below ? z0 + 31 - __clz(below) : Lb;
                             const int R = above ? z0 + __ffs(above) - 1 : Rb;
                             res = L < 0 ? R : (R < 0 ? L : (z - L <= R - z ? L : R));
+                            if (res < 0 || abs(z - res) > kDeepY)
+                                d.fy[(row / d.ny) * ((nz + d.tzy - 1) / d.tzy) + z / d.tzy] = 1;
                         }
                         if (j & 1)
                             packed[j >> 1] |= (uint32_t)(uint16_t)res << 16;
@@ -192,6 +211,8 @@ __global__ void __launch_bounds__(kBlock)
                         res = L;
                     else
                         res = (z - L <= R - z) ? L : R;
+                    if (res < 0 || abs(z - res) > kDeepY)
+                        d.fy[(row / d.ny) * ((nz + d.tzy - 1) / d.tzy) + z / d.tzy] = 1;
                 }
                 out[z] = (int16_t)res;
             }
@@ -208,6 +229,100 @@ __global__ void __launch_bounds__(kBlock)
 // (z - z')^2 + dy^2; a row without background is stored as -32768, whose
 // cost 2^30 + dy^2 is at least kNoBg, above every real cost (< 2^29).
 constexpr int kNoBg = 1 << 30;
+
+
+// Leftmost minimum over an aligned group of G lanes (G a power of two, at
+// most 64): the smallest cost, then the smallest index; index -1 = none.
+__device__ __forceinline__ void group_min(int& c, int& q, int G)
+{
+    for (int d = 1; d < G; d <<= 1)
+    {
+        const int oc = __shfl_xor(c, d), oq = __shfl_xor(q, d);
+        if (oq >= 0 && (q < 0 || oc < c || (oc == c && oq < q)))
+        {
+            c = oc;
+            q = oq;
+        }
+    }
+}
+
+// Leftmost argmin A(y) of cost(y, q) = f(q) + (y - q)^2 over q in [0, n),
+// for every y in [0, n) of the tile's columns c < ncol, by divide and
+// conquer: cost is a Monge array ((y' - y)(q' - q) >= 0 orders the cross
+// terms), so A is non-decreasing in y and A(mid) of a segment lies between
+// the A of the segment's two bounding rows.  Rows are the nodes of the
+// perfect binary tree over [0, 2^L - 1): level l holds 2^l nodes, node j has
+// mid ((2j+1) << (L-1-l)) - 1 and bounding rows (j << (L-l)) - 1 and
+// ((j+1) << (L-l)) - 1, mids of earlier levels (or outside [0, n)).  Each
+// level spreads its (column, node) tasks over the workgroup, G lanes per
+// task while tasks are fewer than lanes; about n log2 n cost evaluations per
+// column.  A column without a finite cost gets -1 everywhere.
+//   cost(c, y, q): the cost, >= lim when q has no background;
+//   put(c, y, q, cost): the result of row y (q = -1: none), to global memory;
+//   get(c, y): A(y) of a row put at an earlier level (-1: none), read back
+//   after the level's barrier (global writes of a workgroup are visible to
+//   the workgroup after __syncthreads).
+// Every lane of the workgroup must call it (barriers, shuffles).
+template <class Cost, class Put, class Get>
+__device__ void column_argmins(int n, int ncol, int TZ, int lim, Cost cost,
+                               Put put, Get get)
+{
+    int L = 1;
+    while ((1 << L) - 1 < n)
+        ++L;
+    const int nthr = (int)blockDim.x;
+    for (int l = 0; l < L; ++l)
+    {
+        const int ntask = TZ << l;
+        int gl = 0;
+        while (gl < 6 && (ntask << (gl + 1)) <= nthr)
+            ++gl;
+        const int G = 1 << gl, ngroups = nthr >> gl;
+        const int sub = (int)threadIdx.x & (G - 1);
+        const int trips = (ntask + ngroups - 1) / ngroups;
+        for (int it = 0; it < trips; ++it)
+        {
+            const int task = it * ngroups + ((int)threadIdx.x >> gl);
+            const int c = task % TZ, j = task / TZ;
+            const int mid = ((2 * j + 1) << (L - 1 - l)) - 1;
+            const bool active = task < ntask && c < ncol && mid < n;
+            int bc = lim, bq = -1;
+            if (active)
+            {
+                const int yl = (j << (L - l)) - 1, yr = ((j + 1) << (L - l)) - 1;
+                int qlo = 0, qhi = n - 1;
+                bool none = false;
+                if (yl >= 0)
+                {
+                    const int a = get(c, yl);
+                    none = a < 0;
+                    qlo = a;
+                }
+                if (yr < n)
+                {
+                    const int a = get(c, yr);
+                    none = none || a < 0;
+                    qhi = a;
+                }
+                if (!none)
+                    for (int q = qlo + sub; q <= qhi; q += G)
+                    {
+                        const int v = cost(c, mid, q);
+                        if (v < bc)
+                        {
+                            bc = v;
+                            bq = q;
+                        }
+                    }
+            }
+            group_min(bc, bq, G);
+            if (active && sub == 0)
+                put(c, mid, bq, bc);
+        }
+        __syncthreads(); // this level's rows are read by the next
+    }
+}
+
 
 template <bool kB, bool kM>
 __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
@@ -262,6 +377,56 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
             }
         }
         __syncthreads();
+        // x-pass tile of voxel (y, z): flagged when its (y, z) cost exceeds
+        // kDeepX^2 or it has none
+        const int nztx = (nz + d.tzx - 1) / d.tzx;
+        auto flag_x = [&](int y, int z, bool none, int cost) {
+            if (none || cost > kDeepX * kDeepX)
+                d.fx[(int64_t)y * nztx + z / d.tzx] = 1;
+        };
+        if (d.fy[T])
+        {
+            // a thick tile: every row of its columns by divide and conquer
+            const int ncol = min(TZ, nz - z0);
+            uint32_t* ftc = d.ft + base + z0;
+            auto get = [&](int c, int y) -> int {
+                const uint32_t v = ftc[(int64_t)y * nz + c];
+                return v == VC_NONE ? -1 : (int)(v >> 16);
+            };
+            if (kM)
+                column_argmins(
+                    ny, ncol, TZ, kNoBg,
+                    [&](int c, int y, int q) -> int {
+                        const int e = colz[q * SY + c];
+                        return __mul24(e, e) + (y - q) * (y - q);
+                    },
+                    [&](int c, int y, int q, int v) {
+                        ftc[(int64_t)y * nz + c] =
+                            q < 0 ? VC_NONE
+                                  : ((uint32_t)q << 16) |
+                                        (uint32_t)(uint16_t)(z0 + c - colz[q * SY + c]);
+                        flag_x(y, z0 + c, q < 0, v);
+                    },
+                    get);
+            else
+                column_argmins(
+                    ny, ncol, TZ, kInf,
+                    [&](int c, int y, int q) -> int {
+                        const int e = colz[q * SY + c];
+                        return e < 0 ? kInf
+                                     : (z0 + c - e) * (z0 + c - e) + (y - q) * (y - q);
+                    },
+                    [&](int c, int y, int q, int v) {
+                        ftc[(int64_t)y * nz + c] =
+                            q < 0 ? VC_NONE
+                                  : ((uint32_t)q << 16) | (uint32_t)(uint16_t)colz[q * SY + c];
+                        flag_x(y, z0 + c, q < 0, v);
+                    },
+                    get);
+            if (threadIdx.x == 0)
+                d.fy[T] = 0;
+            continue; // column_argmins ended with a barrier
+        }
         // search lanes: each wavefront takes blocks of ZG z x YG y (8 x 8
         // for TZ >= 8); a compact block keeps the lanes' search lengths
         // close, so fewer lanes idle while the longest search runs
@@ -322,6 +487,7 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                     outv = ((uint32_t)by << 16) |
                            (uint32_t)(uint16_t)(z - colz[by * SY + tz]);
                 d.ft[base + (int64_t)y * nz + z] = outv;
+                flag_x(y, z, by < 0, best);
             }
             else
             {
@@ -362,6 +528,7 @@ __global__ void __launch_bounds__(kBlock) edt_pass_y(vc_work W)
                     outv = ((uint32_t)by << 16) |
                            (uint32_t)(uint16_t)colz[by * SY + tz];
                 d.ft[base + (int64_t)y * nz + z] = outv;
+                flag_x(y, z, by < 0, best);
             }
         }
         __syncthreads(); // colz is rewritten by the next item
@@ -415,6 +582,7 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
         const int64_t sx = (int64_t)ny * nz;
         const int64_t off = (int64_t)y * nz + z;
         uint32_t* ft = d.ft;
+
         constexpr int U = 16;
         for (int x = tx0; x < nx; x += txs * U)
         {
@@ -439,6 +607,42 @@ __global__ void __launch_bounds__(kBlock) edt_pass_x(vc_work W)
             }
         }
         __syncthreads();
+        if (d.fx[T])
+        {
+            // a thick tile: every row of its columns by divide and conquer
+            // from the staged pairs (a row's A comes back from its written
+            // feature)
+            const int ncol = min(TZ, nz - z0);
+            const int sh = d.pk.by + d.pk.bz;
+            uint32_t* ftc = ft + (int64_t)y * nz + z0;
+            column_argmins(
+                nx, ncol, TZ, kInf,
+                [&](int c, int xx, int q) -> int {
+                    return dot_cost(colyz[q * TZ + c], (xx - q) * (xx - q));
+                },
+                [&](int c, int xx, int q, int v) {
+                    const int64_t o = (int64_t)xx * sx + (int64_t)y * nz + z0 + c;
+                    if (q < 0)
+                    {
+                        ft[o] = VC_NONE;
+                        d.edt[o] = -1;
+                    }
+                    else
+                    {
+                        const uint32_t e = colyz[q * TZ + c];
+                        ft[o] = vc_pack_xyz(d.pk, q, y - (int)(int16_t)(e >> 16),
+                                            z0 + c - (int)(int16_t)(e & 0xFFFFu));
+                        d.edt[o] = (int32_t)v;
+                    }
+                },
+                [&](int c, int xx) -> int {
+                    const uint32_t v = ftc[(int64_t)xx * sx + c];
+                    return v == VC_NONE ? -1 : (int)(v >> sh);
+                });
+            if (threadIdx.x == 0)
+                d.fx[T] = 0;
+            continue; // column_argmins ended with a barrier
+        }
         if (zin)
             for (int x = tx0; x < nx; x += txs)
             {
@@ -667,6 +871,26 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 } // namespace
+
+namespace
+{
+__global__ void __launch_bounds__(256) fill_u8(uint8_t* p, int64_t n, uint8_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+} // namespace
+
+int vc_fill_u8(vc_ctx* ctx, uint8_t* p, size_t n, uint8_t v)
+{
+    if (n == 0)
+        return VC_OK;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    fill_u8<<<g, 256, 0, ctx->stream>>>(p, (int64_t)n, v);
+    VC_CHECK_LAUNCH(ctx);
+    return VC_OK;
+}
 
 int vc_edt_sched(vc_ctx* ctx, vc_sched* s)
 {

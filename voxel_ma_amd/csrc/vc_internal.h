@@ -106,6 +106,7 @@ struct vc_vol
     uint8_t* dflag = nullptr;  // 2 x words: revisit flags per z parity
     uint64_t* alive2 = nullptr; // tile sweeps: the other alive plane
     uint32_t* tchg = nullptr;   // tile sweeps: 2 x tiles neighbour masks
+    uint8_t* eflag = nullptr;   // EDT thick-tile flags: nx*nz (y pass), ny*nz (x pass)
     // boundary stage results (computed once per volume)
     bool bnd_done = false;
     float* bnd_sites = nullptr;   // 3 floats per site
@@ -168,6 +169,8 @@ struct vc_vdesc
     uint32_t* tchg;   // 2 x tiles (by sweep parity): bit (dx+1)*9+(dy+1)*3+
                       // (dz+1) = the tile removed a voxel in the halo of its
                       // neighbour at (dx, dy, dz) (bit 13: anywhere)
+    uint8_t* fy;      // EDT y-pass tiles to compute by divide and conquer
+    uint8_t* fx;      // (set by the pass before, cleared by the tile)
     int ntx, nty;     // tiles along x and y (32 rows each)
 };
 
@@ -252,6 +255,9 @@ int vc_lut_build(vc_ctx* ctx);
 int vc_noalloc_setup(vc_ctx* ctx);
 void vc_noalloc_teardown(vc_ctx* ctx);
 int vc_edt_sched(vc_ctx* ctx, vc_sched* s);
+// n bytes of device memory set to v by a kernel on the context stream (no
+// memset node: callers may be under graph capture)
+int vc_fill_u8(vc_ctx* ctx, uint8_t* p, size_t n, uint8_t v);
 int vc_measure_sched(vc_ctx* ctx, vc_sched* s);
 // levels: n x n_levels (row i = ascending levels of volume i)
 int vc_thin_sched(vc_ctx* ctx, vc_sched* s, const float* levels,

@@ -51,7 +51,7 @@ int bits_for(int n)
 // volume; returns the bytes needed
 struct Layout
 {
-    size_t edt, ft, lam2, lvl, alive, elig, dflag, alive2, tchg;
+    size_t edt, ft, lam2, lvl, alive, elig, dflag, alive2, tchg, eflag;
     size_t desc, T, L, ctl, gany, vany, ctr, end;
 };
 
@@ -76,6 +76,7 @@ Layout layout(int nx, int ny, int nz)
     take(l.dflag, 2 * words);
     take(l.alive2, 8 * words);
     take(l.tchg, 2 * 4 * tiles);
+    take(l.eflag, (size_t)nz * ((size_t)nx + ny));
     take(l.desc, sizeof(vc_vdesc));
     take(l.T, 8 * (size_t)VC_MAX_LEVELS);
     take(l.L, 4 * (size_t)VC_MAX_LEVELS);
@@ -224,6 +225,7 @@ extern "C" int vc_skeleton_noalloc(vc_ctx* ctx, const uint8_t* d_occ, int nx,
     v->dflag = base + L.dflag;
     v->alive2 = reinterpret_cast<uint64_t*>(base + L.alive2);
     v->tchg = reinterpret_cast<uint32_t*>(base + L.tchg);
+    v->eflag = base + L.eflag;
     v->edt_done = v->thin_done = false;
 
     vc_sched* s = ctx->na_sched;
@@ -249,6 +251,8 @@ extern "C" int vc_skeleton_noalloc(vc_ctx* ctx, const uint8_t* d_occ, int nx,
     d.d1 = v->dflag + words;
     d.alive2 = v->alive2;
     d.tchg = v->tchg;
+    d.fy = v->eflag;
+    d.fx = v->eflag + (size_t)nx * nz;
     d.ntx = (nx + 31) / 32;
     d.nty = (ny + 31) / 32;
     s->voxels = N;
@@ -267,6 +271,10 @@ extern "C" int vc_skeleton_noalloc(vc_ctx* ctx, const uint8_t* d_occ, int nx,
     // one volume: no work tables, so this allocates nothing; it uploads the
     // view into the scratch array
     rc = vc_sched_apply(ctx, s, vc_schedule_default());
+    // the EDT's thick-tile flags start cleared (the array may hold another
+    // call's data there; a stale flag would cost time only)
+    if (rc == VC_OK)
+        rc = vc_fill_u8(ctx, v->eflag, (size_t)nz * ((size_t)nx + ny), 0);
     if (rc == VC_OK)
         rc = vc_edt_sched(ctx, s);
     if (rc == VC_OK)

@@ -131,6 +131,13 @@ int vc_vol_ensure(vc_ctx* ctx, vc_vol* v)
                          ((v->nz + 63) / 64);
     if (!v->tchg && (rc = vc_alloc(ctx, (void**)&v->tchg, 2 * 4 * tiles)) != VC_OK)
         return rc;
+    if (!v->eflag)
+    {
+        const size_t nf = (size_t)v->nz * ((size_t)v->nx + v->ny);
+        if ((rc = vc_alloc(ctx, (void**)&v->eflag, nf)) != VC_OK)
+            return rc;
+        VC_HIP(ctx, hipMemsetAsync(v->eflag, 0, nf, ctx->stream));
+    }
     return VC_OK;
 }
 
@@ -273,6 +280,8 @@ int vc_sched_build(vc_ctx* ctx, vc_vol* const* vols, int n,
         d.d1 = v->dflag + words;
         d.alive2 = v->alive2;
         d.tchg = v->tchg;
+        d.fy = v->eflag;
+        d.fx = v->eflag + (size_t)v->nx * v->nz;
         d.ntx = (int)cdiv(v->nx, kTileXY);
         d.nty = (int)cdiv(v->ny, kTileXY);
         s->voxels += v->N;
