@@ -104,9 +104,9 @@ struct Box
 // channels at a time, chunk-major (16 B per voxel and chunk, so the 16 lanes
 // of a tile read consecutive entries).  Within a slice the K pairs (tap,
 // chunk) run tap-major, four per MFMA (lane group g takes pair 4 j + g), so a
-// layer with fewer than 4 chunks still fills K; A comes from LDS, B (the
-// weights of that tap and chunk, shared by every workgroup) from global
-// memory, each B reused by the wavefront's MT tiles.
+// layer with fewer than 4 chunks still fills K; the voxel operand comes
+// from LDS, the weight operand (that tap and chunk, shared by every
+// workgroup) from global memory, each reused by the wavefront's MT tiles.
 //
 // kCat: the input is the decoder's concatenation [up | skip] without the
 // concatenated tensor: channels below c1 are the coarse level's (nearest
@@ -116,8 +116,23 @@ struct Box
 // four per MFMA (lane group g takes pair p0 + g).  S is a compile-time
 // constant here, so a pair's tap and chunk, and the tap's offsets, are
 // constant-divisor arithmetic (the run-time division by S had cost ~30 VALU
-// instructions per MFMA); the loop stays rolled, which keeps the registers
-// of the kernel, and with them its occupancy, where they were.
+// instructions per MFMA).
+//
+// The weights are the MFMA's first operand and the voxels its second, so a
+// lane ends with 4 consecutive output channels of one voxel (channel
+// n 16 + 4 g + r of voxel i of the tile) and writes them with one 8-byte store.
+//
+// With two chunks per slice (every layer's slices: all channel counts here
+// are multiples of 16), lane group g takes tap 2 j + (g >> 1) and chunk g & 1
+// of MFMA j; tap 2 j's region offset is a constant after unrolling, and tap
+// 2 j + 1 lies one of three fixed steps further (+1 in z, or the wrap into
+// the next y or x row), so a lane's LDS address is one of three registers
+// plus an immediate, and the weights of MFMA j sit at a wave-uniform offset.
+constexpr int conv_tap_off(int tap, int RY, int RZ)
+{
+    return ((tap / 9 - 1) * RY + (tap / 3) % 3 - 1) * RZ + tap % 3 - 1;
+}
+
 template <int NT, int MT, int NR, int RY, int RZ, int S>
 __device__ __forceinline__ void conv_slice(const bf16x8* __restrict__ reg,
                                            const int (&tb)[MT],
@@ -128,27 +143,105 @@ __device__ __forceinline__ void conv_slice(const bf16x8* __restrict__ reg,
     constexpr int NP = 27 * S;
     const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
     const bf16x8 zero8 = {};
-#pragma unroll VC_CONV_UNROLL
-    for (int p0 = 0; p0 < NP; p0 += 4)
+    if constexpr (S == 2)
     {
-        const int p = p0 + g;
-        const bool v = p < NP;
-        const int tap = v ? p / S : 13, c = v ? p % S : 0;
-        const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-        const int ro = c * NR + (dx * RY + dy) * RZ + dz + i;
-        const int wo = (tap * cin8 + s0 + c) * Cout + i;
-        bf16x8 a[MT];
+        constexpr int NG = (NP + 3) / 4;
+        const int h = g >> 1, c = g & 1;
+        const int lb = c * NR + i;
+        const int st[3] = {h, h * (RZ - 2), h * (RY * RZ - 2 * RZ - 2)};
+        const int wl = (h * cin8 + c) * Cout + i;
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
-            a[m] = v ? reg[ro + tb[m]] : zero8;
+        for (int j = 0; j < NG; ++j)
+        {
+            const int t0 = 2 * j;
+            const bool v = 4 * j + g < NP; // false only for tap 27 of the last MFMA
+            const int step = t0 % 3 < 2 ? 0 : ((t0 / 3) % 3 < 2 ? 1 : 2);
+            const int ro = lb + conv_tap_off(t0, RY, RZ) + (v ? st[step] : 0);
+            const bf16x8* Wj = W + (t0 * cin8 + s0) * Cout;
+            bf16x8 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                a[m] = v ? reg[ro + tb[m]] : zero8;
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+            {
+                const bf16x8 b = v ? Wj[wl + n * 16] : zero8;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[m], acc[m][n],
+                                                                        0, 0, 0);
+            }
+        }
+    }
+    else
+    {
+#pragma unroll VC_CONV_UNROLL
+        for (int p0 = 0; p0 < NP; p0 += 4)
+        {
+            const int p = p0 + g;
+            const bool v = p < NP;
+            const int tap = v ? p / S : 13, c = v ? p % S : 0;
+            const int ro = c * NR + conv_tap_off(tap, RY, RZ) + i;
+            const int wo = (tap * cin8 + s0 + c) * Cout + i;
+            bf16x8 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                a[m] = v ? reg[ro + tb[m]] : zero8;
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+            {
+                const bf16x8 b = v ? W[wo + n * 16] : zero8;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[m], acc[m][n],
+                                                                        0, 0, 0);
+            }
+        }
+    }
+}
+
+// Bias, ReLU and the zeroing outside the padded volume for one wavefront's
+// MT tiles: lane (i, g) holds channels n 16 + 4 g .. + 3 of voxel i.
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+template <int NT, int MT, int BY, int NZT>
+__device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], bf16* __restrict__ out,
+                                           const float* __restrict__ bias, int relu,
+                                           int x0, int y0, int z0, int X, int Y, int Z,
+                                           const Box& box)
+{
+    constexpr int Cout = NT * 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    float bv[NT][4];
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            bv[n][r] = bias ? bias[n * 16 + 4 * g + r] : 0.0f;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        const int x = x0 + row / BY, y = y0 + row % BY, z = z0 + 16 * seg + i;
+        if (x >= X || y >= Y || z >= Z)
+            continue;
+        const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                           (unsigned)(box.oy + y) < (unsigned)box.py &&
+                           (unsigned)(box.oz + z) < (unsigned)box.pz;
+        bf16* o = out + (((int64_t)x * Y + y) * Z + z) * Cout + 4 * g;
 #pragma unroll
         for (int n = 0; n < NT; ++n)
         {
-            const bf16x8 b = v ? W[wo + n * 16] : zero8;
+            bf16x4 q;
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
-                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][n], 0,
-                                                                    0, 0);
+            for (int r = 0; r < 4; ++r)
+            {
+                float val = acc[m][n][r] + bv[n][r];
+                if (relu)
+                    val = fmaxf(val, 0.0f);
+                q[r] = inbox ? (bf16)val : (bf16)0.0f;
+            }
+            *reinterpret_cast<bf16x4*>(o + n * 16) = q;
         }
     }
 }
@@ -223,36 +316,7 @@ __global__ void __launch_bounds__(kBlock)
         else
             conv_slice<NT, MT, NR, RY, RZ, 1>(reg, tb, W, s0, cin8, acc);
     }
-    // lane holds D[4g + r][i] of tile m: voxel 4 g + r of the tile, channel
-    // n * 16 + i
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-    {
-        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
-        const int x = x0 + row / BY, y = y0 + row % BY;
-        if (x >= X || y >= Y)
-            continue;
-        const bool inxy = (unsigned)(box.ox + x) < (unsigned)box.px &&
-                          (unsigned)(box.oy + y) < (unsigned)box.py;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-            const int z = z0 + 16 * seg + 4 * g + r;
-            if (z >= Z)
-                continue;
-            const int64_t v = ((int64_t)x * Y + y) * Z + z;
-            const bool inbox = inxy && (unsigned)(box.oz + z) < (unsigned)box.pz;
-#pragma unroll
-            for (int n = 0; n < NT; ++n)
-            {
-                const int co = n * 16 + i;
-                float val = acc[m][n][r] + (bias ? bias[co] : 0.0f);
-                if (relu)
-                    val = fmaxf(val, 0.0f);
-                out[v * Cout + co] = inbox ? (bf16)val : (bf16)0.0f;
-            }
-        }
-    }
+    conv_store<NT, MT, BY, NZT>(acc, out, bias, relu, x0, y0, z0, X, Y, Z, box);
 }
 
 // Weight gradient partials.  Block (chunk, tg): taps (dx, dy) = (tg/3 - 1,
@@ -770,13 +834,13 @@ __global__ void __launch_bounds__(kBlock)
     const int by = blk % nby, bx = blk / nby;
     const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
     int tb[MT];
-    f32x4 acc[MT];
+    f32x4 acc[MT][1];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
     {
         const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
         tb[m] = ((row / BY + 1) * RY + (row % BY + 1)) * RZ + 1 + 16 * seg;
-        acc[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc[m][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     }
     for (int e = threadIdx.x; e < NR; e += kBlock)
     {
@@ -809,31 +873,11 @@ __global__ void __launch_bounds__(kBlock)
         {
             const u32x4 w = {tv[0] ? reg2[ro[0] + tb[m]] : 0u, tv[1] ? reg2[ro[1] + tb[m]] : 0u,
                              tv[2] ? reg2[ro[2] + tb[m]] : 0u, tv[3] ? reg2[ro[3] + tb[m]] : 0u};
-            acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w), b,
-                                                             acc[m], 0, 0, 0);
+            acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                b, __builtin_bit_cast(bf16x8, w), acc[m][0], 0, 0, 0);
         }
     }
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-    {
-        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
-        const int x = x0 + row / BY, y = y0 + row % BY;
-        if (x >= X || y >= Y)
-            continue;
-        const bool inxy = (unsigned)(box.ox + x) < (unsigned)box.px &&
-                          (unsigned)(box.oy + y) < (unsigned)box.py;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-            const int z = z0 + 16 * seg + 4 * g + r;
-            if (z >= Z)
-                continue;
-            const int64_t v = ((int64_t)x * Y + y) * Z + z;
-            const bool inbox = inxy && (unsigned)(box.oz + z) < (unsigned)box.pz;
-            const float val = fmaxf(acc[m][r] + bias[i], 0.0f);
-            out[v * 16 + i] = inbox ? (bf16)val : (bf16)0.0f;
-        }
-    }
+    conv_store<1, MT, BY, NZT>(acc, out, bias, 1, x0, y0, z0, X, Y, Z, box);
 }
 
 // first layer's W[27][2][16] -> W4[8][16][8] (pair p slot e: tap 4p + e/2,
