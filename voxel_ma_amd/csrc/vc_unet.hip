@@ -246,11 +246,97 @@ __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], bf16* __r
     }
 }
 
+// Staging of one slice (SC chunks) of a brick's input region.  Thread t
+// stages chunk t % SC of region voxels t / SC + k kBlock / SC.  A brick whose
+// region lies inside the volume (every brick but those on its faces) takes
+// the first branch: no bounds checks and 32-bit offsets from a wave-uniform
+// base (the host passes off32 when a region's extent fits them), with all of
+// a thread's loads issued before its LDS stores.
+template <int NR, int RY, int RZ, int SC, bool kCat>
+__device__ __forceinline__ void conv_stage(bf16x8* __restrict__ reg, const bf16* __restrict__ in,
+                                           const bf16* __restrict__ in2, int c1, int cinp,
+                                           int s0, int x0, int y0, int z0, int X, int Y,
+                                           int Z, bool inner)
+{
+    constexpr int NIT = (NR * SC + kBlock - 1) / kBlock, RPI = kBlock / SC;
+    static_assert(kBlock % SC == 0, "chunk per thread");
+    const int c = threadIdx.x % SC, r0 = threadIdx.x / SC;
+    const int ch = s0 + c, c18 = c1 >> 3, c2 = cinp - c1;
+    const bf16x8 zero8 = {};
+    if (inner)
+    {
+        const bf16* base;
+        int cs, ys, zs, ax = 0, ay = 0, az = 0;
+        if (kCat && ch < c18)
+        {
+            // coarse voxel (x >> 1, ...) of the region voxel: offsets from
+            // the coarse voxel of the region origin
+            ax = (x0 - 1) & 1;
+            ay = (y0 - 1) & 1;
+            az = (z0 - 1) & 1;
+            ys = Y >> 1;
+            zs = Z >> 1;
+            cs = c1;
+            base = in + ((((int64_t)((x0 - 1) >> 1)) * ys + ((y0 - 1) >> 1)) * zs +
+                         ((z0 - 1) >> 1)) * c1 + ch * 8;
+        }
+        else
+        {
+            ys = Y;
+            zs = Z;
+            cs = kCat ? c2 : cinp;
+            base = (kCat ? in2 : in) + ((((int64_t)(x0 - 1)) * Y + (y0 - 1)) * Z + (z0 - 1)) * cs +
+                   (kCat ? ch - c18 : ch) * 8;
+        }
+        const int sh = kCat && ch < c18 ? 1 : 0;
+        bf16x8 v[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k)
+        {
+            const int r = min(r0 + k * RPI, NR - 1);
+            const int rz = r % RZ, rxy = r / RZ, rx = rxy / RY, ry = rxy % RY;
+            const int off = ((((ax + rx) >> sh) * ys + ((ay + ry) >> sh)) * zs +
+                             ((az + rz) >> sh)) * cs;
+            v[k] = *reinterpret_cast<const bf16x8*>(base + off);
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k)
+        {
+            const int r = r0 + k * RPI;
+            if (k < NIT - 1 || r < NR)
+                reg[c * NR + r] = v[k];
+        }
+        return;
+    }
+    for (int r = r0; r < NR; r += RPI)
+    {
+        const int rz = r % RZ, rxy = r / RZ;
+        const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
+        bf16x8 v = zero8;
+        if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
+            (unsigned)zz < (unsigned)Z)
+        {
+            if (kCat && ch < c18)
+                v = *reinterpret_cast<const bf16x8*>(
+                    in + ((((int64_t)(xx >> 1) * (Y >> 1)) + (yy >> 1)) * (Z >> 1) +
+                          (zz >> 1)) * c1 + ch * 8);
+            else if (kCat)
+                v = *reinterpret_cast<const bf16x8*>(
+                    in2 + (((int64_t)xx * Y + yy) * Z + zz) * c2 + (ch - c18) * 8);
+            else
+                v = *reinterpret_cast<const bf16x8*>(
+                    in + (((int64_t)xx * Y + yy) * Z + zz) * cinp + ch * 8);
+        }
+        reg[c * NR + r] = v;
+    }
+}
+
 template <int NT, int BX, int BY, int NZT, int SC, bool kCat>
 __global__ void __launch_bounds__(kBlock)
     conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
                  int cinp, bf16* __restrict__ out, const bf16x8* __restrict__ W,
-                 const float* __restrict__ bias, int X, int Y, int Z, int relu, Box box)
+                 const float* __restrict__ bias, int X, int Y, int Z, int relu, Box box,
+                 int off32)
 {
     constexpr int Cout = NT * 16;
     constexpr int MT = BX * BY * NZT / 4;
@@ -258,9 +344,8 @@ __global__ void __launch_bounds__(kBlock)
     constexpr int BZ = 16 * NZT, RY = BY + 2, RZ = BZ + 2;
     constexpr int NR = (BX + 2) * RY * RZ;
     __shared__ bf16x8 reg[SC * NR];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = lane & 15, g = lane >> 4;
-    const int cin8 = cinp >> 3, c18 = c1 >> 3, c2 = cinp - c1;
+    const int wv = threadIdx.x >> 6;
+    const int cin8 = cinp >> 3;
     const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
     int blk = blockIdx.x;
     const int bz = blk % nbz;
@@ -281,40 +366,16 @@ __global__ void __launch_bounds__(kBlock)
         const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
         tb[m] = ((row / BY + 1) * RY + (row % BY + 1)) * RZ + 1 + 16 * seg;
     }
-    const bf16x8 zero8 = {};
+    // the chunk count is a multiple of SC (checked by the host)
+    const bool inner = off32 && x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 + BX < X &&
+                       y0 + BY < Y && z0 + BZ < Z;
     for (int s0 = 0; s0 < cin8; s0 += SC)
     {
-        const int S = min(SC, cin8 - s0);
         __syncthreads(); // the previous slice's reads are done
-        for (int e = threadIdx.x; e < NR * S; e += kBlock)
-        {
-            // S is SC or (an odd chunk count's last slice) 1: constant divisors
-            const int c = S == SC ? e % SC : 0, r = S == SC ? e / SC : e;
-            const int rz = r % RZ, rxy = r / RZ;
-            const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
-            bf16x8 v = zero8;
-            if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
-                (unsigned)zz < (unsigned)Z)
-            {
-                const int ch = s0 + c;
-                if (kCat && ch < c18)
-                    v = *reinterpret_cast<const bf16x8*>(
-                        in + ((((int64_t)(xx >> 1) * (Y >> 1)) + (yy >> 1)) * (Z >> 1) +
-                              (zz >> 1)) * c1 + ch * 8);
-                else if (kCat)
-                    v = *reinterpret_cast<const bf16x8*>(
-                        in2 + (((int64_t)xx * Y + yy) * Z + zz) * c2 + (ch - c18) * 8);
-                else
-                    v = *reinterpret_cast<const bf16x8*>(
-                        in + (((int64_t)xx * Y + yy) * Z + zz) * cinp + ch * 8);
-            }
-            reg[c * NR + r] = v;
-        }
+        conv_stage<NR, RY, RZ, SC, kCat>(reg, in, in2, c1, cinp, s0, x0, y0, z0, X, Y, Z,
+                                         inner);
         __syncthreads();
-        if (S == SC)
-            conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
-        else
-            conv_slice<NT, MT, NR, RY, RZ, 1>(reg, tb, W, s0, cin8, acc);
+        conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
     }
     conv_store<NT, MT, BY, NZT>(acc, out, bias, relu, x0, y0, z0, X, Y, Z, box);
 }
@@ -1244,9 +1305,11 @@ void launch_conv_t(hipStream_t S, const bf16* in, const bf16* in2, int c1, int c
 {
     const int64_t nb = (int64_t)((X + kCBX - 1) / kCBX) * ((Y + kCBY - 1) / kCBY) *
                        ((Z + 16 * kCNZT - 1) / (16 * kCNZT));
+    // a region spans BX + 2 x planes: its offsets from its first voxel fit 31 bits
+    const int off32 = (int64_t)(kCBX + 2) * Y * Z * cinp < (int64_t(1) << 31);
     conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat><<<(unsigned)nb, kBlock, 0, S>>>(
         in, in2, c1, cinp, out, reinterpret_cast<const bf16x8*>(W), bias, X, Y, Z, relu,
-        box);
+        box, off32);
 }
 
 // box: the region's origin and the padded-volume extent at this level.  With
@@ -1258,6 +1321,8 @@ int launch_conv(vc_ctx* ctx, const bf16* in, int cinp, bf16* out, int cout,
 {
     hipStream_t S = ctx->stream;
     (void)kc;
+    if (cinp % (8 * kCSC) != 0 || (in2 && c1 % 8 != 0))
+        return vc_fail(ctx, VC_ERR_INVALID, "conv: %d input channels", cinp);
 #define VC_CONV_CASE(nt)                                                              \
     case nt:                                                                          \
         if (in2)                                                                      \
