@@ -201,23 +201,26 @@ __device__ __forceinline__ void conv_slice(const bf16x8* __restrict__ reg,
 }
 
 // Bias, ReLU and the zeroing outside the padded volume for one wavefront's
-// MT tiles: lane (i, g) holds channels n 16 + 4 g .. + 3 of voxel i.
+// MT tiles: lane (i, g) holds channels n 16 + 4 g .. + 3 of voxel i.  With
+// off32 the output offsets from the brick's first voxel fit 31 bits.
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-template <int NT, int MT, int BY, int NZT>
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <int NT, int MT, int BY, int NZT, bool kRelu>
 __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], bf16* __restrict__ out,
-                                           const float* __restrict__ bias, int relu,
-                                           int x0, int y0, int z0, int X, int Y, int Z,
-                                           const Box& box)
+                                           const float* __restrict__ bias, int x0, int y0,
+                                           int z0, int X, int Y, int Z, const Box& box,
+                                           int off32)
 {
     constexpr int Cout = NT * 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 15, g = lane >> 4;
-    float bv[NT][4];
+    f32x4 bv[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             bv[n][r] = bias ? bias[n * 16 + 4 * g + r] : 0.0f;
+    bf16* ob = out + (((int64_t)x0 * Y + y0) * Z + z0) * Cout + 4 * g;
 #pragma unroll
     for (int m = 0; m < MT; ++m)
     {
@@ -228,20 +231,19 @@ __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], bf16* __r
         const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
                            (unsigned)(box.oy + y) < (unsigned)box.py &&
                            (unsigned)(box.oz + z) < (unsigned)box.pz;
-        bf16* o = out + (((int64_t)x * Y + y) * Z + z) * Cout + 4 * g;
+        const int lo = ((row / BY) * Y + row % BY) * Z + 16 * seg + i;
+        bf16* o = off32 ? ob + lo * Cout : ob + (int64_t)lo * Cout;
 #pragma unroll
         for (int n = 0; n < NT; ++n)
         {
-            bf16x4 q;
+            f32x4 val = acc[m][n] + bv[n];
+            if (kRelu)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-            {
-                float val = acc[m][n][r] + bv[n][r];
-                if (relu)
-                    val = fmaxf(val, 0.0f);
-                q[r] = inbox ? (bf16)val : (bf16)0.0f;
-            }
-            *reinterpret_cast<bf16x4*>(o + n * 16) = q;
+                for (int r = 0; r < 4; ++r)
+                    val[r] = fmaxf(val[r], 0.0f);
+            u32x2 q = __builtin_bit_cast(u32x2, __builtin_convertvector(val, bf16x4));
+            q = inbox ? q : u32x2{0u, 0u};
+            *reinterpret_cast<u32x2*>(o + n * 16) = q;
         }
     }
 }
@@ -331,12 +333,11 @@ __device__ __forceinline__ void conv_stage(bf16x8* __restrict__ reg, const bf16*
     }
 }
 
-template <int NT, int BX, int BY, int NZT, int SC, bool kCat>
+template <int NT, int BX, int BY, int NZT, int SC, bool kCat, bool kRelu>
 __global__ void __launch_bounds__(kBlock)
     conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
                  int cinp, bf16* __restrict__ out, const bf16x8* __restrict__ W,
-                 const float* __restrict__ bias, int X, int Y, int Z, int relu, Box box,
-                 int off32)
+                 const float* __restrict__ bias, int X, int Y, int Z, Box box, int off32)
 {
     constexpr int Cout = NT * 16;
     constexpr int MT = BX * BY * NZT / 4;
@@ -377,7 +378,7 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
         conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
     }
-    conv_store<NT, MT, BY, NZT>(acc, out, bias, relu, x0, y0, z0, X, Y, Z, box);
+    conv_store<NT, MT, BY, NZT, kRelu>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
 }
 
 // Weight gradient partials.  Block (chunk, tg): taps (dx, dy) = (tg/3 - 1,
@@ -879,7 +880,7 @@ template <int BX, int BY, int NZT>
 __global__ void __launch_bounds__(kBlock)
     conv_first_lds(const bf16* __restrict__ in, bf16* __restrict__ out,
                    const bf16x8* __restrict__ W4, const float* __restrict__ bias, int X,
-                   int Y, int Z, Box box)
+                   int Y, int Z, Box box, int off32)
 {
     constexpr int MT = BX * BY * NZT / 4;
     static_assert(MT * 4 == BX * BY * NZT, "tiles per wavefront");
@@ -938,7 +939,7 @@ __global__ void __launch_bounds__(kBlock)
                 b, __builtin_bit_cast(bf16x8, w), acc[m][0], 0, 0, 0);
         }
     }
-    conv_store<1, MT, BY, NZT>(acc, out, bias, 1, x0, y0, z0, X, Y, Z, box);
+    conv_store<1, MT, BY, NZT, true>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
 }
 
 // first layer's W[27][2][16] -> W4[8][16][8] (pair p slot e: tap 4p + e/2,
@@ -1298,18 +1299,23 @@ int pack_weights(vc_ctx* ctx, vc_surrogate* s)
 // input staged SC chunks at a time
 constexpr int kCBX = VC_CONV_BX, kCBY = VC_CONV_BY, kCNZT = VC_CONV_NZT, kCSC = VC_CONV_SC;
 
-template <int NT, bool kCat>
+// whether a brick's input region (BX + 2 x planes) and output brick fit 31-bit
+// element offsets from their first voxel
+int conv_off32(int Y, int Z, int cmax)
+{
+    return (int64_t)(kCBX + 2) * Y * Z * cmax < (int64_t(1) << 31);
+}
+
+template <int NT, bool kCat, bool kRelu>
 void launch_conv_t(hipStream_t S, const bf16* in, const bf16* in2, int c1, int cinp,
                    bf16* out, const bf16* W, const float* bias, int X, int Y, int Z,
-                   int relu, const Box& box)
+                   const Box& box)
 {
     const int64_t nb = (int64_t)((X + kCBX - 1) / kCBX) * ((Y + kCBY - 1) / kCBY) *
                        ((Z + 16 * kCNZT - 1) / (16 * kCNZT));
-    // a region spans BX + 2 x planes: its offsets from its first voxel fit 31 bits
-    const int off32 = (int64_t)(kCBX + 2) * Y * Z * cinp < (int64_t(1) << 31);
-    conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat><<<(unsigned)nb, kBlock, 0, S>>>(
-        in, in2, c1, cinp, out, reinterpret_cast<const bf16x8*>(W), bias, X, Y, Z, relu,
-        box, off32);
+    conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat, kRelu><<<(unsigned)nb, kBlock, 0, S>>>(
+        in, in2, c1, cinp, out, reinterpret_cast<const bf16x8*>(W), bias, X, Y, Z, box,
+        conv_off32(Y, Z, std::max(cinp, NT * 16)));
 }
 
 // box: the region's origin and the padded-volume extent at this level.  With
@@ -1323,13 +1329,19 @@ int launch_conv(vc_ctx* ctx, const bf16* in, int cinp, bf16* out, int cout,
     (void)kc;
     if (cinp % (8 * kCSC) != 0 || (in2 && c1 % 8 != 0))
         return vc_fail(ctx, VC_ERR_INVALID, "conv: %d input channels", cinp);
+    // the decoder's concatenations are forward (ReLU) layers
+    if (in2 && !relu)
+        return vc_fail(ctx, VC_ERR_INVALID, "conv: concatenated input without ReLU");
 #define VC_CONV_CASE(nt)                                                              \
     case nt:                                                                          \
         if (in2)                                                                      \
-            launch_conv_t<nt, true>(S, in, in2, c1, cinp, out, W, bias, X, Y, Z, relu, b); \
+            launch_conv_t<nt, true, true>(S, in, in2, c1, cinp, out, W, bias, X, Y, Z, b); \
+        else if (relu)                                                                \
+            launch_conv_t<nt, false, true>(S, in, nullptr, 0, cinp, out, W, bias, X, Y, Z, \
+                                           b);                                        \
         else                                                                          \
-            launch_conv_t<nt, false>(S, in, nullptr, 0, cinp, out, W, bias, X, Y, Z, relu, \
-                                     b);                                              \
+            launch_conv_t<nt, false, false>(S, in, nullptr, 0, cinp, out, W, bias, X, Y, \
+                                            Z, b);                                    \
         break;
     switch (cout / 16)
     {
@@ -1413,7 +1425,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
             conv_first_lds<kCBX, kCBY, kCNZT><<<(unsigned)nb, kBlock, 0, S>>>(
                 x, s->act[0], reinterpret_cast<const bf16x8*>(s->W4),
                 s->theta + param_offset(0) + 27LL * L.cin * L.cout, s->X, s->Y, s->Z,
-                level_box(s, 0));
+                level_box(s, 0), conv_off32(s->Y, s->Z, 16));
             VC_CHECK_LAUNCH(ctx);
             return VC_OK;
         }
