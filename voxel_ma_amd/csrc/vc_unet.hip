@@ -248,6 +248,21 @@ __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], bf16* __r
     }
 }
 
+// The brick of a workgroup.  Workgroups are dealt round-robin to the 8 XCDs
+// (each with its own L2); XCD k takes the k-th eighth of the bricks in order
+// (z fastest), so bricks sharing halo planes of their input regions share an
+// L2.  The grid is 8 ceil(nb / 8) workgroups; -1 past the last brick.
+__device__ __forceinline__ int conv_brick(int nb)
+{
+    const int per = (nb + 7) >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    return b < nb ? b : -1;
+}
+inline unsigned conv_grid(int64_t nb)
+{
+    return (unsigned)(8 * ((nb + 7) / 8));
+}
+
 // Staging of one slice (SC chunks) of a brick's input region.  Thread t
 // stages chunk t % SC of region voxels t / SC + k kBlock / SC.  A brick whose
 // region lies inside the volume (every brick but those on its faces) takes
@@ -348,7 +363,9 @@ __global__ void __launch_bounds__(kBlock)
     const int wv = threadIdx.x >> 6;
     const int cin8 = cinp >> 3;
     const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
-    int blk = blockIdx.x;
+    int blk = conv_brick(nbz * nby * ((X + BX - 1) / BX));
+    if (blk < 0)
+        return;
     const int bz = blk % nbz;
     blk /= nbz;
     const int by = blk % nby, bx = blk / nby;
@@ -890,7 +907,9 @@ __global__ void __launch_bounds__(kBlock)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 15, g = lane >> 4;
     const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
-    int blk = blockIdx.x;
+    int blk = conv_brick(nbz * nby * ((X + BX - 1) / BX));
+    if (blk < 0)
+        return;
     const int bz = blk % nbz;
     blk /= nbz;
     const int by = blk % nby, bx = blk / nby;
@@ -1313,7 +1332,7 @@ void launch_conv_t(hipStream_t S, const bf16* in, const bf16* in2, int c1, int c
 {
     const int64_t nb = (int64_t)((X + kCBX - 1) / kCBX) * ((Y + kCBY - 1) / kCBY) *
                        ((Z + 16 * kCNZT - 1) / (16 * kCNZT));
-    conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat, kRelu><<<(unsigned)nb, kBlock, 0, S>>>(
+    conv_fwd_lds<NT, kCBX, kCBY, kCNZT, kCSC, kCat, kRelu><<<conv_grid(nb), kBlock, 0, S>>>(
         in, in2, c1, cinp, out, reinterpret_cast<const bf16x8*>(W), bias, X, Y, Z, box,
         conv_off32(Y, Z, std::max(cinp, NT * 16)));
 }
@@ -1422,7 +1441,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
             const int64_t nb = (int64_t)((s->X + kCBX - 1) / kCBX) *
                                ((s->Y + kCBY - 1) / kCBY) *
                                ((s->Z + 16 * kCNZT - 1) / (16 * kCNZT));
-            conv_first_lds<kCBX, kCBY, kCNZT><<<(unsigned)nb, kBlock, 0, S>>>(
+            conv_first_lds<kCBX, kCBY, kCNZT><<<conv_grid(nb), kBlock, 0, S>>>(
                 x, s->act[0], reinterpret_cast<const bf16x8*>(s->W4),
                 s->theta + param_offset(0) + 27LL * L.cin * L.cout, s->X, s->Y, s->Z,
                 level_box(s, 0), conv_off32(s->Y, s->Z, 16));
