@@ -46,6 +46,9 @@ constexpr int kHalo = VC_SURROGATE_HALO;
 #ifndef VC_CONV_SC
 #define VC_CONV_SC 2
 #endif
+#ifndef VC_CONV_GX
+#define VC_CONV_GX 4
+#endif
 #ifndef VC_CONV_UNROLL
 #define VC_CONV_UNROLL 4
 #endif
@@ -363,12 +366,16 @@ __global__ void __launch_bounds__(kBlock)
     const int wv = threadIdx.x >> 6;
     const int cin8 = cinp >> 3;
     const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
-    int blk = conv_brick(nbz * nby * ((X + BX - 1) / BX));
+    const int nbx = (X + BX - 1) / BX;
+    int blk = conv_brick(nbz * nby * nbx);
     if (blk < 0)
         return;
     const int bz = blk % nbz;
     blk /= nbz;
-    const int by = blk % nby, bx = blk / nby;
+    // x groups of VC_CONV_GX bricks, y within a group, then the groups
+    const int q = blk / (nby * VC_CONV_GX), rem = blk % (nby * VC_CONV_GX);
+    const int gsz = min(VC_CONV_GX, nbx - q * VC_CONV_GX);
+    const int by = rem / gsz, bx = q * VC_CONV_GX + rem % gsz;
     const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
     f32x4 acc[MT][NT];
 #pragma unroll
@@ -907,12 +914,16 @@ __global__ void __launch_bounds__(kBlock)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 15, g = lane >> 4;
     const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
-    int blk = conv_brick(nbz * nby * ((X + BX - 1) / BX));
+    const int nbx = (X + BX - 1) / BX;
+    int blk = conv_brick(nbz * nby * nbx);
     if (blk < 0)
         return;
     const int bz = blk % nbz;
     blk /= nbz;
-    const int by = blk % nby, bx = blk / nby;
+    // x groups of VC_CONV_GX bricks, y within a group, then the groups
+    const int q = blk / (nby * VC_CONV_GX), rem = blk % (nby * VC_CONV_GX);
+    const int gsz = min(VC_CONV_GX, nbx - q * VC_CONV_GX);
+    const int by = rem / gsz, bx = q * VC_CONV_GX + rem % gsz;
     const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
     int tb[MT];
     f32x4 acc[MT][1];
