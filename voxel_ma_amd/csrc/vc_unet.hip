@@ -730,11 +730,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // network input over a region of dims (X, Y, Z) whose voxel (0,0,0) is
-// volume voxel (ox, oy, oz): [occ, sqrt(max(edt, 0)) / 32, 0 x 6]
+// volume voxel (ox, oy, oz): x0 = [occ, sqrt(max(edt, 0)) / 32] (the first
+// layer's input, 4 bytes per voxel) and, for training, x8 = the same
+// padded to 8 channels (the layout of layer 0's weight gradient)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(kBlock)
     prep_input(const uint8_t* __restrict__ occ, const int32_t* __restrict__ edt,
                int nx, int ny, int nz, int ox, int oy, int oz,
-               bf16* __restrict__ x0, int X, int Y, int Z)
+               bf16* __restrict__ x0, bf16* __restrict__ x8, int X, int Y, int Z)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t V = (int64_t)X * Y * Z;
@@ -753,10 +756,15 @@ __global__ void __launch_bounds__(kBlock)
         const int32_t e = edt[i];
         d = e > 0 ? sqrtf((float)e) / kScale : 0.0f;
     }
-    bf16x8 w = {};
-    w[0] = (bf16)o;
-    w[1] = (bf16)d;
-    reinterpret_cast<bf16x8*>(x0)[v] = w;
+    const bf16x2 w2 = {(bf16)o, (bf16)d};
+    reinterpret_cast<bf16x2*>(x0)[v] = w2;
+    if (x8)
+    {
+        bf16x8 w = {};
+        w[0] = w2[0];
+        w[1] = w2[1];
+        reinterpret_cast<bf16x8*>(x8)[v] = w;
+    }
 }
 
 // head: out[v][o] = sum_c d0[v][c] Wh[c][o] + bh[o] (FP32)
@@ -934,16 +942,41 @@ __global__ void __launch_bounds__(kBlock)
         tb[m] = ((row / BY + 1) * RY + (row % BY + 1)) * RZ + 1 + 16 * seg;
         acc[m][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    for (int e = threadIdx.x; e < NR; e += kBlock)
+    // the region (a brick inside the volume: no bounds checks, 32-bit offsets
+    // from its first voxel, every load issued before the LDS stores)
+    constexpr int NIT = (NR + kBlock - 1) / kBlock;
+    if (off32 && x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 + BX < X && y0 + BY < Y &&
+        z0 + BZ < Z)
     {
-        const int rz = e % RZ, rxy = e / RZ;
-        const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
-        uint32_t v = 0u;
-        if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
-            (unsigned)zz < (unsigned)Z)
-            v = *reinterpret_cast<const uint32_t*>(in + (((int64_t)xx * Y + yy) * Z + zz) * 8);
-        reg2[e] = v;
+        const uint32_t* base = reinterpret_cast<const uint32_t*>(in) +
+                               (((int64_t)(x0 - 1)) * Y + (y0 - 1)) * Z + (z0 - 1);
+        uint32_t v[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k)
+        {
+            const int e = min((int)threadIdx.x + k * kBlock, NR - 1);
+            const int rz = e % RZ, rxy = e / RZ;
+            v[k] = base[((rxy / RY) * Y + rxy % RY) * Z + rz];
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k)
+        {
+            const int e = threadIdx.x + k * kBlock;
+            if (k < NIT - 1 || e < NR)
+                reg2[e] = v[k];
+        }
     }
+    else
+        for (int e = threadIdx.x; e < NR; e += kBlock)
+        {
+            const int rz = e % RZ, rxy = e / RZ;
+            const int xx = x0 - 1 + rxy / RY, yy = y0 - 1 + rxy % RY, zz = z0 - 1 + rz;
+            uint32_t v = 0u;
+            if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
+                (unsigned)zz < (unsigned)Z)
+                v = reinterpret_cast<const uint32_t*>(in)[((int64_t)xx * Y + yy) * Z + zz];
+            reg2[e] = v;
+        }
     __syncthreads();
 #pragma unroll
     for (int p0 = 0; p0 < 8; p0 += 4)
@@ -1164,6 +1197,7 @@ struct vc_surrogate
     std::vector<void*> ws;
     bf16* act[14] = {}; // ReLU outputs of L0a..L6b
     bf16 *x0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+    bf16* x8 = nullptr; // the input padded to 8 channels (training)
     bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
     float* out = nullptr;
     // backward workspace (allocated on first use)
@@ -1198,7 +1232,7 @@ void free_ws(vc_surrogate* s)
         s->act[l] = nullptr;
         s->gact[l] = nullptr;
     }
-    s->x0 = s->p1 = s->p2 = s->p3 = s->cat2 = s->cat1 = s->cat0 = nullptr;
+    s->x0 = s->x8 = s->p1 = s->p2 = s->p3 = s->cat2 = s->cat1 = s->cat0 = nullptr;
     s->gcat2 = s->gcat1 = s->gcat0 = s->gp1 = s->gp2 = s->gp3 = nullptr;
     s->out = s->dout = s->part = nullptr;
     s->part_floats = 0;
@@ -1247,7 +1281,7 @@ int ensure_ws(vc_ctx* ctx, vc_surrogate* s, int X, int Y, int Z, bool bw)
             return ws_alloc(ctx, s, (void**)p,
                             2 * (size_t)level_voxels(s, level) * C);
         };
-        if ((rc = A(&s->x0, 0, 8)) != VC_OK)
+        if ((rc = A(&s->x0, 0, 2)) != VC_OK)
             return rc;
         for (int l = 0; l < 14; ++l)
             if ((rc = A(&s->act[l], kLayers[l].level, kLayers[l].cout)) !=
@@ -1272,7 +1306,7 @@ int ensure_ws(vc_ctx* ctx, vc_surrogate* s, int X, int Y, int Z, bool bw)
             if ((rc = A(&s->gact[l], kLayers[l].level, kLayers[l].cout)) !=
                 VC_OK)
                 return rc;
-        if ((rc = A(&s->gact[0], 0, 16)) != VC_OK ||
+        if ((rc = A(&s->gact[0], 0, 16)) != VC_OK || (rc = A(&s->x8, 0, 8)) != VC_OK ||
             (rc = A(&s->gp1, 1, 16)) != VC_OK || (rc = A(&s->gp2, 2, 32)) != VC_OK ||
             (rc = A(&s->gp3, 3, 48)) != VC_OK || (rc = A(&s->gcat2, 2, 112)) != VC_OK ||
             (rc = A(&s->gcat1, 1, 80)) != VC_OK || (rc = A(&s->gcat0, 0, 48)) != VC_OK)
@@ -1677,7 +1711,7 @@ int backward(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
     if ((rc = layer_back(ctx, s, 1, s->gact[1], s->act[0], s->gact[0])) !=
         VC_OK)
         return rc;
-    return layer_back(ctx, s, 0, s->gact[0], s->x0, nullptr);
+    return layer_back(ctx, s, 0, s->gact[0], s->x8, nullptr);
 }
 
 int round8(int n) { return (n + 7) / 8 * 8; }
@@ -1696,7 +1730,7 @@ int train_volume(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, float* loss_dev,
     s->pad[2] = s->Z;
     hipStream_t S = ctx->stream;
     prep_input<<<blocks_for(level_voxels(s, 0)), kBlock, 0, S>>>(
-        v->occ, v->edt, v->nx, v->ny, v->nz, 0, 0, 0, s->x0, s->X, s->Y, s->Z);
+        v->occ, v->edt, v->nx, v->ny, v->nz, 0, 0, 0, s->x0, s->x8, s->X, s->Y, s->Z);
     VC_CHECK_LAUNCH(ctx);
     if ((rc = forward(ctx, s, true)) != VC_OK)
         return rc;
@@ -1771,7 +1805,7 @@ int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
                     s->pad[a] = round8(dims[a]);
                 prep_input<<<blocks_for(VR), kBlock, 0, S>>>(
                     v->occ, v->edt, v->nx, v->ny, v->nz, ox, oy, oz, s->x0,
-                    s->X, s->Y, s->Z);
+                    nullptr, s->X, s->Y, s->Z);
                 VC_CHECK_LAUNCH(ctx);
                 if ((rc = forward(ctx, s, false)) != VC_OK)
                     return rc;
