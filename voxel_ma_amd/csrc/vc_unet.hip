@@ -272,6 +272,63 @@ inline unsigned conv_grid(int64_t nb)
 // the first branch: no bounds checks and 32-bit offsets from a wave-uniform
 // base (the host passes off32 when a region's extent fits them), with all of
 // a thread's loads issued before its LDS stores.
+// The last layer of inference with the head fused: its activations are not
+// stored; lane group g takes tile g, gathers the 16 BF16-rounded channels of
+// voxel i (the values the stored layer would hold) and evaluates the head as
+// head_fwd does (out = bias, then one FMA per channel in channel order), so
+// the outputs are those of the unfused pair.
+template <int MT, int BY, int NZT>
+__device__ __forceinline__ void conv_store_head(const f32x4 (&acc)[MT][1],
+                                                const float* __restrict__ bias,
+                                                const float* __restrict__ Wh,
+                                                float* __restrict__ hout, int x0, int y0,
+                                                int z0, int X, int Y, int Z, const Box& box)
+{
+    static_assert(MT == 4, "one tile per lane group");
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    float bv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        bv[r] = bias[4 * g + r];
+    float H[16];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+    {
+        const int t = wv * MT + m, row = t / NZT, seg = t % NZT;
+        const int x = x0 + row / BY, y = y0 + row % BY, z = z0 + 16 * seg + i;
+        const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                           (unsigned)(box.oy + y) < (unsigned)box.py &&
+                           (unsigned)(box.oz + z) < (unsigned)box.pz;
+        float h[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            h[r] = inbox ? (float)(bf16)fmaxf(acc[m][0][r] + bv[r], 0.0f) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                const float v = __shfl(h[r], i + 16 * k);
+                if (g == m)
+                    H[4 * k + r] = v;
+            }
+    }
+    const int t = wv * MT + g, row = t / NZT, seg = t % NZT;
+    const int x = x0 + row / BY, y = y0 + row % BY, z = z0 + 16 * seg + i;
+    if (x >= X || y >= Y || z >= Z)
+        return;
+    float o0 = Wh[kHeadIn * kHeadOut], o1 = Wh[kHeadIn * kHeadOut + 1];
+#pragma unroll
+    for (int c = 0; c < kHeadIn; ++c)
+    {
+        o0 = __builtin_fmaf(H[c], Wh[c * 2], o0);
+        o1 = __builtin_fmaf(H[c], Wh[c * 2 + 1], o1);
+    }
+    const int64_t v = ((int64_t)x * Y + y) * Z + z;
+    *reinterpret_cast<float2*>(hout + 2 * v) = make_float2(o0, o1);
+}
+
 template <int NR, int RY, int RZ, int SC, bool kCat>
 __device__ __forceinline__ void conv_stage(bf16x8* __restrict__ reg, const bf16* __restrict__ in,
                                            const bf16* __restrict__ in2, int c1, int cinp,
@@ -351,11 +408,12 @@ __device__ __forceinline__ void conv_stage(bf16x8* __restrict__ reg, const bf16*
     }
 }
 
-template <int NT, int BX, int BY, int NZT, int SC, bool kCat, bool kRelu>
+template <int NT, int BX, int BY, int NZT, int SC, bool kCat, bool kRelu, bool kHead = false>
 __global__ void __launch_bounds__(kBlock)
     conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
                  int cinp, bf16* __restrict__ out, const bf16x8* __restrict__ W,
-                 const float* __restrict__ bias, int X, int Y, int Z, Box box, int off32)
+                 const float* __restrict__ bias, int X, int Y, int Z, Box box, int off32,
+                 const float* __restrict__ Wh = nullptr, float* __restrict__ hout = nullptr)
 {
     constexpr int Cout = NT * 16;
     constexpr int MT = BX * BY * NZT / 4;
@@ -402,7 +460,10 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
         conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
     }
-    conv_store<NT, MT, BY, NZT, kRelu>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
+    if constexpr (kHead)
+        conv_store_head<MT, BY, NZT>(acc, bias, Wh, hout, x0, y0, z0, X, Y, Z, box);
+    else
+        conv_store<NT, MT, BY, NZT, kRelu>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
 }
 
 // Weight gradient partials.  Block (chunk, tg): taps (dx, dy) = (tg/3 - 1,
@@ -1545,7 +1606,26 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
                              s->theta + param_offset(l) + 27LL * L.cin * L.cout, X, Y,
                              Z, s->kc[l], 1, level_box(s, L.level), skips[d], c1s[d]);
         }
-        if (rc != VC_OK || (rc = conv(9 + 2 * d, s->act[8 + 2 * d])) != VC_OK)
+        if (rc != VC_OK)
+            return rc;
+        if (d == 2 && !keep_cat)
+        {
+            // inference: the last layer with the head fused (its activations
+            // are not kept)
+            const LayerSpec& L = kLayers[13];
+            const int64_t nb = (int64_t)((X + kCBX - 1) / kCBX) * ((Y + kCBY - 1) / kCBY) *
+                               ((Z + 16 * kCNZT - 1) / (16 * kCNZT));
+            conv_fwd_lds<1, kCBX, kCBY, kCNZT, kCSC, false, true, true>
+                <<<conv_grid(nb), kBlock, 0, S>>>(
+                    s->act[12], nullptr, 0, cinp_of(L.cin), nullptr,
+                    reinterpret_cast<const bf16x8*>(s->Wf[13]),
+                    s->theta + param_offset(13) + 27LL * L.cin * L.cout, X, Y, Z,
+                    level_box(s, 0), conv_off32(Y, Z, cinp_of(L.cin)),
+                    s->theta + param_offset(14), s->out);
+            VC_CHECK_LAUNCH(ctx);
+            return VC_OK;
+        }
+        if ((rc = conv(9 + 2 * d, s->act[8 + 2 * d])) != VC_OK)
             return rc;
     }
     const int64_t V0 = level_voxels(s, 0);
