@@ -1563,6 +1563,33 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
         e = hipMemsetAsync(acc, 0, 8 * kSlots * kAcc, S);
     if (e == hipSuccess)
         e = hipMemcpyAsync(dtbl, htbl, sizeof(htbl), hipMemcpyHostToDevice, S);
+    // the Euler terms and the thin-layer components (vectorised path) need only
+    // the occupancy: they run on the second stream, beside the counts, the
+    // scans, the host's allocation of the outputs and the emit passes, and
+    // join before the accumulators are read
+    hipStream_t SE = vec ? ctx->stream_aux : S;
+    if (e == hipSuccess && vec)
+        e = hipEventRecord(ctx->ev_fork, S);
+    if (e == hipSuccess && vec)
+        e = hipStreamWaitEvent(SE, ctx->ev_fork, 0);
+    if (e == hipSuccess && vec)
+    {
+        const int64_t nzc = v->nz / kVz;
+        const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
+        const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
+        const unsigned gv = (unsigned)etiles;
+        euler_rows_v<<<gl, kBlock, 0, SE>>>(v->occ, D, dtbl, acc);
+        layer_boundary_v<<<gv, kBlock, 0, SE>>>(v->occ, D, bnd);
+        const int tiles_x = (v->nx + kTX - 1) / kTX, tiles_y = (v->ny + kTY - 1) / kTY,
+                  tiles_z = (int)((nzc + kTZ - 1) / kTZ);
+        const int64_t n_tiles = (int64_t)tiles_x * tiles_y * tiles_z;
+        layer_tile<<<(unsigned)(8 * ((n_tiles + 7) / 8)), kTile, 0, SE>>>(
+            v->occ, bnd, D, tiles_y, tiles_z, n_tiles, lay8, lab, acc);
+        layer_border<<<gv, kBlock, 0, SE>>>(lay8, D, lab, acc);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipEventRecord(ctx->ev_join, SE);
+    }
     if (e == hipSuccess)
     {
         if (vec)
@@ -1580,24 +1607,8 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
         e = hipMemcpyAsync(&tot[0], ts + etiles, 4, hipMemcpyDeviceToHost, S);
     if (e == hipSuccess)
         e = hipMemcpyAsync(&tot[1], tf + etiles, 4, hipMemcpyDeviceToHost, S);
-    // Euler terms
-    if (e == hipSuccess && vec)
-    {
-        const int64_t nzc = v->nz / kVz;
-        const int64_t lrows = (int64_t)(v->nx + 1) * (v->ny + 1) * nzc;
-        const unsigned gl = (unsigned)((lrows + kBlock - 1) / kBlock);
-        const unsigned gv = (unsigned)etiles;
-        euler_rows_v<<<gl, kBlock, 0, S>>>(v->occ, D, dtbl, acc);
-        layer_boundary_v<<<gv, kBlock, 0, S>>>(v->occ, D, bnd);
-        const int tiles_x = (v->nx + kTX - 1) / kTX, tiles_y = (v->ny + kTY - 1) / kTY,
-                  tiles_z = (int)((nzc + kTZ - 1) / kTZ);
-        const int64_t n_tiles = (int64_t)tiles_x * tiles_y * tiles_z;
-        layer_tile<<<(unsigned)(8 * ((n_tiles + 7) / 8)), kTile, 0, S>>>(
-            v->occ, bnd, D, tiles_y, tiles_z, n_tiles, lay8, lab, acc);
-        layer_border<<<gv, kBlock, 0, S>>>(lay8, D, lab, acc);
-        e = hipGetLastError();
-    }
-    else if (e == hipSuccess)
+    // Euler terms (the per-voxel path: on this stream)
+    if (e == hipSuccess && !vec)
     {
         const dim3 gl((unsigned)((v->nz + 1 + kBlock - 1) / kBlock),
                       (unsigned)(v->ny + 1), (unsigned)(v->nx + 1));
@@ -1613,19 +1624,17 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
         e = hipGetLastError();
     }
     unsigned long long hacc[kSlots * kAcc];
-    if (e == hipSuccess)
+    if (e == hipSuccess && !vec)
         e = hipMemcpyAsync(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost, S);
     if (e == hipSuccess)
         e = hipStreamSynchronize(S);
     if (e != hipSuccess)
+    {
+        if (vec)
+            (void)hipStreamSynchronize(SE);
         return vc_fail(ctx, VC_ERR_HIP, "boundary count/Euler: %s", hipGetErrorString(e));
+    }
     const int64_t ns = tot[0], nf = tot[1];
-    long long sum[kAcc] = {0};
-    for (int sl = 0; sl < kSlots; ++sl)
-        for (int q = 0; q < kAcc; ++q)
-            sum[q] += (long long)hacc[sl * kAcc + q];
-    if (sum[7] != 0)
-        return vc_fail(ctx, VC_ERR_GUARD, "boundary: union-find retry bound reached");
 
     vc_free(v->bnd_sites);
     vc_free(v->bnd_quads);
@@ -1633,7 +1642,11 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
     v->bnd_quads = nullptr;
     if ((rc = vc_alloc(ctx, (void**)&v->bnd_sites, 12 * (size_t)ns)) != VC_OK ||
         (rc = vc_alloc(ctx, (void**)&v->bnd_quads, 16 * (size_t)nf)) != VC_OK)
+    {
+        if (vec)
+            (void)hipStreamSynchronize(SE); // the workspace's last user is done
         return rc;
+    }
     if (vec)
     {
         bnd_emit_sites_v<<<(unsigned)etiles, kBlock, 0, S>>>(D, ccnt, vmask, ts,
@@ -1649,8 +1662,25 @@ int boundary_compute(vc_ctx* ctx, vc_vol* v, bool vec, int64_t etiles)
                                                            v->bnd_quads);
     }
     e = hipGetLastError();
+    // join the Euler chain, then read the accumulators
+    if (e == hipSuccess && vec)
+        e = hipStreamWaitEvent(S, ctx->ev_join, 0);
+    if (e == hipSuccess && vec)
+        e = hipMemcpyAsync(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost, S);
+    if (e == hipSuccess && vec)
+        e = hipStreamSynchronize(S);
     if (e != hipSuccess)
+    {
+        if (vec)
+            (void)hipStreamSynchronize(SE);
         return vc_fail(ctx, VC_ERR_HIP, "boundary emit: %s", hipGetErrorString(e));
+    }
+    long long sum[kAcc] = {0};
+    for (int sl = 0; sl < kSlots; ++sl)
+        for (int q = 0; q < kAcc; ++q)
+            sum[q] += (long long)hacc[sl * kAcc + q];
+    if (sum[7] != 0)
+        return vc_fail(ctx, VC_ERR_GUARD, "boundary: union-find retry bound reached");
 
     vc_euler& E = v->bnd_euler;
     std::memset(&E, 0, sizeof(E));

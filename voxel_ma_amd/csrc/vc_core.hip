@@ -136,6 +136,11 @@ int vc_ctx_create(int device, vc_ctx** out)
                                                 hipStreamNonBlocking) !=
                            hipSuccess)
         rc = VC_ERR_HIP;
+    if (rc == VC_OK &&
+        (hipStreamCreateWithFlags(&ctx->stream_aux, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess))
+        rc = VC_ERR_HIP;
     if (rc == VC_OK)
         rc = vc_lut_build(ctx);
     if (rc == VC_OK)
@@ -169,6 +174,15 @@ int vc_ctx_destroy(vc_ctx* ctx)
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
     }
+    if (ctx->stream_aux)
+    {
+        (void)hipStreamSynchronize(ctx->stream_aux);
+        (void)hipStreamDestroy(ctx->stream_aux);
+    }
+    if (ctx->ev_fork)
+        (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join)
+        (void)hipEventDestroy(ctx->ev_join);
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -38,6 +38,10 @@ struct vc_ctx
     int device = 0;
     int cu_count = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for independent kernel chains within
+    // one call, forked from and joined back to `stream` by the two events
+    hipStream_t stream_aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t* d_lut = nullptr; // simple-point table, 2^21 words
     std::string err;
     // autotuned schedules by batch shape (volumes, voxels, largest axis)
