@@ -15,7 +15,10 @@
  *
  * Every compute entry point is enqueued on the context's HIP stream and
  * returns without waiting, except the vc_get_* / *_host variants, which wait
- * for their own results.  One host thread per context.
+ * for their own results.  One host thread per context.  (The boundary pass
+ * also runs kernels on a second stream of the context, forked from and
+ * joined back to the context stream by events within the call, so its
+ * results are ordered on the context stream like every other.)
  *
  * Reference entry points replaced (reference file:line):
  *   vc_volume_upload      DenseVolume / Volume3DScalar::getDataAt
@@ -71,8 +74,9 @@ typedef struct vc_ctx vc_ctx;
 typedef struct vc_vol vc_vol;
 
 /* ---- context ----------------------------------------------------------- */
-/* Binds a HIP device, creates the context stream and builds the (26,6)
- * simple-point table (2^26 bits, 8 MiB) in HBM. */
+/* Binds a HIP device, creates the context stream (and a second, internal
+ * one, both of default priority) and builds the (26,6) simple-point table
+ * (2^26 bits, 8 MiB) in HBM. */
 int vc_ctx_create(int device, vc_ctx** out);
 int vc_ctx_destroy(vc_ctx* ctx);
 /* copies the last error message (NUL-terminated, truncated to n) */
