@@ -83,7 +83,11 @@ int vc_surrogate_measure(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
                          vc_surrogate_stats* stats);
 /* Copy of an activation of the last forward pass as float (which = layer
  * index 0..13 for the ReLU outputs of L0a..L6b, 14 for the two head
- * outputs), channels last, over the padded volume. */
+ * outputs), channels last, over the padded volume.  After an inference
+ * (vc_surrogate_infer / vc_surrogate_measure) layer 13 is not available
+ * (VC_ERR_INVALID): inference evaluates the head inside that layer's
+ * convolution and does not store its activations; after a training pass
+ * (vc_surrogate_gradients / train) it is. */
 int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
                             float* host, int64_t n);
 

@@ -172,18 +172,26 @@ def test_training_lowers_loss(vc):
 
 def test_inference_equals_training_forward(vc, small):
     """Inference reads the decoder's concatenated inputs straight from the
-    coarse level and the skip tensor; the training forward materializes the
-    concatenation first.  Both give the same network output, bit for bit."""
+    coarse level and the skip tensor, and evaluates the head inside the last
+    convolution (that layer's activations are not stored); the training
+    forward materializes the concatenation and the last layer's activations
+    and runs the head as its own kernel.  Both give the same network output,
+    bit for bit."""
     occ, edt, lam2, vol = small
     net = V.Surrogate(vc, seed=7)
     net.set_params(_theta(7))
     net.gradients(vol)
     shape = tuple(-(-d // 8) * 8 for d in occ.shape)
     out_train = net.activation(14, shape + (2,))
-    last_train = net.activation(13, shape + (16,))
+    assert np.abs(out_train).max() > 0
+    net.set_params(_theta(8))  # a different network in between: the output
+    net.infer(vol)             # buffer must be rewritten by the next call
+    assert not np.array_equal(net.activation(14, shape + (2,)), out_train)
+    net.set_params(_theta(7))
     net.infer(vol)
-    assert np.array_equal(net.activation(13, shape + (16,)), last_train)
     assert np.array_equal(net.activation(14, shape + (2,)), out_train)
+    with pytest.raises(V.VcError):  # not stored by an inference
+        net.activation(13, shape + (16,))
     net.release()
 
 

@@ -1259,6 +1259,9 @@ struct vc_surrogate
     bf16* act[14] = {}; // ReLU outputs of L0a..L6b
     bf16 *x0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
     bf16* x8 = nullptr; // the input padded to 8 channels (training)
+    // whether act[13] holds the last forward pass's values (an inference
+    // evaluates the head inside that layer's convolution and stores nothing)
+    bool act13_valid = false;
     bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
     float* out = nullptr;
     // backward workspace (allocated on first use)
@@ -1540,6 +1543,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
     hipStream_t S = ctx->stream;
     const bf16* in = s->x0;
     int rc;
+    s->act13_valid = keep_cat;
     auto conv = [&](int l, const bf16* x) {
         const LayerSpec& L = kLayers[l];
         if (l == 0)
@@ -2260,6 +2264,10 @@ int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
 {
     if (!ctx || !s || !host || which < 0 || which > 14 || s->X == 0)
         return VC_ERR_INVALID;
+    if (which == 13 && !s->act13_valid)
+        return vc_fail(ctx, VC_ERR_INVALID,
+                       "activation 13 is not stored by an inference (the head is "
+                       "evaluated inside its convolution)");
     const int64_t V =
         which == 14 ? level_voxels(s, 0) : level_voxels(s, kLayers[which].level);
     const int C = which == 14 ? 2 : kLayers[which].cout;
