@@ -46,6 +46,9 @@ constexpr int kHalo = VC_SURROGATE_HALO;
 #ifndef VC_CONV_SC
 #define VC_CONV_SC 2
 #endif
+#ifndef VC_CONV_COARSE
+#define VC_CONV_COARSE 1
+#endif
 #ifndef VC_CONV_GX
 #define VC_CONV_GX 4
 #endif
@@ -199,6 +202,102 @@ __device__ __forceinline__ void conv_slice(const bf16x8* __restrict__ reg,
                     acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[m], acc[m][n],
                                                                         0, 0, 0);
             }
+        }
+    }
+}
+
+// The decoder concatenation's coarse-level chunks (channels below c1: the
+// nearest up-sampling of the level below) are staged at the coarse level: the
+// fine region voxel (rx, ry, rz) (fine coordinate x0 - 1 + rx, ...) is the
+// coarse voxel ((x0 - 1 + rx) >> 1, ...), i.e. coarse region voxel ((rx + 1) >> 1,
+// (ry + 1) >> 1, (rz + 1) >> 1) for an even brick origin, so a 4 x 4 x 16 brick
+// stages 4 x 4 x 10 coarse voxels per chunk instead of 6 x 6 x 18 fine ones
+// (the same values: each coarse voxel once instead of up to 8 times).
+template <int BX, int BY, int BZ>
+struct CoarseRegion
+{
+    static constexpr int RX = BX / 2 + 2, RY = BY / 2 + 2, RZ = BZ / 2 + 2,
+                         NR = RX * RY * RZ;
+};
+
+template <int BX, int BY, int BZ, int SC>
+__device__ __forceinline__ void conv_stage_coarse(bf16x8* __restrict__ reg,
+                                                  const bf16* __restrict__ in, int c1, int s0,
+                                                  int x0, int y0, int z0, int X, int Y, int Z)
+{
+    using CR = CoarseRegion<BX, BY, BZ>;
+    constexpr int NE = SC * CR::NR, NIT = (NE + kBlock - 1) / kBlock, RPI = kBlock / SC;
+    static_assert(kBlock % SC == 0, "chunk per thread");
+    const int c = threadIdx.x % SC, r0 = threadIdx.x / SC;
+    const int Xc = X >> 1, Yc = Y >> 1, Zc = Z >> 1;
+    const int cx0 = (x0 - 1) >> 1, cy0 = (y0 - 1) >> 1, cz0 = (z0 - 1) >> 1;
+    bf16x8 v[NIT];
+    uint32_t okm = 0;
+#pragma unroll
+    for (int k = 0; k < NIT; ++k)
+    {
+        const int r = min(r0 + k * RPI, CR::NR - 1);
+        const int cz = cz0 + r % CR::RZ, cy = cy0 + (r / CR::RZ) % CR::RY,
+                  cx = cx0 + r / (CR::RZ * CR::RY);
+        const bool ok = (unsigned)cx < (unsigned)Xc && (unsigned)cy < (unsigned)Yc &&
+                        (unsigned)cz < (unsigned)Zc;
+        okm |= ok ? 1u << k : 0u;
+        const int64_t o = ok ? (((int64_t)cx * Yc + cy) * Zc + cz) * c1 + (s0 + c) * 8 : 0;
+        v[k] = *reinterpret_cast<const bf16x8*>(in + o);
+    }
+    const bf16x8 zero8 = {};
+#pragma unroll
+    for (int k = 0; k < NIT; ++k)
+    {
+        const int r = r0 + k * RPI;
+        if (r < CR::NR)
+            reg[c * CR::NR + r] = (okm >> k) & 1u ? v[k] : zero8;
+    }
+}
+
+// The MFMAs of such a slice (two chunks; the pairing of conv_slice's two-chunk
+// path, the voxel operands read through the coarse addressing).  Wavefront w
+// holds brick x row w, tile m brick y row m.
+template <int NT, int MT, int BX, int BY, int BZ>
+__device__ __forceinline__ void conv_slice_coarse(const bf16x8* __restrict__ reg,
+                                                  const bf16x8* __restrict__ W, int s0,
+                                                  int cin8, f32x4 (&acc)[MT][NT])
+{
+    using CR = CoarseRegion<BX, BY, BZ>;
+    constexpr int Cout = NT * 16, NP = 27 * 2, NG = (NP + 3) / 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4, h = g >> 1, c = g & 1;
+    const bf16x8 zero8 = {};
+    // coarse z of fine region z 1 + i + dz, dz = -1, 0, 1
+    const int zl[3] = {(i + 1) >> 1, (i + 2) >> 1, (i + 3) >> 1};
+    const int lc = c * CR::NR;
+    const int wl = (h * cin8 + c) * Cout + i;
+#pragma unroll
+    for (int j = 0; j < NG; ++j)
+    {
+        const int t0 = 2 * j, t1 = 2 * j + 1 < 27 ? 2 * j + 1 : 2 * j;
+        const bool v = 4 * j + g < NP; // false only for tap 27 of the last MFMA
+        const int dx0 = t0 / 9 - 1, dy0 = (t0 / 3) % 3 - 1, dz0 = t0 % 3 - 1;
+        const int dx1 = t1 / 9 - 1, dy1 = (t1 / 3) % 3 - 1, dz1 = t1 % 3 - 1;
+        const int x0r = ((wv + 2 + dx0) >> 1) * (CR::RY * CR::RZ) + zl[dz0 + 1];
+        const int x1r = ((wv + 2 + dx1) >> 1) * (CR::RY * CR::RZ) + zl[dz1 + 1];
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+        {
+            const int a0 = x0r + ((m + 2 + dy0) >> 1) * CR::RZ;
+            const int a1 = x1r + ((m + 2 + dy1) >> 1) * CR::RZ;
+            a[m] = v ? reg[lc + (h ? a1 : a0)] : zero8;
+        }
+        const bf16x8* Wj = W + (t0 * cin8 + s0) * Cout;
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+        {
+            const bf16x8 b = v ? Wj[wl + n * 16] : zero8;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[m], acc[m][n], 0, 0,
+                                                                    0);
         }
     }
 }
@@ -452,7 +551,18 @@ __global__ void __launch_bounds__(kBlock)
     // the chunk count is a multiple of SC (checked by the host)
     const bool inner = off32 && x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 + BX < X &&
                        y0 + BY < Y && z0 + BZ < Z;
-    for (int s0 = 0; s0 < cin8; s0 += SC)
+    constexpr bool kCoarse = kCat && SC == 2 && NZT == 1 && MT == BY && BX % 2 == 0 &&
+                             BY % 2 == 0 && VC_CONV_COARSE;
+    int s0 = 0;
+    if constexpr (kCoarse)
+        for (; s0 + SC <= (c1 >> 3); s0 += SC)
+        {
+            __syncthreads(); // the previous slice's reads are done
+            conv_stage_coarse<BX, BY, BZ, SC>(reg, in, c1, s0, x0, y0, z0, X, Y, Z);
+            __syncthreads();
+            conv_slice_coarse<NT, MT, BX, BY, BZ>(reg, W, s0, cin8, acc);
+        }
+    for (; s0 < cin8; s0 += SC)
     {
         __syncthreads(); // the previous slice's reads are done
         conv_stage<NR, RY, RZ, SC, kCat>(reg, in, in2, c1, cinp, s0, x0, y0, z0, X, Y, Z,
