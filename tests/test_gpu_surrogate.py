@@ -171,12 +171,12 @@ def test_training_lowers_loss(vc):
 
 
 def test_inference_equals_training_forward(vc, small):
-    """Inference reads the decoder's concatenated inputs straight from the
-    coarse level and the skip tensor, and evaluates the head inside the last
-    convolution (that layer's activations are not stored); the training
-    forward materializes the concatenation and the last layer's activations
-    and runs the head as its own kernel.  Both give the same network output,
-    bit for bit."""
+    """Inference runs layers 0 and 1 in one kernel, reads the decoder's
+    concatenated inputs straight from the coarse level and the skip tensor,
+    and evaluates the head inside the last convolution (layers 0 and 13 are
+    not stored); the training forward stores every layer, materializes the
+    concatenation and runs the head as its own kernel.  Both give the same
+    network output, bit for bit."""
     occ, edt, lam2, vol = small
     net = V.Surrogate(vc, seed=7)
     net.set_params(_theta(7))
@@ -190,8 +190,9 @@ def test_inference_equals_training_forward(vc, small):
     net.set_params(_theta(7))
     net.infer(vol)
     assert np.array_equal(net.activation(14, shape + (2,)), out_train)
-    with pytest.raises(V.VcError):  # not stored by an inference
-        net.activation(13, shape + (16,))
+    for which in (0, 13):  # not stored by an inference
+        with pytest.raises(V.VcError):
+            net.activation(which, shape + (16,))
     net.release()
 
 

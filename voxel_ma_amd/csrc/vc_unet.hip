@@ -49,6 +49,9 @@ constexpr int kHalo = VC_SURROGATE_HALO;
 #ifndef VC_CONV_COARSE
 #define VC_CONV_COARSE 1
 #endif
+#ifndef VC_CONV_FUSE01
+#define VC_CONV_FUSE01 1
+#endif
 #ifndef VC_CONV_GX
 #define VC_CONV_GX 4
 #endif
@@ -1176,6 +1179,126 @@ __global__ void __launch_bounds__(kBlock)
     conv_store<1, MT, BY, NZT, true>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
 }
 
+// Inference: the first two layers in one kernel.  Layer 1's input region
+// (6 x 6 x 18 voxels around a 4 x 4 x 16 brick) is layer 0's output there,
+// computed into LDS from the 2-channel input over an 8 x 8 x 20 region instead
+// of being stored by one kernel and staged back by the next.  Layer 0's values
+// are those of conv_first_lds bit for bit (the same K pairs, MFMA order, bias,
+// ReLU, BF16 rounding and zeroing outside the padded box; region voxels
+// outside the volume are 0 as layer 1's staging reads them), so layer 1's
+// output equals the unfused pair's.  Layer 0's 648 region voxels form 41
+// tiles of 16 (the last one half used), spread over the 4 wavefronts.
+template <int BX, int BY>
+__global__ void __launch_bounds__(kBlock)
+    conv01_lds(const bf16* __restrict__ in, bf16* __restrict__ out,
+               const bf16x8* __restrict__ W4, const float* __restrict__ bias0,
+               const bf16x8* __restrict__ W1, const float* __restrict__ bias1, int X, int Y,
+               int Z, Box box, int off32)
+{
+    constexpr int NZT = 1, MT = BX * BY / 4, SC = 2;
+    static_assert(MT == BY, "a wavefront holds one brick x row");
+    constexpr int BZ = 16, RY = BY + 2, RZ = BZ + 2, NR = (BX + 2) * RY * RZ;
+    constexpr int IY = BY + 4, IZ = BZ + 4, NI = (BX + 4) * IY * IZ; // layer 0's input
+    constexpr int NT0 = (NR + 15) / 16;                               // layer 0 tiles
+    __shared__ uint32_t in2[NI];
+    __shared__ bf16x8 reg[SC * NR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int nbz = (Z + BZ - 1) / BZ, nby = (Y + BY - 1) / BY;
+    const int nbx = (X + BX - 1) / BX;
+    int blk = conv_brick(nbz * nby * nbx);
+    if (blk < 0)
+        return;
+    const int bz = blk % nbz;
+    blk /= nbz;
+    const int q = blk / (nby * VC_CONV_GX), rem = blk % (nby * VC_CONV_GX);
+    const int gsz = min(VC_CONV_GX, nbx - q * VC_CONV_GX);
+    const int by = rem / gsz, bx = q * VC_CONV_GX + rem % gsz;
+    const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
+    // layer 0's input region: voxels x0 - 2 .. x0 + BX + 1, ...
+    const uint32_t* in32 = reinterpret_cast<const uint32_t*>(in);
+    for (int e = threadIdx.x; e < NI; e += kBlock)
+    {
+        const int iz = e % IZ, ixy = e / IZ;
+        const int xx = x0 - 2 + ixy / IY, yy = y0 - 2 + ixy % IY, zz = z0 - 2 + iz;
+        uint32_t v = 0u;
+        if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
+            (unsigned)zz < (unsigned)Z)
+            v = in32[((int64_t)xx * Y + yy) * Z + zz];
+        in2[e] = v;
+    }
+    __syncthreads();
+    // layer 0 over layer 1's region (voxel r = (rx RY + ry) RZ + rz)
+    {
+        float bv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            bv[r] = bias0[4 * g + r];
+        bf16x8 wb[2];
+        int to[2][4];
+        bool tv[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+            const int p = 4 * h + g; // taps 4p .. 4p + 3 (pair 7 is all zero)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int tap = 4 * p + j;
+                tv[h][j] = tap < 27;
+                const int tt = tv[h][j] ? tap : 13;
+                to[h][j] = ((tt / 9 - 1) * IY + (tt / 3) % 3 - 1) * IZ + tt % 3 - 1;
+            }
+            wb[h] = W4[p * 16 + i];
+        }
+        for (int t = wv; t < NT0; t += kBlock / 64)
+        {
+            const int r = min(16 * t + i, NR - 1);
+            const int rz = r % RZ, rxy = r / RZ, rx = rxy / RY, ry = rxy % RY;
+            // the voxel's centre in layer 0's input region
+            const int ic = ((rx + 1) * IY + (ry + 1)) * IZ + rz + 1;
+            f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+            {
+                const u32x4 w = {tv[h][0] ? in2[ic + to[h][0]] : 0u,
+                                 tv[h][1] ? in2[ic + to[h][1]] : 0u,
+                                 tv[h][2] ? in2[ic + to[h][2]] : 0u,
+                                 tv[h][3] ? in2[ic + to[h][3]] : 0u};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb[h], __builtin_bit_cast(bf16x8, w),
+                                                              acc, 0, 0, 0);
+            }
+            // lane (i, g): channels 4 g .. 4 g + 3 of voxel 16 t + i
+            const int x = x0 - 1 + rx, y = y0 - 1 + ry, z = z0 - 1 + rz;
+            const bool invol = (unsigned)x < (unsigned)X && (unsigned)y < (unsigned)Y &&
+                               (unsigned)z < (unsigned)Z;
+            const bool inbox = (unsigned)(box.ox + x) < (unsigned)box.px &&
+                               (unsigned)(box.oy + y) < (unsigned)box.py &&
+                               (unsigned)(box.oz + z) < (unsigned)box.pz;
+            f32x4 val = acc + f32x4{bv[0], bv[1], bv[2], bv[3]};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                val[k] = fmaxf(val[k], 0.0f);
+            u32x2 qv = __builtin_bit_cast(u32x2, __builtin_convertvector(val, bf16x4));
+            qv = invol && inbox ? qv : u32x2{0u, 0u};
+            if (16 * t + i < NR)
+                reinterpret_cast<u32x2*>(reg + (g >> 1) * NR + r)[g & 1] = qv;
+        }
+    }
+    __syncthreads();
+    // layer 1 (16 -> 16) over the staged region
+    int tb[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+        tb[m] = ((wv + 1) * RY + (m + 1)) * RZ + 1;
+    f32x4 acc[MT][1];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+        acc[m][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    conv_slice<1, MT, NR, RY, RZ, SC>(reg, tb, W1, 0, 2, acc);
+    conv_store<1, MT, BY, NZT, true>(acc, out, bias1, x0, y0, z0, X, Y, Z, box, off32);
+}
+
 // first layer's W[27][2][16] -> W4[8][16][8] (pair p slot e: tap 4p + e/2,
 // channel e % 2)
 __global__ void __launch_bounds__(kBlock)
@@ -1369,9 +1492,10 @@ struct vc_surrogate
     bf16* act[14] = {}; // ReLU outputs of L0a..L6b
     bf16 *x0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
     bf16* x8 = nullptr; // the input padded to 8 channels (training)
-    // whether act[13] holds the last forward pass's values (an inference
-    // evaluates the head inside that layer's convolution and stores nothing)
-    bool act13_valid = false;
+    // whether act[0] and act[13] hold the last forward pass's values (an
+    // inference computes layers 0 and 1 in one kernel and evaluates the head
+    // inside layer 13's convolution, storing neither)
+    bool full_acts = false;
     bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
     float* out = nullptr;
     // backward workspace (allocated on first use)
@@ -1653,7 +1777,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
     hipStream_t S = ctx->stream;
     const bf16* in = s->x0;
     int rc;
-    s->act13_valid = keep_cat;
+    s->full_acts = keep_cat;
     auto conv = [&](int l, const bf16* x) {
         const LayerSpec& L = kLayers[l];
         if (l == 0)
@@ -1679,8 +1803,25 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
         Z = s->Z >> lv;
     };
     int X, Y, Z;
-    if ((rc = conv(0, in)) != VC_OK || (rc = conv(1, s->act[0])) != VC_OK)
-        return rc;
+    if (keep_cat || !VC_CONV_FUSE01)
+    {
+        if ((rc = conv(0, in)) != VC_OK || (rc = conv(1, s->act[0])) != VC_OK)
+            return rc;
+    }
+    else
+    {
+        // inference: layers 0 and 1 in one kernel (layer 0's activations are
+        // not stored)
+        const int64_t nb = (int64_t)((s->X + kCBX - 1) / kCBX) * ((s->Y + kCBY - 1) / kCBY) *
+                           ((s->Z + 15) / 16);
+        conv01_lds<kCBX, kCBY><<<conv_grid(nb), kBlock, 0, S>>>(
+            in, s->act[1], reinterpret_cast<const bf16x8*>(s->W4),
+            s->theta + param_offset(0) + 27LL * kLayers[0].cin * kLayers[0].cout,
+            reinterpret_cast<const bf16x8*>(s->Wf[1]),
+            s->theta + param_offset(1) + 27LL * kLayers[1].cin * kLayers[1].cout, s->X, s->Y,
+            s->Z, level_box(s, 0), conv_off32(s->Y, s->Z, 16));
+        VC_CHECK_LAUNCH(ctx);
+    }
     // encoder
     bf16* pools[3] = {s->p1, s->p2, s->p3};
     const int pc[3] = {16, 32, 48};
@@ -2374,10 +2515,11 @@ int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
 {
     if (!ctx || !s || !host || which < 0 || which > 14 || s->X == 0)
         return VC_ERR_INVALID;
-    if (which == 13 && !s->act13_valid)
+    if ((which == 0 || which == 13) && !s->full_acts)
         return vc_fail(ctx, VC_ERR_INVALID,
-                       "activation 13 is not stored by an inference (the head is "
-                       "evaluated inside its convolution)");
+                       "activation %d is not stored by an inference (layers 0 and 1 run "
+                       "in one kernel; the head is evaluated inside layer 13's "
+                       "convolution)", which);
     const int64_t V =
         which == 14 ? level_voxels(s, 0) : level_voxels(s, kLayers[which].level);
     const int C = which == 14 ? 2 : kLayers[which].cout;
