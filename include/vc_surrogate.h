@@ -84,10 +84,10 @@ int vc_surrogate_measure(vc_ctx* ctx, vc_surrogate* s, vc_vol* vol,
 /* Copy of an activation of the last forward pass as float (which = layer
  * index 0..13 for the ReLU outputs of L0a..L6b, 14 for the two head
  * outputs), channels last, over the padded volume.  After an inference
- * (vc_surrogate_infer / vc_surrogate_measure) layer 13 is not available
- * (VC_ERR_INVALID): inference evaluates the head inside that layer's
- * convolution and does not store its activations; after a training pass
- * (vc_surrogate_gradients / train) it is. */
+ * (vc_surrogate_infer / vc_surrogate_measure) layers 0 and 13 are not
+ * available (VC_ERR_INVALID): inference computes layers 0 and 1 in one kernel
+ * and evaluates the head inside layer 13's convolution, storing neither;
+ * after a training pass (vc_surrogate_gradients / train) every layer is. */
 int vc_surrogate_activation(vc_ctx* ctx, vc_surrogate* s, int which,
                             float* host, int64_t n);
 
