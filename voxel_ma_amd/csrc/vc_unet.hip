@@ -1188,12 +1188,14 @@ __global__ void __launch_bounds__(kBlock)
 // outside the volume are 0 as layer 1's staging reads them), so layer 1's
 // output equals the unfused pair's.  Layer 0's 648 region voxels form 41
 // tiles of 16 (the last one half used), spread over the 4 wavefronts.
+// The network input (prep_input's two channels) is computed from the volume's
+// occupancy and EDT as the region is staged, so it is never stored either.
 template <int BX, int BY>
 __global__ void __launch_bounds__(kBlock)
-    conv01_lds(const bf16* __restrict__ in, bf16* __restrict__ out,
-               const bf16x8* __restrict__ W4, const float* __restrict__ bias0,
-               const bf16x8* __restrict__ W1, const float* __restrict__ bias1, int X, int Y,
-               int Z, Box box, int off32)
+    conv01_lds(const uint8_t* __restrict__ occ, const int32_t* __restrict__ edt, int nx,
+               int ny, int nz, bf16* __restrict__ out, const bf16x8* __restrict__ W4,
+               const float* __restrict__ bias0, const bf16x8* __restrict__ W1,
+               const float* __restrict__ bias1, int X, int Y, int Z, Box box, int off32)
 {
     constexpr int NZT = 1, MT = BX * BY / 4, SC = 2;
     static_assert(MT == BY, "a wavefront holds one brick x row");
@@ -1215,16 +1217,26 @@ __global__ void __launch_bounds__(kBlock)
     const int gsz = min(VC_CONV_GX, nbx - q * VC_CONV_GX);
     const int by = rem / gsz, bx = q * VC_CONV_GX + rem % gsz;
     const int x0 = BX * bx, y0 = BY * by, z0 = bz * BZ;
-    // layer 0's input region: voxels x0 - 2 .. x0 + BX + 1, ...
-    const uint32_t* in32 = reinterpret_cast<const uint32_t*>(in);
+    // layer 0's input region: voxels x0 - 2 .. x0 + BX + 1, ... of the padded
+    // region, whose voxel (0, 0, 0) is volume voxel (box.ox, box.oy, box.oz):
+    // [occ, sqrt(max(edt, 0)) / 32] as prep_input computes it, 0 outside
     for (int e = threadIdx.x; e < NI; e += kBlock)
     {
         const int iz = e % IZ, ixy = e / IZ;
         const int xx = x0 - 2 + ixy / IY, yy = y0 - 2 + ixy % IY, zz = z0 - 2 + iz;
+        const int gx = box.ox + xx, gy = box.oy + yy, gz = box.oz + zz;
         uint32_t v = 0u;
         if ((unsigned)xx < (unsigned)X && (unsigned)yy < (unsigned)Y &&
-            (unsigned)zz < (unsigned)Z)
-            v = in32[((int64_t)xx * Y + yy) * Z + zz];
+            (unsigned)zz < (unsigned)Z && (unsigned)gx < (unsigned)nx &&
+            (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz)
+        {
+            const int64_t gi = ((int64_t)gx * ny + gy) * nz + gz;
+            const float o = occ[gi] ? 1.0f : 0.0f;
+            const int32_t ed = edt[gi];
+            const float dd = ed > 0 ? sqrtf((float)ed) / kScale : 0.0f;
+            const bf16x2 w2 = {(bf16)o, (bf16)dd};
+            v = __builtin_bit_cast(uint32_t, w2);
+        }
         in2[e] = v;
     }
     __syncthreads();
@@ -1496,6 +1508,11 @@ struct vc_surrogate
     // inference computes layers 0 and 1 in one kernel and evaluates the head
     // inside layer 13's convolution, storing neither)
     bool full_acts = false;
+    // the volume an inference reads (layers 0 and 1 in one kernel read its
+    // occupancy and EDT directly)
+    const uint8_t* inf_occ = nullptr;
+    const int32_t* inf_edt = nullptr;
+    int inf_n[3] = {0, 0, 0};
     bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
     float* out = nullptr;
     // backward workspace (allocated on first use)
@@ -1814,8 +1831,11 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
         // not stored)
         const int64_t nb = (int64_t)((s->X + kCBX - 1) / kCBX) * ((s->Y + kCBY - 1) / kCBY) *
                            ((s->Z + 15) / 16);
+        if (!s->inf_occ)
+            return vc_fail(ctx, VC_ERR_INVALID, "inference without its volume");
         conv01_lds<kCBX, kCBY><<<conv_grid(nb), kBlock, 0, S>>>(
-            in, s->act[1], reinterpret_cast<const bf16x8*>(s->W4),
+            s->inf_occ, s->inf_edt, s->inf_n[0], s->inf_n[1], s->inf_n[2], s->act[1],
+            reinterpret_cast<const bf16x8*>(s->W4),
             s->theta + param_offset(0) + 27LL * kLayers[0].cin * kLayers[0].cout,
             reinterpret_cast<const bf16x8*>(s->Wf[1]),
             s->theta + param_offset(1) + 27LL * kLayers[1].cin * kLayers[1].cout, s->X, s->Y,
@@ -2138,11 +2158,26 @@ int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
                 s->org[2] = oz;
                 for (int a = 0; a < 3; ++a)
                     s->pad[a] = round8(dims[a]);
-                prep_input<<<blocks_for(VR), kBlock, 0, S>>>(
-                    v->occ, v->edt, v->nx, v->ny, v->nz, ox, oy, oz, s->x0,
-                    nullptr, s->X, s->Y, s->Z);
-                VC_CHECK_LAUNCH(ctx);
-                if ((rc = forward(ctx, s, false)) != VC_OK)
+                if (VC_CONV_FUSE01)
+                {
+                    // layers 0 and 1 read the volume directly
+                    s->inf_occ = v->occ;
+                    s->inf_edt = v->edt;
+                    s->inf_n[0] = v->nx;
+                    s->inf_n[1] = v->ny;
+                    s->inf_n[2] = v->nz;
+                }
+                else
+                {
+                    prep_input<<<blocks_for(VR), kBlock, 0, S>>>(
+                        v->occ, v->edt, v->nx, v->ny, v->nz, ox, oy, oz, s->x0,
+                        nullptr, s->X, s->Y, s->Z);
+                    VC_CHECK_LAUNCH(ctx);
+                }
+                rc = forward(ctx, s, false);
+                s->inf_occ = nullptr;
+                s->inf_edt = nullptr;
+                if (rc != VC_OK)
                     return rc;
                 copy_core<<<blocks_for(VR), kBlock, 0, S>>>(
                     s->out, s->X, s->Y, s->Z, ox, oy, oz, tx, ty, tz,
