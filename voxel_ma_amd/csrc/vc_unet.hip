@@ -52,6 +52,9 @@ constexpr int kHalo = VC_SURROGATE_HALO;
 #ifndef VC_CONV_FUSE01
 #define VC_CONV_FUSE01 1
 #endif
+#ifndef VC_CONV_FUSED_HEAD_CORE
+#define VC_CONV_FUSED_HEAD_CORE 1
+#endif
 #ifndef VC_CONV_GX
 #define VC_CONV_GX 4
 #endif
@@ -379,12 +382,22 @@ inline unsigned conv_grid(int64_t nb)
 // voxel i (the values the stored layer would hold) and evaluates the head as
 // head_fwd does (out = bias, then one FMA per channel in channel order), so
 // the outputs are those of the unfused pair.
+// With `core` (an inference's lam / rel arrays), the outputs of the region's
+// core voxels that lie in the volume are also written there as copy_core
+// writes them (lam = out0 * 32, rel = exp(out1)).
+struct CoreOut
+{
+    float* lam;
+    float* rel;
+    int nx, ny, nz, cx0, cy0, cz0, cx1, cy1, cz1;
+};
 template <int MT, int BY, int NZT>
 __device__ __forceinline__ void conv_store_head(const f32x4 (&acc)[MT][1],
                                                 const float* __restrict__ bias,
                                                 const float* __restrict__ Wh,
                                                 float* __restrict__ hout, int x0, int y0,
-                                                int z0, int X, int Y, int Z, const Box& box)
+                                                int z0, int X, int Y, int Z, const Box& box,
+                                                const CoreOut& core)
 {
     static_assert(MT == 4, "one tile per lane group");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -429,6 +442,14 @@ __device__ __forceinline__ void conv_store_head(const f32x4 (&acc)[MT][1],
     }
     const int64_t v = ((int64_t)x * Y + y) * Z + z;
     *reinterpret_cast<float2*>(hout + 2 * v) = make_float2(o0, o1);
+    const int gx = box.ox + x, gy = box.oy + y, gz = box.oz + z;
+    if (core.lam && gx >= core.cx0 && gx < core.cx1 && gy >= core.cy0 && gy < core.cy1 &&
+        gz >= core.cz0 && gz < core.cz1 && gx < core.nx && gy < core.ny && gz < core.nz)
+    {
+        const int64_t i = ((int64_t)gx * core.ny + gy) * core.nz + gz;
+        core.lam[i] = o0 * kScale;
+        core.rel[i] = expf(o1);
+    }
 }
 
 template <int NR, int RY, int RZ, int SC, bool kCat>
@@ -515,7 +536,8 @@ __global__ void __launch_bounds__(kBlock)
     conv_fwd_lds(const bf16* __restrict__ in, const bf16* __restrict__ in2, int c1,
                  int cinp, bf16* __restrict__ out, const bf16x8* __restrict__ W,
                  const float* __restrict__ bias, int X, int Y, int Z, Box box, int off32,
-                 const float* __restrict__ Wh = nullptr, float* __restrict__ hout = nullptr)
+                 const float* __restrict__ Wh = nullptr, float* __restrict__ hout = nullptr,
+                 CoreOut core = CoreOut{})
 {
     constexpr int Cout = NT * 16;
     constexpr int MT = BX * BY * NZT / 4;
@@ -574,7 +596,7 @@ __global__ void __launch_bounds__(kBlock)
         conv_slice<NT, MT, NR, RY, RZ, SC>(reg, tb, W, s0, cin8, acc);
     }
     if constexpr (kHead)
-        conv_store_head<MT, BY, NZT>(acc, bias, Wh, hout, x0, y0, z0, X, Y, Z, box);
+        conv_store_head<MT, BY, NZT>(acc, bias, Wh, hout, x0, y0, z0, X, Y, Z, box, core);
     else
         conv_store<NT, MT, BY, NZT, kRelu>(acc, out, bias, x0, y0, z0, X, Y, Z, box, off32);
 }
@@ -1513,6 +1535,8 @@ struct vc_surrogate
     const uint8_t* inf_occ = nullptr;
     const int32_t* inf_edt = nullptr;
     int inf_n[3] = {0, 0, 0};
+    // and where its head writes the core voxels' lam / rel (no copy_core)
+    CoreOut inf_core{};
     bf16 *cat2 = nullptr, *cat1 = nullptr, *cat0 = nullptr;
     float* out = nullptr;
     // backward workspace (allocated on first use)
@@ -1896,7 +1920,7 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
                     reinterpret_cast<const bf16x8*>(s->Wf[13]),
                     s->theta + param_offset(13) + 27LL * L.cin * L.cout, X, Y, Z,
                     level_box(s, 0), conv_off32(Y, Z, cinp_of(L.cin)),
-                    s->theta + param_offset(14), s->out);
+                    s->theta + param_offset(14), s->out, s->inf_core);
             VC_CHECK_LAUNCH(ctx);
             return VC_OK;
         }
@@ -2174,16 +2198,23 @@ int run_infer(vc_ctx* ctx, vc_surrogate* s, const vc_vol* v, int tile_core)
                         nullptr, s->X, s->Y, s->Z);
                     VC_CHECK_LAUNCH(ctx);
                 }
+                // the fused head writes the core's lam / rel itself
+                s->inf_core = CoreOut{s->vlam, s->vrel, v->nx, v->ny, v->nz, tx, ty, tz,
+                                      tx + step[0], ty + step[1], tz + step[2]};
                 rc = forward(ctx, s, false);
                 s->inf_occ = nullptr;
                 s->inf_edt = nullptr;
+                s->inf_core = CoreOut{};
                 if (rc != VC_OK)
                     return rc;
-                copy_core<<<blocks_for(VR), kBlock, 0, S>>>(
-                    s->out, s->X, s->Y, s->Z, ox, oy, oz, tx, ty, tz,
-                    tx + step[0], ty + step[1], tz + step[2], s->vlam, s->vrel,
-                    v->nx, v->ny, v->nz);
-                VC_CHECK_LAUNCH(ctx);
+                if (!VC_CONV_FUSED_HEAD_CORE)
+                {
+                    copy_core<<<blocks_for(VR), kBlock, 0, S>>>(
+                        s->out, s->X, s->Y, s->Z, ox, oy, oz, tx, ty, tz,
+                        tx + step[0], ty + step[1], tz + step[2], s->vlam, s->vrel,
+                        v->nx, v->ny, v->nz);
+                    VC_CHECK_LAUNCH(ctx);
+                }
             }
     return VC_OK;
 }
