@@ -831,25 +831,38 @@ __global__ void __launch_bounds__(kBlock)
     pool_fwd(const bf16* __restrict__ a, bf16* __restrict__ p, int C, int X,
              int Y, int Z)
 {
-    const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1;
-    const int64_t n = (int64_t)Xp * Yp * Zp * C;
+    // one thread per 8 channels of a pooled voxel (C is a multiple of 8):
+    // 16-byte loads and stores; each channel is summed in the cell order
+    // (x, y, z lexicographic) from 0, then scaled by 1/8
+    const int Xp = X >> 1, Yp = Y >> 1, Zp = Z >> 1, C8 = C >> 3;
+    const int64_t n = (int64_t)Xp * Yp * Zp * C8;
     VC_GRID_STRIDE(t, n)
     {
-        const int c = (int)(t % C);
-        const int64_t v = t / C;
+        const int c8 = (int)(t % C8);
+        const int64_t v = t / C8;
         const int z = (int)(v % Zp);
         const int64_t r = v / Zp;
         const int y = (int)(r % Yp), x = (int)(r / Yp);
-        float s = 0.0f;
+        float s[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            s[k] = 0.0f;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
         {
             const int64_t cv =
                 (((int64_t)(2 * x + (q >> 2)) * Y) + (2 * y + ((q >> 1) & 1))) * Z +
                 (2 * z + (q & 1));
-            s += (float)a[cv * C + c];
+            const bf16x8 w = *reinterpret_cast<const bf16x8*>(a + cv * C + 8 * c8);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                s[k] += (float)w[k];
         }
-        p[t] = (bf16)(s * 0.125f);
+        bf16x8 o;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            o[k] = (bf16)(s[k] * 0.125f);
+        *reinterpret_cast<bf16x8*>(p + v * C + 8 * c8) = o;
     }
 }
 
@@ -1873,8 +1886,8 @@ int forward(vc_ctx* ctx, vc_surrogate* s, bool keep_cat)
     {
         dims(lv, X, Y, Z);
         const int64_t n = level_voxels(s, lv + 1) * pc[lv];
-        pool_fwd<<<grid_for(n), kBlock, 0, S>>>(s->act[2 * lv + 1], pools[lv],
-                                                   pc[lv], X, Y, Z);
+        pool_fwd<<<grid_for(n / 8), kBlock, 0, S>>>(s->act[2 * lv + 1], pools[lv],
+                                                       pc[lv], X, Y, Z);
         VC_CHECK_LAUNCH(ctx);
         if ((rc = conv(2 * lv + 2, pools[lv])) != VC_OK ||
             (rc = conv(2 * lv + 3, s->act[2 * lv + 2])) != VC_OK)
